@@ -1,0 +1,138 @@
+/* sofa_amd.h -- C ABI of the MI355X (gfx950) implicit corotational-FEM step.
+ *
+ * One shared library, libsofa_amd.so, exports everything below.  Plain C types only: device
+ * vectors are pointers to AoS xyz triples (SOFA's Vec3d layout, VecTypes.h:200-234), streams
+ * are hipStream_t passed as void*.  Every function returns an int status (SOFA_AMD_OK = 0,
+ * negative on error) and never throws; sofa_amd_last_error() gives the message.  This maps
+ * onto SOFA's msg_error + ComponentState::Invalid convention (TetrahedronFEMForceField.inl
+ * :1262-1311) in the plugin adapter.
+ *
+ * Reference interfaces each entry point stands in for are cited per function (paths relative
+ * to the sofa-framework/sofa source tree).
+ */
+#ifndef SOFA_AMD_H
+#define SOFA_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define SOFA_AMD_OK            0
+#define SOFA_AMD_EINVAL       -1   /* invalid argument / mesh index out of range            */
+#define SOFA_AMD_EDEVICE      -2   /* HIP runtime error or no gfx950 device                  */
+#define SOFA_AMD_ENOMEM       -3   /* device allocation failed / admission refused           */
+#define SOFA_AMD_EDEGENERATE  -4   /* zero-volume tetrahedron (K / (36 V) would divide by 0)  */
+#define SOFA_AMD_ESTATE       -5   /* call out of order (e.g. step before commit)            */
+#define SOFA_AMD_ENAN         -6   /* non-finite state detected                              */
+
+/* precision modes */
+#define SOFA_AMD_FP64   0   /* everything in double: the parity mode                          */
+#define SOFA_AMD_FP32   1   /* element arithmetic, state and CG vectors in float; FP64 dots  */
+#define SOFA_AMD_MIXED  2   /* FP32 element arithmetic + FP32 inner CG, FP64 state and
+                               iterative refinement (r = b - A x in FP64)                     */
+
+/* mass kinds */
+#define SOFA_AMD_MASS_UNIFORM   0   /* UniformMass (UniformMass.inl:403-496)                 */
+#define SOFA_AMD_MASS_DIAGONAL  1   /* DiagonalMass on tetrahedra (DiagonalMass.inl:1060-1080)*/
+
+typedef struct sofa_amd_engine_s* sofa_amd_engine;
+
+/* CG statistics of the last solve of one scene: CGLinearSolver's "graph" Data and the
+ * AdvancedTimer "CG iterations" value (CGLinearSolver.inl:109-114,306). */
+typedef struct sofa_amd_cg_stats {
+    int32_t iterations;     /* nb_iter at loop exit                                          */
+    int32_t end_reason;     /* 0 iterations, 1 tolerance, 2 threshold, 3 den == 0, 4 |b| == 0 */
+    int32_t step_count;     /* CGLinearSolver::timeStepCount                                 */
+    int32_t ir_sweeps;      /* iterative-refinement sweeps (MIXED), else 0                   */
+    double  err;            /* last sqrt(r.r)/|b|                                            */
+    double  den;            /* last p.A.p                                                    */
+    double  time;           /* context time of the scene                                     */
+} sofa_amd_cg_stats;
+
+/* ---- library ---------------------------------------------------------------------------- */
+
+/* Plugin-module identity, as PluginManager resolves it (PluginManager.cpp:89-94). */
+const char* sofa_amd_module_name(void);
+const char* sofa_amd_module_version(void);
+/* Message of the last failing call on this thread. */
+const char* sofa_amd_last_error(void);
+/* 0 if `device` is a usable gfx950 GPU; the plugin's moduleIsInitialized() reports it
+ * (PluginManager.cpp:258-266 unloads a plugin that answers false). */
+int sofa_amd_device_check(int device);
+
+/* ---- engine: a set of device-resident scenes advanced in merged launches ----------------- */
+
+/* No reference counterpart (SOFA runs one scene per process); holds device buffers. */
+int sofa_amd_engine_create(int device, int precision, sofa_amd_engine* out);
+int sofa_amd_engine_destroy(sofa_amd_engine eng);
+
+/* A scene = MechanicalObject<Vec3d> (MechanicalObject.h:85-93) with rest positions
+ * `positions` (n_nodes x 3, host).  Returns the scene id. */
+int sofa_amd_scene_create(sofa_amd_engine eng, int n_nodes, const double* positions, int* out_scene);
+
+/* TetrahedronFEMForceField<Vec3d> method="large": init + reinit
+ * (TetrahedronFEMForceField.inl:1257-1545).  `tets` n_tets x 4 node indices (host);
+ * youngModulus / poissonRatio per element or a single value
+ * (BaseLinearElasticityFEMForceField.inl:113-144). */
+int sofa_amd_scene_set_fem(sofa_amd_engine eng, int scene, int n_tets, const int32_t* tets,
+                           const double* young, int n_young, const double* poisson, int n_poisson);
+
+/* UniformMass vertexMass (UniformMass.inl:403-496) or DiagonalMass massDensity on the FEM's
+ * tetrahedra (DiagonalMass.inl:1060-1080, 536-559, 1393-1413). */
+int sofa_amd_scene_set_mass(sofa_amd_engine eng, int scene, int kind, double value);
+
+/* Context gravity (Node::setGravity), used by the masses' addForce. */
+int sofa_amd_scene_set_gravity(sofa_amd_engine eng, int scene, double gx, double gy, double gz);
+
+/* FixedProjectiveConstraint indices (FixedProjectiveConstraint.inl:184-207). */
+int sofa_amd_scene_set_fixed(sofa_amd_engine eng, int scene, const int32_t* indices, int n);
+
+/* EulerImplicitSolver rayleighStiffness / rayleighMass (EulerImplicitSolver.cpp:40-51) and
+ * CGLinearSolver iterations / tolerance / threshold (CGLinearSolver.inl:36-40), dt. */
+int sofa_amd_scene_set_solver(sofa_amd_engine eng, int scene, double dt, double rayleigh_stiffness,
+                              double rayleigh_mass, int cg_iterations, double cg_tolerance,
+                              double cg_threshold);
+
+/* Uploads every scene created so far; after this the scenes can be stepped. */
+int sofa_amd_engine_commit(sofa_amd_engine eng);
+
+/* node::animate x n_steps (Simulation.cpp:207-227 -> DefaultAnimationLoop::step ->
+ * EulerImplicitSolver::solve -> CGLinearSolver::solve) for the listed scenes, merged into one
+ * launch per phase.  Asynchronous on `stream` (NULL = the engine's stream). */
+int sofa_amd_engine_step(sofa_amd_engine eng, const int32_t* scenes, int n_scenes, int n_steps,
+                         void* stream);
+
+/* Blocks until the engine's work is done. */
+int sofa_amd_engine_sync(sofa_amd_engine eng);
+
+/* Host copies of a scene's position / velocity / force (n_nodes x 3 doubles each; NULL skips). */
+int sofa_amd_scene_read(sofa_amd_engine eng, int scene, double* x, double* v, double* f);
+/* Overwrites position / velocity (NULL skips). */
+int sofa_amd_scene_write(sofa_amd_engine eng, int scene, const double* x, const double* v);
+int sofa_amd_scene_stats(sofa_amd_engine eng, int scene, sofa_amd_cg_stats* out);
+/* Device-side element data of one element as the reference stores it (for parity checks):
+ * rotations[e] (3x3), K diagonal/off-diagonal/shear (3), rotated rest coordinates (4x3). */
+int sofa_amd_scene_element(sofa_amd_engine eng, int scene, int elem, double* rot9, double* k3,
+                           double* rest12);
+
+/* ---- component-level calls, one TetrahedronFEMForceField call each ----------------------- */
+
+/* TetrahedronFEMForceField::addForce (TetrahedronFEMForceField.inl:1548-1604):
+ * f += F(x), and the element rotations are updated from x.  x_dev, f_dev: device, n_nodes x 3
+ * doubles (AoS). */
+int sofa_amd_fem_add_force(sofa_amd_engine eng, int scene, const double* x_dev, double* f_dev,
+                           void* stream);
+
+/* TetrahedronFEMForceField::addDForce (TetrahedronFEMForceField.inl:1606-1636):
+ * df -= kFactor * K(x) * dx with the rotations of the last addForce. */
+int sofa_amd_fem_add_dforce(sofa_amd_engine eng, int scene, const double* dx_dev, double* df_dev,
+                            double k_factor, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SOFA_AMD_H */

@@ -1,0 +1,906 @@
+// fem_oracle.cpp -- TEST INFRASTRUCTURE ONLY.
+//
+// FP64 CPU restatement of SOFA's implicit corotational-FEM time step, used as the
+// correctness oracle by tests/, by __graft_entry__.smoke() and as the CPU baseline leg
+// of bench.py.  Nothing in the product (sofa_amd/) links, loads or calls this file.
+//
+// It follows the reference line by line, in the reference's own operation order:
+//   TetrahedronFEMForceField<Vec3d>  Sofa/Component/SolidMechanics/FEM/Elastic/src/sofa/
+//       component/solidmechanics/fem/elastic/TetrahedronFEMForceField.inl
+//         computeStrainDisplacement :134-203, peudo_determinant_for_coef :205-209,
+//         computeMaterialStiffness :256-291, computeForce (addForce form) :293-415,
+//         computeForce (dForce form) :417-521, initSmall/accumulateForceSmall :526-620,
+//         applyStiffnessSmall :724-749, computeRotationLarge :755-778, initLarge :835-868,
+//         accumulateForceLarge :870-986, applyStiffnessCorotational :1193-1237,
+//         init/reinit :1257-1545, addForce :1548-1604, addDForce :1606-1636
+//   BaseLinearElasticityFEMForceField.inl:113-144 (per-element E / nu lookup)
+//   sofa/type/Vec.h:483-580 (norm2 / normalize), :775-780 (cross); Mat.h:577-587 (Mat*Vec)
+//   sofa/geometry/Tetrahedron.h:55-83 (volume = |signed volume|)
+//   UniformMass.inl:329-345,403-419,471-496; DiagonalMass.inl:536-559,1060-1080,1393-1413
+//   FixedProjectiveConstraint.inl:184-207
+//   OscillatorProjectiveConstraint.inl:65-106
+//   CGLinearSolver.inl:71-315, CGLinearSolver.cpp:40-64 (GraphScattered cgstep_beta/alpha)
+//   GraphScatteredTypes.cpp:33-45 (A*p = project((mM + bB + kK) p))
+//   EulerImplicitSolver.cpp:83-300
+//   MechanicalObject.inl:2076-2356 (vOp, vMultiOp, vDot), BaseMechanicalState.cpp:42-80
+//   BaseForceField.cpp:38-47, Mass.inl:94-104, MechanicalParams.h:62-64
+//   DefaultAnimationLoop.cpp:273-329 (solve, then projectPositionAndVelocity, then time += dt)
+//
+// Build: oracle/Makefile (g++ -O2 -ffp-contract=off, so that no multiply-add is fused:
+// the reference's x86-64 build does not fuse either).
+//
+// Dot-product order: the reference sums r.r node by node in ascending order
+// (MechanicalObject.inl:2334-2356).  The GPU sums per 256-node block with a fixed tree and
+// then over blocks.  `dot_mode = 1` replays that GPU order exactly (see dot_blocked), so the
+// GPU trajectory can be compared bit for bit as well as against the reference order.
+
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <vector>
+#include <array>
+#include <memory>
+#include <algorithm>
+
+namespace oracle {
+
+typedef double Real;
+
+struct Vec3 {
+    Real v[3];
+    Vec3() { v[0] = v[1] = v[2] = 0; }
+    Vec3(Real a, Real b, Real c) { v[0] = a; v[1] = b; v[2] = c; }
+    Real& operator[](int i) { return v[i]; }
+    Real operator[](int i) const { return v[i]; }
+};
+
+static inline Vec3 sub(const Vec3& a, const Vec3& b) { return Vec3(a[0] - b[0], a[1] - b[1], a[2] - b[2]); }
+// Vec.h:407-413 -- r = a0*b0; r += a1*b1; r += a2*b2
+static inline Real dot3(const Vec3& a, const Vec3& b) {
+    Real r = a[0] * b[0];
+    r += a[1] * b[1];
+    r += a[2] * b[2];
+    return r;
+}
+// Vec.h:775-780
+static inline Vec3 cross(const Vec3& a, const Vec3& b) {
+    return Vec3(a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]);
+}
+// Vec.h:483-492 (norm2), 497-500 (norm), 535-578 (normalizeWithNorm / normalized)
+static inline Vec3 normalized(const Vec3& a) {
+    Vec3 r = a;
+    Real n2 = r[0] * r[0];
+    n2 += r[1] * r[1];
+    n2 += r[2] * r[2];
+    Real n = std::sqrt(n2);
+    if (n > std::numeric_limits<Real>::epsilon()) {
+        for (int i = 0; i < 3; i++) r[i] /= n;
+    }
+    return r;
+}
+
+struct Mat3 {
+    Real m[3][3];
+    Mat3() { std::memset(m, 0, sizeof(m)); }
+};
+// Mat.h:577-587: r(i) = m(i,0)*v0; r(i) += m(i,j)*v[j]
+static inline Vec3 mul(const Mat3& M, const Vec3& v) {
+    Vec3 r;
+    for (int i = 0; i < 3; i++) {
+        r[i] = M.m[i][0] * v[0];
+        r[i] += M.m[i][1] * v[1];
+        r[i] += M.m[i][2] * v[2];
+    }
+    return r;
+}
+static inline Mat3 transpose(const Mat3& M) {
+    Mat3 t;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) t.m[i][j] = M.m[j][i];
+    return t;
+}
+
+// Tetrahedron.h:55-83
+static inline Real tet_volume(const Vec3& n0, const Vec3& n1, const Vec3& n2, const Vec3& n3) {
+    Vec3 a = sub(n1, n0), b = sub(n2, n0), c = sub(n3, n0);
+    return std::abs(dot3(cross(a, b), c) / static_cast<Real>(6));
+}
+
+typedef std::array<int, 4> Tetra;
+
+// ------------------------------------------------------------------------------------------
+// Force-field interface (ForceField.h:87,110; Mass.h:69-77)
+// ------------------------------------------------------------------------------------------
+struct ForceField {
+    virtual ~ForceField() {}
+    virtual void addForce(std::vector<Vec3>& f, const std::vector<Vec3>& x, const std::vector<Vec3>& v) = 0;
+    // BaseForceField::addMBKdx (BaseForceField.cpp:38-47) / Mass::addMBKdx (Mass.inl:94-104)
+    virtual void addMBKdx(std::vector<Vec3>& df, const std::vector<Vec3>& dx, Real mFact, Real bFact, Real kFact) = 0;
+};
+
+// ------------------------------------------------------------------------------------------
+// TetrahedronFEMForceField<Vec3d>
+// ------------------------------------------------------------------------------------------
+struct TetraFEM : public ForceField {
+    enum Method { LARGE = 0, SMALL = 1 };
+    int method = LARGE;
+    std::vector<Tetra> elems;
+    std::vector<Vec3> initialPoints;
+    std::vector<Real> young, poisson;
+    Real rayleighStiffness = 0;      // BaseForceField.cpp:34 default 0
+
+    // per-element data, reference layout (TetrahedronFEMForceField.h:111-128)
+    std::vector<std::array<Real, 36>> K;       // MaterialStiffness 6x6
+    std::vector<std::array<Real, 72>> J;       // StrainDisplacement 12x6
+    std::vector<Mat3> rotations, initialRotations;
+    std::vector<std::array<Vec3, 4>> rotatedInitial;
+    Real restVolume = 0;
+
+    Real& Kel(size_t i, int r, int c) { return K[i][r * 6 + c]; }
+    Real& Jel(size_t i, int r, int c) { return J[i][r * 6 + c]; }
+
+    // BaseLinearElasticityFEMForceField.inl:113-144
+    Real vecRealInElement(size_t e, const std::vector<Real>& d, Real def) const {
+        if (d.size() > e) return d[e];
+        if (!d.empty()) return d[0];
+        return def;
+    }
+
+    // :205-209  (M is 2x3, row-major m[r][c])
+    static Real pd(const Real M[2][3]) {
+        return M[0][1] * M[1][2] - M[1][1] * M[0][2] - M[0][0] * M[1][2] + M[1][0] * M[0][2] + M[0][0] * M[1][1] - M[1][0] * M[0][1];
+    }
+
+    // :134-203
+    void computeStrainDisplacement(size_t i, const Vec3& a, const Vec3& b, const Vec3& c, const Vec3& d) {
+        Real M[2][3];
+        auto& Jm = J[i];
+        std::fill(Jm.begin(), Jm.end(), 0.0);
+        auto S = [&](int r, int cc, Real v) { Jm[r * 6 + cc] = v; };
+        M[0][0] = b[1]; M[0][1] = c[1]; M[0][2] = d[1];
+        M[1][0] = b[2]; M[1][1] = c[2]; M[1][2] = d[2];
+        { Real v = -pd(M); S(0, 0, v); S(1, 3, v); S(2, 5, v); }
+        M[0][0] = b[0]; M[0][1] = c[0]; M[0][2] = d[0];
+        { Real v = pd(M); S(0, 3, v); S(1, 1, v); S(2, 4, v); }
+        M[1][0] = b[1]; M[1][1] = c[1]; M[1][2] = d[1];
+        { Real v = -pd(M); S(0, 5, v); S(1, 4, v); S(2, 2, v); }
+
+        M[0][0] = c[1]; M[0][1] = d[1]; M[0][2] = a[1];
+        M[1][0] = c[2]; M[1][1] = d[2]; M[1][2] = a[2];
+        { Real v = pd(M); S(3, 0, v); S(4, 3, v); S(5, 5, v); }
+        M[0][0] = c[0]; M[0][1] = d[0]; M[0][2] = a[0];
+        { Real v = -pd(M); S(3, 3, v); S(4, 1, v); S(5, 4, v); }
+        M[1][0] = c[1]; M[1][1] = d[1]; M[1][2] = a[1];
+        { Real v = pd(M); S(3, 5, v); S(4, 4, v); S(5, 2, v); }
+
+        M[0][0] = d[1]; M[0][1] = a[1]; M[0][2] = b[1];
+        M[1][0] = d[2]; M[1][1] = a[2]; M[1][2] = b[2];
+        { Real v = -pd(M); S(6, 0, v); S(7, 3, v); S(8, 5, v); }
+        M[0][0] = d[0]; M[0][1] = a[0]; M[0][2] = b[0];
+        { Real v = pd(M); S(6, 3, v); S(7, 1, v); S(8, 4, v); }
+        M[1][0] = d[1]; M[1][1] = a[1]; M[1][2] = b[1];
+        { Real v = -pd(M); S(6, 5, v); S(7, 4, v); S(8, 2, v); }
+
+        M[0][0] = a[1]; M[0][1] = b[1]; M[0][2] = c[1];
+        M[1][0] = a[2]; M[1][1] = b[2]; M[1][2] = c[2];
+        { Real v = pd(M); S(9, 0, v); S(10, 3, v); S(11, 5, v); }
+        M[0][0] = a[0]; M[0][1] = b[0]; M[0][2] = c[0];
+        { Real v = -pd(M); S(9, 3, v); S(10, 1, v); S(11, 4, v); }
+        M[1][0] = a[1]; M[1][1] = b[1]; M[1][2] = c[1];
+        { Real v = pd(M); S(9, 5, v); S(10, 4, v); S(11, 2, v); }
+    }
+
+    // :256-291
+    void computeMaterialStiffness(size_t i, int a, int b, int c, int d) {
+        const Real youngModulus = 1.0f * vecRealInElement(i, young, 5000);   // localStiffnessFactor empty
+        const Real poissonRatio = vecRealInElement(i, poisson, 0.45);
+        auto& Km = K[i];
+        std::fill(Km.begin(), Km.end(), 0.0);
+        Km[0 * 6 + 0] = Km[1 * 6 + 1] = Km[2 * 6 + 2] = 1;
+        Km[0 * 6 + 1] = Km[0 * 6 + 2] = Km[1 * 6 + 0] = Km[1 * 6 + 2] = Km[2 * 6 + 0] = Km[2 * 6 + 1] = poissonRatio / (1 - poissonRatio);
+        Km[3 * 6 + 3] = Km[4 * 6 + 4] = Km[5 * 6 + 5] = (1 - 2 * poissonRatio) / (2 * (1 - poissonRatio));
+        const Real s = (youngModulus * (1 - poissonRatio)) / ((1 + poissonRatio) * (1 - 2 * poissonRatio));
+        for (auto& k : Km) k *= s;
+        const Real vol = tet_volume(initialPoints[a], initialPoints[b], initialPoints[c], initialPoints[d]);
+        restVolume += vol;
+        const Real div = vol * 36;
+        for (auto& k : Km) k /= div;
+    }
+
+    // :755-778
+    static Mat3 computeRotationLarge(const std::vector<Vec3>& p, int a, int b, int c) {
+        const Vec3 edgex = normalized(sub(p[b], p[a]));
+        Vec3 edgey = sub(p[c], p[a]);
+        const Vec3 edgez = normalized(cross(edgex, edgey));
+        edgey = cross(edgez, edgex);
+        Mat3 r;
+        for (int k = 0; k < 3; k++) {
+            r.m[0][k] = edgex[k];
+            r.m[1][k] = edgey[k];
+            r.m[2][k] = edgez[k];
+        }
+        return r;
+    }
+
+    // :835-868
+    void initLarge(size_t i, int a, int b, int c, int d) {
+        Mat3 R01 = computeRotationLarge(initialPoints, a, b, c);
+        initialRotations[i] = transpose(R01);
+        rotations[i] = initialRotations[i];
+        auto& ri = rotatedInitial[i];
+        ri[0] = mul(R01, initialPoints[a]);
+        ri[1] = mul(R01, initialPoints[b]);
+        ri[2] = mul(R01, initialPoints[c]);
+        ri[3] = mul(R01, initialPoints[d]);
+        for (int k = 1; k < 4; k++)
+            for (int q = 0; q < 3; q++) ri[k][q] -= ri[0][q];
+        ri[0] = Vec3(0, 0, 0);
+        computeStrainDisplacement(i, ri[0], ri[1], ri[2], ri[3]);
+    }
+
+    // :526-531
+    void initSmall(size_t i, int a, int b, int c, int d) {
+        computeStrainDisplacement(i, initialPoints[a], initialPoints[b], initialPoints[c], initialPoints[d]);
+    }
+
+    // init (:1257-1378) + reinit (:1391-1545) for an explicit tetra list
+    void init(const std::vector<Vec3>& rest, const std::vector<Tetra>& tets, const std::vector<Real>& E, const std::vector<Real>& nu, int meth) {
+        method = meth;
+        elems = tets;
+        initialPoints = rest;
+        young = E.empty() ? std::vector<Real>{5000} : E;
+        poisson = nu.empty() ? std::vector<Real>{0.45} : nu;
+        // BaseLinearElasticityFEMForceField::checkYoungModulus / checkPoissonRatio (:49-84)
+        for (auto& y : young) if (y < 0) y = 5000;
+        for (auto& p : poisson) if (p < 0 || p >= 0.5) p = 0.45;
+        const size_t ne = elems.size();
+        K.assign(ne, {});
+        J.assign(ne, {});
+        restVolume = 0;
+        if (method == LARGE) {
+            rotations.assign(ne, Mat3());
+            initialRotations.assign(ne, Mat3());
+            rotatedInitial.assign(ne, {});
+        }
+        for (size_t i = 0; i < ne; i++) {
+            const Tetra& t = elems[i];
+            computeMaterialStiffness(i, t[0], t[1], t[2], t[3]);
+            if (method == LARGE) initLarge(i, t[0], t[1], t[2], t[3]);
+            else initSmall(i, t[0], t[1], t[2], t[3]);
+        }
+    }
+
+    // :293-415 (plasticity disabled: plasticMaxThreshold defaults to 0)
+    void computeForceF(Real F[12], const Real D[12], size_t i) {
+        auto Jf = [&](int r, int c) { return J[i][r * 6 + c]; };
+        auto Kf = [&](int r, int c) { return K[i][r * 6 + c]; };
+        Real JtD[6];
+        JtD[0] = Jf(0, 0) * D[0] + Jf(3, 0) * D[3] + Jf(6, 0) * D[6] + Jf(9, 0) * D[9];
+        JtD[1] = Jf(1, 1) * D[1] + Jf(4, 1) * D[4] + Jf(7, 1) * D[7] + Jf(10, 1) * D[10];
+        JtD[2] = Jf(2, 2) * D[2] + Jf(5, 2) * D[5] + Jf(8, 2) * D[8] + Jf(11, 2) * D[11];
+        JtD[3] = Jf(0, 3) * D[0] + Jf(1, 3) * D[1] + Jf(3, 3) * D[3] + Jf(4, 3) * D[4] + Jf(6, 3) * D[6] + Jf(7, 3) * D[7] + Jf(9, 3) * D[9] + Jf(10, 3) * D[10];
+        JtD[4] = Jf(1, 4) * D[1] + Jf(2, 4) * D[2] + Jf(4, 4) * D[4] + Jf(5, 4) * D[5] + Jf(7, 4) * D[7] + Jf(8, 4) * D[8] + Jf(10, 4) * D[10] + Jf(11, 4) * D[11];
+        JtD[5] = Jf(0, 5) * D[0] + Jf(2, 5) * D[2] + Jf(3, 5) * D[3] + Jf(5, 5) * D[5] + Jf(6, 5) * D[6] + Jf(8, 5) * D[8] + Jf(9, 5) * D[9] + Jf(11, 5) * D[11];
+        Real KJtD[6];
+        KJtD[0] = Kf(0, 0) * JtD[0] + Kf(0, 1) * JtD[1] + Kf(0, 2) * JtD[2];
+        KJtD[1] = Kf(1, 0) * JtD[0] + Kf(1, 1) * JtD[1] + Kf(1, 2) * JtD[2];
+        KJtD[2] = Kf(2, 0) * JtD[0] + Kf(2, 1) * JtD[1] + Kf(2, 2) * JtD[2];
+        KJtD[3] = Kf(3, 3) * JtD[3];
+        KJtD[4] = Kf(4, 4) * JtD[4];
+        KJtD[5] = Kf(5, 5) * JtD[5];
+        jkjtd(F, KJtD, i);
+    }
+
+    // the common tail of both computeForce forms: F = J * KJtD (:384-413 and :490-519)
+    void jkjtd(Real F[12], const Real KJtD[6], size_t i) {
+        auto Jf = [&](int r, int c) { return J[i][r * 6 + c]; };
+        F[0] = Jf(0, 0) * KJtD[0] + Jf(0, 3) * KJtD[3] + Jf(0, 5) * KJtD[5];
+        F[1] = Jf(1, 1) * KJtD[1] + Jf(1, 3) * KJtD[3] + Jf(1, 4) * KJtD[4];
+        F[2] = Jf(2, 2) * KJtD[2] + Jf(2, 4) * KJtD[4] + Jf(2, 5) * KJtD[5];
+        F[3] = Jf(3, 0) * KJtD[0] + Jf(3, 3) * KJtD[3] + Jf(3, 5) * KJtD[5];
+        F[4] = Jf(4, 1) * KJtD[1] + Jf(4, 3) * KJtD[3] + Jf(4, 4) * KJtD[4];
+        F[5] = Jf(5, 2) * KJtD[2] + Jf(5, 4) * KJtD[4] + Jf(5, 5) * KJtD[5];
+        F[6] = Jf(6, 0) * KJtD[0] + Jf(6, 3) * KJtD[3] + Jf(6, 5) * KJtD[5];
+        F[7] = Jf(7, 1) * KJtD[1] + Jf(7, 3) * KJtD[3] + Jf(7, 4) * KJtD[4];
+        F[8] = Jf(8, 2) * KJtD[2] + Jf(8, 4) * KJtD[4] + Jf(8, 5) * KJtD[5];
+        F[9] = Jf(9, 0) * KJtD[0] + Jf(9, 3) * KJtD[3] + Jf(9, 5) * KJtD[5];
+        F[10] = Jf(10, 1) * KJtD[1] + Jf(10, 3) * KJtD[3] + Jf(10, 4) * KJtD[4];
+        F[11] = Jf(11, 2) * KJtD[2] + Jf(11, 4) * KJtD[4] + Jf(11, 5) * KJtD[5];
+    }
+
+    // :417-521 (dForce form, with `KJtD *= fact`)
+    void computeForceDF(Real F[12], const Real D[12], size_t i, Real fact) {
+        auto Jf = [&](int r, int c) { return J[i][r * 6 + c]; };
+        auto Kf = [&](int r, int c) { return K[i][r * 6 + c]; };
+        Real JtD[6];
+        JtD[0] = Jf(0, 0) * D[0] + Jf(3, 0) * D[3] + Jf(6, 0) * D[6] + Jf(9, 0) * D[9];
+        JtD[1] = Jf(1, 1) * D[1] + Jf(4, 1) * D[4] + Jf(7, 1) * D[7] + Jf(10, 1) * D[10];
+        JtD[2] = Jf(2, 2) * D[2] + Jf(5, 2) * D[5] + Jf(8, 2) * D[8] + Jf(11, 2) * D[11];
+        JtD[3] = Jf(0, 3) * D[0] + Jf(1, 3) * D[1] + Jf(3, 3) * D[3] + Jf(4, 3) * D[4] + Jf(6, 3) * D[6] + Jf(7, 3) * D[7] + Jf(9, 3) * D[9] + Jf(10, 3) * D[10];
+        JtD[4] = Jf(1, 4) * D[1] + Jf(2, 4) * D[2] + Jf(4, 4) * D[4] + Jf(5, 4) * D[5] + Jf(7, 4) * D[7] + Jf(8, 4) * D[8] + Jf(10, 4) * D[10] + Jf(11, 4) * D[11];
+        JtD[5] = Jf(0, 5) * D[0] + Jf(2, 5) * D[2] + Jf(3, 5) * D[3] + Jf(5, 5) * D[5] + Jf(6, 5) * D[6] + Jf(8, 5) * D[8] + Jf(9, 5) * D[9] + Jf(11, 5) * D[11];
+        Real KJtD[6];
+        KJtD[0] = Kf(0, 0) * JtD[0] + Kf(0, 1) * JtD[1] + Kf(0, 2) * JtD[2];
+        KJtD[1] = Kf(1, 0) * JtD[0] + Kf(1, 1) * JtD[1] + Kf(1, 2) * JtD[2];
+        KJtD[2] = Kf(2, 0) * JtD[0] + Kf(2, 1) * JtD[1] + Kf(2, 2) * JtD[2];
+        KJtD[3] = Kf(3, 3) * JtD[3];
+        KJtD[4] = Kf(4, 4) * JtD[4];
+        KJtD[5] = Kf(5, 5) * JtD[5];
+        for (int k = 0; k < 6; k++) KJtD[k] *= static_cast<Real>(fact);
+        jkjtd(F, KJtD, i);
+    }
+
+    // :870-986 (assembling == false, updateStiffnessMatrix == false)
+    void accumulateForceLarge(std::vector<Vec3>& f, const std::vector<Vec3>& p, size_t i) {
+        const Tetra& index = elems[i];
+        Mat3 R02 = computeRotationLarge(p, index[0], index[1], index[2]);
+        rotations[i] = transpose(R02);
+        Vec3 deforme[4];
+        for (int k = 0; k < 4; k++) deforme[k] = mul(R02, p[index[k]]);
+        deforme[1][0] -= deforme[0][0];
+        deforme[2][0] -= deforme[0][0];
+        deforme[2][1] -= deforme[0][1];
+        for (int q = 0; q < 3; q++) deforme[3][q] -= deforme[0][q];
+        const auto& ri = rotatedInitial[i];
+        Real D[12];
+        D[0] = 0; D[1] = 0; D[2] = 0;
+        D[3] = ri[1][0] - deforme[1][0];
+        D[4] = 0; D[5] = 0;
+        D[6] = ri[2][0] - deforme[2][0];
+        D[7] = ri[2][1] - deforme[2][1];
+        D[8] = 0;
+        D[9] = ri[3][0] - deforme[3][0];
+        D[10] = ri[3][1] - deforme[3][1];
+        D[11] = ri[3][2] - deforme[3][2];
+        Real F[12];
+        computeForceF(F, D, i);
+        for (int k = 0; k < 12; k += 3) {
+            Vec3 c = mul(rotations[i], Vec3(F[k], F[k + 1], F[k + 2]));
+            for (int q = 0; q < 3; q++) f[index[k / 3]][q] += c[q];
+        }
+    }
+
+    // :533-620 (assembling == false)
+    void accumulateForceSmall(std::vector<Vec3>& f, const std::vector<Vec3>& p, size_t i) {
+        const Tetra& t = elems[i];
+        const int a = t[0], b = t[1], c = t[2], d = t[3];
+        const auto& ip = initialPoints;
+        Real D[12];
+        D[0] = 0; D[1] = 0; D[2] = 0;
+        for (int q = 0; q < 3; q++) {
+            D[3 + q] = ip[b][q] - ip[a][q] - p[b][q] + p[a][q];
+            D[6 + q] = ip[c][q] - ip[a][q] - p[c][q] + p[a][q];
+            D[9 + q] = ip[d][q] - ip[a][q] - p[d][q] + p[a][q];
+        }
+        Real F[12];
+        computeForceF(F, D, i);
+        for (int q = 0; q < 3; q++) {
+            f[a][q] += F[q];
+            f[b][q] += F[3 + q];
+            f[c][q] += F[6 + q];
+            f[d][q] += F[9 + q];
+        }
+    }
+
+    // :1548-1604
+    void addForce(std::vector<Vec3>& f, const std::vector<Vec3>& x, const std::vector<Vec3>&) override {
+        for (size_t i = 0; i < elems.size(); i++) {
+            if (method == LARGE) accumulateForceLarge(f, x, i);
+            else accumulateForceSmall(f, x, i);
+        }
+    }
+
+    // :1193-1237
+    void applyStiffnessCorotational(std::vector<Vec3>& f, const std::vector<Vec3>& x, size_t i, Real fact) {
+        const Tetra& t = elems[i];
+        const Mat3& R = rotations[i];
+        Real X[12];
+        for (int k = 0; k < 4; k++) {
+            const Vec3& u = x[t[k]];
+            X[3 * k + 0] = R.m[0][0] * u[0] + R.m[1][0] * u[1] + R.m[2][0] * u[2];
+            X[3 * k + 1] = R.m[0][1] * u[0] + R.m[1][1] * u[1] + R.m[2][1] * u[2];
+            X[3 * k + 2] = R.m[0][2] * u[0] + R.m[1][2] * u[1] + R.m[2][2] * u[2];
+        }
+        Real F[12];
+        computeForceDF(F, X, i, fact);
+        for (int k = 0; k < 4; k++) {
+            Vec3& o = f[t[k]];
+            o[0] -= R.m[0][0] * F[3 * k] + R.m[0][1] * F[3 * k + 1] + R.m[0][2] * F[3 * k + 2];
+            o[1] -= R.m[1][0] * F[3 * k] + R.m[1][1] * F[3 * k + 1] + R.m[1][2] * F[3 * k + 2];
+            o[2] -= R.m[2][0] * F[3 * k] + R.m[2][1] * F[3 * k + 1] + R.m[2][2] * F[3 * k + 2];
+        }
+    }
+
+    // :724-749 (applyStiffnessSmall: X = dx, f -= F)
+    void applyStiffnessSmall(std::vector<Vec3>& f, const std::vector<Vec3>& x, size_t i, Real fact) {
+        const Tetra& t = elems[i];
+        Real X[12];
+        for (int k = 0; k < 4; k++)
+            for (int q = 0; q < 3; q++) X[3 * k + q] = x[t[k]][q];
+        Real F[12];
+        computeForceDF(F, X, i, fact);
+        for (int k = 0; k < 4; k++)
+            for (int q = 0; q < 3; q++) f[t[k]][q] -= F[3 * k + q];
+    }
+
+    // :1606-1636; kFactor = (Real) kFactorIncludingRayleighDamping(mparams, rayleighStiffness)
+    void addDForce(std::vector<Vec3>& df, const std::vector<Vec3>& dx, Real kFactor) {
+        for (size_t i = 0; i < elems.size(); i++) {
+            if (method == SMALL) applyStiffnessSmall(df, dx, i, kFactor);
+            else applyStiffnessCorotational(df, dx, i, kFactor);
+        }
+    }
+
+    void addMBKdx(std::vector<Vec3>& df, const std::vector<Vec3>& dx, Real, Real bFact, Real kFact) override {
+        // BaseForceField.cpp:38-47
+        const Real kFactor = kFact + bFact * rayleighStiffness;
+        if (kFactor != 0.0 || bFact != 0.0) addDForce(df, dx, kFactor);
+    }
+};
+
+// ------------------------------------------------------------------------------------------
+// Masses
+// ------------------------------------------------------------------------------------------
+struct MassBase : public ForceField {
+    Vec3 gravity;
+    Real rayleighMass = 0;   // Mass.h rayleighMass Data, default 0
+    virtual void addMDx(std::vector<Vec3>& res, const std::vector<Vec3>& dx, Real factor) = 0;
+    void addMBKdx(std::vector<Vec3>& df, const std::vector<Vec3>& dx, Real mFact, Real bFact, Real) override {
+        // Mass.inl:94-104: ForceField::addMBKdx calls Mass::addDForce, a no-op; then addMDx
+        const Real m = mFact - bFact * rayleighMass;
+        if (m != 0.0) addMDx(df, dx, m);
+    }
+};
+
+// UniformMass<Vec3d> (UniformMass.inl)
+struct UniformMass : public MassBase {
+    Real vertexMass = 1;
+    void addMDx(std::vector<Vec3>& res, const std::vector<Vec3>& dx, Real factor) override {
+        Real m = vertexMass;                             // :403-419
+        if (factor != 1.0) m *= Real(factor);
+        for (size_t i = 0; i < res.size(); i++)
+            for (int q = 0; q < 3; q++) res[i][q] += dx[i][q] * m;
+    }
+    void addForce(std::vector<Vec3>& f, const std::vector<Vec3>&, const std::vector<Vec3>&) override {
+        Vec3 mg(gravity[0] * vertexMass, gravity[1] * vertexMass, gravity[2] * vertexMass);   // :471-496
+        for (size_t i = 0; i < f.size(); i++)
+            for (int q = 0; q < 3; q++) f[i][q] += mg[q];
+    }
+};
+
+// DiagonalMass<Vec3d> with a tetrahedral topology (DiagonalMass.inl)
+struct DiagonalMass : public MassBase {
+    std::vector<Real> masses;
+    void init(const std::vector<Vec3>& pos, const std::vector<Tetra>& tets, Real density) {
+        masses.assign(pos.size(), 0.0);                  // :1060-1080
+        for (const Tetra& t : tets) {
+            const Real vol = tet_volume(pos[t[0]], pos[t[1]], pos[t[2]], pos[t[3]]);
+            const Real mass = (density * vol) / Real(4.0);
+            for (int j = 0; j < 4; j++) masses[t[j]] += mass;
+        }
+    }
+    void addMDx(std::vector<Vec3>& res, const std::vector<Vec3>& dx, Real factor) override {
+        const size_t n = masses.size();                  // :536-559
+        if (factor == 1.0) {
+            for (size_t i = 0; i < n; i++)
+                for (int q = 0; q < 3; q++) res[i][q] += dx[i][q] * masses[i];
+        } else {
+            for (size_t i = 0; i < n; i++)
+                for (int q = 0; q < 3; q++) res[i][q] += (dx[i][q] * masses[i]) * Real(factor);
+        }
+    }
+    void addForce(std::vector<Vec3>& f, const std::vector<Vec3>&, const std::vector<Vec3>&) override {
+        for (size_t i = 0; i < masses.size(); i++)      // :1393-1413
+            for (int q = 0; q < 3; q++) f[i][q] += gravity[q] * masses[i];
+    }
+};
+
+// Test-only 1-D linear spring toward an anchor (used to pin the Euler recurrence of
+// EulerImplicitSolverDynamic_test.cpp:76-101 with the closed form): f_y = -K (x_y - y0).
+struct AxisSpring : public ForceField {
+    int node = 0, axis = 1;
+    Real stiffness = 0, anchor = 0;
+    void addForce(std::vector<Vec3>& f, const std::vector<Vec3>& x, const std::vector<Vec3>&) override {
+        f[node][axis] += -stiffness * (x[node][axis] - anchor);
+    }
+    void addMBKdx(std::vector<Vec3>& df, const std::vector<Vec3>& dx, Real, Real, Real kFact) override {
+        if (kFact != 0.0) df[node][axis] += -stiffness * dx[node][axis] * kFact;
+    }
+};
+
+// ------------------------------------------------------------------------------------------
+// Scene: MechanicalObject + mass + FEM (+ test springs) + projective constraints,
+// EulerImplicitSolver + CGLinearSolver<GraphScatteredMatrix, GraphScatteredVector>
+// ------------------------------------------------------------------------------------------
+struct Oscillator { int index; Vec3 mean, amplitude; Real pulsation, phase; };
+
+struct Scene {
+    std::vector<Vec3> x, v, f, x0;
+    std::unique_ptr<MassBase> mass;
+    std::unique_ptr<TetraFEM> fem;
+    std::vector<std::unique_ptr<ForceField>> extra;
+    std::vector<int> fixed;
+    bool fixAll = false;
+    std::vector<Oscillator> oscillators;
+    Vec3 gravity;
+    Real rayleighStiffness = 0, rayleighMass = 0;
+    unsigned cgIterations = 25;
+    Real cgTolerance = 1e-5, cgThreshold = 1e-5;
+    int dotMode = 0;           // 0: reference order; 1: GPU block order (256-node blocks)
+    Real time = 0;
+    int timeStepCount = 0;     // CGLinearSolver::timeStepCount
+    bool equilibriumReached = false;
+    // statistics of the last solve
+    int lastIterations = 0;
+    int lastEndCond = 0;       // 0 iterations, 1 tolerance, 2 threshold, 3 den==0, 4 null b
+    Real lastErr = 0, lastDen = 0;
+
+    size_t n() const { return x.size(); }
+
+    // force fields in scene order: mass first, then the FEM, then extras
+    std::vector<ForceField*> forceFields() {
+        std::vector<ForceField*> r;
+        if (mass) r.push_back(mass.get());
+        if (fem) r.push_back(fem.get());
+        for (auto& e : extra) r.push_back(e.get());
+        return r;
+    }
+
+    // MechanicalObject::vDot (MechanicalObject.inl:2334-2356): r += va[i]*vb[i]
+    Real dot_ref(const std::vector<Vec3>& a, const std::vector<Vec3>& b) const {
+        Real r = 0.0;
+        for (size_t i = 0; i < a.size(); i++) r += dot3(a[i], b[i]);
+        return r;
+    }
+    // GPU order: per node s_i = dot3 (same as above); 256-node blocks reduced by the tree
+    // sh[t] += sh[t+s] for s = 128..1; block partials reduced by a 256-thread pass where thread
+    // t sums partials t, t+256, ... from 0, followed by the same tree.
+    Real dot_blocked(const std::vector<Vec3>& a, const std::vector<Vec3>& b) const {
+        const size_t BS = 256;
+        const size_t nb = (a.size() + BS - 1) / BS;
+        std::vector<Real> partial(nb);
+        Real sh[BS];
+        for (size_t blk = 0; blk < nb; blk++) {
+            for (size_t t = 0; t < BS; t++) {
+                size_t i = blk * BS + t;
+                sh[t] = (i < a.size()) ? dot3(a[i], b[i]) : 0.0;
+            }
+            for (size_t s = BS / 2; s > 0; s >>= 1)
+                for (size_t t = 0; t < s; t++) sh[t] = sh[t] + sh[t + s];
+            partial[blk] = sh[0];
+        }
+        for (size_t t = 0; t < BS; t++) {
+            Real acc = 0.0;
+            for (size_t k = t; k < nb; k += BS) acc = acc + partial[k];
+            sh[t] = acc;
+        }
+        for (size_t s = BS / 2; s > 0; s >>= 1)
+            for (size_t t = 0; t < s; t++) sh[t] = sh[t] + sh[t + s];
+        return sh[0];
+    }
+    Real vdot(const std::vector<Vec3>& a, const std::vector<Vec3>& b) const {
+        return dotMode == 1 ? dot_blocked(a, b) : dot_ref(a, b);
+    }
+
+    // FixedProjectiveConstraint::projectResponse (:184-207) then Oscillator (:65-75)
+    void projectResponse(std::vector<Vec3>& r) {
+        if (fixAll) { for (auto& e : r) e = Vec3(); }
+        else for (int i : fixed) r[i] = Vec3();
+        for (const auto& o : oscillators) r[o.index] = Vec3();
+    }
+
+    // GraphScatteredMatrix::apply (GraphScatteredTypes.cpp:33-45)
+    void applyA(std::vector<Vec3>& q, const std::vector<Vec3>& p, Real mF, Real bF, Real kF) {
+        for (auto& e : q) e = Vec3();                         // propagateDxAndResetDf
+        for (ForceField* ff : forceFields()) ff->addMBKdx(q, p, mF, bF, kF);
+        projectResponse(q);
+    }
+
+    // CGLinearSolver<GraphScattered>::solve (CGLinearSolver.inl:71-315); warmStart = false
+    void cgSolve(std::vector<Vec3>& xs, const std::vector<Vec3>& b, Real mF, Real bF, Real kF) {
+        const size_t N = n();
+        std::vector<Vec3> p(N), q(N), r(N);
+        Real rho, rho_1 = 0, alpha, beta;
+        for (auto& e : xs) e = Vec3();                        // x.clear()
+        r = b;                                                // r = b
+        const Real normb = std::sqrt(vdot(b, b));             // b.norm()
+        unsigned nb_iter = 0;
+        int endcond = 0;
+        lastErr = 1; lastDen = 0;
+        if (normb != 0.0) {
+            for (nb_iter = 1; nb_iter <= cgIterations; nb_iter++) {
+                rho = vdot(r, r);
+                const Real normr = std::sqrt(rho);
+                const Real err = normr / normb;
+                lastErr = err;
+                if (err <= cgTolerance) {
+                    if (nb_iter == 1 && timeStepCount == 0) {
+                        // warning only: the loop goes on
+                    } else {
+                        if (nb_iter == 1 && !equilibriumReached) equilibriumReached = true;
+                        if (nb_iter > 1) equilibriumReached = false;
+                        endcond = 1;
+                        break;
+                    }
+                }
+                if (nb_iter == 1) {
+                    p = r;
+                } else {
+                    beta = rho / rho_1;
+                    // cgstep_beta, GraphScattered: p.eq(r, p, beta) -> vOp_avf: p *= beta; p += r
+                    for (size_t i = 0; i < N; i++)
+                        for (int k = 0; k < 3; k++) { p[i][k] *= beta; p[i][k] += r[i][k]; }
+                }
+                applyA(q, p, mF, bF, kF);
+                const Real den = vdot(p, q);
+                lastDen = den;
+                if (den != 0.0) {
+                    if (std::fabs(den) <= cgThreshold) {
+                        if (nb_iter == 1 && timeStepCount == 0) {
+                            // warning only
+                        } else {
+                            if (nb_iter == 1 && !equilibriumReached) equilibriumReached = true;
+                            if (nb_iter > 1) equilibriumReached = false;
+                            endcond = 2;
+                            break;
+                        }
+                    }
+                    alpha = rho / den;
+                    // cgstep_alpha: vMultiOp -> vOp(x, x, p, alpha) and vOp(r, r, q, -alpha)
+                    for (size_t i = 0; i < N; i++)
+                        for (int k = 0; k < 3; k++) xs[i][k] += p[i][k] * alpha;
+                    for (size_t i = 0; i < N; i++)
+                        for (int k = 0; k < 3; k++) r[i][k] += q[i][k] * (-alpha);
+                } else {
+                    endcond = 3;
+                    break;
+                }
+                rho_1 = rho;
+            }
+            if (nb_iter > cgIterations) nb_iter = cgIterations;
+        } else {
+            endcond = 4;
+        }
+        timeStepCount++;
+        lastIterations = (int)nb_iter;
+        lastEndCond = endcond;
+    }
+
+    // EulerImplicitSolver::solve (EulerImplicitSolver.cpp:83-300), firstOrder = false,
+    // trapezoidalScheme = false, solveConstraint = false; then DefaultAnimationLoop.
+    void step(Real dt) {
+        const size_t N = n();
+        const Real h = dt;
+        const Real tr = 1.0;
+        // computeForce(f, clear, accumulate): resetForce then addForce of every force field
+        f.assign(N, Vec3());
+        for (ForceField* ff : forceFields()) ff->addForce(f, x, v);
+        // b = f; b += (-rm M + (h+rs) K) v (addMBKv, B = 0); b *= h; project
+        std::vector<Vec3> b = f;
+        {
+            const Real mF = -rayleighMass, bF = 0, kF = h * tr + rayleighStiffness;
+            for (ForceField* ff : forceFields()) ff->addMBKdx(b, v, mF, bF, kF);
+        }
+        for (auto& e : b)
+            for (int k = 0; k < 3; k++) e[k] *= h;
+        projectResponse(b);
+        // system factors (:167-173)
+        const Real mFact = 1 + tr * h * rayleighMass;
+        const Real bFact = -tr * h;
+        const Real kFact = -tr * h * (tr * h + rayleighStiffness);
+        std::vector<Vec3> xs(N);
+        cgSolve(xs, b, mFact, bFact, kFact);
+        // vMultiOp fast path (MechanicalObject.inl:2209-2230): v += x; pos += v*h
+        const Real fxv = (Real)h;
+        for (size_t i = 0; i < N; i++)
+            for (int k = 0; k < 3; k++) {
+                v[i][k] += xs[i][k];
+                x[i][k] += v[i][k] * fxv;
+            }
+        // projectPositionAndVelocity with the context time still at the step start
+        // (OscillatorProjectiveConstraint.inl:77-106; FixedProjectiveConstraint's position and
+        // velocity projections are no-ops by default)
+        for (const auto& o : oscillators) {
+            const Real t = time;
+            for (int k = 0; k < 3; k++) {
+                x[o.index][k] = o.mean[k] + o.amplitude[k] * std::sin(o.pulsation * t + o.phase);
+                v[o.index][k] = o.amplitude[k] * o.pulsation * std::cos(o.pulsation * t + o.phase);
+            }
+        }
+        time += dt;
+    }
+};
+
+} // namespace oracle
+
+// ==========================================================================================
+// C interface for the tests (ctypes)
+// ==========================================================================================
+using namespace oracle;
+
+extern "C" {
+
+void* orc_create(int n_nodes, const double* positions) {
+    Scene* s = new Scene();
+    s->x.resize(n_nodes);
+    for (int i = 0; i < n_nodes; i++) s->x[i] = Vec3(positions[3 * i], positions[3 * i + 1], positions[3 * i + 2]);
+    s->x0 = s->x;
+    s->v.assign(n_nodes, Vec3());
+    s->f.assign(n_nodes, Vec3());
+    return s;
+}
+
+void orc_destroy(void* h) { delete static_cast<Scene*>(h); }
+
+void orc_set_gravity(void* h, double gx, double gy, double gz) {
+    Scene* s = static_cast<Scene*>(h);
+    s->gravity = Vec3(gx, gy, gz);
+    if (s->mass) s->mass->gravity = s->gravity;
+}
+
+int orc_set_fem(void* h, int n_tets, const int* tets, const double* young, int n_young,
+                const double* poisson, int n_poisson, int method) {
+    Scene* s = static_cast<Scene*>(h);
+    std::vector<Tetra> t(n_tets);
+    for (int i = 0; i < n_tets; i++)
+        for (int k = 0; k < 4; k++) {
+            t[i][k] = tets[4 * i + k];
+            if (t[i][k] < 0 || (size_t)t[i][k] >= s->n()) return -1;
+        }
+    s->fem.reset(new TetraFEM());
+    s->fem->init(s->x0, t, std::vector<Real>(young, young + n_young), std::vector<Real>(poisson, poisson + n_poisson), method);
+    return 0;
+}
+
+void orc_set_uniform_mass(void* h, double vertex_mass) {
+    Scene* s = static_cast<Scene*>(h);
+    UniformMass* m = new UniformMass();
+    m->vertexMass = vertex_mass;
+    m->gravity = s->gravity;
+    s->mass.reset(m);
+}
+
+// UniformMass::initFromTotalMass (UniformMass.inl:329-345): vertexMass = totalMass / Real(n)
+void orc_set_uniform_total_mass(void* h, double total_mass) {
+    Scene* s = static_cast<Scene*>(h);
+    orc_set_uniform_mass(h, total_mass / Real(s->n()));
+}
+
+void orc_set_diagonal_mass(void* h, double density) {
+    Scene* s = static_cast<Scene*>(h);
+    DiagonalMass* m = new DiagonalMass();
+    m->init(s->x0, s->fem ? s->fem->elems : std::vector<Tetra>(), density);
+    m->gravity = s->gravity;
+    s->mass.reset(m);
+}
+
+void orc_add_axis_spring(void* h, int node, int axis, double stiffness, double anchor) {
+    Scene* s = static_cast<Scene*>(h);
+    AxisSpring* sp = new AxisSpring();
+    sp->node = node; sp->axis = axis; sp->stiffness = stiffness; sp->anchor = anchor;
+    s->extra.emplace_back(sp);
+}
+
+void orc_set_fixed(void* h, const int* idx, int n, int fix_all) {
+    Scene* s = static_cast<Scene*>(h);
+    s->fixed.assign(idx, idx + n);
+    s->fixAll = fix_all != 0;
+}
+
+void orc_add_oscillator(void* h, int index, const double* mean, const double* amplitude, double pulsation, double phase) {
+    Scene* s = static_cast<Scene*>(h);
+    Oscillator o;
+    o.index = index;
+    o.mean = Vec3(mean[0], mean[1], mean[2]);
+    o.amplitude = Vec3(amplitude[0], amplitude[1], amplitude[2]);
+    o.pulsation = pulsation;
+    o.phase = phase;
+    s->oscillators.push_back(o);
+}
+
+void orc_set_solver(void* h, double rayleigh_stiffness, double rayleigh_mass, int cg_iterations,
+                    double cg_tolerance, double cg_threshold) {
+    Scene* s = static_cast<Scene*>(h);
+    s->rayleighStiffness = rayleigh_stiffness;
+    s->rayleighMass = rayleigh_mass;
+    s->cgIterations = (unsigned)cg_iterations;
+    s->cgTolerance = cg_tolerance < 0 ? 1e-5 : cg_tolerance;     // CGLinearSolver::init :49-69
+    s->cgThreshold = cg_threshold < 0 ? 1e-5 : cg_threshold;
+}
+
+void orc_set_dot_mode(void* h, int mode) { static_cast<Scene*>(h)->dotMode = mode; }
+
+void orc_step(void* h, double dt, int nsteps) {
+    Scene* s = static_cast<Scene*>(h);
+    for (int i = 0; i < nsteps; i++) s->step(dt);
+}
+
+// which: 0 position, 1 velocity, 2 force (last computeForce), 3 rest position
+void orc_get_vec(void* h, int which, double* out) {
+    Scene* s = static_cast<Scene*>(h);
+    const std::vector<Vec3>* src = which == 0 ? &s->x : which == 1 ? &s->v : which == 2 ? &s->f : &s->x0;
+    for (size_t i = 0; i < src->size(); i++)
+        for (int k = 0; k < 3; k++) out[3 * i + k] = (*src)[i][k];
+}
+
+void orc_set_vec(void* h, int which, const double* in) {
+    Scene* s = static_cast<Scene*>(h);
+    std::vector<Vec3>* dst = which == 0 ? &s->x : &s->v;
+    for (size_t i = 0; i < dst->size(); i++)
+        for (int k = 0; k < 3; k++) (*dst)[i][k] = in[3 * i + k];
+}
+
+// stats: [iterations, endcond, timeStepCount], [lastErr, lastDen, time]
+void orc_get_stats(void* h, int* istats, double* dstats) {
+    Scene* s = static_cast<Scene*>(h);
+    istats[0] = s->lastIterations;
+    istats[1] = s->lastEndCond;
+    istats[2] = s->timeStepCount;
+    dstats[0] = s->lastErr;
+    dstats[1] = s->lastDen;
+    dstats[2] = s->time;
+}
+
+// Component-level calls, the per-call unit of TetrahedronFEMForceField:
+// f += addForce(x) (updates the rotations); df += addDForce(dx, kFactor)
+void orc_fem_add_force(void* h, const double* xin, double* fio) {
+    Scene* s = static_cast<Scene*>(h);
+    const size_t N = s->n();
+    std::vector<Vec3> xx(N), ff(N), vv(N);
+    for (size_t i = 0; i < N; i++) {
+        xx[i] = Vec3(xin[3 * i], xin[3 * i + 1], xin[3 * i + 2]);
+        ff[i] = Vec3(fio[3 * i], fio[3 * i + 1], fio[3 * i + 2]);
+    }
+    s->fem->addForce(ff, xx, vv);
+    for (size_t i = 0; i < N; i++)
+        for (int k = 0; k < 3; k++) fio[3 * i + k] = ff[i][k];
+}
+
+void orc_fem_add_dforce(void* h, const double* dxin, double* dfio, double kFactor) {
+    Scene* s = static_cast<Scene*>(h);
+    const size_t N = s->n();
+    std::vector<Vec3> dx(N), df(N);
+    for (size_t i = 0; i < N; i++) {
+        dx[i] = Vec3(dxin[3 * i], dxin[3 * i + 1], dxin[3 * i + 2]);
+        df[i] = Vec3(dfio[3 * i], dfio[3 * i + 1], dfio[3 * i + 2]);
+    }
+    s->fem->addDForce(df, dx, (Real)kFactor);
+    for (size_t i = 0; i < N; i++)
+        for (int k = 0; k < 3; k++) dfio[3 * i + k] = df[i][k];
+}
+
+// element data as the reference stores it: R0 (_initialRotations), R (rotations),
+// K (6x6), J (12x6), rotated rest coordinates (4x3)
+void orc_fem_get_element(void* h, int e, double* R0, double* R, double* K, double* J, double* rest) {
+    Scene* s = static_cast<Scene*>(h);
+    TetraFEM* fem = s->fem.get();
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+            if (fem->method == TetraFEM::LARGE) {
+                R0[3 * i + j] = fem->initialRotations[e].m[i][j];
+                R[3 * i + j] = fem->rotations[e].m[i][j];
+            } else {
+                R0[3 * i + j] = R[3 * i + j] = (i == j) ? 1.0 : 0.0;
+            }
+        }
+    for (int i = 0; i < 36; i++) K[i] = fem->K[e][i];
+    for (int i = 0; i < 72; i++) J[i] = fem->J[e][i];
+    for (int k = 0; k < 4; k++)
+        for (int q = 0; q < 3; q++)
+            rest[3 * k + q] = (fem->method == TetraFEM::LARGE) ? fem->rotatedInitial[e][k][q] : fem->initialPoints[fem->elems[e][k]][q];
+}
+
+double orc_fem_rest_volume(void* h) { return static_cast<Scene*>(h)->fem->restVolume; }
+
+void orc_get_masses(void* h, double* out) {
+    Scene* s = static_cast<Scene*>(h);
+    if (auto* d = dynamic_cast<DiagonalMass*>(s->mass.get())) {
+        for (size_t i = 0; i < d->masses.size(); i++) out[i] = d->masses[i];
+    } else if (auto* u = dynamic_cast<UniformMass*>(s->mass.get())) {
+        for (size_t i = 0; i < s->n(); i++) out[i] = u->vertexMass;
+    }
+}
+
+} // extern "C"

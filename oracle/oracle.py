@@ -1,0 +1,169 @@
+"""ctypes loader for the FP64 CPU oracle (fem_oracle.cpp) -- TEST INFRASTRUCTURE ONLY.
+
+Imported by tests/, by __graft_entry__.smoke() (as the checker) and by bench.py's
+cpu_baseline leg.  The product package (sofa_amd) never imports it.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+
+
+def build():
+    subprocess.check_call(["make", "-s", "-C", _HERE, "liboracle.so"])
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        L.orc_create.restype = ctypes.c_void_p
+        L.orc_create.argtypes = [ctypes.c_int, _dp]
+        L.orc_destroy.argtypes = [ctypes.c_void_p]
+        L.orc_set_gravity.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_double]
+        L.orc_set_fem.restype = ctypes.c_int
+        L.orc_set_fem.argtypes = [ctypes.c_void_p, ctypes.c_int, _ip, _dp, ctypes.c_int, _dp, ctypes.c_int, ctypes.c_int]
+        L.orc_set_uniform_mass.argtypes = [ctypes.c_void_p, ctypes.c_double]
+        L.orc_set_uniform_total_mass.argtypes = [ctypes.c_void_p, ctypes.c_double]
+        L.orc_set_diagonal_mass.argtypes = [ctypes.c_void_p, ctypes.c_double]
+        L.orc_add_axis_spring.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double]
+        L.orc_set_fixed.argtypes = [ctypes.c_void_p, _ip, ctypes.c_int, ctypes.c_int]
+        L.orc_add_oscillator.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp, ctypes.c_double, ctypes.c_double]
+        L.orc_set_solver.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_double, ctypes.c_double]
+        L.orc_set_dot_mode.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.orc_step.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_int]
+        L.orc_get_vec.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp]
+        L.orc_set_vec.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp]
+        L.orc_get_stats.argtypes = [ctypes.c_void_p, _ip, _dp]
+        L.orc_fem_add_force.argtypes = [ctypes.c_void_p, _dp, _dp]
+        L.orc_fem_add_dforce.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_double]
+        L.orc_fem_get_element.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp, _dp, _dp, _dp]
+        L.orc_fem_rest_volume.restype = ctypes.c_double
+        L.orc_fem_rest_volume.argtypes = [ctypes.c_void_p]
+        L.orc_get_masses.argtypes = [ctypes.c_void_p, _dp]
+        _lib = L
+    return _lib
+
+
+def _d(a):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return a, a.ctypes.data_as(_dp)
+
+
+def _i(a):
+    a = np.ascontiguousarray(a, dtype=np.int32)
+    return a, a.ctypes.data_as(_ip)
+
+
+class OracleScene:
+    """One SOFA scene: MechanicalObject + mass + TetrahedronFEMForceField + constraints,
+    stepped by EulerImplicitSolver + CGLinearSolver in FP64 on the CPU."""
+
+    def __init__(self, positions):
+        L = lib()
+        self.n = int(len(positions))
+        pos, pp = _d(positions)
+        self._h = ctypes.c_void_p(L.orc_create(self.n, pp))
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.orc_destroy(self._h)
+            self._h = None
+
+    def set_gravity(self, g):
+        lib().orc_set_gravity(self._h, float(g[0]), float(g[1]), float(g[2]))
+
+    def set_fem(self, tets, young, poisson, method="large"):
+        t, tp = _i(np.asarray(tets).reshape(-1))
+        y, yp = _d(np.atleast_1d(young))
+        p, pp = _d(np.atleast_1d(poisson))
+        m = {"large": 0, "small": 1}[method]
+        rc = lib().orc_set_fem(self._h, len(t) // 4, tp, yp, len(y), pp, len(p), m)
+        if rc != 0:
+            raise ValueError("invalid tetrahedron index")
+
+    def set_uniform_mass(self, vertex_mass=None, total_mass=None):
+        if total_mass is not None:
+            lib().orc_set_uniform_total_mass(self._h, float(total_mass))
+        else:
+            lib().orc_set_uniform_mass(self._h, float(vertex_mass))
+
+    def set_diagonal_mass(self, density):
+        lib().orc_set_diagonal_mass(self._h, float(density))
+
+    def add_axis_spring(self, node, axis, stiffness, anchor):
+        lib().orc_add_axis_spring(self._h, int(node), int(axis), float(stiffness), float(anchor))
+
+    def set_fixed(self, idx, fix_all=False):
+        i, ip = _i(idx)
+        lib().orc_set_fixed(self._h, ip, len(i), 1 if fix_all else 0)
+
+    def add_oscillator(self, index, mean, amplitude, pulsation, phase=0.0):
+        m, mp = _d(mean)
+        a, ap = _d(amplitude)
+        lib().orc_add_oscillator(self._h, int(index), mp, ap, float(pulsation), float(phase))
+
+    def set_solver(self, rayleigh_stiffness=0.0, rayleigh_mass=0.0, iterations=25, tolerance=1e-5, threshold=1e-5):
+        lib().orc_set_solver(self._h, float(rayleigh_stiffness), float(rayleigh_mass), int(iterations),
+                             float(tolerance), float(threshold))
+
+    def set_dot_mode(self, mode):
+        lib().orc_set_dot_mode(self._h, int(mode))
+
+    def step(self, dt, nsteps=1):
+        lib().orc_step(self._h, float(dt), int(nsteps))
+
+    def get(self, which="x"):
+        out = np.zeros((self.n, 3), dtype=np.float64)
+        lib().orc_get_vec(self._h, {"x": 0, "v": 1, "f": 2, "x0": 3}[which], out.ctypes.data_as(_dp))
+        return out
+
+    def set(self, which, arr):
+        a, ap = _d(arr)
+        lib().orc_set_vec(self._h, {"x": 0, "v": 1}[which], ap)
+
+    def stats(self):
+        ist = np.zeros(3, dtype=np.int32)
+        dst = np.zeros(3, dtype=np.float64)
+        lib().orc_get_stats(self._h, ist.ctypes.data_as(_ip), dst.ctypes.data_as(_dp))
+        return {"iterations": int(ist[0]), "endcond": int(ist[1]), "step_count": int(ist[2]),
+                "err": float(dst[0]), "den": float(dst[1]), "time": float(dst[2])}
+
+    def fem_add_force(self, x, f):
+        x, xp = _d(x)
+        f = np.array(f, dtype=np.float64, copy=True)
+        lib().orc_fem_add_force(self._h, xp, f.ctypes.data_as(_dp))
+        return f
+
+    def fem_add_dforce(self, dx, df, kfactor):
+        dx, dxp = _d(dx)
+        df = np.array(df, dtype=np.float64, copy=True)
+        lib().orc_fem_add_dforce(self._h, dxp, df.ctypes.data_as(_dp), float(kfactor))
+        return df
+
+    def fem_element(self, e):
+        R0 = np.zeros(9); R = np.zeros(9); K = np.zeros(36); J = np.zeros(72); rest = np.zeros(12)
+        lib().orc_fem_get_element(self._h, int(e), R0.ctypes.data_as(_dp), R.ctypes.data_as(_dp),
+                                  K.ctypes.data_as(_dp), J.ctypes.data_as(_dp), rest.ctypes.data_as(_dp))
+        return {"R0": R0.reshape(3, 3), "R": R.reshape(3, 3), "K": K.reshape(6, 6), "J": J.reshape(12, 6),
+                "rest": rest.reshape(4, 3)}
+
+    def rest_volume(self):
+        return lib().orc_fem_rest_volume(self._h)
+
+    def masses(self):
+        out = np.zeros(self.n, dtype=np.float64)
+        lib().orc_get_masses(self._h, out.ctypes.data_as(_dp))
+        return out

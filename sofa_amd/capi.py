@@ -1,0 +1,197 @@
+"""ctypes binding of libsofa_amd.so (include/sofa_amd.h).
+
+The library is the product: this module only marshals arguments.  It raises if the library
+has not been built -- there is no CPU fallback anywhere in the package.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libsofa_amd.so")
+
+OK, EINVAL, EDEVICE, ENOMEM, EDEGENERATE, ESTATE, ENAN = 0, -1, -2, -3, -4, -5, -6
+FP64, FP32, MIXED = 0, 1, 2
+MASS_UNIFORM, MASS_DIAGONAL = 0, 1
+END_REASONS = {0: "iterations", 1: "tolerance", 2: "threshold", 3: "den==0", 4: "null norm of vector b"}
+
+# every symbol include/sofa_amd.h declares
+EXPORTS = [
+    "sofa_amd_module_name", "sofa_amd_module_version", "sofa_amd_last_error", "sofa_amd_device_check",
+    "sofa_amd_engine_create", "sofa_amd_engine_destroy", "sofa_amd_scene_create", "sofa_amd_scene_set_fem",
+    "sofa_amd_scene_set_mass", "sofa_amd_scene_set_gravity", "sofa_amd_scene_set_fixed",
+    "sofa_amd_scene_set_solver", "sofa_amd_engine_commit", "sofa_amd_engine_step", "sofa_amd_engine_sync",
+    "sofa_amd_scene_read", "sofa_amd_scene_write", "sofa_amd_scene_stats", "sofa_amd_scene_element",
+    "sofa_amd_fem_add_force", "sofa_amd_fem_add_dforce",
+]
+
+
+class SofaAmdError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"sofa_amd error {code}: {msg}")
+        self.code = code
+
+
+class CGStats(ctypes.Structure):
+    _fields_ = [("iterations", ctypes.c_int32), ("end_reason", ctypes.c_int32), ("step_count", ctypes.c_int32),
+                ("ir_sweeps", ctypes.c_int32), ("err", ctypes.c_double), ("den", ctypes.c_double),
+                ("time", ctypes.c_double)]
+
+
+_lib = None
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int32)
+_vp = ctypes.c_void_p
+
+
+def lib():
+    """Load libsofa_amd.so; raises if it was not built (no fallback)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: build it with `python -m sofa_amd.build` "
+                          "(or __graft_entry__.build()); there is no CPU fallback")
+    L = ctypes.CDLL(LIB_PATH)
+    c_int, c_double = ctypes.c_int, ctypes.c_double
+    sig = {
+        "sofa_amd_module_name": (ctypes.c_char_p, []),
+        "sofa_amd_module_version": (ctypes.c_char_p, []),
+        "sofa_amd_last_error": (ctypes.c_char_p, []),
+        "sofa_amd_device_check": (c_int, [c_int]),
+        "sofa_amd_engine_create": (c_int, [c_int, c_int, ctypes.POINTER(_vp)]),
+        "sofa_amd_engine_destroy": (c_int, [_vp]),
+        "sofa_amd_scene_create": (c_int, [_vp, c_int, _dp, _ip]),
+        "sofa_amd_scene_set_fem": (c_int, [_vp, c_int, c_int, _ip, _dp, c_int, _dp, c_int]),
+        "sofa_amd_scene_set_mass": (c_int, [_vp, c_int, c_int, c_double]),
+        "sofa_amd_scene_set_gravity": (c_int, [_vp, c_int, c_double, c_double, c_double]),
+        "sofa_amd_scene_set_fixed": (c_int, [_vp, c_int, _ip, c_int]),
+        "sofa_amd_scene_set_solver": (c_int, [_vp, c_int, c_double, c_double, c_double, c_int, c_double, c_double]),
+        "sofa_amd_engine_commit": (c_int, [_vp]),
+        "sofa_amd_engine_step": (c_int, [_vp, _ip, c_int, c_int, _vp]),
+        "sofa_amd_engine_sync": (c_int, [_vp]),
+        "sofa_amd_scene_read": (c_int, [_vp, c_int, _dp, _dp, _dp]),
+        "sofa_amd_scene_write": (c_int, [_vp, c_int, _dp, _dp]),
+        "sofa_amd_scene_stats": (c_int, [_vp, c_int, ctypes.POINTER(CGStats)]),
+        "sofa_amd_scene_element": (c_int, [_vp, c_int, c_int, _dp, _dp, _dp]),
+        "sofa_amd_fem_add_force": (c_int, [_vp, c_int, _vp, _vp, _vp]),
+        "sofa_amd_fem_add_dforce": (c_int, [_vp, c_int, _vp, _vp, c_double, _vp]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def check(rc):
+    if rc != OK:
+        msg = lib().sofa_amd_last_error().decode(errors="replace")
+        raise SofaAmdError(rc, msg)
+    return rc
+
+
+def _d(a):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return a, a.ctypes.data_as(_dp)
+
+
+def _i(a):
+    a = np.ascontiguousarray(a, dtype=np.int32)
+    return a, a.ctypes.data_as(_ip)
+
+
+class Engine:
+    """A set of device-resident scenes advanced together (one launch per phase)."""
+
+    def __init__(self, device=0, precision=FP64):
+        L = lib()
+        h = _vp()
+        check(L.sofa_amd_engine_create(int(device), int(precision), ctypes.byref(h)))
+        self._h = h
+        self.precision = precision
+        self.n_nodes = []
+        self.n_tets = []
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().sofa_amd_engine_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- scene construction ----
+    def add_scene(self, positions, tets, young, poisson, fixed, mass_kind=MASS_UNIFORM, mass_value=1.0,
+                  gravity=(0.0, 0.0, 0.0), dt=0.01, rayleigh_stiffness=0.0, rayleigh_mass=0.0,
+                  cg_iterations=25, cg_tolerance=1e-5, cg_threshold=1e-5):
+        L = lib()
+        pos, pp = _d(np.asarray(positions).reshape(-1, 3))
+        sid = ctypes.c_int32()
+        check(L.sofa_amd_scene_create(self._h, len(pos), pp, ctypes.byref(sid)))
+        s = sid.value
+        t, tp = _i(np.asarray(tets).reshape(-1))
+        y, yp = _d(np.atleast_1d(young))
+        nu, nup = _d(np.atleast_1d(poisson))
+        check(L.sofa_amd_scene_set_fem(self._h, s, len(t) // 4, tp, yp, len(y), nup, len(nu)))
+        check(L.sofa_amd_scene_set_mass(self._h, s, int(mass_kind), float(mass_value)))
+        check(L.sofa_amd_scene_set_gravity(self._h, s, *[float(g) for g in gravity]))
+        fx, fxp = _i(np.atleast_1d(fixed) if len(np.atleast_1d(fixed)) else np.zeros(0, np.int32))
+        check(L.sofa_amd_scene_set_fixed(self._h, s, fxp, len(fx)))
+        check(L.sofa_amd_scene_set_solver(self._h, s, float(dt), float(rayleigh_stiffness), float(rayleigh_mass),
+                                          int(cg_iterations), float(cg_tolerance), float(cg_threshold)))
+        self.n_nodes.append(len(pos))
+        self.n_tets.append(len(t) // 4)
+        return s
+
+    def commit(self):
+        check(lib().sofa_amd_engine_commit(self._h))
+
+    # ---- stepping ----
+    def step(self, scenes=None, n_steps=1, stream=None):
+        ids = np.arange(len(self.n_nodes), dtype=np.int32) if scenes is None else np.asarray(scenes, dtype=np.int32)
+        a, ap = _i(ids)
+        check(lib().sofa_amd_engine_step(self._h, ap, len(a), int(n_steps), stream))
+
+    def sync(self):
+        check(lib().sofa_amd_engine_sync(self._h))
+
+    # ---- state ----
+    def read(self, scene):
+        n = self.n_nodes[scene]
+        x = np.zeros((n, 3)); v = np.zeros((n, 3)); f = np.zeros((n, 3))
+        check(lib().sofa_amd_scene_read(self._h, int(scene), x.ctypes.data_as(_dp), v.ctypes.data_as(_dp),
+                                        f.ctypes.data_as(_dp)))
+        return x, v, f
+
+    def write(self, scene, x=None, v=None):
+        xa = None if x is None else _d(np.asarray(x).reshape(-1))
+        va = None if v is None else _d(np.asarray(v).reshape(-1))
+        check(lib().sofa_amd_scene_write(self._h, int(scene), None if xa is None else xa[1],
+                                         None if va is None else va[1]))
+
+    def stats(self, scene):
+        st = CGStats()
+        check(lib().sofa_amd_scene_stats(self._h, int(scene), ctypes.byref(st)))
+        return {"iterations": st.iterations, "end_reason": st.end_reason, "step_count": st.step_count,
+                "ir_sweeps": st.ir_sweeps, "err": st.err, "den": st.den, "time": st.time}
+
+    def element(self, scene, e):
+        rot = np.zeros(9); k = np.zeros(3); rest = np.zeros(12)
+        check(lib().sofa_amd_scene_element(self._h, int(scene), int(e), rot.ctypes.data_as(_dp),
+                                           k.ctypes.data_as(_dp), rest.ctypes.data_as(_dp)))
+        return rot.reshape(3, 3), k, rest.reshape(4, 3)
+
+    # ---- component-level calls on device pointers ----
+    def fem_add_force(self, scene, x_dev_ptr, f_dev_ptr, stream=None):
+        check(lib().sofa_amd_fem_add_force(self._h, int(scene), x_dev_ptr, f_dev_ptr, stream))
+
+    def fem_add_dforce(self, scene, dx_dev_ptr, df_dev_ptr, k_factor, stream=None):
+        check(lib().sofa_amd_fem_add_dforce(self._h, int(scene), dx_dev_ptr, df_dev_ptr, float(k_factor), stream))

@@ -1,0 +1,334 @@
+// fem_device.h -- gfx950 device code of the corotational tetrahedral FEM step.
+//
+// Element math restates TetrahedronFEMForceField (method "large"):
+//   computeRotationLarge              TetrahedronFEMForceField.inl:755-778
+//   accumulateForceLarge              TetrahedronFEMForceField.inl:870-986
+//   applyStiffnessCorotational        TetrahedronFEMForceField.inl:1193-1237
+//   computeForce (both forms)         TetrahedronFEMForceField.inl:293-521
+//   computeStrainDisplacement / pd    TetrahedronFEMForceField.inl:134-209
+// with the same operation order, so that with contraction off the FP64 kernels produce the
+// reference's bits.  The strain-displacement matrix J is never stored: its 12 distinct
+// entries ("groups" g0..g11, each repeated on three (row, column) positions) are recomputed
+// from the element's rotated rest coordinates.  Exact layout (FP64): the 9 rest coordinates
+// b, c, d of the local frame (a = 0), so every pseudo-determinant is evaluated exactly as the
+// reference does, rounding noise included.  Compact layout (FP32): the 6 coordinates that are
+// non-zero in exact arithmetic (b0, c0, c1, d0, d1, d2); g6 = g9 = g10 = 0 there.
+//
+// Element arrays are structure-of-arrays (one coalesced stream per component); node vectors
+// are AoS xyz triples, the layout SOFA hands over (VecTypes.h:200-234), which also keeps the
+// 4 node gathers of an element at one or two cache lines each.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <float.h>
+
+namespace sofa_amd {
+
+constexpr int kBlock = 256;   // threads per block for every kernel (4 wave64)
+
+// ---------------------------------------------------------------------------------------
+// per-scene device record; one per scene, written by the last block of each reduction
+// ---------------------------------------------------------------------------------------
+struct InstDev {
+    // ranges in the flattened index spaces
+    int node_begin, node_count, elem_begin, elem_count;
+    int nblk_begin, nblk_count;      // node blocks of this scene (reduction partials)
+    int mass_kind;                    // 0 uniform, 1 diagonal
+    int max_iter;
+    double dt, rm, rs;
+    double vertex_mass;
+    double gravity[3];
+    double mg[3];                     // gravity * vertexMass (UniformMass.inl:486-487)
+    double k_rhs, m_rhs;              // (h + rs), -rm           (EulerImplicitSolver.cpp:152-154)
+    double k_cg, m_cg;                // -h(h + rs), 1 + h rm    (EulerImplicitSolver.cpp:169-171)
+    double tol, thr;
+    // CG scalars (CGLinearSolver.inl:86-270)
+    double normb, rho, rho_1, den, alpha, beta, err;
+    int iter;                         // current nb_iter
+    int stopped;                      // 1 once the loop has exited
+    int end_reason;                   // 0 iterations, 1 tolerance, 2 threshold, 3 den==0, 4 |b|==0
+    int first_solve;                  // timeStepCount == 0
+    int step_count;
+    int last_iters;
+    double time;
+    unsigned int counter;             // arrivals of the current reduction
+    int pad_;
+};
+
+struct Blk { int inst, begin, end, pad; };   // a block's scene and [begin, end) range
+
+// ---------------------------------------------------------------------------------------
+// scalar helpers
+// ---------------------------------------------------------------------------------------
+template <typename R> struct Eps;
+template <> struct Eps<double> { static constexpr double v = DBL_EPSILON; };
+template <> struct Eps<float> { static constexpr float v = FLT_EPSILON; };
+
+template <typename R>
+__device__ __forceinline__ R pdet(R m00, R m01, R m02, R m10, R m11, R m12) {
+    // peudo_determinant_for_coef, TetrahedronFEMForceField.inl:205-209
+    return m01 * m12 - m11 * m02 - m00 * m12 + m10 * m02 + m00 * m11 - m10 * m01;
+}
+
+// normalize per Vec.h:535-578: divide by sqrt(norm2) unless the norm is <= epsilon
+template <typename R>
+__device__ __forceinline__ void normalize3(R& x, R& y, R& z) {
+    R n2 = x * x;
+    n2 += y * y;
+    n2 += z * z;
+    const R n = sqrt(n2);
+    if (n > Eps<R>::v) { x /= n; y /= n; z /= n; }
+}
+
+template <typename T> __device__ __forceinline__ void load3(const T* __restrict__ v, int n, T& a, T& b, T& c) {
+    const T* p = v + 3 * (int64_t)n;
+    a = p[0]; b = p[1]; c = p[2];
+}
+
+// ---------------------------------------------------------------------------------------
+// element record view
+// ---------------------------------------------------------------------------------------
+template <typename R, int NR>
+struct ElemView {
+    const int4* __restrict__ tet;   // global node ids
+    R* __restrict__ rot;            // 9 SoA streams: rotations[e](i,j) at rot[(3i+j)*E + e]
+    const R* __restrict__ rest;     // NR SoA streams of rotated rest coordinates
+    const R* __restrict__ kmat;     // 3 SoA streams: K00, K01, K33 (already / (36 V))
+    int64_t E;                      // stride of the SoA streams
+};
+
+// the 12 distinct strain-displacement entries of one element
+template <typename R, int NR>
+__device__ __forceinline__ void jgroups(const ElemView<R, NR>& ev, int64_t e, R g[12], R rc[9]) {
+    if constexpr (NR == 9) {
+#pragma unroll
+        for (int k = 0; k < 9; k++) rc[k] = ev.rest[k * ev.E + e];
+        const R a0 = 0, a1 = 0, a2 = 0;
+        const R b0 = rc[0], b1 = rc[1], b2 = rc[2];
+        const R c0 = rc[3], c1 = rc[4], c2 = rc[5];
+        const R d0 = rc[6], d1 = rc[7], d2 = rc[8];
+        g[0] = -pdet(b1, c1, d1, b2, c2, d2);
+        g[1] = pdet(b0, c0, d0, b2, c2, d2);
+        g[2] = -pdet(b0, c0, d0, b1, c1, d1);
+        g[3] = pdet(c1, d1, a1, c2, d2, a2);
+        g[4] = -pdet(c0, d0, a0, c2, d2, a2);
+        g[5] = pdet(c0, d0, a0, c1, d1, a1);
+        g[6] = -pdet(d1, a1, b1, d2, a2, b2);
+        g[7] = pdet(d0, a0, b0, d2, a2, b2);
+        g[8] = -pdet(d0, a0, b0, d1, a1, b1);
+        g[9] = pdet(a1, b1, c1, a2, b2, c2);
+        g[10] = -pdet(a0, b0, c0, a2, b2, c2);
+        g[11] = pdet(a0, b0, c0, a1, b1, c1);
+    } else {
+        // compact: rc = b0, c0, c1, d0, d1, d2 (b1 = b2 = c2 = 0)
+        R r6[6];
+#pragma unroll
+        for (int k = 0; k < 6; k++) r6[k] = ev.rest[k * ev.E + e];
+        const R b0 = r6[0], c0 = r6[1], c1 = r6[2], d0 = r6[3], d1 = r6[4], d2 = r6[5];
+        rc[0] = b0; rc[1] = 0; rc[2] = 0; rc[3] = c0; rc[4] = c1; rc[5] = 0; rc[6] = d0; rc[7] = d1; rc[8] = d2;
+        g[0] = -(c1 * d2);
+        g[1] = c0 * d2 - b0 * d2;
+        g[2] = -(((c0 * d1 - c1 * d0) - b0 * d1) + b0 * c1);
+        g[3] = c1 * d2;
+        g[4] = -(c0 * d2);
+        g[5] = c0 * d1 - c1 * d0;
+        g[6] = 0;
+        g[7] = d2 * b0;
+        g[8] = -(d1 * b0);
+        g[9] = 0;
+        g[10] = 0;
+        g[11] = b0 * c1;
+    }
+}
+
+// JtD = J^T D (12 -> 6), the reference's term order (zero groups dropped in compact layout)
+template <typename R, int NR>
+__device__ __forceinline__ void jt_mul(const R g[12], const R D[12], R JtD[6]) {
+    if constexpr (NR == 9) {
+        JtD[0] = g[0] * D[0] + g[3] * D[3] + g[6] * D[6] + g[9] * D[9];
+        JtD[1] = g[1] * D[1] + g[4] * D[4] + g[7] * D[7] + g[10] * D[10];
+        JtD[2] = g[2] * D[2] + g[5] * D[5] + g[8] * D[8] + g[11] * D[11];
+        JtD[3] = g[1] * D[0] + g[0] * D[1] + g[4] * D[3] + g[3] * D[4] + g[7] * D[6] + g[6] * D[7] + g[10] * D[9] + g[9] * D[10];
+        JtD[4] = g[2] * D[1] + g[1] * D[2] + g[5] * D[4] + g[4] * D[5] + g[8] * D[7] + g[7] * D[8] + g[11] * D[10] + g[10] * D[11];
+        JtD[5] = g[2] * D[0] + g[0] * D[2] + g[5] * D[3] + g[3] * D[5] + g[8] * D[6] + g[6] * D[8] + g[11] * D[9] + g[9] * D[11];
+    } else {
+        JtD[0] = g[0] * D[0] + g[3] * D[3];
+        JtD[1] = g[1] * D[1] + g[4] * D[4] + g[7] * D[7];
+        JtD[2] = g[2] * D[2] + g[5] * D[5] + g[8] * D[8] + g[11] * D[11];
+        JtD[3] = g[1] * D[0] + g[0] * D[1] + g[4] * D[3] + g[3] * D[4] + g[7] * D[6];
+        JtD[4] = g[2] * D[1] + g[1] * D[2] + g[5] * D[4] + g[4] * D[5] + g[8] * D[7] + g[7] * D[8] + g[11] * D[10];
+        JtD[5] = g[2] * D[0] + g[0] * D[2] + g[5] * D[3] + g[3] * D[5] + g[8] * D[6] + g[11] * D[9];
+    }
+}
+
+// KJtD = K JtD, then F = J KJtD (6 -> 12)
+template <typename R>
+__device__ __forceinline__ void k_mul(const R k[3], const R JtD[6], R KJ[6]) {
+    const R K00 = k[0], K01 = k[1], K33 = k[2];
+    KJ[0] = K00 * JtD[0] + K01 * JtD[1] + K01 * JtD[2];
+    KJ[1] = K01 * JtD[0] + K00 * JtD[1] + K01 * JtD[2];
+    KJ[2] = K01 * JtD[0] + K01 * JtD[1] + K00 * JtD[2];
+    KJ[3] = K33 * JtD[3];
+    KJ[4] = K33 * JtD[4];
+    KJ[5] = K33 * JtD[5];
+}
+
+template <typename R>
+__device__ __forceinline__ void j_mul(const R g[12], const R KJ[6], R F[12]) {
+#pragma unroll
+    for (int n = 0; n < 4; n++) {
+        const R gA = g[3 * n], gB = g[3 * n + 1], gC = g[3 * n + 2];
+        F[3 * n + 0] = gA * KJ[0] + gB * KJ[3] + gC * KJ[5];
+        F[3 * n + 1] = gB * KJ[1] + gA * KJ[3] + gC * KJ[4];
+        F[3 * n + 2] = gC * KJ[2] + gB * KJ[4] + gA * KJ[5];
+    }
+}
+
+// c = rot * (F[3n], F[3n+1], F[3n+2]) (Mat.h:577-587 order)
+template <typename R>
+__device__ __forceinline__ void rot_apply(const R rot[9], const R F[12], R c[12]) {
+#pragma unroll
+    for (int n = 0; n < 4; n++) {
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+            c[3 * n + i] = rot[3 * i + 0] * F[3 * n] + rot[3 * i + 1] * F[3 * n + 1] + rot[3 * i + 2] * F[3 * n + 2];
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// element force: accumulateForceLarge; returns the rotation and the 4 nodal contributions
+// (to be ADDED to f); positions read as VT and converted to R
+// ---------------------------------------------------------------------------------------
+template <typename R, int NR, typename VT>
+__device__ __forceinline__ void elem_force(const ElemView<R, NR>& ev, int64_t e, const int4 t,
+                                           const VT* __restrict__ x, R rot[9], R c[12], R g[12]) {
+    VT xa0, xa1, xa2, xb0, xb1, xb2, xc0, xc1, xc2, xd0, xd1, xd2;
+    load3(x, t.x, xa0, xa1, xa2);
+    load3(x, t.y, xb0, xb1, xb2);
+    load3(x, t.z, xc0, xc1, xc2);
+    load3(x, t.w, xd0, xd1, xd2);
+    const R p[4][3] = {{(R)xa0, (R)xa1, (R)xa2}, {(R)xb0, (R)xb1, (R)xb2},
+                       {(R)xc0, (R)xc1, (R)xc2}, {(R)xd0, (R)xd1, (R)xd2}};
+    // computeRotationLarge (rows ex, ey, ez of R_0_2)
+    R ex0 = p[1][0] - p[0][0], ex1 = p[1][1] - p[0][1], ex2 = p[1][2] - p[0][2];
+    normalize3(ex0, ex1, ex2);
+    R ey0 = p[2][0] - p[0][0], ey1 = p[2][1] - p[0][1], ey2 = p[2][2] - p[0][2];
+    R ez0 = ex1 * ey2 - ex2 * ey1, ez1 = ex2 * ey0 - ex0 * ey2, ez2 = ex0 * ey1 - ex1 * ey0;
+    normalize3(ez0, ez1, ez2);
+    ey0 = ez1 * ex2 - ez2 * ex1;
+    ey1 = ez2 * ex0 - ez0 * ex2;
+    ey2 = ez0 * ex1 - ez1 * ex0;
+    const R r02[3][3] = {{ex0, ex1, ex2}, {ey0, ey1, ey2}, {ez0, ez1, ez2}};
+    // rotations[e] = transpose(R_0_2)
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) rot[3 * i + j] = r02[j][i];
+    // deforme[k] = R_0_2 * p_k, then relative to deforme[0] on the used components
+    R df[4][3];
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+#pragma unroll
+        for (int i = 0; i < 3; i++) df[k][i] = r02[i][0] * p[k][0] + r02[i][1] * p[k][1] + r02[i][2] * p[k][2];
+    df[1][0] -= df[0][0];
+    df[2][0] -= df[0][0];
+    df[2][1] -= df[0][1];
+    df[3][0] -= df[0][0];
+    df[3][1] -= df[0][1];
+    df[3][2] -= df[0][2];
+    R rc[9];
+    jgroups<R, NR>(ev, e, g, rc);
+    R D[12];
+    D[0] = 0; D[1] = 0; D[2] = 0;
+    D[3] = rc[0] - df[1][0];
+    D[4] = 0; D[5] = 0;
+    D[6] = rc[3] - df[2][0];
+    D[7] = rc[4] - df[2][1];
+    D[8] = 0;
+    D[9] = rc[6] - df[3][0];
+    D[10] = rc[7] - df[3][1];
+    D[11] = rc[8] - df[3][2];
+    R JtD[6], KJ[6], F[12];
+    jt_mul<R, NR>(g, D, JtD);
+    const R k3[3] = {ev.kmat[e], ev.kmat[ev.E + e], ev.kmat[2 * ev.E + e]};
+    k_mul(k3, JtD, KJ);
+    j_mul(g, KJ, F);
+    rot_apply(rot, F, c);
+}
+
+// ---------------------------------------------------------------------------------------
+// element dforce: applyStiffnessCorotational; returns the 4 nodal contributions (to be
+// SUBTRACTED from df).  dx for node k is supplied by the caller (already gathered).
+// ---------------------------------------------------------------------------------------
+template <typename R, int NR>
+__device__ __forceinline__ void elem_dforce(const ElemView<R, NR>& ev, int64_t e, const R rot[9],
+                                            const R u[12], R fact, R c[12]) {
+    R X[12];
+#pragma unroll
+    for (int n = 0; n < 4; n++) {
+#pragma unroll
+        for (int j = 0; j < 3; j++)
+            X[3 * n + j] = rot[0 * 3 + j] * u[3 * n] + rot[1 * 3 + j] * u[3 * n + 1] + rot[2 * 3 + j] * u[3 * n + 2];
+    }
+    R g[12], rc[9];
+    jgroups<R, NR>(ev, e, g, rc);
+    R JtD[6], KJ[6], F[12];
+    jt_mul<R, NR>(g, X, JtD);
+    const R k3[3] = {ev.kmat[e], ev.kmat[ev.E + e], ev.kmat[2 * ev.E + e]};
+    k_mul(k3, JtD, KJ);
+#pragma unroll
+    for (int k = 0; k < 6; k++) KJ[k] *= fact;
+    j_mul(g, KJ, F);
+    rot_apply(rot, F, c);
+}
+
+template <typename R, int NR>
+__device__ __forceinline__ void load_rot(const ElemView<R, NR>& ev, int64_t e, R rot[9]) {
+#pragma unroll
+    for (int k = 0; k < 9; k++) rot[k] = ev.rot[k * ev.E + e];
+}
+
+// ---------------------------------------------------------------------------------------
+// block reduction (fixed tree, replayed by the oracle's dot_blocked) and the last-block
+// pattern: every block writes its partial, the last block to arrive (integer ticket, no
+// float atomics) reduces the scene's partials in a fixed order.
+// ---------------------------------------------------------------------------------------
+__device__ __forceinline__ double block_sum(double v, double* sh) {
+    const int t = threadIdx.x;
+    sh[t] = v;
+    __syncthreads();
+#pragma unroll
+    for (int s = kBlock / 2; s > 0; s >>= 1) {
+        if (t < s) sh[t] = sh[t] + sh[t + s];
+        __syncthreads();
+    }
+    const double r = sh[0];
+    __syncthreads();
+    return r;
+}
+
+// returns true in every thread of the last-arriving block of scene `ins`; `total` then holds
+// the scene's sum over its node blocks
+__device__ __forceinline__ bool last_block_reduce(double blockval, double* partial, InstDev* ins,
+                                                  int blk, double* sh, double& total) {
+    __shared__ int am_last;
+    if (threadIdx.x == 0) {
+        partial[blk] = blockval;
+        __threadfence();
+        const unsigned ticket = atomicAdd(&ins->counter, 1u);
+        am_last = (ticket == (unsigned)(ins->nblk_count - 1)) ? 1 : 0;
+    }
+    __syncthreads();
+    if (!am_last) return false;
+    __threadfence();
+    const int nb = ins->nblk_count;
+    const int b0 = ins->nblk_begin;
+    double acc = 0.0;
+    for (int k = threadIdx.x; k < nb; k += kBlock) acc = acc + partial[b0 + k];
+    total = block_sum(acc, sh);
+    if (threadIdx.x == 0) atomicExch(&ins->counter, 0u);
+    return true;
+}
+
+}  // namespace sofa_amd

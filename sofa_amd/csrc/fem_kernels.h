@@ -1,0 +1,400 @@
+// fem_kernels.h -- the kernels of one implicit Euler step, batched over scenes.
+//
+// One launch per phase covers every scene of the launch set: element kernels walk a list of
+// element blocks, node kernels a list of node blocks; each block belongs to exactly one scene
+// (Blk.inst), so per-scene scalars (CG alpha, beta, stop flags) are block-uniform and a scene
+// that has left its CG loop costs one early return per block.
+//
+// Step schedule (EulerImplicitSolver.cpp:83-300 + CGLinearSolver.inl:71-315):
+//   k_rhs_elem    rotations from x, element force F(x), element K v          (1 element pass)
+//   k_rhs_node    f = m g + sum F; b = h (f - rm M v - (h+rs) K v); project;
+//                 r = b; x_cg = 0; |b| and rho = r.r                          (1 node pass)
+//   per CG iteration:
+//     k_cg_elem   p = r + beta p at the 4 gathered nodes; element K p        (1 element pass)
+//     k_cg_node   p (owned node); q = (1 + h rm) M p - h(h+rs) K p; project; p.q
+//     k_cg_update x_cg += alpha p; r -= alpha q; r.r
+//   k_euler_update  v += x_cg; x += h v
+//   k_finalize      time += dt, timeStepCount++
+// Node sums run over each node's incidence list in ascending element id: the order in which
+// the reference's element loop accumulates into that node.  No float atomics anywhere.
+#pragma once
+
+#include "fem_device.h"
+
+namespace sofa_amd {
+
+struct NodeArrays {
+    const int* __restrict__ inc_ptr;   // per node [inc_ptr[n], inc_ptr[n+1]) in inc
+    const int* __restrict__ inc;       // element*4 + local vertex, ascending element id
+    const double* __restrict__ mass;   // per-node mass (DiagonalMass) or unused
+    const int8_t* __restrict__ fixed;  // 1 where FixedProjectiveConstraint projects to 0
+};
+
+// mass term of addMDx: UniformMass.inl:403-419 and DiagonalMass.inl:536-559
+template <typename VT>
+__device__ __forceinline__ void mass_mdx(const InstDev& in, const NodeArrays& na, int n, double factor,
+                                         const VT d[3], VT out[3]) {
+    if (in.mass_kind == 0) {
+        VT m = (VT)in.vertex_mass;
+        if (factor != 1.0) m *= (VT)factor;
+        out[0] += d[0] * m; out[1] += d[1] * m; out[2] += d[2] * m;
+    } else {
+        const VT m = (VT)na.mass[n];
+        if (factor == 1.0) {
+            out[0] += d[0] * m; out[1] += d[1] * m; out[2] += d[2] * m;
+        } else {
+            const VT fct = (VT)factor;
+            out[0] += (d[0] * m) * fct; out[1] += (d[1] * m) * fct; out[2] += (d[2] * m) * fct;
+        }
+    }
+}
+
+template <typename VT>
+__device__ __forceinline__ double dot3d(const VT a[3], const VT b[3]) {
+    double r = (double)a[0] * (double)b[0];
+    r += (double)a[1] * (double)b[1];
+    r += (double)a[2] * (double)b[2];
+    return r;
+}
+
+// ---------------------------------------------------------------------------------------
+// RHS
+// ---------------------------------------------------------------------------------------
+template <typename R, int NR, typename VT>
+__global__ __launch_bounds__(kBlock) void k_rhs_elem(ElemView<R, NR> ev, const Blk* __restrict__ eblk,
+                                                      const InstDev* __restrict__ inst, const VT* __restrict__ x,
+                                                      const VT* __restrict__ v, VT* __restrict__ ef_f,
+                                                      VT* __restrict__ ef_v) {
+    const Blk bk = eblk[blockIdx.x];
+    const int64_t e = (int64_t)bk.begin + threadIdx.x;
+    if (e >= bk.end) return;
+    const InstDev& in = inst[bk.inst];
+    const int4 t = ev.tet[e];
+    R rot[9], c[12], g[12];
+    elem_force<R, NR, VT>(ev, e, t, x, rot, c, g);
+#pragma unroll
+    for (int k = 0; k < 9; k++) ev.rot[k * ev.E + e] = rot[k];
+#pragma unroll
+    for (int k = 0; k < 12; k++) ef_f[12 * e + k] = (VT)c[k];
+    // FEM part of addMBKv(b, v): kFactor = h + rs, skipped when 0 (BaseForceField.cpp:38-47)
+    if (in.k_rhs != 0.0) {
+        R u[12];
+        const int nid[4] = {t.x, t.y, t.z, t.w};
+#pragma unroll
+        for (int n = 0; n < 4; n++) {
+            VT a, b, cc;
+            load3(v, nid[n], a, b, cc);
+            u[3 * n] = (R)a; u[3 * n + 1] = (R)b; u[3 * n + 2] = (R)cc;
+        }
+        elem_dforce<R, NR>(ev, e, rot, u, (R)in.k_rhs, c);
+#pragma unroll
+        for (int k = 0; k < 12; k++) ef_v[12 * e + k] = (VT)c[k];
+    } else {
+#pragma unroll
+        for (int k = 0; k < 12; k++) ef_v[12 * e + k] = (VT)0;
+    }
+}
+
+template <typename VT>
+__global__ __launch_bounds__(kBlock) void k_rhs_node(const Blk* __restrict__ nblk, InstDev* __restrict__ inst,
+                                                      NodeArrays na, const VT* __restrict__ ef_f,
+                                                      const VT* __restrict__ ef_v, const VT* __restrict__ v,
+                                                      VT* __restrict__ f, VT* __restrict__ b, VT* __restrict__ r,
+                                                      VT* __restrict__ xcg, double* __restrict__ partial) {
+    __shared__ double sh[kBlock];
+    const Blk bk = nblk[blockIdx.x];
+    InstDev* ins = &inst[bk.inst];
+    const int n = bk.begin + threadIdx.x;
+    double bb = 0.0;
+    if (n < bk.end) {
+        const InstDev& in = *ins;
+        // computeForce: f = 0; mass addForce (f += m g); FEM addForce (f += R F, element order)
+        VT fn[3];
+        if (in.mass_kind == 0) {
+            fn[0] = (VT)0 + (VT)in.mg[0]; fn[1] = (VT)0 + (VT)in.mg[1]; fn[2] = (VT)0 + (VT)in.mg[2];
+        } else {
+            const VT m = (VT)na.mass[n];
+            fn[0] = (VT)0 + (VT)in.gravity[0] * m; fn[1] = (VT)0 + (VT)in.gravity[1] * m; fn[2] = (VT)0 + (VT)in.gravity[2] * m;
+        }
+        const int s0 = na.inc_ptr[n], s1 = na.inc_ptr[n + 1];
+        for (int s = s0; s < s1; s++) {
+            const int64_t slot = na.inc[s];
+            fn[0] += ef_f[3 * slot]; fn[1] += ef_f[3 * slot + 1]; fn[2] += ef_f[3 * slot + 2];
+        }
+        f[3 * (int64_t)n] = fn[0]; f[3 * (int64_t)n + 1] = fn[1]; f[3 * (int64_t)n + 2] = fn[2];
+        // b = f; b += (-rm) M v (mass first); b -= (h+rs) K v (FEM, element order); b *= h
+        VT bn[3] = {fn[0], fn[1], fn[2]};
+        if (in.m_rhs != 0.0) {
+            VT vn[3];
+            load3(v, n, vn[0], vn[1], vn[2]);
+            mass_mdx(in, na, n, in.m_rhs, vn, bn);
+        }
+        for (int s = s0; s < s1; s++) {
+            const int64_t slot = na.inc[s];
+            bn[0] -= ef_v[3 * slot]; bn[1] -= ef_v[3 * slot + 1]; bn[2] -= ef_v[3 * slot + 2];
+        }
+        const VT h = (VT)in.dt;
+        bn[0] *= h; bn[1] *= h; bn[2] *= h;
+        if (na.fixed[n]) { bn[0] = 0; bn[1] = 0; bn[2] = 0; }
+        const int64_t o = 3 * (int64_t)n;
+        b[o] = bn[0]; b[o + 1] = bn[1]; b[o + 2] = bn[2];
+        r[o] = bn[0]; r[o + 1] = bn[1]; r[o + 2] = bn[2];
+        xcg[o] = 0; xcg[o + 1] = 0; xcg[o + 2] = 0;
+        bb = dot3d(bn, bn);
+    }
+    const double bsum = block_sum(bb, sh);
+    double total;
+    if (last_block_reduce(bsum, partial, ins, bk.pad, sh, total) && threadIdx.x == 0) {
+        // CGLinearSolver.inl:86-160 for nb_iter = 1 (rho = r.r = b.b)
+        ins->normb = sqrt(total);
+        ins->rho = total;
+        ins->rho_1 = 0.0;
+        ins->iter = 1;
+        ins->stopped = 0;
+        ins->end_reason = 0;
+        ins->err = 1.0;
+        ins->den = 0.0;
+        if (ins->normb == 0.0) {
+            ins->stopped = 1; ins->end_reason = 4; ins->last_iters = 0;
+        } else {
+            const double err = sqrt(total) / ins->normb;
+            ins->err = err;
+            if (err <= ins->tol && !ins->first_solve) { ins->stopped = 1; ins->end_reason = 1; ins->last_iters = 1; }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// CG iteration
+// ---------------------------------------------------------------------------------------
+template <typename R, int NR, typename VT>
+__global__ __launch_bounds__(kBlock) void k_cg_elem(ElemView<R, NR> ev, const Blk* __restrict__ eblk,
+                                                     const InstDev* __restrict__ inst, const VT* __restrict__ r,
+                                                     const VT* __restrict__ p, VT* __restrict__ ef) {
+    const Blk bk = eblk[blockIdx.x];
+    const InstDev& in = inst[bk.inst];
+    if (in.stopped) return;
+    const int64_t e = (int64_t)bk.begin + threadIdx.x;
+    if (e >= bk.end) return;
+    const int4 t = ev.tet[e];
+    const int first = in.iter == 1;
+    const VT beta = (VT)in.beta;
+    R u[12];
+    const int nid[4] = {t.x, t.y, t.z, t.w};
+#pragma unroll
+    for (int n = 0; n < 4; n++) {
+        VT r0, r1, r2;
+        load3(r, nid[n], r0, r1, r2);
+        if (!first) {
+            // cgstep_beta, GraphScattered: p *= beta; p += r (CGLinearSolver.cpp:40-44)
+            VT p0, p1, p2;
+            load3(p, nid[n], p0, p1, p2);
+            p0 *= beta; p0 += r0;
+            p1 *= beta; p1 += r1;
+            p2 *= beta; p2 += r2;
+            r0 = p0; r1 = p1; r2 = p2;
+        }
+        u[3 * n] = (R)r0; u[3 * n + 1] = (R)r1; u[3 * n + 2] = (R)r2;
+    }
+    R rot[9], c[12];
+    load_rot(ev, e, rot);
+    elem_dforce<R, NR>(ev, e, rot, u, (R)in.k_cg, c);
+#pragma unroll
+    for (int k = 0; k < 12; k++) ef[12 * e + k] = (VT)c[k];
+}
+
+template <typename VT>
+__global__ __launch_bounds__(kBlock) void k_cg_node(const Blk* __restrict__ nblk, InstDev* __restrict__ inst,
+                                                     NodeArrays na, const VT* __restrict__ ef,
+                                                     const VT* __restrict__ r, VT* __restrict__ p,
+                                                     VT* __restrict__ q, double* __restrict__ partial) {
+    __shared__ double sh[kBlock];
+    const Blk bk = nblk[blockIdx.x];
+    InstDev* ins = &inst[bk.inst];
+    if (ins->stopped) return;
+    const int n = bk.begin + threadIdx.x;
+    double pq = 0.0;
+    if (n < bk.end) {
+        const InstDev& in = *ins;
+        const int64_t o = 3 * (int64_t)n;
+        VT pn[3] = {r[o], r[o + 1], r[o + 2]};
+        if (in.iter != 1) {
+            const VT beta = (VT)in.beta;
+            VT po[3] = {p[o], p[o + 1], p[o + 2]};
+#pragma unroll
+            for (int k = 0; k < 3; k++) { po[k] *= beta; po[k] += pn[k]; pn[k] = po[k]; }
+        }
+        p[o] = pn[0]; p[o + 1] = pn[1]; p[o + 2] = pn[2];
+        // q = 0; mass addMDx(q, p, 1 + h rm); FEM addDForce(q, p, -h(h+rs)); projectResponse(q)
+        VT qn[3] = {0, 0, 0};
+        if (in.m_cg != 0.0) mass_mdx(in, na, n, in.m_cg, pn, qn);
+        const int s0 = na.inc_ptr[n], s1 = na.inc_ptr[n + 1];
+        for (int s = s0; s < s1; s++) {
+            const int64_t slot = na.inc[s];
+            qn[0] -= ef[3 * slot]; qn[1] -= ef[3 * slot + 1]; qn[2] -= ef[3 * slot + 2];
+        }
+        if (na.fixed[n]) { qn[0] = 0; qn[1] = 0; qn[2] = 0; }
+        q[o] = qn[0]; q[o + 1] = qn[1]; q[o + 2] = qn[2];
+        pq = dot3d(pn, qn);
+    }
+    const double bsum = block_sum(pq, sh);
+    double den;
+    if (last_block_reduce(bsum, partial, ins, bk.pad, sh, den) && threadIdx.x == 0) {
+        // CGLinearSolver.inl:207-265
+        ins->den = den;
+        const bool exempt = (ins->iter == 1 && ins->first_solve);
+        if (den != 0.0) {
+            if (fabs(den) <= ins->thr && !exempt) {
+                ins->stopped = 1; ins->end_reason = 2; ins->last_iters = ins->iter;
+            } else {
+                ins->alpha = ins->rho / den;
+            }
+        } else {
+            ins->stopped = 1; ins->end_reason = 3; ins->last_iters = ins->iter;
+        }
+    }
+}
+
+template <typename VT>
+__global__ __launch_bounds__(kBlock) void k_cg_update(const Blk* __restrict__ nblk, InstDev* __restrict__ inst,
+                                                       const VT* __restrict__ p, const VT* __restrict__ q,
+                                                       VT* __restrict__ xcg, VT* __restrict__ r,
+                                                       double* __restrict__ partial) {
+    __shared__ double sh[kBlock];
+    const Blk bk = nblk[blockIdx.x];
+    InstDev* ins = &inst[bk.inst];
+    if (ins->stopped) return;
+    const int n = bk.begin + threadIdx.x;
+    double rr = 0.0;
+    if (n < bk.end) {
+        // cgstep_alpha: x += p alpha; r += q (-alpha) (CGLinearSolver.cpp:46-64, vOp_v_inc_bf)
+        const VT a = (VT)ins->alpha, ma = (VT)(-ins->alpha);
+        const int64_t o = 3 * (int64_t)n;
+        VT rn[3];
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            xcg[o + k] += p[o + k] * a;
+            rn[k] = r[o + k] + q[o + k] * ma;
+            r[o + k] = rn[k];
+        }
+        rr = dot3d(rn, rn);
+    }
+    const double bsum = block_sum(rr, sh);
+    double rho;
+    if (last_block_reduce(bsum, partial, ins, bk.pad, sh, rho) && threadIdx.x == 0) {
+        // rho_1 = rho; next iteration: rho = r.r, err test, beta = rho / rho_1
+        ins->rho_1 = ins->rho;
+        ins->rho = rho;
+        const int it = ins->iter + 1;
+        ins->iter = it;
+        if (it > ins->max_iter) {
+            ins->stopped = 1; ins->end_reason = 0; ins->last_iters = ins->max_iter;
+        } else {
+            const double err = sqrt(rho) / ins->normb;
+            ins->err = err;
+            if (err <= ins->tol) {
+                ins->stopped = 1; ins->end_reason = 1; ins->last_iters = it;
+            } else {
+                ins->beta = rho / ins->rho_1;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// integration and bookkeeping
+// ---------------------------------------------------------------------------------------
+template <typename VT>
+__global__ __launch_bounds__(kBlock) void k_euler_update(const Blk* __restrict__ nblk, const InstDev* __restrict__ inst,
+                                                          const VT* __restrict__ xcg, VT* __restrict__ v,
+                                                          VT* __restrict__ x) {
+    const Blk bk = nblk[blockIdx.x];
+    const int n = bk.begin + threadIdx.x;
+    if (n >= bk.end) return;
+    // MechanicalObject::vMultiOp fast path (MechanicalObject.inl:2209-2230)
+    const VT h = (VT)inst[bk.inst].dt;
+    const int64_t o = 3 * (int64_t)n;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        const VT vv = v[o + k] + xcg[o + k];
+        v[o + k] = vv;
+        x[o + k] = x[o + k] + vv * h;
+    }
+}
+
+__global__ void k_finalize(const int* __restrict__ scenes, int n, InstDev* __restrict__ inst) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    InstDev* in = &inst[scenes[i]];
+    if (!in->stopped) { in->stopped = 1; in->last_iters = in->max_iter; }
+    in->first_solve = 0;
+    in->step_count += 1;
+    in->time += in->dt;
+}
+
+// ---------------------------------------------------------------------------------------
+// component-level kernels (one TetrahedronFEMForceField call on caller vectors)
+// ---------------------------------------------------------------------------------------
+// tet holds flattened node ids; the caller's vectors start at the scene's first node `nb`
+template <typename R, int NR, typename VT>
+__global__ __launch_bounds__(kBlock) void k_elem_add_force(ElemView<R, NR> ev, int64_t e0, int64_t e1, int nb,
+                                                            const VT* __restrict__ x, VT* __restrict__ ef) {
+    const int64_t e = e0 + (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (e >= e1) return;
+    int4 t = ev.tet[e];
+    t.x -= nb; t.y -= nb; t.z -= nb; t.w -= nb;
+    R rot[9], c[12], g[12];
+    elem_force<R, NR, VT>(ev, e, t, x, rot, c, g);
+#pragma unroll
+    for (int k = 0; k < 9; k++) ev.rot[k * ev.E + e] = rot[k];
+#pragma unroll
+    for (int k = 0; k < 12; k++) ef[12 * e + k] = (VT)c[k];
+}
+
+template <typename R, int NR, typename VT>
+__global__ __launch_bounds__(kBlock) void k_elem_add_dforce(ElemView<R, NR> ev, int64_t e0, int64_t e1, int nb,
+                                                             const VT* __restrict__ dx, R fact, VT* __restrict__ ef) {
+    const int64_t e = e0 + (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (e >= e1) return;
+    const int4 t = ev.tet[e];
+    const int nid[4] = {t.x - nb, t.y - nb, t.z - nb, t.w - nb};
+    R u[12];
+#pragma unroll
+    for (int n = 0; n < 4; n++) {
+        VT a, b, c;
+        load3(dx, nid[n], a, b, c);
+        u[3 * n] = (R)a; u[3 * n + 1] = (R)b; u[3 * n + 2] = (R)c;
+    }
+    R rot[9], c[12];
+    load_rot(ev, e, rot);
+    elem_dforce<R, NR>(ev, e, rot, u, fact, c);
+#pragma unroll
+    for (int k = 0; k < 12; k++) ef[12 * e + k] = (VT)c[k];
+}
+
+// out[n] (+/-)= sum of the node's contributions; node ids are global, `out` is indexed from
+// the scene's first node
+template <typename VT>
+__global__ __launch_bounds__(kBlock) void k_gather(const int* __restrict__ inc_ptr, const int* __restrict__ inc,
+                                                    int n0, int n1, const VT* __restrict__ ef, VT* __restrict__ out,
+                                                    int subtract) {
+    const int n = n0 + blockIdx.x * kBlock + threadIdx.x;
+    if (n >= n1) return;
+    const int64_t o = 3 * (int64_t)(n - n0);
+    VT a[3] = {out[o], out[o + 1], out[o + 2]};
+    const int s0 = inc_ptr[n], s1 = inc_ptr[n + 1];
+    if (subtract) {
+        for (int s = s0; s < s1; s++) {
+            const int64_t slot = inc[s];
+            a[0] -= ef[3 * slot]; a[1] -= ef[3 * slot + 1]; a[2] -= ef[3 * slot + 2];
+        }
+    } else {
+        for (int s = s0; s < s1; s++) {
+            const int64_t slot = inc[s];
+            a[0] += ef[3 * slot]; a[1] += ef[3 * slot + 1]; a[2] += ef[3 * slot + 2];
+        }
+    }
+    out[o] = a[0]; out[o + 1] = a[1]; out[o + 2] = a[2];
+}
+
+}  // namespace sofa_amd

@@ -1,0 +1,256 @@
+"""The FP64 CPU oracle, pinned before it is trusted.
+
+* against every expected value the reference's own tests hold for this path
+  (tests/golden/reference_goldens.json: checkInit, checkFEMValues, the single-tet step test,
+  the implicit-Euler closed-form recurrence);
+* against what the mathematics fixes where those leave gaps: R R^T = I, momentum balance,
+  rigid-motion invariance, the textbook linear-tetrahedron stiffness V B^T D B at rest,
+  symmetry / rank of the element stiffness, finite differences of addForce (small method),
+  and CG against a dense direct solve of the same system.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from sofa_amd import meshes
+from oracle import OracleScene
+
+GOLD = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "reference_goldens.json")))
+
+
+def test_checkInit_single_tet():
+    g = GOLD["checkInit"]
+    s = OracleScene(np.array(g["positions"], float))
+    s.set_fem([g["tetra"]], [g["young"]], [g["poisson"]], "large")
+    s.set_diagonal_mass(0.1)
+    el = s.fem_element(0)
+    tol = g["tolerance"]
+    np.testing.assert_allclose(el["R0"], g["initRot"], atol=tol)
+    np.testing.assert_allclose(el["R"], g["curRot"], atol=tol)
+    np.testing.assert_allclose(el["rest"], g["initPos"], atol=tol)
+    np.testing.assert_allclose(el["K"], g["stiffness"], atol=tol)
+    np.testing.assert_allclose(el["J"], g["strainD"], atol=tol)
+
+
+def golden_grid_scene(dot_mode=0):
+    g = GOLD["checkFEMValues"]
+    pos, hexa = meshes.regular_grid(g["grid_n"], g["grid_min"], g["grid_max"])
+    tets = meshes.split_hexa_6(hexa)
+    s = OracleScene(pos)
+    s.set_gravity(g["gravity"])
+    s.set_fem(tets, [g["young"]], [g["poisson"]], "large")
+    s.set_diagonal_mass(g["mass_density"])
+    s.set_fixed(meshes.box_roi(pos, g["box_roi"]))
+    s.set_solver(0.0, 0.0, g["cg"]["iterations"], g["cg"]["tolerance"], g["cg"]["threshold"])
+    s.set_dot_mode(dot_mode)
+    return s, pos, tets
+
+
+def test_checkFEMValues_grid():
+    g = GOLD["checkFEMValues"]
+    s, pos, tets = golden_grid_scene()
+    assert len(pos) == 160
+    np.testing.assert_allclose(pos[g["node"]], g["position_before"], atol=g["tolerance"])
+    s.step(g["dt"], g["steps"])
+    x = s.get("x")
+    np.testing.assert_allclose(x[g["node"]], g["position_after"], atol=g["tolerance"])
+    el = s.fem_element(g["element"])
+    tol = g["tolerance"]
+    np.testing.assert_allclose(el["R0"], g["initRot"], atol=tol)
+    np.testing.assert_allclose(el["R"], g["curRot"], atol=tol)
+    np.testing.assert_allclose(el["rest"], g["initPos"], atol=tol)
+    np.testing.assert_allclose(el["K"], g["stiffness"], atol=tol)
+    np.testing.assert_allclose(el["J"], g["strainD"], atol=tol)
+    # the reference's non-zero rounding residue in J (1.18424e-14 entries) is reproduced
+    assert el["J"][0, 3] != 0.0 and abs(el["J"][0, 3]) < 1e-12
+
+
+def test_checkFEMValues_gpu_dot_order_close():
+    """The GPU's blocked dot order changes the trajectory only at rounding level."""
+    s0, _, _ = golden_grid_scene(0)
+    s1, _, _ = golden_grid_scene(1)
+    s0.step(0.01, 100)
+    s1.step(0.01, 100)
+    x0, x1 = s0.get("x"), s1.get("x")
+    assert np.max(np.abs(x0 - x1)) / np.max(np.abs(x0)) < 1e-12
+
+
+def test_stepTest_small_analytic_forces():
+    g = GOLD["stepTest_small"]
+    s = OracleScene(np.array(g["rest"], float))
+    s.set_fem([[0, 1, 2, 3]], [g["young"]], [g["poisson"]], "small")
+    f = s.fem_add_force(np.array(g["x"], float), np.zeros((4, 3)))
+    fb, ft = np.sqrt(3.0) * 10.0 / 9.0, -np.sqrt(3.0) * 10.0 / 3.0
+    expected = np.array([[0, 0, fb], [0, 0, fb], [0, 0, fb], [0, 0, ft]])
+    # ForceField_test tolerance: errorMax (100) * 1e6 * epsilon, with x given to ~1e-9
+    np.testing.assert_allclose(f, expected, atol=1e-6)
+
+
+@pytest.mark.parametrize("case", GOLD["eulerDynamic"]["cases"])
+def test_euler_closed_form(case):
+    """EulerImplicitSolver + CG on a 1-DOF spring against the closed-form recurrence."""
+    g = GOLD["eulerDynamic"]
+    h, K, m, rm, rk = case["h"], case["K"], case["m"], case["rm"], case["rk"]
+    s = OracleScene(np.array([[0.0, g["z0"], 0.0]]))
+    s.set_gravity((0.0, -g["g"], 0.0))
+    s.set_uniform_mass(vertex_mass=m)
+    s.add_axis_spring(0, 1, K, g["z0"])
+    s.set_fixed(np.zeros(0, np.int32))
+    s.set_solver(rk, rm, g["cg"]["iterations"], g["cg"]["tolerance"], g["cg"]["threshold"])
+    size = int(g["final_time"] / h)
+    xs, vs = [g["z0"]], [g["v0"]]
+    den = h * (h + rk) * K + (1 + h * rm) * m
+    const = (-(rk + h) * K - rm * m)
+    for i in range(1, size + 1):
+        vs.append(vs[i - 1] + h * (-K * (xs[i - 1] - g["z0"]) - m * g["g"] + vs[i - 1] * const) / den)
+        xs.append(xs[i - 1] + h * vs[i])
+    for i in range(size):
+        y = s.get("x")[0, 1]
+        assert abs(y - xs[i]) <= max(case["tolerance"], 4e-15), (i, y, xs[i])
+        s.step(h, 1)
+
+
+# ------------------------------------------------------------------------------------------
+# mathematical invariants
+# ------------------------------------------------------------------------------------------
+def small_mesh():
+    pos, tets, fixed = meshes.beam((3, 3, 5), (1.0, 1.0, 2.0))
+    return pos, tets, fixed
+
+
+def element_stiffness(s, n, k=1.0):
+    """Assemble the global stiffness of addDForce (df = -k K dx) by probing unit vectors."""
+    N = 3 * n
+    Kg = np.zeros((N, N))
+    for j in range(N):
+        dx = np.zeros((n, 3))
+        dx.reshape(-1)[j] = 1.0
+        df = s.fem_add_dforce(dx, np.zeros((n, 3)), k)
+        Kg[:, j] = -df.reshape(-1)
+    return Kg
+
+
+def test_rotations_orthonormal_and_momentum():
+    pos, tets, _ = small_mesh()
+    rng = np.random.default_rng(7)
+    s = OracleScene(pos)
+    s.set_fem(tets, [1e5], [0.45], "large")
+    x = pos + 0.05 * rng.standard_normal(pos.shape)
+    f = s.fem_add_force(x, np.zeros_like(x))
+    assert np.linalg.norm(f.sum(axis=0)) < 1e-9 * np.abs(f).max()
+    for e in range(len(tets)):
+        R = s.fem_element(e)["R"]
+        np.testing.assert_allclose(R @ R.T, np.eye(3), atol=1e-14)
+        assert abs(np.linalg.det(R) - 1.0) < 1e-14
+
+
+def test_rigid_motion_invariance():
+    """addForce(Q x + t) = Q addForce(x) for a rotation Q: corotational frame-indifference."""
+    pos, tets, _ = small_mesh()
+    rng = np.random.default_rng(11)
+    x = pos + 0.03 * rng.standard_normal(pos.shape)
+    th = 0.7
+    Q = np.array([[np.cos(th), -np.sin(th), 0], [np.sin(th), np.cos(th), 0], [0, 0, 1.0]])
+    ax = np.array([0.3, -0.5, 0.8]) / np.linalg.norm([0.3, -0.5, 0.8])
+    Kx = np.array([[0, -ax[2], ax[1]], [ax[2], 0, -ax[0]], [-ax[1], ax[0], 0]])
+    Q = Q @ (np.eye(3) + np.sin(0.4) * Kx + (1 - np.cos(0.4)) * Kx @ Kx)
+    s = OracleScene(pos)
+    s.set_fem(tets, [1e5], [0.3], "large")
+    f1 = s.fem_add_force(x, np.zeros_like(x))
+    f2 = s.fem_add_force(x @ Q.T + np.array([1.0, -2.0, 0.5]), np.zeros_like(x))
+    np.testing.assert_allclose(f2, f1 @ Q.T, atol=1e-8 * np.abs(f1).max())
+
+
+def test_rest_stiffness_equals_textbook_linear_tet():
+    """At rest (R = R0) the element stiffness is the textbook V B^T D B of the linear tet,
+    rotated into the world frame: independent restatement with shape-function gradients."""
+    rng = np.random.default_rng(5)
+    P = rng.standard_normal((4, 3))
+    if np.linalg.det(np.stack([P[1] - P[0], P[2] - P[0], P[3] - P[0]])) < 0:
+        P[[1, 2]] = P[[2, 1]]
+    E, nu = 1234.0, 0.31
+    s = OracleScene(P)
+    s.set_fem([[0, 1, 2, 3]], [E], [nu], "large")
+    s.fem_add_force(P, np.zeros((4, 3)))      # rotations from the rest shape
+    Kg = element_stiffness(s, 4)
+    # textbook: B from gradients of barycentric shape functions
+    M = np.ones((4, 4)); M[:, 1:] = P
+    grads = np.linalg.inv(M)[1:, :].T           # (4 nodes, 3) dN_i/dx
+    V = abs(np.linalg.det(M)) / 6.0
+    B = np.zeros((6, 12))
+    for i in range(4):
+        gx, gy, gz = grads[i]
+        B[:, 3 * i:3 * i + 3] = [[gx, 0, 0], [0, gy, 0], [0, 0, gz], [gy, gx, 0], [0, gz, gy], [gz, 0, gx]]
+    lam = E * nu / ((1 + nu) * (1 - 2 * nu)); mu = E / (2 * (1 + nu))
+    D = np.zeros((6, 6)); D[:3, :3] = lam; D[np.arange(3), np.arange(3)] += 2 * mu
+    D[3, 3] = D[4, 4] = D[5, 5] = mu
+    Kt = V * B.T @ D @ B
+    np.testing.assert_allclose(Kg, Kt, rtol=1e-10, atol=1e-10 * np.abs(Kt).max())
+
+
+def test_element_stiffness_symmetric_psd_rank6():
+    pos, tets, _ = small_mesh()
+    rng = np.random.default_rng(3)
+    s = OracleScene(pos[tets[4]])
+    s.set_fem([[0, 1, 2, 3]], [1e5], [0.45], "large")
+    s.fem_add_force(pos[tets[4]] + 0.02 * rng.standard_normal((4, 3)), np.zeros((4, 3)))
+    Kg = element_stiffness(s, 4)
+    np.testing.assert_allclose(Kg, Kg.T, atol=1e-9 * np.abs(Kg).max())
+    w = np.linalg.eigvalsh(0.5 * (Kg + Kg.T))
+    assert w.min() > -1e-9 * w.max()
+    assert int(np.sum(w > 1e-8 * w.max())) == 6
+
+
+def test_small_method_finite_difference():
+    """addDForce = -(d addForce / dx) dx exactly for the linear (small) method
+    (the check of ForceFieldTestCreation.h:344-366)."""
+    pos, tets, _ = small_mesh()
+    rng = np.random.default_rng(9)
+    s = OracleScene(pos)
+    s.set_fem(tets, [1e5], [0.45], "small")
+    x = pos + 0.01 * rng.standard_normal(pos.shape)
+    dx = 1e-4 * rng.standard_normal(pos.shape)
+    f0 = s.fem_add_force(x, np.zeros_like(x))
+    f1 = s.fem_add_force(x + dx, np.zeros_like(x))
+    df = s.fem_add_dforce(dx, np.zeros_like(x), 1.0)
+    np.testing.assert_allclose(df, f1 - f0, atol=1e-9 * np.abs(df).max())
+
+
+def test_cg_converges_to_dense_solve():
+    """One implicit step with a converged CG equals the dense solve of
+    ((1 + h rm) M + h (h + rs) K) dv = h (f - rm M v - (h + rs) K v) on the free nodes."""
+    pos, tets, fixed = small_mesh()
+    n = len(pos)
+    h, rs, rm, m = 0.01, 0.1, 0.1, 2.5
+    g = np.array([0.0, -9.81, 0.0])
+
+    def scene(iters):
+        s = OracleScene(pos)
+        s.set_gravity(g)
+        s.set_fem(tets, [1e4], [0.4], "large")
+        s.set_uniform_mass(vertex_mass=m)
+        s.set_fixed(fixed)
+        s.set_solver(rs, rm, iters, 1e-15, 1e-30)
+        return s
+
+    rng = np.random.default_rng(2)
+    x0 = pos + 0.01 * rng.standard_normal(pos.shape)
+    v0 = 0.1 * rng.standard_normal(pos.shape)
+    x0[fixed] = pos[fixed]; v0[fixed] = 0
+    s = scene(2000)
+    s.set("x", x0); s.set("v", v0)
+    s.step(h, 1)
+    v1 = s.get("v")
+    # dense system from the oracle's own operators at x0
+    probe = scene(1)
+    f = probe.fem_add_force(x0, np.tile(m * g, (n, 1)))
+    Kg = element_stiffness(probe, n)
+    free = np.setdiff1d(np.arange(n), fixed)
+    dof = (3 * free[:, None] + np.arange(3)).ravel()
+    A = (1 + h * rm) * m * np.eye(3 * n) + h * (h + rs) * Kg
+    rhs = h * (f.reshape(-1) - rm * m * v0.reshape(-1) - (h + rs) * Kg @ v0.reshape(-1))
+    dv = np.zeros(3 * n)
+    dv[dof] = np.linalg.solve(A[np.ix_(dof, dof)], rhs[dof])
+    np.testing.assert_allclose(v1.reshape(-1), v0.reshape(-1) + dv, atol=1e-10 * np.abs(dv).max())
