@@ -29,10 +29,13 @@ extern "C" {
 #define SOFA_AMD_ENAN         -6   /* non-finite state detected                              */
 
 /* precision modes */
-#define SOFA_AMD_FP64   0   /* everything in double: the parity mode                          */
-#define SOFA_AMD_FP32   1   /* element arithmetic, state and CG vectors in float; FP64 dots  */
-#define SOFA_AMD_MIXED  2   /* FP32 element arithmetic + FP32 inner CG, FP64 state and
-                               iterative refinement (r = b - A x in FP64)                     */
+#define SOFA_AMD_FP64        0   /* double, every sum in the reference's order: the parity mode */
+#define SOFA_AMD_FP32        1   /* float element arithmetic, state and CG vectors; FP64 dots;
+                                    tiled LDS-gather kernels                                     */
+#define SOFA_AMD_MIXED       2   /* FP32 element arithmetic + FP32 inner CG, FP64 state and
+                                    iterative refinement (r = b - A x in FP64)                   */
+#define SOFA_AMD_FP64_TILED  3   /* double on the tiled kernels: nodes shared by several tiles
+                                    are summed tile by tile (deterministic, not reference order) */
 
 /* mass kinds */
 #define SOFA_AMD_MASS_UNIFORM   0   /* UniformMass (UniformMass.inl:403-496)                 */
@@ -122,13 +125,15 @@ int sofa_amd_scene_element(sofa_amd_engine eng, int scene, int elem, double* rot
 
 /* TetrahedronFEMForceField::addForce (TetrahedronFEMForceField.inl:1548-1604):
  * f += F(x), and the element rotations are updated from x.  x_dev, f_dev: device, n_nodes x 3
- * doubles (AoS). */
-int sofa_amd_fem_add_force(sofa_amd_engine eng, int scene, const double* x_dev, double* f_dev,
+ * AoS in the engine's state type: double for SOFA_AMD_FP64 (Vec3d), float for SOFA_AMD_FP32
+ * (Vec3f), as a MechanicalObject<Vec3d> / <Vec3f> would hold them. */
+int sofa_amd_fem_add_force(sofa_amd_engine eng, int scene, const void* x_dev, void* f_dev,
                            void* stream);
 
 /* TetrahedronFEMForceField::addDForce (TetrahedronFEMForceField.inl:1606-1636):
- * df -= kFactor * K(x) * dx with the rotations of the last addForce. */
-int sofa_amd_fem_add_dforce(sofa_amd_engine eng, int scene, const double* dx_dev, double* df_dev,
+ * df -= kFactor * K(x) * dx with the rotations of the last addForce; same vector types as
+ * sofa_amd_fem_add_force. */
+int sofa_amd_fem_add_dforce(sofa_amd_engine eng, int scene, const void* dx_dev, void* df_dev,
                             double k_factor, void* stream);
 
 #ifdef __cplusplus

@@ -14,7 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libsofa_amd.so")
 
 OK, EINVAL, EDEVICE, ENOMEM, EDEGENERATE, ESTATE, ENAN = 0, -1, -2, -3, -4, -5, -6
-FP64, FP32, MIXED = 0, 1, 2
+FP64, FP32, MIXED, FP64_TILED = 0, 1, 2, 3
 MASS_UNIFORM, MASS_DIAGONAL = 0, 1
 END_REASONS = {0: "iterations", 1: "tolerance", 2: "threshold", 3: "den==0", 4: "null norm of vector b"}
 

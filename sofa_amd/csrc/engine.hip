@@ -20,6 +20,8 @@
 
 #include "../../include/sofa_amd.h"
 #include "fem_kernels.h"
+#include "fem_tiled.h"
+#include "tiling.h"
 
 namespace sofa_amd {
 
@@ -150,8 +152,8 @@ struct EngineBase {
     virtual void write(int scene, const double* x, const double* v) = 0;
     virtual void stats(int scene, sofa_amd_cg_stats* out) = 0;
     virtual void element(int scene, int e, double* rot9, double* k3, double* rest12) = 0;
-    virtual void add_force(int scene, const double* x, double* f, hipStream_t s) = 0;
-    virtual void add_dforce(int scene, const double* dx, double* df, double k, hipStream_t s) = 0;
+    virtual void add_force(int scene, const void* x, void* f, hipStream_t s) = 0;
+    virtual void add_dforce(int scene, const void* dx, void* df, double k, hipStream_t s) = 0;
 };
 
 template <typename T>
@@ -164,32 +166,45 @@ static T* dalloc(size_t n) {
 
 template <typename R, int NR, typename VT>
 struct EngineT : EngineBase {
+    bool tiled = false;
     int64_t Ntot = 0, Etot = 0;
     // node vectors (AoS xyz)
     VT *x = nullptr, *v = nullptr, *f = nullptr, *b = nullptr, *r = nullptr, *p = nullptr, *q = nullptr, *xcg = nullptr;
     double* mass = nullptr;
     int8_t* fixed = nullptr;
-    int *inc_ptr = nullptr, *inc = nullptr;
-    // element record
+    int *inc_ptr = nullptr, *inc = nullptr;        // node -> (position*4 + vertex), ascending element id
+    // element record (SoA, in tile order on the tiled path, else in element order)
     int4* tet = nullptr;
     R *rot = nullptr, *rest = nullptr, *kmat = nullptr;
     VT *ef = nullptr, *ef2 = nullptr;
-    double* ef_d = nullptr;   // double contributions for the component-level calls
+    std::vector<std::vector<int>> iperm;           // per scene: original element -> position
     // scene records and block lists
     InstDev* inst = nullptr;
     std::vector<InstDev> inst_h;
     std::vector<Blk> eblk_h, nblk_h;
     double* partial = nullptr;
+    // tiled path
+    std::vector<TileDesc> tiles_h;
+    std::vector<FinBlk> fin_h;
+    std::vector<NodeBlk> nb_h;
+    std::vector<int> tile_begin, tile_count, fin_begin, fin_count, nbk_begin, nbk_count;
+    TileDesc* tiles_d = nullptr;
+    Ranges* ranges_d = nullptr;
+    uint2* ltet = nullptr;
+    int *ln_gid = nullptr, *ln_inc_ptr = nullptr, *ln_pslot = nullptr, *sh_gid = nullptr, *sh_ptr = nullptr;
+    uint16_t* ln_inc = nullptr;
+    VT *sp0 = nullptr, *sp1 = nullptr;
+    double *pbb = nullptr, *pden = nullptr, *prho = nullptr;
     // launch-set cache
     std::vector<int32_t> set_ids;
-    Blk *set_eblk = nullptr, *set_nblk = nullptr;
+    void *set_a = nullptr, *set_b = nullptr, *set_c = nullptr;
     int* set_scenes = nullptr;
-    int set_ne = 0, set_nn = 0, set_cap_e = 0, set_cap_n = 0, set_cap_s = 0, set_maxit = 0;
+    int set_na = 0, set_nb = 0, set_nc = 0, set_maxit = 0;
 
     ~EngineT() override {
         void* ptrs[] = {x, v, f, b, r, p, q, xcg, mass, fixed, inc_ptr, inc, tet, rot, rest, kmat, ef, ef2,
-                        inst, partial, set_eblk, set_nblk, set_scenes};
-        if (ef_d && (void*)ef_d != (void*)ef) (void)hipFree(ef_d);
+                        inst, partial, tiles_d, ranges_d, ltet, ln_gid, ln_inc_ptr, ln_pslot, sh_gid, sh_ptr,
+                        ln_inc, sp0, sp1, pbb, pden, prho, set_a, set_b, set_c, set_scenes};
         for (void* ptr : ptrs)
             if (ptr) (void)hipFree(ptr);
     }
@@ -197,11 +212,15 @@ struct EngineT : EngineBase {
     ElemView<R, NR> view() const { return ElemView<R, NR>{tet, rot, rest, kmat, Etot}; }
     NodeArrays nodes() const { return NodeArrays{inc_ptr, inc, mass, fixed}; }
 
+    template <typename T>
+    void upload(T*& dst, const std::vector<T>& src) {
+        dst = dalloc<T>(src.size());
+        if (!src.empty()) HIPCHK(hipMemcpy(dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice));
+    }
+
     void commit() override {
         HIPCHK(hipSetDevice(device));
-        // placement
         Ntot = 0; Etot = 0;
-        eblk_h.clear(); nblk_h.clear();
         for (auto& s : scenes) {
             s.node_begin = (int)Ntot;
             s.elem_begin = (int)Etot;
@@ -212,11 +231,21 @@ struct EngineT : EngineBase {
             throw std::invalid_argument("mesh too large for 32-bit indices");
         const int S = (int)scenes.size();
         inst_h.assign(S, InstDev());
+        eblk_h.clear(); nblk_h.clear();
+        tiles_h.clear(); fin_h.clear(); nb_h.clear();
+        tile_begin.assign(S, 0); tile_count.assign(S, 0); fin_begin.assign(S, 0); fin_count.assign(S, 0);
+        nbk_begin.assign(S, 0); nbk_count.assign(S, 0);
+        iperm.assign(S, {});
         std::vector<double> xh(3 * Ntot), massh(Ntot, 0.0);
         std::vector<int8_t> fixh(Ntot, 0);
         std::vector<int4> teth(Etot);
         std::vector<R> roth(9 * Etot), resth(NR * Etot), kh(3 * Etot);
         std::vector<int> cnt(Ntot + 1, 0);
+        // tiled-path host tables (global ids)
+        std::vector<uint2> ltet_h(tiled ? Etot : 0);
+        std::vector<int> lgid, lptr{0}, lps, shg, shp{0};
+        std::vector<uint16_t> linc;
+        int slot_base = 0;
         for (int si = 0; si < S; si++) {
             SceneHost& s = scenes[si];
             const int nE = (int)(s.tets.size() / 4);
@@ -224,6 +253,15 @@ struct EngineT : EngineBase {
             for (int i = 0; i < s.n_nodes; i++) {
                 for (int k = 0; k < 3; k++) P[i].v[k] = s.rest[3 * i + k];
                 for (int k = 0; k < 3; k++) xh[3 * (s.node_begin + (int64_t)i) + k] = s.rest[3 * i + k];
+            }
+            TilePlan plan;
+            std::vector<int>& ip = iperm[si];
+            if (tiled && nE > 0) {
+                plan = plan_tiles(s.n_nodes, s.rest.data(), s.tets.data(), nE, kTileElems, kTileNodes);
+                ip = plan.iperm;
+            } else {
+                ip.resize(nE);
+                for (int e = 0; e < nE; e++) ip[e] = e;
             }
             for (int e = 0; e < nE; e++) {
                 const int32_t* t = &s.tets[4 * (size_t)e];
@@ -234,7 +272,7 @@ struct EngineT : EngineBase {
                 host::ElemInit ei;
                 if (!host::init_element(P, t, E, nu, ei))
                     throw std::domain_error("degenerate tetrahedron " + std::to_string(e) + " in scene " + std::to_string(si));
-                const int64_t ge = s.elem_begin + (int64_t)e;
+                const int64_t ge = s.elem_begin + (int64_t)ip[e];
                 teth[ge] = make_int4(t[0] + s.node_begin, t[1] + s.node_begin, t[2] + s.node_begin, t[3] + s.node_begin);
                 for (int k = 0; k < 9; k++) roth[k * Etot + ge] = (R)ei.rot[k];
                 if (NR == 9) {
@@ -254,7 +292,7 @@ struct EngineT : EngineBase {
                 if (i < 0 || i >= s.n_nodes) throw std::invalid_argument("fixed index out of range");
                 fixh[s.node_begin + i] = 1;
             }
-            // blocks
+            // exact-path blocks: 256 elements / 256 nodes
             s.eblk_begin = (int)eblk_h.size();
             for (int e0 = 0; e0 < nE; e0 += kBlock)
                 eblk_h.push_back(Blk{si, s.elem_begin + e0, s.elem_begin + std::min(nE, e0 + kBlock), 0});
@@ -264,6 +302,56 @@ struct EngineT : EngineBase {
                 nblk_h.push_back(Blk{si, s.node_begin + n0, s.node_begin + std::min(s.n_nodes, n0 + kBlock),
                                      (int)nblk_h.size()});
             s.nblk_count = (int)nblk_h.size() - s.nblk_begin;
+            if (tiled) {
+                // tiles
+                tile_begin[si] = (int)tiles_h.size();
+                const int T = (int)plan.t_ebegin.size();
+                const int lbase = (int)lgid.size();
+                for (int ti = 0; ti < T; ti++) {
+                    TileDesc td;
+                    td.scene = si;
+                    td.e_begin = s.elem_begin + plan.t_ebegin[ti];
+                    td.e_count = plan.t_ecount[ti];
+                    td.ln_begin = lbase + plan.t_lnbegin[ti];
+                    td.ln_count = plan.t_lncount[ti];
+                    td.first = (ti == 0);
+                    td.pslot = (int)tiles_h.size();
+                    td.pad = 0;
+                    tiles_h.push_back(td);
+                }
+                tile_count[si] = T;
+                for (int i = 0; i < nE; i++) ltet_h[s.elem_begin + i] = plan.ltet[i];
+                const int ibase = (int)linc.size();
+                for (size_t j = 0; j < plan.ln_gid.size(); j++) {
+                    lgid.push_back(plan.ln_gid[j] + s.node_begin);
+                    lptr.push_back(ibase + plan.ln_inc_ptr[j + 1]);
+                    lps.push_back(plan.ln_pslot[j] < 0 ? -1 : plan.ln_pslot[j] + slot_base);
+                }
+                linc.insert(linc.end(), plan.ln_inc.begin(), plan.ln_inc.end());
+                // shared / element-free nodes
+                std::vector<int> shared = plan.sh_gid, sptr = plan.sh_ptr;
+                if (nE == 0) {   // no elements: every node is finished by the finalize kernel
+                    shared.resize(s.n_nodes);
+                    for (int i = 0; i < s.n_nodes; i++) shared[i] = i;
+                    sptr.assign(s.n_nodes + 1, 0);
+                }
+                const int shbase = (int)shg.size();
+                for (size_t j = 0; j < shared.size(); j++) {
+                    shg.push_back(shared[j] + s.node_begin);
+                    shp.push_back(slot_base + sptr[j + 1]);
+                }
+                fin_begin[si] = (int)fin_h.size();
+                for (int j0 = 0; j0 < (int)shared.size(); j0 += kBlock)
+                    fin_h.push_back(FinBlk{si, shbase + j0, shbase + std::min((int)shared.size(), j0 + kBlock), 0});
+                fin_count[si] = (int)fin_h.size() - fin_begin[si];
+                slot_base += sptr.back();
+                nbk_begin[si] = (int)nb_h.size();
+                const int per = kBlock * kNodesPerThread;
+                for (int n0 = 0; n0 < s.n_nodes; n0 += per)
+                    nb_h.push_back(NodeBlk{si, s.node_begin + n0, s.node_begin + std::min(s.n_nodes, n0 + per),
+                                           (int)nb_h.size(), n0 == 0, 0, 0, 0});
+                nbk_count[si] = (int)nb_h.size() - nbk_begin[si];
+            }
             // scene record
             InstDev& in = inst_h[si];
             std::memset(&in, 0, sizeof(in));
@@ -287,100 +375,154 @@ struct EngineT : EngineBase {
             in.tol = s.tol < 0 ? 1e-5 : s.tol;
             in.thr = s.thr < 0 ? 1e-5 : s.thr;
             in.first_solve = 1;
-            in.stopped = 1;
+            in.stopped = tiled ? 0 : 1;
         }
-        // incidence CSR: slots e*4+j sorted by node, then by element (counting sort keeps
-        // ascending slot order because elements are visited in order)
+        const int n_tiles = (int)tiles_h.size();
+        for (auto& fb : fin_h) fb.pslot = n_tiles + (int)(&fb - fin_h.data());
+        // incidence CSR for the exact path and the component calls: node -> position*4 + vertex,
+        // visited in ascending original element id
         for (int64_t i = 0; i < Ntot; i++) cnt[i + 1] += cnt[i];
         std::vector<int> inch(4 * Etot), fillp(cnt.begin(), cnt.end() - 1);
-        for (int64_t ge = 0; ge < Etot; ge++) {
-            const int4 t = teth[ge];
-            const int nid[4] = {t.x, t.y, t.z, t.w};
-            for (int k = 0; k < 4; k++) inch[fillp[nid[k]]++] = (int)(4 * ge + k);
+        for (int si = 0; si < S; si++) {
+            const SceneHost& s = scenes[si];
+            const int nE = (int)(s.tets.size() / 4);
+            for (int e = 0; e < nE; e++) {
+                const int64_t ge = s.elem_begin + (int64_t)iperm[si][e];
+                for (int k = 0; k < 4; k++) inch[fillp[s.node_begin + s.tets[4 * (size_t)e + k]]++] = (int)(4 * ge + k);
+            }
         }
         // device buffers
-        auto up = [&](auto* dst, const auto& src) {
-            HIPCHK(hipMemcpy(dst, src.data(), src.size() * sizeof(src[0]), hipMemcpyHostToDevice));
-        };
         x = dalloc<VT>(3 * Ntot); v = dalloc<VT>(3 * Ntot); f = dalloc<VT>(3 * Ntot); b = dalloc<VT>(3 * Ntot);
         r = dalloc<VT>(3 * Ntot); p = dalloc<VT>(3 * Ntot); q = dalloc<VT>(3 * Ntot); xcg = dalloc<VT>(3 * Ntot);
-        mass = dalloc<double>(Ntot); fixed = dalloc<int8_t>(Ntot);
-        inc_ptr = dalloc<int>(Ntot + 1); inc = dalloc<int>(4 * Etot);
-        tet = dalloc<int4>(Etot); rot = dalloc<R>(9 * Etot); rest = dalloc<R>(NR * Etot); kmat = dalloc<R>(3 * Etot);
-        ef = dalloc<VT>(12 * Etot); ef2 = dalloc<VT>(12 * Etot);
-        if constexpr (sizeof(VT) == sizeof(double)) ef_d = (double*)ef;
-        inst = dalloc<InstDev>(S); partial = dalloc<double>(nblk_h.size());
         std::vector<VT> xv(3 * Ntot);
         for (int64_t i = 0; i < 3 * Ntot; i++) xv[i] = (VT)xh[i];
-        up(x, xv);
-        HIPCHK(hipMemset(v, 0, 3 * Ntot * sizeof(VT)));
-        HIPCHK(hipMemset(f, 0, 3 * Ntot * sizeof(VT)));
-        HIPCHK(hipMemset(b, 0, 3 * Ntot * sizeof(VT)));
-        HIPCHK(hipMemset(r, 0, 3 * Ntot * sizeof(VT)));
-        HIPCHK(hipMemset(p, 0, 3 * Ntot * sizeof(VT)));
-        HIPCHK(hipMemset(q, 0, 3 * Ntot * sizeof(VT)));
-        HIPCHK(hipMemset(xcg, 0, 3 * Ntot * sizeof(VT)));
-        up(mass, massh); up(fixed, fixh);
-        up(inc_ptr, cnt); up(inc, inch);
-        up(tet, teth); up(rot, roth); up(rest, resth); up(kmat, kh);
-        up(inst, inst_h);
+        HIPCHK(hipMemcpy(x, xv.data(), xv.size() * sizeof(VT), hipMemcpyHostToDevice));
+        for (VT* vec : {v, f, b, r, p, q, xcg}) HIPCHK(hipMemset(vec, 0, 3 * Ntot * sizeof(VT)));
+        upload(mass, massh); upload(fixed, fixh);
+        upload(inc_ptr, cnt); upload(inc, inch);
+        upload(tet, teth); upload(rot, roth); upload(rest, resth); upload(kmat, kh);
+        ef = dalloc<VT>(12 * Etot);
+        if (!tiled) ef2 = dalloc<VT>(12 * Etot);
+        upload(inst, inst_h);
+        partial = dalloc<double>(nblk_h.size());
         HIPCHK(hipMemset(partial, 0, std::max<size_t>(1, nblk_h.size()) * sizeof(double)));
+        if (tiled) {
+            std::vector<Ranges> rg(S);
+            for (int si = 0; si < S; si++)
+                rg[si] = Ranges{tile_begin[si], tile_count[si], n_tiles + fin_begin[si], fin_count[si], nbk_begin[si],
+                                nbk_count[si]};
+            upload(tiles_d, tiles_h);
+            upload(ranges_d, rg);
+            upload(ltet, ltet_h);
+            upload(ln_gid, lgid); upload(ln_inc_ptr, lptr); upload(ln_pslot, lps); upload(ln_inc, linc);
+            upload(sh_gid, shg); upload(sh_ptr, shp);
+            sp0 = dalloc<VT>(3 * (size_t)std::max(1, slot_base));
+            sp1 = dalloc<VT>(3 * (size_t)std::max(1, slot_base));
+            const size_t np = n_tiles + fin_h.size();
+            pbb = dalloc<double>(np); pden = dalloc<double>(np); prho = dalloc<double>(nb_h.size());
+            HIPCHK(hipMemset(pbb, 0, std::max<size_t>(1, np) * sizeof(double)));
+            HIPCHK(hipMemset(pden, 0, std::max<size_t>(1, np) * sizeof(double)));
+            HIPCHK(hipMemset(prho, 0, std::max<size_t>(1, nb_h.size()) * sizeof(double)));
+        }
         HIPCHK(hipDeviceSynchronize());
         committed = true;
     }
 
+    // block lists of a launch set (a: element blocks / tiles, b: node blocks / finalize blocks,
+    // c: tiled node blocks)
+    template <typename T>
+    static void push_range(std::vector<T>& dst, const std::vector<T>& src, int b0, int n) {
+        dst.insert(dst.end(), src.begin() + b0, src.begin() + b0 + n);
+    }
+    template <typename T>
+    void put(void*& buf, const std::vector<T>& v) {
+        if (buf) HIPCHK(hipFree(buf));
+        buf = dalloc<T>(v.size());
+        if (!v.empty()) HIPCHK(hipMemcpy(buf, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    }
     void prepare_set(const int32_t* ids, int n, hipStream_t s) {
         std::vector<int32_t> want(ids, ids + n);
-        if (want == set_ids && set_eblk) return;
-        std::vector<Blk> eb, nb;
-        int maxit = 0;
-        for (int32_t id : want) {
-            const SceneHost& sc = scenes[id];
-            eb.insert(eb.end(), eblk_h.begin() + sc.eblk_begin, eblk_h.begin() + sc.eblk_begin + sc.eblk_count);
-            nb.insert(nb.end(), nblk_h.begin() + sc.nblk_begin, nblk_h.begin() + sc.nblk_begin + sc.nblk_count);
-            maxit = std::max(maxit, sc.cg_iter);
-        }
+        if (want == set_ids && set_scenes) return;
         HIPCHK(hipStreamSynchronize(s));   // the previous lists may still be in use
-        if ((int)eb.size() > set_cap_e) {
-            if (set_eblk) HIPCHK(hipFree(set_eblk));
-            set_cap_e = (int)eb.size();
-            set_eblk = dalloc<Blk>(set_cap_e);
+        int maxit = 0;
+        if (!tiled) {
+            std::vector<Blk> eb, nb;
+            for (int32_t id : want) {
+                const SceneHost& sc = scenes[id];
+                push_range(eb, eblk_h, sc.eblk_begin, sc.eblk_count);
+                push_range(nb, nblk_h, sc.nblk_begin, sc.nblk_count);
+                maxit = std::max(maxit, sc.cg_iter);
+            }
+            put(set_a, eb); put(set_b, nb);
+            set_na = (int)eb.size(); set_nb = (int)nb.size(); set_nc = 0;
+        } else {
+            std::vector<TileDesc> ta;
+            std::vector<FinBlk> fa;
+            std::vector<NodeBlk> na;
+            for (int32_t id : want) {
+                push_range(ta, tiles_h, tile_begin[id], tile_count[id]);
+                push_range(fa, fin_h, fin_begin[id], fin_count[id]);
+                push_range(na, nb_h, nbk_begin[id], nbk_count[id]);
+                maxit = std::max(maxit, scenes[id].cg_iter);
+            }
+            put(set_a, ta); put(set_b, fa); put(set_c, na);
+            set_na = (int)ta.size(); set_nb = (int)fa.size(); set_nc = (int)na.size();
         }
-        if ((int)nb.size() > set_cap_n) {
-            if (set_nblk) HIPCHK(hipFree(set_nblk));
-            set_cap_n = (int)nb.size();
-            set_nblk = dalloc<Blk>(set_cap_n);
-        }
-        if (n > set_cap_s) {
-            if (set_scenes) HIPCHK(hipFree(set_scenes));
-            set_cap_s = n;
-            set_scenes = dalloc<int>(set_cap_s);
-        }
-        if (!eb.empty()) HIPCHK(hipMemcpy(set_eblk, eb.data(), eb.size() * sizeof(Blk), hipMemcpyHostToDevice));
-        if (!nb.empty()) HIPCHK(hipMemcpy(set_nblk, nb.data(), nb.size() * sizeof(Blk), hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(set_scenes, want.data(), n * sizeof(int), hipMemcpyHostToDevice));
+        std::vector<int> sc(want.begin(), want.end());
+        void* tmp = set_scenes;
+        put(tmp, sc);
+        set_scenes = (int*)tmp;
         set_ids = want;
-        set_ne = (int)eb.size();
-        set_nn = (int)nb.size();
         set_maxit = maxit;
     }
 
     void step(const int32_t* ids, int n, int nsteps, hipStream_t s) override {
         prepare_set(ids, n, s);
+        if (tiled) { step_tiled(n, nsteps, s); return; }
         const ElemView<R, NR> ev = view();
         const NodeArrays na = nodes();
+        const Blk* eb = (const Blk*)set_a;
+        const Blk* nb = (const Blk*)set_b;
         for (int st = 0; st < nsteps; st++) {
-            if (set_ne > 0) k_rhs_elem<R, NR, VT><<<set_ne, kBlock, 0, s>>>(ev, set_eblk, inst, x, v, ef, ef2);
-            if (set_nn > 0) k_rhs_node<VT><<<set_nn, kBlock, 0, s>>>(set_nblk, inst, na, ef, ef2, v, f, b, r, xcg, partial);
+            if (set_na > 0) k_rhs_elem<R, NR, VT><<<set_na, kBlock, 0, s>>>(ev, eb, inst, x, v, ef, ef2);
+            if (set_nb > 0) k_rhs_node<VT><<<set_nb, kBlock, 0, s>>>(nb, inst, na, ef, ef2, v, f, b, r, xcg, partial);
             for (int it = 0; it < set_maxit; it++) {
-                if (set_ne > 0) k_cg_elem<R, NR, VT><<<set_ne, kBlock, 0, s>>>(ev, set_eblk, inst, r, p, ef);
-                if (set_nn > 0) {
-                    k_cg_node<VT><<<set_nn, kBlock, 0, s>>>(set_nblk, inst, na, ef, r, p, q, partial);
-                    k_cg_update<VT><<<set_nn, kBlock, 0, s>>>(set_nblk, inst, p, q, xcg, r, partial);
+                if (set_na > 0) k_cg_elem<R, NR, VT><<<set_na, kBlock, 0, s>>>(ev, eb, inst, r, p, ef);
+                if (set_nb > 0) {
+                    k_cg_node<VT><<<set_nb, kBlock, 0, s>>>(nb, inst, na, ef, r, p, q, partial);
+                    k_cg_update<VT><<<set_nb, kBlock, 0, s>>>(nb, inst, p, q, xcg, r, partial);
                 }
             }
-            if (set_nn > 0) k_euler_update<VT><<<set_nn, kBlock, 0, s>>>(set_nblk, inst, xcg, v, x);
+            if (set_nb > 0) k_euler_update<VT><<<set_nb, kBlock, 0, s>>>(nb, inst, xcg, v, x);
             k_finalize<<<(n + 63) / 64, 64, 0, s>>>(set_scenes, n, inst);
+        }
+        HIPCHK(hipGetLastError());
+    }
+
+    void step_tiled(int n, int nsteps, hipStream_t s) {
+        TileArgs<R, NR> ta;
+        ta.tiles = (const TileDesc*)set_a;
+        ta.inst = inst;
+        ta.ranges = ranges_d;
+        ta.ev = view();
+        ta.ltet = ltet;
+        ta.tn = TileNodes{ln_gid, ln_inc_ptr, ln_inc, ln_pslot};
+        ta.mass = mass;
+        ta.fixed = fixed;
+        const FinBlk* fb = (const FinBlk*)set_b;
+        const NodeBlk* nb = (const NodeBlk*)set_c;
+        const SharedNodes sn{sh_gid, sh_ptr};
+        for (int st = 0; st < nsteps; st++) {
+            if (set_na > 0) k_tile_rhs<R, NR, VT><<<set_na, kBlock, 0, s>>>(ta, x, v, f, b, r, xcg, sp0, sp1, pbb);
+            if (set_nb > 0) k_fin_rhs<VT><<<set_nb, kBlock, 0, s>>>(fb, inst, sn, mass, fixed, v, sp0, sp1, f, b, r, xcg, pbb);
+            for (int it = 1; it <= set_maxit; it++) {
+                if (set_na > 0)
+                    k_tile_apply<R, NR, VT><<<set_na, kBlock, 0, s>>>(ta, it, r, p, q, sp0, it == 1 ? pbb : prho, pden);
+                if (set_nb > 0) k_fin_apply<VT><<<set_nb, kBlock, 0, s>>>(fb, inst, sn, mass, fixed, it, r, p, q, sp0, pden);
+                if (set_nc > 0) k_update<VT><<<set_nc, kBlock, 0, s>>>(nb, inst, ranges_d, it, p, q, xcg, r, pden, prho);
+            }
+            if (set_nc > 0) k_integrate<VT><<<set_nc, kBlock, 0, s>>>(nb, inst, xcg, v, x);
+            k_step_end_tiled<<<(n + 63) / 64, 64, 0, s>>>(set_scenes, n, inst);
         }
         HIPCHK(hipGetLastError());
     }
@@ -424,7 +566,7 @@ struct EngineT : EngineBase {
     }
     void element(int scene, int e, double* rot9, double* k3, double* rest12) override {
         HIPCHK(hipDeviceSynchronize());
-        const int64_t ge = scenes[scene].elem_begin + (int64_t)e;
+        const int64_t ge = scenes[scene].elem_begin + (int64_t)iperm[scene][e];
         R tmp;
         for (int k = 0; k < 9; k++) {
             HIPCHK(hipMemcpy(&tmp, rot + k * Etot + ge, sizeof(R), hipMemcpyDeviceToHost));
@@ -451,39 +593,35 @@ struct EngineT : EngineBase {
         for (int k = 0; k < 9; k++) rest12[3 + k] = rc[k];
     }
 
-    double* component_buffer() {
-        if (!ef_d) ef_d = dalloc<double>(12 * Etot);
-        return ef_d;
-    }
-    void add_force(int scene, const double* xd, double* fd, hipStream_t s) override {
+    // component-level calls: vectors in the engine's state precision (double for FP64,
+    // float for FP32), scene-local AoS xyz
+    void add_force(int scene, const void* xd, void* fd, hipStream_t s) override {
         const SceneHost& sc = scenes[scene];
-        double* efd = component_buffer();
         const int64_t nE = (int64_t)sc.tets.size() / 4;
         if (nE > 0) {
             const int grid = (int)((nE + kBlock - 1) / kBlock);
-            k_elem_add_force<R, NR, double><<<grid, kBlock, 0, s>>>(view(), sc.elem_begin, sc.elem_begin + nE,
-                                                                   sc.node_begin, xd, efd);
+            k_elem_add_force<R, NR, VT><<<grid, kBlock, 0, s>>>(view(), sc.elem_begin, sc.elem_begin + nE,
+                                                               sc.node_begin, (const VT*)xd, ef);
         }
         const int gn = (sc.n_nodes + kBlock - 1) / kBlock;
         if (gn > 0)
-            k_gather<double><<<gn, kBlock, 0, s>>>(inc_ptr, inc, sc.node_begin, sc.node_begin + sc.n_nodes,
-                                                   efd, fd, 0);
+            k_gather<VT><<<gn, kBlock, 0, s>>>(inc_ptr, inc, sc.node_begin, sc.node_begin + sc.n_nodes, ef,
+                                               (VT*)fd, 0);
         HIPCHK(hipGetLastError());
     }
-    void add_dforce(int scene, const double* dxd, double* dfd, double kf, hipStream_t s) override {
+    void add_dforce(int scene, const void* dxd, void* dfd, double kf, hipStream_t s) override {
         const SceneHost& sc = scenes[scene];
-        double* efd = component_buffer();
         const int64_t nE = (int64_t)sc.tets.size() / 4;
         if (nE > 0) {
             const int grid = (int)((nE + kBlock - 1) / kBlock);
             // addDForce: kFactor = (Real) mparams->kFactor(); applied as `KJtD *= fact`
-            k_elem_add_dforce<R, NR, double><<<grid, kBlock, 0, s>>>(view(), sc.elem_begin, sc.elem_begin + nE,
-                                                                    sc.node_begin, dxd, (R)kf, efd);
+            k_elem_add_dforce<R, NR, VT><<<grid, kBlock, 0, s>>>(view(), sc.elem_begin, sc.elem_begin + nE,
+                                                                sc.node_begin, (const VT*)dxd, (R)kf, ef);
         }
         const int gn = (sc.n_nodes + kBlock - 1) / kBlock;
         if (gn > 0)
-            k_gather<double><<<gn, kBlock, 0, s>>>(inc_ptr, inc, sc.node_begin, sc.node_begin + sc.n_nodes,
-                                                   efd, dfd, 1);
+            k_gather<VT><<<gn, kBlock, 0, s>>>(inc_ptr, inc, sc.node_begin, sc.node_begin + sc.n_nodes, ef,
+                                               (VT*)dfd, 1);
         HIPCHK(hipGetLastError());
     }
 };
@@ -537,9 +675,19 @@ int sofa_amd_engine_create(int device, int precision, sofa_amd_engine* out) {
     int rc = sofa_amd_device_check(device);
     if (rc != SOFA_AMD_OK) return rc;
     std::unique_ptr<EngineBase> impl;
-    if (precision == SOFA_AMD_FP64) impl.reset(new EngineT<double, 9, double>());
-    else if (precision == SOFA_AMD_FP32) impl.reset(new EngineT<float, 6, float>());
-    else return fail(SOFA_AMD_EINVAL, "unsupported precision mode");
+    if (precision == SOFA_AMD_FP64) {
+        impl.reset(new EngineT<double, 9, double>());
+    } else if (precision == SOFA_AMD_FP64_TILED) {
+        auto* e = new EngineT<double, 9, double>();
+        e->tiled = true;
+        impl.reset(e);
+    } else if (precision == SOFA_AMD_FP32) {
+        auto* e = new EngineT<float, 6, float>();
+        e->tiled = true;
+        impl.reset(e);
+    } else {
+        return fail(SOFA_AMD_EINVAL, "unsupported precision mode");
+    }
     impl->device = device;
     impl->precision = precision;
     HIPCHK(hipSetDevice(device));
@@ -707,7 +855,7 @@ int sofa_amd_scene_element(sofa_amd_engine eng, int scene, int elem, double* rot
     GUARD_END
 }
 
-int sofa_amd_fem_add_force(sofa_amd_engine eng, int scene, const double* x_dev, double* f_dev, void* stream) {
+int sofa_amd_fem_add_force(sofa_amd_engine eng, int scene, const void* x_dev, void* f_dev, void* stream) {
     GUARD_BEGIN
     if (!scene_of(eng, scene) || !x_dev || !f_dev) return fail(SOFA_AMD_EINVAL, "invalid argument");
     if (!eng->impl->committed) return fail(SOFA_AMD_ESTATE, "engine not committed");
@@ -717,7 +865,7 @@ int sofa_amd_fem_add_force(sofa_amd_engine eng, int scene, const double* x_dev, 
     GUARD_END
 }
 
-int sofa_amd_fem_add_dforce(sofa_amd_engine eng, int scene, const double* dx_dev, double* df_dev, double k_factor,
+int sofa_amd_fem_add_dforce(sofa_amd_engine eng, int scene, const void* dx_dev, void* df_dev, double k_factor,
                             void* stream) {
     GUARD_BEGIN
     if (!scene_of(eng, scene) || !dx_dev || !df_dev) return fail(SOFA_AMD_EINVAL, "invalid argument");

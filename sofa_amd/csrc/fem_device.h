@@ -52,8 +52,9 @@ struct InstDev {
     int step_count;
     int last_iters;
     double time;
-    unsigned int counter;             // arrivals of the current reduction
+    unsigned int counter;             // arrivals of the current reduction (exact path)
     int pad_;
+    double rho_slot[2];               // rho of iterations k-1 and k (tiled path), by k & 1
 };
 
 struct Blk { int inst, begin, end, pad; };   // a block's scene and [begin, end) range
@@ -200,16 +201,9 @@ __device__ __forceinline__ void rot_apply(const R rot[9], const R F[12], R c[12]
 // element force: accumulateForceLarge; returns the rotation and the 4 nodal contributions
 // (to be ADDED to f); positions read as VT and converted to R
 // ---------------------------------------------------------------------------------------
-template <typename R, int NR, typename VT>
-__device__ __forceinline__ void elem_force(const ElemView<R, NR>& ev, int64_t e, const int4 t,
-                                           const VT* __restrict__ x, R rot[9], R c[12], R g[12]) {
-    VT xa0, xa1, xa2, xb0, xb1, xb2, xc0, xc1, xc2, xd0, xd1, xd2;
-    load3(x, t.x, xa0, xa1, xa2);
-    load3(x, t.y, xb0, xb1, xb2);
-    load3(x, t.z, xc0, xc1, xc2);
-    load3(x, t.w, xd0, xd1, xd2);
-    const R p[4][3] = {{(R)xa0, (R)xa1, (R)xa2}, {(R)xb0, (R)xb1, (R)xb2},
-                       {(R)xc0, (R)xc1, (R)xc2}, {(R)xd0, (R)xd1, (R)xd2}};
+template <typename R, int NR>
+__device__ __forceinline__ void elem_force_p(const ElemView<R, NR>& ev, int64_t e, const R p[4][3], R rot[9],
+                                             R c[12], R g[12]) {
     // computeRotationLarge (rows ex, ey, ez of R_0_2)
     R ex0 = p[1][0] - p[0][0], ex1 = p[1][1] - p[0][1], ex2 = p[1][2] - p[0][2];
     normalize3(ex0, ex1, ex2);
@@ -255,6 +249,19 @@ __device__ __forceinline__ void elem_force(const ElemView<R, NR>& ev, int64_t e,
     k_mul(k3, JtD, KJ);
     j_mul(g, KJ, F);
     rot_apply(rot, F, c);
+}
+
+template <typename R, int NR, typename VT>
+__device__ __forceinline__ void elem_force(const ElemView<R, NR>& ev, int64_t e, const int4 t,
+                                           const VT* __restrict__ x, R rot[9], R c[12], R g[12]) {
+    VT xa0, xa1, xa2, xb0, xb1, xb2, xc0, xc1, xc2, xd0, xd1, xd2;
+    load3(x, t.x, xa0, xa1, xa2);
+    load3(x, t.y, xb0, xb1, xb2);
+    load3(x, t.z, xc0, xc1, xc2);
+    load3(x, t.w, xd0, xd1, xd2);
+    const R p[4][3] = {{(R)xa0, (R)xa1, (R)xa2}, {(R)xb0, (R)xb1, (R)xb2},
+                       {(R)xc0, (R)xc1, (R)xc2}, {(R)xd0, (R)xd1, (R)xd2}};
+    elem_force_p<R, NR>(ev, e, p, rot, c, g);
 }
 
 // ---------------------------------------------------------------------------------------

@@ -1,0 +1,483 @@
+// fem_tiled.h -- the tiled fast path: one workgroup per element tile, node gather in LDS.
+//
+// Elements are grouped into spatially compact tiles (Morton order of their centroids, cut so
+// that a tile has at most kTileElems elements and kTileNodes distinct nodes).  One workgroup
+// per tile:
+//   1. stages the tile's nodes (p = r + beta p for the CG apply; x and v for the RHS) in LDS,
+//   2. computes every element's 12 nodal contributions (applyStiffnessCorotational /
+//      accumulateForceLarge arithmetic, shared with the exact path) into LDS,
+//   3. sums them per node over the node's incidence list in ascending element id.
+// A node touched by one tile only ("interior") is finished in the tile: q = (1 + h rm) m p,
+// then the contributions subtracted one by one in the reference's element order, projection,
+// store, and its p.q enters the tile's partial.  A node touched by several tiles ("shared")
+// gets one partial sum per tile, added in ascending tile order by k_fin_apply.  Every step is
+// deterministic; no atomics.
+//
+// Scalar reductions (|b|^2, p.q, r.r) are written as per-block partials and re-reduced, in a
+// fixed order, by every block of the kernel that consumes them (the first block of a scene
+// persists the scalars), so no block ever waits on or fences against another block.
+#pragma once
+
+#include "fem_device.h"
+
+namespace sofa_amd {
+
+constexpr int kTileElems = 512;     // max elements per tile
+constexpr int kTileNodes = 512;     // max distinct nodes per tile
+constexpr int kNodesPerThread = 4;  // node kernels: 256 threads x 4 nodes per block
+
+struct TileDesc {
+    int scene;
+    int e_begin, e_count;     // permuted element range
+    int ln_begin, ln_count;   // tile-local node range (into ln_* arrays)
+    int first;                // first tile of its scene (persists the scene's scalars)
+    int pslot;                // slot of this tile's partial in the tile/finalize partial buffers
+    int pad;
+};
+
+struct FinBlk { int scene, begin, end, pslot; };           // shared-node range
+struct NodeBlk { int scene, begin, end, pslot, first, pad0, pad1, pad2; };  // node range
+
+struct TileNodes {
+    const int* __restrict__ gid;        // global node id of each tile-local node
+    const int* __restrict__ inc_ptr;    // incidence range (absolute, into inc)
+    const uint16_t* __restrict__ inc;   // local element slot * 4 + vertex, ascending element id
+    const int* __restrict__ pslot;      // -1: interior; else shared-partial slot of (node, tile)
+};
+
+struct SharedNodes {
+    const int* __restrict__ gid;        // shared (and element-free) nodes
+    const int* __restrict__ ptr;        // their partial slots [ptr[i], ptr[i+1]) in tile order
+};
+
+// per-scene slot ranges of the two partial buffers
+struct Ranges {
+    int t0, nt;    // tile partials
+    int f0, nf;    // finalize-block partials (offset into the same buffer)
+    int b0, nb;    // node-block partials (rho buffer)
+};
+
+// fixed-order sum of buf[a0 .. a0+na) then buf[c0 .. c0+nc), all threads of the block
+__device__ __forceinline__ double reduce2(const double* __restrict__ buf, int a0, int na, int c0, int nc, double* sh) {
+    double acc = 0.0;
+    for (int k = threadIdx.x; k < na; k += kBlock) acc = acc + buf[a0 + k];
+    for (int k = threadIdx.x; k < nc; k += kBlock) acc = acc + buf[c0 + k];
+    return block_sum(acc, sh);
+}
+
+template <typename VT>
+__device__ __forceinline__ void ld3(const VT* __restrict__ v, int n, VT o[3]) {
+    const VT* q = v + 3 * (int64_t)n;
+    o[0] = q[0]; o[1] = q[1]; o[2] = q[2];
+}
+template <typename VT>
+__device__ __forceinline__ void st3(VT* __restrict__ v, int n, const VT o[3]) {
+    VT* q = v + 3 * (int64_t)n;
+    q[0] = o[0]; q[1] = o[1]; q[2] = o[2];
+}
+
+// stop the scene's CG loop: every block of the kernel reaches the same decision; the first
+// block records it
+__device__ __forceinline__ void record_stop(InstDev* ins, bool first, int reason, int iters, double err) {
+    if (first && threadIdx.x == 0) {
+        ins->stopped = 1;
+        ins->end_reason = reason;
+        ins->last_iters = iters;
+        ins->err = err;
+    }
+}
+
+template <typename R, int NR>
+struct TileArgs {
+    const TileDesc* __restrict__ tiles;
+    InstDev* __restrict__ inst;
+    const Ranges* __restrict__ ranges;
+    ElemView<R, NR> ev;                 // permuted element SoA (ev.tet unused)
+    const uint2* __restrict__ ltet;     // 4 x uint16 tile-local node indices per element
+    TileNodes tn;
+    const double* __restrict__ mass;
+    const int8_t* __restrict__ fixed;
+};
+
+__device__ __forceinline__ void unpack_ltet(uint2 w, int l[4]) {
+    l[0] = (int)(w.x & 0xffffu); l[1] = (int)(w.x >> 16); l[2] = (int)(w.y & 0xffffu); l[3] = (int)(w.y >> 16);
+}
+
+// ---------------------------------------------------------------------------------------
+// RHS: rotations, element force, K v; interior nodes finished, shared nodes -> partials
+// ---------------------------------------------------------------------------------------
+template <typename R, int NR, typename VT>
+__global__ __launch_bounds__(kBlock) void k_tile_rhs(TileArgs<R, NR> a, const VT* __restrict__ x,
+                                                      const VT* __restrict__ v, VT* __restrict__ f,
+                                                      VT* __restrict__ b, VT* __restrict__ r, VT* __restrict__ xcg,
+                                                      VT* __restrict__ spf, VT* __restrict__ spv,
+                                                      double* __restrict__ pbb) {
+    __shared__ R lx[kTileNodes * 3];
+    __shared__ R lv[kTileNodes * 3];
+    __shared__ R cf[kTileElems * 12];
+    __shared__ R cv[kTileElems * 12];
+    __shared__ double sh[kBlock];
+    const TileDesc td = a.tiles[blockIdx.x];
+    const InstDev& in = a.inst[td.scene];
+    for (int j = threadIdx.x; j < td.ln_count; j += kBlock) {
+        const int g = a.tn.gid[td.ln_begin + j];
+        VT t[3];
+        ld3(x, g, t);
+        lx[3 * j] = (R)t[0]; lx[3 * j + 1] = (R)t[1]; lx[3 * j + 2] = (R)t[2];
+        ld3(v, g, t);
+        lv[3 * j] = (R)t[0]; lv[3 * j + 1] = (R)t[1]; lv[3 * j + 2] = (R)t[2];
+    }
+    __syncthreads();
+    const R krhs = (R)in.k_rhs;
+    for (int k = threadIdx.x; k < td.e_count; k += kBlock) {
+        const int64_t e = (int64_t)td.e_begin + k;
+        int l[4];
+        unpack_ltet(a.ltet[e], l);
+        R p[4][3], u[12], rot[9], c[12], g[12];
+#pragma unroll
+        for (int n = 0; n < 4; n++)
+#pragma unroll
+            for (int i = 0; i < 3; i++) { p[n][i] = lx[3 * l[n] + i]; u[3 * n + i] = lv[3 * l[n] + i]; }
+        elem_force_p<R, NR>(a.ev, e, p, rot, c, g);
+#pragma unroll
+        for (int i = 0; i < 9; i++) a.ev.rot[i * a.ev.E + e] = rot[i];
+#pragma unroll
+        for (int i = 0; i < 12; i++) cf[12 * k + i] = c[i];
+        if (in.k_rhs != 0.0) {
+            elem_dforce<R, NR>(a.ev, e, rot, u, krhs, c);
+        } else {
+#pragma unroll
+            for (int i = 0; i < 12; i++) c[i] = 0;
+        }
+#pragma unroll
+        for (int i = 0; i < 12; i++) cv[12 * k + i] = c[i];
+    }
+    __syncthreads();
+    double bb = 0.0;
+    const VT h = (VT)in.dt;
+    for (int j = threadIdx.x; j < td.ln_count; j += kBlock) {
+        const int li = td.ln_begin + j;
+        const int g = a.tn.gid[li];
+        const int s0 = a.tn.inc_ptr[li], s1 = a.tn.inc_ptr[li + 1];
+        const int ps = a.tn.pslot[li];
+        if (ps < 0) {
+            VT fn[3];
+            if (in.mass_kind == 0) {
+                fn[0] = (VT)0 + (VT)in.mg[0]; fn[1] = (VT)0 + (VT)in.mg[1]; fn[2] = (VT)0 + (VT)in.mg[2];
+            } else {
+                const VT m = (VT)a.mass[g];
+#pragma unroll
+                for (int i = 0; i < 3; i++) fn[i] = (VT)0 + (VT)in.gravity[i] * m;
+            }
+            for (int s = s0; s < s1; s++) {
+                const int sl = a.tn.inc[s];
+#pragma unroll
+                for (int i = 0; i < 3; i++) fn[i] += (VT)cf[3 * sl + i];
+            }
+            st3(f, g, fn);
+            VT bn[3] = {fn[0], fn[1], fn[2]};
+            if (in.m_rhs != 0.0) {
+                const VT vn[3] = {(VT)lv[3 * j], (VT)lv[3 * j + 1], (VT)lv[3 * j + 2]};
+                mass_mdx(in, NodeArrays{nullptr, nullptr, a.mass, a.fixed}, g, in.m_rhs, vn, bn);
+            }
+            for (int s = s0; s < s1; s++) {
+                const int sl = a.tn.inc[s];
+#pragma unroll
+                for (int i = 0; i < 3; i++) bn[i] -= (VT)cv[3 * sl + i];
+            }
+            bn[0] *= h; bn[1] *= h; bn[2] *= h;
+            if (a.fixed[g]) { bn[0] = 0; bn[1] = 0; bn[2] = 0; }
+            st3(b, g, bn);
+            st3(r, g, bn);
+            const VT z[3] = {0, 0, 0};
+            st3(xcg, g, z);
+            bb += dot3d(bn, bn);
+        } else {
+            VT pf[3] = {0, 0, 0}, pv[3] = {0, 0, 0};
+            for (int s = s0; s < s1; s++) {
+                const int sl = a.tn.inc[s];
+#pragma unroll
+                for (int i = 0; i < 3; i++) { pf[i] += (VT)cf[3 * sl + i]; pv[i] += (VT)cv[3 * sl + i]; }
+            }
+            st3(spf, ps, pf);
+            st3(spv, ps, pv);
+        }
+    }
+    const double tot = block_sum(bb, sh);
+    if (threadIdx.x == 0) pbb[td.pslot] = tot;
+}
+
+template <typename VT>
+__global__ __launch_bounds__(kBlock) void k_fin_rhs(const FinBlk* __restrict__ fb_, const InstDev* __restrict__ inst,
+                                                     SharedNodes sn, const double* __restrict__ mass,
+                                                     const int8_t* __restrict__ fixed, const VT* __restrict__ v,
+                                                     const VT* __restrict__ spf, const VT* __restrict__ spv,
+                                                     VT* __restrict__ f, VT* __restrict__ b, VT* __restrict__ r,
+                                                     VT* __restrict__ xcg, double* __restrict__ pbb) {
+    __shared__ double sh[kBlock];
+    const FinBlk fb = fb_[blockIdx.x];
+    const InstDev& in = inst[fb.scene];
+    const int j = fb.begin + threadIdx.x;
+    double bb = 0.0;
+    if (j < fb.end) {
+        const int g = sn.gid[j];
+        VT fn[3];
+        if (in.mass_kind == 0) {
+            fn[0] = (VT)0 + (VT)in.mg[0]; fn[1] = (VT)0 + (VT)in.mg[1]; fn[2] = (VT)0 + (VT)in.mg[2];
+        } else {
+            const VT m = (VT)mass[g];
+#pragma unroll
+            for (int i = 0; i < 3; i++) fn[i] = (VT)0 + (VT)in.gravity[i] * m;
+        }
+        const int s0 = sn.ptr[j], s1 = sn.ptr[j + 1];
+        for (int s = s0; s < s1; s++) {
+#pragma unroll
+            for (int i = 0; i < 3; i++) fn[i] += spf[3 * (int64_t)s + i];
+        }
+        st3(f, g, fn);
+        VT bn[3] = {fn[0], fn[1], fn[2]};
+        if (in.m_rhs != 0.0) {
+            VT vn[3];
+            ld3(v, g, vn);
+            mass_mdx(in, NodeArrays{nullptr, nullptr, mass, fixed}, g, in.m_rhs, vn, bn);
+        }
+        for (int s = s0; s < s1; s++) {
+#pragma unroll
+            for (int i = 0; i < 3; i++) bn[i] -= spv[3 * (int64_t)s + i];
+        }
+        const VT h = (VT)in.dt;
+        bn[0] *= h; bn[1] *= h; bn[2] *= h;
+        if (fixed[g]) { bn[0] = 0; bn[1] = 0; bn[2] = 0; }
+        st3(b, g, bn);
+        st3(r, g, bn);
+        const VT z[3] = {0, 0, 0};
+        st3(xcg, g, z);
+        bb = dot3d(bn, bn);
+    }
+    const double tot = block_sum(bb, sh);
+    if (threadIdx.x == 0) pbb[fb.pslot] = tot;
+}
+
+// ---------------------------------------------------------------------------------------
+// CG iteration `iter` (1-based): K1 tile apply, K2 shared finalize, K3 update
+// ---------------------------------------------------------------------------------------
+template <typename R, int NR, typename VT>
+__global__ __launch_bounds__(kBlock) void k_tile_apply(TileArgs<R, NR> a, int iter, const VT* __restrict__ r,
+                                                        VT* __restrict__ p, VT* __restrict__ q, VT* __restrict__ sp,
+                                                        const double* __restrict__ pin,   // |b|^2 (iter 1) or r.r partials
+                                                        double* __restrict__ pden) {
+    __shared__ R lu[kTileNodes * 3];
+    __shared__ R lc[kTileElems * 12];
+    __shared__ double sh[kBlock];
+    const TileDesc td = a.tiles[blockIdx.x];
+    InstDev* ins = &a.inst[td.scene];
+    if (ins->stopped) return;
+    const Ranges rg = a.ranges[td.scene];
+    const bool first = td.first != 0;
+    // ---- scalars: rho, the tolerance test, beta (CGLinearSolver.inl:130-196)
+    double beta = 0.0;
+    if (iter == 1) {
+        const double bb = reduce2(pin, rg.t0, rg.nt, rg.f0, rg.nf, sh);
+        const double normb = sqrt(bb);
+        if (normb == 0.0) { record_stop(ins, first, 4, 0, 1.0); return; }
+        const double err = sqrt(bb) / normb;
+        if (err <= ins->tol && !ins->first_solve) { record_stop(ins, first, 1, 1, err); return; }
+        if (first && threadIdx.x == 0) {
+            ins->normb = normb; ins->rho_slot[1] = bb; ins->iter = 1; ins->err = err;
+        }
+    } else {
+        if (iter > ins->max_iter) { record_stop(ins, first, 0, ins->max_iter, ins->err); return; }
+        const double rho = reduce2(pin, rg.b0, rg.nb, 0, 0, sh);
+        const double err = sqrt(rho) / ins->normb;
+        if (err <= ins->tol) { record_stop(ins, first, 1, iter, err); return; }
+        const double rho_1 = ins->rho_slot[(iter - 1) & 1];
+        beta = rho / rho_1;
+        if (first && threadIdx.x == 0) {
+            ins->rho_slot[iter & 1] = rho; ins->beta = beta; ins->iter = iter; ins->err = err;
+        }
+    }
+    // ---- stage p = r + beta p (interior nodes: this tile owns p)
+    const VT vb = (VT)beta;
+    for (int j = threadIdx.x; j < td.ln_count; j += kBlock) {
+        const int li = td.ln_begin + j;
+        const int g = a.tn.gid[li];
+        VT pp[3];
+        ld3(r, g, pp);
+        if (iter != 1) {
+            VT po[3];
+            ld3(p, g, po);
+#pragma unroll
+            for (int i = 0; i < 3; i++) { po[i] *= vb; po[i] += pp[i]; pp[i] = po[i]; }
+        }
+        if (a.tn.pslot[li] < 0) st3(p, g, pp);
+        lu[3 * j] = (R)pp[0]; lu[3 * j + 1] = (R)pp[1]; lu[3 * j + 2] = (R)pp[2];
+    }
+    __syncthreads();
+    const R kcg = (R)ins->k_cg;
+    for (int k = threadIdx.x; k < td.e_count; k += kBlock) {
+        const int64_t e = (int64_t)td.e_begin + k;
+        int l[4];
+        unpack_ltet(a.ltet[e], l);
+        R u[12], rot[9], c[12];
+#pragma unroll
+        for (int n = 0; n < 4; n++)
+#pragma unroll
+            for (int i = 0; i < 3; i++) u[3 * n + i] = lu[3 * l[n] + i];
+        load_rot(a.ev, e, rot);
+        elem_dforce<R, NR>(a.ev, e, rot, u, kcg, c);
+#pragma unroll
+        for (int i = 0; i < 12; i++) lc[12 * k + i] = c[i];
+    }
+    __syncthreads();
+    const InstDev& in = *ins;
+    double pq = 0.0;
+    for (int j = threadIdx.x; j < td.ln_count; j += kBlock) {
+        const int li = td.ln_begin + j;
+        const int s0 = a.tn.inc_ptr[li], s1 = a.tn.inc_ptr[li + 1];
+        const int ps = a.tn.pslot[li];
+        if (ps < 0) {
+            const int g = a.tn.gid[li];
+            const VT pp[3] = {(VT)lu[3 * j], (VT)lu[3 * j + 1], (VT)lu[3 * j + 2]};
+            VT qn[3] = {0, 0, 0};
+            if (in.m_cg != 0.0) mass_mdx(in, NodeArrays{nullptr, nullptr, a.mass, a.fixed}, g, in.m_cg, pp, qn);
+            for (int s = s0; s < s1; s++) {
+                const int sl = a.tn.inc[s];
+#pragma unroll
+                for (int i = 0; i < 3; i++) qn[i] -= (VT)lc[3 * sl + i];
+            }
+            if (a.fixed[g]) { qn[0] = 0; qn[1] = 0; qn[2] = 0; }
+            st3(q, g, qn);
+            pq += dot3d(pp, qn);
+        } else {
+            VT acc[3] = {0, 0, 0};
+            for (int s = s0; s < s1; s++) {
+                const int sl = a.tn.inc[s];
+#pragma unroll
+                for (int i = 0; i < 3; i++) acc[i] -= (VT)lc[3 * sl + i];
+            }
+            st3(sp, ps, acc);
+        }
+    }
+    const double tot = block_sum(pq, sh);
+    if (threadIdx.x == 0) pden[td.pslot] = tot;
+}
+
+template <typename VT>
+__global__ __launch_bounds__(kBlock) void k_fin_apply(const FinBlk* __restrict__ fb_, InstDev* __restrict__ inst,
+                                                       SharedNodes sn, const double* __restrict__ mass,
+                                                       const int8_t* __restrict__ fixed, int iter,
+                                                       const VT* __restrict__ r, VT* __restrict__ p,
+                                                       VT* __restrict__ q, const VT* __restrict__ sp,
+                                                       double* __restrict__ pden) {
+    __shared__ double sh[kBlock];
+    const FinBlk fb = fb_[blockIdx.x];
+    const InstDev& in = inst[fb.scene];
+    if (in.stopped) return;
+    const int j = fb.begin + threadIdx.x;
+    double pq = 0.0;
+    if (j < fb.end) {
+        const int g = sn.gid[j];
+        VT pp[3];
+        ld3(r, g, pp);
+        if (iter != 1) {
+            const VT vb = (VT)in.beta;
+            VT po[3];
+            ld3(p, g, po);
+#pragma unroll
+            for (int i = 0; i < 3; i++) { po[i] *= vb; po[i] += pp[i]; pp[i] = po[i]; }
+        }
+        st3(p, g, pp);
+        VT qn[3] = {0, 0, 0};
+        if (in.m_cg != 0.0) mass_mdx(in, NodeArrays{nullptr, nullptr, mass, fixed}, g, in.m_cg, pp, qn);
+        const int s0 = sn.ptr[j], s1 = sn.ptr[j + 1];
+        for (int s = s0; s < s1; s++) {
+#pragma unroll
+            for (int i = 0; i < 3; i++) qn[i] += sp[3 * (int64_t)s + i];
+        }
+        if (fixed[g]) { qn[0] = 0; qn[1] = 0; qn[2] = 0; }
+        st3(q, g, qn);
+        pq = dot3d(pp, qn);
+    }
+    const double tot = block_sum(pq, sh);
+    if (threadIdx.x == 0) pden[fb.pslot] = tot;
+}
+
+template <typename VT>
+__global__ __launch_bounds__(kBlock) void k_update(const NodeBlk* __restrict__ nb_, InstDev* __restrict__ inst,
+                                                    const Ranges* __restrict__ ranges, int iter,
+                                                    const VT* __restrict__ p, const VT* __restrict__ q,
+                                                    VT* __restrict__ xcg, VT* __restrict__ r,
+                                                    const double* __restrict__ pden, double* __restrict__ prho) {
+    __shared__ double sh[kBlock];
+    const NodeBlk nb = nb_[blockIdx.x];
+    InstDev* ins = &inst[nb.scene];
+    if (ins->stopped) return;
+    const Ranges rg = ranges[nb.scene];
+    const bool first = nb.first != 0;
+    // den = p.q, threshold test, alpha (CGLinearSolver.inl:207-265)
+    const double den = reduce2(pden, rg.t0, rg.nt, rg.f0, rg.nf, sh);
+    const bool exempt = (iter == 1 && ins->first_solve);
+    if (den == 0.0) { record_stop(ins, first, 3, iter, ins->err); if (first && threadIdx.x == 0) ins->den = den; return; }
+    if (fabs(den) <= ins->thr && !exempt) {
+        record_stop(ins, first, 2, iter, ins->err);
+        if (first && threadIdx.x == 0) ins->den = den;
+        return;
+    }
+    const double alpha = ins->rho_slot[iter & 1] / den;
+    if (first && threadIdx.x == 0) { ins->den = den; ins->alpha = alpha; }
+    const VT va = (VT)alpha, vma = (VT)(-alpha);
+    double rr = 0.0;
+#pragma unroll
+    for (int k = 0; k < kNodesPerThread; k++) {
+        const int n = nb.begin + k * kBlock + threadIdx.x;
+        if (n < nb.end) {
+            const int64_t o = 3 * (int64_t)n;
+            VT rn[3];
+#pragma unroll
+            for (int i = 0; i < 3; i++) {
+                xcg[o + i] += p[o + i] * va;
+                rn[i] = r[o + i] + q[o + i] * vma;
+                r[o + i] = rn[i];
+            }
+            rr += dot3d(rn, rn);
+        }
+    }
+    const double tot = block_sum(rr, sh);
+    if (threadIdx.x == 0) prho[nb.pslot] = tot;
+}
+
+template <typename VT>
+__global__ __launch_bounds__(kBlock) void k_integrate(const NodeBlk* __restrict__ nb_, InstDev* __restrict__ inst,
+                                                       const VT* __restrict__ xcg, VT* __restrict__ v,
+                                                       VT* __restrict__ x) {
+    const NodeBlk nb = nb_[blockIdx.x];
+    InstDev* ins = &inst[nb.scene];
+    const VT h = (VT)ins->dt;
+#pragma unroll
+    for (int k = 0; k < kNodesPerThread; k++) {
+        const int n = nb.begin + k * kBlock + threadIdx.x;
+        if (n < nb.end) {
+            const int64_t o = 3 * (int64_t)n;
+#pragma unroll
+            for (int i = 0; i < 3; i++) {
+                const VT vv = v[o + i] + xcg[o + i];
+                v[o + i] = vv;
+                x[o + i] = x[o + i] + vv * h;
+            }
+        }
+    }
+}
+
+// scene bookkeeping after the update (one thread per scene of the launch set)
+__global__ void k_step_end_tiled(const int* __restrict__ scenes, int n, InstDev* __restrict__ inst) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    InstDev* in = &inst[scenes[i]];
+    if (!in->stopped) { in->stopped = 1; in->end_reason = 0; in->last_iters = in->max_iter; }
+    in->first_solve = 0;
+    in->step_count += 1;
+    in->time += in->dt;
+    in->stopped = 0;   // ready for the next solve
+}
+
+}  // namespace sofa_amd

@@ -1,0 +1,170 @@
+// tiling.h -- host planning of element tiles for the LDS-gather kernels (fem_tiled.h).
+//
+// Elements are ordered by the Morton code of their rest centroid (ties by element id) and cut
+// greedily into tiles of at most `max_elems` elements and `max_nodes` distinct nodes.  For
+// each tile-local node the incidence list (local element slot * 4 + vertex) is sorted by
+// ORIGINAL element id: the order in which the reference's element loop accumulates into that
+// node (TetrahedronFEMForceField.inl:1630-1634).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+#include <numeric>
+#include <stdexcept>
+#include <vector>
+
+namespace sofa_amd {
+
+struct TilePlan {
+    std::vector<int> t_ebegin, t_ecount, t_lnbegin, t_lncount;   // per tile (scene-local offsets)
+    std::vector<int> perm;          // permuted position -> original element id
+    std::vector<int> iperm;         // original element id -> permuted position
+    std::vector<uint2> ltet;        // per permuted element: 4 x uint16 tile-local node index
+    std::vector<int> ln_gid;        // tile-local node -> scene-local node id
+    std::vector<int> ln_inc_ptr;    // size(ln_gid) + 1
+    std::vector<uint16_t> ln_inc;   // local slot * 4 + vertex
+    std::vector<int> ln_pslot;      // -1 interior, else scene-local shared-partial slot
+    std::vector<int> sh_gid;        // nodes finished by the finalize kernel (ascending)
+    std::vector<int> sh_ptr;        // size(sh_gid) + 1
+    int n_slots = 0;
+};
+
+static inline uint64_t spread3(uint64_t v) {   // 21 bits -> every third bit
+    v &= 0x1fffffULL;
+    v = (v | (v << 32)) & 0x1f00000000ffffULL;
+    v = (v | (v << 16)) & 0x1f0000ff0000ffULL;
+    v = (v | (v << 8)) & 0x100f00f00f00f00fULL;
+    v = (v | (v << 4)) & 0x10c30c30c30c30c3ULL;
+    v = (v | (v << 2)) & 0x1249249249249249ULL;
+    return v;
+}
+
+static inline TilePlan plan_tiles(int n_nodes, const double* rest, const int32_t* tets, int E, int max_elems,
+                                  int max_nodes) {
+    TilePlan P;
+    // Morton order of element centroids
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for (int i = 0; i < n_nodes; i++)
+        for (int k = 0; k < 3; k++) {
+            lo[k] = std::min(lo[k], rest[3 * i + k]);
+            hi[k] = std::max(hi[k], rest[3 * i + k]);
+        }
+    double ext = 0;
+    for (int k = 0; k < 3; k++) ext = std::max(ext, hi[k] - lo[k]);
+    if (!(ext > 0)) ext = 1;
+    std::vector<uint64_t> code(E);
+    for (int e = 0; e < E; e++) {
+        uint64_t c = 0;
+        for (int k = 0; k < 3; k++) {
+            double s = 0;
+            for (int v = 0; v < 4; v++) s += rest[3 * tets[4 * e + v] + k];
+            double u = (s * 0.25 - lo[k]) / ext;
+            u = std::min(std::max(u, 0.0), 1.0);
+            c |= spread3((uint64_t)(u * 2097151.0)) << k;
+        }
+        code[e] = c;
+    }
+    P.perm.resize(E);
+    std::iota(P.perm.begin(), P.perm.end(), 0);
+    std::stable_sort(P.perm.begin(), P.perm.end(), [&](int a, int b) { return code[a] < code[b]; });
+    P.iperm.assign(E, 0);
+    for (int i = 0; i < E; i++) P.iperm[P.perm[i]] = i;
+
+    // greedy cut
+    std::vector<int> stamp(n_nodes, -1);
+    int t = -1, ecount = 0, ncount = 0;
+    for (int i = 0; i < E; i++) {
+        const int e = P.perm[i];
+        int fresh = 0;
+        for (int v = 0; v < 4; v++) {
+            const int n = tets[4 * e + v];
+            bool dup = false;
+            for (int w = 0; w < v; w++) dup |= (tets[4 * e + w] == n);
+            if (!dup && stamp[n] != t) fresh++;
+        }
+        if (t < 0 || ecount + 1 > max_elems || ncount + fresh > max_nodes) {
+            t++;
+            P.t_ebegin.push_back(i);
+            P.t_ecount.push_back(0);
+            ecount = 0;
+            ncount = 0;
+            fresh = 0;
+            for (int v = 0; v < 4; v++) {
+                const int n = tets[4 * e + v];
+                bool dup = false;
+                for (int w = 0; w < v; w++) dup |= (tets[4 * e + w] == n);
+                if (!dup) fresh++;
+            }
+        }
+        for (int v = 0; v < 4; v++) stamp[tets[4 * e + v]] = t;
+        ecount++;
+        ncount += fresh;
+        P.t_ecount[t] = ecount;
+    }
+    const int T = (int)P.t_ebegin.size();
+
+    // tile-local nodes, local tets, incidences
+    std::vector<int> loc(n_nodes, -1), tiles_of(n_nodes, 0);
+    P.ltet.resize(E);
+    P.t_lnbegin.resize(T);
+    P.t_lncount.resize(T);
+    P.ln_inc_ptr.push_back(0);
+    std::vector<std::pair<int, int>> incl;   // (original element, slot*4+v)
+    for (int ti = 0; ti < T; ti++) {
+        const int e0 = P.t_ebegin[ti], ec = P.t_ecount[ti];
+        std::vector<int> nodes;
+        nodes.reserve(4 * ec);
+        for (int k = 0; k < ec; k++)
+            for (int v = 0; v < 4; v++) nodes.push_back(tets[4 * P.perm[e0 + k] + v]);
+        std::sort(nodes.begin(), nodes.end());
+        nodes.erase(std::unique(nodes.begin(), nodes.end()), nodes.end());
+        if ((int)nodes.size() > max_nodes) throw std::logic_error("tile node budget exceeded");
+        P.t_lnbegin[ti] = (int)P.ln_gid.size();
+        P.t_lncount[ti] = (int)nodes.size();
+        for (size_t j = 0; j < nodes.size(); j++) {
+            loc[nodes[j]] = (int)j;
+            tiles_of[nodes[j]]++;
+        }
+        std::vector<std::vector<std::pair<int, int>>> per(nodes.size());
+        for (int k = 0; k < ec; k++) {
+            const int e = P.perm[e0 + k];
+            uint32_t l[4];
+            for (int v = 0; v < 4; v++) {
+                l[v] = (uint32_t)loc[tets[4 * e + v]];
+                per[l[v]].push_back({e, k * 4 + v});
+            }
+            P.ltet[e0 + k] = make_uint2(l[0] | (l[1] << 16), l[2] | (l[3] << 16));
+        }
+        for (size_t j = 0; j < nodes.size(); j++) {
+            auto& lst = per[j];
+            std::sort(lst.begin(), lst.end());
+            for (auto& pr : lst) P.ln_inc.push_back((uint16_t)pr.second);
+            P.ln_inc_ptr.push_back((int)P.ln_inc.size());
+            P.ln_gid.push_back(nodes[j]);
+        }
+    }
+    // shared nodes (in >1 tile) and element-free nodes go to the finalize kernel
+    std::vector<int> shidx(n_nodes, -1), fill(n_nodes, 0);
+    P.sh_ptr.push_back(0);
+    for (int n = 0; n < n_nodes; n++) {
+        if (tiles_of[n] != 1) {
+            shidx[n] = (int)P.sh_gid.size();
+            P.sh_gid.push_back(n);
+            P.sh_ptr.push_back(P.sh_ptr.back() + tiles_of[n]);
+        }
+    }
+    P.n_slots = P.sh_ptr.back();
+    P.ln_pslot.assign(P.ln_gid.size(), -1);
+    for (int ti = 0; ti < T; ti++) {
+        for (int j = 0; j < P.t_lncount[ti]; j++) {
+            const int li = P.t_lnbegin[ti] + j;
+            const int n = P.ln_gid[li];
+            if (shidx[n] >= 0) P.ln_pslot[li] = P.sh_ptr[shidx[n]] + fill[n]++;
+        }
+    }
+    return P;
+}
+
+}  // namespace sofa_amd
