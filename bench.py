@@ -80,42 +80,38 @@ def time_steps(eng, sid, steps, warmup):
     return total_ms, wall, lat
 
 
-def dforce_roofline(c, precision, reps):
-    """addDForce (component call = element kernel + node gather) timed with HIP events on the
-    stream it is launched on; algorithmic bytes per call E(16 + n_rec p_e) + N 6 p_v."""
+def apply_roofline(c, precision, reps, warm_steps=3):
+    """The dominant kernel: the tiled CG operator apply (k_tile_apply: p = r + beta p staged in
+    LDS, element K p, per-node sums in LDS, q of tile-owned nodes, partials of shared nodes),
+    launched `reps` times on a state advanced by `warm_steps` steps and timed with HIP events on
+    the stream it is launched on.
+    Algorithmic bytes per launch: E (16 + 18 p_e) + N 12 p_v
+      per element: 16 B of indices (4 x u16 local node ids + 4 x u16 incidence slots),
+                   rotation 9, rest coordinates 6, stiffness 3 reals;
+      per node: r and p read, p and q written (3 reals each)."""
     import torch
     from sofa_amd import capi
     eng, sid = make_engine(c, precision)
     n, E = len(c["pos"]), len(c["tets"])
-    tdt = torch.float64 if precision == capi.FP64 else torch.float32
-    g = torch.Generator(device="cpu").manual_seed(0)
-    x = torch.tensor(c["pos"], dtype=tdt).cuda() + 0.01 * torch.randn((n, 3), generator=g).to(tdt).cuda()
-    dx = torch.randn((n, 3), generator=g).to(tdt).cuda()
-    df = torch.zeros((n, 3), dtype=tdt, device="cuda")
-    f = torch.zeros((n, 3), dtype=tdt, device="cuda")
-    torch.cuda.synchronize()
     stream = bench_stream()
     sp = stream.cuda_stream
-    eng.fem_add_force(sid, x.data_ptr(), f.data_ptr(), stream=sp)
-    k = -c["dt"] * (c["dt"] + c["rs"])
-    for _ in range(3):
-        eng.fem_add_dforce(sid, dx.data_ptr(), df.data_ptr(), k, stream=sp)
+    eng.step([sid], warm_steps, stream=sp)
+    eng.bench_apply(sid, 3, stream=sp)
     torch.cuda.synchronize()
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
     e0.record(stream)
-    for _ in range(reps):
-        eng.fem_add_dforce(sid, dx.data_ptr(), df.data_ptr(), k, stream=sp)
+    eng.bench_apply(sid, reps, stream=sp)
     e1.record(stream)
     torch.cuda.synchronize()
     us = e0.elapsed_time(e1) * 1e3 / reps
-    if precision == capi.FP64:
-        pe, pv, nrec = 8, 8, 21      # R 9 + 9 rest coordinates + K 3
-    else:
-        pe, pv, nrec = 4, 4, 18      # R 9 + 6 rest coordinates + K 3
-    bytes_alg = E * (16 + nrec * pe) + n * 6 * pv
+    tiling = eng.tiling(sid)
     eng.close()
-    return us, bytes_alg
+    pe = pv = 8 if precision == capi.FP64_TILED else 4
+    bytes_alg = E * (16 + 18 * pe) + n * 12 * pv
+    if precision == capi.FP64_TILED:
+        bytes_alg = E * (16 + 21 * pe) + n * 12 * pv     # exact record: 9 rest coordinates
+    return us, bytes_alg, tiling
 
 
 def copy_probe(mb=512, reps=5):
@@ -210,15 +206,19 @@ def main():
     ms_per_step = total_ms / args.steps
     value = E * args.steps / (total_ms / 1e3)
 
-    # FP64 leg (same workload, parity mode)
+    # FP64 legs (same workload): the parity mode (every sum in the reference's order) and the
+    # tiled FP64 kernels
+    k64 = max(5, args.steps // 5)
     eng64, sid64 = make_engine(c, capi.FP64)
-    total64, _, lat64 = time_steps(eng64, sid64, max(5, args.steps // 5), 2)
+    total64, _, lat64 = time_steps(eng64, sid64, k64, 2)
     eng64.close()
-    value64 = E * max(5, args.steps // 5) / (total64 / 1e3)
+    engt, sidt = make_engine(c, capi.FP64_TILED)
+    total64t, _, lat64t = time_steps(engt, sidt, k64, 2)
+    engt.close()
 
-    us, bytes_alg = dforce_roofline(c, capi.FP32, args.dforce_reps)
+    us, bytes_alg, tiling = apply_roofline(c, capi.FP32, args.dforce_reps)
     achieved = bytes_alg / (us * 1e-6) / 1e9
-    us64, bytes64 = dforce_roofline(c, capi.FP64, max(10, args.dforce_reps // 4))
+    us64, bytes64, _ = apply_roofline(c, capi.FP64_TILED, max(10, args.dforce_reps // 4))
     probe = copy_probe()
 
     err = None if args.no_error else position_error(c, args.error_steps)
@@ -244,14 +244,18 @@ def main():
         "p99_ms": float(np.percentile(lat, 99)),
         "wall_s_timed": wall,
         "cg_last": {"iterations": st["iterations"], "end_reason": capi.END_REASONS.get(st["end_reason"])},
-        "fp64": {"value": value64, "ms_per_step": total64 / max(5, args.steps // 5),
-                 "p50_ms": float(np.percentile(lat64, 50)), "p99_ms": float(np.percentile(lat64, 99)),
-                 "dforce_us": us64, "dforce_bytes": bytes64,
-                 "dforce_frac": bytes64 / (us64 * 1e-6) / 1e9 / HBM_PEAK_GBS},
+        "fp64_parity_mode": {"value": E * k64 / (total64 / 1e3), "ms_per_step": total64 / k64,
+                             "p50_ms": float(np.percentile(lat64, 50)), "p99_ms": float(np.percentile(lat64, 99))},
+        "fp64_tiled": {"value": E * k64 / (total64t / 1e3), "ms_per_step": total64t / k64,
+                       "p50_ms": float(np.percentile(lat64t, 50)), "p99_ms": float(np.percentile(lat64t, 99)),
+                       "apply_us": us64, "apply_bytes": bytes64,
+                       "apply_frac": bytes64 / (us64 * 1e-6) / 1e9 / HBM_PEAK_GBS},
+        "tiling": tiling,
         "error_vs_fp64": err,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "addDForce (k_elem_add_dforce + k_gather), FP32", "us_per_launch": us,
+                     "kernel": "k_tile_apply<float> (CG operator: addDForce + mass + projection + p update)",
+                     "us_per_launch": us,
                      "algorithmic_bytes": bytes_alg},
         "copy_probe": probe,
         "cpu_baseline": cpu,

@@ -108,6 +108,11 @@ int sofa_amd_engine_commit(sofa_amd_engine eng);
 int sofa_amd_engine_step(sofa_amd_engine eng, const int32_t* scenes, int n_scenes, int n_steps,
                          void* stream);
 
+/* Replay each step from a captured hipGraph (one graph launch per step; the graph of a launch
+ * set is captured on its first step and kept until the set changes).  Off by default.  No
+ * reference counterpart: SOFA has no device graphs. */
+int sofa_amd_engine_set_graphs(sofa_amd_engine eng, int enable);
+
 /* Blocks until the engine's work is done. */
 int sofa_amd_engine_sync(sofa_amd_engine eng);
 
@@ -135,6 +140,19 @@ int sofa_amd_fem_add_force(sofa_amd_engine eng, int scene, const void* x_dev, vo
  * sofa_amd_fem_add_force. */
 int sofa_amd_fem_add_dforce(sofa_amd_engine eng, int scene, const void* dx_dev, void* df_dev,
                             double k_factor, void* stream);
+
+/* ---- measurement and introspection (no reference counterpart) ---------------------------- */
+
+/* `reps` launches of the scene's CG operator kernel (tiled engines: the tile apply kernel with
+ * the mid-loop data flow p = r + beta p, q = A p) on the current state, for roofline timing
+ * with events on `stream`.  Leaves p, q and the CG scalars of the scene undefined: step the
+ * scene afresh (or discard the engine) afterwards. */
+int sofa_amd_engine_bench_apply(sofa_amd_engine eng, int scene, int reps, void* stream);
+
+/* Tiling of a scene: out7 = {tiles, tile-local nodes, nodes finished by the shared-node
+ * kernel, shared partial slots, incidences, elements, nodes}; zeros but the last two on the
+ * exact (untiled) path. */
+int sofa_amd_scene_tiling(sofa_amd_engine eng, int scene, int32_t* out7);
 
 #ifdef __cplusplus
 }

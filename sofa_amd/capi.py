@@ -25,7 +25,8 @@ EXPORTS = [
     "sofa_amd_scene_set_mass", "sofa_amd_scene_set_gravity", "sofa_amd_scene_set_fixed",
     "sofa_amd_scene_set_solver", "sofa_amd_engine_commit", "sofa_amd_engine_step", "sofa_amd_engine_sync",
     "sofa_amd_scene_read", "sofa_amd_scene_write", "sofa_amd_scene_stats", "sofa_amd_scene_element",
-    "sofa_amd_fem_add_force", "sofa_amd_fem_add_dforce",
+    "sofa_amd_fem_add_force", "sofa_amd_fem_add_dforce", "sofa_amd_engine_bench_apply", "sofa_amd_scene_tiling",
+    "sofa_amd_engine_set_graphs",
 ]
 
 
@@ -79,6 +80,9 @@ def lib():
         "sofa_amd_scene_element": (c_int, [_vp, c_int, c_int, _dp, _dp, _dp]),
         "sofa_amd_fem_add_force": (c_int, [_vp, c_int, _vp, _vp, _vp]),
         "sofa_amd_fem_add_dforce": (c_int, [_vp, c_int, _vp, _vp, c_double, _vp]),
+        "sofa_amd_engine_bench_apply": (c_int, [_vp, c_int, c_int, _vp]),
+        "sofa_amd_scene_tiling": (c_int, [_vp, c_int, _ip]),
+        "sofa_amd_engine_set_graphs": (c_int, [_vp, c_int]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -163,6 +167,9 @@ class Engine:
     def sync(self):
         check(lib().sofa_amd_engine_sync(self._h))
 
+    def set_graphs(self, enable=True):
+        check(lib().sofa_amd_engine_set_graphs(self._h, 1 if enable else 0))
+
     # ---- state ----
     def read(self, scene):
         n = self.n_nodes[scene]
@@ -195,3 +202,13 @@ class Engine:
 
     def fem_add_dforce(self, scene, dx_dev_ptr, df_dev_ptr, k_factor, stream=None):
         check(lib().sofa_amd_fem_add_dforce(self._h, int(scene), dx_dev_ptr, df_dev_ptr, float(k_factor), stream))
+
+    # ---- measurement ----
+    def bench_apply(self, scene, reps, stream=None):
+        check(lib().sofa_amd_engine_bench_apply(self._h, int(scene), int(reps), stream))
+
+    def tiling(self, scene):
+        out = np.zeros(7, dtype=np.int32)
+        check(lib().sofa_amd_scene_tiling(self._h, int(scene), out.ctypes.data_as(_ip)))
+        keys = ("tiles", "tile_local_nodes", "finalize_nodes", "shared_slots", "incidences", "elements", "nodes")
+        return dict(zip(keys, (int(v) for v in out)))

@@ -4,8 +4,12 @@
 // Element initialisation restates TetrahedronFEMForceField::reinit for method "large"
 // (TetrahedronFEMForceField.inl:1391-1545): computeMaterialStiffness (:256-291), initLarge
 // (:835-868) with computeRotationLarge (:755-778).  It runs once per scene on the host in
-// double, in the reference's operation order (this file is compiled with contraction off), and
-// the result is rounded to the element precision of the engine.
+// double, in the reference's operation order (contraction off in these functions whatever the
+// translation unit's flags), and the result is rounded to the element precision of the engine.
+//
+// The file is compiled twice: once with contraction off (the FP64 engines and the C ABI) and
+// once with SOFA_AMD_TU_FP32 and contraction on (the FP32 engine, whose kernels may fuse
+// multiply-adds), so that each precision gets its own device code.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -18,7 +22,7 @@
 #include <string>
 #include <vector>
 
-#include "../../include/sofa_amd.h"
+#include "sofa_amd.h"
 #include "fem_kernels.h"
 #include "fem_tiled.h"
 #include "tiling.h"
@@ -48,17 +52,22 @@ struct HipError : std::runtime_error {
 // ---------------------------------------------------------------------------------------
 namespace host {
 struct V3 { double v[3]; };
-static inline V3 sub(const V3& a, const V3& b) { return {{a.v[0] - b.v[0], a.v[1] - b.v[1], a.v[2] - b.v[2]}}; }
+static inline V3 sub(const V3& a, const V3& b) {
+    return {{a.v[0] - b.v[0], a.v[1] - b.v[1], a.v[2] - b.v[2]}};
+}
 static inline V3 cross(const V3& a, const V3& b) {
+#pragma clang fp contract(off)
     return {{a.v[1] * b.v[2] - a.v[2] * b.v[1], a.v[2] * b.v[0] - a.v[0] * b.v[2], a.v[0] * b.v[1] - a.v[1] * b.v[0]}};
 }
 static inline double dot(const V3& a, const V3& b) {
+#pragma clang fp contract(off)
     double r = a.v[0] * b.v[0];
     r += a.v[1] * b.v[1];
     r += a.v[2] * b.v[2];
     return r;
 }
 static inline V3 normalized(V3 a) {
+#pragma clang fp contract(off)
     double n2 = a.v[0] * a.v[0];
     n2 += a.v[1] * a.v[1];
     n2 += a.v[2] * a.v[2];
@@ -69,6 +78,7 @@ static inline V3 normalized(V3 a) {
 }
 // Tetrahedron.h:55-83
 static inline double volume(const V3& n0, const V3& n1, const V3& n2, const V3& n3) {
+#pragma clang fp contract(off)
     return std::abs(dot(cross(sub(n1, n0), sub(n2, n0)), sub(n3, n0)) / 6.0);
 }
 
@@ -80,6 +90,7 @@ struct ElemInit {
 };
 
 static bool init_element(const std::vector<V3>& P, const int32_t* t, double E, double nu, ElemInit& out) {
+#pragma clang fp contract(off)
     const V3 &pa = P[t[0]], &pb = P[t[1]], &pc = P[t[2]], &pd = P[t[3]];
     // computeMaterialStiffness (localStiffnessFactor empty: youngModulus = 1.0f * E)
     const double young = 1.0f * E;
@@ -145,6 +156,7 @@ struct EngineBase {
     hipStream_t stream = nullptr;
     std::vector<SceneHost> scenes;
     bool committed = false;
+    bool use_graphs = false;
     virtual ~EngineBase() {}
     virtual void commit() = 0;
     virtual void step(const int32_t* ids, int n, int nsteps, hipStream_t s) = 0;
@@ -154,6 +166,9 @@ struct EngineBase {
     virtual void element(int scene, int e, double* rot9, double* k3, double* rest12) = 0;
     virtual void add_force(int scene, const void* x, void* f, hipStream_t s) = 0;
     virtual void add_dforce(int scene, const void* dx, void* df, double k, hipStream_t s) = 0;
+    virtual void bench_apply(int scene, int reps, hipStream_t s) = 0;
+    virtual void tiling(int scene, int32_t* out) = 0;
+    virtual void reset_graph() = 0;
 };
 
 template <typename T>
@@ -187,7 +202,7 @@ struct EngineT : EngineBase {
     std::vector<TileDesc> tiles_h;
     std::vector<FinBlk> fin_h;
     std::vector<NodeBlk> nb_h;
-    std::vector<int> tile_begin, tile_count, fin_begin, fin_count, nbk_begin, nbk_count;
+    std::vector<int> tile_begin, tile_count, fin_begin, fin_count, nbk_begin, nbk_count, shared_slots;
     TileDesc* tiles_d = nullptr;
     Ranges* ranges_d = nullptr;
     uint2* ltet = nullptr;
@@ -200,8 +215,10 @@ struct EngineT : EngineBase {
     void *set_a = nullptr, *set_b = nullptr, *set_c = nullptr;
     int* set_scenes = nullptr;
     int set_na = 0, set_nb = 0, set_nc = 0, set_maxit = 0;
+    hipGraphExec_t gexec = nullptr;
 
     ~EngineT() override {
+        drop_graph();
         void* ptrs[] = {x, v, f, b, r, p, q, xcg, mass, fixed, inc_ptr, inc, tet, rot, rest, kmat, ef, ef2,
                         inst, partial, tiles_d, ranges_d, ltet, ln_gid, ln_inc_ptr, ln_pslot, sh_gid, sh_ptr,
                         ln_inc, sp0, sp1, pbb, pden, prho, set_a, set_b, set_c, set_scenes};
@@ -234,7 +251,7 @@ struct EngineT : EngineBase {
         eblk_h.clear(); nblk_h.clear();
         tiles_h.clear(); fin_h.clear(); nb_h.clear();
         tile_begin.assign(S, 0); tile_count.assign(S, 0); fin_begin.assign(S, 0); fin_count.assign(S, 0);
-        nbk_begin.assign(S, 0); nbk_count.assign(S, 0);
+        nbk_begin.assign(S, 0); nbk_count.assign(S, 0); shared_slots.assign(S, 0);
         iperm.assign(S, {});
         std::vector<double> xh(3 * Ntot), massh(Ntot, 0.0);
         std::vector<int8_t> fixh(Ntot, 0);
@@ -342,9 +359,11 @@ struct EngineT : EngineBase {
                 }
                 fin_begin[si] = (int)fin_h.size();
                 for (int j0 = 0; j0 < (int)shared.size(); j0 += kBlock)
-                    fin_h.push_back(FinBlk{si, shbase + j0, shbase + std::min((int)shared.size(), j0 + kBlock), 0});
+                    fin_h.push_back(FinBlk{si, shbase + j0, shbase + std::min((int)shared.size(), j0 + kBlock), 0,
+                                           j0 / kBlock, 0, 0, 0});
                 fin_count[si] = (int)fin_h.size() - fin_begin[si];
                 slot_base += sptr.back();
+                shared_slots[si] = sptr.back();
                 nbk_begin[si] = (int)nb_h.size();
                 const int per = kBlock * kNodesPerThread;
                 for (int n0 = 0; n0 < s.n_nodes; n0 += per)
@@ -444,6 +463,8 @@ struct EngineT : EngineBase {
         std::vector<int32_t> want(ids, ids + n);
         if (want == set_ids && set_scenes) return;
         HIPCHK(hipStreamSynchronize(s));   // the previous lists may still be in use
+        HIPCHK(hipDeviceSynchronize());
+        drop_graph();
         int maxit = 0;
         if (!tiled) {
             std::vector<Blk> eb, nb;
@@ -476,30 +497,27 @@ struct EngineT : EngineBase {
         set_maxit = maxit;
     }
 
-    void step(const int32_t* ids, int n, int nsteps, hipStream_t s) override {
-        prepare_set(ids, n, s);
-        if (tiled) { step_tiled(n, nsteps, s); return; }
+    // one implicit Euler step of the current launch set, enqueued on `s`
+    void launch_step(int n, hipStream_t s) {
+        if (tiled) { launch_step_tiled(n, s); return; }
         const ElemView<R, NR> ev = view();
         const NodeArrays na = nodes();
         const Blk* eb = (const Blk*)set_a;
         const Blk* nb = (const Blk*)set_b;
-        for (int st = 0; st < nsteps; st++) {
-            if (set_na > 0) k_rhs_elem<R, NR, VT><<<set_na, kBlock, 0, s>>>(ev, eb, inst, x, v, ef, ef2);
-            if (set_nb > 0) k_rhs_node<VT><<<set_nb, kBlock, 0, s>>>(nb, inst, na, ef, ef2, v, f, b, r, xcg, partial);
-            for (int it = 0; it < set_maxit; it++) {
-                if (set_na > 0) k_cg_elem<R, NR, VT><<<set_na, kBlock, 0, s>>>(ev, eb, inst, r, p, ef);
-                if (set_nb > 0) {
-                    k_cg_node<VT><<<set_nb, kBlock, 0, s>>>(nb, inst, na, ef, r, p, q, partial);
-                    k_cg_update<VT><<<set_nb, kBlock, 0, s>>>(nb, inst, p, q, xcg, r, partial);
-                }
+        if (set_na > 0) k_rhs_elem<R, NR, VT><<<set_na, kBlock, 0, s>>>(ev, eb, inst, x, v, ef, ef2);
+        if (set_nb > 0) k_rhs_node<VT><<<set_nb, kBlock, 0, s>>>(nb, inst, na, ef, ef2, v, f, b, r, xcg, partial);
+        for (int it = 0; it < set_maxit; it++) {
+            if (set_na > 0) k_cg_elem<R, NR, VT><<<set_na, kBlock, 0, s>>>(ev, eb, inst, r, p, ef);
+            if (set_nb > 0) {
+                k_cg_node<VT><<<set_nb, kBlock, 0, s>>>(nb, inst, na, ef, r, p, q, partial);
+                k_cg_update<VT><<<set_nb, kBlock, 0, s>>>(nb, inst, p, q, xcg, r, partial);
             }
-            if (set_nb > 0) k_euler_update<VT><<<set_nb, kBlock, 0, s>>>(nb, inst, xcg, v, x);
-            k_finalize<<<(n + 63) / 64, 64, 0, s>>>(set_scenes, n, inst);
         }
-        HIPCHK(hipGetLastError());
+        if (set_nb > 0) k_euler_update<VT><<<set_nb, kBlock, 0, s>>>(nb, inst, xcg, v, x);
+        k_finalize<<<(n + 63) / 64, 64, 0, s>>>(set_scenes, n, inst);
     }
 
-    void step_tiled(int n, int nsteps, hipStream_t s) {
+    void launch_step_tiled(int n, hipStream_t s) {
         TileArgs<R, NR> ta;
         ta.tiles = (const TileDesc*)set_a;
         ta.inst = inst;
@@ -512,19 +530,83 @@ struct EngineT : EngineBase {
         const FinBlk* fb = (const FinBlk*)set_b;
         const NodeBlk* nb = (const NodeBlk*)set_c;
         const SharedNodes sn{sh_gid, sh_ptr};
-        for (int st = 0; st < nsteps; st++) {
-            if (set_na > 0) k_tile_rhs<R, NR, VT><<<set_na, kBlock, 0, s>>>(ta, x, v, f, b, r, xcg, sp0, sp1, pbb);
-            if (set_nb > 0) k_fin_rhs<VT><<<set_nb, kBlock, 0, s>>>(fb, inst, sn, mass, fixed, v, sp0, sp1, f, b, r, xcg, pbb);
-            for (int it = 1; it <= set_maxit; it++) {
-                if (set_na > 0)
-                    k_tile_apply<R, NR, VT><<<set_na, kBlock, 0, s>>>(ta, it, r, p, q, sp0, it == 1 ? pbb : prho, pden);
-                if (set_nb > 0) k_fin_apply<VT><<<set_nb, kBlock, 0, s>>>(fb, inst, sn, mass, fixed, it, r, p, q, sp0, pden);
-                if (set_nc > 0) k_update<VT><<<set_nc, kBlock, 0, s>>>(nb, inst, ranges_d, it, p, q, xcg, r, pden, prho);
-            }
-            if (set_nc > 0) k_integrate<VT><<<set_nc, kBlock, 0, s>>>(nb, inst, xcg, v, x);
-            k_step_end_tiled<<<(n + 63) / 64, 64, 0, s>>>(set_scenes, n, inst);
+        if (set_na > 0) k_tile_rhs<R, NR, VT><<<set_na, kBlock, 0, s>>>(ta, x, v, f, b, r, xcg, sp0, sp1, pbb);
+        if (set_nb > 0) k_fin_rhs<VT><<<set_nb, kBlock, 0, s>>>(fb, inst, ranges_d, sn, mass, fixed, v, sp0, sp1, f, b, r, xcg, pbb);
+        for (int it = 1; it <= set_maxit; it++) {
+            if (set_na > 0)
+                k_tile_apply<R, NR, VT><<<set_na, kBlock, 0, s>>>(ta, it, r, p, q, sp0, it == 1 ? pbb : prho, pden);
+            if (set_nb > 0) k_fin_apply<VT><<<set_nb, kBlock, 0, s>>>(fb, inst, ranges_d, sn, mass, fixed, it, r, p, q, sp0, pden);
+            if (set_nc > 0) k_update<VT><<<set_nc, kBlock, 0, s>>>(nb, inst, ranges_d, it, p, q, xcg, r, pden, prho);
         }
+        if (set_nc > 0) k_integrate<VT><<<set_nc, kBlock, 0, s>>>(nb, inst, xcg, v, x);
+        k_step_end_tiled<<<(n + 63) / 64, 64, 0, s>>>(set_scenes, n, inst);
+    }
+
+    void drop_graph() {
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        gexec = nullptr;
+    }
+    void reset_graph() override { drop_graph(); }
+
+    void step(const int32_t* ids, int n, int nsteps, hipStream_t s) override {
+        prepare_set(ids, n, s);
+        if (!use_graphs) {
+            for (int st = 0; st < nsteps; st++) launch_step(n, s);
+            HIPCHK(hipGetLastError());
+            return;
+        }
+        // the whole step (RHS, every CG iteration, update) is captured once per launch set and
+        // replayed: one graph launch per step instead of 3 n_cg + 5 kernel launches
+        if (!gexec && nsteps > 0) {
+            hipGraph_t graph = nullptr;
+            HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            launch_step(n, s);
+            const hipError_t le = hipGetLastError();
+            const hipError_t ce = hipStreamEndCapture(s, &graph);
+            if (le != hipSuccess) throw HipError(std::string("launch during capture: ") + hipGetErrorString(le));
+            if (ce != hipSuccess) throw HipError(std::string("hipStreamEndCapture: ") + hipGetErrorString(ce));
+            const hipError_t ie = hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            if (ie != hipSuccess) { gexec = nullptr; throw HipError(std::string("hipGraphInstantiate: ") + hipGetErrorString(ie)); }
+        }
+        for (int st = 0; st < nsteps; st++) HIPCHK(hipGraphLaunch(gexec, s));
+    }
+
+    // measurement hook: `reps` launches of the scene's CG operator kernel (the tiled apply, with
+    // the iteration-2 data flow: p = r + beta p, q = A p) on the current state
+    void bench_apply(int scene, int reps, hipStream_t s) override {
+        if (!tiled) throw std::invalid_argument("bench_apply needs a tiled engine");
+        const int32_t id = scene;
+        prepare_set(&id, 1, s);
+        TileArgs<R, NR> ta;
+        ta.tiles = (const TileDesc*)set_a;
+        ta.inst = inst;
+        ta.ranges = ranges_d;
+        ta.ev = view();
+        ta.ltet = ltet;
+        ta.tn = TileNodes{ln_gid, ln_inc_ptr, ln_inc, ln_pslot};
+        ta.mass = mass;
+        ta.fixed = fixed;
+        for (int i = 0; i < reps; i++)
+            if (set_na > 0) k_tile_apply<R, NR, VT><<<set_na, kBlock, 0, s>>>(ta, 2, r, p, q, sp0, prho, pden);
         HIPCHK(hipGetLastError());
+    }
+    // tiles, tile-local nodes, nodes finished by the finalize kernel, shared partial slots,
+    // incidences, elements, nodes
+    void tiling(int scene, int32_t* out) override {
+        const SceneHost& sc = scenes[scene];
+        for (int k = 0; k < 7; k++) out[k] = 0;
+        out[5] = (int32_t)(sc.tets.size() / 4);
+        out[6] = sc.n_nodes;
+        if (!tiled) return;
+        out[0] = tile_count[scene];
+        for (int t = 0; t < tile_count[scene]; t++) out[1] += tiles_h[tile_begin[scene] + t].ln_count;
+        for (int b = 0; b < fin_count[scene]; b++) {
+            const FinBlk& fb = fin_h[fin_begin[scene] + b];
+            out[2] += fb.end - fb.begin;
+        }
+        out[3] = shared_slots[scene];
+        out[4] = 4 * out[5];
     }
 
     void read_vec(const VT* src, int scene, double* out) {
@@ -626,8 +708,19 @@ struct EngineT : EngineBase {
     }
 };
 
+#if defined(SOFA_AMD_TU_FP32)
+EngineBase* make_engine_fp32() {
+    auto* e = new EngineT<float, 6, float>();
+    e->tiled = true;
+    return e;
+}
+#else
+EngineBase* make_engine_fp32();
+#endif
+
 }  // namespace sofa_amd
 
+#if !defined(SOFA_AMD_TU_FP32)
 // =========================================================================================
 // C ABI
 // =========================================================================================
@@ -682,9 +775,7 @@ int sofa_amd_engine_create(int device, int precision, sofa_amd_engine* out) {
         e->tiled = true;
         impl.reset(e);
     } else if (precision == SOFA_AMD_FP32) {
-        auto* e = new EngineT<float, 6, float>();
-        e->tiled = true;
-        impl.reset(e);
+        impl.reset(make_engine_fp32());
     } else {
         return fail(SOFA_AMD_EINVAL, "unsupported precision mode");
     }
@@ -876,4 +967,36 @@ int sofa_amd_fem_add_dforce(sofa_amd_engine eng, int scene, const void* dx_dev, 
     GUARD_END
 }
 
+int sofa_amd_engine_bench_apply(sofa_amd_engine eng, int scene, int reps, void* stream) {
+    GUARD_BEGIN
+    if (!scene_of(eng, scene) || reps < 0) return fail(SOFA_AMD_EINVAL, "invalid argument");
+    if (!eng->impl->committed) return fail(SOFA_AMD_ESTATE, "engine not committed");
+    HIPCHK(hipSetDevice(eng->impl->device));
+    eng->impl->bench_apply(scene, reps, stream ? (hipStream_t)stream : eng->impl->stream);
+    return SOFA_AMD_OK;
+    GUARD_END
+}
+
+int sofa_amd_scene_tiling(sofa_amd_engine eng, int scene, int32_t* out7) {
+    GUARD_BEGIN
+    if (!scene_of(eng, scene) || !out7) return fail(SOFA_AMD_EINVAL, "invalid argument");
+    if (!eng->impl->committed) return fail(SOFA_AMD_ESTATE, "engine not committed");
+    eng->impl->tiling(scene, out7);
+    return SOFA_AMD_OK;
+    GUARD_END
+}
+
+int sofa_amd_engine_set_graphs(sofa_amd_engine eng, int enable) {
+    GUARD_BEGIN
+    if (!eng || !eng->impl) return fail(SOFA_AMD_EINVAL, "null engine");
+    HIPCHK(hipSetDevice(eng->impl->device));
+    HIPCHK(hipDeviceSynchronize());
+    eng->impl->use_graphs = enable != 0;
+    eng->impl->reset_graph();
+    return SOFA_AMD_OK;
+    GUARD_END
+}
+
 }  // extern "C"
+
+#endif  // !SOFA_AMD_TU_FP32

@@ -290,6 +290,66 @@ __device__ __forceinline__ void elem_dforce(const ElemView<R, NR>& ev, int64_t e
     rot_apply(rot, F, c);
 }
 
+// the same 12 strain-displacement groups from rest coordinates already in registers
+// (NR == 9: b, c, d; NR == 6: b0, c0, c1, d0, d1, d2)
+template <typename R, int NR>
+__device__ __forceinline__ void jgroups_r(const R* rin, R g[12]) {
+    if constexpr (NR == 9) {
+        const R a0 = 0, a1 = 0, a2 = 0;
+        const R b0 = rin[0], b1 = rin[1], b2 = rin[2];
+        const R c0 = rin[3], c1 = rin[4], c2 = rin[5];
+        const R d0 = rin[6], d1 = rin[7], d2 = rin[8];
+        g[0] = -pdet(b1, c1, d1, b2, c2, d2);
+        g[1] = pdet(b0, c0, d0, b2, c2, d2);
+        g[2] = -pdet(b0, c0, d0, b1, c1, d1);
+        g[3] = pdet(c1, d1, a1, c2, d2, a2);
+        g[4] = -pdet(c0, d0, a0, c2, d2, a2);
+        g[5] = pdet(c0, d0, a0, c1, d1, a1);
+        g[6] = -pdet(d1, a1, b1, d2, a2, b2);
+        g[7] = pdet(d0, a0, b0, d2, a2, b2);
+        g[8] = -pdet(d0, a0, b0, d1, a1, b1);
+        g[9] = pdet(a1, b1, c1, a2, b2, c2);
+        g[10] = -pdet(a0, b0, c0, a2, b2, c2);
+        g[11] = pdet(a0, b0, c0, a1, b1, c1);
+    } else {
+        const R b0 = rin[0], c0 = rin[1], c1 = rin[2], d0 = rin[3], d1 = rin[4], d2 = rin[5];
+        g[0] = -(c1 * d2);
+        g[1] = c0 * d2 - b0 * d2;
+        g[2] = -(((c0 * d1 - c1 * d0) - b0 * d1) + b0 * c1);
+        g[3] = c1 * d2;
+        g[4] = -(c0 * d2);
+        g[5] = c0 * d1 - c1 * d0;
+        g[6] = 0;
+        g[7] = d2 * b0;
+        g[8] = -(d1 * b0);
+        g[9] = 0;
+        g[10] = 0;
+        g[11] = b0 * c1;
+    }
+}
+
+// applyStiffnessCorotational with the element record in registers
+template <typename R, int NR>
+__device__ __forceinline__ void elem_dforce_r(const R rot[9], const R* rin, const R k3[3], const R u[12],
+                                              R fact, R c[12]) {
+    R X[12];
+#pragma unroll
+    for (int n = 0; n < 4; n++) {
+#pragma unroll
+        for (int j = 0; j < 3; j++)
+            X[3 * n + j] = rot[0 * 3 + j] * u[3 * n] + rot[1 * 3 + j] * u[3 * n + 1] + rot[2 * 3 + j] * u[3 * n + 2];
+    }
+    R g[12];
+    jgroups_r<R, NR>(rin, g);
+    R JtD[6], KJ[6], F[12];
+    jt_mul<R, NR>(g, X, JtD);
+    k_mul(k3, JtD, KJ);
+#pragma unroll
+    for (int k = 0; k < 6; k++) KJ[k] *= fact;
+    j_mul(g, KJ, F);
+    rot_apply(rot, F, c);
+}
+
 template <typename R, int NR>
 __device__ __forceinline__ void load_rot(const ElemView<R, NR>& ev, int64_t e, R rot[9]) {
 #pragma unroll

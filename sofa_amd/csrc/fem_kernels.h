@@ -322,7 +322,7 @@ __global__ __launch_bounds__(kBlock) void k_euler_update(const Blk* __restrict__
     }
 }
 
-__global__ void k_finalize(const int* __restrict__ scenes, int n, InstDev* __restrict__ inst) {
+static __global__ void k_finalize(const int* __restrict__ scenes, int n, InstDev* __restrict__ inst) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     InstDev* in = &inst[scenes[i]];

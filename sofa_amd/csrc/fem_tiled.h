@@ -35,7 +35,7 @@ struct TileDesc {
     int pad;
 };
 
-struct FinBlk { int scene, begin, end, pslot; };           // shared-node range
+struct FinBlk { int scene, begin, end, pslot, bi, pad0, pad1, pad2; };  // shared-node range; bi = index in scene
 struct NodeBlk { int scene, begin, end, pslot, first, pad0, pad1, pad2; };  // node range
 
 struct TileNodes {
@@ -85,6 +85,18 @@ __device__ __forceinline__ void record_stop(InstDev* ins, bool first, int reason
         ins->last_iters = iters;
         ins->err = err;
     }
+}
+
+// the scene's tile partials are folded into its finalize-block partials (finalize block bi
+// of nf adds tiles [t0 + bi*ceil(nt/nf), ...)), so consumers reduce only nf values
+__device__ __forceinline__ double fold_tiles(const double* __restrict__ buf, const Ranges& rg, int bi) {
+    if (rg.nf <= 0) return 0.0;
+    const int chunk = (rg.nt + rg.nf - 1) / rg.nf;
+    const int a0 = rg.t0 + bi * chunk;
+    const int a1 = min(rg.t0 + rg.nt, a0 + chunk);
+    double acc = 0.0;
+    for (int k = a0; k < a1; k++) acc = acc + buf[k];
+    return acc;
 }
 
 template <typename R, int NR>
@@ -209,7 +221,7 @@ __global__ __launch_bounds__(kBlock) void k_tile_rhs(TileArgs<R, NR> a, const VT
 
 template <typename VT>
 __global__ __launch_bounds__(kBlock) void k_fin_rhs(const FinBlk* __restrict__ fb_, const InstDev* __restrict__ inst,
-                                                     SharedNodes sn, const double* __restrict__ mass,
+                                                     const Ranges* __restrict__ ranges, SharedNodes sn, const double* __restrict__ mass,
                                                      const int8_t* __restrict__ fixed, const VT* __restrict__ v,
                                                      const VT* __restrict__ spf, const VT* __restrict__ spv,
                                                      VT* __restrict__ f, VT* __restrict__ b, VT* __restrict__ r,
@@ -254,8 +266,8 @@ __global__ __launch_bounds__(kBlock) void k_fin_rhs(const FinBlk* __restrict__ f
         st3(xcg, g, z);
         bb = dot3d(bn, bn);
     }
-    const double tot = block_sum(bb, sh);
-    if (threadIdx.x == 0) pbb[fb.pslot] = tot;
+    const double own = block_sum(bb, sh);
+    if (threadIdx.x == 0) pbb[fb.pslot] = fold_tiles(pbb, ranges[fb.scene], fb.bi) + own;
 }
 
 // ---------------------------------------------------------------------------------------
@@ -266,18 +278,43 @@ __global__ __launch_bounds__(kBlock) void k_tile_apply(TileArgs<R, NR> a, int it
                                                         VT* __restrict__ p, VT* __restrict__ q, VT* __restrict__ sp,
                                                         const double* __restrict__ pin,   // |b|^2 (iter 1) or r.r partials
                                                         double* __restrict__ pden) {
+    constexpr int EPT = kTileElems / kBlock;   // elements per thread
     __shared__ R lu[kTileNodes * 3];
     __shared__ R lc[kTileElems * 12];
+    __shared__ uint16_t linc[kTileElems * 4];
+    __shared__ int lptr[kTileNodes + 1];
     __shared__ double sh[kBlock];
     const TileDesc td = a.tiles[blockIdx.x];
     InstDev* ins = &a.inst[td.scene];
     if (ins->stopped) return;
+    // ---- 1. issue every independent load first: the element records of this thread (kept in
+    //         registers) and the tile's incidence tables (copied to LDS)
+    R rot[EPT][9], rin[EPT][NR], kk[EPT][3];
+    uint2 lt[EPT];
+#pragma unroll
+    for (int m = 0; m < EPT; m++) {
+        const int k = threadIdx.x + m * kBlock;
+        if (k < td.e_count) {
+            const int64_t e = (int64_t)td.e_begin + k;
+            lt[m] = a.ltet[e];
+#pragma unroll
+            for (int i = 0; i < 9; i++) rot[m][i] = a.ev.rot[i * a.ev.E + e];
+#pragma unroll
+            for (int i = 0; i < NR; i++) rin[m][i] = a.ev.rest[i * a.ev.E + e];
+#pragma unroll
+            for (int i = 0; i < 3; i++) kk[m][i] = a.ev.kmat[i * a.ev.E + e];
+        }
+    }
+    const int inc0 = a.tn.inc_ptr[td.ln_begin];
+    const int ninc = 4 * td.e_count;
+    for (int j = threadIdx.x; j < ninc; j += kBlock) linc[j] = a.tn.inc[inc0 + j];
+    for (int j = threadIdx.x; j <= td.ln_count; j += kBlock) lptr[j] = a.tn.inc_ptr[td.ln_begin + j] - inc0;
     const Ranges rg = a.ranges[td.scene];
     const bool first = td.first != 0;
-    // ---- scalars: rho, the tolerance test, beta (CGLinearSolver.inl:130-196)
+    // ---- 2. scalars: rho, the tolerance test, beta (CGLinearSolver.inl:130-196)
     double beta = 0.0;
     if (iter == 1) {
-        const double bb = reduce2(pin, rg.t0, rg.nt, rg.f0, rg.nf, sh);
+        const double bb = rg.nf > 0 ? reduce2(pin, rg.f0, rg.nf, 0, 0, sh) : reduce2(pin, rg.t0, rg.nt, 0, 0, sh);
         const double normb = sqrt(bb);
         if (normb == 0.0) { record_stop(ins, first, 4, 0, 1.0); return; }
         const double err = sqrt(bb) / normb;
@@ -296,7 +333,7 @@ __global__ __launch_bounds__(kBlock) void k_tile_apply(TileArgs<R, NR> a, int it
             ins->rho_slot[iter & 1] = rho; ins->beta = beta; ins->iter = iter; ins->err = err;
         }
     }
-    // ---- stage p = r + beta p (interior nodes: this tile owns p)
+    // ---- 3. stage p = r + beta p (interior nodes: this tile owns p and writes it back)
     const VT vb = (VT)beta;
     for (int j = threadIdx.x; j < td.ln_count; j += kBlock) {
         const int li = td.ln_begin + j;
@@ -313,27 +350,31 @@ __global__ __launch_bounds__(kBlock) void k_tile_apply(TileArgs<R, NR> a, int it
         lu[3 * j] = (R)pp[0]; lu[3 * j + 1] = (R)pp[1]; lu[3 * j + 2] = (R)pp[2];
     }
     __syncthreads();
+    // ---- 4. element K p into LDS
     const R kcg = (R)ins->k_cg;
-    for (int k = threadIdx.x; k < td.e_count; k += kBlock) {
-        const int64_t e = (int64_t)td.e_begin + k;
-        int l[4];
-        unpack_ltet(a.ltet[e], l);
-        R u[12], rot[9], c[12];
 #pragma unroll
-        for (int n = 0; n < 4; n++)
+    for (int m = 0; m < EPT; m++) {
+        const int k = threadIdx.x + m * kBlock;
+        if (k < td.e_count) {
+            int l[4];
+            unpack_ltet(lt[m], l);
+            R u[12], c[12];
 #pragma unroll
-            for (int i = 0; i < 3; i++) u[3 * n + i] = lu[3 * l[n] + i];
-        load_rot(a.ev, e, rot);
-        elem_dforce<R, NR>(a.ev, e, rot, u, kcg, c);
+            for (int n = 0; n < 4; n++)
 #pragma unroll
-        for (int i = 0; i < 12; i++) lc[12 * k + i] = c[i];
+                for (int i = 0; i < 3; i++) u[3 * n + i] = lu[3 * l[n] + i];
+            elem_dforce_r<R, NR>(rot[m], rin[m], kk[m], u, kcg, c);
+#pragma unroll
+            for (int i = 0; i < 12; i++) lc[12 * k + i] = c[i];
+        }
     }
     __syncthreads();
+    // ---- 5. per-node sums in ascending element id, from LDS only
     const InstDev& in = *ins;
     double pq = 0.0;
     for (int j = threadIdx.x; j < td.ln_count; j += kBlock) {
         const int li = td.ln_begin + j;
-        const int s0 = a.tn.inc_ptr[li], s1 = a.tn.inc_ptr[li + 1];
+        const int s0 = lptr[j], s1 = lptr[j + 1];
         const int ps = a.tn.pslot[li];
         if (ps < 0) {
             const int g = a.tn.gid[li];
@@ -341,7 +382,7 @@ __global__ __launch_bounds__(kBlock) void k_tile_apply(TileArgs<R, NR> a, int it
             VT qn[3] = {0, 0, 0};
             if (in.m_cg != 0.0) mass_mdx(in, NodeArrays{nullptr, nullptr, a.mass, a.fixed}, g, in.m_cg, pp, qn);
             for (int s = s0; s < s1; s++) {
-                const int sl = a.tn.inc[s];
+                const int sl = linc[s];
 #pragma unroll
                 for (int i = 0; i < 3; i++) qn[i] -= (VT)lc[3 * sl + i];
             }
@@ -351,7 +392,7 @@ __global__ __launch_bounds__(kBlock) void k_tile_apply(TileArgs<R, NR> a, int it
         } else {
             VT acc[3] = {0, 0, 0};
             for (int s = s0; s < s1; s++) {
-                const int sl = a.tn.inc[s];
+                const int sl = linc[s];
 #pragma unroll
                 for (int i = 0; i < 3; i++) acc[i] -= (VT)lc[3 * sl + i];
             }
@@ -364,7 +405,8 @@ __global__ __launch_bounds__(kBlock) void k_tile_apply(TileArgs<R, NR> a, int it
 
 template <typename VT>
 __global__ __launch_bounds__(kBlock) void k_fin_apply(const FinBlk* __restrict__ fb_, InstDev* __restrict__ inst,
-                                                       SharedNodes sn, const double* __restrict__ mass,
+                                                       const Ranges* __restrict__ ranges, SharedNodes sn,
+                                                       const double* __restrict__ mass,
                                                        const int8_t* __restrict__ fixed, int iter,
                                                        const VT* __restrict__ r, VT* __restrict__ p,
                                                        VT* __restrict__ q, const VT* __restrict__ sp,
@@ -398,8 +440,8 @@ __global__ __launch_bounds__(kBlock) void k_fin_apply(const FinBlk* __restrict__
         st3(q, g, qn);
         pq = dot3d(pp, qn);
     }
-    const double tot = block_sum(pq, sh);
-    if (threadIdx.x == 0) pden[fb.pslot] = tot;
+    const double own = block_sum(pq, sh);
+    if (threadIdx.x == 0) pden[fb.pslot] = fold_tiles(pden, ranges[fb.scene], fb.bi) + own;
 }
 
 template <typename VT>
@@ -412,10 +454,17 @@ __global__ __launch_bounds__(kBlock) void k_update(const NodeBlk* __restrict__ n
     const NodeBlk nb = nb_[blockIdx.x];
     InstDev* ins = &inst[nb.scene];
     if (ins->stopped) return;
+    // node data first (registers), then the reduction
+    VT pv[kNodesPerThread][3], qv[kNodesPerThread][3], xv[kNodesPerThread][3], rv[kNodesPerThread][3];
+#pragma unroll
+    for (int k = 0; k < kNodesPerThread; k++) {
+        const int n = nb.begin + k * kBlock + threadIdx.x;
+        if (n < nb.end) { ld3(p, n, pv[k]); ld3(q, n, qv[k]); ld3(xcg, n, xv[k]); ld3(r, n, rv[k]); }
+    }
     const Ranges rg = ranges[nb.scene];
     const bool first = nb.first != 0;
     // den = p.q, threshold test, alpha (CGLinearSolver.inl:207-265)
-    const double den = reduce2(pden, rg.t0, rg.nt, rg.f0, rg.nf, sh);
+    const double den = rg.nf > 0 ? reduce2(pden, rg.f0, rg.nf, 0, 0, sh) : reduce2(pden, rg.t0, rg.nt, 0, 0, sh);
     const bool exempt = (iter == 1 && ins->first_solve);
     if (den == 0.0) { record_stop(ins, first, 3, iter, ins->err); if (first && threadIdx.x == 0) ins->den = den; return; }
     if (fabs(den) <= ins->thr && !exempt) {
@@ -431,14 +480,14 @@ __global__ __launch_bounds__(kBlock) void k_update(const NodeBlk* __restrict__ n
     for (int k = 0; k < kNodesPerThread; k++) {
         const int n = nb.begin + k * kBlock + threadIdx.x;
         if (n < nb.end) {
-            const int64_t o = 3 * (int64_t)n;
-            VT rn[3];
+            VT xn[3], rn[3];
 #pragma unroll
             for (int i = 0; i < 3; i++) {
-                xcg[o + i] += p[o + i] * va;
-                rn[i] = r[o + i] + q[o + i] * vma;
-                r[o + i] = rn[i];
+                xn[i] = xv[k][i] + pv[k][i] * va;
+                rn[i] = rv[k][i] + qv[k][i] * vma;
             }
+            st3(xcg, n, xn);
+            st3(r, n, rn);
             rr += dot3d(rn, rn);
         }
     }
@@ -469,7 +518,7 @@ __global__ __launch_bounds__(kBlock) void k_integrate(const NodeBlk* __restrict_
 }
 
 // scene bookkeeping after the update (one thread per scene of the launch set)
-__global__ void k_step_end_tiled(const int* __restrict__ scenes, int n, InstDev* __restrict__ inst) {
+static __global__ void k_step_end_tiled(const int* __restrict__ scenes, int n, InstDev* __restrict__ inst) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     InstDev* in = &inst[scenes[i]];
