@@ -1,16 +1,18 @@
 #!/usr/bin/env python3
 """Benchmark of the implicit corotational-FEM step on one MI355X.
 
-Workload (one "step" = one implicit Euler step = RHS + CG + update, i.e. one node::animate):
-the cantilever beam of the plain-at-scale configuration: RegularGridTopology 33x33x257 nodes
-over 1x1x8 m, 5 tetrahedra per hexahedron (1,310,720 tets, 279,873 nodes), E = 1e5 Pa,
-nu = 0.45, UniformMass totalMass = 1000 kg/m^3 * 8 m^3, z = 0 face fixed, gravity -9.81 y,
-EulerImplicitSolver rs = rm = 0.1, dt = 0.005, CGLinearSolver 50 iterations / 1e-9 / 1e-9.
+A "step" is one implicit Euler step (node::animate: RHS, CG, update) of every scene of the
+workload.  Headline workload (`value`): the batched job mix of 256 independent cantilever beams
+(node counts log-uniform in [500, 50,000], nx = ny in {3..9}, 1 x 1 x 4 m, E log-uniform in
+[1e4, 1e6], nu uniform in [0.3, 0.49], z = 0 face fixed, UniformMass 1000 kg/m^3, CG 25
+iterations, dt 0.01), FP32 element arithmetic with FP64 reductions, all 256 beams advanced in
+one launch per phase with the step replayed from a hipGraph.  Beside it, on the same job mix:
+the plain path (one instance at a time, its own launches per phase), and the position error
+against the FP64 path after 100 steps.  Secondary workload: the single 1.31M-tet beam
+(33x33x257 nodes, CG 50, dt 0.005).
 
-Prints ONE JSON line: element-steps/s of the FP32 path (whole job, inputs resident in HBM),
-p50/p99 per-step latency, the position error against the FP64 path after 100 steps, the
-roofline of the addDForce launch (HIP events on its stream) and the CPU oracle timed on this
-host (a bounded sample of the same workload).
+Prints ONE JSON line.  Inputs are resident in HBM before any timed region; timing uses HIP
+events on the stream the work is launched on.
 """
 from __future__ import annotations
 
@@ -26,30 +28,12 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0   # MI355X HBM3E, 8 TB/s
-
-
-def cfg1():
-    from sofa_amd import meshes
-    pos, tets, fixed = meshes.beam((33, 33, 257), (1.0, 1.0, 8.0))
-    return dict(pos=pos, tets=tets, fixed=fixed, E=1e5, nu=0.45, total_mass=8000.0, g=(0.0, -9.81, 0.0),
-                rs=0.1, rm=0.1, dt=0.005, iters=50, tol=1e-9, thr=1e-9)
-
-
-def make_engine(c, precision):
-    from sofa_amd import capi
-    eng = capi.Engine(0, precision)
-    n = len(c["pos"])
-    sid = eng.add_scene(c["pos"], c["tets"], [c["E"]], [c["nu"]], c["fixed"], mass_kind=capi.MASS_UNIFORM,
-                        mass_value=c["total_mass"] / float(n), gravity=c["g"], dt=c["dt"],
-                        rayleigh_stiffness=c["rs"], rayleigh_mass=c["rm"], cg_iterations=c["iters"],
-                        cg_tolerance=c["tol"], cg_threshold=c["thr"])
-    eng.commit()
-    return eng, sid
+_STREAM = None
 
 
 def bench_stream():
-    """A dedicated stream (default priority): the library's calls and the HIP events that time
-    them must share a non-null stream handle (NULL selects the engine's own stream)."""
+    """A dedicated stream (default priority): library calls and the HIP events that time them
+    share this non-null stream handle (NULL would select the engine's own stream)."""
     import torch
     global _STREAM
     if _STREAM is None:
@@ -57,65 +41,86 @@ def bench_stream():
     return _STREAM
 
 
-_STREAM = None
+def make_engine(specs, precision, graphs=True):
+    from sofa_amd import capi, workloads
+    eng = capi.Engine(0, precision)
+    ids = [workloads.add_to_engine(eng, s) for s in specs]
+    eng.commit()
+    eng.set_graphs(graphs)
+    return eng, ids
 
 
-def time_steps(eng, sid, steps, warmup):
+def time_merged(eng, ids, steps, warmup):
+    """Steps of all scenes together; per-step latency from events around each merged step."""
     import torch
     stream = bench_stream()
     sp = stream.cuda_stream
-    eng.step([sid], warmup, stream=sp)
+    eng.step(ids, warmup, stream=sp)
     torch.cuda.synchronize()
     ev = [torch.cuda.Event(enable_timing=True) for _ in range(steps + 1)]
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
     ev[0].record(stream)
     for i in range(steps):
-        eng.step([sid], 1, stream=sp)
+        eng.step(ids, 1, stream=sp)
         ev[i + 1].record(stream)
     torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
     total_ms = ev[0].elapsed_time(ev[steps])
     lat = np.array([ev[i].elapsed_time(ev[i + 1]) for i in range(steps)])
-    return total_ms, wall, lat
+    return total_ms, lat
 
 
-def apply_roofline(c, precision, reps, warm_steps=3):
-    """The dominant kernel: the tiled CG operator apply (k_tile_apply: p = r + beta p staged in
-    LDS, element K p, per-node sums in LDS, q of tile-owned nodes, partials of shared nodes),
-    launched `reps` times on a state advanced by `warm_steps` steps and timed with HIP events on
-    the stream it is launched on.
-    Algorithmic bytes per launch: E (16 + 18 p_e) + N 12 p_v
-      per element: 16 B of indices (4 x u16 local node ids + 4 x u16 incidence slots),
-                   rotation 9, rest coordinates 6, stiffness 3 reals;
-      per node: r and p read, p and q written (3 reals each)."""
+def time_plain(eng, ids, steps, warmup):
+    """One instance at a time: each scene's step is its own launch sequence; per-step latency of
+    a scene = events around that scene's step."""
     import torch
-    from sofa_amd import capi
-    eng, sid = make_engine(c, precision)
-    n, E = len(c["pos"]), len(c["tets"])
     stream = bench_stream()
     sp = stream.cuda_stream
-    eng.step([sid], warm_steps, stream=sp)
-    eng.bench_apply(sid, 3, stream=sp)
+    for _ in range(warmup):
+        for s in ids:
+            eng.step([s], 1, stream=sp)
+    torch.cuda.synchronize()
+    n = len(ids)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(steps * n + 1)]
+    ev[0].record(stream)
+    k = 0
+    for _ in range(steps):
+        for s in ids:
+            eng.step([s], 1, stream=sp)
+            k += 1
+            ev[k].record(stream)
+    torch.cuda.synchronize()
+    total_ms = ev[0].elapsed_time(ev[k])
+    lat = np.array([ev[i].elapsed_time(ev[i + 1]) for i in range(k)])
+    return total_ms, lat
+
+
+def apply_roofline(eng, reps, pe, pv, nrec, E, N):
+    """The dominant kernel: the tiled CG operator apply k_tile_apply (p = r + beta p staged in LDS,
+    element K p, per-node sums in LDS, q of tile-owned nodes, partials of shared nodes) over
+    every scene of the engine in one launch, timed with HIP events on its stream.
+    Algorithmic bytes per launch: E (16 + nrec p_e) + N 12 p_v
+      per element: 16 B of indices (4 local node ids + 4 incidence slots, u16 each) and nrec
+                   reals (rotation 9, rest coordinates 6 (FP32) or 9 (FP64), stiffness 3);
+      per node: r and p read, p and q written."""
+    import torch
+    stream = bench_stream()
+    sp = stream.cuda_stream
+    eng.bench_apply(-1, 3, stream=sp)
     torch.cuda.synchronize()
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
     e0.record(stream)
-    eng.bench_apply(sid, reps, stream=sp)
+    eng.bench_apply(-1, reps, stream=sp)
     e1.record(stream)
     torch.cuda.synchronize()
     us = e0.elapsed_time(e1) * 1e3 / reps
-    tiling = eng.tiling(sid)
-    eng.close()
-    pe = pv = 8 if precision == capi.FP64_TILED else 4
-    bytes_alg = E * (16 + 18 * pe) + n * 12 * pv
-    if precision == capi.FP64_TILED:
-        bytes_alg = E * (16 + 21 * pe) + n * 12 * pv     # exact record: 9 rest coordinates
-    return us, bytes_alg, tiling
+    bytes_alg = E * (16 + nrec * pe) + N * 12 * pv
+    ach = bytes_alg / (us * 1e-6) / 1e9
+    return {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
+            "traffic": None, "us_per_launch": us, "algorithmic_bytes": bytes_alg}
 
 
 def copy_probe(mb=512, reps=5):
-    """A plain device copy of known size (calibrates the HBM byte counters)."""
+    """A plain device copy of known size (calibrates the HBM byte counters; STREAM-copy rate)."""
     import torch
     nbytes = mb * (1 << 20)
     a = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda").fill_(1.0)
@@ -134,56 +139,64 @@ def copy_probe(mb=512, reps=5):
     return {"bytes": 2 * nbytes, "us": us, "GBs": 2 * nbytes / us / 1e3}
 
 
-def position_error(c, steps):
-    """FP32 vs FP64 positions after `steps` steps: eps_rel = max|x - x64| / max|x64| and the
-    stricter eps_disp = max|x - x64| / max|x64 - x0|."""
+def position_error(specs, precision, steps):
+    """Positions of `precision` vs the FP64 parity path after `steps` steps, per scene:
+    eps_rel = max_i |x_i - x64_i| / max_i |x64_i|, eps_disp = max_i |x_i - x64_i| / max_i |x64_i - x0_i|.
+    The FP64 parity path is bit-identical to the CPU oracle replaying the GPU's dot order and
+    within 1e-10 of the oracle in the reference's order (tests/test_gpu_parity.py)."""
     from sofa_amd import capi
-    out = {}
     res = {}
-    for name, prec in (("fp64", capi.FP64), ("fp32", capi.FP32)):
-        eng, sid = make_engine(c, prec)
-        eng.step([sid], steps)
-        res[name] = eng.read(sid)[0]
-        st = eng.stats(sid)
-        out[f"{name}_cg_iterations_last"] = st["iterations"]
+    for name, prec in (("fp64", capi.FP64), ("test", precision)):
+        eng, ids = make_engine(specs, prec, graphs=True)
+        eng.step(ids, steps)
+        res[name] = [eng.read(i)[0] for i in ids]
         eng.close()
-    d = np.linalg.norm(res["fp32"] - res["fp64"], axis=1)
-    out["eps_rel"] = float(d.max() / np.linalg.norm(res["fp64"], axis=1).max())
-    disp = np.linalg.norm(res["fp64"] - c["pos"], axis=1).max()
-    out["eps_disp"] = float(d.max() / disp) if disp > 0 else None
-    out["steps"] = steps
-    out["max_displacement_m"] = float(disp)
-    return out
+    er, ed = [], []
+    for x, x64, s in zip(res["test"], res["fp64"], specs):
+        d = np.linalg.norm(x - x64, axis=1).max()
+        er.append(d / np.linalg.norm(x64, axis=1).max())
+        disp = np.linalg.norm(x64 - s["pos"], axis=1).max()
+        ed.append(d / disp if disp > 0 else 0.0)
+    return {"steps": steps, "eps_rel_max": float(max(er)), "eps_rel_median": float(np.median(er)),
+            "eps_disp_max": float(max(ed)), "scenes": len(specs)}
 
 
-def cpu_baseline(c, steps):
-    """The FP64 oracle (a C++ restatement of the reference path, one thread) on the host."""
+def cpu_baseline(specs, steps, stride):
+    """The FP64 oracle (C++ restatement of the reference path, one thread) on this host, over a
+    bounded sample: every `stride`-th beam of the mix, `steps` step(s) each."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     from oracle import OracleScene
-    s = OracleScene(c["pos"])
-    s.set_gravity(c["g"])
-    s.set_fem(c["tets"], [c["E"]], [c["nu"]], "large")
-    s.set_uniform_mass(total_mass=c["total_mass"])
-    s.set_fixed(c["fixed"])
-    s.set_solver(c["rs"], c["rm"], c["iters"], c["tol"], c["thr"])
-    t0 = time.perf_counter()
-    s.step(c["dt"], steps)
-    dt = time.perf_counter() - t0
-    return {"value": len(c["tets"]) * steps / dt, "unit": "element-steps/s", "cores": 1, "kind": "port",
-            "sample": f"{steps} step(s) of the same 1.31M-tet beam, FP64, CG 50, 1 thread "
-                      f"({dt:.1f} s); oracle/fem_oracle.cpp built -O2 -ffp-contract=off"}
+    elem_steps, secs = 0, 0.0
+    for s in specs[::stride]:
+        o = OracleScene(s["pos"])
+        o.set_gravity(s["g"])
+        o.set_fem(s["tets"], np.atleast_1d(s["E"]), np.atleast_1d(s["nu"]), "large")
+        o.set_uniform_mass(vertex_mass=s["mass"][1])
+        o.set_fixed(s["fixed"])
+        o.set_solver(s["rs"], s["rm"], s["iters"], s["tol"], s["thr"])
+        t0 = time.perf_counter()
+        o.step(s["dt"], steps)
+        secs += time.perf_counter() - t0
+        elem_steps += len(s["tets"]) * steps
+    n = len(specs[::stride])
+    return {"value": elem_steps / secs, "unit": "element-steps/s", "cores": 1, "kind": "port",
+            "sample": f"{n} of the 256 beams (every {stride}th), {steps} step(s) each, FP64, CG 25, one thread: "
+                      f"{elem_steps} element-steps in {secs:.1f} s; oracle/fem_oracle.cpp (-O2 -ffp-contract=off)"}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--dforce-reps", type=int, default=100)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--plain-steps", type=int, default=3)
+    ap.add_argument("--apply-reps", type=int, default=50)
     ap.add_argument("--error-steps", type=int, default=100)
     ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--cpu-stride", type=int, default=4)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-error", action="store_true")
+    ap.add_argument("--no-cfg1", action="store_true")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -193,71 +206,80 @@ def main():
         sys.exit(2)
 
     import torch
-    from sofa_amd import capi
+    from sofa_amd import capi, workloads
     torch.cuda.set_device(0)
-    c = cfg1()
-    n, E = len(c["pos"]), len(c["tets"])
 
-    # main timed leg: FP32 plain path
-    eng, sid = make_engine(c, capi.FP32)
-    total_ms, wall, lat = time_steps(eng, sid, args.steps, args.warmup)
-    st = eng.stats(sid)
-    eng.close()
-    ms_per_step = total_ms / args.steps
+    # ---- headline: cfg3 job mix, batched vs plain, FP32
+    specs = workloads.cfg3()
+    E = sum(len(s["tets"]) for s in specs)
+    N = sum(len(s["pos"]) for s in specs)
+    eng, ids = make_engine(specs, capi.FP32, graphs=True)
+    total_ms, lat = time_merged(eng, ids, args.steps, args.warmup)
     value = E * args.steps / (total_ms / 1e3)
+    roof = apply_roofline(eng, args.apply_reps, 4, 4, 18, E, N)
+    roof["kernel"] = "k_tile_apply<float> over the 256-beam mix (CG operator: addDForce + mass + projection + p update)"
+    tiling = {k: 0 for k in ("tiles", "tile_local_nodes", "finalize_nodes", "shared_slots")}
+    for i in ids:
+        t = eng.tiling(i)
+        for k in tiling:
+            tiling[k] += t[k]
+    eng.close()
 
-    # FP64 legs (same workload): the parity mode (every sum in the reference's order) and the
-    # tiled FP64 kernels
-    k64 = max(5, args.steps // 5)
-    eng64, sid64 = make_engine(c, capi.FP64)
-    total64, _, lat64 = time_steps(eng64, sid64, k64, 2)
-    eng64.close()
-    engt, sidt = make_engine(c, capi.FP64_TILED)
-    total64t, _, lat64t = time_steps(engt, sidt, k64, 2)
-    engt.close()
+    engp, idsp = make_engine(specs, capi.FP32, graphs=True)
+    plain_ms, plain_lat = time_plain(engp, idsp, args.plain_steps, 1)
+    engp.close()
+    value_plain = E * args.plain_steps / (plain_ms / 1e3)
 
-    us, bytes_alg, tiling = apply_roofline(c, capi.FP32, args.dforce_reps)
-    achieved = bytes_alg / (us * 1e-6) / 1e9
-    us64, bytes64, _ = apply_roofline(c, capi.FP64_TILED, max(10, args.dforce_reps // 4))
+    err = None if args.no_error else position_error(specs, capi.FP32, args.error_steps)
     probe = copy_probe()
 
-    err = None if args.no_error else position_error(c, args.error_steps)
-    cpu = None if args.no_cpu_baseline else cpu_baseline(c, args.cpu_steps)
+    # ---- secondary: the single 1.31M-tet beam
+    cfg1 = None
+    if not args.no_cfg1:
+        c1 = workloads.cfg1()
+        e1, i1 = make_engine([c1], capi.FP32, graphs=True)
+        t1, l1 = time_merged(e1, i1, 20, 3)
+        r1 = apply_roofline(e1, args.apply_reps, 4, 4, 18, len(c1["tets"]), len(c1["pos"]))
+        e1.close()
+        e64, i64 = make_engine([c1], capi.FP64, graphs=True)
+        t64, _ = time_merged(e64, i64, 5, 1)
+        e64.close()
+        cfg1 = {"workload": "beam 33x33x257 nodes, 1x1x8 m: 1,310,720 tets, 279,873 nodes, CG 50, dt 0.005",
+                "fp32": {"value": len(c1["tets"]) * 20 / (t1 / 1e3), "ms_per_step": t1 / 20,
+                         "p50_ms": float(np.percentile(l1, 50)), "p99_ms": float(np.percentile(l1, 99)),
+                         "apply_us": r1["us_per_launch"], "apply_frac": r1["frac"]},
+                "fp64_parity": {"value": len(c1["tets"]) * 5 / (t64 / 1e3), "ms_per_step": t64 / 5}}
+
+    cpu = None if args.no_cpu_baseline else cpu_baseline(specs, args.cpu_steps, args.cpu_stride)
 
     out = {
-        "metric": "FEM element-steps/s (implicit corotational FEM step: TetrahedronFEMForceField large + CG + implicit Euler)",
+        "metric": "FEM element-steps/s, batched vs plain, at <=1e-6 relative position error (implicit corotational FEM step)",
         "value": value,
         "unit": "element-steps/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
+        "ms_per_step": total_ms / args.steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": "cantilever beam 33x33x257 nodes, 1x1x8 m, 5-tet split: 1,310,720 tets, "
-                               "279,873 nodes; CG 50 iterations, dt 0.005, E 1e5, nu 0.45; plain one-instance path",
-                   "tets": E, "nodes": n, "cg_iterations": c["iters"], "path": "plain", "precision": "fp32"},
+        "config": {"workload": "256 independent cantilever beams (seed 3): nodes log-uniform in [500, 50000], "
+                               "nx=ny in {3..9}, 1x1x4 m, 5-tet split, E log-uniform [1e4, 1e6], nu U[0.3, 0.49], "
+                               "CG 25, dt 0.01; batched: one launch per phase for all beams, hipGraph step replay",
+                   "scenes": len(specs), "tets": E, "nodes": N, "precision": "fp32 element/state, fp64 reductions"},
         "p50_ms": float(np.percentile(lat, 50)),
         "p99_ms": float(np.percentile(lat, 99)),
-        "wall_s_timed": wall,
-        "cg_last": {"iterations": st["iterations"], "end_reason": capi.END_REASONS.get(st["end_reason"])},
-        "fp64_parity_mode": {"value": E * k64 / (total64 / 1e3), "ms_per_step": total64 / k64,
-                             "p50_ms": float(np.percentile(lat64, 50)), "p99_ms": float(np.percentile(lat64, 99))},
-        "fp64_tiled": {"value": E * k64 / (total64t / 1e3), "ms_per_step": total64t / k64,
-                       "p50_ms": float(np.percentile(lat64t, 50)), "p99_ms": float(np.percentile(lat64t, 99)),
-                       "apply_us": us64, "apply_bytes": bytes64,
-                       "apply_frac": bytes64 / (us64 * 1e-6) / 1e9 / HBM_PEAK_GBS},
-        "tiling": tiling,
+        "plain": {"value": value_plain, "ms_per_step_all_scenes": plain_ms / args.plain_steps,
+                  "p50_ms_per_scene_step": float(np.percentile(plain_lat, 50)),
+                  "p99_ms_per_scene_step": float(np.percentile(plain_lat, 99)), "steps": args.plain_steps,
+                  "batched_over_plain": value / value_plain},
         "error_vs_fp64": err,
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_tile_apply<float> (CG operator: addDForce + mass + projection + p update)",
-                     "us_per_launch": us,
-                     "algorithmic_bytes": bytes_alg},
+        "roofline": roof,
+        "tiling": tiling,
         "copy_probe": probe,
+        "cfg1_single_beam": cfg1,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -143,8 +143,9 @@ int sofa_amd_fem_add_dforce(sofa_amd_engine eng, int scene, const void* dx_dev, 
 
 /* ---- measurement and introspection (no reference counterpart) ---------------------------- */
 
-/* `reps` launches of the scene's CG operator kernel (tiled engines: the tile apply kernel with
- * the mid-loop data flow p = r + beta p, q = A p) on the current state, for roofline timing
+/* `reps` launches of the CG operator kernel of one scene (scene >= 0) or of every scene of the
+ * engine in one merged launch (scene = -1) -- tiled engines: the tile apply kernel with the
+ * mid-loop data flow p = r + beta p, q = A p -- on the current state, for roofline timing
  * with events on `stream`.  Leaves p, q and the CG scalars of the scene undefined: step the
  * scene afresh (or discard the engine) afterwards. */
 int sofa_amd_engine_bench_apply(sofa_amd_engine eng, int scene, int reps, void* stream);

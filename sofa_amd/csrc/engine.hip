@@ -17,6 +17,7 @@
 #include <cstdio>
 #include <cstring>
 #include <limits>
+#include <map>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -211,17 +212,27 @@ struct EngineT : EngineBase {
     VT *sp0 = nullptr, *sp1 = nullptr;
     double *pbb = nullptr, *pden = nullptr, *prho = nullptr;
     // launch-set cache
-    std::vector<int32_t> set_ids;
-    void *set_a = nullptr, *set_b = nullptr, *set_c = nullptr;
-    int* set_scenes = nullptr;
-    int set_na = 0, set_nb = 0, set_nc = 0, set_maxit = 0;
-    hipGraphExec_t gexec = nullptr;
+    // launch sets: the block lists (a: element blocks / tiles, b: node blocks / finalize blocks,
+    // c: tiled node blocks) and the captured step graph of each set of scenes stepped together
+    struct LaunchSet {
+        void *a = nullptr, *b = nullptr, *c = nullptr;
+        int* scenes = nullptr;
+        int na = 0, nb = 0, nc = 0, maxit = 0, n = 0;
+        hipGraphExec_t gexec = nullptr;
+        ~LaunchSet() {
+            if (gexec) (void)hipGraphExecDestroy(gexec);
+            for (void* ptr : {a, b, c, (void*)scenes})
+                if (ptr) (void)hipFree(ptr);
+        }
+    };
+    std::map<std::vector<int32_t>, std::unique_ptr<LaunchSet>> sets;
+    LaunchSet* cur = nullptr;
 
     ~EngineT() override {
-        drop_graph();
+        sets.clear();
         void* ptrs[] = {x, v, f, b, r, p, q, xcg, mass, fixed, inc_ptr, inc, tet, rot, rest, kmat, ef, ef2,
                         inst, partial, tiles_d, ranges_d, ltet, ln_gid, ln_inc_ptr, ln_pslot, sh_gid, sh_ptr,
-                        ln_inc, sp0, sp1, pbb, pden, prho, set_a, set_b, set_c, set_scenes};
+                        ln_inc, sp0, sp1, pbb, pden, prho};
         for (void* ptr : ptrs)
             if (ptr) (void)hipFree(ptr);
     }
@@ -454,17 +465,19 @@ struct EngineT : EngineBase {
         dst.insert(dst.end(), src.begin() + b0, src.begin() + b0 + n);
     }
     template <typename T>
-    void put(void*& buf, const std::vector<T>& v) {
-        if (buf) HIPCHK(hipFree(buf));
+    static void put(void*& buf, const std::vector<T>& v) {
         buf = dalloc<T>(v.size());
         if (!v.empty()) HIPCHK(hipMemcpy(buf, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
     }
     void prepare_set(const int32_t* ids, int n, hipStream_t s) {
         std::vector<int32_t> want(ids, ids + n);
-        if (want == set_ids && set_scenes) return;
-        HIPCHK(hipStreamSynchronize(s));   // the previous lists may still be in use
-        HIPCHK(hipDeviceSynchronize());
-        drop_graph();
+        auto it = sets.find(want);
+        if (it != sets.end()) { cur = it->second.get(); return; }
+        if (sets.size() >= 4096) {       // bounded cache: drop every set (nothing is in flight after the sync)
+            HIPCHK(hipDeviceSynchronize());
+            sets.clear();
+        }
+        std::unique_ptr<LaunchSet> ls(new LaunchSet());
         int maxit = 0;
         if (!tiled) {
             std::vector<Blk> eb, nb;
@@ -474,8 +487,8 @@ struct EngineT : EngineBase {
                 push_range(nb, nblk_h, sc.nblk_begin, sc.nblk_count);
                 maxit = std::max(maxit, sc.cg_iter);
             }
-            put(set_a, eb); put(set_b, nb);
-            set_na = (int)eb.size(); set_nb = (int)nb.size(); set_nc = 0;
+            put(ls->a, eb); put(ls->b, nb);
+            ls->na = (int)eb.size(); ls->nb = (int)nb.size(); ls->nc = 0;
         } else {
             std::vector<TileDesc> ta;
             std::vector<FinBlk> fa;
@@ -486,15 +499,17 @@ struct EngineT : EngineBase {
                 push_range(na, nb_h, nbk_begin[id], nbk_count[id]);
                 maxit = std::max(maxit, scenes[id].cg_iter);
             }
-            put(set_a, ta); put(set_b, fa); put(set_c, na);
-            set_na = (int)ta.size(); set_nb = (int)fa.size(); set_nc = (int)na.size();
+            put(ls->a, ta); put(ls->b, fa); put(ls->c, na);
+            ls->na = (int)ta.size(); ls->nb = (int)fa.size(); ls->nc = (int)na.size();
         }
-        std::vector<int> sc(want.begin(), want.end());
-        void* tmp = set_scenes;
-        put(tmp, sc);
-        set_scenes = (int*)tmp;
-        set_ids = want;
-        set_maxit = maxit;
+        void* sc = nullptr;
+        put(sc, std::vector<int>(want.begin(), want.end()));
+        ls->scenes = (int*)sc;
+        ls->maxit = maxit;
+        ls->n = n;
+        cur = ls.get();
+        sets[want] = std::move(ls);
+        (void)s;
     }
 
     // one implicit Euler step of the current launch set, enqueued on `s`
@@ -502,24 +517,24 @@ struct EngineT : EngineBase {
         if (tiled) { launch_step_tiled(n, s); return; }
         const ElemView<R, NR> ev = view();
         const NodeArrays na = nodes();
-        const Blk* eb = (const Blk*)set_a;
-        const Blk* nb = (const Blk*)set_b;
-        if (set_na > 0) k_rhs_elem<R, NR, VT><<<set_na, kBlock, 0, s>>>(ev, eb, inst, x, v, ef, ef2);
-        if (set_nb > 0) k_rhs_node<VT><<<set_nb, kBlock, 0, s>>>(nb, inst, na, ef, ef2, v, f, b, r, xcg, partial);
-        for (int it = 0; it < set_maxit; it++) {
-            if (set_na > 0) k_cg_elem<R, NR, VT><<<set_na, kBlock, 0, s>>>(ev, eb, inst, r, p, ef);
-            if (set_nb > 0) {
-                k_cg_node<VT><<<set_nb, kBlock, 0, s>>>(nb, inst, na, ef, r, p, q, partial);
-                k_cg_update<VT><<<set_nb, kBlock, 0, s>>>(nb, inst, p, q, xcg, r, partial);
+        const Blk* eb = (const Blk*)cur->a;
+        const Blk* nb = (const Blk*)cur->b;
+        if (cur->na > 0) k_rhs_elem<R, NR, VT><<<cur->na, kBlock, 0, s>>>(ev, eb, inst, x, v, ef, ef2);
+        if (cur->nb > 0) k_rhs_node<VT><<<cur->nb, kBlock, 0, s>>>(nb, inst, na, ef, ef2, v, f, b, r, xcg, partial);
+        for (int it = 0; it < cur->maxit; it++) {
+            if (cur->na > 0) k_cg_elem<R, NR, VT><<<cur->na, kBlock, 0, s>>>(ev, eb, inst, r, p, ef);
+            if (cur->nb > 0) {
+                k_cg_node<VT><<<cur->nb, kBlock, 0, s>>>(nb, inst, na, ef, r, p, q, partial);
+                k_cg_update<VT><<<cur->nb, kBlock, 0, s>>>(nb, inst, p, q, xcg, r, partial);
             }
         }
-        if (set_nb > 0) k_euler_update<VT><<<set_nb, kBlock, 0, s>>>(nb, inst, xcg, v, x);
-        k_finalize<<<(n + 63) / 64, 64, 0, s>>>(set_scenes, n, inst);
+        if (cur->nb > 0) k_euler_update<VT><<<cur->nb, kBlock, 0, s>>>(nb, inst, xcg, v, x);
+        k_finalize<<<(n + 63) / 64, 64, 0, s>>>(cur->scenes, n, inst);
     }
 
     void launch_step_tiled(int n, hipStream_t s) {
         TileArgs<R, NR> ta;
-        ta.tiles = (const TileDesc*)set_a;
+        ta.tiles = (const TileDesc*)cur->a;
         ta.inst = inst;
         ta.ranges = ranges_d;
         ta.ev = view();
@@ -527,26 +542,28 @@ struct EngineT : EngineBase {
         ta.tn = TileNodes{ln_gid, ln_inc_ptr, ln_inc, ln_pslot};
         ta.mass = mass;
         ta.fixed = fixed;
-        const FinBlk* fb = (const FinBlk*)set_b;
-        const NodeBlk* nb = (const NodeBlk*)set_c;
+        const FinBlk* fb = (const FinBlk*)cur->b;
+        const NodeBlk* nb = (const NodeBlk*)cur->c;
         const SharedNodes sn{sh_gid, sh_ptr};
-        if (set_na > 0) k_tile_rhs<R, NR, VT><<<set_na, kBlock, 0, s>>>(ta, x, v, f, b, r, xcg, sp0, sp1, pbb);
-        if (set_nb > 0) k_fin_rhs<VT><<<set_nb, kBlock, 0, s>>>(fb, inst, ranges_d, sn, mass, fixed, v, sp0, sp1, f, b, r, xcg, pbb);
-        for (int it = 1; it <= set_maxit; it++) {
-            if (set_na > 0)
-                k_tile_apply<R, NR, VT><<<set_na, kBlock, 0, s>>>(ta, it, r, p, q, sp0, it == 1 ? pbb : prho, pden);
-            if (set_nb > 0) k_fin_apply<VT><<<set_nb, kBlock, 0, s>>>(fb, inst, ranges_d, sn, mass, fixed, it, r, p, q, sp0, pden);
-            if (set_nc > 0) k_update<VT><<<set_nc, kBlock, 0, s>>>(nb, inst, ranges_d, it, p, q, xcg, r, pden, prho);
+        if (cur->na > 0) k_tile_rhs<R, NR, VT><<<cur->na, kBlock, 0, s>>>(ta, x, v, f, b, r, xcg, sp0, sp1, pbb);
+        if (cur->nb > 0) k_fin_rhs<VT><<<cur->nb, kBlock, 0, s>>>(fb, inst, ranges_d, sn, mass, fixed, v, sp0, sp1, f, b, r, xcg, pbb);
+        for (int it = 1; it <= cur->maxit; it++) {
+            if (cur->na > 0)
+                k_tile_apply<R, NR, VT><<<cur->na, kBlock, 0, s>>>(ta, it, r, p, q, sp0, it == 1 ? pbb : prho, pden);
+            if (cur->nb > 0) k_fin_apply<VT><<<cur->nb, kBlock, 0, s>>>(fb, inst, ranges_d, sn, mass, fixed, it, r, p, q, sp0, pden);
+            if (cur->nc > 0) k_update<VT><<<cur->nc, kBlock, 0, s>>>(nb, inst, ranges_d, it, p, q, xcg, r, pden, prho);
         }
-        if (set_nc > 0) k_integrate<VT><<<set_nc, kBlock, 0, s>>>(nb, inst, xcg, v, x);
-        k_step_end_tiled<<<(n + 63) / 64, 64, 0, s>>>(set_scenes, n, inst);
+        if (cur->nc > 0) k_integrate<VT><<<cur->nc, kBlock, 0, s>>>(nb, inst, xcg, v, x);
+        k_step_end_tiled<<<(n + 63) / 64, 64, 0, s>>>(cur->scenes, n, inst);
     }
 
-    void drop_graph() {
-        if (gexec) (void)hipGraphExecDestroy(gexec);
-        gexec = nullptr;
+    void reset_graph() override {
+        for (auto& kv : sets)
+            if (kv.second->gexec) {
+                (void)hipGraphExecDestroy(kv.second->gexec);
+                kv.second->gexec = nullptr;
+            }
     }
-    void reset_graph() override { drop_graph(); }
 
     void step(const int32_t* ids, int n, int nsteps, hipStream_t s) override {
         prepare_set(ids, n, s);
@@ -557,6 +574,7 @@ struct EngineT : EngineBase {
         }
         // the whole step (RHS, every CG iteration, update) is captured once per launch set and
         // replayed: one graph launch per step instead of 3 n_cg + 5 kernel launches
+        hipGraphExec_t& gexec = cur->gexec;
         if (!gexec && nsteps > 0) {
             hipGraph_t graph = nullptr;
             HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
@@ -576,10 +594,12 @@ struct EngineT : EngineBase {
     // the iteration-2 data flow: p = r + beta p, q = A p) on the current state
     void bench_apply(int scene, int reps, hipStream_t s) override {
         if (!tiled) throw std::invalid_argument("bench_apply needs a tiled engine");
-        const int32_t id = scene;
-        prepare_set(&id, 1, s);
+        std::vector<int32_t> ids;
+        if (scene >= 0) ids.push_back(scene);
+        else for (int i = 0; i < (int)scenes.size(); i++) ids.push_back(i);
+        prepare_set(ids.data(), (int)ids.size(), s);
         TileArgs<R, NR> ta;
-        ta.tiles = (const TileDesc*)set_a;
+        ta.tiles = (const TileDesc*)cur->a;
         ta.inst = inst;
         ta.ranges = ranges_d;
         ta.ev = view();
@@ -588,7 +608,7 @@ struct EngineT : EngineBase {
         ta.mass = mass;
         ta.fixed = fixed;
         for (int i = 0; i < reps; i++)
-            if (set_na > 0) k_tile_apply<R, NR, VT><<<set_na, kBlock, 0, s>>>(ta, 2, r, p, q, sp0, prho, pden);
+            if (cur->na > 0) k_tile_apply<R, NR, VT><<<cur->na, kBlock, 0, s>>>(ta, 2, r, p, q, sp0, prho, pden);
         HIPCHK(hipGetLastError());
     }
     // tiles, tile-local nodes, nodes finished by the finalize kernel, shared partial slots,
@@ -969,7 +989,7 @@ int sofa_amd_fem_add_dforce(sofa_amd_engine eng, int scene, const void* dx_dev, 
 
 int sofa_amd_engine_bench_apply(sofa_amd_engine eng, int scene, int reps, void* stream) {
     GUARD_BEGIN
-    if (!scene_of(eng, scene) || reps < 0) return fail(SOFA_AMD_EINVAL, "invalid argument");
+    if (!eng || !eng->impl || (scene >= 0 && !scene_of(eng, scene)) || reps < 0) return fail(SOFA_AMD_EINVAL, "invalid argument");
     if (!eng->impl->committed) return fail(SOFA_AMD_ESTATE, "engine not committed");
     HIPCHK(hipSetDevice(eng->impl->device));
     eng->impl->bench_apply(scene, reps, stream ? (hipStream_t)stream : eng->impl->stream);
