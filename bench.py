@@ -8,8 +8,13 @@ workload.  Headline workload (`value`): the batched job mix of 256 independent c
 iterations, dt 0.01), FP32 element arithmetic with FP64 reductions, all 256 beams advanced in
 one launch per phase with the step replayed from a hipGraph.  Beside it, on the same job mix:
 the plain path (one instance at a time, its own launches per phase), and the position error
-against the FP64 path after 100 steps.  Secondary workload: the single 1.31M-tet beam
-(33x33x257 nodes, CG 50, dt 0.005).
+against the FP64 path after 100 steps.  Secondary workloads: the single 1.31M-tet beam
+(33x33x257 nodes, CG 50, dt 0.005), FP32 and the FP64 parity path; the 1.34M-tet ellipsoid
+(per-element materials, consistent MeshMatrixMass, oscillator-driven face) in the MIXED mode
+(FP32 inner CG of 30 iterations in at most 3 FP64 refinement sweeps) beside the FP64 tiled
+path with the same tolerance, both against FP64 run to convergence; the served job stream
+(Poisson arrivals of beam and ellipsoid jobs, continuous batching in a live engine, admission by
+an HBM budget) batched vs plain over a bounded window.
 
 Prints ONE JSON line.  Inputs are resident in HBM before any timed region; timing uses HIP
 events on the stream the work is launched on.
@@ -161,6 +166,58 @@ def position_error(specs, precision, steps):
             "eps_disp_max": float(max(ed)), "scenes": len(specs)}
 
 
+def cfg2_mixed(steps, err_steps):
+    """Ellipsoid workload: MIXED vs FP64 tiled (CG 90 / 1e-8) throughput, and both trajectories
+    against FP64 tiled run to |r|/|b| <= 1e-11 (400 iterations), after `err_steps` steps."""
+    from sofa_amd import capi, workloads
+    c = workloads.cfg2()
+    E, N = len(c["tets"]), len(c["pos"])
+    out = {"workload": f"ellipsoid 0.15x0.10x0.08 m, jittered-grid Delaunay h=2.93 mm: {E} tets, {N} nodes, "
+                       "E 10^U(3,5), nu U(0.40,0.49) per element, MeshMatrixMass 1050 consistent, x<-0.13 fixed, "
+                       "x>0.13 oscillating 5 mm at 1 Hz, dt 0.01, CG tolerance 1e-8"}
+    xs = {}
+    for name, prec, spec in (("mixed", capi.MIXED, c), ("fp64_tiled", capi.FP64_TILED, c),
+                             ("fp64_converged", capi.FP64_TILED, dict(c, iters=400, tol=1e-11))):
+        eng, ids = make_engine([spec], prec, graphs=True)
+        if name != "fp64_converged":
+            t, lat = time_merged(eng, ids, steps, 2)
+            st = eng.stats(ids[0])
+            out[name] = {"value": E * steps / (t / 1e3), "ms_per_step": t / steps,
+                         "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+                         "last_solve": {"iterations": st["iterations"], "ir_sweeps": st["ir_sweeps"],
+                                        "err": st["err"], "end": capi.END_REASONS.get(st["end_reason"])}}
+        eng.close()
+        eng, ids = make_engine([spec], prec, graphs=True)
+        eng.step(ids, err_steps)
+        xs[name] = eng.read(ids[0])[0]
+        eng.close()
+    ref = xs["fp64_converged"]
+    for name in ("mixed", "fp64_tiled"):
+        d = np.linalg.norm(xs[name] - ref, axis=1).max()
+        out[name]["eps_rel_vs_converged"] = float(d / np.linalg.norm(ref, axis=1).max())
+        out[name]["eps_disp_vs_converged"] = float(d / np.linalg.norm(ref - c["pos"], axis=1).max())
+    out["error_steps"] = err_steps
+    return out
+
+
+def serve_cfg4(seconds, arrivals, budget_gb):
+    """cfg4 job stream (200 jobs/s Poisson, seed 4) over the first `arrivals` seconds of arrivals,
+    served for `seconds` of wall time, batched and plain, FP32."""
+    from sofa_amd import capi, serving, workloads
+    jobs = workloads.cfg4_jobs(rate=200.0, duration=arrivals, seed=4)
+    out = {"workload": f"cfg4 job stream: Poisson 200 jobs/s for {arrivals:g} s ({len(jobs)} jobs), beams and voxel "
+                       "ellipsoids, nodes log-uniform [1e3, 2e5], steps U{50..500}, oscillating BC; "
+                       f"{seconds:g} s served per mode, HBM budget {budget_gb:g} GB, FP32, eager launches"}
+    for batched in (True, False):
+        srv = serving.Server(capi.FP32, budget_bytes=int(budget_gb * (1 << 30)), batched=batched)
+        res = srv.run(jobs, wall_limit=seconds, stream=bench_stream(),
+                      log=lambda m: print(m, file=sys.stderr, flush=True))
+        srv.close()
+        out["batched" if batched else "plain"] = res
+    out["served_over_plain"] = out["batched"]["value"] / out["plain"]["value"]
+    return out
+
+
 def cpu_baseline(specs, steps, stride):
     """The FP64 oracle (C++ restatement of the reference path, one thread) on this host, over a
     bounded sample: every `stride`-th beam of the mix, `steps` step(s) each."""
@@ -197,6 +254,13 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-error", action="store_true")
     ap.add_argument("--no-cfg1", action="store_true")
+    ap.add_argument("--no-cfg2", action="store_true")
+    ap.add_argument("--cfg2-steps", type=int, default=20)
+    ap.add_argument("--cfg2-error-steps", type=int, default=30)
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-seconds", type=float, default=15.0)
+    ap.add_argument("--serve-arrivals", type=float, default=5.0)
+    ap.add_argument("--serve-budget-gb", type=float, default=8.0)
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -250,6 +314,9 @@ def main():
                          "apply_us": r1["us_per_launch"], "apply_frac": r1["frac"]},
                 "fp64_parity": {"value": len(c1["tets"]) * 5 / (t64 / 1e3), "ms_per_step": t64 / 5}}
 
+    cfg2 = None if args.no_cfg2 else cfg2_mixed(args.cfg2_steps, args.cfg2_error_steps)
+    served = None if args.no_serve else serve_cfg4(args.serve_seconds, args.serve_arrivals, args.serve_budget_gb)
+
     cpu = None if args.no_cpu_baseline else cpu_baseline(specs, args.cpu_steps, args.cpu_stride)
 
     out = {
@@ -280,6 +347,8 @@ def main():
         "tiling": tiling,
         "copy_probe": probe,
         "cfg1_single_beam": cfg1,
+        "cfg2_ellipsoid_mixed": cfg2,
+        "cfg4_served": served,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -38,8 +38,12 @@ extern "C" {
                                     are summed tile by tile (deterministic, not reference order) */
 
 /* mass kinds */
-#define SOFA_AMD_MASS_UNIFORM   0   /* UniformMass (UniformMass.inl:403-496)                 */
-#define SOFA_AMD_MASS_DIAGONAL  1   /* DiagonalMass on tetrahedra (DiagonalMass.inl:1060-1080)*/
+#define SOFA_AMD_MASS_UNIFORM            0  /* UniformMass (UniformMass.inl:403-496)                  */
+#define SOFA_AMD_MASS_DIAGONAL           1  /* DiagonalMass on tetrahedra (DiagonalMass.inl:1060-1080) */
+#define SOFA_AMD_MASS_MESHMATRIX         2  /* MeshMatrixMass, lumping = false: consistent mass with
+                                               vertex and edge terms (MeshMatrixMass.inl:547-657,
+                                               1987-2092)                                          */
+#define SOFA_AMD_MASS_MESHMATRIX_LUMPED  3  /* MeshMatrixMass, lumping = true (coefficient 2.5)      */
 
 typedef struct sofa_amd_engine_s* sofa_amd_engine;
 
@@ -83,9 +87,21 @@ int sofa_amd_scene_create(sofa_amd_engine eng, int n_nodes, const double* positi
 int sofa_amd_scene_set_fem(sofa_amd_engine eng, int scene, int n_tets, const int32_t* tets,
                            const double* young, int n_young, const double* poisson, int n_poisson);
 
-/* UniformMass vertexMass (UniformMass.inl:403-496) or DiagonalMass massDensity on the FEM's
- * tetrahedra (DiagonalMass.inl:1060-1080, 536-559, 1393-1413). */
+/* UniformMass vertexMass (UniformMass.inl:403-496), DiagonalMass massDensity on the FEM's
+ * tetrahedra (DiagonalMass.inl:1060-1080, 536-559, 1393-1413) or a uniform MeshMatrixMass
+ * massDensity. */
 int sofa_amd_scene_set_mass(sofa_amd_engine eng, int scene, int kind, double value);
+
+/* MeshMatrixMass massDensity, one value or one per tetrahedron (MeshMatrixMass.h:88). */
+int sofa_amd_scene_set_mass_density(sofa_amd_engine eng, int scene, int kind, const double* density, int n);
+
+/* OscillatorProjectiveConstraint entries (OscillatorProjectiveConstraint.inl:65-106): node index,
+ * mean (3), amplitude (3), pulsation, phase.  The node's response is cleared in the solve and,
+ * after it, x = mean + amplitude sin(w t + phase), v = amplitude w cos(w t + phase) with t the
+ * scene time at the start of the step. */
+int sofa_amd_scene_add_oscillators(sofa_amd_engine eng, int scene, int n, const int32_t* indices,
+                                   const double* mean, const double* amplitude, const double* pulsation,
+                                   const double* phase);
 
 /* Context gravity (Node::setGravity), used by the masses' addForce. */
 int sofa_amd_scene_set_gravity(sofa_amd_engine eng, int scene, double gx, double gy, double gz);
@@ -99,6 +115,16 @@ int sofa_amd_scene_set_solver(sofa_amd_engine eng, int scene, double dt, double 
                               double rayleigh_mass, int cg_iterations, double cg_tolerance,
                               double cg_threshold);
 
+/* SOFA_AMD_MIXED engines: the linear solve of each step is iterative refinement -- r = b - A x
+ * in FP64, stop when |r| / |b| <= cg_tolerance (the CGLinearSolver tolerance of
+ * sofa_amd_scene_set_solver) or after `max_sweeps` corrections, each correction an FP32 CG
+ * solve of A d = r of at most `inner_iterations` iterations stopping at relative residual
+ * `inner_tolerance`.  Defaults 30, 1e-6, 3.  No reference counterpart (SOFA's CG runs in the
+ * state's precision); it replaces CGLinearSolver::solve (CGLinearSolver.inl:71-315) in this mode.
+ * sofa_amd_cg_stats reports the inner iterations summed over the sweeps and the sweep count. */
+int sofa_amd_scene_set_refinement(sofa_amd_engine eng, int scene, int inner_iterations,
+                                  double inner_tolerance, int max_sweeps);
+
 /* Uploads every scene created so far; after this the scenes can be stepped. */
 int sofa_amd_engine_commit(sofa_amd_engine eng);
 
@@ -107,6 +133,24 @@ int sofa_amd_engine_commit(sofa_amd_engine eng);
  * launch per phase.  Asynchronous on `stream` (NULL = the engine's stream). */
 int sofa_amd_engine_step(sofa_amd_engine eng, const int32_t* scenes, int n_scenes, int n_steps,
                          void* stream);
+
+/* ---- live scenes (tiled engines: FP32, FP64_TILED, MIXED) -------------------------------- */
+
+/* Instead of sofa_amd_engine_commit: reserve device arenas for up to `max_nodes` nodes,
+ * `max_tets` tetrahedra and `max_scenes` scenes in total, after which scenes are created,
+ * committed and released one at a time while the others keep running (the serving loop's
+ * continuous batching; admission is by the arenas' room).  No reference counterpart: SOFA
+ * builds one scene graph per process. */
+int sofa_amd_engine_reserve(sofa_amd_engine eng, int64_t max_nodes, int64_t max_tets, int max_scenes);
+/* Places one created scene in the reserved arenas; SOFA_AMD_ENOMEM (and nothing kept) if they
+ * have no room for it now. */
+int sofa_amd_scene_commit(sofa_amd_engine eng, int scene);
+/* Waits for the device, then returns the scene's ranges to the arenas; its id stays invalid. */
+int sofa_amd_scene_release(sofa_amd_engine eng, int scene);
+/* Arena use: out22[2k] used, out22[2k+1] capacity, k = nodes, elements, incidences, tiles,
+ * tile-local nodes, shared nodes, shared partial slots, shared-node blocks, node blocks,
+ * oscillator entries, scene records. */
+int sofa_amd_engine_usage(sofa_amd_engine eng, int64_t* out22);
 
 /* Replay each step from a captured hipGraph (one graph launch per step; the graph of a launch
  * set is captured on its first step and kept until the set changes).  Off by default.  No

@@ -17,6 +17,7 @@
 //   sofa/type/Vec.h:483-580 (norm2 / normalize), :775-780 (cross); Mat.h:577-587 (Mat*Vec)
 //   sofa/geometry/Tetrahedron.h:55-83 (volume = |signed volume|)
 //   UniformMass.inl:329-345,403-419,471-496; DiagonalMass.inl:536-559,1060-1080,1393-1413
+//   MeshMatrixMass.inl:547-657,1387-1494,1987-2092; TetrahedronSetTopologyContainer.cpp:105-147
 //   FixedProjectiveConstraint.inl:184-207
 //   OscillatorProjectiveConstraint.inl:65-106
 //   CGLinearSolver.inl:71-315, CGLinearSolver.cpp:40-64 (GraphScattered cgstep_beta/alpha)
@@ -43,6 +44,7 @@
 #include <array>
 #include <memory>
 #include <algorithm>
+#include <map>
 
 namespace oracle {
 
@@ -497,6 +499,77 @@ struct DiagonalMass : public MassBase {
     }
 };
 
+// MeshMatrixMass<Vec3d> on a tetrahedral topology (MeshMatrixMass.inl): vertex mass rho V / 10
+// per tet vertex (:547-600), edge mass rho V / 20 per tet edge (:603-650), edges enumerated as
+// TetrahedronSetTopologyContainer::createEdgeSetArray does (TetrahedronSetTopologyContainer.cpp
+// :105-147; local edges {0,1},{0,2},{0,3},{1,2},{1,3},{2,3}, Topology.cpp:44), lumping
+// coefficient 2.5 for tetrahedra (:1476-1489)
+struct MeshMatrixMass : public MassBase {
+    bool lumping = false;
+    Real lumpCoeff = 2.5;
+    std::vector<Real> vertexMass, edgeMass;
+    std::vector<std::array<int, 2>> edges;
+    void init(const std::vector<Vec3>& pos, const std::vector<Tetra>& tets, const std::vector<Real>& density, bool lump) {
+        lumping = lump;
+        static const int le[6][2] = {{0, 1}, {0, 2}, {0, 3}, {1, 2}, {1, 3}, {2, 3}};
+        std::map<std::pair<int, int>, int> emap;
+        std::vector<std::array<int, 6>> edgesInTet(tets.size());
+        edges.clear();
+        for (size_t i = 0; i < tets.size(); i++)
+            for (int j = 0; j < 6; j++) {
+                int v1 = tets[i][le[j][0]], v2 = tets[i][le[j][1]];
+                std::pair<int, int> e = v1 < v2 ? std::make_pair(v1, v2) : std::make_pair(v2, v1);
+                auto it = emap.find(e);
+                if (it == emap.end()) {
+                    const int idx = (int)emap.size();
+                    emap[e] = idx;
+                    edges.push_back({e.first, e.second});
+                    edgesInTet[i][j] = idx;
+                } else {
+                    edgesInTet[i][j] = it->second;
+                }
+            }
+        vertexMass.assign(pos.size(), 0.0);
+        edgeMass.assign(edges.size(), 0.0);
+        auto dens = [&](size_t i) { return density.size() > i ? density[i] : density[0]; };
+        if (!lumping) {
+            for (size_t i = 0; i < tets.size(); i++) {
+                const Tetra& t = tets[i];
+                const Real vol = tet_volume(pos[t[0]], pos[t[1]], pos[t[2]], pos[t[3]]);
+                const Real mass = (dens(i) * vol) / Real(20.0);
+                for (int j = 0; j < 6; j++) edgeMass[edgesInTet[i][j]] += mass;
+            }
+        }
+        for (size_t i = 0; i < tets.size(); i++) {
+            const Tetra& t = tets[i];
+            const Real vol = tet_volume(pos[t[0]], pos[t[1]], pos[t[2]], pos[t[3]]);
+            const Real mass = (dens(i) * vol) / Real(10.0);
+            for (int j = 0; j < 4; j++) vertexMass[t[j]] += mass;
+        }
+    }
+    // :1987-2047
+    void addMDx(std::vector<Vec3>& res, const std::vector<Vec3>& dx, Real factor) override {
+        if (lumping) {
+            for (size_t i = 0; i < dx.size(); i++)
+                for (int k = 0; k < 3; k++) res[i][k] += ((dx[i][k] * vertexMass[i]) * lumpCoeff) * Real(factor);
+        } else {
+            for (size_t i = 0; i < dx.size(); i++)
+                for (int k = 0; k < 3; k++) res[i][k] += (dx[i][k] * vertexMass[i]) * Real(factor);
+            for (size_t j = 0; j < edges.size(); j++) {
+                const Real tempMass = edgeMass[j] * Real(factor);
+                const int e0 = edges[j][0], e1 = edges[j][1];
+                for (int k = 0; k < 3; k++) res[e0][k] += dx[e1][k] * tempMass;
+                for (int k = 0; k < 3; k++) res[e1][k] += dx[e0][k] * tempMass;
+            }
+        }
+    }
+    // :2073-2092 (gravity always with the lumped vertex mass)
+    void addForce(std::vector<Vec3>& f, const std::vector<Vec3>&, const std::vector<Vec3>&) override {
+        for (size_t i = 0; i < f.size(); i++)
+            for (int k = 0; k < 3; k++) f[i][k] += (gravity[k] * vertexMass[i]) * lumpCoeff;
+    }
+};
+
 // Test-only 1-D linear spring toward an anchor (used to pin the Euler recurrence of
 // EulerImplicitSolverDynamic_test.cpp:76-101 with the closed form): f_y = -K (x_y - y0).
 struct AxisSpring : public ForceField {
@@ -900,6 +973,34 @@ void orc_get_masses(void* h, double* out) {
         for (size_t i = 0; i < d->masses.size(); i++) out[i] = d->masses[i];
     } else if (auto* u = dynamic_cast<UniformMass*>(s->mass.get())) {
         for (size_t i = 0; i < s->n(); i++) out[i] = u->vertexMass;
+    } else if (auto* mm = dynamic_cast<MeshMatrixMass*>(s->mass.get())) {
+        for (size_t i = 0; i < mm->vertexMass.size(); i++) out[i] = mm->vertexMass[i];
+    }
+}
+
+void orc_set_meshmatrix_mass(void* h, const double* density, int n_density, int lumping) {
+    Scene* s = static_cast<Scene*>(h);
+    MeshMatrixMass* m = new MeshMatrixMass();
+    m->init(s->x0, s->fem ? s->fem->elems : std::vector<Tetra>(), std::vector<Real>(density, density + n_density),
+            lumping != 0);
+    m->gravity = s->gravity;
+    s->mass.reset(m);
+}
+
+int orc_get_edge_count(void* h) {
+    Scene* s = static_cast<Scene*>(h);
+    auto* m = dynamic_cast<MeshMatrixMass*>(s->mass.get());
+    return m ? (int)m->edges.size() : 0;
+}
+
+void orc_get_edges(void* h, int* edges, double* edge_mass) {
+    Scene* s = static_cast<Scene*>(h);
+    auto* m = dynamic_cast<MeshMatrixMass*>(s->mass.get());
+    if (!m) return;
+    for (size_t j = 0; j < m->edges.size(); j++) {
+        edges[2 * j] = m->edges[j][0];
+        edges[2 * j + 1] = m->edges[j][1];
+        edge_mass[j] = m->edgeMass[j];
     }
 }
 

@@ -53,6 +53,10 @@ def lib():
         L.orc_fem_rest_volume.restype = ctypes.c_double
         L.orc_fem_rest_volume.argtypes = [ctypes.c_void_p]
         L.orc_get_masses.argtypes = [ctypes.c_void_p, _dp]
+        L.orc_set_meshmatrix_mass.argtypes = [ctypes.c_void_p, _dp, ctypes.c_int, ctypes.c_int]
+        L.orc_get_edge_count.restype = ctypes.c_int
+        L.orc_get_edge_count.argtypes = [ctypes.c_void_p]
+        L.orc_get_edges.argtypes = [ctypes.c_void_p, _ip, _dp]
         _lib = L
     return _lib
 
@@ -102,6 +106,17 @@ class OracleScene:
 
     def set_diagonal_mass(self, density):
         lib().orc_set_diagonal_mass(self._h, float(density))
+
+    def set_meshmatrix_mass(self, density, lumping=False):
+        d, dp = _d(np.atleast_1d(density))
+        lib().orc_set_meshmatrix_mass(self._h, dp, len(d), 1 if lumping else 0)
+
+    def edges(self):
+        ne = lib().orc_get_edge_count(self._h)
+        e = np.zeros((ne, 2), dtype=np.int32)
+        m = np.zeros(ne, dtype=np.float64)
+        lib().orc_get_edges(self._h, e.ctypes.data_as(_ip), m.ctypes.data_as(_dp))
+        return e, m
 
     def add_axis_spring(self, node, axis, stiffness, anchor):
         lib().orc_add_axis_spring(self._h, int(node), int(axis), float(stiffness), float(anchor))

@@ -15,7 +15,7 @@ LIB_PATH = os.path.join(HERE, "libsofa_amd.so")
 
 OK, EINVAL, EDEVICE, ENOMEM, EDEGENERATE, ESTATE, ENAN = 0, -1, -2, -3, -4, -5, -6
 FP64, FP32, MIXED, FP64_TILED = 0, 1, 2, 3
-MASS_UNIFORM, MASS_DIAGONAL = 0, 1
+MASS_UNIFORM, MASS_DIAGONAL, MASS_MESHMATRIX, MASS_MESHMATRIX_LUMPED = 0, 1, 2, 3
 END_REASONS = {0: "iterations", 1: "tolerance", 2: "threshold", 3: "den==0", 4: "null norm of vector b"}
 
 # every symbol include/sofa_amd.h declares
@@ -26,7 +26,9 @@ EXPORTS = [
     "sofa_amd_scene_set_solver", "sofa_amd_engine_commit", "sofa_amd_engine_step", "sofa_amd_engine_sync",
     "sofa_amd_scene_read", "sofa_amd_scene_write", "sofa_amd_scene_stats", "sofa_amd_scene_element",
     "sofa_amd_fem_add_force", "sofa_amd_fem_add_dforce", "sofa_amd_engine_bench_apply", "sofa_amd_scene_tiling",
-    "sofa_amd_engine_set_graphs",
+    "sofa_amd_engine_set_graphs", "sofa_amd_scene_set_mass_density", "sofa_amd_scene_add_oscillators",
+    "sofa_amd_scene_set_refinement", "sofa_amd_engine_reserve", "sofa_amd_scene_commit", "sofa_amd_scene_release",
+    "sofa_amd_engine_usage",
 ]
 
 
@@ -83,6 +85,13 @@ def lib():
         "sofa_amd_engine_bench_apply": (c_int, [_vp, c_int, c_int, _vp]),
         "sofa_amd_scene_tiling": (c_int, [_vp, c_int, _ip]),
         "sofa_amd_engine_set_graphs": (c_int, [_vp, c_int]),
+        "sofa_amd_scene_set_mass_density": (c_int, [_vp, c_int, c_int, _dp, c_int]),
+        "sofa_amd_scene_add_oscillators": (c_int, [_vp, c_int, c_int, _ip, _dp, _dp, _dp, _dp]),
+        "sofa_amd_scene_set_refinement": (c_int, [_vp, c_int, c_int, c_double, c_int]),
+        "sofa_amd_engine_reserve": (c_int, [_vp, ctypes.c_int64, ctypes.c_int64, c_int]),
+        "sofa_amd_scene_commit": (c_int, [_vp, c_int]),
+        "sofa_amd_scene_release": (c_int, [_vp, c_int]),
+        "sofa_amd_engine_usage": (c_int, [_vp, ctypes.POINTER(ctypes.c_int64)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -135,7 +144,9 @@ class Engine:
     # ---- scene construction ----
     def add_scene(self, positions, tets, young, poisson, fixed, mass_kind=MASS_UNIFORM, mass_value=1.0,
                   gravity=(0.0, 0.0, 0.0), dt=0.01, rayleigh_stiffness=0.0, rayleigh_mass=0.0,
-                  cg_iterations=25, cg_tolerance=1e-5, cg_threshold=1e-5):
+                  cg_iterations=25, cg_tolerance=1e-5, cg_threshold=1e-5, oscillators=None):
+        """oscillators: None or dict(index, mean (n,3), amplitude (n,3), pulsation (n,), phase (n,));
+        mass_value may be an array of per-element densities for the MeshMatrixMass kinds."""
         L = lib()
         pos, pp = _d(np.asarray(positions).reshape(-1, 3))
         sid = ctypes.c_int32()
@@ -145,7 +156,19 @@ class Engine:
         y, yp = _d(np.atleast_1d(young))
         nu, nup = _d(np.atleast_1d(poisson))
         check(L.sofa_amd_scene_set_fem(self._h, s, len(t) // 4, tp, yp, len(y), nup, len(nu)))
-        check(L.sofa_amd_scene_set_mass(self._h, s, int(mass_kind), float(mass_value)))
+        mv = np.atleast_1d(np.asarray(mass_value, dtype=np.float64))
+        if mv.size > 1:
+            d, dp = _d(mv)
+            check(L.sofa_amd_scene_set_mass_density(self._h, s, int(mass_kind), dp, len(d)))
+        else:
+            check(L.sofa_amd_scene_set_mass(self._h, s, int(mass_kind), float(mv[0])))
+        if oscillators is not None and len(oscillators["index"]):
+            oi, oip = _i(oscillators["index"])
+            om, omp = _d(np.asarray(oscillators["mean"]).reshape(-1))
+            oa, oap = _d(np.asarray(oscillators["amplitude"]).reshape(-1))
+            ow, owp = _d(oscillators["pulsation"])
+            op, opp = _d(oscillators["phase"])
+            check(L.sofa_amd_scene_add_oscillators(self._h, s, len(oi), oip, omp, oap, owp, opp))
         check(L.sofa_amd_scene_set_gravity(self._h, s, *[float(g) for g in gravity]))
         fx, fxp = _i(np.atleast_1d(fixed) if len(np.atleast_1d(fixed)) else np.zeros(0, np.int32))
         check(L.sofa_amd_scene_set_fixed(self._h, s, fxp, len(fx)))
@@ -158,6 +181,29 @@ class Engine:
     def commit(self):
         check(lib().sofa_amd_engine_commit(self._h))
 
+    # ---- live scenes (tiled engines) ----
+    def reserve(self, max_nodes, max_tets, max_scenes):
+        """Arenas for scenes committed and released one at a time (the serving loop)."""
+        check(lib().sofa_amd_engine_reserve(self._h, int(max_nodes), int(max_tets), int(max_scenes)))
+
+    def commit_scene(self, scene):
+        """Places one scene; False if the reservation has no room for it now."""
+        rc = lib().sofa_amd_scene_commit(self._h, int(scene))
+        if rc == ENOMEM:
+            return False
+        check(rc)
+        return True
+
+    def release_scene(self, scene):
+        check(lib().sofa_amd_scene_release(self._h, int(scene)))
+
+    def usage(self):
+        out = (ctypes.c_int64 * 22)()
+        check(lib().sofa_amd_engine_usage(self._h, out))
+        names = ("nodes", "elements", "incidences", "tiles", "tile_local_nodes", "shared_nodes", "shared_slots",
+                 "shared_blocks", "node_blocks", "oscillators", "scenes")
+        return {k: (int(out[2 * i]), int(out[2 * i + 1])) for i, k in enumerate(names)}
+
     # ---- stepping ----
     def step(self, scenes=None, n_steps=1, stream=None):
         ids = np.arange(len(self.n_nodes), dtype=np.int32) if scenes is None else np.asarray(scenes, dtype=np.int32)
@@ -166,6 +212,11 @@ class Engine:
 
     def sync(self):
         check(lib().sofa_amd_engine_sync(self._h))
+
+    def set_refinement(self, scene, inner_iterations=30, inner_tolerance=1e-6, max_sweeps=3):
+        """MIXED engines: FP32 inner CG limits and FP64 refinement sweeps per step."""
+        check(lib().sofa_amd_scene_set_refinement(self._h, int(scene), int(inner_iterations),
+                                                  float(inner_tolerance), int(max_sweeps)))
 
     def set_graphs(self, enable=True):
         check(lib().sofa_amd_engine_set_graphs(self._h, 1 if enable else 0))

@@ -17,15 +17,18 @@
 #include <cstdio>
 #include <cstring>
 #include <limits>
+#include <array>
 #include <map>
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "sofa_amd.h"
 #include "fem_kernels.h"
 #include "fem_tiled.h"
+#include "fem_mixed.h"
 #include "tiling.h"
 
 namespace sofa_amd {
@@ -140,9 +143,18 @@ struct SceneHost {
     double mass_value = 1.0;                // vertexMass or massDensity
     double g[3] = {0, 0, 0};
     std::vector<int32_t> fixed{0};          // FixedProjectiveConstraint default indices = {0}
+    std::vector<double> density{1.0};       // MeshMatrixMass massDensity (one or per element)
+    // OscillatorProjectiveConstraint entries
+    std::vector<int32_t> osc_idx;
+    std::vector<double> osc_mean, osc_amp, osc_w, osc_phase;
     double dt = 0.01, rs = 0, rm = 0;
     int cg_iter = 25;
     double tol = 1e-5, thr = 1e-5;
+    // MIXED: inner FP32 CG iterations / relative tolerance per sweep, sweeps per step at most
+    int inner_iter = 30;
+    double inner_tol = 1e-6;
+    int max_sweeps = 3;
+    bool released = false;                  // live engines: the scene's data has been released
     // placement after commit
     int node_begin = 0, elem_begin = 0;
     int nblk_begin = 0, nblk_count = 0, eblk_begin = 0, eblk_count = 0;
@@ -151,6 +163,60 @@ struct SceneHost {
 // ---------------------------------------------------------------------------------------
 // typed engine
 // ---------------------------------------------------------------------------------------
+// first-fit allocator of index ranges in [0, cap): the index spaces of a tiled engine (nodes,
+// elements, tiles, ...) are arenas, so that scenes can be placed and released while the others
+// keep their device data where it is
+struct RangeAlloc {
+    int64_t cap = 0, used = 0;
+    std::map<int64_t, int64_t> free_;   // begin -> length
+    void reset(int64_t c) {
+        cap = c; used = 0; free_.clear();
+        if (c > 0) free_[0] = c;
+    }
+    int64_t alloc(int64_t n) {          // -1: no room
+        if (n <= 0) return 0;
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second < n) continue;
+            const int64_t b = it->first, len = it->second;
+            free_.erase(it);
+            if (len > n) free_[b + n] = len - n;
+            used += n;
+            return b;
+        }
+        return -1;
+    }
+    void release(int64_t b, int64_t n) {
+        if (n <= 0) return;
+        used -= n;
+        auto it = free_.emplace(b, n).first;
+        auto nx = std::next(it);
+        if (nx != free_.end() && it->first + it->second == nx->first) { it->second += nx->second; free_.erase(nx); }
+        if (it != free_.begin()) {
+            auto pv = std::prev(it);
+            if (pv->first + pv->second == it->first) { pv->second += it->second; free_.erase(it); }
+        }
+    }
+};
+
+// sizes in the index spaces of a tiled engine (one scene's needs, or an engine's capacity)
+struct Extent {
+    int64_t node = 0;    // nodes (+1 per scene: its incidence CSR ends inside its own range)
+    int64_t elem = 0;    // elements
+    int64_t inc = 0;     // incidences (4 per element): CSR entries and tile-local entries
+    int64_t tile = 0;    // tiles (and their partial slots)
+    int64_t lnode = 0;   // tile-local nodes (+1 per tile)
+    int64_t shn = 0;     // nodes finished by the shared-node kernel (+1 per scene)
+    int64_t slot = 0;    // shared-node partial slots
+    int64_t fin = 0;     // shared-node blocks (and their partial slots)
+    int64_t nblk = 0;    // node blocks (and their partial slots)
+    int64_t osc = 0;     // oscillator entries
+    int64_t sid = 0;     // scene records
+    void add(const Extent& o) {
+        node += o.node; elem += o.elem; inc += o.inc; tile += o.tile; lnode += o.lnode; shn += o.shn;
+        slot += o.slot; fin += o.fin; nblk += o.nblk; osc += o.osc; sid += o.sid;
+    }
+};
+
 struct EngineBase {
     int device = 0;
     int precision = SOFA_AMD_FP64;
@@ -170,6 +236,14 @@ struct EngineBase {
     virtual void bench_apply(int scene, int reps, hipStream_t s) = 0;
     virtual void tiling(int scene, int32_t* out) = 0;
     virtual void reset_graph() = 0;
+    // live scenes (tiled engines): capacity, one scene placed / released at a time
+    virtual bool live() const = 0;
+    virtual void reserve(const Extent& c) = 0;
+    virtual bool commit_scene(int scene) = 0;
+    virtual void release_scene(int scene) = 0;
+    virtual bool placed(int scene) const = 0;
+    virtual void usage(int64_t* out22) const = 0;
+    bool live_mode = false;           // reserved: scenes are committed and released one by one
 };
 
 template <typename T>
@@ -180,10 +254,24 @@ static T* dalloc(size_t n) {
     return p;
 }
 
+// the inner FP32 CG of a MIXED engine: its kernels are the FP32 engine's (compiled in the
+// translation unit that allows fused multiply-adds), on float copies of the element record
+struct InnerCg {
+    TileArgs<float, 6> ta;          // inst = the inner scene records
+    const FinBlk* fb;
+    const NodeBlk* nb;
+    int na, nfb, nnb, maxit;
+    SharedNodes sn;
+    float *r, *p, *q, *d, *sp;
+    double *pden, *prho;            // prho: |r|^2 partials per node block (iteration 1 included)
+};
+void launch_inner_cg(const InnerCg& c, hipStream_t s);
+
 template <typename R, int NR, typename VT>
 struct EngineT : EngineBase {
     bool tiled = false;
-    int64_t Ntot = 0, Etot = 0;
+    bool mixed = false;             // FP64 engine whose CG is FP32 inside FP64 refinement
+    int64_t Ntot = 0, Etot = 0;     // node / element capacity (the SoA stride of element arrays)
     // node vectors (AoS xyz)
     VT *x = nullptr, *v = nullptr, *f = nullptr, *b = nullptr, *r = nullptr, *p = nullptr, *q = nullptr, *xcg = nullptr;
     double* mass = nullptr;
@@ -194,16 +282,18 @@ struct EngineT : EngineBase {
     R *rot = nullptr, *rest = nullptr, *kmat = nullptr;
     VT *ef = nullptr, *ef2 = nullptr;
     std::vector<std::vector<int>> iperm;           // per scene: original element -> position
-    // scene records and block lists
+    VT* p2 = nullptr;                              // previous CG direction (exact path)
+    int *em_ptr = nullptr, *em_nbr = nullptr;      // MeshMatrixMass edges per node (exact path)
+    double* em_val = nullptr;
+    R* emass = nullptr;                            // per element rho V / 20 (tiled path)
+    int* osc_node = nullptr;
+    double *osc_mean = nullptr, *osc_amp = nullptr, *osc_w = nullptr, *osc_phase = nullptr;
+    // scene records
     InstDev* inst = nullptr;
-    std::vector<InstDev> inst_h;
+    // exact path: block lists of every scene, reduction partials
     std::vector<Blk> eblk_h, nblk_h;
     double* partial = nullptr;
-    // tiled path
-    std::vector<TileDesc> tiles_h;
-    std::vector<FinBlk> fin_h;
-    std::vector<NodeBlk> nb_h;
-    std::vector<int> tile_begin, tile_count, fin_begin, fin_count, nbk_begin, nbk_count, shared_slots;
+    // tiled path: device tables, partial buffers
     TileDesc* tiles_d = nullptr;
     Ranges* ranges_d = nullptr;
     uint2* ltet = nullptr;
@@ -211,17 +301,33 @@ struct EngineT : EngineBase {
     uint16_t* ln_inc = nullptr;
     VT *sp0 = nullptr, *sp1 = nullptr;
     double *pbb = nullptr, *pden = nullptr, *prho = nullptr;
-    // launch-set cache
+    // MIXED: float element record and inner CG vectors, inner scene records, |r|^2 partials
+    float *rot32 = nullptr, *rest32 = nullptr, *kmat32 = nullptr, *emass32 = nullptr;
+    float *r32 = nullptr, *p32 = nullptr, *q32 = nullptr, *d32 = nullptr, *sp32 = nullptr;
+    double *pden32 = nullptr, *prho32 = nullptr, *prr = nullptr;
+    InstDev* inst_in = nullptr;
+    // placement of each scene: where its ranges begin in every index space, its block lists
+    struct Place {
+        Extent at, size;
+        bool placed = false;
+        std::vector<TileDesc> tiles;
+        std::vector<FinBlk> fins;
+        std::vector<NodeBlk> nblks;
+        std::vector<Blk> oblks;
+    };
+    std::vector<Place> place;
+    Extent cap;
+    RangeAlloc a_node, a_elem, a_inc, a_tile, a_lnode, a_shn, a_slot, a_fin, a_nblk, a_osc, a_sid;
     // launch sets: the block lists (a: element blocks / tiles, b: node blocks / finalize blocks,
     // c: tiled node blocks) and the captured step graph of each set of scenes stepped together
     struct LaunchSet {
-        void *a = nullptr, *b = nullptr, *c = nullptr;
-        int* scenes = nullptr;
-        int na = 0, nb = 0, nc = 0, maxit = 0, n = 0;
+        void *a = nullptr, *b = nullptr, *c = nullptr, *o = nullptr;
+        int* scenes = nullptr;          // scene record indices
+        int na = 0, nb = 0, nc = 0, no = 0, maxit = 0, maxsw = 0, n = 0;
         hipGraphExec_t gexec = nullptr;
         ~LaunchSet() {
             if (gexec) (void)hipGraphExecDestroy(gexec);
-            for (void* ptr : {a, b, c, (void*)scenes})
+            for (void* ptr : {a, b, c, o, (void*)scenes})
                 if (ptr) (void)hipFree(ptr);
         }
     };
@@ -230,198 +336,293 @@ struct EngineT : EngineBase {
 
     ~EngineT() override {
         sets.clear();
+        free_device();
+    }
+    void free_device() {
         void* ptrs[] = {x, v, f, b, r, p, q, xcg, mass, fixed, inc_ptr, inc, tet, rot, rest, kmat, ef, ef2,
                         inst, partial, tiles_d, ranges_d, ltet, ln_gid, ln_inc_ptr, ln_pslot, sh_gid, sh_ptr,
-                        ln_inc, sp0, sp1, pbb, pden, prho};
+                        ln_inc, sp0, sp1, pbb, pden, prho, p2, em_ptr, em_nbr, em_val, emass, osc_node,
+                        osc_mean, osc_amp, osc_w, osc_phase, rot32, rest32, kmat32, emass32, r32, p32, q32,
+                        d32, sp32, pden32, prho32, prr, inst_in};
         for (void* ptr : ptrs)
             if (ptr) (void)hipFree(ptr);
     }
 
+    bool live() const override { return tiled; }
+
     ElemView<R, NR> view() const { return ElemView<R, NR>{tet, rot, rest, kmat, Etot}; }
-    NodeArrays nodes() const { return NodeArrays{inc_ptr, inc, mass, fixed}; }
+    NodeArrays nodes() const { return NodeArrays{inc_ptr, inc, mass, fixed, em_ptr, em_nbr, em_val}; }
+    Oscillators oscillators() const { return Oscillators{osc_node, osc_mean, osc_amp, osc_w, osc_phase}; }
+    int node_begin(int scene) const { return (int)place[scene].at.node; }
+    int elem_begin(int scene) const { return (int)place[scene].at.elem; }
+    int sid_of(int scene) const { return (int)place[scene].at.sid; }
+
+    // MeshMatrixMass edges (consistent mass): TetrahedronSetTopologyContainer::createEdgeSetArray
+    // order (local edges {0,1},{0,2},{0,3},{1,2},{1,3},{2,3}, first appearance, endpoints
+    // sorted), edge mass (rho V) / 20 summed in tetrahedron order (MeshMatrixMass.inl:603-650);
+    // each node lists its edges in ascending edge id with the other end (global ids)
+    void build_edges(const SceneHost& s, int nE, const std::vector<host::V3>& P, std::vector<int>& emp,
+                     std::vector<int>& emn, std::vector<double>& emv) {
+        std::vector<std::vector<std::pair<int, double>>> per(s.n_nodes);
+        if (s.mass_kind == SOFA_AMD_MASS_MESHMATRIX) {
+            static const int le[6][2] = {{0, 1}, {0, 2}, {0, 3}, {1, 2}, {1, 3}, {2, 3}};
+            std::map<std::pair<int, int>, int> emap;
+            std::vector<std::array<int, 2>> edges;
+            std::vector<double> emass_e;
+            for (int e = 0; e < nE; e++) {
+                const int32_t* t = &s.tets[4 * (size_t)e];
+                const double rho = s.density.size() > (size_t)e ? s.density[e] : s.density[0];
+                const double m = (rho * host::volume(P[t[0]], P[t[1]], P[t[2]], P[t[3]])) / 20.0;
+                int ids[6];
+                for (int j = 0; j < 6; j++) {
+                    int v1 = t[le[j][0]], v2 = t[le[j][1]];
+                    std::pair<int, int> key = v1 < v2 ? std::make_pair(v1, v2) : std::make_pair(v2, v1);
+                    auto it = emap.find(key);
+                    if (it == emap.end()) {
+                        ids[j] = (int)edges.size();
+                        emap[key] = ids[j];
+                        edges.push_back({key.first, key.second});
+                        emass_e.push_back(0.0);
+                    } else {
+                        ids[j] = it->second;
+                    }
+                }
+                for (int j = 0; j < 6; j++) emass_e[ids[j]] += m;
+            }
+            for (size_t j = 0; j < edges.size(); j++) {
+                per[edges[j][0]].push_back({edges[j][1], emass_e[j]});
+                per[edges[j][1]].push_back({edges[j][0], emass_e[j]});
+            }
+        }
+        for (int i = 0; i < s.n_nodes; i++) {
+            for (auto& pr : per[i]) {
+                emn.push_back(pr.first + s.node_begin);
+                emv.push_back(pr.second);
+            }
+            emp.push_back((int)emn.size());
+        }
+    }
 
     template <typename T>
     void upload(T*& dst, const std::vector<T>& src) {
         dst = dalloc<T>(src.size());
         if (!src.empty()) HIPCHK(hipMemcpy(dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice));
     }
+    template <typename T>
+    static void put_at(T* dst, int64_t at, const T* src, size_t n) {
+        if (n) HIPCHK(hipMemcpy(dst + at, src, n * sizeof(T), hipMemcpyHostToDevice));
+    }
+    template <typename T>
+    static void put_at(T* dst, int64_t at, const std::vector<T>& src) { put_at(dst, at, src.data(), src.size()); }
+
+    // ---- one scene's element record, masses and incidence (host, scene-local ids) ----------
+    struct ElemHost {
+        int nE = 0;
+        std::vector<int4> tet;                       // placement order, scene-local node ids
+        std::vector<R> rot, rest, kmat, emass;       // SoA with stride nE
+        std::vector<float> rest_lo, k_lo, em_lo;     // MIXED
+        std::vector<double> mass;                    // per node (DiagonalMass / MeshMatrixMass)
+        std::vector<int> inc_ptr, inc;               // n + 1, 4 nE: position * 4 + vertex
+        std::vector<int8_t> fixed;
+        std::vector<host::V3> P;
+    };
+    // TetrahedronFEMForceField::reinit per element (rotation, rest frame, K), the masses' vertex
+    // terms, FixedProjectiveConstraint / oscillator masks, the incidence in ascending element id
+    void setup_scene(int si, const std::vector<int>& ip, ElemHost& h) {
+        const SceneHost& s = scenes[si];
+        const int nE = (int)(s.tets.size() / 4), n = s.n_nodes;
+        h.nE = nE;
+        h.P.resize(n);
+        for (int i = 0; i < n; i++)
+            for (int k = 0; k < 3; k++) h.P[i].v[k] = s.rest[3 * i + k];
+        h.tet.resize(nE);
+        h.rot.assign(9 * (size_t)nE, (R)0); h.rest.assign(NR * (size_t)nE, (R)0); h.kmat.assign(3 * (size_t)nE, (R)0);
+        h.emass.assign(nE, (R)0);
+        if (mixed) { h.rest_lo.assign(6 * (size_t)nE, 0.f); h.k_lo.assign(3 * (size_t)nE, 0.f); h.em_lo.assign(nE, 0.f); }
+        h.mass.assign(n, 0.0);
+        std::vector<int> cnt(n + 1, 0);
+        for (int e = 0; e < nE; e++) {
+            const int32_t* t = &s.tets[4 * (size_t)e];
+            for (int k = 0; k < 4; k++)
+                if (t[k] < 0 || t[k] >= n) throw std::invalid_argument("tetrahedron index out of range");
+            const double E = s.young.size() > (size_t)e ? s.young[e] : s.young[0];
+            const double nu = s.poisson.size() > (size_t)e ? s.poisson[e] : s.poisson[0];
+            host::ElemInit ei;
+            if (!host::init_element(h.P, t, E, nu, ei))
+                throw std::domain_error("degenerate tetrahedron " + std::to_string(e) + " in scene " + std::to_string(si));
+            const int64_t pe = ip[e];
+            h.tet[pe] = make_int4(t[0], t[1], t[2], t[3]);
+            for (int k = 0; k < 9; k++) h.rot[k * (size_t)nE + pe] = (R)ei.rot[k];
+            const int pick[6] = {0, 3, 4, 6, 7, 8};   // b0, c0, c1, d0, d1, d2
+            if (NR == 9) {
+                for (int k = 0; k < 9; k++) h.rest[k * (size_t)nE + pe] = (R)ei.rest[k];
+            } else {
+                for (int k = 0; k < 6; k++) h.rest[k * (size_t)nE + pe] = (R)ei.rest[pick[k]];
+            }
+            for (int k = 0; k < 3; k++) h.kmat[k * (size_t)nE + pe] = (R)ei.k[k];
+            if (mixed) {
+                for (int k = 0; k < 6; k++) h.rest_lo[k * (size_t)nE + pe] = (float)ei.rest[pick[k]];
+                for (int k = 0; k < 3; k++) h.k_lo[k * (size_t)nE + pe] = (float)ei.k[k];
+            }
+            for (int k = 0; k < 4; k++) cnt[t[k] + 1]++;
+            if (s.mass_kind == SOFA_AMD_MASS_DIAGONAL) {
+                const double m = (s.mass_value * ei.vol) / 4.0;     // DiagonalMass.inl:1075-1080
+                for (int k = 0; k < 4; k++) h.mass[t[k]] += m;
+            } else if (s.mass_kind == SOFA_AMD_MASS_MESHMATRIX || s.mass_kind == SOFA_AMD_MASS_MESHMATRIX_LUMPED) {
+                const double rho = s.density.size() > (size_t)e ? s.density[e] : s.density[0];
+                const double m = (rho * ei.vol) / 10.0;              // MeshMatrixMass.inl:584
+                for (int k = 0; k < 4; k++) h.mass[t[k]] += m;
+                if (s.mass_kind == SOFA_AMD_MASS_MESHMATRIX) {
+                    h.emass[pe] = (R)((rho * ei.vol) / 20.0);
+                    if (mixed) h.em_lo[pe] = (float)((rho * ei.vol) / 20.0);
+                }
+            }
+        }
+        for (int i = 0; i < n; i++) cnt[i + 1] += cnt[i];
+        h.inc_ptr = cnt;
+        h.inc.assign(4 * (size_t)nE, 0);
+        std::vector<int> fillp(cnt.begin(), cnt.end() - 1);
+        for (int e = 0; e < nE; e++)
+            for (int k = 0; k < 4; k++) h.inc[fillp[s.tets[4 * (size_t)e + k]]++] = 4 * ip[e] + k;
+        h.fixed.assign(n, 0);
+        for (int32_t i : s.fixed) {
+            if (i < 0 || i >= n) throw std::invalid_argument("fixed index out of range");
+            h.fixed[i] = 1;
+        }
+        // oscillators: projectResponse clears their nodes like a fixed constraint
+        for (int32_t i : s.osc_idx) {
+            if (i < 0 || i >= n) throw std::invalid_argument("oscillator index out of range");
+            h.fixed[i] = 1;
+        }
+    }
+
+    // the scene record: EulerImplicitSolver / CGLinearSolver parameters and the factors of the
+    // mechanical operations (EulerImplicitSolver.cpp:147-171)
+    InstDev make_inst(int si) const {
+        const SceneHost& s = scenes[si];
+        InstDev in;
+        std::memset(&in, 0, sizeof(in));
+        in.node_count = s.n_nodes;
+        in.elem_count = (int)(s.tets.size() / 4);
+        in.mass_kind = s.mass_kind;
+        in.max_iter = s.cg_iter;
+        in.dt = s.dt; in.rm = s.rm; in.rs = s.rs;
+        in.vertex_mass = s.mass_kind == SOFA_AMD_MASS_UNIFORM ? s.mass_value : 0.0;
+        for (int k = 0; k < 3; k++) {
+            in.gravity[k] = s.g[k];
+            // UniformMass::addForce: mg = theGravity * m, in the state's precision
+            in.mg[k] = (double)((VT)s.g[k] * (VT)in.vertex_mass);
+        }
+        const double h = s.dt, tr = 1.0;
+        in.k_rhs = h * tr + s.rs;
+        in.m_rhs = -s.rm;
+        in.m_cg = 1 + tr * h * s.rm;
+        in.k_cg = -tr * h * (tr * h + s.rs);
+        in.tol = s.tol < 0 ? 1e-5 : s.tol;
+        in.thr = s.thr < 0 ? 1e-5 : s.thr;
+        in.first_solve = 1;
+        in.stopped = tiled ? 0 : 1;
+        in.max_sweeps = s.max_sweeps;
+        return in;
+    }
+    // the inner FP32 CG's record: the same operator, FP32 CG limits, no threshold test (the
+    // inner right-hand side shrinks sweep by sweep), closed until a sweep opens it
+    InstDev make_inner(int si, const InstDev& o) const {
+        InstDev in = o;
+        in.max_iter = scenes[si].inner_iter;
+        in.tol = scenes[si].inner_tol;
+        in.thr = 0.0;
+        in.first_solve = 0;
+        in.stopped = 1;
+        in.mg[0] = in.mg[1] = in.mg[2] = 0.0;
+        return in;
+    }
 
     void commit() override {
         HIPCHK(hipSetDevice(device));
+        if (tiled) commit_tiled();
+        else commit_exact();
+        committed = true;
+    }
+
+    // ---- exact path: every scene placed once, in order, into exactly sized buffers ---------
+    void commit_exact() {
+        const int S = (int)scenes.size();
         Ntot = 0; Etot = 0;
-        for (auto& s : scenes) {
-            s.node_begin = (int)Ntot;
-            s.elem_begin = (int)Etot;
-            Ntot += s.n_nodes;
-            Etot += (int64_t)s.tets.size() / 4;
+        place.assign(S, Place());
+        for (int si = 0; si < S; si++) {
+            place[si].at.node = Ntot;
+            place[si].at.elem = Etot;
+            place[si].at.sid = si;
+            place[si].placed = true;
+            scenes[si].node_begin = (int)Ntot;
+            scenes[si].elem_begin = (int)Etot;
+            Ntot += scenes[si].n_nodes;
+            Etot += (int64_t)scenes[si].tets.size() / 4;
         }
         if (Ntot >= (int64_t)1 << 30 || Etot * 4 >= (int64_t)1 << 31)
             throw std::invalid_argument("mesh too large for 32-bit indices");
-        const int S = (int)scenes.size();
-        inst_h.assign(S, InstDev());
+        std::vector<InstDev> inst_h(S);
         eblk_h.clear(); nblk_h.clear();
-        tiles_h.clear(); fin_h.clear(); nb_h.clear();
-        tile_begin.assign(S, 0); tile_count.assign(S, 0); fin_begin.assign(S, 0); fin_count.assign(S, 0);
-        nbk_begin.assign(S, 0); nbk_count.assign(S, 0); shared_slots.assign(S, 0);
         iperm.assign(S, {});
         std::vector<double> xh(3 * Ntot), massh(Ntot, 0.0);
         std::vector<int8_t> fixh(Ntot, 0);
         std::vector<int4> teth(Etot);
         std::vector<R> roth(9 * Etot), resth(NR * Etot), kh(3 * Etot);
-        std::vector<int> cnt(Ntot + 1, 0);
-        // tiled-path host tables (global ids)
-        std::vector<uint2> ltet_h(tiled ? Etot : 0);
-        std::vector<int> lgid, lptr{0}, lps, shg, shp{0};
-        std::vector<uint16_t> linc;
-        int slot_base = 0;
+        std::vector<int> cnt{0}, inch(4 * Etot);
+        std::vector<int> emp{0}, emn;
+        std::vector<double> emv;
+        std::vector<int> oscn;
+        std::vector<double> oscm, osca, oscw, oscp;
         for (int si = 0; si < S; si++) {
             SceneHost& s = scenes[si];
             const int nE = (int)(s.tets.size() / 4);
-            std::vector<host::V3> P(s.n_nodes);
-            for (int i = 0; i < s.n_nodes; i++) {
-                for (int k = 0; k < 3; k++) P[i].v[k] = s.rest[3 * i + k];
-                for (int k = 0; k < 3; k++) xh[3 * (s.node_begin + (int64_t)i) + k] = s.rest[3 * i + k];
-            }
-            TilePlan plan;
+            const int nb0 = s.node_begin;
+            const int64_t eb0 = s.elem_begin;
             std::vector<int>& ip = iperm[si];
-            if (tiled && nE > 0) {
-                plan = plan_tiles(s.n_nodes, s.rest.data(), s.tets.data(), nE, kTileElems, kTileNodes);
-                ip = plan.iperm;
-            } else {
-                ip.resize(nE);
-                for (int e = 0; e < nE; e++) ip[e] = e;
+            ip.resize(nE);
+            for (int e = 0; e < nE; e++) ip[e] = e;
+            ElemHost h;
+            setup_scene(si, ip, h);
+            for (int i = 0; i < s.n_nodes; i++) {
+                for (int k = 0; k < 3; k++) xh[3 * (nb0 + (int64_t)i) + k] = s.rest[3 * i + k];
+                massh[nb0 + i] = h.mass[i];
+                fixh[nb0 + i] = h.fixed[i];
             }
             for (int e = 0; e < nE; e++) {
-                const int32_t* t = &s.tets[4 * (size_t)e];
-                for (int k = 0; k < 4; k++)
-                    if (t[k] < 0 || t[k] >= s.n_nodes) throw std::invalid_argument("tetrahedron index out of range");
-                const double E = s.young.size() > (size_t)e ? s.young[e] : s.young[0];
-                const double nu = s.poisson.size() > (size_t)e ? s.poisson[e] : s.poisson[0];
-                host::ElemInit ei;
-                if (!host::init_element(P, t, E, nu, ei))
-                    throw std::domain_error("degenerate tetrahedron " + std::to_string(e) + " in scene " + std::to_string(si));
-                const int64_t ge = s.elem_begin + (int64_t)ip[e];
-                teth[ge] = make_int4(t[0] + s.node_begin, t[1] + s.node_begin, t[2] + s.node_begin, t[3] + s.node_begin);
-                for (int k = 0; k < 9; k++) roth[k * Etot + ge] = (R)ei.rot[k];
-                if (NR == 9) {
-                    for (int k = 0; k < 9; k++) resth[k * Etot + ge] = (R)ei.rest[k];
-                } else {
-                    const int pick[6] = {0, 3, 4, 6, 7, 8};   // b0, c0, c1, d0, d1, d2
-                    for (int k = 0; k < 6; k++) resth[k * Etot + ge] = (R)ei.rest[pick[k]];
-                }
-                for (int k = 0; k < 3; k++) kh[k * Etot + ge] = (R)ei.k[k];
-                for (int k = 0; k < 4; k++) cnt[s.node_begin + t[k] + 1]++;
-                if (s.mass_kind == SOFA_AMD_MASS_DIAGONAL) {
-                    const double m = (s.mass_value * ei.vol) / 4.0;     // DiagonalMass.inl:1075-1080
-                    for (int k = 0; k < 4; k++) massh[s.node_begin + t[k]] += m;
-                }
+                const int4 t = h.tet[e];
+                teth[eb0 + e] = make_int4(t.x + nb0, t.y + nb0, t.z + nb0, t.w + nb0);
+                for (int k = 0; k < 9; k++) roth[k * Etot + eb0 + e] = h.rot[k * (size_t)nE + e];
+                for (int k = 0; k < NR; k++) resth[k * Etot + eb0 + e] = h.rest[k * (size_t)nE + e];
+                for (int k = 0; k < 3; k++) kh[k * Etot + eb0 + e] = h.kmat[k * (size_t)nE + e];
             }
-            for (int32_t i : s.fixed) {
-                if (i < 0 || i >= s.n_nodes) throw std::invalid_argument("fixed index out of range");
-                fixh[s.node_begin + i] = 1;
+            const int ibase = cnt.back();
+            for (int i = 0; i < s.n_nodes; i++) cnt.push_back(ibase + h.inc_ptr[i + 1]);
+            for (int j = 0; j < 4 * nE; j++) inch[ibase + j] = h.inc[j] + (int)(4 * eb0);
+            build_edges(s, nE, h.P, emp, emn, emv);
+            const int ob = (int)oscn.size();
+            for (size_t j = 0; j < s.osc_idx.size(); j++) {
+                oscn.push_back(nb0 + s.osc_idx[j]);
+                for (int k = 0; k < 3; k++) { oscm.push_back(s.osc_mean[3 * j + k]); osca.push_back(s.osc_amp[3 * j + k]); }
+                oscw.push_back(s.osc_w[j]);
+                oscp.push_back(s.osc_phase[j]);
             }
-            // exact-path blocks: 256 elements / 256 nodes
+            for (int j0 = 0; j0 < (int)s.osc_idx.size(); j0 += kBlock)
+                place[si].oblks.push_back(Blk{si, ob + j0, ob + std::min((int)s.osc_idx.size(), j0 + kBlock), 0});
+            // blocks: 256 elements / 256 nodes
             s.eblk_begin = (int)eblk_h.size();
             for (int e0 = 0; e0 < nE; e0 += kBlock)
-                eblk_h.push_back(Blk{si, s.elem_begin + e0, s.elem_begin + std::min(nE, e0 + kBlock), 0});
+                eblk_h.push_back(Blk{si, (int)eb0 + e0, (int)eb0 + std::min(nE, e0 + kBlock), 0});
             s.eblk_count = (int)eblk_h.size() - s.eblk_begin;
             s.nblk_begin = (int)nblk_h.size();
             for (int n0 = 0; n0 < s.n_nodes; n0 += kBlock)
-                nblk_h.push_back(Blk{si, s.node_begin + n0, s.node_begin + std::min(s.n_nodes, n0 + kBlock),
-                                     (int)nblk_h.size()});
+                nblk_h.push_back(Blk{si, nb0 + n0, nb0 + std::min(s.n_nodes, n0 + kBlock), (int)nblk_h.size()});
             s.nblk_count = (int)nblk_h.size() - s.nblk_begin;
-            if (tiled) {
-                // tiles
-                tile_begin[si] = (int)tiles_h.size();
-                const int T = (int)plan.t_ebegin.size();
-                const int lbase = (int)lgid.size();
-                for (int ti = 0; ti < T; ti++) {
-                    TileDesc td;
-                    td.scene = si;
-                    td.e_begin = s.elem_begin + plan.t_ebegin[ti];
-                    td.e_count = plan.t_ecount[ti];
-                    td.ln_begin = lbase + plan.t_lnbegin[ti];
-                    td.ln_count = plan.t_lncount[ti];
-                    td.first = (ti == 0);
-                    td.pslot = (int)tiles_h.size();
-                    td.pad = 0;
-                    tiles_h.push_back(td);
-                }
-                tile_count[si] = T;
-                for (int i = 0; i < nE; i++) ltet_h[s.elem_begin + i] = plan.ltet[i];
-                const int ibase = (int)linc.size();
-                for (size_t j = 0; j < plan.ln_gid.size(); j++) {
-                    lgid.push_back(plan.ln_gid[j] + s.node_begin);
-                    lptr.push_back(ibase + plan.ln_inc_ptr[j + 1]);
-                    lps.push_back(plan.ln_pslot[j] < 0 ? -1 : plan.ln_pslot[j] + slot_base);
-                }
-                linc.insert(linc.end(), plan.ln_inc.begin(), plan.ln_inc.end());
-                // shared / element-free nodes
-                std::vector<int> shared = plan.sh_gid, sptr = plan.sh_ptr;
-                if (nE == 0) {   // no elements: every node is finished by the finalize kernel
-                    shared.resize(s.n_nodes);
-                    for (int i = 0; i < s.n_nodes; i++) shared[i] = i;
-                    sptr.assign(s.n_nodes + 1, 0);
-                }
-                const int shbase = (int)shg.size();
-                for (size_t j = 0; j < shared.size(); j++) {
-                    shg.push_back(shared[j] + s.node_begin);
-                    shp.push_back(slot_base + sptr[j + 1]);
-                }
-                fin_begin[si] = (int)fin_h.size();
-                for (int j0 = 0; j0 < (int)shared.size(); j0 += kBlock)
-                    fin_h.push_back(FinBlk{si, shbase + j0, shbase + std::min((int)shared.size(), j0 + kBlock), 0,
-                                           j0 / kBlock, 0, 0, 0});
-                fin_count[si] = (int)fin_h.size() - fin_begin[si];
-                slot_base += sptr.back();
-                shared_slots[si] = sptr.back();
-                nbk_begin[si] = (int)nb_h.size();
-                const int per = kBlock * kNodesPerThread;
-                for (int n0 = 0; n0 < s.n_nodes; n0 += per)
-                    nb_h.push_back(NodeBlk{si, s.node_begin + n0, s.node_begin + std::min(s.n_nodes, n0 + per),
-                                           (int)nb_h.size(), n0 == 0, 0, 0, 0});
-                nbk_count[si] = (int)nb_h.size() - nbk_begin[si];
-            }
-            // scene record
             InstDev& in = inst_h[si];
-            std::memset(&in, 0, sizeof(in));
-            in.node_begin = s.node_begin; in.node_count = s.n_nodes;
-            in.elem_begin = s.elem_begin; in.elem_count = nE;
+            in = make_inst(si);
+            in.node_begin = nb0; in.elem_begin = (int)eb0;
             in.nblk_begin = s.nblk_begin; in.nblk_count = s.nblk_count;
-            in.mass_kind = s.mass_kind;
-            in.max_iter = s.cg_iter;
-            in.dt = s.dt; in.rm = s.rm; in.rs = s.rs;
-            in.vertex_mass = s.mass_kind == SOFA_AMD_MASS_UNIFORM ? s.mass_value : 0.0;
-            for (int k = 0; k < 3; k++) {
-                in.gravity[k] = s.g[k];
-                // UniformMass::addForce: mg = theGravity * m, in the state's precision
-                in.mg[k] = (double)((VT)s.g[k] * (VT)in.vertex_mass);
-            }
-            const double h = s.dt, tr = 1.0;
-            in.k_rhs = h * tr + s.rs;
-            in.m_rhs = -s.rm;
-            in.m_cg = 1 + tr * h * s.rm;
-            in.k_cg = -tr * h * (tr * h + s.rs);
-            in.tol = s.tol < 0 ? 1e-5 : s.tol;
-            in.thr = s.thr < 0 ? 1e-5 : s.thr;
-            in.first_solve = 1;
-            in.stopped = tiled ? 0 : 1;
         }
-        const int n_tiles = (int)tiles_h.size();
-        for (auto& fb : fin_h) fb.pslot = n_tiles + (int)(&fb - fin_h.data());
-        // incidence CSR for the exact path and the component calls: node -> position*4 + vertex,
-        // visited in ascending original element id
-        for (int64_t i = 0; i < Ntot; i++) cnt[i + 1] += cnt[i];
-        std::vector<int> inch(4 * Etot), fillp(cnt.begin(), cnt.end() - 1);
-        for (int si = 0; si < S; si++) {
-            const SceneHost& s = scenes[si];
-            const int nE = (int)(s.tets.size() / 4);
-            for (int e = 0; e < nE; e++) {
-                const int64_t ge = s.elem_begin + (int64_t)iperm[si][e];
-                for (int k = 0; k < 4; k++) inch[fillp[s.node_begin + s.tets[4 * (size_t)e + k]]++] = (int)(4 * ge + k);
-            }
-        }
-        // device buffers
         x = dalloc<VT>(3 * Ntot); v = dalloc<VT>(3 * Ntot); f = dalloc<VT>(3 * Ntot); b = dalloc<VT>(3 * Ntot);
         r = dalloc<VT>(3 * Ntot); p = dalloc<VT>(3 * Ntot); q = dalloc<VT>(3 * Ntot); xcg = dalloc<VT>(3 * Ntot);
         std::vector<VT> xv(3 * Ntot);
@@ -431,31 +632,295 @@ struct EngineT : EngineBase {
         upload(mass, massh); upload(fixed, fixh);
         upload(inc_ptr, cnt); upload(inc, inch);
         upload(tet, teth); upload(rot, roth); upload(rest, resth); upload(kmat, kh);
+        upload(em_ptr, emp); upload(em_nbr, emn); upload(em_val, emv);
+        upload(osc_node, oscn); upload(osc_mean, oscm); upload(osc_amp, osca); upload(osc_w, oscw); upload(osc_phase, oscp);
         ef = dalloc<VT>(12 * Etot);
-        if (!tiled) ef2 = dalloc<VT>(12 * Etot);
+        ef2 = dalloc<VT>(12 * Etot);
+        p2 = dalloc<VT>(3 * Ntot);
+        HIPCHK(hipMemset(p2, 0, 3 * Ntot * sizeof(VT)));
         upload(inst, inst_h);
         partial = dalloc<double>(nblk_h.size());
         HIPCHK(hipMemset(partial, 0, std::max<size_t>(1, nblk_h.size()) * sizeof(double)));
-        if (tiled) {
-            std::vector<Ranges> rg(S);
-            for (int si = 0; si < S; si++)
-                rg[si] = Ranges{tile_begin[si], tile_count[si], n_tiles + fin_begin[si], fin_count[si], nbk_begin[si],
-                                nbk_count[si]};
-            upload(tiles_d, tiles_h);
-            upload(ranges_d, rg);
-            upload(ltet, ltet_h);
-            upload(ln_gid, lgid); upload(ln_inc_ptr, lptr); upload(ln_pslot, lps); upload(ln_inc, linc);
-            upload(sh_gid, shg); upload(sh_ptr, shp);
-            sp0 = dalloc<VT>(3 * (size_t)std::max(1, slot_base));
-            sp1 = dalloc<VT>(3 * (size_t)std::max(1, slot_base));
-            const size_t np = n_tiles + fin_h.size();
-            pbb = dalloc<double>(np); pden = dalloc<double>(np); prho = dalloc<double>(nb_h.size());
-            HIPCHK(hipMemset(pbb, 0, std::max<size_t>(1, np) * sizeof(double)));
-            HIPCHK(hipMemset(pden, 0, std::max<size_t>(1, np) * sizeof(double)));
-            HIPCHK(hipMemset(prho, 0, std::max<size_t>(1, nb_h.size()) * sizeof(double)));
+        HIPCHK(hipDeviceSynchronize());
+    }
+
+    // ---- tiled path: arenas, scenes placed one at a time ------------------------------------
+    struct TiledHost {
+        TilePlan plan;
+        ElemHost eh;
+        std::vector<int> sh_gid, sh_ptr;   // nodes finished by the shared-node kernel
+        Extent need;
+    };
+    void build_tiled(int si, TiledHost& th) {
+        const SceneHost& s = scenes[si];
+        const int nE = (int)(s.tets.size() / 4), n = s.n_nodes;
+        std::vector<int>& ip = iperm[si];
+        if (nE > 0) {
+            th.plan = plan_tiles(n, s.rest.data(), s.tets.data(), nE, kTileElems, kTileNodes);
+            ip = th.plan.iperm;
+        } else {
+            ip.clear();
+        }
+        setup_scene(si, ip, th.eh);
+        th.sh_gid = th.plan.sh_gid;
+        th.sh_ptr = th.plan.sh_ptr;
+        if (nE == 0) {   // no elements: every node is finished by the shared-node kernel
+            th.sh_gid.resize(n);
+            for (int i = 0; i < n; i++) th.sh_gid[i] = i;
+            th.sh_ptr.assign(n + 1, 0);
+        }
+        if (th.sh_ptr.empty()) th.sh_ptr.push_back(0);
+        Extent& e = th.need;
+        const int64_t T = (int64_t)th.plan.t_ebegin.size();
+        const int64_t Sn = (int64_t)th.sh_gid.size();
+        e.node = n + 1;
+        e.elem = nE;
+        e.inc = 4 * (int64_t)nE;
+        e.tile = T;
+        e.lnode = (int64_t)th.plan.ln_gid.size() + T;
+        e.shn = Sn + 1;
+        e.slot = th.sh_ptr.back();
+        e.fin = (Sn + kBlock - 1) / kBlock;
+        e.nblk = (n + kBlock * kNodesPerThread - 1) / (kBlock * kNodesPerThread);
+        e.osc = (int64_t)s.osc_idx.size();
+        e.sid = 1;
+    }
+
+    RangeAlloc* arenas(int k) {
+        RangeAlloc* a[] = {&a_node, &a_elem, &a_inc, &a_tile, &a_lnode, &a_shn, &a_slot, &a_fin, &a_nblk, &a_osc, &a_sid};
+        return a[k];
+    }
+    static int64_t& field(Extent& e, int k) {
+        int64_t* f[] = {&e.node, &e.elem, &e.inc, &e.tile, &e.lnode, &e.shn, &e.slot, &e.fin, &e.nblk, &e.osc, &e.sid};
+        return *f[k];
+    }
+
+    // device buffers of capacity `c` in every index space (all scenes' data is dropped)
+    void reserve(const Extent& c) override {
+        if (!tiled) throw std::invalid_argument("live scenes need a tiled engine (FP32, FP64_TILED or MIXED)");
+        HIPCHK(hipSetDevice(device));
+        HIPCHK(hipDeviceSynchronize());
+        sets.clear();
+        cur = nullptr;
+        free_device();
+        x = v = f = b = r = p = q = xcg = nullptr; mass = nullptr; fixed = nullptr; inc_ptr = inc = nullptr;
+        tet = nullptr; rot = rest = kmat = nullptr; ef = ef2 = nullptr; p2 = nullptr; em_ptr = em_nbr = nullptr;
+        em_val = nullptr; emass = nullptr; osc_node = nullptr; osc_mean = osc_amp = osc_w = osc_phase = nullptr;
+        inst = nullptr; partial = nullptr; tiles_d = nullptr; ranges_d = nullptr; ltet = nullptr;
+        ln_gid = ln_inc_ptr = ln_pslot = sh_gid = sh_ptr = nullptr; ln_inc = nullptr; sp0 = sp1 = nullptr;
+        pbb = pden = prho = nullptr; rot32 = rest32 = kmat32 = emass32 = nullptr;
+        r32 = p32 = q32 = d32 = sp32 = nullptr; pden32 = prho32 = prr = nullptr; inst_in = nullptr;
+        cap = c;
+        if (cap.node >= (int64_t)1 << 30 || cap.inc >= (int64_t)1 << 31 || cap.lnode >= (int64_t)1 << 31)
+            throw std::invalid_argument("capacity too large for 32-bit indices");
+        for (int k = 0; k < 11; k++) arenas(k)->reset(field(cap, k));
+        Ntot = cap.node; Etot = cap.elem;
+        const size_t N = std::max<int64_t>(1, cap.node), E = std::max<int64_t>(1, cap.elem);
+        for (VT** vec : {&x, &v, &f, &b, &r, &p, &q, &xcg}) {
+            *vec = dalloc<VT>(3 * N);
+            HIPCHK(hipMemset(*vec, 0, 3 * N * sizeof(VT)));
+        }
+        mass = dalloc<double>(N); fixed = dalloc<int8_t>(N); inc_ptr = dalloc<int>(N);
+        tet = dalloc<int4>(E); rot = dalloc<R>(9 * E); rest = dalloc<R>(NR * E); kmat = dalloc<R>(3 * E);
+        emass = dalloc<R>(E); ltet = dalloc<uint2>(E); ef = dalloc<VT>(12 * E);
+        inc = dalloc<int>(cap.inc); ln_inc = dalloc<uint16_t>(cap.inc);
+        tiles_d = dalloc<TileDesc>(cap.tile);
+        const size_t np = cap.tile + cap.fin;
+        pbb = dalloc<double>(np); pden = dalloc<double>(np);
+        ln_gid = dalloc<int>(cap.lnode); ln_inc_ptr = dalloc<int>(cap.lnode); ln_pslot = dalloc<int>(cap.lnode);
+        sh_gid = dalloc<int>(cap.shn); sh_ptr = dalloc<int>(cap.shn);
+        sp0 = dalloc<VT>(3 * (size_t)std::max<int64_t>(1, cap.slot)); sp1 = dalloc<VT>(3 * (size_t)std::max<int64_t>(1, cap.slot));
+        prho = dalloc<double>(cap.nblk);
+        osc_node = dalloc<int>(cap.osc); osc_mean = dalloc<double>(3 * cap.osc); osc_amp = dalloc<double>(3 * cap.osc);
+        osc_w = dalloc<double>(cap.osc); osc_phase = dalloc<double>(cap.osc);
+        inst = dalloc<InstDev>(cap.sid); ranges_d = dalloc<Ranges>(cap.sid);
+        for (double* pb : {pbb, pden}) HIPCHK(hipMemset(pb, 0, std::max<size_t>(1, np) * sizeof(double)));
+        HIPCHK(hipMemset(prho, 0, std::max<int64_t>(1, cap.nblk) * sizeof(double)));
+        if (mixed) {
+            rot32 = dalloc<float>(9 * E); rest32 = dalloc<float>(6 * E); kmat32 = dalloc<float>(3 * E);
+            emass32 = dalloc<float>(E);
+            for (float** vec : {&r32, &p32, &q32, &d32}) {
+                *vec = dalloc<float>(3 * N);
+                HIPCHK(hipMemset(*vec, 0, 3 * N * sizeof(float)));
+            }
+            sp32 = dalloc<float>(3 * (size_t)std::max<int64_t>(1, cap.slot));
+            pden32 = dalloc<double>(np); prho32 = dalloc<double>(cap.nblk); prr = dalloc<double>(cap.nblk);
+            HIPCHK(hipMemset(pden32, 0, std::max<size_t>(1, np) * sizeof(double)));
+            HIPCHK(hipMemset(prho32, 0, std::max<int64_t>(1, cap.nblk) * sizeof(double)));
+            HIPCHK(hipMemset(prr, 0, std::max<int64_t>(1, cap.nblk) * sizeof(double)));
+            inst_in = dalloc<InstDev>(cap.sid);
+        }
+        place.assign(scenes.size(), Place());
+        iperm.resize(scenes.size());
+        HIPCHK(hipDeviceSynchronize());
+    }
+
+    void release_ranges(Place& pl) {
+        for (int k = 0; k < 11; k++) arenas(k)->release(field(pl.at, k), field(pl.size, k));
+        pl.placed = false;
+        pl.tiles.clear(); pl.fins.clear(); pl.nblks.clear(); pl.oblks.clear();
+    }
+
+    // allocate the scene's ranges and upload its tables; false (nothing kept) if it does not fit
+    bool place_scene(int si, TiledHost& th) {
+        if ((int)place.size() <= si) place.resize(si + 1);
+        Place& pl = place[si];
+        pl.size = th.need;
+        for (int k = 0; k < 11; k++) {
+            const int64_t at = arenas(k)->alloc(field(pl.size, k));
+            if (at < 0) {
+                for (int j = 0; j < k; j++) arenas(j)->release(field(pl.at, j), field(pl.size, j));
+                return false;
+            }
+            field(pl.at, k) = at;
+        }
+        pl.placed = true;
+        const SceneHost& s = scenes[si];
+        const int n = s.n_nodes, nE = th.eh.nE;
+        const int nb0 = (int)pl.at.node, sid = (int)pl.at.sid;
+        const int64_t eb0 = pl.at.elem, ib0 = pl.at.inc;
+        const TilePlan& P = th.plan;
+        scenes[si].node_begin = nb0;
+        scenes[si].elem_begin = (int)eb0;
+        // nodes: state, masses, masks, incidence CSR (n + 1 entries: the end stays in range)
+        std::vector<VT> xv(3 * (size_t)(n + 1), (VT)0), zero(3 * (size_t)(n + 1), (VT)0);
+        for (int i = 0; i < 3 * n; i++) xv[i] = (VT)s.rest[i];
+        put_at(x, 3 * (int64_t)nb0, xv);
+        for (VT* vec : {v, f, b, r, p, q, xcg}) put_at(vec, 3 * (int64_t)nb0, zero);
+        std::vector<double> mh(th.eh.mass);
+        mh.push_back(0.0);
+        put_at(mass, nb0, mh);
+        std::vector<int8_t> fx(th.eh.fixed);
+        fx.push_back(1);
+        put_at(fixed, nb0, fx);
+        std::vector<int> ptr(n + 1);
+        for (int i = 0; i <= n; i++) ptr[i] = (int)ib0 + th.eh.inc_ptr[i];
+        put_at(inc_ptr, nb0, ptr);
+        std::vector<int> incv(th.eh.inc);
+        for (int& j : incv) j += (int)(4 * eb0);
+        put_at(inc, ib0, incv);
+        // elements
+        std::vector<int4> tv(th.eh.tet);
+        for (int4& t : tv) { t.x += nb0; t.y += nb0; t.z += nb0; t.w += nb0; }
+        put_at(tet, eb0, tv);
+        for (int k = 0; k < 9; k++) put_at(rot, k * Etot + eb0, th.eh.rot.data() + k * (size_t)nE, nE);
+        for (int k = 0; k < NR; k++) put_at(rest, k * Etot + eb0, th.eh.rest.data() + k * (size_t)nE, nE);
+        for (int k = 0; k < 3; k++) put_at(kmat, k * Etot + eb0, th.eh.kmat.data() + k * (size_t)nE, nE);
+        put_at(emass, eb0, th.eh.emass);
+        put_at(ltet, eb0, P.ltet);
+        put_at(ln_inc, ib0, P.ln_inc);
+        if (mixed) {
+            for (int k = 0; k < 6; k++) put_at(rest32, k * Etot + eb0, th.eh.rest_lo.data() + k * (size_t)nE, nE);
+            for (int k = 0; k < 3; k++) put_at(kmat32, k * Etot + eb0, th.eh.k_lo.data() + k * (size_t)nE, nE);
+            put_at(emass32, eb0, th.eh.em_lo);
+            std::vector<float> rl(9 * (size_t)nE);
+            for (size_t j = 0; j < rl.size(); j++) rl[j] = (float)th.eh.rot[j];
+            for (int k = 0; k < 9; k++) put_at(rot32, k * Etot + eb0, rl.data() + k * (size_t)nE, nE);
+        }
+        // tiles and their local nodes (each tile's list keeps one extra entry for its end)
+        const int T = (int)P.t_ebegin.size();
+        std::vector<TileDesc> tds(T);
+        std::vector<int> lg, lp, lps;
+        lg.reserve(pl.size.lnode); lp.reserve(pl.size.lnode); lps.reserve(pl.size.lnode);
+        for (int t = 0; t < T; t++) {
+            TileDesc& td = tds[t];
+            td.scene = sid;
+            td.e_begin = (int)eb0 + P.t_ebegin[t];
+            td.e_count = P.t_ecount[t];
+            td.ln_begin = (int)pl.at.lnode + (int)lg.size();
+            td.ln_count = P.t_lncount[t];
+            td.first = (t == 0);
+            td.pslot = (int)pl.at.tile + t;
+            td.pad = 0;
+            for (int j = 0; j <= P.t_lncount[t]; j++) {
+                const int li = P.t_lnbegin[t] + j;
+                const bool end = j == P.t_lncount[t];
+                lg.push_back(end ? nb0 : P.ln_gid[li] + nb0);
+                lp.push_back((int)ib0 + P.ln_inc_ptr[li]);
+                lps.push_back(end || P.ln_pslot[li] < 0 ? -1 : P.ln_pslot[li] + (int)pl.at.slot);
+            }
+        }
+        put_at(tiles_d, pl.at.tile, tds);
+        put_at(ln_gid, pl.at.lnode, lg); put_at(ln_inc_ptr, pl.at.lnode, lp); put_at(ln_pslot, pl.at.lnode, lps);
+        pl.tiles = tds;
+        // shared nodes and their blocks
+        const int Sn = (int)th.sh_gid.size();
+        std::vector<int> sg(Sn + 1), spp(Sn + 1);
+        for (int j = 0; j < Sn; j++) sg[j] = th.sh_gid[j] + nb0;
+        sg[Sn] = nb0;
+        for (int j = 0; j <= Sn; j++) spp[j] = (int)pl.at.slot + th.sh_ptr[j];
+        put_at(sh_gid, pl.at.shn, sg); put_at(sh_ptr, pl.at.shn, spp);
+        for (int j0 = 0, bi = 0; j0 < Sn; j0 += kBlock, bi++)
+            pl.fins.push_back(FinBlk{sid, (int)pl.at.shn + j0, (int)pl.at.shn + std::min(Sn, j0 + kBlock),
+                                     (int)(cap.tile + pl.at.fin) + bi, bi, 0, 0, 0});
+        const int per = kBlock * kNodesPerThread;
+        for (int n0 = 0, bi = 0; n0 < n; n0 += per, bi++)
+            pl.nblks.push_back(NodeBlk{sid, nb0 + n0, nb0 + std::min(n, n0 + per), (int)pl.at.nblk + bi, n0 == 0, 0, 0, 0});
+        // oscillators
+        const int no = (int)s.osc_idx.size();
+        if (no > 0) {
+            std::vector<int> on(no);
+            for (int j = 0; j < no; j++) on[j] = s.osc_idx[j] + nb0;
+            put_at(osc_node, pl.at.osc, on);
+            put_at(osc_mean, 3 * pl.at.osc, s.osc_mean); put_at(osc_amp, 3 * pl.at.osc, s.osc_amp);
+            put_at(osc_w, pl.at.osc, s.osc_w); put_at(osc_phase, pl.at.osc, s.osc_phase);
+            for (int j0 = 0; j0 < no; j0 += kBlock)
+                pl.oblks.push_back(Blk{sid, (int)pl.at.osc + j0, (int)pl.at.osc + std::min(no, j0 + kBlock), 0});
+        }
+        // scene records
+        InstDev in = make_inst(si);
+        in.node_begin = nb0; in.elem_begin = (int)eb0;
+        put_at(inst, sid, &in, 1);
+        const Ranges rg{(int)pl.at.tile, T, (int)(cap.tile + pl.at.fin), (int)pl.size.fin, (int)pl.at.nblk,
+                        (int)pl.size.nblk};
+        put_at(ranges_d, sid, &rg, 1);
+        if (mixed) {
+            const InstDev ii = make_inner(si, in);
+            put_at(inst_in, sid, &ii, 1);
+        }
+        return true;
+    }
+
+    void commit_tiled() {
+        const int S = (int)scenes.size();
+        iperm.assign(S, {});
+        std::vector<TiledHost> th(S);
+        Extent tot;
+        for (int si = 0; si < S; si++) {
+            build_tiled(si, th[si]);
+            tot.add(th[si].need);
+        }
+        reserve(tot);
+        for (int si = 0; si < S; si++) {
+            if (!place_scene(si, th[si])) throw std::runtime_error("internal: scene does not fit its exact reservation");
+            th[si] = TiledHost();
         }
         HIPCHK(hipDeviceSynchronize());
-        committed = true;
+    }
+
+    // live engines: place one more scene; false if the arenas have no room for it now
+    bool commit_scene(int si) override {
+        if (si >= (int)place.size()) place.resize(si + 1);
+        if (si >= (int)iperm.size()) iperm.resize(si + 1);
+        if (place[si].placed) throw std::logic_error("scene already committed");
+        // the ranges it gets are free: no work in flight reads them, so the (synchronous)
+        // uploads need no device-wide wait
+        TiledHost th;
+        build_tiled(si, th);
+        return place_scene(si, th);
+    }
+    void release_scene(int si) override {
+        HIPCHK(hipDeviceSynchronize());   // nothing in flight reads the ranges any more
+        sets.clear();                     // launch sets holding the scene's blocks
+        cur = nullptr;
+        release_ranges(place[si]);
+        iperm[si].clear();
+        iperm[si].shrink_to_fit();
+    }
+    bool placed(int si) const override { return si < (int)place.size() && place[si].placed; }
+    void usage(int64_t* out) const override {
+        const RangeAlloc* a[] = {&a_node, &a_elem, &a_inc, &a_tile, &a_lnode, &a_shn, &a_slot, &a_fin, &a_nblk, &a_osc, &a_sid};
+        for (int k = 0; k < 11; k++) { out[2 * k] = a[k]->used; out[2 * k + 1] = a[k]->cap; }
     }
 
     // block lists of a launch set (a: element blocks / tiles, b: node blocks / finalize blocks,
@@ -479,6 +944,7 @@ struct EngineT : EngineBase {
         }
         std::unique_ptr<LaunchSet> ls(new LaunchSet());
         int maxit = 0;
+        std::vector<int> sids;
         if (!tiled) {
             std::vector<Blk> eb, nb;
             for (int32_t id : want) {
@@ -494,16 +960,25 @@ struct EngineT : EngineBase {
             std::vector<FinBlk> fa;
             std::vector<NodeBlk> na;
             for (int32_t id : want) {
-                push_range(ta, tiles_h, tile_begin[id], tile_count[id]);
-                push_range(fa, fin_h, fin_begin[id], fin_count[id]);
-                push_range(na, nb_h, nbk_begin[id], nbk_count[id]);
-                maxit = std::max(maxit, scenes[id].cg_iter);
+                const Place& pl = place[id];
+                ta.insert(ta.end(), pl.tiles.begin(), pl.tiles.end());
+                fa.insert(fa.end(), pl.fins.begin(), pl.fins.end());
+                na.insert(na.end(), pl.nblks.begin(), pl.nblks.end());
+                maxit = std::max(maxit, mixed ? scenes[id].inner_iter : scenes[id].cg_iter);
+                ls->maxsw = std::max(ls->maxsw, scenes[id].max_sweeps);
             }
             put(ls->a, ta); put(ls->b, fa); put(ls->c, na);
             ls->na = (int)ta.size(); ls->nb = (int)fa.size(); ls->nc = (int)na.size();
         }
+        std::vector<Blk> ob;
+        for (int32_t id : want) {
+            ob.insert(ob.end(), place[id].oblks.begin(), place[id].oblks.end());
+            sids.push_back(sid_of(id));
+        }
+        put(ls->o, ob);
+        ls->no = (int)ob.size();
         void* sc = nullptr;
-        put(sc, std::vector<int>(want.begin(), want.end()));
+        put(sc, sids);
         ls->scenes = (int*)sc;
         ls->maxit = maxit;
         ls->n = n;
@@ -514,6 +989,9 @@ struct EngineT : EngineBase {
 
     // one implicit Euler step of the current launch set, enqueued on `s`
     void launch_step(int n, hipStream_t s) {
+        if constexpr (std::is_same<VT, double>::value) {
+            if (mixed) { launch_step_mixed(n, s); return; }
+        }
         if (tiled) { launch_step_tiled(n, s); return; }
         const ElemView<R, NR> ev = view();
         const NodeArrays na = nodes();
@@ -521,18 +999,22 @@ struct EngineT : EngineBase {
         const Blk* nb = (const Blk*)cur->b;
         if (cur->na > 0) k_rhs_elem<R, NR, VT><<<cur->na, kBlock, 0, s>>>(ev, eb, inst, x, v, ef, ef2);
         if (cur->nb > 0) k_rhs_node<VT><<<cur->nb, kBlock, 0, s>>>(nb, inst, na, ef, ef2, v, f, b, r, xcg, partial);
-        for (int it = 0; it < cur->maxit; it++) {
-            if (cur->na > 0) k_cg_elem<R, NR, VT><<<cur->na, kBlock, 0, s>>>(ev, eb, inst, r, p, ef);
+        for (int it = 1; it <= cur->maxit; it++) {
+            // direction of iteration it in buffer it & 1, the previous one in the other
+            VT* p_new = (it & 1) ? p : p2;
+            const VT* p_old = (it & 1) ? p2 : p;
+            if (cur->na > 0) k_cg_elem<R, NR, VT><<<cur->na, kBlock, 0, s>>>(ev, eb, inst, r, p_old, ef);
             if (cur->nb > 0) {
-                k_cg_node<VT><<<cur->nb, kBlock, 0, s>>>(nb, inst, na, ef, r, p, q, partial);
-                k_cg_update<VT><<<cur->nb, kBlock, 0, s>>>(nb, inst, p, q, xcg, r, partial);
+                k_cg_node<VT><<<cur->nb, kBlock, 0, s>>>(nb, inst, na, ef, r, p_old, p_new, q, partial);
+                k_cg_update<VT><<<cur->nb, kBlock, 0, s>>>(nb, inst, p_new, q, xcg, r, partial);
             }
         }
         if (cur->nb > 0) k_euler_update<VT><<<cur->nb, kBlock, 0, s>>>(nb, inst, xcg, v, x);
+        if (cur->no > 0) k_oscillate<VT><<<cur->no, kBlock, 0, s>>>((const Blk*)cur->o, inst, oscillators(), x, v);
         k_finalize<<<(n + 63) / 64, 64, 0, s>>>(cur->scenes, n, inst);
     }
 
-    void launch_step_tiled(int n, hipStream_t s) {
+    TileArgs<R, NR> tile_args() const {
         TileArgs<R, NR> ta;
         ta.tiles = (const TileDesc*)cur->a;
         ta.inst = inst;
@@ -542,6 +1024,13 @@ struct EngineT : EngineBase {
         ta.tn = TileNodes{ln_gid, ln_inc_ptr, ln_inc, ln_pslot};
         ta.mass = mass;
         ta.fixed = fixed;
+        ta.emass = emass;
+        ta.rot_lo = nullptr;
+        return ta;
+    }
+
+    void launch_step_tiled(int n, hipStream_t s) {
+        const TileArgs<R, NR> ta = tile_args();
         const FinBlk* fb = (const FinBlk*)cur->b;
         const NodeBlk* nb = (const NodeBlk*)cur->c;
         const SharedNodes sn{sh_gid, sh_ptr};
@@ -549,12 +1038,58 @@ struct EngineT : EngineBase {
         if (cur->nb > 0) k_fin_rhs<VT><<<cur->nb, kBlock, 0, s>>>(fb, inst, ranges_d, sn, mass, fixed, v, sp0, sp1, f, b, r, xcg, pbb);
         for (int it = 1; it <= cur->maxit; it++) {
             if (cur->na > 0)
-                k_tile_apply<R, NR, VT><<<cur->na, kBlock, 0, s>>>(ta, it, r, p, q, sp0, it == 1 ? pbb : prho, pden);
+                k_tile_apply<R, NR, VT><<<cur->na, kBlock, 0, s>>>(ta, it, r, p, q, sp0, it == 1 ? pbb : prho, pden, 0);
             if (cur->nb > 0) k_fin_apply<VT><<<cur->nb, kBlock, 0, s>>>(fb, inst, ranges_d, sn, mass, fixed, it, r, p, q, sp0, pden);
             if (cur->nc > 0) k_update<VT><<<cur->nc, kBlock, 0, s>>>(nb, inst, ranges_d, it, p, q, xcg, r, pden, prho);
         }
         if (cur->nc > 0) k_integrate<VT><<<cur->nc, kBlock, 0, s>>>(nb, inst, xcg, v, x);
+        if (cur->no > 0) k_oscillate<VT><<<cur->no, kBlock, 0, s>>>((const Blk*)cur->o, inst, oscillators(), x, v);
         k_step_end_tiled<<<(n + 63) / 64, 64, 0, s>>>(cur->scenes, n, inst);
+    }
+
+    // MIXED step: FP64 RHS (which also writes the float rotations), refinement sweeps with the
+    // FP32 inner CG, FP64 update
+    void launch_step_mixed(int n, hipStream_t s) {
+        TileArgs<R, NR> ta = tile_args();
+        ta.rot_lo = rot32;
+        const FinBlk* fb = (const FinBlk*)cur->b;
+        const NodeBlk* nb = (const NodeBlk*)cur->c;
+        const SharedNodes sn{sh_gid, sh_ptr};
+        InnerCg ic;
+        ic.ta.tiles = ta.tiles;
+        ic.ta.inst = inst_in;
+        ic.ta.ranges = ranges_d;
+        ic.ta.ev = ElemView<float, 6>{tet, rot32, rest32, kmat32, Etot};
+        ic.ta.ltet = ltet;
+        ic.ta.tn = ta.tn;
+        ic.ta.mass = mass;
+        ic.ta.fixed = fixed;
+        ic.ta.emass = emass32;
+        ic.ta.rot_lo = nullptr;
+        ic.fb = fb; ic.nb = nb;
+        ic.na = cur->na; ic.nfb = cur->nb; ic.nnb = cur->nc; ic.maxit = cur->maxit;
+        ic.sn = sn;
+        ic.r = r32; ic.p = p32; ic.q = q32; ic.d = d32; ic.sp = sp32;
+        ic.pden = pden32; ic.prho = prho32;
+        if (cur->na > 0) k_tile_rhs<R, NR, VT><<<cur->na, kBlock, 0, s>>>(ta, x, v, f, b, r, xcg, sp0, sp1, pbb);
+        if (cur->nb > 0) k_fin_rhs<VT><<<cur->nb, kBlock, 0, s>>>(fb, inst, ranges_d, sn, mass, fixed, v, sp0, sp1, f, b, r, xcg, pbb);
+        for (int sw = 1; sw <= cur->maxsw + 1; sw++) {
+            if (sw > 1) {   // q = A x, FP64
+                if (cur->na > 0)
+                    k_tile_apply<R, NR, VT><<<cur->na, kBlock, 0, s>>>(ta, 0, xcg, nullptr, q, sp0, nullptr, pden, 0);
+                if (cur->nb > 0)
+                    k_fin_apply<VT><<<cur->nb, kBlock, 0, s>>>(fb, inst, ranges_d, sn, mass, fixed, 0, xcg, nullptr, q, sp0, pden);
+            }
+            if (cur->nc > 0) k_ir_residual<<<cur->nc, kBlock, 0, s>>>(nb, inst, sw, b, q, r32, d32, prr, prho32);
+            k_ir_control<<<n, kBlock, 0, s>>>(cur->scenes, inst, inst_in, ranges_d, sw, prr);
+            if (sw <= cur->maxsw) {
+                launch_inner_cg(ic, s);
+                if (cur->nc > 0) k_ir_accumulate<<<cur->nc, kBlock, 0, s>>>(nb, inst, inst_in, d32, xcg);
+            }
+        }
+        if (cur->nc > 0) k_integrate<VT><<<cur->nc, kBlock, 0, s>>>(nb, inst, xcg, v, x);
+        if (cur->no > 0) k_oscillate<VT><<<cur->no, kBlock, 0, s>>>((const Blk*)cur->o, inst, oscillators(), x, v);
+        k_step_end_mixed<<<(n + 63) / 64, 64, 0, s>>>(cur->scenes, n, inst, inst_in);
     }
 
     void reset_graph() override {
@@ -596,19 +1131,11 @@ struct EngineT : EngineBase {
         if (!tiled) throw std::invalid_argument("bench_apply needs a tiled engine");
         std::vector<int32_t> ids;
         if (scene >= 0) ids.push_back(scene);
-        else for (int i = 0; i < (int)scenes.size(); i++) ids.push_back(i);
+        else for (int i = 0; i < (int)scenes.size(); i++) if (placed(i)) ids.push_back(i);
         prepare_set(ids.data(), (int)ids.size(), s);
-        TileArgs<R, NR> ta;
-        ta.tiles = (const TileDesc*)cur->a;
-        ta.inst = inst;
-        ta.ranges = ranges_d;
-        ta.ev = view();
-        ta.ltet = ltet;
-        ta.tn = TileNodes{ln_gid, ln_inc_ptr, ln_inc, ln_pslot};
-        ta.mass = mass;
-        ta.fixed = fixed;
+        const TileArgs<R, NR> ta = tile_args();
         for (int i = 0; i < reps; i++)
-            if (cur->na > 0) k_tile_apply<R, NR, VT><<<cur->na, kBlock, 0, s>>>(ta, 2, r, p, q, sp0, prho, pden);
+            if (cur->na > 0) k_tile_apply<R, NR, VT><<<cur->na, kBlock, 0, s>>>(ta, 2, r, p, q, sp0, prho, pden, 0);
         HIPCHK(hipGetLastError());
     }
     // tiles, tile-local nodes, nodes finished by the finalize kernel, shared partial slots,
@@ -619,20 +1146,18 @@ struct EngineT : EngineBase {
         out[5] = (int32_t)(sc.tets.size() / 4);
         out[6] = sc.n_nodes;
         if (!tiled) return;
-        out[0] = tile_count[scene];
-        for (int t = 0; t < tile_count[scene]; t++) out[1] += tiles_h[tile_begin[scene] + t].ln_count;
-        for (int b = 0; b < fin_count[scene]; b++) {
-            const FinBlk& fb = fin_h[fin_begin[scene] + b];
-            out[2] += fb.end - fb.begin;
-        }
-        out[3] = shared_slots[scene];
+        const Place& pl = place[scene];
+        out[0] = (int32_t)pl.tiles.size();
+        for (const TileDesc& td : pl.tiles) out[1] += td.ln_count;
+        for (const FinBlk& fb : pl.fins) out[2] += fb.end - fb.begin;
+        out[3] = (int32_t)pl.size.slot;
         out[4] = 4 * out[5];
     }
 
     void read_vec(const VT* src, int scene, double* out) {
         const SceneHost& sc = scenes[scene];
         std::vector<VT> tmp(3 * (size_t)sc.n_nodes);
-        HIPCHK(hipMemcpy(tmp.data(), src + 3 * (int64_t)sc.node_begin, tmp.size() * sizeof(VT), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(tmp.data(), src + 3 * (int64_t)node_begin(scene), tmp.size() * sizeof(VT), hipMemcpyDeviceToHost));
         for (size_t i = 0; i < tmp.size(); i++) out[i] = (double)tmp[i];
     }
     void read(int scene, double* xo, double* vo, double* fo) override {
@@ -647,28 +1172,28 @@ struct EngineT : EngineBase {
         std::vector<VT> tmp(3 * (size_t)sc.n_nodes);
         if (xi) {
             for (size_t i = 0; i < tmp.size(); i++) tmp[i] = (VT)xi[i];
-            HIPCHK(hipMemcpy(x + 3 * (int64_t)sc.node_begin, tmp.data(), tmp.size() * sizeof(VT), hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(x + 3 * (int64_t)node_begin(scene), tmp.data(), tmp.size() * sizeof(VT), hipMemcpyHostToDevice));
         }
         if (vi) {
             for (size_t i = 0; i < tmp.size(); i++) tmp[i] = (VT)vi[i];
-            HIPCHK(hipMemcpy(v + 3 * (int64_t)sc.node_begin, tmp.data(), tmp.size() * sizeof(VT), hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(v + 3 * (int64_t)node_begin(scene), tmp.data(), tmp.size() * sizeof(VT), hipMemcpyHostToDevice));
         }
     }
     void stats(int scene, sofa_amd_cg_stats* out) override {
         InstDev d;
         HIPCHK(hipDeviceSynchronize());
-        HIPCHK(hipMemcpy(&d, inst + scene, sizeof(d), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(&d, inst + sid_of(scene), sizeof(d), hipMemcpyDeviceToHost));
         out->iterations = d.last_iters;
         out->end_reason = d.end_reason;
         out->step_count = d.step_count;
-        out->ir_sweeps = 0;
+        out->ir_sweeps = mixed ? d.ir_sweeps : 0;
         out->err = d.err;
         out->den = d.den;
         out->time = d.time;
     }
     void element(int scene, int e, double* rot9, double* k3, double* rest12) override {
         HIPCHK(hipDeviceSynchronize());
-        const int64_t ge = scenes[scene].elem_begin + (int64_t)iperm[scene][e];
+        const int64_t ge = elem_begin(scene) + (int64_t)iperm[scene][e];
         R tmp;
         for (int k = 0; k < 9; k++) {
             HIPCHK(hipMemcpy(&tmp, rot + k * Etot + ge, sizeof(R), hipMemcpyDeviceToHost));
@@ -700,30 +1225,28 @@ struct EngineT : EngineBase {
     void add_force(int scene, const void* xd, void* fd, hipStream_t s) override {
         const SceneHost& sc = scenes[scene];
         const int64_t nE = (int64_t)sc.tets.size() / 4;
+        const int nb0 = node_begin(scene);
+        const int64_t eb0 = elem_begin(scene);
         if (nE > 0) {
             const int grid = (int)((nE + kBlock - 1) / kBlock);
-            k_elem_add_force<R, NR, VT><<<grid, kBlock, 0, s>>>(view(), sc.elem_begin, sc.elem_begin + nE,
-                                                               sc.node_begin, (const VT*)xd, ef);
+            k_elem_add_force<R, NR, VT><<<grid, kBlock, 0, s>>>(view(), eb0, eb0 + nE, nb0, (const VT*)xd, ef);
         }
         const int gn = (sc.n_nodes + kBlock - 1) / kBlock;
-        if (gn > 0)
-            k_gather<VT><<<gn, kBlock, 0, s>>>(inc_ptr, inc, sc.node_begin, sc.node_begin + sc.n_nodes, ef,
-                                               (VT*)fd, 0);
+        if (gn > 0) k_gather<VT><<<gn, kBlock, 0, s>>>(inc_ptr, inc, nb0, nb0 + sc.n_nodes, ef, (VT*)fd, 0);
         HIPCHK(hipGetLastError());
     }
     void add_dforce(int scene, const void* dxd, void* dfd, double kf, hipStream_t s) override {
         const SceneHost& sc = scenes[scene];
         const int64_t nE = (int64_t)sc.tets.size() / 4;
+        const int nb0 = node_begin(scene);
+        const int64_t eb0 = elem_begin(scene);
         if (nE > 0) {
             const int grid = (int)((nE + kBlock - 1) / kBlock);
             // addDForce: kFactor = (Real) mparams->kFactor(); applied as `KJtD *= fact`
-            k_elem_add_dforce<R, NR, VT><<<grid, kBlock, 0, s>>>(view(), sc.elem_begin, sc.elem_begin + nE,
-                                                                sc.node_begin, (const VT*)dxd, (R)kf, ef);
+            k_elem_add_dforce<R, NR, VT><<<grid, kBlock, 0, s>>>(view(), eb0, eb0 + nE, nb0, (const VT*)dxd, (R)kf, ef);
         }
         const int gn = (sc.n_nodes + kBlock - 1) / kBlock;
-        if (gn > 0)
-            k_gather<VT><<<gn, kBlock, 0, s>>>(inc_ptr, inc, sc.node_begin, sc.node_begin + sc.n_nodes, ef,
-                                               (VT*)dfd, 1);
+        if (gn > 0) k_gather<VT><<<gn, kBlock, 0, s>>>(inc_ptr, inc, nb0, nb0 + sc.n_nodes, ef, (VT*)dfd, 1);
         HIPCHK(hipGetLastError());
     }
 };
@@ -733,6 +1256,17 @@ EngineBase* make_engine_fp32() {
     auto* e = new EngineT<float, 6, float>();
     e->tiled = true;
     return e;
+}
+
+void launch_inner_cg(const InnerCg& c, hipStream_t s) {
+    for (int it = 1; it <= c.maxit; it++) {
+        if (c.na > 0) k_tile_apply<float, 6, float><<<c.na, kBlock, 0, s>>>(c.ta, it, c.r, c.p, c.q, c.sp, c.prho, c.pden, 1);
+        if (c.nfb > 0)
+            k_fin_apply<float><<<c.nfb, kBlock, 0, s>>>(c.fb, c.ta.inst, c.ta.ranges, c.sn, c.ta.mass, c.ta.fixed, it, c.r,
+                                                         c.p, c.q, c.sp, c.pden);
+        if (c.nnb > 0)
+            k_update<float><<<c.nnb, kBlock, 0, s>>>(c.nb, c.ta.inst, c.ta.ranges, it, c.p, c.q, c.d, c.r, c.pden, c.prho);
+    }
 }
 #else
 EngineBase* make_engine_fp32();
@@ -763,6 +1297,20 @@ static SceneHost* scene_of(sofa_amd_engine eng, int scene) {
     if (!eng || !eng->impl) return nullptr;
     if (scene < 0 || scene >= (int)eng->impl->scenes.size()) return nullptr;
     return &eng->impl->scenes[scene];
+}
+
+// a scene's parameters can change until it is placed on the device (engine commit, or its own
+// commit on a live engine)
+static bool editable(sofa_amd_engine eng, int scene) {
+    EngineBase* e = eng->impl.get();
+    if (e->live_mode) return !e->placed(scene);
+    return !e->committed;
+}
+
+// a scene can be stepped / read once it is placed
+static bool usable(sofa_amd_engine eng, int scene) {
+    EngineBase* e = eng->impl.get();
+    return e->committed && e->placed(scene);
 }
 
 extern "C" {
@@ -796,6 +1344,11 @@ int sofa_amd_engine_create(int device, int precision, sofa_amd_engine* out) {
         impl.reset(e);
     } else if (precision == SOFA_AMD_FP32) {
         impl.reset(make_engine_fp32());
+    } else if (precision == SOFA_AMD_MIXED) {
+        auto* e = new EngineT<double, 9, double>();
+        e->tiled = true;
+        e->mixed = true;
+        impl.reset(e);
     } else {
         return fail(SOFA_AMD_EINVAL, "unsupported precision mode");
     }
@@ -826,7 +1379,7 @@ int sofa_amd_engine_destroy(sofa_amd_engine eng) {
 int sofa_amd_scene_create(sofa_amd_engine eng, int n_nodes, const double* positions, int* out_scene) {
     GUARD_BEGIN
     if (!eng || !eng->impl || !positions || !out_scene || n_nodes <= 0) return fail(SOFA_AMD_EINVAL, "invalid argument");
-    if (eng->impl->committed) return fail(SOFA_AMD_ESTATE, "engine already committed");
+    if (eng->impl->committed && !eng->impl->live_mode) return fail(SOFA_AMD_ESTATE, "engine already committed");
     SceneHost s;
     s.n_nodes = n_nodes;
     s.rest.assign(positions, positions + 3 * (size_t)n_nodes);
@@ -843,7 +1396,7 @@ int sofa_amd_scene_set_fem(sofa_amd_engine eng, int scene, int n_tets, const int
     GUARD_BEGIN
     SceneHost* s = scene_of(eng, scene);
     if (!s || n_tets < 0 || (n_tets > 0 && !tets)) return fail(SOFA_AMD_EINVAL, "invalid argument");
-    if (eng->impl->committed) return fail(SOFA_AMD_ESTATE, "engine already committed");
+    if (!editable(eng, scene)) return fail(SOFA_AMD_ESTATE, "scene already committed");
     for (int i = 0; i < 4 * n_tets; i++)
         if (tets[i] < 0 || tets[i] >= s->n_nodes) return fail(SOFA_AMD_EINVAL, "tetrahedron index out of range");
     s->tets.assign(tets, tets + 4 * (size_t)n_tets);
@@ -858,18 +1411,48 @@ int sofa_amd_scene_set_fem(sofa_amd_engine eng, int scene, int n_tets, const int
 
 int sofa_amd_scene_set_mass(sofa_amd_engine eng, int scene, int kind, double value) {
     SceneHost* s = scene_of(eng, scene);
-    if (!s || (kind != SOFA_AMD_MASS_UNIFORM && kind != SOFA_AMD_MASS_DIAGONAL) || !(value > 0))
+    if (!s || kind < SOFA_AMD_MASS_UNIFORM || kind > SOFA_AMD_MASS_MESHMATRIX_LUMPED || !(value > 0))
         return fail(SOFA_AMD_EINVAL, "invalid mass");
-    if (eng->impl->committed) return fail(SOFA_AMD_ESTATE, "engine already committed");
+    if (!editable(eng, scene)) return fail(SOFA_AMD_ESTATE, "scene already committed");
     s->mass_kind = kind;
     s->mass_value = value;
+    s->density.assign(1, value);
+    return SOFA_AMD_OK;
+}
+
+int sofa_amd_scene_set_mass_density(sofa_amd_engine eng, int scene, int kind, const double* density, int n) {
+    SceneHost* s = scene_of(eng, scene);
+    if (!s || (kind != SOFA_AMD_MASS_MESHMATRIX && kind != SOFA_AMD_MASS_MESHMATRIX_LUMPED) || !density || n <= 0)
+        return fail(SOFA_AMD_EINVAL, "invalid mass density");
+    if (!editable(eng, scene)) return fail(SOFA_AMD_ESTATE, "scene already committed");
+    for (int i = 0; i < n; i++)
+        if (!(density[i] > 0)) return fail(SOFA_AMD_EINVAL, "non-positive mass density");
+    s->mass_kind = kind;
+    s->mass_value = density[0];
+    s->density.assign(density, density + n);
+    return SOFA_AMD_OK;
+}
+
+int sofa_amd_scene_add_oscillators(sofa_amd_engine eng, int scene, int n, const int32_t* indices, const double* mean,
+                                   const double* amplitude, const double* pulsation, const double* phase) {
+    SceneHost* s = scene_of(eng, scene);
+    if (!s || n < 0 || (n > 0 && (!indices || !mean || !amplitude || !pulsation || !phase)))
+        return fail(SOFA_AMD_EINVAL, "invalid argument");
+    if (!editable(eng, scene)) return fail(SOFA_AMD_ESTATE, "scene already committed");
+    for (int i = 0; i < n; i++) {
+        if (indices[i] < 0 || indices[i] >= s->n_nodes) return fail(SOFA_AMD_EINVAL, "oscillator index out of range");
+        s->osc_idx.push_back(indices[i]);
+        for (int k = 0; k < 3; k++) { s->osc_mean.push_back(mean[3 * i + k]); s->osc_amp.push_back(amplitude[3 * i + k]); }
+        s->osc_w.push_back(pulsation[i]);
+        s->osc_phase.push_back(phase[i]);
+    }
     return SOFA_AMD_OK;
 }
 
 int sofa_amd_scene_set_gravity(sofa_amd_engine eng, int scene, double gx, double gy, double gz) {
     SceneHost* s = scene_of(eng, scene);
     if (!s) return fail(SOFA_AMD_EINVAL, "invalid scene");
-    if (eng->impl->committed) return fail(SOFA_AMD_ESTATE, "engine already committed");
+    if (!editable(eng, scene)) return fail(SOFA_AMD_ESTATE, "scene already committed");
     s->g[0] = gx; s->g[1] = gy; s->g[2] = gz;
     return SOFA_AMD_OK;
 }
@@ -877,7 +1460,7 @@ int sofa_amd_scene_set_gravity(sofa_amd_engine eng, int scene, double gx, double
 int sofa_amd_scene_set_fixed(sofa_amd_engine eng, int scene, const int32_t* indices, int n) {
     SceneHost* s = scene_of(eng, scene);
     if (!s || n < 0 || (n > 0 && !indices)) return fail(SOFA_AMD_EINVAL, "invalid argument");
-    if (eng->impl->committed) return fail(SOFA_AMD_ESTATE, "engine already committed");
+    if (!editable(eng, scene)) return fail(SOFA_AMD_ESTATE, "scene already committed");
     for (int i = 0; i < n; i++)
         if (indices[i] < 0 || indices[i] >= s->n_nodes) return fail(SOFA_AMD_EINVAL, "fixed index out of range");
     s->fixed.assign(indices, indices + n);
@@ -888,8 +1471,20 @@ int sofa_amd_scene_set_solver(sofa_amd_engine eng, int scene, double dt, double 
                               double tol, double thr) {
     SceneHost* s = scene_of(eng, scene);
     if (!s || !(dt > 0) || iters < 0) return fail(SOFA_AMD_EINVAL, "invalid solver parameters");
-    if (eng->impl->committed) return fail(SOFA_AMD_ESTATE, "engine already committed");
+    if (!editable(eng, scene)) return fail(SOFA_AMD_ESTATE, "scene already committed");
     s->dt = dt; s->rs = rs; s->rm = rm; s->cg_iter = iters; s->tol = tol; s->thr = thr;
+    return SOFA_AMD_OK;
+}
+
+int sofa_amd_scene_set_refinement(sofa_amd_engine eng, int scene, int inner_iterations, double inner_tolerance,
+                                  int max_sweeps) {
+    SceneHost* s = scene_of(eng, scene);
+    if (!s || inner_iterations < 1 || max_sweeps < 1 || !(inner_tolerance >= 0))
+        return fail(SOFA_AMD_EINVAL, "invalid refinement parameters");
+    if (!editable(eng, scene)) return fail(SOFA_AMD_ESTATE, "scene already committed");
+    s->inner_iter = inner_iterations;
+    s->inner_tol = inner_tolerance;
+    s->max_sweeps = max_sweeps;
     return SOFA_AMD_OK;
 }
 
@@ -898,6 +1493,7 @@ int sofa_amd_engine_commit(sofa_amd_engine eng) {
     if (!eng || !eng->impl) return fail(SOFA_AMD_EINVAL, "null engine");
     if (eng->impl->committed) return fail(SOFA_AMD_ESTATE, "engine already committed");
     if (eng->impl->scenes.empty()) return fail(SOFA_AMD_EINVAL, "no scene");
+    if (eng->impl->live_mode) return fail(SOFA_AMD_ESTATE, "live engine: commit scenes one by one");
     eng->impl->commit();
     return SOFA_AMD_OK;
     GUARD_END
@@ -909,6 +1505,7 @@ int sofa_amd_engine_step(sofa_amd_engine eng, const int32_t* scenes, int n_scene
     if (!eng->impl->committed) return fail(SOFA_AMD_ESTATE, "engine not committed");
     for (int i = 0; i < n_scenes; i++) {
         if (scenes[i] < 0 || scenes[i] >= (int)eng->impl->scenes.size()) return fail(SOFA_AMD_EINVAL, "invalid scene id");
+        if (!eng->impl->placed(scenes[i])) return fail(SOFA_AMD_ESTATE, "scene not committed");
         for (int j = 0; j < i; j++)
             if (scenes[j] == scenes[i]) return fail(SOFA_AMD_EINVAL, "duplicate scene id");
     }
@@ -931,7 +1528,7 @@ int sofa_amd_engine_sync(sofa_amd_engine eng) {
 int sofa_amd_scene_read(sofa_amd_engine eng, int scene, double* x, double* v, double* f) {
     GUARD_BEGIN
     if (!scene_of(eng, scene)) return fail(SOFA_AMD_EINVAL, "invalid scene");
-    if (!eng->impl->committed) return fail(SOFA_AMD_ESTATE, "engine not committed");
+    if (!usable(eng, scene)) return fail(SOFA_AMD_ESTATE, "scene not committed");
     eng->impl->read(scene, x, v, f);
     return SOFA_AMD_OK;
     GUARD_END
@@ -940,7 +1537,7 @@ int sofa_amd_scene_read(sofa_amd_engine eng, int scene, double* x, double* v, do
 int sofa_amd_scene_write(sofa_amd_engine eng, int scene, const double* x, const double* v) {
     GUARD_BEGIN
     if (!scene_of(eng, scene)) return fail(SOFA_AMD_EINVAL, "invalid scene");
-    if (!eng->impl->committed) return fail(SOFA_AMD_ESTATE, "engine not committed");
+    if (!usable(eng, scene)) return fail(SOFA_AMD_ESTATE, "scene not committed");
     eng->impl->write(scene, x, v);
     return SOFA_AMD_OK;
     GUARD_END
@@ -949,7 +1546,7 @@ int sofa_amd_scene_write(sofa_amd_engine eng, int scene, const double* x, const 
 int sofa_amd_scene_stats(sofa_amd_engine eng, int scene, sofa_amd_cg_stats* out) {
     GUARD_BEGIN
     if (!scene_of(eng, scene) || !out) return fail(SOFA_AMD_EINVAL, "invalid argument");
-    if (!eng->impl->committed) return fail(SOFA_AMD_ESTATE, "engine not committed");
+    if (!usable(eng, scene)) return fail(SOFA_AMD_ESTATE, "scene not committed");
     eng->impl->stats(scene, out);
     return SOFA_AMD_OK;
     GUARD_END
@@ -960,7 +1557,7 @@ int sofa_amd_scene_element(sofa_amd_engine eng, int scene, int elem, double* rot
     SceneHost* s = scene_of(eng, scene);
     if (!s || !rot9 || !k3 || !rest12 || elem < 0 || elem >= (int)(s->tets.size() / 4))
         return fail(SOFA_AMD_EINVAL, "invalid argument");
-    if (!eng->impl->committed) return fail(SOFA_AMD_ESTATE, "engine not committed");
+    if (!usable(eng, scene)) return fail(SOFA_AMD_ESTATE, "scene not committed");
     eng->impl->element(scene, elem, rot9, k3, rest12);
     return SOFA_AMD_OK;
     GUARD_END
@@ -969,7 +1566,7 @@ int sofa_amd_scene_element(sofa_amd_engine eng, int scene, int elem, double* rot
 int sofa_amd_fem_add_force(sofa_amd_engine eng, int scene, const void* x_dev, void* f_dev, void* stream) {
     GUARD_BEGIN
     if (!scene_of(eng, scene) || !x_dev || !f_dev) return fail(SOFA_AMD_EINVAL, "invalid argument");
-    if (!eng->impl->committed) return fail(SOFA_AMD_ESTATE, "engine not committed");
+    if (!usable(eng, scene)) return fail(SOFA_AMD_ESTATE, "scene not committed");
     HIPCHK(hipSetDevice(eng->impl->device));
     eng->impl->add_force(scene, x_dev, f_dev, stream ? (hipStream_t)stream : eng->impl->stream);
     return SOFA_AMD_OK;
@@ -980,7 +1577,7 @@ int sofa_amd_fem_add_dforce(sofa_amd_engine eng, int scene, const void* dx_dev, 
                             void* stream) {
     GUARD_BEGIN
     if (!scene_of(eng, scene) || !dx_dev || !df_dev) return fail(SOFA_AMD_EINVAL, "invalid argument");
-    if (!eng->impl->committed) return fail(SOFA_AMD_ESTATE, "engine not committed");
+    if (!usable(eng, scene)) return fail(SOFA_AMD_ESTATE, "scene not committed");
     HIPCHK(hipSetDevice(eng->impl->device));
     eng->impl->add_dforce(scene, dx_dev, df_dev, k_factor, stream ? (hipStream_t)stream : eng->impl->stream);
     return SOFA_AMD_OK;
@@ -990,7 +1587,8 @@ int sofa_amd_fem_add_dforce(sofa_amd_engine eng, int scene, const void* dx_dev, 
 int sofa_amd_engine_bench_apply(sofa_amd_engine eng, int scene, int reps, void* stream) {
     GUARD_BEGIN
     if (!eng || !eng->impl || (scene >= 0 && !scene_of(eng, scene)) || reps < 0) return fail(SOFA_AMD_EINVAL, "invalid argument");
-    if (!eng->impl->committed) return fail(SOFA_AMD_ESTATE, "engine not committed");
+    if (!eng->impl->committed || (scene >= 0 && !eng->impl->placed(scene)))
+        return fail(SOFA_AMD_ESTATE, "scene not committed");
     HIPCHK(hipSetDevice(eng->impl->device));
     eng->impl->bench_apply(scene, reps, stream ? (hipStream_t)stream : eng->impl->stream);
     return SOFA_AMD_OK;
@@ -1000,8 +1598,74 @@ int sofa_amd_engine_bench_apply(sofa_amd_engine eng, int scene, int reps, void* 
 int sofa_amd_scene_tiling(sofa_amd_engine eng, int scene, int32_t* out7) {
     GUARD_BEGIN
     if (!scene_of(eng, scene) || !out7) return fail(SOFA_AMD_EINVAL, "invalid argument");
-    if (!eng->impl->committed) return fail(SOFA_AMD_ESTATE, "engine not committed");
+    if (!usable(eng, scene)) return fail(SOFA_AMD_ESTATE, "scene not committed");
     eng->impl->tiling(scene, out7);
+    return SOFA_AMD_OK;
+    GUARD_END
+}
+
+int sofa_amd_engine_reserve(sofa_amd_engine eng, int64_t max_nodes, int64_t max_tets, int max_scenes) {
+    GUARD_BEGIN
+    if (!eng || !eng->impl || max_nodes <= 0 || max_tets < 0 || max_scenes <= 0)
+        return fail(SOFA_AMD_EINVAL, "invalid capacity");
+    EngineBase* e = eng->impl.get();
+    if (!e->live()) return fail(SOFA_AMD_EINVAL, "live scenes need a tiled engine (FP32, FP64_TILED or MIXED)");
+    if (e->committed) return fail(SOFA_AMD_ESTATE, "engine already committed");
+    Extent c;
+    c.node = max_nodes + max_scenes;
+    c.elem = max_tets;
+    c.inc = 4 * max_tets;
+    c.tile = max_tets / 128 + max_scenes;
+    c.lnode = 2 * max_nodes + c.tile;
+    c.shn = max_nodes + max_scenes;
+    c.slot = 2 * max_nodes;
+    c.fin = max_nodes / kBlock + max_scenes;
+    c.nblk = max_nodes / (kBlock * kNodesPerThread) + max_scenes;
+    c.osc = max_nodes;
+    c.sid = max_scenes;
+    e->reserve(c);
+    e->committed = true;
+    e->live_mode = true;
+    return SOFA_AMD_OK;
+    GUARD_END
+}
+
+int sofa_amd_scene_commit(sofa_amd_engine eng, int scene) {
+    GUARD_BEGIN
+    if (!scene_of(eng, scene)) return fail(SOFA_AMD_EINVAL, "invalid scene");
+    EngineBase* e = eng->impl.get();
+    if (!e->live_mode) return fail(SOFA_AMD_ESTATE, "engine not reserved (sofa_amd_engine_reserve)");
+    if (e->placed(scene)) return fail(SOFA_AMD_ESTATE, "scene already committed");
+    HIPCHK(hipSetDevice(e->device));
+    if (!e->commit_scene(scene)) return fail(SOFA_AMD_ENOMEM, "no room for the scene in the engine's reservation");
+    return SOFA_AMD_OK;
+    GUARD_END
+}
+
+int sofa_amd_scene_release(sofa_amd_engine eng, int scene) {
+    GUARD_BEGIN
+    if (!scene_of(eng, scene)) return fail(SOFA_AMD_EINVAL, "invalid scene");
+    EngineBase* e = eng->impl.get();
+    if (!e->live_mode) return fail(SOFA_AMD_ESTATE, "engine not reserved (sofa_amd_engine_reserve)");
+    if (!e->placed(scene)) return fail(SOFA_AMD_ESTATE, "scene not committed");
+    HIPCHK(hipSetDevice(e->device));
+    e->release_scene(scene);
+    SceneHost& s = e->scenes[scene];
+    s.rest.clear(); s.rest.shrink_to_fit();
+    s.tets.clear(); s.tets.shrink_to_fit();
+    s.young.assign(1, 5000.0); s.poisson.assign(1, 0.45); s.density.assign(1, 1.0);
+    s.osc_idx.clear(); s.osc_mean.clear(); s.osc_amp.clear(); s.osc_w.clear(); s.osc_phase.clear();
+    s.fixed.clear();
+    s.released = true;
+    return SOFA_AMD_OK;
+    GUARD_END
+}
+
+int sofa_amd_engine_usage(sofa_amd_engine eng, int64_t* out22) {
+    GUARD_BEGIN
+    if (!eng || !eng->impl || !out22) return fail(SOFA_AMD_EINVAL, "invalid argument");
+    if (!eng->impl->live_mode) return fail(SOFA_AMD_ESTATE, "engine not reserved (sofa_amd_engine_reserve)");
+    eng->impl->usage(out22);
     return SOFA_AMD_OK;
     GUARD_END
 }

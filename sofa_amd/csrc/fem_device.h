@@ -55,6 +55,11 @@ struct InstDev {
     unsigned int counter;             // arrivals of the current reduction (exact path)
     int pad_;
     double rho_slot[2];               // rho of iterations k-1 and k (tiled path), by k & 1
+    // iterative refinement (MIXED engines; the scene record of the inner FP32 CG is separate)
+    int max_sweeps;                   // inner solves per step at most
+    int ir_sweeps;                    // inner solves done in the last step
+    int cg_total;                     // inner CG iterations summed over the sweeps of a step
+    int pad2_;
 };
 
 struct Blk { int inst, begin, end, pad; };   // a block's scene and [begin, end) range

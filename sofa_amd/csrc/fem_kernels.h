@@ -26,19 +26,26 @@ namespace sofa_amd {
 struct NodeArrays {
     const int* __restrict__ inc_ptr;   // per node [inc_ptr[n], inc_ptr[n+1]) in inc
     const int* __restrict__ inc;       // element*4 + local vertex, ascending element id
-    const double* __restrict__ mass;   // per-node mass (DiagonalMass) or unused
-    const int8_t* __restrict__ fixed;  // 1 where FixedProjectiveConstraint projects to 0
+    const double* __restrict__ mass;   // per-node mass (DiagonalMass, MeshMatrixMass vertex mass)
+    const int8_t* __restrict__ fixed;  // 1 where a projective constraint clears the response
+    const int* __restrict__ em_ptr;    // MeshMatrixMass edges of each node, ascending edge id:
+    const int* __restrict__ em_nbr;    //   the other end
+    const double* __restrict__ em_val; //   the edge mass rho V / 20 summed over the edge's tets
 };
 
-// mass term of addMDx: UniformMass.inl:403-419 and DiagonalMass.inl:536-559
+// mass kinds of a scene
+constexpr int kMassUniform = 0, kMassDiagonal = 1, kMassMatrix = 2, kMassMatrixLumped = 3;
+constexpr double kLumpTet = 2.5;   // MeshMatrixMass lumping coefficient of tetrahedra (MeshMatrixMass.inl:1476-1489)
+
+// vertex part of addMDx: UniformMass.inl:403-419, DiagonalMass.inl:536-559, MeshMatrixMass.inl:1987-2030
 template <typename VT>
 __device__ __forceinline__ void mass_mdx(const InstDev& in, const NodeArrays& na, int n, double factor,
                                          const VT d[3], VT out[3]) {
-    if (in.mass_kind == 0) {
+    if (in.mass_kind == kMassUniform) {
         VT m = (VT)in.vertex_mass;
         if (factor != 1.0) m *= (VT)factor;
         out[0] += d[0] * m; out[1] += d[1] * m; out[2] += d[2] * m;
-    } else {
+    } else if (in.mass_kind == kMassDiagonal) {
         const VT m = (VT)na.mass[n];
         if (factor == 1.0) {
             out[0] += d[0] * m; out[1] += d[1] * m; out[2] += d[2] * m;
@@ -46,6 +53,44 @@ __device__ __forceinline__ void mass_mdx(const InstDev& in, const NodeArrays& na
             const VT fct = (VT)factor;
             out[0] += (d[0] * m) * fct; out[1] += (d[1] * m) * fct; out[2] += (d[2] * m) * fct;
         }
+    } else if (in.mass_kind == kMassMatrix) {
+        const VT m = (VT)na.mass[n], fct = (VT)factor;
+        out[0] += (d[0] * m) * fct; out[1] += (d[1] * m) * fct; out[2] += (d[2] * m) * fct;
+    } else {
+        const VT m = (VT)na.mass[n], fct = (VT)factor, lc = (VT)kLumpTet;
+        out[0] += ((d[0] * m) * lc) * fct; out[1] += ((d[1] * m) * lc) * fct; out[2] += ((d[2] * m) * lc) * fct;
+    }
+}
+
+// gravity of the mass's addForce: UniformMass.inl:471-496 (m g), DiagonalMass.inl:1393-1413
+// (g m_i), MeshMatrixMass.inl:2073-2092 (g m_i 2.5)
+template <typename VT>
+__device__ __forceinline__ void mass_gravity(const InstDev& in, const double* __restrict__ mass, int n, VT f[3]) {
+    if (in.mass_kind == kMassUniform) {
+        f[0] = (VT)0 + (VT)in.mg[0]; f[1] = (VT)0 + (VT)in.mg[1]; f[2] = (VT)0 + (VT)in.mg[2];
+    } else if (in.mass_kind == kMassDiagonal) {
+        const VT m = (VT)mass[n];
+#pragma unroll
+        for (int i = 0; i < 3; i++) f[i] = (VT)0 + (VT)in.gravity[i] * m;
+    } else {
+        const VT m = (VT)mass[n], lc = (VT)kLumpTet;
+#pragma unroll
+        for (int i = 0; i < 3; i++) f[i] = (VT)0 + ((VT)in.gravity[i] * m) * lc;
+    }
+}
+
+// edge part of the consistent MeshMatrixMass addMDx (MeshMatrixMass.inl:2014-2026): for each
+// edge of node n in ascending edge id, out += d[other] * (edgeMass * factor); `dof(j, v)` gives
+// the value of the multiplied vector at node j
+template <typename VT, typename F>
+__device__ __forceinline__ void mass_edges(const NodeArrays& na, int n, double factor, F dof, VT out[3]) {
+    const int s0 = na.em_ptr[n], s1 = na.em_ptr[n + 1];
+    const VT fct = (VT)factor;
+    for (int s = s0; s < s1; s++) {
+        const VT tm = (VT)na.em_val[s] * fct;
+        VT d[3];
+        dof(na.em_nbr[s], d);
+        out[0] += d[0] * tm; out[1] += d[1] * tm; out[2] += d[2] * tm;
     }
 }
 
@@ -110,12 +155,7 @@ __global__ __launch_bounds__(kBlock) void k_rhs_node(const Blk* __restrict__ nbl
         const InstDev& in = *ins;
         // computeForce: f = 0; mass addForce (f += m g); FEM addForce (f += R F, element order)
         VT fn[3];
-        if (in.mass_kind == 0) {
-            fn[0] = (VT)0 + (VT)in.mg[0]; fn[1] = (VT)0 + (VT)in.mg[1]; fn[2] = (VT)0 + (VT)in.mg[2];
-        } else {
-            const VT m = (VT)na.mass[n];
-            fn[0] = (VT)0 + (VT)in.gravity[0] * m; fn[1] = (VT)0 + (VT)in.gravity[1] * m; fn[2] = (VT)0 + (VT)in.gravity[2] * m;
-        }
+        mass_gravity(in, na.mass, n, fn);
         const int s0 = na.inc_ptr[n], s1 = na.inc_ptr[n + 1];
         for (int s = s0; s < s1; s++) {
             const int64_t slot = na.inc[s];
@@ -128,6 +168,8 @@ __global__ __launch_bounds__(kBlock) void k_rhs_node(const Blk* __restrict__ nbl
             VT vn[3];
             load3(v, n, vn[0], vn[1], vn[2]);
             mass_mdx(in, na, n, in.m_rhs, vn, bn);
+            if (in.mass_kind == kMassMatrix)
+                mass_edges(na, n, in.m_rhs, [&](int j, VT d[3]) { load3(v, j, d[0], d[1], d[2]); }, bn);
         }
         for (int s = s0; s < s1; s++) {
             const int64_t slot = na.inc[s];
@@ -170,7 +212,7 @@ __global__ __launch_bounds__(kBlock) void k_rhs_node(const Blk* __restrict__ nbl
 template <typename R, int NR, typename VT>
 __global__ __launch_bounds__(kBlock) void k_cg_elem(ElemView<R, NR> ev, const Blk* __restrict__ eblk,
                                                      const InstDev* __restrict__ inst, const VT* __restrict__ r,
-                                                     const VT* __restrict__ p, VT* __restrict__ ef) {
+                                                     const VT* __restrict__ p_old, VT* __restrict__ ef) {
     const Blk bk = eblk[blockIdx.x];
     const InstDev& in = inst[bk.inst];
     if (in.stopped) return;
@@ -188,7 +230,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_elem(ElemView<R, NR> ev, const Bl
         if (!first) {
             // cgstep_beta, GraphScattered: p *= beta; p += r (CGLinearSolver.cpp:40-44)
             VT p0, p1, p2;
-            load3(p, nid[n], p0, p1, p2);
+            load3(p_old, nid[n], p0, p1, p2);
             p0 *= beta; p0 += r0;
             p1 *= beta; p1 += r1;
             p2 *= beta; p2 += r2;
@@ -206,8 +248,9 @@ __global__ __launch_bounds__(kBlock) void k_cg_elem(ElemView<R, NR> ev, const Bl
 template <typename VT>
 __global__ __launch_bounds__(kBlock) void k_cg_node(const Blk* __restrict__ nblk, InstDev* __restrict__ inst,
                                                      NodeArrays na, const VT* __restrict__ ef,
-                                                     const VT* __restrict__ r, VT* __restrict__ p,
-                                                     VT* __restrict__ q, double* __restrict__ partial) {
+                                                     const VT* __restrict__ r, const VT* __restrict__ p_old,
+                                                     VT* __restrict__ p, VT* __restrict__ q,
+                                                     double* __restrict__ partial) {
     __shared__ double sh[kBlock];
     const Blk bk = nblk[blockIdx.x];
     InstDev* ins = &inst[bk.inst];
@@ -217,17 +260,27 @@ __global__ __launch_bounds__(kBlock) void k_cg_node(const Blk* __restrict__ nblk
     if (n < bk.end) {
         const InstDev& in = *ins;
         const int64_t o = 3 * (int64_t)n;
-        VT pn[3] = {r[o], r[o + 1], r[o + 2]};
-        if (in.iter != 1) {
-            const VT beta = (VT)in.beta;
-            VT po[3] = {p[o], p[o + 1], p[o + 2]};
+        const bool first = in.iter == 1;
+        const VT beta = (VT)in.beta;
+        // p = r (first iteration) or p *= beta; p += r (cgstep_beta), from the previous p
+        auto pdir = [&](int j, VT d[3]) {
+            load3(r, j, d[0], d[1], d[2]);
+            if (!first) {
+                VT po[3];
+                load3(p_old, j, po[0], po[1], po[2]);
 #pragma unroll
-            for (int k = 0; k < 3; k++) { po[k] *= beta; po[k] += pn[k]; pn[k] = po[k]; }
-        }
+                for (int k = 0; k < 3; k++) { po[k] *= beta; po[k] += d[k]; d[k] = po[k]; }
+            }
+        };
+        VT pn[3];
+        pdir(n, pn);
         p[o] = pn[0]; p[o + 1] = pn[1]; p[o + 2] = pn[2];
         // q = 0; mass addMDx(q, p, 1 + h rm); FEM addDForce(q, p, -h(h+rs)); projectResponse(q)
         VT qn[3] = {0, 0, 0};
-        if (in.m_cg != 0.0) mass_mdx(in, na, n, in.m_cg, pn, qn);
+        if (in.m_cg != 0.0) {
+            mass_mdx(in, na, n, in.m_cg, pn, qn);
+            if (in.mass_kind == kMassMatrix) mass_edges(na, n, in.m_cg, pdir, qn);
+        }
         const int s0 = na.inc_ptr[n], s1 = na.inc_ptr[n + 1];
         for (int s = s0; s < s1; s++) {
             const int64_t slot = na.inc[s];
@@ -330,6 +383,36 @@ static __global__ void k_finalize(const int* __restrict__ scenes, int n, InstDev
     in->first_solve = 0;
     in->step_count += 1;
     in->time += in->dt;
+}
+
+// ---------------------------------------------------------------------------------------
+// OscillatorProjectiveConstraint::projectPosition / projectVelocity (OscillatorProjective
+// Constraint.inl:77-106) at the context time of the step start: x = mean + a sin(w t + phi),
+// v = a w cos(w t + phi)
+// ---------------------------------------------------------------------------------------
+struct Oscillators {
+    const int* __restrict__ node;        // global node id
+    const double* __restrict__ mean;     // 3 per entry
+    const double* __restrict__ amp;      // 3 per entry
+    const double* __restrict__ w;        // pulsation
+    const double* __restrict__ phase;
+};
+
+template <typename VT>
+__global__ __launch_bounds__(kBlock) void k_oscillate(const Blk* __restrict__ oblk, const InstDev* __restrict__ inst,
+                                                       Oscillators os, VT* __restrict__ x, VT* __restrict__ v) {
+    const Blk bk = oblk[blockIdx.x];
+    const int i = bk.begin + threadIdx.x;
+    if (i >= bk.end) return;
+    const double t = inst[bk.inst].time;
+    const double w = os.w[i], ph = os.phase[i];
+    const double sn = sin(w * t + ph), cs = cos(w * t + ph);
+    const int64_t o = 3 * (int64_t)os.node[i];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        x[o + k] = (VT)(os.mean[3 * i + k] + os.amp[3 * i + k] * sn);
+        v[o + k] = (VT)((os.amp[3 * i + k] * w) * cs);
+    }
 }
 
 // ---------------------------------------------------------------------------------------

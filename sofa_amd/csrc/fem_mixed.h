@@ -1,0 +1,132 @@
+// fem_mixed.h -- iterative refinement for the MIXED precision mode.
+//
+// The implicit Euler system A dv = b (A = (1 + h rm) M - h (h + rs) K, b from the FP64 RHS
+// kernels) is solved by refinement sweeps: with the FP64 correction sum x (x = 0 at the start),
+//   r = b - A x            FP64 operator on the FP64 element record (k_tile_apply, iter 0)
+//   stop if |r| / |b| <= tolerance, or after max_sweeps inner solves
+//   A d = (float) r        FP32 CG (the FP32 tiled kernels, a scene record of its own)
+//   x += d                 FP64
+// Every sweep is launched for every scene of the launch set; a scene that has converged skips
+// the work (the kernels test its `stopped` flag), so the step stays one fixed launch sequence
+// that can be captured in a graph.  Reductions are per-block partials re-reduced in a fixed
+// order, as on the tiled path; nothing here uses atomics.
+#pragma once
+
+#include "fem_tiled.h"
+
+namespace sofa_amd {
+
+// r = b (sweep 1) or b - A x (q = A x from the FP64 operator); r32 = (float) r; d = 0.
+// Partials of |r|^2 (FP64 values) and |r32|^2 per node block.
+static __global__ __launch_bounds__(kBlock) void k_ir_residual(const NodeBlk* __restrict__ nb_,
+                                                                const InstDev* __restrict__ inst, int sweep,
+                                                                const double* __restrict__ b,
+                                                                const double* __restrict__ q,
+                                                                float* __restrict__ r32, float* __restrict__ d32,
+                                                                double* __restrict__ prr, double* __restrict__ prr32) {
+    __shared__ double sh[kBlock];
+    const NodeBlk nb = nb_[blockIdx.x];
+    if (inst[nb.scene].stopped) return;
+    double rr = 0.0, rr32 = 0.0;
+#pragma unroll
+    for (int k = 0; k < kNodesPerThread; k++) {
+        const int n = nb.begin + k * kBlock + threadIdx.x;
+        if (n < nb.end) {
+            double rn[3];
+            ld3(b, n, rn);
+            if (sweep > 1) {
+                double qn[3];
+                ld3(q, n, qn);
+#pragma unroll
+                for (int i = 0; i < 3; i++) rn[i] = rn[i] - qn[i];
+            }
+            const float rf[3] = {(float)rn[0], (float)rn[1], (float)rn[2]};
+            const float z[3] = {0.f, 0.f, 0.f};
+            st3(r32, n, rf);
+            st3(d32, n, z);
+            rr += dot3d(rn, rn);
+            rr32 += dot3d(rf, rf);
+        }
+    }
+    const double t0 = block_sum(rr, sh);
+    const double t1 = block_sum(rr32, sh);
+    if (threadIdx.x == 0) { prr[nb.pslot] = t0; prr32[nb.pslot] = t1; }
+}
+
+// One block per scene of the launch set: the refinement test of sweep `sweep` (1-based; sweep
+// max_sweeps + 1 only measures the final residual).  Opens or closes the inner solve.
+static __global__ __launch_bounds__(kBlock) void k_ir_control(const int* __restrict__ scenes, InstDev* __restrict__ inst,
+                                                               InstDev* __restrict__ inner,
+                                                               const Ranges* __restrict__ ranges, int sweep,
+                                                               const double* __restrict__ prr) {
+    __shared__ double sh[kBlock];
+    const int si = scenes[blockIdx.x];
+    InstDev* o = &inst[si];
+    InstDev* in = &inner[si];
+    if (o->stopped) {
+        if (threadIdx.x == 0) in->stopped = 1;
+        return;
+    }
+    const Ranges rg = ranges[si];
+    const double rr = reduce2(prr, rg.b0, rg.nb, 0, 0, sh);
+    if (threadIdx.x != 0) return;
+    if (sweep == 1) o->normb = sqrt(rr);
+    const double normb = o->normb;
+    o->ir_sweeps = sweep - 1;
+    if (normb == 0.0) {
+        o->stopped = 1; o->end_reason = 4; o->err = 0.0; in->stopped = 1;
+        return;
+    }
+    const double err = sqrt(rr) / normb;
+    o->err = err;
+    if (err <= o->tol) {
+        o->stopped = 1; o->end_reason = 1; in->stopped = 1;
+    } else if (sweep > o->max_sweeps) {
+        o->stopped = 1; o->end_reason = 0; in->stopped = 1;
+    } else {
+        // a fresh inner solve: its kernels start at iteration 1
+        in->stopped = 0; in->end_reason = 0; in->last_iters = 0; in->first_solve = 0;
+    }
+}
+
+// x += d after an inner solve (scenes whose refinement had already stopped skip it); the
+// first block of a scene adds the inner iteration count to the step's total.
+static __global__ __launch_bounds__(kBlock) void k_ir_accumulate(const NodeBlk* __restrict__ nb_, InstDev* __restrict__ inst,
+                                                                  InstDev* __restrict__ inner,
+                                                                  const float* __restrict__ d32, double* __restrict__ x) {
+    const NodeBlk nb = nb_[blockIdx.x];
+    InstDev* o = &inst[nb.scene];
+    if (o->stopped) return;
+    if (nb.first && threadIdx.x == 0) {
+        InstDev* in = &inner[nb.scene];
+        // an inner loop that ran every iteration did not record its exit
+        const int its = in->stopped ? in->last_iters : in->max_iter;
+        o->cg_total += its;
+    }
+#pragma unroll
+    for (int k = 0; k < kNodesPerThread; k++) {
+        const int n = nb.begin + k * kBlock + threadIdx.x;
+        if (n < nb.end) {
+            const int64_t j = 3 * (int64_t)n;
+#pragma unroll
+            for (int i = 0; i < 3; i++) x[j + i] = x[j + i] + (double)d32[j + i];
+        }
+    }
+}
+
+// scene bookkeeping at the end of a MIXED step (one thread per scene of the launch set)
+static __global__ void k_step_end_mixed(const int* __restrict__ scenes, int n, InstDev* __restrict__ inst,
+                                        InstDev* __restrict__ inner) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    InstDev* o = &inst[scenes[i]];
+    o->last_iters = o->cg_total;
+    o->cg_total = 0;
+    o->first_solve = 0;
+    o->step_count += 1;
+    o->time += o->dt;
+    o->stopped = 0;
+    inner[scenes[i]].stopped = 1;
+}
+
+}  // namespace sofa_amd

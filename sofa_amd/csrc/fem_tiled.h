@@ -109,7 +109,22 @@ struct TileArgs {
     TileNodes tn;
     const double* __restrict__ mass;
     const int8_t* __restrict__ fixed;
+    const R* __restrict__ emass;        // per element rho V / 20 (consistent MeshMatrixMass), else unused
+    float* __restrict__ rot_lo;         // MIXED engines: float copy of the rotations (written by the RHS)
 };
+
+// consistent-mass edge terms of one element, folded into its nodal contributions (which the
+// node sums subtract): vertex v of the element receives emass * factor * (sum of the other
+// three vertices' values), i.e. the element's share of the edge loop of MeshMatrixMass::addMDx
+template <typename R>
+__device__ __forceinline__ void add_edge_mass(R mf, const R u[12], R c[12]) {
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        const R sum = u[i] + u[3 + i] + u[6 + i] + u[9 + i];
+#pragma unroll
+        for (int v = 0; v < 4; v++) c[3 * v + i] -= mf * (sum - u[3 * v + i]);
+    }
+}
 
 __device__ __forceinline__ void unpack_ltet(uint2 w, int l[4]) {
     l[0] = (int)(w.x & 0xffffu); l[1] = (int)(w.x >> 16); l[2] = (int)(w.y & 0xffffu); l[3] = (int)(w.y >> 16);
@@ -153,6 +168,10 @@ __global__ __launch_bounds__(kBlock) void k_tile_rhs(TileArgs<R, NR> a, const VT
         elem_force_p<R, NR>(a.ev, e, p, rot, c, g);
 #pragma unroll
         for (int i = 0; i < 9; i++) a.ev.rot[i * a.ev.E + e] = rot[i];
+        if (a.rot_lo) {
+#pragma unroll
+            for (int i = 0; i < 9; i++) a.rot_lo[i * a.ev.E + e] = (float)rot[i];
+        }
 #pragma unroll
         for (int i = 0; i < 12; i++) cf[12 * k + i] = c[i];
         if (in.k_rhs != 0.0) {
@@ -161,6 +180,7 @@ __global__ __launch_bounds__(kBlock) void k_tile_rhs(TileArgs<R, NR> a, const VT
 #pragma unroll
             for (int i = 0; i < 12; i++) c[i] = 0;
         }
+        if (in.mass_kind == kMassMatrix && in.m_rhs != 0.0) add_edge_mass((R)in.m_rhs * a.emass[e], u, c);
 #pragma unroll
         for (int i = 0; i < 12; i++) cv[12 * k + i] = c[i];
     }
@@ -174,13 +194,7 @@ __global__ __launch_bounds__(kBlock) void k_tile_rhs(TileArgs<R, NR> a, const VT
         const int ps = a.tn.pslot[li];
         if (ps < 0) {
             VT fn[3];
-            if (in.mass_kind == 0) {
-                fn[0] = (VT)0 + (VT)in.mg[0]; fn[1] = (VT)0 + (VT)in.mg[1]; fn[2] = (VT)0 + (VT)in.mg[2];
-            } else {
-                const VT m = (VT)a.mass[g];
-#pragma unroll
-                for (int i = 0; i < 3; i++) fn[i] = (VT)0 + (VT)in.gravity[i] * m;
-            }
+            mass_gravity(in, a.mass, g, fn);
             for (int s = s0; s < s1; s++) {
                 const int sl = a.tn.inc[s];
 #pragma unroll
@@ -234,13 +248,7 @@ __global__ __launch_bounds__(kBlock) void k_fin_rhs(const FinBlk* __restrict__ f
     if (j < fb.end) {
         const int g = sn.gid[j];
         VT fn[3];
-        if (in.mass_kind == 0) {
-            fn[0] = (VT)0 + (VT)in.mg[0]; fn[1] = (VT)0 + (VT)in.mg[1]; fn[2] = (VT)0 + (VT)in.mg[2];
-        } else {
-            const VT m = (VT)mass[g];
-#pragma unroll
-            for (int i = 0; i < 3; i++) fn[i] = (VT)0 + (VT)in.gravity[i] * m;
-        }
+        mass_gravity(in, mass, g, fn);
         const int s0 = sn.ptr[j], s1 = sn.ptr[j + 1];
         for (int s = s0; s < s1; s++) {
 #pragma unroll
@@ -271,13 +279,16 @@ __global__ __launch_bounds__(kBlock) void k_fin_rhs(const FinBlk* __restrict__ f
 }
 
 // ---------------------------------------------------------------------------------------
-// CG iteration `iter` (1-based): K1 tile apply, K2 shared finalize, K3 update
+// CG iteration `iter` (1-based): K1 tile apply, K2 shared finalize, K3 update.
+// iter = 0 is a plain operator application q = A r (no CG scalars, p untouched), used for
+// the FP64 residual of iterative refinement.
 // ---------------------------------------------------------------------------------------
 template <typename R, int NR, typename VT>
 __global__ __launch_bounds__(kBlock) void k_tile_apply(TileArgs<R, NR> a, int iter, const VT* __restrict__ r,
                                                         VT* __restrict__ p, VT* __restrict__ q, VT* __restrict__ sp,
                                                         const double* __restrict__ pin,   // |b|^2 (iter 1) or r.r partials
-                                                        double* __restrict__ pden) {
+                                                        double* __restrict__ pden,
+                                                        int pin_nodes) {   // 1: iter-1 |b|^2 partials per node block
     constexpr int EPT = kTileElems / kBlock;   // elements per thread
     __shared__ R lu[kTileNodes * 3];
     __shared__ R lc[kTileElems * 12];
@@ -289,8 +300,9 @@ __global__ __launch_bounds__(kBlock) void k_tile_apply(TileArgs<R, NR> a, int it
     if (ins->stopped) return;
     // ---- 1. issue every independent load first: the element records of this thread (kept in
     //         registers) and the tile's incidence tables (copied to LDS)
-    R rot[EPT][9], rin[EPT][NR], kk[EPT][3];
+    R rot[EPT][9], rin[EPT][NR], kk[EPT][3], em[EPT];
     uint2 lt[EPT];
+    const bool consistent = ins->mass_kind == kMassMatrix;
 #pragma unroll
     for (int m = 0; m < EPT; m++) {
         const int k = threadIdx.x + m * kBlock;
@@ -303,6 +315,7 @@ __global__ __launch_bounds__(kBlock) void k_tile_apply(TileArgs<R, NR> a, int it
             for (int i = 0; i < NR; i++) rin[m][i] = a.ev.rest[i * a.ev.E + e];
 #pragma unroll
             for (int i = 0; i < 3; i++) kk[m][i] = a.ev.kmat[i * a.ev.E + e];
+            em[m] = consistent ? a.emass[e] : (R)0;
         }
     }
     const int inc0 = a.tn.inc_ptr[td.ln_begin];
@@ -314,7 +327,8 @@ __global__ __launch_bounds__(kBlock) void k_tile_apply(TileArgs<R, NR> a, int it
     // ---- 2. scalars: rho, the tolerance test, beta (CGLinearSolver.inl:130-196)
     double beta = 0.0;
     if (iter == 1) {
-        const double bb = rg.nf > 0 ? reduce2(pin, rg.f0, rg.nf, 0, 0, sh) : reduce2(pin, rg.t0, rg.nt, 0, 0, sh);
+        const double bb = pin_nodes ? reduce2(pin, rg.b0, rg.nb, 0, 0, sh)
+                                    : rg.nf > 0 ? reduce2(pin, rg.f0, rg.nf, 0, 0, sh) : reduce2(pin, rg.t0, rg.nt, 0, 0, sh);
         const double normb = sqrt(bb);
         if (normb == 0.0) { record_stop(ins, first, 4, 0, 1.0); return; }
         const double err = sqrt(bb) / normb;
@@ -322,7 +336,7 @@ __global__ __launch_bounds__(kBlock) void k_tile_apply(TileArgs<R, NR> a, int it
         if (first && threadIdx.x == 0) {
             ins->normb = normb; ins->rho_slot[1] = bb; ins->iter = 1; ins->err = err;
         }
-    } else {
+    } else if (iter > 1) {
         if (iter > ins->max_iter) { record_stop(ins, first, 0, ins->max_iter, ins->err); return; }
         const double rho = reduce2(pin, rg.b0, rg.nb, 0, 0, sh);
         const double err = sqrt(rho) / ins->normb;
@@ -340,13 +354,13 @@ __global__ __launch_bounds__(kBlock) void k_tile_apply(TileArgs<R, NR> a, int it
         const int g = a.tn.gid[li];
         VT pp[3];
         ld3(r, g, pp);
-        if (iter != 1) {
+        if (iter > 1) {
             VT po[3];
             ld3(p, g, po);
 #pragma unroll
             for (int i = 0; i < 3; i++) { po[i] *= vb; po[i] += pp[i]; pp[i] = po[i]; }
         }
-        if (a.tn.pslot[li] < 0) st3(p, g, pp);
+        if (iter > 0 && a.tn.pslot[li] < 0) st3(p, g, pp);
         lu[3 * j] = (R)pp[0]; lu[3 * j + 1] = (R)pp[1]; lu[3 * j + 2] = (R)pp[2];
     }
     __syncthreads();
@@ -364,6 +378,7 @@ __global__ __launch_bounds__(kBlock) void k_tile_apply(TileArgs<R, NR> a, int it
 #pragma unroll
                 for (int i = 0; i < 3; i++) u[3 * n + i] = lu[3 * l[n] + i];
             elem_dforce_r<R, NR>(rot[m], rin[m], kk[m], u, kcg, c);
+            if (consistent) add_edge_mass((R)ins->m_cg * em[m], u, c);
 #pragma unroll
             for (int i = 0; i < 12; i++) lc[12 * k + i] = c[i];
         }
@@ -421,14 +436,14 @@ __global__ __launch_bounds__(kBlock) void k_fin_apply(const FinBlk* __restrict__
         const int g = sn.gid[j];
         VT pp[3];
         ld3(r, g, pp);
-        if (iter != 1) {
+        if (iter > 1) {
             const VT vb = (VT)in.beta;
             VT po[3];
             ld3(p, g, po);
 #pragma unroll
             for (int i = 0; i < 3; i++) { po[i] *= vb; po[i] += pp[i]; pp[i] = po[i]; }
         }
-        st3(p, g, pp);
+        if (iter > 0) st3(p, g, pp);
         VT qn[3] = {0, 0, 0};
         if (in.m_cg != 0.0) mass_mdx(in, NodeArrays{nullptr, nullptr, mass, fixed}, g, in.m_cg, pp, qn);
         const int s0 = sn.ptr[j], s1 = sn.ptr[j + 1];

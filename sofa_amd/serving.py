@@ -1,0 +1,173 @@
+"""Serving loop: a stream of independent FEM jobs on one GPU with continuous batching.
+
+Jobs arrive over time (wall clock).  A job is admitted when its device footprint fits the HBM
+budget next to the running jobs (first come, first served: a job that does not fit holds the
+queue), is placed in a live engine (`Engine.reserve` / `commit_scene`), and from then on is
+advanced one implicit Euler step per tick together with every other running job, in ONE merged
+launch sequence per tick; a job leaves (`release_scene`) after its last step and its room is
+reused.  The alternative "plain" mode advances the same running jobs one at a time, each with
+its own launches (the one-instance-at-a-time path the batched mode is compared with).
+
+Reported: element-steps/s over the run; per-step latency = device time of the merged step (HIP
+events around it on its stream) attributed to every job it advances, p50 / p99 over all
+(job, step) pairs; admission wait = wall time from arrival to the start of the job's first
+step, p50 / p99.  The scheduler is host code around the C ABI; all per-step work is on the GPU.
+"""
+from __future__ import annotations
+
+import collections
+import concurrent.futures as cf
+import time
+
+import numpy as np
+
+from . import capi, workloads
+
+# device bytes per node / per tetrahedron of a tiled engine (the arenas of
+# sofa_amd_engine_reserve): node vectors x v f b r p q x_cg (3 reals each), mass (8), mask (1),
+# incidence pointer (4), tile-local node tables (~1.3 x 12 B), shared tables; per element: index
+# (16 + 8), rotation 9, rest 6 or 9, stiffness 3, edge mass 1 reals, element force 12 reals,
+# incidences 4 x (4 + 2)
+def footprint_bytes(precision, nodes, tets):
+    pv = 4 if precision == capi.FP32 else 8
+    pe = 4 if precision == capi.FP32 else 8
+    nr = 6 if precision == capi.FP32 else 9
+    per_node = 8 * 3 * pv + 8 + 1 + 4 + 16 + 12 + 6 * pv
+    per_tet = 24 + (9 + nr + 3 + 1) * pe + 12 * pv + 24
+    if precision == capi.MIXED:
+        per_node += 4 * 3 * 4 + 3 * 4
+        per_tet += (9 + 6 + 3 + 1) * 4
+    return per_node * nodes + per_tet * tets
+
+
+class Server:
+    """One GPU, one live engine, a FIFO admission queue bounded by an HBM budget."""
+
+    def __init__(self, precision=capi.FP32, budget_bytes=8 << 30, max_scenes=2048, batched=True,
+                 build_workers=4):
+        self.precision = precision
+        self.budget = int(budget_bytes)
+        self.batched = batched
+        # arena capacity: the budget expressed in nodes and tetrahedra (a tetrahedral mesh has
+        # about 5-6 tetrahedra per node; the node arena is sized for the sparser side)
+        per_tet = footprint_bytes(precision, 0, 1)
+        max_tets = self.budget // per_tet
+        max_nodes = max_tets // 3
+        self.eng = capi.Engine(0, precision)
+        self.eng.reserve(max_nodes, max_tets, max_scenes)
+        self.pool = cf.ThreadPoolExecutor(build_workers)
+
+    def close(self):
+        self.pool.shutdown(wait=False, cancel_futures=True)
+        self.eng.close()
+
+    def run(self, jobs, wall_limit=30.0, stream=None, build=workloads.build_job, log=None, on_finish=None):
+        """Serve `jobs` (dicts with `arrival` seconds and `steps`, built by `build`) until every
+        job is done or `wall_limit` seconds have passed.  `on_finish(engine, scene, job_index)` is
+        called for each finished job before its scene is released.  Returns the measurements."""
+        import torch
+        if stream is None:
+            stream = torch.cuda.Stream()
+        sp = stream.cuda_stream
+        jobs = sorted(jobs, key=lambda j: j["arrival"])
+        specs = {}                                    # job index -> Future of its scene spec
+        nxt = 0                                       # next job not yet arrived
+        queue = collections.deque()                   # arrived, not admitted (job indices)
+        active = {}                                   # scene id -> [job index, steps left, tets]
+        used = 0
+        admitted, done, rejected = {}, 0, 0
+        pending = {}                                  # job index -> created, not yet placed scene
+        waits, step_records = [], []                  # (start event, end event, [tets per job])
+        elem_steps = 0
+        t0 = time.perf_counter()
+        inflight = collections.deque()
+
+        def prefetch(k):
+            for j in range(k, min(len(jobs), k + 32)):
+                if j not in specs:
+                    specs[j] = self.pool.submit(build, jobs[j])
+
+        prefetch(0)
+        while True:
+            now = time.perf_counter() - t0
+            if now >= wall_limit:
+                break
+            while nxt < len(jobs) and jobs[nxt]["arrival"] <= now:
+                queue.append(nxt)
+                nxt += 1
+            prefetch(queue[0] if queue else nxt)
+            # admission: FIFO while the head fits the budget
+            while queue:
+                j = queue[0]
+                spec = specs[j].result()
+                need = footprint_bytes(self.precision, len(spec["pos"]), len(spec["tets"]))
+                if used + need > self.budget and active:
+                    break
+                sid = pending.get(j)
+                if sid is None:
+                    sid = pending[j] = workloads.add_to_engine(self.eng, spec)
+                if not self.eng.commit_scene(sid):
+                    if active:
+                        break                          # arenas full (fragmentation): wait
+                    queue.popleft()                    # larger than the whole reservation
+                    del specs[j], pending[j]
+                    rejected += 1
+                    continue
+                del pending[j]
+                queue.popleft()
+                del specs[j]
+                used += need
+                active[sid] = [j, jobs[j]["steps"], len(spec["tets"]), need]
+                admitted[j] = time.perf_counter() - t0
+                waits.append(admitted[j] - jobs[j]["arrival"])
+            if not active:
+                if nxt >= len(jobs) and not queue:
+                    break
+                time.sleep(0.001)
+                continue
+            ids = list(active.keys())
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            if self.batched:
+                self.eng.step(ids, 1, stream=sp)
+            else:
+                for s in ids:
+                    self.eng.step([s], 1, stream=sp)
+            e1.record(stream)
+            tets = [active[s][2] for s in ids]
+            step_records.append((e0, e1, tets))
+            elem_steps += sum(tets)
+            inflight.append(e1)
+            finished = []
+            for s in ids:
+                active[s][1] -= 1
+                if active[s][1] == 0:
+                    finished.append(s)
+            # at most two ticks in flight: the host prepares the next one meanwhile
+            while len(inflight) > 1:
+                inflight.popleft().synchronize()
+            for s in finished:
+                j, _, _, need = active.pop(s)
+                if on_finish is not None:
+                    on_finish(self.eng, s, j)
+                self.eng.release_scene(s)
+                used -= need
+                done += 1
+            if log is not None and len(step_records) % 50 == 0:
+                log(f"t={now:.1f}s ticks={len(step_records)} active={len(active)} queue={len(queue)} done={done}")
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        lat = []
+        for e0, e1, tets in step_records:
+            ms = e0.elapsed_time(e1)
+            lat.extend([ms] * len(tets))
+        lat = np.asarray(lat) if lat else np.zeros(1)
+        waits_a = np.asarray(waits) if waits else np.zeros(1)
+        return {"mode": "batched" if self.batched else "plain", "wall_s": wall, "element_steps": int(elem_steps),
+                "value": elem_steps / wall, "ticks": len(step_records), "jobs_arrived": nxt,
+                "jobs_admitted": len(admitted), "jobs_done": done, "jobs_rejected": rejected, "jobs_running_at_end": len(active),
+                "queue_at_end": len(queue),
+                "step_latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
+                "admission_wait_s": {"p50": float(np.percentile(waits_a, 50)), "p99": float(np.percentile(waits_a, 99))},
+                "budget_bytes": self.budget}

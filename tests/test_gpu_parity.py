@@ -323,7 +323,7 @@ def test_tiled_component_calls_and_element_map():
     eng.close()
 
 
-@pytest.mark.parametrize("prec", ["FP64", "FP32", "FP64_TILED"])
+@pytest.mark.parametrize("prec", ["FP64", "FP32", "FP64_TILED", "MIXED"])
 def test_graph_replay_equals_eager(prec):
     """A step captured as a hipGraph and replayed gives, replay after replay, the same bits as
     the eager launches (eager reads and synchronisations between the replays)."""
@@ -353,3 +353,245 @@ def test_graph_replay_equals_eager(prec):
     assert np.array_equal(eager.read(ids_e[0])[0], graph.read(ids_g[0])[0])
     eager.close()
     graph.close()
+
+
+# ------------------------------------------------------------------------------------------
+# MeshMatrixMass and OscillatorProjectiveConstraint
+# ------------------------------------------------------------------------------------------
+def beam_meshmatrix(lumped=False, osc=True, seed=21):
+    """A 7x6x17 beam with per-element density (MeshMatrixMass), per-element E and nu, the z=0
+    face fixed and the z=max face driven by oscillators."""
+    rng = np.random.default_rng(seed)
+    pos, tets, fixed = meshes.beam((7, 6, 17), (1.0, 0.8, 3.0))
+    ne = len(tets)
+    c = dict(pos=pos, tets=tets, fixed=fixed, E=10 ** rng.uniform(3.5, 5.0, ne), nu=rng.uniform(0.3, 0.45, ne),
+             density=rng.uniform(900.0, 1100.0, ne), lumped=lumped, g=(0.0, -9.81, 0.0),
+             rs=0.1, rm=0.1, dt=0.01, iters=30, tol=1e-10, thr=1e-12, osc=None)
+    if osc:
+        top = np.nonzero(pos[:, 2] == pos[:, 2].max())[0].astype(np.int32)
+        amp = np.zeros((len(top), 3)); amp[:, 0] = 0.005
+        c["osc"] = dict(index=top, mean=pos[top], amplitude=amp, pulsation=np.full(len(top), 2 * np.pi),
+                        phase=np.zeros(len(top)))
+    return c
+
+
+def oracle_mm(c, dot_mode=0):
+    from oracle import OracleScene
+    s = OracleScene(c["pos"])
+    s.set_gravity(c["g"])
+    s.set_fem(c["tets"], c["E"], c["nu"], "large")
+    s.set_meshmatrix_mass(c["density"], lumping=c["lumped"])
+    s.set_fixed(c["fixed"])
+    if c["osc"] is not None:
+        o = c["osc"]
+        for k in range(len(o["index"])):
+            s.add_oscillator(o["index"][k], o["mean"][k], o["amplitude"][k], o["pulsation"][k], o["phase"][k])
+    s.set_solver(c["rs"], c["rm"], c["iters"], c["tol"], c["thr"])
+    s.set_dot_mode(dot_mode)
+    return s
+
+
+def gpu_mm(eng, c):
+    capi = _capi()
+    kind = capi.MASS_MESHMATRIX_LUMPED if c["lumped"] else capi.MASS_MESHMATRIX
+    return eng.add_scene(c["pos"], c["tets"], c["E"], c["nu"], c["fixed"], mass_kind=kind, mass_value=c["density"],
+                         gravity=c["g"], dt=c["dt"], rayleigh_stiffness=c["rs"], rayleigh_mass=c["rm"],
+                         cg_iterations=c["iters"], cg_tolerance=c["tol"], cg_threshold=c["thr"], oscillators=c["osc"])
+
+
+@pytest.mark.parametrize("lumped", [False, True])
+def test_meshmatrix_fp64_bitexact(lumped):
+    """Consistent / lumped MeshMatrixMass, no oscillator: FP64 parity path bit-exact against the
+    oracle in the GPU's dot order after 30 steps, <= 1e-10 against the reference order."""
+    capi = _capi()
+    print(f"case: meshmatrix fp64 exact lumped={lumped}", flush=True)
+    c = beam_meshmatrix(lumped=lumped, osc=False)
+    eng = capi.Engine(0, capi.FP64)
+    sid = gpu_mm(eng, c)
+    eng.commit()
+    eng.step([sid], 30)
+    x, v, f = eng.read(sid)
+    o1 = oracle_mm(c, 1)
+    o1.step(c["dt"], 30)
+    assert np.array_equal(x, o1.get("x")), rel_err(x, o1.get("x"))
+    assert np.array_equal(v, o1.get("v"))
+    o0 = oracle_mm(c, 0)
+    o0.step(c["dt"], 30)
+    assert rel_err(x, o0.get("x")) <= 1e-10
+    eng.close()
+
+
+@pytest.mark.parametrize("prec,tol", [("FP64", 1e-10), ("FP64_TILED", 1e-10), ("FP32", 2e-5)])
+def test_meshmatrix_oscillator_trajectory(prec, tol):
+    """Consistent MeshMatrixMass + per-element materials + oscillator-driven face, 40 steps:
+    relative position and velocity error against the oracle (reference order) within `tol`
+    (the device sin/cos of the oscillator are within 1 ulp of the host's)."""
+    capi = _capi()
+    print(f"case: meshmatrix + oscillators {prec}", flush=True)
+    c = beam_meshmatrix(lumped=False, osc=True)
+    eng = capi.Engine(0, getattr(capi, prec))
+    sid = gpu_mm(eng, c)
+    eng.commit()
+    eng.step([sid], 40)
+    x, v, f = eng.read(sid)
+    o = oracle_mm(c, 0)
+    o.step(c["dt"], 40)
+    assert rel_err(x, o.get("x")) <= tol
+    vr = o.get("v")
+    assert float(np.max(np.linalg.norm(v - vr, axis=1)) / np.max(np.linalg.norm(vr, axis=1))) <= 10 * tol
+    top = c["osc"]["index"]
+    # the driven nodes follow the prescribed motion at the start time of the last step
+    t = 39 * c["dt"]
+    exp_x = c["osc"]["mean"][:, 0] + 0.005 * np.sin(2 * np.pi * t)
+    assert np.max(np.abs(x[top, 0] - exp_x)) <= (1e-12 if prec != "FP32" else 1e-6)
+    eng.close()
+
+
+# ------------------------------------------------------------------------------------------
+# MIXED: FP32 inner CG inside FP64 iterative refinement
+# ------------------------------------------------------------------------------------------
+def converged(c, iters=400, tol=1e-12):
+    """The same scene with the reference CG run to convergence (the comparison point of a
+    solver that is not the reference's CG)."""
+    d = dict(c)
+    d.update(iters=iters, tol=tol, thr=1e-30)
+    return d
+
+
+@pytest.mark.parametrize("which", ["cfg0", "meshmatrix"])
+def test_mixed_matches_converged_fp64(which):
+    """MIXED with the refinement run to |r|/|b| <= 1e-10 stays within 1e-6 relative (positions
+    and velocities) of the FP64 oracle whose CG is run to 1e-12, after 100 (cfg0) / 40 steps."""
+    capi = _capi()
+    print(f"case: mixed vs converged fp64 oracle, {which}", flush=True)
+    if which == "cfg0":
+        c = beam_cfg0()
+        c.update(tol=1e-10)
+        nsteps = 100
+        eng = capi.Engine(0, capi.MIXED)
+        sid = add_gpu_scene(eng, c)
+        o = make_oracle(converged(c))
+    else:
+        c = beam_meshmatrix(lumped=False, osc=True)
+        c.update(tol=1e-10)
+        nsteps = 40
+        eng = capi.Engine(0, capi.MIXED)
+        sid = gpu_mm(eng, c)
+        o = oracle_mm(converged(c))
+    eng.set_refinement(sid, inner_iterations=30, inner_tolerance=1e-6, max_sweeps=8)
+    eng.commit()
+    eng.step([sid], nsteps)
+    x, v, f = eng.read(sid)
+    st = eng.stats(sid)
+    o.step(c["dt"], nsteps)
+    ex, ev = rel_err(x, o.get("x")), rel_err(v, o.get("v"))
+    print(f"  eps_rel x {ex:.3e} v {ev:.3e} stats {st}", flush=True)
+    assert ex <= 1e-6 and ev <= 1e-6
+    assert st["end_reason"] == 1 and 1 <= st["ir_sweeps"] <= 8 and st["err"] <= 1e-10
+    assert st["step_count"] == nsteps
+    eng.close()
+
+
+def test_mixed_sweep_limit_and_batching():
+    """Sweep and iteration limits are kept (<= 3 sweeps of <= 30 FP32 iterations), and a MIXED
+    scene stepped in a batch gives the same bits as stepped alone."""
+    capi = _capi()
+    print("case: mixed limits and batching", flush=True)
+    cfgs = [beam_cfg0(), beam_medium(3e5)]
+    batch = capi.Engine(0, capi.MIXED)
+    ids = [add_gpu_scene(batch, c) for c in cfgs]
+    batch.commit()
+    batch.step(ids, 10)
+    for k, c in enumerate(cfgs):
+        st = batch.stats(ids[k])
+        assert st["ir_sweeps"] <= 3 and st["iterations"] <= 3 * 30 and st["step_count"] == 10
+        alone = capi.Engine(0, capi.MIXED)
+        sid = add_gpu_scene(alone, c)
+        alone.commit()
+        alone.step([sid], 10)
+        xa, va, _ = alone.read(sid)
+        xb, vb, _ = batch.read(ids[k])
+        assert np.array_equal(xa, xb) and np.array_equal(va, vb)
+        assert alone.stats(sid) == st
+        alone.close()
+    batch.close()
+
+
+# ------------------------------------------------------------------------------------------
+# Live scenes: arenas, scenes committed and released while others keep running
+# ------------------------------------------------------------------------------------------
+@pytest.mark.parametrize("prec", ["FP32", "MIXED"])
+def test_live_scenes_match_static(prec):
+    """Scenes committed into a reserved engine, stepped in changing sets, one released and its
+    room reused: every scene's trajectory is bit-identical to the same scene stepped alone in a
+    statically committed engine; a scene larger than the free room is refused (ENOMEM)."""
+    capi = _capi()
+    print(f"case: live scenes {prec}", flush=True)
+    P = getattr(capi, prec)
+    a_cfg, b_cfg = beam_cfg0(), beam_medium(3e5)
+    live = capi.Engine(0, P)
+    nn = len(a_cfg["pos"]) + len(b_cfg["pos"]) + 16
+    nt = len(a_cfg["tets"]) + len(b_cfg["tets"])
+    live.reserve(nn, nt, 4)
+    a = add_gpu_scene(live, a_cfg)
+    assert live.commit_scene(a)
+    b = add_gpu_scene(live, b_cfg)
+    assert live.commit_scene(b)
+    live.step([a, b], 4)
+    big = add_gpu_scene(live, b_cfg)           # no room while a and b hold the arenas
+    assert not live.commit_scene(big)
+    live.release_scene(a)
+    c = add_gpu_scene(live, a_cfg)              # reuses a's ranges
+    assert live.commit_scene(c)
+    live.step([b, c], 3)
+    live.step([c], 2)
+    u = live.usage()
+    assert u["scenes"][0] == 2 and u["nodes"][0] == len(a_cfg["pos"]) + len(b_cfg["pos"]) + 2
+    for cfg, sid, nsteps in ((b_cfg, b, 7), (a_cfg, c, 5)):
+        ref = capi.Engine(0, P)
+        rid = add_gpu_scene(ref, cfg)
+        ref.commit()
+        ref.step([rid], nsteps)
+        xr, vr, _ = ref.read(rid)
+        xl, vl, _ = live.read(sid)
+        assert np.array_equal(xr, xl) and np.array_equal(vr, vl), (cfg is a_cfg, nsteps)
+        assert ref.stats(rid) == live.stats(sid)
+        ref.close()
+    with pytest.raises(capi.SofaAmdError):
+        live.step([a], 1)                       # released
+    live.close()
+
+
+def test_served_jobs_match_alone():
+    """Continuous batching: 12 small jobs arriving over 0.3 s, admitted under a budget that holds
+    only some at once, advanced in merged steps with joins and departures; every job's final
+    state is bit-identical to the same job stepped alone."""
+    capi = _capi()
+    from sofa_amd import serving, workloads
+    print("case: served jobs vs alone", flush=True)
+    rng = np.random.default_rng(7)
+    jobs = []
+    for k in range(12):
+        dims = (int(rng.integers(3, 6)), int(rng.integers(3, 6)), int(rng.integers(4, 12)))
+        jobs.append(dict(arrival=0.025 * k, steps=int(rng.integers(3, 12)), kind="beam", dims=dims,
+                         E=10 ** rng.uniform(4, 6), nu=rng.uniform(0.3, 0.45), amp=rng.uniform(0, 0.01),
+                         freq=rng.uniform(0.5, 5)))
+    budget = 4 * serving.footprint_bytes(capi.FP32, 11 * 5 * 5, 5 * 10 * 4 * 4)
+    srv = serving.Server(capi.FP32, budget_bytes=budget, max_scenes=16)
+    finals = {}
+
+    def keep(eng, sid, j):
+        finals[j] = eng.read(sid)[:2]
+
+    res = srv.run(jobs, wall_limit=60.0, on_finish=keep)
+    srv.close()
+    assert res["jobs_done"] == len(jobs) and len(finals) == len(jobs)
+    for j, job in enumerate(jobs):
+        spec = workloads.build_job(job)
+        ref = capi.Engine(0, capi.FP32)
+        rid = workloads.add_to_engine(ref, spec)
+        ref.commit()
+        ref.step([rid], job["steps"])
+        xr, vr, _ = ref.read(rid)
+        assert np.array_equal(xr, finals[j][0]) and np.array_equal(vr, finals[j][1]), j
+        ref.close()
