@@ -36,6 +36,11 @@ HBM_PEAK_GBS = 8000.0   # MI355X HBM3E, 8 TB/s
 _STREAM = None
 
 
+def log(msg):
+    """Progress of the sections on stderr (stdout carries the one JSON line)."""
+    print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
+
+
 def bench_stream():
     """A dedicated stream (default priority): library calls and the HIP events that time them
     share this non-null stream handle (NULL would select the engine's own stream)."""
@@ -219,26 +224,38 @@ def serve_cfg4(seconds, arrivals, budget_gb):
 
 
 def cpu_baseline(specs, steps, stride):
-    """The FP64 oracle (C++ restatement of the reference path, one thread) on this host, over a
-    bounded sample: every `stride`-th beam of the mix, `steps` step(s) each."""
+    """The FP64 oracle (C++ restatement of the reference path) on this host, over a bounded
+    sample: every `stride`-th beam of the mix, `steps` step(s) each, timed with the element loops
+    on all usable cores (at most 16, the host share of one GPU) and on one thread (the
+    reference's own loops are serial); both give the same bits."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     from oracle import OracleScene
-    elem_steps, secs = 0, 0.0
-    for s in specs[::stride]:
-        o = OracleScene(s["pos"])
-        o.set_gravity(s["g"])
-        o.set_fem(s["tets"], np.atleast_1d(s["E"]), np.atleast_1d(s["nu"]), "large")
-        o.set_uniform_mass(vertex_mass=s["mass"][1])
-        o.set_fixed(s["fixed"])
-        o.set_solver(s["rs"], s["rm"], s["iters"], s["tol"], s["thr"])
-        t0 = time.perf_counter()
-        o.step(s["dt"], steps)
-        secs += time.perf_counter() - t0
-        elem_steps += len(s["tets"]) * steps
+    threads = max(1, min(16, len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()))
+    res = {}
+    for nt in (threads, 1):
+        elem_steps, secs = 0, 0.0
+        for s in specs[::stride]:
+            o = OracleScene(s["pos"])
+            o.set_gravity(s["g"])
+            o.set_fem(s["tets"], np.atleast_1d(s["E"]), np.atleast_1d(s["nu"]), "large")
+            o.set_uniform_mass(vertex_mass=s["mass"][1])
+            o.set_fixed(s["fixed"])
+            o.set_solver(s["rs"], s["rm"], s["iters"], s["tol"], s["thr"])
+            o.set_threads(nt)
+            t0 = time.perf_counter()
+            o.step(s["dt"], steps)
+            secs += time.perf_counter() - t0
+            elem_steps += len(s["tets"]) * steps
+        res[nt] = (elem_steps, secs)
     n = len(specs[::stride])
-    return {"value": elem_steps / secs, "unit": "element-steps/s", "cores": 1, "kind": "port",
-            "sample": f"{n} of the 256 beams (every {stride}th), {steps} step(s) each, FP64, CG 25, one thread: "
-                      f"{elem_steps} element-steps in {secs:.1f} s; oracle/fem_oracle.cpp (-O2 -ffp-contract=off)"}
+    es, secs = res[threads]
+    es1, secs1 = res[1]
+    return {"value": es / secs, "unit": "element-steps/s", "cores": threads, "kind": "port",
+            "sample": f"{n} of the 256 beams (every {stride}th), {steps} step(s) each, FP64, CG 25: {es} element-steps "
+                      f"in {secs:.2f} s on {threads} threads (element loops parallel, per-node sums in element order); "
+                      f"one thread: {es1 / secs1:.4g} element-steps/s ({secs1:.2f} s); oracle/fem_oracle.cpp "
+                      "(-O2 -ffp-contract=off)",
+            "single_thread_value": es1 / secs1}
 
 
 def main():
@@ -282,6 +299,8 @@ def main():
     value = E * args.steps / (total_ms / 1e3)
     roof = apply_roofline(eng, args.apply_reps, 4, 4, 18, E, N)
     roof["kernel"] = "k_tile_apply<float> over the 256-beam mix (CG operator: addDForce + mass + projection + p update)"
+    log(f"batched: {value:.4g} element-steps/s, {total_ms / args.steps:.3f} ms/step; apply {roof['us_per_launch']:.2f} us "
+        f"= {roof['achieved']:.0f} GB/s")
     tiling = {k: 0 for k in ("tiles", "tile_local_nodes", "finalize_nodes", "shared_slots")}
     for i in ids:
         t = eng.tiling(i)
@@ -293,9 +312,12 @@ def main():
     plain_ms, plain_lat = time_plain(engp, idsp, args.plain_steps, 1)
     engp.close()
     value_plain = E * args.plain_steps / (plain_ms / 1e3)
+    log(f"plain: {value_plain:.4g} element-steps/s; batched / plain = {value / value_plain:.2f}")
 
     err = None if args.no_error else position_error(specs, capi.FP32, args.error_steps)
+    log(f"error vs fp64: {err}")
     probe = copy_probe()
+    log(f"copy probe: {probe}")
 
     # ---- secondary: the single 1.31M-tet beam
     cfg1 = None
@@ -313,11 +335,15 @@ def main():
                          "p50_ms": float(np.percentile(l1, 50)), "p99_ms": float(np.percentile(l1, 99)),
                          "apply_us": r1["us_per_launch"], "apply_frac": r1["frac"]},
                 "fp64_parity": {"value": len(c1["tets"]) * 5 / (t64 / 1e3), "ms_per_step": t64 / 5}}
+        log(f"cfg1: {cfg1}")
 
     cfg2 = None if args.no_cfg2 else cfg2_mixed(args.cfg2_steps, args.cfg2_error_steps)
+    log(f"cfg2: {cfg2}")
     served = None if args.no_serve else serve_cfg4(args.serve_seconds, args.serve_arrivals, args.serve_budget_gb)
+    log(f"cfg4: {served}")
 
     cpu = None if args.no_cpu_baseline else cpu_baseline(specs, args.cpu_steps, args.cpu_stride)
+    log(f"cpu baseline: {cpu}")
 
     out = {
         "metric": "FEM element-steps/s, batched vs plain, at <=1e-6 relative position error (implicit corotational FEM step)",

@@ -109,6 +109,10 @@ int sofa_amd_scene_set_gravity(sofa_amd_engine eng, int scene, double gx, double
 /* FixedProjectiveConstraint indices (FixedProjectiveConstraint.inl:184-207). */
 int sofa_amd_scene_set_fixed(sofa_amd_engine eng, int scene, const int32_t* indices, int n);
 
+/* FixedProjectiveConstraint fixAll (FixedProjectiveConstraint.inl:190-195): every node's response
+ * is cleared. */
+int sofa_amd_scene_set_fix_all(sofa_amd_engine eng, int scene, int fix_all);
+
 /* EulerImplicitSolver rayleighStiffness / rayleighMass (EulerImplicitSolver.cpp:40-51) and
  * CGLinearSolver iterations / tolerance / threshold (CGLinearSolver.inl:36-40), dt. */
 int sofa_amd_scene_set_solver(sofa_amd_engine eng, int scene, double dt, double rayleigh_stiffness,

@@ -30,6 +30,11 @@
 // Build: oracle/Makefile (g++ -O2 -ffp-contract=off, so that no multiply-add is fused:
 // the reference's x86-64 build does not fuse either).
 //
+// Threads (orc_set_threads, the CPU baseline on all cores): the element loops of addForce and
+// addDForce (method "large") run in parallel writing each element's 12 nodal values, then each
+// node applies its values in ascending element id -- the same operations in the same order per
+// node as the serial loop, so the result is bit-identical for any thread count.
+//
 // Dot-product order: the reference sums r.r node by node in ascending order
 // (MechanicalObject.inl:2334-2356).  The GPU sums per 256-node block with a fixed tree and
 // then over blocks.  `dot_mode = 1` replays that GPU order exactly (see dot_blocked), so the
@@ -139,6 +144,24 @@ struct TetraFEM : public ForceField {
     std::vector<Mat3> rotations, initialRotations;
     std::vector<std::array<Vec3, 4>> rotatedInitial;
     Real restVolume = 0;
+
+    // parallel element loops: node -> (element * 4 + vertex) in ascending element id, and the
+    // per-element nodal values
+    int nthreads = 1;
+    std::vector<int> incPtr, incIdx;
+    std::vector<Real> contrib;
+    void ensureIncidence(size_t nNodes) {
+        if (incPtr.size() == nNodes + 1) return;
+        incPtr.assign(nNodes + 1, 0);
+        for (const Tetra& t : elems)
+            for (int k = 0; k < 4; k++) incPtr[t[k] + 1]++;
+        for (size_t i = 0; i < nNodes; i++) incPtr[i + 1] += incPtr[i];
+        incIdx.assign(4 * elems.size(), 0);
+        std::vector<int> fill(incPtr.begin(), incPtr.end() - 1);
+        for (size_t e = 0; e < elems.size(); e++)
+            for (int k = 0; k < 4; k++) incIdx[fill[elems[e][k]]++] = (int)(4 * e + k);
+        contrib.assign(12 * elems.size(), 0.0);
+    }
 
     Real& Kel(size_t i, int r, int c) { return K[i][r * 6 + c]; }
     Real& Jel(size_t i, int r, int c) { return J[i][r * 6 + c]; }
@@ -336,6 +359,14 @@ struct TetraFEM : public ForceField {
 
     // :870-986 (assembling == false, updateStiffnessMatrix == false)
     void accumulateForceLarge(std::vector<Vec3>& f, const std::vector<Vec3>& p, size_t i) {
+        Real c[12];
+        forceLarge(p, i, c);
+        const Tetra& index = elems[i];
+        for (int k = 0; k < 12; k += 3)
+            for (int q = 0; q < 3; q++) f[index[k / 3]][q] += c[k + q];
+    }
+    // the element part of accumulateForceLarge: rotation, R F per vertex into out[12]
+    void forceLarge(const std::vector<Vec3>& p, size_t i, Real out[12]) {
         const Tetra& index = elems[i];
         Mat3 R02 = computeRotationLarge(p, index[0], index[1], index[2]);
         rotations[i] = transpose(R02);
@@ -360,7 +391,7 @@ struct TetraFEM : public ForceField {
         computeForceF(F, D, i);
         for (int k = 0; k < 12; k += 3) {
             Vec3 c = mul(rotations[i], Vec3(F[k], F[k + 1], F[k + 2]));
-            for (int q = 0; q < 3; q++) f[index[k / 3]][q] += c[q];
+            for (int q = 0; q < 3; q++) out[k + q] = c[q];
         }
     }
 
@@ -388,6 +419,17 @@ struct TetraFEM : public ForceField {
 
     // :1548-1604
     void addForce(std::vector<Vec3>& f, const std::vector<Vec3>& x, const std::vector<Vec3>&) override {
+        if (nthreads > 1 && method == LARGE) {
+            ensureIncidence(f.size());
+            const long E = (long)elems.size(), N = (long)f.size();
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+            for (long i = 0; i < E; i++) forceLarge(x, (size_t)i, &contrib[12 * i]);
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+            for (long nd = 0; nd < N; nd++)
+                for (int s = incPtr[nd]; s < incPtr[nd + 1]; s++)
+                    for (int q = 0; q < 3; q++) f[nd][q] += contrib[3 * incIdx[s] + q];
+            return;
+        }
         for (size_t i = 0; i < elems.size(); i++) {
             if (method == LARGE) accumulateForceLarge(f, x, i);
             else accumulateForceSmall(f, x, i);
@@ -396,6 +438,14 @@ struct TetraFEM : public ForceField {
 
     // :1193-1237
     void applyStiffnessCorotational(std::vector<Vec3>& f, const std::vector<Vec3>& x, size_t i, Real fact) {
+        Real o[12];
+        dforceCorotational(x, i, fact, o);
+        const Tetra& t = elems[i];
+        for (int k = 0; k < 4; k++)
+            for (int q = 0; q < 3; q++) f[t[k]][q] -= o[3 * k + q];
+    }
+    // the element part of applyStiffnessCorotational: the value each vertex's df loses
+    void dforceCorotational(const std::vector<Vec3>& x, size_t i, Real fact, Real out[12]) {
         const Tetra& t = elems[i];
         const Mat3& R = rotations[i];
         Real X[12];
@@ -408,10 +458,9 @@ struct TetraFEM : public ForceField {
         Real F[12];
         computeForceDF(F, X, i, fact);
         for (int k = 0; k < 4; k++) {
-            Vec3& o = f[t[k]];
-            o[0] -= R.m[0][0] * F[3 * k] + R.m[0][1] * F[3 * k + 1] + R.m[0][2] * F[3 * k + 2];
-            o[1] -= R.m[1][0] * F[3 * k] + R.m[1][1] * F[3 * k + 1] + R.m[1][2] * F[3 * k + 2];
-            o[2] -= R.m[2][0] * F[3 * k] + R.m[2][1] * F[3 * k + 1] + R.m[2][2] * F[3 * k + 2];
+            out[3 * k + 0] = R.m[0][0] * F[3 * k] + R.m[0][1] * F[3 * k + 1] + R.m[0][2] * F[3 * k + 2];
+            out[3 * k + 1] = R.m[1][0] * F[3 * k] + R.m[1][1] * F[3 * k + 1] + R.m[1][2] * F[3 * k + 2];
+            out[3 * k + 2] = R.m[2][0] * F[3 * k] + R.m[2][1] * F[3 * k + 1] + R.m[2][2] * F[3 * k + 2];
         }
     }
 
@@ -429,6 +478,17 @@ struct TetraFEM : public ForceField {
 
     // :1606-1636; kFactor = (Real) kFactorIncludingRayleighDamping(mparams, rayleighStiffness)
     void addDForce(std::vector<Vec3>& df, const std::vector<Vec3>& dx, Real kFactor) {
+        if (nthreads > 1 && method == LARGE) {
+            ensureIncidence(df.size());
+            const long E = (long)elems.size(), N = (long)df.size();
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+            for (long i = 0; i < E; i++) dforceCorotational(dx, (size_t)i, kFactor, &contrib[12 * i]);
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+            for (long nd = 0; nd < N; nd++)
+                for (int s = incPtr[nd]; s < incPtr[nd + 1]; s++)
+                    for (int q = 0; q < 3; q++) df[nd][q] -= contrib[3 * incIdx[s] + q];
+            return;
+        }
         for (size_t i = 0; i < elems.size(); i++) {
             if (method == SMALL) applyStiffnessSmall(df, dx, i, kFactor);
             else applyStiffnessCorotational(df, dx, i, kFactor);
@@ -884,6 +944,11 @@ void orc_set_solver(void* h, double rayleigh_stiffness, double rayleigh_mass, in
 }
 
 void orc_set_dot_mode(void* h, int mode) { static_cast<Scene*>(h)->dotMode = mode; }
+
+void orc_set_threads(void* h, int nthreads) {
+    Scene* s = static_cast<Scene*>(h);
+    if (s->fem) s->fem->nthreads = nthreads < 1 ? 1 : nthreads;
+}
 
 void orc_step(void* h, double dt, int nsteps) {
     Scene* s = static_cast<Scene*>(h);

@@ -43,6 +43,7 @@ def lib():
         L.orc_add_oscillator.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp, ctypes.c_double, ctypes.c_double]
         L.orc_set_solver.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_double, ctypes.c_double]
         L.orc_set_dot_mode.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.orc_set_threads.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.orc_step.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_int]
         L.orc_get_vec.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp]
         L.orc_set_vec.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp]
@@ -136,6 +137,10 @@ class OracleScene:
 
     def set_dot_mode(self, mode):
         lib().orc_set_dot_mode(self._h, int(mode))
+
+    def set_threads(self, n):
+        """Threads of the FEM element loops (results identical for any count)."""
+        lib().orc_set_threads(self._h, int(n))
 
     def step(self, dt, nsteps=1):
         lib().orc_step(self._h, float(dt), int(nsteps))

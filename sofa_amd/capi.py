@@ -28,7 +28,7 @@ EXPORTS = [
     "sofa_amd_fem_add_force", "sofa_amd_fem_add_dforce", "sofa_amd_engine_bench_apply", "sofa_amd_scene_tiling",
     "sofa_amd_engine_set_graphs", "sofa_amd_scene_set_mass_density", "sofa_amd_scene_add_oscillators",
     "sofa_amd_scene_set_refinement", "sofa_amd_engine_reserve", "sofa_amd_scene_commit", "sofa_amd_scene_release",
-    "sofa_amd_engine_usage",
+    "sofa_amd_engine_usage", "sofa_amd_scene_set_fix_all",
 ]
 
 
@@ -92,6 +92,7 @@ def lib():
         "sofa_amd_scene_commit": (c_int, [_vp, c_int]),
         "sofa_amd_scene_release": (c_int, [_vp, c_int]),
         "sofa_amd_engine_usage": (c_int, [_vp, ctypes.POINTER(ctypes.c_int64)]),
+        "sofa_amd_scene_set_fix_all": (c_int, [_vp, c_int, c_int]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -212,6 +213,10 @@ class Engine:
 
     def sync(self):
         check(lib().sofa_amd_engine_sync(self._h))
+
+    def set_fix_all(self, scene, fix_all=True):
+        """FixedProjectiveConstraint fixAll."""
+        check(lib().sofa_amd_scene_set_fix_all(self._h, int(scene), 1 if fix_all else 0))
 
     def set_refinement(self, scene, inner_iterations=30, inner_tolerance=1e-6, max_sweeps=3):
         """MIXED engines: FP32 inner CG limits and FP64 refinement sweeps per step."""

@@ -143,6 +143,7 @@ struct SceneHost {
     double mass_value = 1.0;                // vertexMass or massDensity
     double g[3] = {0, 0, 0};
     std::vector<int32_t> fixed{0};          // FixedProjectiveConstraint default indices = {0}
+    bool fix_all = false;                   // FixedProjectiveConstraint fixAll
     std::vector<double> density{1.0};       // MeshMatrixMass massDensity (one or per element)
     // OscillatorProjectiveConstraint entries
     std::vector<int32_t> osc_idx;
@@ -298,6 +299,7 @@ struct EngineT : EngineBase {
     Ranges* ranges_d = nullptr;
     uint2* ltet = nullptr;
     int *ln_gid = nullptr, *ln_inc_ptr = nullptr, *ln_pslot = nullptr, *sh_gid = nullptr, *sh_ptr = nullptr;
+    int* sh_of = nullptr;                          // per node: index into sh_gid / sh_ptr, or -1
     uint16_t* ln_inc = nullptr;
     VT *sp0 = nullptr, *sp1 = nullptr;
     double *pbb = nullptr, *pden = nullptr, *prho = nullptr;
@@ -343,7 +345,7 @@ struct EngineT : EngineBase {
                         inst, partial, tiles_d, ranges_d, ltet, ln_gid, ln_inc_ptr, ln_pslot, sh_gid, sh_ptr,
                         ln_inc, sp0, sp1, pbb, pden, prho, p2, em_ptr, em_nbr, em_val, emass, osc_node,
                         osc_mean, osc_amp, osc_w, osc_phase, rot32, rest32, kmat32, emass32, r32, p32, q32,
-                        d32, sp32, pden32, prho32, prr, inst_in};
+                        d32, sp32, pden32, prho32, prr, inst_in, sh_of};
         for (void* ptr : ptrs)
             if (ptr) (void)hipFree(ptr);
     }
@@ -484,7 +486,7 @@ struct EngineT : EngineBase {
         std::vector<int> fillp(cnt.begin(), cnt.end() - 1);
         for (int e = 0; e < nE; e++)
             for (int k = 0; k < 4; k++) h.inc[fillp[s.tets[4 * (size_t)e + k]]++] = 4 * ip[e] + k;
-        h.fixed.assign(n, 0);
+        h.fixed.assign(n, s.fix_all ? 1 : 0);
         for (int32_t i : s.fixed) {
             if (i < 0 || i >= n) throw std::invalid_argument("fixed index out of range");
             h.fixed[i] = 1;
@@ -709,9 +711,10 @@ struct EngineT : EngineBase {
         inst = nullptr; partial = nullptr; tiles_d = nullptr; ranges_d = nullptr; ltet = nullptr;
         ln_gid = ln_inc_ptr = ln_pslot = sh_gid = sh_ptr = nullptr; ln_inc = nullptr; sp0 = sp1 = nullptr;
         pbb = pden = prho = nullptr; rot32 = rest32 = kmat32 = emass32 = nullptr;
-        r32 = p32 = q32 = d32 = sp32 = nullptr; pden32 = prho32 = prr = nullptr; inst_in = nullptr;
+        r32 = p32 = q32 = d32 = sp32 = nullptr; pden32 = prho32 = prr = nullptr; inst_in = nullptr; sh_of = nullptr;
         cap = c;
-        if (cap.node >= (int64_t)1 << 30 || cap.inc >= (int64_t)1 << 31 || cap.lnode >= (int64_t)1 << 31)
+        if (cap.node >= (int64_t)1 << 30 || cap.inc >= (int64_t)1 << 31 || cap.lnode >= (int64_t)1 << 31 ||
+            cap.slot >= (int64_t)kSlotOwner)
             throw std::invalid_argument("capacity too large for 32-bit indices");
         for (int k = 0; k < 11; k++) arenas(k)->reset(field(cap, k));
         Ntot = cap.node; Etot = cap.elem;
@@ -720,7 +723,7 @@ struct EngineT : EngineBase {
             *vec = dalloc<VT>(3 * N);
             HIPCHK(hipMemset(*vec, 0, 3 * N * sizeof(VT)));
         }
-        mass = dalloc<double>(N); fixed = dalloc<int8_t>(N); inc_ptr = dalloc<int>(N);
+        mass = dalloc<double>(N); fixed = dalloc<int8_t>(N); inc_ptr = dalloc<int>(N); sh_of = dalloc<int>(N);
         tet = dalloc<int4>(E); rot = dalloc<R>(9 * E); rest = dalloc<R>(NR * E); kmat = dalloc<R>(3 * E);
         emass = dalloc<R>(E); ltet = dalloc<uint2>(E); ef = dalloc<VT>(12 * E);
         inc = dalloc<int>(cap.inc); ln_inc = dalloc<uint16_t>(cap.inc);
@@ -819,6 +822,9 @@ struct EngineT : EngineBase {
         }
         // tiles and their local nodes (each tile's list keeps one extra entry for its end)
         const int T = (int)P.t_ebegin.size();
+        // the first slot of each shared node belongs to its first tile (the node's owner tile)
+        std::vector<int> first_slot(n, -1);
+        for (size_t k = 0; k < P.sh_gid.size(); k++) first_slot[P.sh_gid[k]] = P.sh_ptr[k];
         std::vector<TileDesc> tds(T);
         std::vector<int> lg, lp, lps;
         lg.reserve(pl.size.lnode); lp.reserve(pl.size.lnode); lps.reserve(pl.size.lnode);
@@ -829,15 +835,20 @@ struct EngineT : EngineBase {
             td.e_count = P.t_ecount[t];
             td.ln_begin = (int)pl.at.lnode + (int)lg.size();
             td.ln_count = P.t_lncount[t];
-            td.first = (t == 0);
+            td.flags = (t == 0 ? kTileFirst : 0) | (s.mass_kind == SOFA_AMD_MASS_MESHMATRIX ? kTileConsistent : 0);
             td.pslot = (int)pl.at.tile + t;
-            td.pad = 0;
+            td.inc_begin = (int)ib0 + P.ln_inc_ptr[P.t_lnbegin[t]];
             for (int j = 0; j <= P.t_lncount[t]; j++) {
                 const int li = P.t_lnbegin[t] + j;
                 const bool end = j == P.t_lncount[t];
                 lg.push_back(end ? nb0 : P.ln_gid[li] + nb0);
                 lp.push_back((int)ib0 + P.ln_inc_ptr[li]);
-                lps.push_back(end || P.ln_pslot[li] < 0 ? -1 : P.ln_pslot[li] + (int)pl.at.slot);
+                if (end || P.ln_pslot[li] < 0) {
+                    lps.push_back(-1);
+                } else {
+                    const int owner = P.ln_pslot[li] == first_slot[P.ln_gid[li]] ? kSlotOwner : 0;
+                    lps.push_back((P.ln_pslot[li] + (int)pl.at.slot) | owner);
+                }
             }
         }
         put_at(tiles_d, pl.at.tile, tds);
@@ -850,6 +861,9 @@ struct EngineT : EngineBase {
         sg[Sn] = nb0;
         for (int j = 0; j <= Sn; j++) spp[j] = (int)pl.at.slot + th.sh_ptr[j];
         put_at(sh_gid, pl.at.shn, sg); put_at(sh_ptr, pl.at.shn, spp);
+        std::vector<int> of(n + 1, -1);
+        for (int j = 0; j < Sn; j++) of[th.sh_gid[j]] = (int)pl.at.shn + j;
+        put_at(sh_of, nb0, of);
         for (int j0 = 0, bi = 0; j0 < Sn; j0 += kBlock, bi++)
             pl.fins.push_back(FinBlk{sid, (int)pl.at.shn + j0, (int)pl.at.shn + std::min(Sn, j0 + kBlock),
                                      (int)(cap.tile + pl.at.fin) + bi, bi, 0, 0, 0});
@@ -1033,14 +1047,14 @@ struct EngineT : EngineBase {
         const TileArgs<R, NR> ta = tile_args();
         const FinBlk* fb = (const FinBlk*)cur->b;
         const NodeBlk* nb = (const NodeBlk*)cur->c;
-        const SharedNodes sn{sh_gid, sh_ptr};
+        const SharedNodes sn{sh_gid, sh_ptr, sh_of};
         if (cur->na > 0) k_tile_rhs<R, NR, VT><<<cur->na, kBlock, 0, s>>>(ta, x, v, f, b, r, xcg, sp0, sp1, pbb);
         if (cur->nb > 0) k_fin_rhs<VT><<<cur->nb, kBlock, 0, s>>>(fb, inst, ranges_d, sn, mass, fixed, v, sp0, sp1, f, b, r, xcg, pbb);
         for (int it = 1; it <= cur->maxit; it++) {
             if (cur->na > 0)
-                k_tile_apply<R, NR, VT><<<cur->na, kBlock, 0, s>>>(ta, it, r, p, q, sp0, it == 1 ? pbb : prho, pden, 0);
-            if (cur->nb > 0) k_fin_apply<VT><<<cur->nb, kBlock, 0, s>>>(fb, inst, ranges_d, sn, mass, fixed, it, r, p, q, sp0, pden);
-            if (cur->nc > 0) k_update<VT><<<cur->nc, kBlock, 0, s>>>(nb, inst, ranges_d, it, p, q, xcg, r, pden, prho);
+                k_tile_apply<R, NR, VT, kTileThreads><<<cur->na, kTileThreads, 0, s>>>(ta, it, r, p, q, sp0, it == 1 ? pbb : prho, pden, 0);
+            if (cur->nc > 0)
+                k_update<VT><<<cur->nc, kBlock, 0, s>>>(nb, inst, ranges_d, it, sn, sp0, mass, fixed, p, q, xcg, r, pden, prho);
         }
         if (cur->nc > 0) k_integrate<VT><<<cur->nc, kBlock, 0, s>>>(nb, inst, xcg, v, x);
         if (cur->no > 0) k_oscillate<VT><<<cur->no, kBlock, 0, s>>>((const Blk*)cur->o, inst, oscillators(), x, v);
@@ -1054,7 +1068,7 @@ struct EngineT : EngineBase {
         ta.rot_lo = rot32;
         const FinBlk* fb = (const FinBlk*)cur->b;
         const NodeBlk* nb = (const NodeBlk*)cur->c;
-        const SharedNodes sn{sh_gid, sh_ptr};
+        const SharedNodes sn{sh_gid, sh_ptr, sh_of};
         InnerCg ic;
         ic.ta.tiles = ta.tiles;
         ic.ta.inst = inst_in;
@@ -1076,7 +1090,7 @@ struct EngineT : EngineBase {
         for (int sw = 1; sw <= cur->maxsw + 1; sw++) {
             if (sw > 1) {   // q = A x, FP64
                 if (cur->na > 0)
-                    k_tile_apply<R, NR, VT><<<cur->na, kBlock, 0, s>>>(ta, 0, xcg, nullptr, q, sp0, nullptr, pden, 0);
+                    k_tile_apply<R, NR, VT, kTileThreads><<<cur->na, kTileThreads, 0, s>>>(ta, 0, xcg, nullptr, q, sp0, nullptr, pden, 0);
                 if (cur->nb > 0)
                     k_fin_apply<VT><<<cur->nb, kBlock, 0, s>>>(fb, inst, ranges_d, sn, mass, fixed, 0, xcg, nullptr, q, sp0, pden);
             }
@@ -1135,7 +1149,7 @@ struct EngineT : EngineBase {
         prepare_set(ids.data(), (int)ids.size(), s);
         const TileArgs<R, NR> ta = tile_args();
         for (int i = 0; i < reps; i++)
-            if (cur->na > 0) k_tile_apply<R, NR, VT><<<cur->na, kBlock, 0, s>>>(ta, 2, r, p, q, sp0, prho, pden, 0);
+            if (cur->na > 0) k_tile_apply<R, NR, VT, kTileThreads><<<cur->na, kTileThreads, 0, s>>>(ta, 2, r, p, q, sp0, prho, pden, 0);
         HIPCHK(hipGetLastError());
     }
     // tiles, tile-local nodes, nodes finished by the finalize kernel, shared partial slots,
@@ -1260,12 +1274,10 @@ EngineBase* make_engine_fp32() {
 
 void launch_inner_cg(const InnerCg& c, hipStream_t s) {
     for (int it = 1; it <= c.maxit; it++) {
-        if (c.na > 0) k_tile_apply<float, 6, float><<<c.na, kBlock, 0, s>>>(c.ta, it, c.r, c.p, c.q, c.sp, c.prho, c.pden, 1);
-        if (c.nfb > 0)
-            k_fin_apply<float><<<c.nfb, kBlock, 0, s>>>(c.fb, c.ta.inst, c.ta.ranges, c.sn, c.ta.mass, c.ta.fixed, it, c.r,
-                                                         c.p, c.q, c.sp, c.pden);
+        if (c.na > 0) k_tile_apply<float, 6, float, kTileThreads><<<c.na, kTileThreads, 0, s>>>(c.ta, it, c.r, c.p, c.q, c.sp, c.prho, c.pden, 1);
         if (c.nnb > 0)
-            k_update<float><<<c.nnb, kBlock, 0, s>>>(c.nb, c.ta.inst, c.ta.ranges, it, c.p, c.q, c.d, c.r, c.pden, c.prho);
+            k_update<float><<<c.nnb, kBlock, 0, s>>>(c.nb, c.ta.inst, c.ta.ranges, it, c.sn, c.sp, c.ta.mass, c.ta.fixed,
+                                                      c.p, c.q, c.d, c.r, c.pden, c.prho);
     }
 }
 #else
@@ -1464,6 +1476,14 @@ int sofa_amd_scene_set_fixed(sofa_amd_engine eng, int scene, const int32_t* indi
     for (int i = 0; i < n; i++)
         if (indices[i] < 0 || indices[i] >= s->n_nodes) return fail(SOFA_AMD_EINVAL, "fixed index out of range");
     s->fixed.assign(indices, indices + n);
+    return SOFA_AMD_OK;
+}
+
+int sofa_amd_scene_set_fix_all(sofa_amd_engine eng, int scene, int fix_all) {
+    SceneHost* s = scene_of(eng, scene);
+    if (!s) return fail(SOFA_AMD_EINVAL, "invalid scene");
+    if (!editable(eng, scene)) return fail(SOFA_AMD_ESTATE, "scene already committed");
+    s->fix_all = fix_all != 0;
     return SOFA_AMD_OK;
 }
 

@@ -24,7 +24,7 @@ static __global__ __launch_bounds__(kBlock) void k_ir_residual(const NodeBlk* __
                                                                 const double* __restrict__ q,
                                                                 float* __restrict__ r32, float* __restrict__ d32,
                                                                 double* __restrict__ prr, double* __restrict__ prr32) {
-    __shared__ double sh[kBlock];
+    __shared__ double sh[kBlock / 64];
     const NodeBlk nb = nb_[blockIdx.x];
     if (inst[nb.scene].stopped) return;
     double rr = 0.0, rr32 = 0.0;
@@ -48,8 +48,8 @@ static __global__ __launch_bounds__(kBlock) void k_ir_residual(const NodeBlk* __
             rr32 += dot3d(rf, rf);
         }
     }
-    const double t0 = block_sum(rr, sh);
-    const double t1 = block_sum(rr32, sh);
+    const double t0 = wave_block_sum<kBlock>(rr, sh);
+    const double t1 = wave_block_sum<kBlock>(rr32, sh);
     if (threadIdx.x == 0) { prr[nb.pslot] = t0; prr32[nb.pslot] = t1; }
 }
 
@@ -59,7 +59,7 @@ static __global__ __launch_bounds__(kBlock) void k_ir_control(const int* __restr
                                                                InstDev* __restrict__ inner,
                                                                const Ranges* __restrict__ ranges, int sweep,
                                                                const double* __restrict__ prr) {
-    __shared__ double sh[kBlock];
+    __shared__ double sh[kBlock / 64];
     const int si = scenes[blockIdx.x];
     InstDev* o = &inst[si];
     InstDev* in = &inner[si];

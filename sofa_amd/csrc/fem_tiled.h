@@ -22,18 +22,27 @@
 
 namespace sofa_amd {
 
-constexpr int kTileElems = 512;     // max elements per tile
-constexpr int kTileNodes = 512;     // max distinct nodes per tile
+#ifndef SOFA_AMD_TILE_ELEMS
+#define SOFA_AMD_TILE_ELEMS 512
+#endif
+constexpr int kTileElems = SOFA_AMD_TILE_ELEMS;   // max elements per tile
+constexpr int kTileNodes = SOFA_AMD_TILE_ELEMS;   // max distinct nodes per tile
 constexpr int kNodesPerThread = 4;  // node kernels: 256 threads x 4 nodes per block
+#ifndef SOFA_AMD_TILE_THREADS
+#define SOFA_AMD_TILE_THREADS 256
+#endif
+constexpr int kTileThreads = SOFA_AMD_TILE_THREADS;   // threads of the tile apply kernel
 
 struct TileDesc {
     int scene;
     int e_begin, e_count;     // permuted element range
     int ln_begin, ln_count;   // tile-local node range (into ln_* arrays)
-    int first;                // first tile of its scene (persists the scene's scalars)
+    int flags;                // kTileFirst: first tile of its scene (persists the scene's scalars);
+                              // kTileConsistent: the scene has a consistent MeshMatrixMass
     int pslot;                // slot of this tile's partial in the tile/finalize partial buffers
-    int pad;
+    int inc_begin;            // first entry of the tile's incidence lists (ln_inc)
 };
+constexpr int kTileFirst = 1, kTileConsistent = 2;
 
 struct FinBlk { int scene, begin, end, pslot, bi, pad0, pad1, pad2; };  // shared-node range; bi = index in scene
 struct NodeBlk { int scene, begin, end, pslot, first, pad0, pad1, pad2; };  // node range
@@ -42,12 +51,15 @@ struct TileNodes {
     const int* __restrict__ gid;        // global node id of each tile-local node
     const int* __restrict__ inc_ptr;    // incidence range (absolute, into inc)
     const uint16_t* __restrict__ inc;   // local element slot * 4 + vertex, ascending element id
-    const int* __restrict__ pslot;      // -1: interior; else shared-partial slot of (node, tile)
+    const int* __restrict__ pslot;      // -1: interior; else shared-partial slot of (node, tile),
+                                        // | kSlotOwner in the node's first tile
 };
+constexpr int kSlotOwner = 1 << 30;
 
 struct SharedNodes {
     const int* __restrict__ gid;        // shared (and element-free) nodes
     const int* __restrict__ ptr;        // their partial slots [ptr[i], ptr[i+1]) in tile order
+    const int* __restrict__ of;         // per node: its index in gid / ptr, or -1 (tile-owned)
 };
 
 // per-scene slot ranges of the two partial buffers
@@ -57,12 +69,29 @@ struct Ranges {
     int b0, nb;    // node-block partials (rho buffer)
 };
 
+// fixed-order block sum of the tiled kernels (TB threads): butterfly within each 64-lane wave
+// (both lanes of a pair add the same two values, so every lane ends with the same bits), then
+// the wave sums added in wave order; sh holds TB / 64 doubles
+template <int TB>
+__device__ __forceinline__ double wave_block_sum(double v, double* sh) {
+#pragma unroll
+    for (int m = 32; m > 0; m >>= 1) v = v + __shfl_xor(v, m, 64);
+    __syncthreads();                     // a previous call's readers are done with sh
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double r = sh[0];
+#pragma unroll
+    for (int k = 1; k < TB / 64; k++) r = r + sh[k];
+    return r;
+}
+
 // fixed-order sum of buf[a0 .. a0+na) then buf[c0 .. c0+nc), all threads of the block
+template <int TB = kBlock>
 __device__ __forceinline__ double reduce2(const double* __restrict__ buf, int a0, int na, int c0, int nc, double* sh) {
     double acc = 0.0;
-    for (int k = threadIdx.x; k < na; k += kBlock) acc = acc + buf[a0 + k];
-    for (int k = threadIdx.x; k < nc; k += kBlock) acc = acc + buf[c0 + k];
-    return block_sum(acc, sh);
+    for (int k = threadIdx.x; k < na; k += TB) acc = acc + buf[a0 + k];
+    for (int k = threadIdx.x; k < nc; k += TB) acc = acc + buf[c0 + k];
+    return wave_block_sum<TB>(acc, sh);
 }
 
 template <typename VT>
@@ -143,7 +172,7 @@ __global__ __launch_bounds__(kBlock) void k_tile_rhs(TileArgs<R, NR> a, const VT
     __shared__ R lv[kTileNodes * 3];
     __shared__ R cf[kTileElems * 12];
     __shared__ R cv[kTileElems * 12];
-    __shared__ double sh[kBlock];
+    __shared__ double sh[kBlock / 64];
     const TileDesc td = a.tiles[blockIdx.x];
     const InstDev& in = a.inst[td.scene];
     for (int j = threadIdx.x; j < td.ln_count; j += kBlock) {
@@ -191,7 +220,7 @@ __global__ __launch_bounds__(kBlock) void k_tile_rhs(TileArgs<R, NR> a, const VT
         const int li = td.ln_begin + j;
         const int g = a.tn.gid[li];
         const int s0 = a.tn.inc_ptr[li], s1 = a.tn.inc_ptr[li + 1];
-        const int ps = a.tn.pslot[li];
+        const int ps = a.tn.pslot[li] < 0 ? -1 : (a.tn.pslot[li] & (kSlotOwner - 1));
         if (ps < 0) {
             VT fn[3];
             mass_gravity(in, a.mass, g, fn);
@@ -229,7 +258,7 @@ __global__ __launch_bounds__(kBlock) void k_tile_rhs(TileArgs<R, NR> a, const VT
             st3(spv, ps, pv);
         }
     }
-    const double tot = block_sum(bb, sh);
+    const double tot = wave_block_sum<kBlock>(bb, sh);
     if (threadIdx.x == 0) pbb[td.pslot] = tot;
 }
 
@@ -240,7 +269,7 @@ __global__ __launch_bounds__(kBlock) void k_fin_rhs(const FinBlk* __restrict__ f
                                                      const VT* __restrict__ spf, const VT* __restrict__ spv,
                                                      VT* __restrict__ f, VT* __restrict__ b, VT* __restrict__ r,
                                                      VT* __restrict__ xcg, double* __restrict__ pbb) {
-    __shared__ double sh[kBlock];
+    __shared__ double sh[kBlock / 64];
     const FinBlk fb = fb_[blockIdx.x];
     const InstDev& in = inst[fb.scene];
     const int j = fb.begin + threadIdx.x;
@@ -274,7 +303,7 @@ __global__ __launch_bounds__(kBlock) void k_fin_rhs(const FinBlk* __restrict__ f
         st3(xcg, g, z);
         bb = dot3d(bn, bn);
     }
-    const double own = block_sum(bb, sh);
+    const double own = wave_block_sum<kBlock>(bb, sh);
     if (threadIdx.x == 0) pbb[fb.pslot] = fold_tiles(pbb, ranges[fb.scene], fb.bi) + own;
 }
 
@@ -283,29 +312,29 @@ __global__ __launch_bounds__(kBlock) void k_fin_rhs(const FinBlk* __restrict__ f
 // iter = 0 is a plain operator application q = A r (no CG scalars, p untouched), used for
 // the FP64 residual of iterative refinement.
 // ---------------------------------------------------------------------------------------
-template <typename R, int NR, typename VT>
-__global__ __launch_bounds__(kBlock) void k_tile_apply(TileArgs<R, NR> a, int iter, const VT* __restrict__ r,
-                                                        VT* __restrict__ p, VT* __restrict__ q, VT* __restrict__ sp,
-                                                        const double* __restrict__ pin,   // |b|^2 (iter 1) or r.r partials
-                                                        double* __restrict__ pden,
-                                                        int pin_nodes) {   // 1: iter-1 |b|^2 partials per node block
-    constexpr int EPT = kTileElems / kBlock;   // elements per thread
+template <typename R, int NR, typename VT, int TB = kBlock>
+__global__ __launch_bounds__(TB) void k_tile_apply(TileArgs<R, NR> a, int iter, const VT* __restrict__ r,
+                                                    VT* __restrict__ p, VT* __restrict__ q, VT* __restrict__ sp,
+                                                    const double* __restrict__ pin,   // |b|^2 (iter 1) or r.r partials
+                                                    double* __restrict__ pden,
+                                                    int pin_nodes) {   // 1: iter-1 |b|^2 partials per node block
+    constexpr int EPT = kTileElems / TB;                   // elements per thread
+    constexpr int NPT = (kTileNodes + TB - 1) / TB;        // tile-local nodes per thread
     __shared__ R lu[kTileNodes * 3];
     __shared__ R lc[kTileElems * 12];
     __shared__ uint16_t linc[kTileElems * 4];
     __shared__ int lptr[kTileNodes + 1];
-    __shared__ double sh[kBlock];
+    __shared__ double sh[TB / 64];
     const TileDesc td = a.tiles[blockIdx.x];
-    InstDev* ins = &a.inst[td.scene];
-    if (ins->stopped) return;
-    // ---- 1. issue every independent load first: the element records of this thread (kept in
-    //         registers) and the tile's incidence tables (copied to LDS)
+    const bool consistent = (td.flags & kTileConsistent) != 0;
+    // ---- 1. issue every load that depends on the tile descriptor only: the element records of
+    //         this thread (kept in registers), the tile-local node ids / slots / masks, the
+    //         incidence tables, the scene record
     R rot[EPT][9], rin[EPT][NR], kk[EPT][3], em[EPT];
     uint2 lt[EPT];
-    const bool consistent = ins->mass_kind == kMassMatrix;
 #pragma unroll
     for (int m = 0; m < EPT; m++) {
-        const int k = threadIdx.x + m * kBlock;
+        const int k = threadIdx.x + m * TB;
         if (k < td.e_count) {
             const int64_t e = (int64_t)td.e_begin + k;
             lt[m] = a.ltet[e];
@@ -318,17 +347,41 @@ __global__ __launch_bounds__(kBlock) void k_tile_apply(TileArgs<R, NR> a, int it
             em[m] = consistent ? a.emass[e] : (R)0;
         }
     }
-    const int inc0 = a.tn.inc_ptr[td.ln_begin];
+    int gid[NPT], psl[NPT];
+    int8_t fx[NPT];
+#pragma unroll
+    for (int m = 0; m < NPT; m++) {
+        const int j = threadIdx.x + m * TB;
+        if (j < td.ln_count) {
+            const int li = td.ln_begin + j;
+            gid[m] = a.tn.gid[li];
+            psl[m] = a.tn.pslot[li];
+        }
+    }
+    InstDev* ins = &a.inst[td.scene];
     const int ninc = 4 * td.e_count;
-    for (int j = threadIdx.x; j < ninc; j += kBlock) linc[j] = a.tn.inc[inc0 + j];
-    for (int j = threadIdx.x; j <= td.ln_count; j += kBlock) lptr[j] = a.tn.inc_ptr[td.ln_begin + j] - inc0;
+    for (int j = threadIdx.x; j < ninc; j += TB) linc[j] = a.tn.inc[td.inc_begin + j];
+    for (int j = threadIdx.x; j <= td.ln_count; j += TB) lptr[j] = a.tn.inc_ptr[td.ln_begin + j] - td.inc_begin;
+    if (ins->stopped) return;
+    // ---- 2. the tile's nodes: r and p (dependent on the node ids), masks
+    VT rr_[NPT][3], po_[NPT][3];
+#pragma unroll
+    for (int m = 0; m < NPT; m++) {
+        const int j = threadIdx.x + m * TB;
+        if (j < td.ln_count) {
+            ld3(r, gid[m], rr_[m]);
+            if (iter > 1) ld3(p, gid[m], po_[m]);
+            fx[m] = a.fixed[gid[m]];
+        }
+    }
     const Ranges rg = a.ranges[td.scene];
-    const bool first = td.first != 0;
-    // ---- 2. scalars: rho, the tolerance test, beta (CGLinearSolver.inl:130-196)
+    const bool first = (td.flags & kTileFirst) != 0;
+    // ---- 3. scalars: rho, the tolerance test, beta (CGLinearSolver.inl:130-196)
     double beta = 0.0;
     if (iter == 1) {
-        const double bb = pin_nodes ? reduce2(pin, rg.b0, rg.nb, 0, 0, sh)
-                                    : rg.nf > 0 ? reduce2(pin, rg.f0, rg.nf, 0, 0, sh) : reduce2(pin, rg.t0, rg.nt, 0, 0, sh);
+        const double bb = pin_nodes ? reduce2<TB>(pin, rg.b0, rg.nb, 0, 0, sh)
+                                    : rg.nf > 0 ? reduce2<TB>(pin, rg.f0, rg.nf, 0, 0, sh)
+                                                : reduce2<TB>(pin, rg.t0, rg.nt, 0, 0, sh);
         const double normb = sqrt(bb);
         if (normb == 0.0) { record_stop(ins, first, 4, 0, 1.0); return; }
         const double err = sqrt(bb) / normb;
@@ -338,7 +391,7 @@ __global__ __launch_bounds__(kBlock) void k_tile_apply(TileArgs<R, NR> a, int it
         }
     } else if (iter > 1) {
         if (iter > ins->max_iter) { record_stop(ins, first, 0, ins->max_iter, ins->err); return; }
-        const double rho = reduce2(pin, rg.b0, rg.nb, 0, 0, sh);
+        const double rho = reduce2<TB>(pin, rg.b0, rg.nb, 0, 0, sh);
         const double err = sqrt(rho) / ins->normb;
         if (err <= ins->tol) { record_stop(ins, first, 1, iter, err); return; }
         const double rho_1 = ins->rho_slot[(iter - 1) & 1];
@@ -347,28 +400,28 @@ __global__ __launch_bounds__(kBlock) void k_tile_apply(TileArgs<R, NR> a, int it
             ins->rho_slot[iter & 1] = rho; ins->beta = beta; ins->iter = iter; ins->err = err;
         }
     }
-    // ---- 3. stage p = r + beta p (interior nodes: this tile owns p and writes it back)
+    // ---- 4. stage p = r + beta p (interior nodes: this tile owns p and writes it back)
     const VT vb = (VT)beta;
-    for (int j = threadIdx.x; j < td.ln_count; j += kBlock) {
-        const int li = td.ln_begin + j;
-        const int g = a.tn.gid[li];
-        VT pp[3];
-        ld3(r, g, pp);
-        if (iter > 1) {
-            VT po[3];
-            ld3(p, g, po);
 #pragma unroll
-            for (int i = 0; i < 3; i++) { po[i] *= vb; po[i] += pp[i]; pp[i] = po[i]; }
+    for (int m = 0; m < NPT; m++) {
+        const int j = threadIdx.x + m * TB;
+        if (j < td.ln_count) {
+            VT pp[3] = {rr_[m][0], rr_[m][1], rr_[m][2]};
+            if (iter > 1) {
+#pragma unroll
+                for (int i = 0; i < 3; i++) { VT t = po_[m][i] * vb; t += pp[i]; pp[i] = t; }
+            }
+            if (iter > 0 && psl[m] < 0) st3(p, gid[m], pp);
+            lu[3 * j] = (R)pp[0]; lu[3 * j + 1] = (R)pp[1]; lu[3 * j + 2] = (R)pp[2];
         }
-        if (iter > 0 && a.tn.pslot[li] < 0) st3(p, g, pp);
-        lu[3 * j] = (R)pp[0]; lu[3 * j + 1] = (R)pp[1]; lu[3 * j + 2] = (R)pp[2];
     }
     __syncthreads();
-    // ---- 4. element K p into LDS
+    // ---- 5. element K p into LDS
     const R kcg = (R)ins->k_cg;
+    const R mcg = (R)ins->m_cg;
 #pragma unroll
     for (int m = 0; m < EPT; m++) {
-        const int k = threadIdx.x + m * kBlock;
+        const int k = threadIdx.x + m * TB;
         if (k < td.e_count) {
             int l[4];
             unpack_ltet(lt[m], l);
@@ -378,43 +431,56 @@ __global__ __launch_bounds__(kBlock) void k_tile_apply(TileArgs<R, NR> a, int it
 #pragma unroll
                 for (int i = 0; i < 3; i++) u[3 * n + i] = lu[3 * l[n] + i];
             elem_dforce_r<R, NR>(rot[m], rin[m], kk[m], u, kcg, c);
-            if (consistent) add_edge_mass((R)ins->m_cg * em[m], u, c);
+            if (consistent) add_edge_mass(mcg * em[m], u, c);
 #pragma unroll
             for (int i = 0; i < 12; i++) lc[12 * k + i] = c[i];
         }
     }
     __syncthreads();
-    // ---- 5. per-node sums in ascending element id, from LDS only
+    // ---- 6. per-node sums in ascending element id, from LDS only
     const InstDev& in = *ins;
     double pq = 0.0;
-    for (int j = threadIdx.x; j < td.ln_count; j += kBlock) {
-        const int li = td.ln_begin + j;
-        const int s0 = lptr[j], s1 = lptr[j + 1];
-        const int ps = a.tn.pslot[li];
-        if (ps < 0) {
-            const int g = a.tn.gid[li];
-            const VT pp[3] = {(VT)lu[3 * j], (VT)lu[3 * j + 1], (VT)lu[3 * j + 2]};
-            VT qn[3] = {0, 0, 0};
-            if (in.m_cg != 0.0) mass_mdx(in, NodeArrays{nullptr, nullptr, a.mass, a.fixed}, g, in.m_cg, pp, qn);
-            for (int s = s0; s < s1; s++) {
-                const int sl = linc[s];
 #pragma unroll
-                for (int i = 0; i < 3; i++) qn[i] -= (VT)lc[3 * sl + i];
-            }
-            if (a.fixed[g]) { qn[0] = 0; qn[1] = 0; qn[2] = 0; }
-            st3(q, g, qn);
-            pq += dot3d(pp, qn);
-        } else {
-            VT acc[3] = {0, 0, 0};
-            for (int s = s0; s < s1; s++) {
-                const int sl = linc[s];
+    for (int m = 0; m < NPT; m++) {
+        const int j = threadIdx.x + m * TB;
+        if (j < td.ln_count) {
+            const int s0 = lptr[j], s1 = lptr[j + 1];
+            if (psl[m] < 0) {
+                const VT pp[3] = {(VT)lu[3 * j], (VT)lu[3 * j + 1], (VT)lu[3 * j + 2]};
+                VT qn[3] = {0, 0, 0};
+                if (in.m_cg != 0.0) mass_mdx(in, NodeArrays{nullptr, nullptr, a.mass, a.fixed}, gid[m], in.m_cg, pp, qn);
+                for (int s = s0; s < s1; s++) {
+                    const int sl = linc[s];
 #pragma unroll
-                for (int i = 0; i < 3; i++) acc[i] -= (VT)lc[3 * sl + i];
+                    for (int i = 0; i < 3; i++) qn[i] -= (VT)lc[3 * sl + i];
+                }
+                if (fx[m]) { qn[0] = 0; qn[1] = 0; qn[2] = 0; }
+                st3(q, gid[m], qn);
+                pq += dot3d(pp, qn);
+            } else {
+                VT acc[3] = {0, 0, 0};
+                for (int s = s0; s < s1; s++) {
+                    const int sl = linc[s];
+#pragma unroll
+                    for (int i = 0; i < 3; i++) acc[i] -= (VT)lc[3 * sl + i];
+                }
+                st3(sp, psl[m] & (kSlotOwner - 1), acc);
+                // p.q is linear in q: a shared node's term is split over its tiles (p . this
+                // tile's partial), its mass term added by its first tile, so that the tile
+                // partials hold all of p.q and no later kernel has to add to it
+                if (!fx[m]) {
+                    const VT pp[3] = {(VT)lu[3 * j], (VT)lu[3 * j + 1], (VT)lu[3 * j + 2]};
+                    pq += dot3d(pp, acc);
+                    if ((psl[m] & kSlotOwner) && in.m_cg != 0.0) {
+                        VT mq[3] = {0, 0, 0};
+                        mass_mdx(in, NodeArrays{nullptr, nullptr, a.mass, a.fixed}, gid[m], in.m_cg, pp, mq);
+                        pq += dot3d(pp, mq);
+                    }
+                }
             }
-            st3(sp, ps, acc);
         }
     }
-    const double tot = block_sum(pq, sh);
+    const double tot = wave_block_sum<TB>(pq, sh);
     if (threadIdx.x == 0) pden[td.pslot] = tot;
 }
 
@@ -426,7 +492,7 @@ __global__ __launch_bounds__(kBlock) void k_fin_apply(const FinBlk* __restrict__
                                                        const VT* __restrict__ r, VT* __restrict__ p,
                                                        VT* __restrict__ q, const VT* __restrict__ sp,
                                                        double* __restrict__ pden) {
-    __shared__ double sh[kBlock];
+    __shared__ double sh[kBlock / 64];
     const FinBlk fb = fb_[blockIdx.x];
     const InstDev& in = inst[fb.scene];
     if (in.stopped) return;
@@ -455,31 +521,47 @@ __global__ __launch_bounds__(kBlock) void k_fin_apply(const FinBlk* __restrict__
         st3(q, g, qn);
         pq = dot3d(pp, qn);
     }
-    const double own = block_sum(pq, sh);
+    const double own = wave_block_sum<kBlock>(pq, sh);
     if (threadIdx.x == 0) pden[fb.pslot] = fold_tiles(pden, ranges[fb.scene], fb.bi) + own;
 }
 
+// CG iteration `iter` after the tile apply.  Nodes shared by several tiles are finished here:
+// p = r + beta p, q = (1 + h rm) M p + their tile partials in tile order, projection (the work
+// CGLinearSolver's p update and GraphScatteredMatrix::apply do for them).  Then den = p.q from
+// the tile partials (which already hold every node's term), the threshold test and alpha
+// (CGLinearSolver.inl:207-265), x += alpha p, r -= alpha q and the r.r partials.
 template <typename VT>
 __global__ __launch_bounds__(kBlock) void k_update(const NodeBlk* __restrict__ nb_, InstDev* __restrict__ inst,
-                                                    const Ranges* __restrict__ ranges, int iter,
-                                                    const VT* __restrict__ p, const VT* __restrict__ q,
-                                                    VT* __restrict__ xcg, VT* __restrict__ r,
+                                                    const Ranges* __restrict__ ranges, int iter, SharedNodes sn,
+                                                    const VT* __restrict__ sp, const double* __restrict__ mass,
+                                                    const int8_t* __restrict__ fixed, VT* __restrict__ p,
+                                                    const VT* __restrict__ q, VT* __restrict__ xcg, VT* __restrict__ r,
                                                     const double* __restrict__ pden, double* __restrict__ prho) {
-    __shared__ double sh[kBlock];
+    __shared__ double sh[kBlock / 64];
     const NodeBlk nb = nb_[blockIdx.x];
     InstDev* ins = &inst[nb.scene];
-    if (ins->stopped) return;
     // node data first (registers), then the reduction
     VT pv[kNodesPerThread][3], qv[kNodesPerThread][3], xv[kNodesPerThread][3], rv[kNodesPerThread][3];
+    int shk[kNodesPerThread];
 #pragma unroll
     for (int k = 0; k < kNodesPerThread; k++) {
         const int n = nb.begin + k * kBlock + threadIdx.x;
-        if (n < nb.end) { ld3(p, n, pv[k]); ld3(q, n, qv[k]); ld3(xcg, n, xv[k]); ld3(r, n, rv[k]); }
+        if (n < nb.end) {
+            shk[k] = sn.of[n];
+            ld3(xcg, n, xv[k]);
+            ld3(r, n, rv[k]);
+            if (shk[k] < 0) {
+                ld3(p, n, pv[k]);
+                ld3(q, n, qv[k]);
+            } else if (iter > 1) {
+                ld3(p, n, pv[k]);
+            }
+        }
     }
+    if (ins->stopped) return;
     const Ranges rg = ranges[nb.scene];
     const bool first = nb.first != 0;
-    // den = p.q, threshold test, alpha (CGLinearSolver.inl:207-265)
-    const double den = rg.nf > 0 ? reduce2(pden, rg.f0, rg.nf, 0, 0, sh) : reduce2(pden, rg.t0, rg.nt, 0, 0, sh);
+    const double den = reduce2(pden, rg.t0, rg.nt, 0, 0, sh);
     const bool exempt = (iter == 1 && ins->first_solve);
     if (den == 0.0) { record_stop(ins, first, 3, iter, ins->err); if (first && threadIdx.x == 0) ins->den = den; return; }
     if (fabs(den) <= ins->thr && !exempt) {
@@ -489,12 +571,32 @@ __global__ __launch_bounds__(kBlock) void k_update(const NodeBlk* __restrict__ n
     }
     const double alpha = ins->rho_slot[iter & 1] / den;
     if (first && threadIdx.x == 0) { ins->den = den; ins->alpha = alpha; }
+    const InstDev& in = *ins;
+    const VT vb = (VT)in.beta;
     const VT va = (VT)alpha, vma = (VT)(-alpha);
     double rr = 0.0;
 #pragma unroll
     for (int k = 0; k < kNodesPerThread; k++) {
         const int n = nb.begin + k * kBlock + threadIdx.x;
         if (n < nb.end) {
+            if (shk[k] >= 0) {
+                VT pp[3] = {rv[k][0], rv[k][1], rv[k][2]};
+                if (iter > 1) {
+#pragma unroll
+                    for (int i = 0; i < 3; i++) { VT t = pv[k][i] * vb; t += pp[i]; pp[i] = t; }
+                }
+                st3(p, n, pp);
+                VT qn[3] = {0, 0, 0};
+                if (in.m_cg != 0.0) mass_mdx(in, NodeArrays{nullptr, nullptr, mass, fixed}, n, in.m_cg, pp, qn);
+                const int s0 = sn.ptr[shk[k]], s1 = sn.ptr[shk[k] + 1];
+                for (int s = s0; s < s1; s++) {
+#pragma unroll
+                    for (int i = 0; i < 3; i++) qn[i] += sp[3 * (int64_t)s + i];
+                }
+                if (fixed[n]) { qn[0] = 0; qn[1] = 0; qn[2] = 0; }
+#pragma unroll
+                for (int i = 0; i < 3; i++) { pv[k][i] = pp[i]; qv[k][i] = qn[i]; }
+            }
             VT xn[3], rn[3];
 #pragma unroll
             for (int i = 0; i < 3; i++) {
@@ -506,7 +608,7 @@ __global__ __launch_bounds__(kBlock) void k_update(const NodeBlk* __restrict__ n
             rr += dot3d(rn, rn);
         }
     }
-    const double tot = block_sum(rr, sh);
+    const double tot = wave_block_sum<kBlock>(rr, sh);
     if (threadIdx.x == 0) prho[nb.pslot] = tot;
 }
 

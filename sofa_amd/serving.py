@@ -44,31 +44,47 @@ class Server:
     """One GPU, one live engine, a FIFO admission queue bounded by an HBM budget."""
 
     def __init__(self, precision=capi.FP32, budget_bytes=8 << 30, max_scenes=2048, batched=True,
-                 build_workers=4):
+                 build_workers=4, engine=None):
         self.precision = precision
         self.budget = int(budget_bytes)
         self.batched = batched
-        # arena capacity: the budget expressed in nodes and tetrahedra (a tetrahedral mesh has
-        # about 5-6 tetrahedra per node; the node arena is sized for the sparser side)
-        per_tet = footprint_bytes(precision, 0, 1)
-        max_tets = self.budget // per_tet
-        max_nodes = max_tets // 3
-        self.eng = capi.Engine(0, precision)
-        self.eng.reserve(max_nodes, max_tets, max_scenes)
+        if engine is None:
+            # arena capacity: the budget expressed in nodes and tetrahedra (a tetrahedral mesh
+            # has about 5-6 tetrahedra per node; the node arena is sized for the sparser side)
+            per_tet = footprint_bytes(precision, 0, 1)
+            max_tets = self.budget // per_tet
+            max_nodes = max_tets // 3
+            engine = capi.Engine(0, precision)
+            engine.reserve(max_nodes, max_tets, max_scenes)
+        self.eng = engine
         self.pool = cf.ThreadPoolExecutor(build_workers)
 
     def close(self):
         self.pool.shutdown(wait=False, cancel_futures=True)
         self.eng.close()
 
-    def run(self, jobs, wall_limit=30.0, stream=None, build=workloads.build_job, log=None, on_finish=None):
+    def run(self, jobs, wall_limit=30.0, stream=None, build=workloads.build_job, log=None, on_finish=None,
+            event=None):
         """Serve `jobs` (dicts with `arrival` seconds and `steps`, built by `build`) until every
         job is done or `wall_limit` seconds have passed.  `on_finish(engine, scene, job_index)` is
-        called for each finished job before its scene is released.  Returns the measurements."""
-        import torch
-        if stream is None:
-            stream = torch.cuda.Stream()
-        sp = stream.cuda_stream
+        called for each finished job before its scene is released.  `event()` makes the timing
+        events (HIP events by default).  Returns the measurements."""
+        if event is None:
+            import torch
+            if stream is None:
+                stream = torch.cuda.Stream()
+            sp = stream.cuda_stream
+
+            def event():
+                return torch.cuda.Event(enable_timing=True)
+
+            def sync_all():
+                torch.cuda.synchronize()
+        else:
+            sp = None
+
+            def sync_all():
+                pass
         jobs = sorted(jobs, key=lambda j: j["arrival"])
         specs = {}                                    # job index -> Future of its scene spec
         nxt = 0                                       # next job not yet arrived
@@ -99,6 +115,8 @@ class Server:
             # admission: FIFO while the head fits the budget
             while queue:
                 j = queue[0]
+                if j not in specs:
+                    prefetch(j)
                 spec = specs[j].result()
                 need = footprint_bytes(self.precision, len(spec["pos"]), len(spec["tets"]))
                 if used + need > self.budget and active:
@@ -126,8 +144,8 @@ class Server:
                 time.sleep(0.001)
                 continue
             ids = list(active.keys())
-            e0 = torch.cuda.Event(enable_timing=True)
-            e1 = torch.cuda.Event(enable_timing=True)
+            e0 = event()
+            e1 = event()
             e0.record(stream)
             if self.batched:
                 self.eng.step(ids, 1, stream=sp)
@@ -156,7 +174,7 @@ class Server:
                 done += 1
             if log is not None and len(step_records) % 50 == 0:
                 log(f"t={now:.1f}s ticks={len(step_records)} active={len(active)} queue={len(queue)} done={done}")
-        torch.cuda.synchronize()
+        sync_all()
         wall = time.perf_counter() - t0
         lat = []
         for e0, e1, tets in step_records:

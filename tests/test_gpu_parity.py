@@ -595,3 +595,28 @@ def test_served_jobs_match_alone():
         xr, vr, _ = ref.read(rid)
         assert np.array_equal(xr, finals[j][0]) and np.array_equal(vr, finals[j][1]), j
         ref.close()
+
+
+@pytest.mark.parametrize("prec", ["FP64", "FP32"])
+def test_fix_all(prec):
+    """FixedProjectiveConstraint fixAll: every response cleared, the body stays where it is
+    (with an initial velocity it drifts rigidly by h v per step, as the oracle does)."""
+    capi = _capi()
+    print(f"case: fixAll {prec}", flush=True)
+    c = beam_cfg0()
+    eng = capi.Engine(0, getattr(capi, prec))
+    sid = add_gpu_scene(eng, c)
+    eng.set_fix_all(sid)
+    eng.commit()
+    v0 = np.tile([0.01, 0.0, -0.02], (len(c["pos"]), 1))
+    eng.write(sid, v=v0)
+    eng.step([sid], 5)
+    x, v, _ = eng.read(sid)
+    o = make_oracle(c)
+    o.set_fixed(c["fixed"], fix_all=True)
+    o.set("v", v0)
+    o.step(c["dt"], 5)
+    tol = 0.0 if prec == "FP64" else 1e-6
+    assert rel_err(x, o.get("x")) <= tol and np.max(np.abs(v - v0)) <= tol
+    assert eng.stats(sid)["end_reason"] == 4
+    eng.close()

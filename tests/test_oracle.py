@@ -307,3 +307,25 @@ def test_meshmatrixmass_consistent_mass_textbook():
     # addMDx through the step operator: one implicit step with K = 0 (E tiny) moves as M^-1 f
     dx = rng.standard_normal((n, 3))
     assert np.allclose(M @ dx, Mref @ dx)
+
+
+def test_threaded_oracle_bit_identical():
+    """The threaded element loops (the CPU baseline on all cores) give the serial oracle's bits."""
+    from oracle import OracleScene
+    from sofa_amd import meshes
+    pos, tets, fixed = meshes.beam((5, 4, 9), (1.0, 0.8, 2.0))
+    rng = np.random.default_rng(5)
+    E = 10 ** rng.uniform(4, 5, len(tets))
+    out = []
+    for threads in (1, 4):
+        s = OracleScene(pos)
+        s.set_gravity((0.0, -9.81, 0.0))
+        s.set_fem(tets, E, [0.4], "large")
+        s.set_meshmatrix_mass([1000.0], lumping=False)
+        s.set_fixed(fixed)
+        s.set_solver(0.1, 0.1, 20, 1e-12, 1e-20)
+        s.set_threads(threads)
+        s.step(0.01, 5)
+        out.append((s.get("x"), s.get("v"), s.stats()))
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+    assert out[0][2] == out[1][2]

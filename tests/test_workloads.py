@@ -60,3 +60,106 @@ def test_footprint_monotonic():
         assert serving.footprint_bytes(prec, 2000, 5000) > a and serving.footprint_bytes(prec, 1000, 6000) > a
     assert serving.footprint_bytes(capi.FP64_TILED, 1000, 5000) > serving.footprint_bytes(capi.FP32, 1000, 5000)
     assert serving.footprint_bytes(capi.MIXED, 1000, 5000) > serving.footprint_bytes(capi.FP64_TILED, 1000, 5000)
+
+
+class _FakeEngine:
+    """Host stand-in for a live engine: arenas as node / tetrahedron counters, a step log."""
+
+    def __init__(self, max_nodes, max_tets):
+        self.cap = (max_nodes, max_tets)
+        self.used = [0, 0]
+        self.scenes = []          # (nodes, tets)
+        self.placed = set()
+        self.steps = {}
+        self.log = []
+
+    def add_scene(self, positions, tets, *a, **k):
+        self.scenes.append((len(positions), len(tets)))
+        return len(self.scenes) - 1
+
+    def set_refinement(self, *a, **k):
+        pass
+
+    def commit_scene(self, sid):
+        n, t = self.scenes[sid]
+        if self.used[0] + n > self.cap[0] or self.used[1] + t > self.cap[1]:
+            return False
+        self.used[0] += n
+        self.used[1] += t
+        self.placed.add(sid)
+        self.steps[sid] = 0
+        return True
+
+    def step(self, ids, n_steps=1, stream=None):
+        assert all(i in self.placed for i in ids)
+        self.log.append(tuple(ids))
+        for i in ids:
+            self.steps[i] += n_steps
+
+    def release_scene(self, sid):
+        n, t = self.scenes[sid]
+        self.placed.remove(sid)
+        self.used[0] -= n
+        self.used[1] -= t
+
+    def close(self):
+        pass
+
+
+class _FakeEvent:
+    def record(self, stream=None):
+        import time
+        self.t = time.perf_counter()
+
+    def elapsed_time(self, other):
+        return (other.t - self.t) * 1e3
+
+    def synchronize(self):
+        pass
+
+
+def _tiny_jobs(n, seed=1):
+    rng = np.random.default_rng(seed)
+    return [dict(arrival=0.002 * k, steps=int(rng.integers(2, 6)), kind="beam",
+                 dims=(3, 3, int(rng.integers(3, 8))), E=1e5, nu=0.3, amp=0.0, freq=1.0) for k in range(n)]
+
+
+def test_serving_scheduler_batched():
+    """Every job runs exactly its number of steps, merged ticks hold several jobs, the budget is
+    never exceeded, arrivals are admitted first come first served."""
+    from sofa_amd import capi, serving
+    jobs = _tiny_jobs(24)
+    budget = 3 * serving.footprint_bytes(capi.FP32, 3 * 3 * 8, 5 * 2 * 2 * 7)
+    eng = _FakeEngine(10 ** 6, 10 ** 7)
+    seen = []
+    srv = serving.Server(capi.FP32, budget_bytes=budget, engine=eng, batched=True, build_workers=2)
+    res = srv.run(jobs, wall_limit=30.0, event=_FakeEvent, on_finish=lambda e, s, j: seen.append((s, j)))
+    srv.close()
+    assert res["jobs_done"] == 24 and res["jobs_rejected"] == 0 and res["queue_at_end"] == 0
+    assert sorted(j for _, j in seen) == list(range(24))
+    for s, j in seen:
+        assert eng.steps[s] == jobs[j]["steps"]
+    assert max(len(t) for t in eng.log) >= 2                       # merged ticks
+    for t in eng.log:                                              # the budget is never exceeded
+        assert sum(serving.footprint_bytes(capi.FP32, *eng.scenes[s]) for s in t) <= budget
+    first_tick = {}
+    for k, t in enumerate(eng.log):
+        for s in t:
+            first_tick.setdefault(s, k)
+    ticks_by_job = [first_tick[s] for s, j in sorted(seen, key=lambda p: p[1])]
+    assert ticks_by_job == sorted(ticks_by_job)                    # FIFO admission
+    assert res["element_steps"] == sum(eng.scenes[s][1] * eng.steps[s] for s, _ in seen)
+
+
+def test_serving_scheduler_plain_and_refusal():
+    """Plain mode steps one scene per launch; a job larger than the whole reservation is refused
+    instead of blocking the queue forever."""
+    from sofa_amd import capi, serving
+    jobs = _tiny_jobs(6, seed=3)
+    jobs[2] = dict(jobs[2], dims=(9, 9, 40))
+    eng = _FakeEngine(2000, 10 ** 7)
+    srv = serving.Server(capi.FP32, budget_bytes=1 << 40, engine=eng, batched=False, build_workers=2)
+    res = srv.run(jobs, wall_limit=30.0, event=_FakeEvent)
+    srv.close()
+    assert res["jobs_rejected"] == 1 and res["jobs_done"] == 5
+    assert all(len(t) == 1 for t in eng.log)
