@@ -150,58 +150,79 @@ def copy_probe(mb=512, reps=5):
 
 
 def position_error(specs, precision, steps):
-    """Positions of `precision` vs the FP64 parity path after `steps` steps, per scene:
-    eps_rel = max_i |x_i - x64_i| / max_i |x64_i|, eps_disp = max_i |x_i - x64_i| / max_i |x64_i - x0_i|.
-    The FP64 parity path is bit-identical to the CPU oracle replaying the GPU's dot order and
-    within 1e-10 of the oracle in the reference's order (tests/test_gpu_parity.py)."""
+    """Positions after `steps` steps against the reference trajectory: the FP64 parity path with
+    the reference's sequential dot order, which equals the CPU oracle's reference-order run bit
+    for bit (tests/test_gpu_parity.py).  Per scene eps_rel = max_i |x_i - xref_i| / max_i |xref_i|
+    and eps_disp = max_i |x_i - xref_i| / max_i |xref_i - x0_i|.  Beside the tested precision, the
+    FP64 parity path with the parallel (blocked) dot order: it differs from the reference only in
+    the association of the dot-product sums, so its error measures how much the scene itself
+    amplifies rounding over the steps."""
     from sofa_amd import capi
+    runs = (("reference", capi.FP64, True), ("test", precision, False), ("fp64_blocked_dots", capi.FP64, False))
     res = {}
-    for name, prec in (("fp64", capi.FP64), ("test", precision)):
+    for name, prec, seq in runs:
         eng, ids = make_engine(specs, prec, graphs=True)
+        if seq:
+            eng.set_dot_order(True)
         eng.step(ids, steps)
         res[name] = [eng.read(i)[0] for i in ids]
         eng.close()
-    er, ed = [], []
-    for x, x64, s in zip(res["test"], res["fp64"], specs):
-        d = np.linalg.norm(x - x64, axis=1).max()
-        er.append(d / np.linalg.norm(x64, axis=1).max())
-        disp = np.linalg.norm(x64 - s["pos"], axis=1).max()
-        ed.append(d / disp if disp > 0 else 0.0)
-    return {"steps": steps, "eps_rel_max": float(max(er)), "eps_rel_median": float(np.median(er)),
-            "eps_disp_max": float(max(ed)), "scenes": len(specs)}
+    out = {"steps": steps, "scenes": len(specs), "reference": "FP64 parity path, sequential dots (= the oracle, bitwise)"}
+    for name in ("test", "fp64_blocked_dots"):
+        er, ed = [], []
+        for x, xr, s in zip(res[name], res["reference"], specs):
+            d = np.linalg.norm(x - xr, axis=1).max()
+            er.append(d / np.linalg.norm(xr, axis=1).max())
+            disp = np.linalg.norm(xr - s["pos"], axis=1).max()
+            ed.append(d / disp if disp > 0 else 0.0)
+        er = np.asarray(er)
+        out[name] = {"eps_rel_max": float(er.max()), "eps_rel_median": float(np.median(er)),
+                     "eps_rel_p90": float(np.percentile(er, 90)), "scenes_within_1e-6": int((er <= 1e-6).sum()),
+                     "eps_disp_max": float(max(ed))}
+    return out
 
 
 def cfg2_mixed(steps, err_steps):
-    """Ellipsoid workload: MIXED vs FP64 tiled (CG 90 / 1e-8) throughput, and both trajectories
-    against FP64 tiled run to |r|/|b| <= 1e-11 (400 iterations), after `err_steps` steps."""
+    """Ellipsoid workload.  Throughput: MIXED (FP32 inner CG of 30 iterations, at most 3 FP64
+    refinement sweeps) against the FP64 tiled path with the same iteration budget (CG 90), both
+    at tolerance 1e-8 -- on this mesh neither reaches it (the last residual is reported): the
+    systems need thousands of CG iterations.  Accuracy, on the same kind of mesh at 12 mm pitch
+    where the solves can be run to |r|/|b| <= 1e-10 (FP64 CG up to 6000 iterations; MIXED 30 FP32
+    iterations per sweep, up to 300 sweeps): MIXED against FP64 after `err_steps` steps."""
     from sofa_amd import capi, workloads
     c = workloads.cfg2()
     E, N = len(c["tets"]), len(c["pos"])
     out = {"workload": f"ellipsoid 0.15x0.10x0.08 m, jittered-grid Delaunay h=2.93 mm: {E} tets, {N} nodes, "
                        "E 10^U(3,5), nu U(0.40,0.49) per element, MeshMatrixMass 1050 consistent, x<-0.13 fixed, "
                        "x>0.13 oscillating 5 mm at 1 Hz, dt 0.01, CG tolerance 1e-8"}
+    for name, prec in (("mixed", capi.MIXED), ("fp64_tiled", capi.FP64_TILED)):
+        eng, ids = make_engine([c], prec, graphs=True)
+        t, lat = time_merged(eng, ids, steps, 2)
+        st = eng.stats(ids[0])
+        out[name] = {"value": E * steps / (t / 1e3), "ms_per_step": t / steps,
+                     "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+                     "last_solve": {"iterations": st["iterations"], "ir_sweeps": st["ir_sweeps"],
+                                    "err": st["err"], "end": capi.END_REASONS.get(st["end_reason"])}}
+        eng.close()
+    out["mixed_over_fp64"] = out["mixed"]["value"] / out["fp64_tiled"]["value"]
+    # accuracy at convergence
+    cs = workloads.cfg2(pitch=0.012)
     xs = {}
-    for name, prec, spec in (("mixed", capi.MIXED, c), ("fp64_tiled", capi.FP64_TILED, c),
-                             ("fp64_converged", capi.FP64_TILED, dict(c, iters=400, tol=1e-11))):
-        eng, ids = make_engine([spec], prec, graphs=True)
-        if name != "fp64_converged":
-            t, lat = time_merged(eng, ids, steps, 2)
-            st = eng.stats(ids[0])
-            out[name] = {"value": E * steps / (t / 1e3), "ms_per_step": t / steps,
-                         "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
-                         "last_solve": {"iterations": st["iterations"], "ir_sweeps": st["ir_sweeps"],
-                                        "err": st["err"], "end": capi.END_REASONS.get(st["end_reason"])}}
-        eng.close()
-        eng, ids = make_engine([spec], prec, graphs=True)
+    for name, prec, spec in (("mixed", capi.MIXED, dict(cs, tol=1e-10, refinement=dict(inner_iterations=30,
+                                                                                        inner_tolerance=1e-6,
+                                                                                        max_sweeps=300))),
+                             ("fp64", capi.FP64_TILED, dict(cs, iters=6000, tol=1e-10))):
+        eng, ids = make_engine([spec], prec, graphs=False)
         eng.step(ids, err_steps)
-        xs[name] = eng.read(ids[0])[0]
+        xs[name] = (eng.read(ids[0])[0], eng.stats(ids[0]))
         eng.close()
-    ref = xs["fp64_converged"]
-    for name in ("mixed", "fp64_tiled"):
-        d = np.linalg.norm(xs[name] - ref, axis=1).max()
-        out[name]["eps_rel_vs_converged"] = float(d / np.linalg.norm(ref, axis=1).max())
-        out[name]["eps_disp_vs_converged"] = float(d / np.linalg.norm(ref - c["pos"], axis=1).max())
-    out["error_steps"] = err_steps
+    xr = xs["fp64"][0]
+    d = np.linalg.norm(xs["mixed"][0] - xr, axis=1).max()
+    out["accuracy"] = {"mesh": f"same recipe at 12 mm pitch: {len(cs['tets'])} tets, {len(cs['pos'])} nodes",
+                       "steps": err_steps, "eps_rel": float(d / np.linalg.norm(xr, axis=1).max()),
+                       "eps_disp": float(d / np.linalg.norm(xr - cs["pos"], axis=1).max()),
+                       "mixed_last_solve": {k: xs["mixed"][1][k] for k in ("iterations", "ir_sweeps", "err")},
+                       "fp64_last_solve": {k: xs["fp64"][1][k] for k in ("iterations", "err")}}
     return out
 
 

@@ -149,6 +149,10 @@ int sofa_amd_engine_reserve(sofa_amd_engine eng, int64_t max_nodes, int64_t max_
 /* Places one created scene in the reserved arenas; SOFA_AMD_ENOMEM (and nothing kept) if they
  * have no room for it now. */
 int sofa_amd_scene_commit(sofa_amd_engine eng, int scene);
+/* Builds a created scene's host tables (tiling, element set-up) ahead of its commit, which then
+ * only places and uploads them.  May run on another host thread than the one stepping the
+ * engine (several scenes in parallel); the scene must not be changed or committed meanwhile. */
+int sofa_amd_scene_prepare(sofa_amd_engine eng, int scene);
 /* Waits for the device, then returns the scene's ranges to the arenas; its id stays invalid. */
 int sofa_amd_scene_release(sofa_amd_engine eng, int scene);
 /* Arena use: out22[2k] used, out22[2k+1] capacity, k = nodes, elements, incidences, tiles,
@@ -160,6 +164,13 @@ int sofa_amd_engine_usage(sofa_amd_engine eng, int64_t* out22);
  * set is captured on its first step and kept until the set changes).  Off by default.  No
  * reference counterpart: SOFA has no device graphs. */
 int sofa_amd_engine_set_graphs(sofa_amd_engine eng, int enable);
+
+/* SOFA_AMD_FP64 engines: order of the CG dot products (|b|^2, p.q, r.r).  0 (default): per
+ * 256-node block sums combined in a fixed order (parallel, deterministic).  1: the reference's
+ * sequential order, node after node from 0 (MechanicalObject::vDot, MechanicalObject.inl
+ * :2334-2356), so that trajectories equal the reference's bit for bit however sensitive the
+ * scene; costs one dependent add per node and dot. */
+int sofa_amd_engine_set_dot_order(sofa_amd_engine eng, int order);
 
 /* Blocks until the engine's work is done. */
 int sofa_amd_engine_sync(sofa_amd_engine eng);

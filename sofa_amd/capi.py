@@ -28,7 +28,8 @@ EXPORTS = [
     "sofa_amd_fem_add_force", "sofa_amd_fem_add_dforce", "sofa_amd_engine_bench_apply", "sofa_amd_scene_tiling",
     "sofa_amd_engine_set_graphs", "sofa_amd_scene_set_mass_density", "sofa_amd_scene_add_oscillators",
     "sofa_amd_scene_set_refinement", "sofa_amd_engine_reserve", "sofa_amd_scene_commit", "sofa_amd_scene_release",
-    "sofa_amd_engine_usage", "sofa_amd_scene_set_fix_all",
+    "sofa_amd_engine_usage", "sofa_amd_scene_set_fix_all", "sofa_amd_engine_set_dot_order",
+    "sofa_amd_scene_prepare",
 ]
 
 
@@ -93,6 +94,8 @@ def lib():
         "sofa_amd_scene_release": (c_int, [_vp, c_int]),
         "sofa_amd_engine_usage": (c_int, [_vp, ctypes.POINTER(ctypes.c_int64)]),
         "sofa_amd_scene_set_fix_all": (c_int, [_vp, c_int, c_int]),
+        "sofa_amd_engine_set_dot_order": (c_int, [_vp, c_int]),
+        "sofa_amd_scene_prepare": (c_int, [_vp, c_int]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -195,6 +198,11 @@ class Engine:
         check(rc)
         return True
 
+    def prepare_scene(self, scene):
+        """Host tables of a created scene, ahead of commit_scene; callable from a worker thread
+        (the call releases the GIL)."""
+        check(lib().sofa_amd_scene_prepare(self._h, int(scene)))
+
     def release_scene(self, scene):
         check(lib().sofa_amd_scene_release(self._h, int(scene)))
 
@@ -222,6 +230,11 @@ class Engine:
         """MIXED engines: FP32 inner CG limits and FP64 refinement sweeps per step."""
         check(lib().sofa_amd_scene_set_refinement(self._h, int(scene), int(inner_iterations),
                                                   float(inner_tolerance), int(max_sweeps)))
+
+    def set_dot_order(self, sequential=True):
+        """FP64 engines: dot products in the reference's sequential node order (bit-exact
+        trajectories) instead of the parallel blocked order."""
+        check(lib().sofa_amd_engine_set_dot_order(self._h, 1 if sequential else 0))
 
     def set_graphs(self, enable=True):
         check(lib().sofa_amd_engine_set_graphs(self._h, 1 if enable else 0))

@@ -16,7 +16,9 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <deque>
 #include <limits>
+#include <mutex>
 #include <array>
 #include <map>
 #include <memory>
@@ -222,7 +224,8 @@ struct EngineBase {
     int device = 0;
     int precision = SOFA_AMD_FP64;
     hipStream_t stream = nullptr;
-    std::vector<SceneHost> scenes;
+    std::deque<SceneHost> scenes;     // a deque: references stay valid while scenes are added
+    std::mutex scenes_mu;             // guards the container against a concurrent scene_prepare
     bool committed = false;
     bool use_graphs = false;
     virtual ~EngineBase() {}
@@ -240,11 +243,13 @@ struct EngineBase {
     // live scenes (tiled engines): capacity, one scene placed / released at a time
     virtual bool live() const = 0;
     virtual void reserve(const Extent& c) = 0;
+    virtual void prepare_scene(int scene) = 0;
     virtual bool commit_scene(int scene) = 0;
     virtual void release_scene(int scene) = 0;
     virtual bool placed(int scene) const = 0;
     virtual void usage(int64_t* out22) const = 0;
     bool live_mode = false;           // reserved: scenes are committed and released one by one
+    bool seq_dots = false;            // exact path: dot products in the reference's sequential order
 };
 
 template <typename T>
@@ -284,6 +289,7 @@ struct EngineT : EngineBase {
     VT *ef = nullptr, *ef2 = nullptr;
     std::vector<std::vector<int>> iperm;           // per scene: original element -> position
     VT* p2 = nullptr;                              // previous CG direction (exact path)
+    double* nd = nullptr;                          // per-node dot terms (exact path, reference order)
     int *em_ptr = nullptr, *em_nbr = nullptr;      // MeshMatrixMass edges per node (exact path)
     double* em_val = nullptr;
     R* emass = nullptr;                            // per element rho V / 20 (tiled path)
@@ -345,7 +351,7 @@ struct EngineT : EngineBase {
                         inst, partial, tiles_d, ranges_d, ltet, ln_gid, ln_inc_ptr, ln_pslot, sh_gid, sh_ptr,
                         ln_inc, sp0, sp1, pbb, pden, prho, p2, em_ptr, em_nbr, em_val, emass, osc_node,
                         osc_mean, osc_amp, osc_w, osc_phase, rot32, rest32, kmat32, emass32, r32, p32, q32,
-                        d32, sp32, pden32, prho32, prr, inst_in, sh_of};
+                        d32, sp32, pden32, prho32, prr, inst_in, sh_of, nd};
         for (void* ptr : ptrs)
             if (ptr) (void)hipFree(ptr);
     }
@@ -430,8 +436,7 @@ struct EngineT : EngineBase {
     };
     // TetrahedronFEMForceField::reinit per element (rotation, rest frame, K), the masses' vertex
     // terms, FixedProjectiveConstraint / oscillator masks, the incidence in ascending element id
-    void setup_scene(int si, const std::vector<int>& ip, ElemHost& h) {
-        const SceneHost& s = scenes[si];
+    void setup_scene(const SceneHost& s, int si, const std::vector<int>& ip, ElemHost& h) const {
         const int nE = (int)(s.tets.size() / 4), n = s.n_nodes;
         h.nE = nE;
         h.P.resize(n);
@@ -585,7 +590,7 @@ struct EngineT : EngineBase {
             ip.resize(nE);
             for (int e = 0; e < nE; e++) ip[e] = e;
             ElemHost h;
-            setup_scene(si, ip, h);
+            setup_scene(s, si, ip, h);
             for (int i = 0; i < s.n_nodes; i++) {
                 for (int k = 0; k < 3; k++) xh[3 * (nb0 + (int64_t)i) + k] = s.rest[3 * i + k];
                 massh[nb0 + i] = h.mass[i];
@@ -640,6 +645,7 @@ struct EngineT : EngineBase {
         ef2 = dalloc<VT>(12 * Etot);
         p2 = dalloc<VT>(3 * Ntot);
         HIPCHK(hipMemset(p2, 0, 3 * Ntot * sizeof(VT)));
+        nd = dalloc<double>(Ntot);
         upload(inst, inst_h);
         partial = dalloc<double>(nblk_h.size());
         HIPCHK(hipMemset(partial, 0, std::max<size_t>(1, nblk_h.size()) * sizeof(double)));
@@ -650,20 +656,19 @@ struct EngineT : EngineBase {
     struct TiledHost {
         TilePlan plan;
         ElemHost eh;
+        std::vector<int> iperm;            // original element -> position
         std::vector<int> sh_gid, sh_ptr;   // nodes finished by the shared-node kernel
         Extent need;
     };
-    void build_tiled(int si, TiledHost& th) {
-        const SceneHost& s = scenes[si];
+    // host tables of one scene (tiling, element set-up); touches no engine state, so it may run
+    // on another host thread than the one stepping the engine
+    void build_tiled(const SceneHost& s, int si, TiledHost& th) const {
         const int nE = (int)(s.tets.size() / 4), n = s.n_nodes;
-        std::vector<int>& ip = iperm[si];
         if (nE > 0) {
             th.plan = plan_tiles(n, s.rest.data(), s.tets.data(), nE, kTileElems, kTileNodes);
-            ip = th.plan.iperm;
-        } else {
-            ip.clear();
+            th.iperm = th.plan.iperm;
         }
-        setup_scene(si, ip, th.eh);
+        setup_scene(s, si, th.iperm, th.eh);
         th.sh_gid = th.plan.sh_gid;
         th.sh_ptr = th.plan.sh_ptr;
         if (nE == 0) {   // no elements: every node is finished by the shared-node kernel
@@ -767,6 +772,7 @@ struct EngineT : EngineBase {
     // allocate the scene's ranges and upload its tables; false (nothing kept) if it does not fit
     bool place_scene(int si, TiledHost& th) {
         if ((int)place.size() <= si) place.resize(si + 1);
+        if ((int)iperm.size() <= si) iperm.resize(si + 1);
         Place& pl = place[si];
         pl.size = th.need;
         for (int k = 0; k < 11; k++) {
@@ -778,6 +784,7 @@ struct EngineT : EngineBase {
             field(pl.at, k) = at;
         }
         pl.placed = true;
+        iperm[si] = std::move(th.iperm);
         const SceneHost& s = scenes[si];
         const int n = s.n_nodes, nE = th.eh.nE;
         const int nb0 = (int)pl.at.node, sid = (int)pl.at.sid;
@@ -901,7 +908,7 @@ struct EngineT : EngineBase {
         std::vector<TiledHost> th(S);
         Extent tot;
         for (int si = 0; si < S; si++) {
-            build_tiled(si, th[si]);
+            build_tiled(scenes[si], si, th[si]);
             tot.add(th[si].need);
         }
         reserve(tot);
@@ -912,16 +919,45 @@ struct EngineT : EngineBase {
         HIPCHK(hipDeviceSynchronize());
     }
 
-    // live engines: place one more scene; false if the arenas have no room for it now
+    // live engines: the host tables of a scene, built ahead of its commit (any host thread)
+    std::mutex prep_mu;
+    std::map<int, std::unique_ptr<TiledHost>> prepared;
+    void prepare_scene(int si) override {
+        const SceneHost* s;
+        {
+            std::lock_guard<std::mutex> g(scenes_mu);
+            s = &scenes[si];
+        }
+        std::unique_ptr<TiledHost> th(new TiledHost());
+        build_tiled(*s, si, *th);
+        std::lock_guard<std::mutex> g(prep_mu);
+        prepared[si] = std::move(th);
+    }
+    // live engines: place one more scene; false if the arenas have no room for it now (its
+    // prepared tables are kept for the next attempt)
     bool commit_scene(int si) override {
         if (si >= (int)place.size()) place.resize(si + 1);
-        if (si >= (int)iperm.size()) iperm.resize(si + 1);
         if (place[si].placed) throw std::logic_error("scene already committed");
+        std::unique_ptr<TiledHost> th;
+        {
+            std::lock_guard<std::mutex> g(prep_mu);
+            auto it = prepared.find(si);
+            if (it != prepared.end()) th = std::move(it->second);
+        }
+        if (!th) {
+            th.reset(new TiledHost());
+            build_tiled(scenes[si], si, *th);
+        }
         // the ranges it gets are free: no work in flight reads them, so the (synchronous)
         // uploads need no device-wide wait
-        TiledHost th;
-        build_tiled(si, th);
-        return place_scene(si, th);
+        if (place_scene(si, *th)) {
+            std::lock_guard<std::mutex> g(prep_mu);
+            prepared.erase(si);
+            return true;
+        }
+        std::lock_guard<std::mutex> g(prep_mu);
+        prepared[si] = std::move(th);
+        return false;
     }
     void release_scene(int si) override {
         HIPCHK(hipDeviceSynchronize());   // nothing in flight reads the ranges any more
@@ -930,6 +966,8 @@ struct EngineT : EngineBase {
         release_ranges(place[si]);
         iperm[si].clear();
         iperm[si].shrink_to_fit();
+        std::lock_guard<std::mutex> g(prep_mu);
+        prepared.erase(si);
     }
     bool placed(int si) const override { return si < (int)place.size() && place[si].placed; }
     void usage(int64_t* out) const override {
@@ -1012,15 +1050,16 @@ struct EngineT : EngineBase {
         const Blk* eb = (const Blk*)cur->a;
         const Blk* nb = (const Blk*)cur->b;
         if (cur->na > 0) k_rhs_elem<R, NR, VT><<<cur->na, kBlock, 0, s>>>(ev, eb, inst, x, v, ef, ef2);
-        if (cur->nb > 0) k_rhs_node<VT><<<cur->nb, kBlock, 0, s>>>(nb, inst, na, ef, ef2, v, f, b, r, xcg, partial);
+        double* ndp = seq_dots ? nd : nullptr;
+        if (cur->nb > 0) k_rhs_node<VT><<<cur->nb, kBlock, 0, s>>>(nb, inst, na, ef, ef2, v, f, b, r, xcg, partial, ndp);
         for (int it = 1; it <= cur->maxit; it++) {
             // direction of iteration it in buffer it & 1, the previous one in the other
             VT* p_new = (it & 1) ? p : p2;
             const VT* p_old = (it & 1) ? p2 : p;
             if (cur->na > 0) k_cg_elem<R, NR, VT><<<cur->na, kBlock, 0, s>>>(ev, eb, inst, r, p_old, ef);
             if (cur->nb > 0) {
-                k_cg_node<VT><<<cur->nb, kBlock, 0, s>>>(nb, inst, na, ef, r, p_old, p_new, q, partial);
-                k_cg_update<VT><<<cur->nb, kBlock, 0, s>>>(nb, inst, p_new, q, xcg, r, partial);
+                k_cg_node<VT><<<cur->nb, kBlock, 0, s>>>(nb, inst, na, ef, r, p_old, p_new, q, partial, ndp);
+                k_cg_update<VT><<<cur->nb, kBlock, 0, s>>>(nb, inst, p_new, q, xcg, r, partial, ndp);
             }
         }
         if (cur->nb > 0) k_euler_update<VT><<<cur->nb, kBlock, 0, s>>>(nb, inst, xcg, v, x);
@@ -1397,8 +1436,11 @@ int sofa_amd_scene_create(sofa_amd_engine eng, int n_nodes, const double* positi
     s.rest.assign(positions, positions + 3 * (size_t)n_nodes);
     for (double d : s.rest)
         if (!std::isfinite(d)) return fail(SOFA_AMD_EINVAL, "non-finite position");
-    eng->impl->scenes.push_back(std::move(s));
-    *out_scene = (int)eng->impl->scenes.size() - 1;
+    {
+        std::lock_guard<std::mutex> g(eng->impl->scenes_mu);
+        eng->impl->scenes.push_back(std::move(s));
+        *out_scene = (int)eng->impl->scenes.size() - 1;
+    }
     return SOFA_AMD_OK;
     GUARD_END
 }
@@ -1662,6 +1704,20 @@ int sofa_amd_scene_commit(sofa_amd_engine eng, int scene) {
     GUARD_END
 }
 
+int sofa_amd_scene_prepare(sofa_amd_engine eng, int scene) {
+    GUARD_BEGIN
+    if (!eng || !eng->impl) return fail(SOFA_AMD_EINVAL, "null engine");
+    EngineBase* e = eng->impl.get();
+    {
+        std::lock_guard<std::mutex> g(e->scenes_mu);   // another thread may be adding scenes
+        if (scene < 0 || scene >= (int)e->scenes.size()) return fail(SOFA_AMD_EINVAL, "invalid scene");
+    }
+    if (!e->live_mode) return fail(SOFA_AMD_ESTATE, "engine not reserved (sofa_amd_engine_reserve)");
+    e->prepare_scene(scene);
+    return SOFA_AMD_OK;
+    GUARD_END
+}
+
 int sofa_amd_scene_release(sofa_amd_engine eng, int scene) {
     GUARD_BEGIN
     if (!scene_of(eng, scene)) return fail(SOFA_AMD_EINVAL, "invalid scene");
@@ -1686,6 +1742,18 @@ int sofa_amd_engine_usage(sofa_amd_engine eng, int64_t* out22) {
     if (!eng || !eng->impl || !out22) return fail(SOFA_AMD_EINVAL, "invalid argument");
     if (!eng->impl->live_mode) return fail(SOFA_AMD_ESTATE, "engine not reserved (sofa_amd_engine_reserve)");
     eng->impl->usage(out22);
+    return SOFA_AMD_OK;
+    GUARD_END
+}
+
+int sofa_amd_engine_set_dot_order(sofa_amd_engine eng, int order) {
+    GUARD_BEGIN
+    if (!eng || !eng->impl || order < 0 || order > 1) return fail(SOFA_AMD_EINVAL, "invalid argument");
+    if (eng->impl->precision != SOFA_AMD_FP64) return fail(SOFA_AMD_EINVAL, "dot order applies to the FP64 engine");
+    HIPCHK(hipSetDevice(eng->impl->device));
+    HIPCHK(hipDeviceSynchronize());
+    eng->impl->seq_dots = order == 1;
+    eng->impl->reset_graph();
     return SOFA_AMD_OK;
     GUARD_END
 }

@@ -403,4 +403,36 @@ __device__ __forceinline__ bool last_block_reduce(double blockval, double* parti
     return true;
 }
 
+// the same hand-off with the reference's summation order: every thread has stored its node's
+// value in nd; the last-arriving block adds nd[node_begin], nd[node_begin + 1], ... one after
+// the other from 0.0, as MechanicalObject::vDot accumulates (MechanicalObject.inl:2334-2356).
+// One wave does the adds (each lane of a 64-value chunk handed to every lane in order), so the
+// cost is one dependent add per node of the scene.
+__device__ __forceinline__ bool last_block_sequential(const double* nd, InstDev* ins, double& total) {
+    __shared__ int am_last;
+    __threadfence();                  // this thread's nd store
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned ticket = atomicAdd(&ins->counter, 1u);
+        am_last = (ticket == (unsigned)(ins->nblk_count - 1)) ? 1 : 0;
+    }
+    __syncthreads();
+    if (!am_last) return false;
+    __threadfence();
+    double s = 0.0;
+    if (threadIdx.x < 64) {
+        const int b0 = ins->node_begin, nn = ins->node_count;
+        for (int base = 0; base < nn; base += 64) {
+            const int i = base + (int)threadIdx.x;
+            const double v = i < nn ? nd[b0 + i] : 0.0;
+            const int cnt = min(64, nn - base);
+            for (int k = 0; k < cnt; k++) s = s + __shfl(v, k, 64);
+        }
+    }
+    total = s;                        // valid in the first wave
+    __syncthreads();
+    if (threadIdx.x == 0) atomicExch(&ins->counter, 0u);
+    return true;
+}
+
 }  // namespace sofa_amd

@@ -145,7 +145,8 @@ __global__ __launch_bounds__(kBlock) void k_rhs_node(const Blk* __restrict__ nbl
                                                       NodeArrays na, const VT* __restrict__ ef_f,
                                                       const VT* __restrict__ ef_v, const VT* __restrict__ v,
                                                       VT* __restrict__ f, VT* __restrict__ b, VT* __restrict__ r,
-                                                      VT* __restrict__ xcg, double* __restrict__ partial) {
+                                                      VT* __restrict__ xcg, double* __restrict__ partial,
+                                                      double* __restrict__ nd) {   // non-null: reference dot order
     __shared__ double sh[kBlock];
     const Blk bk = nblk[blockIdx.x];
     InstDev* ins = &inst[bk.inst];
@@ -183,10 +184,12 @@ __global__ __launch_bounds__(kBlock) void k_rhs_node(const Blk* __restrict__ nbl
         r[o] = bn[0]; r[o + 1] = bn[1]; r[o + 2] = bn[2];
         xcg[o] = 0; xcg[o + 1] = 0; xcg[o + 2] = 0;
         bb = dot3d(bn, bn);
+        if (nd) nd[n] = bb;
     }
-    const double bsum = block_sum(bb, sh);
     double total;
-    if (last_block_reduce(bsum, partial, ins, bk.pad, sh, total) && threadIdx.x == 0) {
+    const bool last = nd ? last_block_sequential(nd, ins, total)
+                         : last_block_reduce(block_sum(bb, sh), partial, ins, bk.pad, sh, total);
+    if (last && threadIdx.x == 0) {
         // CGLinearSolver.inl:86-160 for nb_iter = 1 (rho = r.r = b.b)
         ins->normb = sqrt(total);
         ins->rho = total;
@@ -250,7 +253,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_node(const Blk* __restrict__ nblk
                                                      NodeArrays na, const VT* __restrict__ ef,
                                                      const VT* __restrict__ r, const VT* __restrict__ p_old,
                                                      VT* __restrict__ p, VT* __restrict__ q,
-                                                     double* __restrict__ partial) {
+                                                     double* __restrict__ partial, double* __restrict__ nd) {
     __shared__ double sh[kBlock];
     const Blk bk = nblk[blockIdx.x];
     InstDev* ins = &inst[bk.inst];
@@ -289,10 +292,12 @@ __global__ __launch_bounds__(kBlock) void k_cg_node(const Blk* __restrict__ nblk
         if (na.fixed[n]) { qn[0] = 0; qn[1] = 0; qn[2] = 0; }
         q[o] = qn[0]; q[o + 1] = qn[1]; q[o + 2] = qn[2];
         pq = dot3d(pn, qn);
+        if (nd) nd[n] = pq;
     }
-    const double bsum = block_sum(pq, sh);
     double den;
-    if (last_block_reduce(bsum, partial, ins, bk.pad, sh, den) && threadIdx.x == 0) {
+    const bool last = nd ? last_block_sequential(nd, ins, den)
+                         : last_block_reduce(block_sum(pq, sh), partial, ins, bk.pad, sh, den);
+    if (last && threadIdx.x == 0) {
         // CGLinearSolver.inl:207-265
         ins->den = den;
         const bool exempt = (ins->iter == 1 && ins->first_solve);
@@ -312,7 +317,7 @@ template <typename VT>
 __global__ __launch_bounds__(kBlock) void k_cg_update(const Blk* __restrict__ nblk, InstDev* __restrict__ inst,
                                                        const VT* __restrict__ p, const VT* __restrict__ q,
                                                        VT* __restrict__ xcg, VT* __restrict__ r,
-                                                       double* __restrict__ partial) {
+                                                       double* __restrict__ partial, double* __restrict__ nd) {
     __shared__ double sh[kBlock];
     const Blk bk = nblk[blockIdx.x];
     InstDev* ins = &inst[bk.inst];
@@ -331,10 +336,12 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(const Blk* __restrict__ nb
             r[o + k] = rn[k];
         }
         rr = dot3d(rn, rn);
+        if (nd) nd[n] = rr;
     }
-    const double bsum = block_sum(rr, sh);
     double rho;
-    if (last_block_reduce(bsum, partial, ins, bk.pad, sh, rho) && threadIdx.x == 0) {
+    const bool last = nd ? last_block_sequential(nd, ins, rho)
+                         : last_block_reduce(block_sum(rr, sh), partial, ins, bk.pad, sh, rho);
+    if (last && threadIdx.x == 0) {
         // rho_1 = rho; next iteration: rho = r.r, err test, beta = rho / rho_1
         ins->rho_1 = ins->rho;
         ins->rho = rho;

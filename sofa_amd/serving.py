@@ -87,23 +87,34 @@ class Server:
                 pass
         jobs = sorted(jobs, key=lambda j: j["arrival"])
         specs = {}                                    # job index -> Future of its scene spec
+        staged = {}                                   # job index -> (scene, Future of its host tables, spec)
         nxt = 0                                       # next job not yet arrived
         queue = collections.deque()                   # arrived, not admitted (job indices)
         active = {}                                   # scene id -> [job index, steps left, tets]
         used = 0
         admitted, done, rejected = {}, 0, 0
-        pending = {}                                  # job index -> created, not yet placed scene
         waits, step_records = [], []                  # (start event, end event, [tets per job])
         elem_steps = 0
         t0 = time.perf_counter()
         inflight = collections.deque()
 
-        def prefetch(k):
-            for j in range(k, min(len(jobs), k + 32)):
-                if j not in specs:
-                    specs[j] = self.pool.submit(build, jobs[j])
+        prepare = getattr(self.eng, "prepare_scene", None)
 
-        prefetch(0)
+        def stage(j, wait=False):
+            """Job j's mesh is built, then its scene created and its host tables (tiling,
+            element set-up) built on a worker thread, while the GPU runs the current tick."""
+            if j in staged:
+                return True
+            if j not in specs:
+                specs[j] = self.pool.submit(build, jobs[j])
+            if not wait and not specs[j].done():
+                return False
+            spec = specs.pop(j).result()
+            sid = workloads.add_to_engine(self.eng, spec)
+            fut = self.pool.submit(prepare, sid) if prepare is not None else None
+            staged[j] = (sid, fut, spec)
+            return True
+
         while True:
             now = time.perf_counter() - t0
             if now >= wall_limit:
@@ -111,29 +122,32 @@ class Server:
             while nxt < len(jobs) and jobs[nxt]["arrival"] <= now:
                 queue.append(nxt)
                 nxt += 1
-            prefetch(queue[0] if queue else nxt)
+            for k, j in enumerate(queue):               # the queue's head is staged ahead
+                if k >= 32:
+                    break
+                stage(j)
             # admission: FIFO while the head fits the budget
             while queue:
                 j = queue[0]
-                if j not in specs:
-                    prefetch(j)
-                spec = specs[j].result()
+                if not stage(j, wait=not active):
+                    break                              # its mesh is still being built
+                sid, fut, spec = staged[j]
                 need = footprint_bytes(self.precision, len(spec["pos"]), len(spec["tets"]))
                 if used + need > self.budget and active:
                     break
-                sid = pending.get(j)
-                if sid is None:
-                    sid = pending[j] = workloads.add_to_engine(self.eng, spec)
+                if fut is not None:
+                    if active and not fut.done():
+                        break                          # host tables not ready: next tick
+                    fut.result()
                 if not self.eng.commit_scene(sid):
                     if active:
                         break                          # arenas full (fragmentation): wait
                     queue.popleft()                    # larger than the whole reservation
-                    del specs[j], pending[j]
+                    del staged[j]
                     rejected += 1
                     continue
-                del pending[j]
                 queue.popleft()
-                del specs[j]
+                del staged[j]
                 used += need
                 active[sid] = [j, jobs[j]["steps"], len(spec["tets"]), need]
                 admitted[j] = time.perf_counter() - t0

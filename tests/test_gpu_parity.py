@@ -620,3 +620,46 @@ def test_fix_all(prec):
     assert rel_err(x, o.get("x")) <= tol and np.max(np.abs(v - v0)) <= tol
     assert eng.stats(sid)["end_reason"] == 4
     eng.close()
+
+
+@pytest.mark.parametrize("which", ["cfg0", "meshmatrix", "sensitive_beam"])
+def test_fp64_reference_dot_order_bitexact(which):
+    """FP64 engine with the reference's sequential dot order: positions and velocities equal the
+    oracle's reference-order trajectory bit for bit -- also on a beam of the 256-beam mix whose
+    trajectory moves by ~1e-5 when only the order of its dot-product sums changes."""
+    capi = _capi()
+    from sofa_amd import workloads
+    print(f"case: fp64 sequential dots, {which}", flush=True)
+    eng = capi.Engine(0, capi.FP64)
+    if which == "cfg0":
+        c = beam_cfg0()
+        sid = add_gpu_scene(eng, c)
+        o = make_oracle(c, 0)
+        nsteps, dt = 100, c["dt"]
+    elif which == "meshmatrix":
+        c = beam_meshmatrix(lumped=False, osc=False)
+        sid = gpu_mm(eng, c)
+        o = oracle_mm(c, 0)
+        nsteps, dt = 30, c["dt"]
+    else:
+        s = workloads.cfg3()[244]                  # E 8.7e5, 1,980 tets, CG 25 far from converged
+        sid = workloads.add_to_engine(eng, s)
+        from oracle import OracleScene
+        o = OracleScene(s["pos"])
+        o.set_gravity(s["g"])
+        o.set_fem(s["tets"], [s["E"]], [s["nu"]], "large")
+        o.set_uniform_mass(vertex_mass=s["mass"][1])
+        o.set_fixed(s["fixed"])
+        o.set_solver(s["rs"], s["rm"], s["iters"], s["tol"], s["thr"])
+        nsteps, dt = 100, s["dt"]
+    eng.commit()
+    eng.set_dot_order(True)
+    eng.set_graphs(True)
+    eng.step([sid], nsteps)
+    x, v, _ = eng.read(sid)
+    o.step(dt, nsteps)
+    assert np.array_equal(x, o.get("x")), rel_err(x, o.get("x"))
+    assert np.array_equal(v, o.get("v"))
+    st, so = eng.stats(sid), o.stats()
+    assert st["iterations"] == so["iterations"] and st["end_reason"] == so["endcond"]
+    eng.close()

@@ -80,6 +80,9 @@ class _FakeEngine:
     def set_refinement(self, *a, **k):
         pass
 
+    def prepare_scene(self, sid):
+        self.prepared = getattr(self, "prepared", set()) | {sid}
+
     def commit_scene(self, sid):
         n, t = self.scenes[sid]
         if self.used[0] + n > self.cap[0] or self.used[1] + t > self.cap[1]:
@@ -91,6 +94,8 @@ class _FakeEngine:
         return True
 
     def step(self, ids, n_steps=1, stream=None):
+        import time
+        time.sleep(0.004)                                          # the device's time per tick
         assert all(i in self.placed for i in ids)
         self.log.append(tuple(ids))
         for i in ids:
