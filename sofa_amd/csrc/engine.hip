@@ -1091,7 +1091,7 @@ struct EngineT : EngineBase {
         if (cur->nb > 0) k_fin_rhs<VT><<<cur->nb, kBlock, 0, s>>>(fb, inst, ranges_d, sn, mass, fixed, v, sp0, sp1, f, b, r, xcg, pbb);
         for (int it = 1; it <= cur->maxit; it++) {
             if (cur->na > 0)
-                k_tile_apply<R, NR, VT, kTileThreads><<<cur->na, kTileThreads, 0, s>>>(ta, it, r, p, q, sp0, it == 1 ? pbb : prho, pden, 0);
+                k_tile_apply<R, NR, VT, kTileThreads, true><<<cur->na, kTileThreads, 0, s>>>(ta, it, r, p, q, sp0, it == 1 ? pbb : prho, pden, 0, x);
             if (cur->nc > 0)
                 k_update<VT><<<cur->nc, kBlock, 0, s>>>(nb, inst, ranges_d, it, sn, sp0, mass, fixed, p, q, xcg, r, pden, prho);
         }
@@ -1129,7 +1129,7 @@ struct EngineT : EngineBase {
         for (int sw = 1; sw <= cur->maxsw + 1; sw++) {
             if (sw > 1) {   // q = A x, FP64
                 if (cur->na > 0)
-                    k_tile_apply<R, NR, VT, kTileThreads><<<cur->na, kTileThreads, 0, s>>>(ta, 0, xcg, nullptr, q, sp0, nullptr, pden, 0);
+                    k_tile_apply<R, NR, VT, kTileThreads, true><<<cur->na, kTileThreads, 0, s>>>(ta, 0, xcg, nullptr, q, sp0, nullptr, pden, 0, x);
                 if (cur->nb > 0)
                     k_fin_apply<VT><<<cur->nb, kBlock, 0, s>>>(fb, inst, ranges_d, sn, mass, fixed, 0, xcg, nullptr, q, sp0, pden);
             }
@@ -1188,7 +1188,7 @@ struct EngineT : EngineBase {
         prepare_set(ids.data(), (int)ids.size(), s);
         const TileArgs<R, NR> ta = tile_args();
         for (int i = 0; i < reps; i++)
-            if (cur->na > 0) k_tile_apply<R, NR, VT, kTileThreads><<<cur->na, kTileThreads, 0, s>>>(ta, 2, r, p, q, sp0, prho, pden, 0);
+            if (cur->na > 0) k_tile_apply<R, NR, VT, kTileThreads, true><<<cur->na, kTileThreads, 0, s>>>(ta, 2, r, p, q, sp0, prho, pden, 0, x);
         HIPCHK(hipGetLastError());
     }
     // tiles, tile-local nodes, nodes finished by the finalize kernel, shared partial slots,
@@ -1313,7 +1313,9 @@ EngineBase* make_engine_fp32() {
 
 void launch_inner_cg(const InnerCg& c, hipStream_t s) {
     for (int it = 1; it <= c.maxit; it++) {
-        if (c.na > 0) k_tile_apply<float, 6, float, kTileThreads><<<c.na, kTileThreads, 0, s>>>(c.ta, it, c.r, c.p, c.q, c.sp, c.prho, c.pden, 1);
+        if (c.na > 0)
+            k_tile_apply<float, 6, float, kTileThreads, false><<<c.na, kTileThreads, 0, s>>>(c.ta, it, c.r, c.p, c.q, c.sp,
+                                                                                        c.prho, c.pden, 1, nullptr);
         if (c.nnb > 0)
             k_update<float><<<c.nnb, kBlock, 0, s>>>(c.nb, c.ta.inst, c.ta.ranges, it, c.sn, c.sp, c.ta.mass, c.ta.fixed,
                                                       c.p, c.q, c.d, c.r, c.pden, c.prho);

@@ -80,6 +80,7 @@ __device__ __forceinline__ R pdet(R m00, R m01, R m02, R m10, R m11, R m12) {
 // normalize per Vec.h:535-578: divide by sqrt(norm2) unless the norm is <= epsilon
 template <typename R>
 __device__ __forceinline__ void normalize3(R& x, R& y, R& z) {
+#pragma clang fp contract(off)
     R n2 = x * x;
     n2 += y * y;
     n2 += z * z;
@@ -206,24 +207,32 @@ __device__ __forceinline__ void rot_apply(const R rot[9], const R F[12], R c[12]
 // element force: accumulateForceLarge; returns the rotation and the 4 nodal contributions
 // (to be ADDED to f); positions read as VT and converted to R
 // ---------------------------------------------------------------------------------------
-template <typename R, int NR>
-__device__ __forceinline__ void elem_force_p(const ElemView<R, NR>& ev, int64_t e, const R p[4][3], R rot[9],
-                                             R c[12], R g[12]) {
-    // computeRotationLarge (rows ex, ey, ez of R_0_2)
-    R ex0 = p[1][0] - p[0][0], ex1 = p[1][1] - p[0][1], ex2 = p[1][2] - p[0][2];
+// computeRotationLarge (TetrahedronFEMForceField.inl:755-778) from the element's first three
+// vertices: rot = transpose(R_0_2), the rows of R_0_2 being ex, ey, ez.  Multiplies and adds are
+// never fused here, so that every kernel that needs the rotation (the force pass, and the tiled
+// operator, which recomputes it from the positions instead of reading it back) gets the same bits.
+template <typename R>
+__device__ __forceinline__ void rotation_large(const R p0[3], const R p1[3], const R p2[3], R rot[9]) {
+#pragma clang fp contract(off)
+    R ex0 = p1[0] - p0[0], ex1 = p1[1] - p0[1], ex2 = p1[2] - p0[2];
     normalize3(ex0, ex1, ex2);
-    R ey0 = p[2][0] - p[0][0], ey1 = p[2][1] - p[0][1], ey2 = p[2][2] - p[0][2];
+    R ey0 = p2[0] - p0[0], ey1 = p2[1] - p0[1], ey2 = p2[2] - p0[2];
     R ez0 = ex1 * ey2 - ex2 * ey1, ez1 = ex2 * ey0 - ex0 * ey2, ez2 = ex0 * ey1 - ex1 * ey0;
     normalize3(ez0, ez1, ez2);
     ey0 = ez1 * ex2 - ez2 * ex1;
     ey1 = ez2 * ex0 - ez0 * ex2;
     ey2 = ez0 * ex1 - ez1 * ex0;
-    const R r02[3][3] = {{ex0, ex1, ex2}, {ey0, ey1, ey2}, {ez0, ez1, ez2}};
-    // rotations[e] = transpose(R_0_2)
-#pragma unroll
-    for (int i = 0; i < 3; i++)
-#pragma unroll
-        for (int j = 0; j < 3; j++) rot[3 * i + j] = r02[j][i];
+    rot[0] = ex0; rot[1] = ey0; rot[2] = ez0;
+    rot[3] = ex1; rot[4] = ey1; rot[5] = ez1;
+    rot[6] = ex2; rot[7] = ey2; rot[8] = ez2;
+}
+
+template <typename R, int NR>
+__device__ __forceinline__ void elem_force_p(const ElemView<R, NR>& ev, int64_t e, const R p[4][3], R rot[9],
+                                             R c[12], R g[12]) {
+    rotation_large(p[0], p[1], p[2], rot);
+    // R_0_2 = transpose(rot)
+    const R r02[3][3] = {{rot[0], rot[3], rot[6]}, {rot[1], rot[4], rot[7]}, {rot[2], rot[5], rot[8]}};
     // deforme[k] = R_0_2 * p_k, then relative to deforme[0] on the used components
     R df[4][3];
 #pragma unroll

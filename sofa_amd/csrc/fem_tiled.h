@@ -312,14 +312,19 @@ __global__ __launch_bounds__(kBlock) void k_fin_rhs(const FinBlk* __restrict__ f
 // iter = 0 is a plain operator application q = A r (no CG scalars, p untouched), used for
 // the FP64 residual of iterative refinement.
 // ---------------------------------------------------------------------------------------
-template <typename R, int NR, typename VT, int TB = kBlock>
+// ROT: the element rotations are recomputed from the positions x (the ones the force pass used:
+// x changes only after the solve), bit for bit the stored ones, instead of read back -- 9 reals
+// less per element and launch, 3 more per tile-local node.  Without ROT they are loaded.
+template <typename R, int NR, typename VT, int TB = kBlock, bool ROT = false>
 __global__ __launch_bounds__(TB) void k_tile_apply(TileArgs<R, NR> a, int iter, const VT* __restrict__ r,
                                                     VT* __restrict__ p, VT* __restrict__ q, VT* __restrict__ sp,
                                                     const double* __restrict__ pin,   // |b|^2 (iter 1) or r.r partials
                                                     double* __restrict__ pden,
-                                                    int pin_nodes) {   // 1: iter-1 |b|^2 partials per node block
+                                                    int pin_nodes,     // 1: iter-1 |b|^2 partials per node block
+                                                    const VT* __restrict__ x) {
     constexpr int EPT = kTileElems / TB;                   // elements per thread
     constexpr int NPT = (kTileNodes + TB - 1) / TB;        // tile-local nodes per thread
+    static_assert(kTileElems * 12 >= kTileNodes * 3, "positions are staged in the contribution buffer");
     __shared__ R lu[kTileNodes * 3];
     __shared__ R lc[kTileElems * 12];
     __shared__ uint16_t linc[kTileElems * 4];
@@ -338,8 +343,10 @@ __global__ __launch_bounds__(TB) void k_tile_apply(TileArgs<R, NR> a, int iter, 
         if (k < td.e_count) {
             const int64_t e = (int64_t)td.e_begin + k;
             lt[m] = a.ltet[e];
+            if (!ROT) {
 #pragma unroll
-            for (int i = 0; i < 9; i++) rot[m][i] = a.ev.rot[i * a.ev.E + e];
+                for (int i = 0; i < 9; i++) rot[m][i] = a.ev.rot[i * a.ev.E + e];
+            }
 #pragma unroll
             for (int i = 0; i < NR; i++) rin[m][i] = a.ev.rest[i * a.ev.E + e];
 #pragma unroll
@@ -364,13 +371,14 @@ __global__ __launch_bounds__(TB) void k_tile_apply(TileArgs<R, NR> a, int iter, 
     for (int j = threadIdx.x; j <= td.ln_count; j += TB) lptr[j] = a.tn.inc_ptr[td.ln_begin + j] - td.inc_begin;
     if (ins->stopped) return;
     // ---- 2. the tile's nodes: r and p (dependent on the node ids), masks
-    VT rr_[NPT][3], po_[NPT][3];
+    VT rr_[NPT][3], po_[NPT][3], xx_[NPT][3];
 #pragma unroll
     for (int m = 0; m < NPT; m++) {
         const int j = threadIdx.x + m * TB;
         if (j < td.ln_count) {
             ld3(r, gid[m], rr_[m]);
             if (iter > 1) ld3(p, gid[m], po_[m]);
+            if (ROT) ld3(x, gid[m], xx_[m]);
             fx[m] = a.fixed[gid[m]];
         }
     }
@@ -413,9 +421,25 @@ __global__ __launch_bounds__(TB) void k_tile_apply(TileArgs<R, NR> a, int iter, 
             }
             if (iter > 0 && psl[m] < 0) st3(p, gid[m], pp);
             lu[3 * j] = (R)pp[0]; lu[3 * j + 1] = (R)pp[1]; lu[3 * j + 2] = (R)pp[2];
+            if (ROT) { lc[3 * j] = (R)xx_[m][0]; lc[3 * j + 1] = (R)xx_[m][1]; lc[3 * j + 2] = (R)xx_[m][2]; }
         }
     }
     __syncthreads();
+    if (ROT) {   // rotations from the staged positions, then the buffer is free again
+#pragma unroll
+        for (int m = 0; m < EPT; m++) {
+            const int k = threadIdx.x + m * TB;
+            if (k < td.e_count) {
+                int l[4];
+                unpack_ltet(lt[m], l);
+                const R p0[3] = {lc[3 * l[0]], lc[3 * l[0] + 1], lc[3 * l[0] + 2]};
+                const R p1[3] = {lc[3 * l[1]], lc[3 * l[1] + 1], lc[3 * l[1] + 2]};
+                const R p2[3] = {lc[3 * l[2]], lc[3 * l[2] + 1], lc[3 * l[2] + 2]};
+                rotation_large(p0, p1, p2, rot[m]);
+            }
+        }
+        __syncthreads();
+    }
     // ---- 5. element K p into LDS
     const R kcg = (R)ins->k_cg;
     const R mcg = (R)ins->m_cg;

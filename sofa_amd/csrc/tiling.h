@@ -1,6 +1,7 @@
 // tiling.h -- host planning of element tiles for the LDS-gather kernels (fem_tiled.h).
 //
-// Elements are ordered by the Morton code of their rest centroid (ties by element id) and cut
+// Elements are ordered by the Morton code of their rest centroid in element units (each axis
+// scaled by the median element extent along it; ties by element id) and cut
 // greedily into tiles of at most `max_elems` elements and `max_nodes` distinct nodes.  For
 // each tile-local node the incidence list (local element slot * 4 + vertex) is sorted by
 // ORIGINAL element id: the order in which the reference's element loop accumulates into that
@@ -51,8 +52,33 @@ static inline TilePlan plan_tiles(int n_nodes, const double* rest, const int32_t
             lo[k] = std::min(lo[k], rest[3 * i + k]);
             hi[k] = std::max(hi[k], rest[3 * i + k]);
         }
+    // the curve runs in element units: each axis is scaled by the typical element extent along
+    // it (median over a sample of elements), so that meshes of flat or long elements (a thin
+    // beam cut into many thin layers) still get compact tiles
+    double scale[3] = {1, 1, 1};
+    {
+        const int stride = std::max(1, E / 65536);
+        std::vector<double> w[3];
+        for (int e = 0; e < E; e += stride)
+            for (int k = 0; k < 3; k++) {
+                double a = 1e300, b = -1e300;
+                for (int v = 0; v < 4; v++) {
+                    const double c = rest[3 * tets[4 * e + v] + k];
+                    a = std::min(a, c);
+                    b = std::max(b, c);
+                }
+                w[k].push_back(b - a);
+            }
+        for (int k = 0; k < 3; k++) {
+            if (w[k].empty()) continue;
+            std::nth_element(w[k].begin(), w[k].begin() + w[k].size() / 2, w[k].end());
+            const double m = w[k][w[k].size() / 2];
+            if (m > 0) scale[k] = m;
+        }
+        if (!(scale[0] > 0 && scale[1] > 0 && scale[2] > 0)) scale[0] = scale[1] = scale[2] = 1;
+    }
     double ext = 0;
-    for (int k = 0; k < 3; k++) ext = std::max(ext, hi[k] - lo[k]);
+    for (int k = 0; k < 3; k++) ext = std::max(ext, (hi[k] - lo[k]) / scale[k]);
     if (!(ext > 0)) ext = 1;
     std::vector<uint64_t> code(E);
     for (int e = 0; e < E; e++) {
@@ -60,7 +86,7 @@ static inline TilePlan plan_tiles(int n_nodes, const double* rest, const int32_t
         for (int k = 0; k < 3; k++) {
             double s = 0;
             for (int v = 0; v < 4; v++) s += rest[3 * tets[4 * e + v] + k];
-            double u = (s * 0.25 - lo[k]) / ext;
+            double u = (s * 0.25 - lo[k]) / scale[k] / ext;
             u = std::min(std::max(u, 0.0), 1.0);
             c |= spread3((uint64_t)(u * 2097151.0)) << k;
         }
