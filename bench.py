@@ -130,6 +130,22 @@ def apply_roofline(eng, reps, pe, pv, nrec, E, N):
             "traffic": None, "us_per_launch": us, "algorithmic_bytes": bytes_alg}
 
 
+def measured_traffic(path="profiles/k_tile_apply_beam_mix_traffic.md"):
+    """HBM bytes per launch of the apply kernel on the beam mix, from the rocprofv3 FETCH_SIZE /
+    WRITE_SIZE runs summarised in the profiles folder (counters scaled by a copy of known size);
+    None when that summary is absent."""
+    import re
+    try:
+        text = open(os.path.join(ROOT, path)).read()
+    except OSError:
+        return None
+    m = re.search(r"```json\s*(\{.*?\})\s*```", text, re.S)
+    if not m:
+        return None
+    d = json.loads(m.group(1))
+    return {"bytes": d["traffic_bytes_per_launch"], "source": path, "kernel": d["kernel"]}
+
+
 def copy_probe(mb=512, reps=5):
     """A plain device copy of known size (calibrates the HBM byte counters; STREAM-copy rate)."""
     import torch
@@ -188,17 +204,19 @@ def cfg2_mixed(steps, err_steps):
     refinement sweeps) against the FP64 tiled path with the same iteration budget (CG 90), both
     at tolerance 1e-8 -- on this mesh neither reaches it (the last residual is reported): the
     systems need thousands of CG iterations.  Accuracy, on the same kind of mesh at 12 mm pitch
-    where the solves can be run to |r|/|b| <= 1e-10 (FP64 CG up to 6000 iterations; MIXED up to 500
-    FP32 iterations per sweep, up to 40 sweeps -- 30-iteration sweeps stall on these systems):
-    MIXED against FP64 after `err_steps` steps."""
+    where the solves can be run to |r|/|b| <= 1e-10 (FP64 CG up to 6000 iterations; MIXED 30 FP32
+    iterations per sweep continuing one search direction, up to 300 sweeps -- restarted
+    30-iteration sweeps stall on these systems): MIXED against FP64 after `err_steps` steps."""
     from sofa_amd import capi, workloads
     c = workloads.cfg2()
     E, N = len(c["tets"]), len(c["pos"])
     out = {"workload": f"ellipsoid 0.15x0.10x0.08 m, jittered-grid Delaunay h=2.93 mm: {E} tets, {N} nodes, "
                        "E 10^U(3,5), nu U(0.40,0.49) per element, MeshMatrixMass 1050 consistent, x<-0.13 fixed, "
                        "x>0.13 oscillating 5 mm at 1 Hz, dt 0.01, CG tolerance 1e-8"}
-    for name, prec in (("mixed", capi.MIXED), ("fp64_tiled", capi.FP64_TILED)):
-        eng, ids = make_engine([c], prec, graphs=True)
+    keep = dict(c, refinement=dict(inner_iterations=30, inner_tolerance=c["tol"], max_sweeps=3, keep_direction=True))
+    for name, prec, spec in (("mixed", capi.MIXED, c), ("mixed_keep_direction", capi.MIXED, keep),
+                             ("fp64_tiled", capi.FP64_TILED, c)):
+        eng, ids = make_engine([spec], prec, graphs=True)
         t, lat = time_merged(eng, ids, steps, 2)
         st = eng.stats(ids[0])
         out[name] = {"value": E * steps / (t / 1e3), "ms_per_step": t / steps,
@@ -207,12 +225,14 @@ def cfg2_mixed(steps, err_steps):
                                     "err": st["err"], "end": capi.END_REASONS.get(st["end_reason"])}}
         eng.close()
     out["mixed_over_fp64"] = out["mixed"]["value"] / out["fp64_tiled"]["value"]
+    out["mixed_keep_direction_over_fp64"] = out["mixed_keep_direction"]["value"] / out["fp64_tiled"]["value"]
     # accuracy at convergence
     cs = workloads.cfg2(pitch=0.012)
     xs = {}
-    for name, prec, spec in (("mixed", capi.MIXED, dict(cs, tol=1e-10, refinement=dict(inner_iterations=500,
-                                                                                        inner_tolerance=1e-6,
-                                                                                        max_sweeps=40))),
+    for name, prec, spec in (("mixed", capi.MIXED, dict(cs, tol=1e-10, refinement=dict(inner_iterations=30,
+                                                                                        inner_tolerance=1e-10,
+                                                                                        max_sweeps=300,
+                                                                                        keep_direction=True))),
                              ("fp64", capi.FP64_TILED, dict(cs, iters=6000, tol=1e-10))):
         eng, ids = make_engine([spec], prec, graphs=False)
         eng.step(ids, err_steps)
@@ -299,6 +319,7 @@ def main():
     ap.add_argument("--cfg2-error-steps", type=int, default=30)
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-fp64-ref", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--serve-seconds", type=float, default=15.0)
     ap.add_argument("--serve-arrivals", type=float, default=5.0)
     ap.add_argument("--serve-budget-gb", type=float, default=8.0)
@@ -323,6 +344,11 @@ def main():
     value = E * args.steps / (total_ms / 1e3)
     roof = apply_roofline(eng, args.apply_reps, 4, 4, 9, E, N)
     roof["kernel"] = "k_tile_apply<float> over the 256-beam mix (CG operator: addDForce + mass + projection + p update)"
+    tr = measured_traffic()
+    if tr is not None:
+        roof["traffic"] = tr["bytes"]
+        roof["traffic_source"] = tr["source"] + " (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)"
+        roof["traffic_over_algorithmic"] = tr["bytes"] / roof["algorithmic_bytes"]
     log(f"batched: {value:.4g} element-steps/s, {total_ms / args.steps:.3f} ms/step; apply {roof['us_per_launch']:.2f} us "
         f"= {roof['achieved']:.0f} GB/s")
     tiling = {k: 0 for k in ("tiles", "tile_local_nodes", "finalize_nodes", "shared_slots")}
@@ -346,11 +372,17 @@ def main():
                     "error": "0 (equal to the reference order bit for bit)"}
         log(f"fp64 reference order, batched: {fp64_ref}")
 
-    engp, idsp = make_engine(specs, capi.FP32, graphs=True)
-    plain_ms, plain_lat = time_plain(engp, idsp, args.plain_steps, 1)
-    engp.close()
-    value_plain = E * args.plain_steps / (plain_ms / 1e3)
-    log(f"plain: {value_plain:.4g} element-steps/s; batched / plain = {value / value_plain:.2f}")
+    plain = None
+    if not args.no_plain:
+        engp, idsp = make_engine(specs, capi.FP32, graphs=True)
+        plain_ms, plain_lat = time_plain(engp, idsp, args.plain_steps, 1)
+        engp.close()
+        value_plain = E * args.plain_steps / (plain_ms / 1e3)
+        log(f"plain: {value_plain:.4g} element-steps/s; batched / plain = {value / value_plain:.2f}")
+        plain = {"value": value_plain, "ms_per_step_all_scenes": plain_ms / args.plain_steps,
+                 "p50_ms_per_scene_step": float(np.percentile(plain_lat, 50)),
+                 "p99_ms_per_scene_step": float(np.percentile(plain_lat, 99)), "steps": args.plain_steps,
+                 "batched_over_plain": value / value_plain}
 
     err = None if args.no_error else position_error(specs, capi.FP32, args.error_steps)
     log(f"error vs fp64: {err}")
@@ -402,10 +434,7 @@ def main():
                    "scenes": len(specs), "tets": E, "nodes": N, "precision": "fp32 element/state, fp64 reductions"},
         "p50_ms": float(np.percentile(lat, 50)),
         "p99_ms": float(np.percentile(lat, 99)),
-        "plain": {"value": value_plain, "ms_per_step_all_scenes": plain_ms / args.plain_steps,
-                  "p50_ms_per_scene_step": float(np.percentile(plain_lat, 50)),
-                  "p99_ms_per_scene_step": float(np.percentile(plain_lat, 99)), "steps": args.plain_steps,
-                  "batched_over_plain": value / value_plain},
+        "plain": plain,
         "error_vs_reference": err,
         "fp64_reference_order_batched": fp64_ref,
         "roofline": roof,

@@ -123,11 +123,15 @@ int sofa_amd_scene_set_solver(sofa_amd_engine eng, int scene, double dt, double 
  * in FP64, stop when |r| / |b| <= cg_tolerance (the CGLinearSolver tolerance of
  * sofa_amd_scene_set_solver) or after `max_sweeps` corrections, each correction an FP32 CG
  * solve of A d = r of at most `inner_iterations` iterations stopping at relative residual
- * `inner_tolerance`.  Defaults 30, 1e-6, 3.  No reference counterpart (SOFA's CG runs in the
- * state's precision); it replaces CGLinearSolver::solve (CGLinearSolver.inl:71-315) in this mode.
- * sofa_amd_cg_stats reports the inner iterations summed over the sweeps and the sweep count. */
+ * `inner_tolerance`.  With keep_direction = 0 each correction starts CG afresh (classic
+ * refinement); with 1 it continues the previous correction's search direction from the
+ * replaced residual (FP32 CG with FP64 residual replacement: the convergence of one unbroken CG,
+ * the inner tolerance then being relative to the first sweep's residual).  Defaults 30, 1e-6, 3,
+ * 0.  No reference counterpart (SOFA's CG runs in the state's precision); it replaces
+ * CGLinearSolver::solve (CGLinearSolver.inl:71-315) in this mode.  sofa_amd_cg_stats reports
+ * the inner iterations summed over the sweeps and the sweep count. */
 int sofa_amd_scene_set_refinement(sofa_amd_engine eng, int scene, int inner_iterations,
-                                  double inner_tolerance, int max_sweeps);
+                                  double inner_tolerance, int max_sweeps, int keep_direction);
 
 /* Uploads every scene created so far; after this the scenes can be stepped. */
 int sofa_amd_engine_commit(sofa_amd_engine eng);

@@ -88,7 +88,7 @@ def lib():
         "sofa_amd_engine_set_graphs": (c_int, [_vp, c_int]),
         "sofa_amd_scene_set_mass_density": (c_int, [_vp, c_int, c_int, _dp, c_int]),
         "sofa_amd_scene_add_oscillators": (c_int, [_vp, c_int, c_int, _ip, _dp, _dp, _dp, _dp]),
-        "sofa_amd_scene_set_refinement": (c_int, [_vp, c_int, c_int, c_double, c_int]),
+        "sofa_amd_scene_set_refinement": (c_int, [_vp, c_int, c_int, c_double, c_int, c_int]),
         "sofa_amd_engine_reserve": (c_int, [_vp, ctypes.c_int64, ctypes.c_int64, c_int]),
         "sofa_amd_scene_commit": (c_int, [_vp, c_int]),
         "sofa_amd_scene_release": (c_int, [_vp, c_int]),
@@ -226,10 +226,12 @@ class Engine:
         """FixedProjectiveConstraint fixAll."""
         check(lib().sofa_amd_scene_set_fix_all(self._h, int(scene), 1 if fix_all else 0))
 
-    def set_refinement(self, scene, inner_iterations=30, inner_tolerance=1e-6, max_sweeps=3):
-        """MIXED engines: FP32 inner CG limits and FP64 refinement sweeps per step."""
+    def set_refinement(self, scene, inner_iterations=30, inner_tolerance=1e-6, max_sweeps=3, keep_direction=False):
+        """MIXED engines: FP32 inner CG limits and FP64 refinement sweeps per step; with
+        keep_direction the sweeps continue one CG (residual replacement) instead of restarting."""
         check(lib().sofa_amd_scene_set_refinement(self._h, int(scene), int(inner_iterations),
-                                                  float(inner_tolerance), int(max_sweeps)))
+                                                  float(inner_tolerance), int(max_sweeps),
+                                                  1 if keep_direction else 0))
 
     def set_dot_order(self, sequential=True):
         """FP64 engines: dot products in the reference's sequential node order (bit-exact

@@ -157,6 +157,7 @@ struct SceneHost {
     int inner_iter = 30;
     double inner_tol = 1e-6;
     int max_sweeps = 3;
+    int keep_dir = 0;                       // sweeps continue the previous search direction
     bool released = false;                  // live engines: the scene's data has been released
     // placement after commit
     int node_begin = 0, elem_begin = 0;
@@ -542,6 +543,9 @@ struct EngineT : EngineBase {
         in.first_solve = 0;
         in.stopped = 1;
         in.mg[0] = in.mg[1] = in.mg[2] = 0.0;
+        in.keep_dir = scenes[si].keep_dir;
+        in.sweep = 0;
+        in.rho_last = 0.0;
         return in;
     }
 
@@ -1541,7 +1545,7 @@ int sofa_amd_scene_set_solver(sofa_amd_engine eng, int scene, double dt, double 
 }
 
 int sofa_amd_scene_set_refinement(sofa_amd_engine eng, int scene, int inner_iterations, double inner_tolerance,
-                                  int max_sweeps) {
+                                  int max_sweeps, int keep_direction) {
     SceneHost* s = scene_of(eng, scene);
     if (!s || inner_iterations < 1 || max_sweeps < 1 || !(inner_tolerance >= 0))
         return fail(SOFA_AMD_EINVAL, "invalid refinement parameters");
@@ -1549,6 +1553,7 @@ int sofa_amd_scene_set_refinement(sofa_amd_engine eng, int scene, int inner_iter
     s->inner_iter = inner_iterations;
     s->inner_tol = inner_tolerance;
     s->max_sweeps = max_sweeps;
+    s->keep_dir = keep_direction != 0;
     return SOFA_AMD_OK;
 }
 

@@ -59,7 +59,10 @@ struct InstDev {
     int max_sweeps;                   // inner solves per step at most
     int ir_sweeps;                    // inner solves done in the last step
     int cg_total;                     // inner CG iterations summed over the sweeps of a step
-    int pad2_;
+    int keep_dir;                     // inner record: sweeps continue the previous search direction
+    int sweep;                        // inner record: current sweep (1-based)
+    int pad3_;
+    double rho_last;                  // inner record: r.r that built the current direction
 };
 
 struct Blk { int inst, begin, end, pad; };   // a block's scene and [begin, end) range

@@ -84,7 +84,16 @@ static __global__ __launch_bounds__(kBlock) void k_ir_control(const int* __restr
     } else if (sweep > o->max_sweeps) {
         o->stopped = 1; o->end_reason = 0; in->stopped = 1;
     } else {
-        // a fresh inner solve: its kernels start at iteration 1
+        // the next inner solve: its kernels start at iteration 1.  Keeping the direction, the
+        // r.r that built it is the one of the last iteration whose direction update ran (a
+        // tolerance exit in the operator kernel at iteration j stops before building p_j)
+        if (in->keep_dir && sweep > 1) {
+            int lj = in->last_iters;
+            if (!in->stopped) lj = in->max_iter;
+            else if (in->end_reason == 1) lj = lj - 1;
+            in->rho_last = in->rho_slot[lj & 1];
+        }
+        in->sweep = sweep;
         in->stopped = 0; in->end_reason = 0; in->last_iters = 0; in->first_solve = 0;
     }
 }

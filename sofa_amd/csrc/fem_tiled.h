@@ -366,6 +366,8 @@ __global__ __launch_bounds__(TB) void k_tile_apply(TileArgs<R, NR> a, int iter, 
         }
     }
     InstDev* ins = &a.inst[td.scene];
+    // a refinement sweep that keeps the previous search direction continues CG at its iteration 1
+    const bool cont = iter == 1 && ins->keep_dir && ins->sweep > 1;
     const int ninc = 4 * td.e_count;
     for (int j = threadIdx.x; j < ninc; j += TB) linc[j] = a.tn.inc[td.inc_begin + j];
     for (int j = threadIdx.x; j <= td.ln_count; j += TB) lptr[j] = a.tn.inc_ptr[td.ln_begin + j] - td.inc_begin;
@@ -377,7 +379,7 @@ __global__ __launch_bounds__(TB) void k_tile_apply(TileArgs<R, NR> a, int iter, 
         const int j = threadIdx.x + m * TB;
         if (j < td.ln_count) {
             ld3(r, gid[m], rr_[m]);
-            if (iter > 1) ld3(p, gid[m], po_[m]);
+            if (iter > 1 || cont) ld3(p, gid[m], po_[m]);
             if (ROT) ld3(x, gid[m], xx_[m]);
             fx[m] = a.fixed[gid[m]];
         }
@@ -386,7 +388,17 @@ __global__ __launch_bounds__(TB) void k_tile_apply(TileArgs<R, NR> a, int iter, 
     const bool first = (td.flags & kTileFirst) != 0;
     // ---- 3. scalars: rho, the tolerance test, beta (CGLinearSolver.inl:130-196)
     double beta = 0.0;
-    if (iter == 1) {
+    if (cont) {
+        // the refinement replaced r by the true residual: beta from its r.r and the r.r that
+        // built p; the tolerance is relative to the first sweep's |r|
+        const double rho = reduce2<TB>(pin, rg.b0, rg.nb, 0, 0, sh);
+        const double err = sqrt(rho) / ins->normb;
+        if (rho == 0.0 || err <= ins->tol) { record_stop(ins, first, 1, 1, err); return; }
+        beta = rho / ins->rho_last;
+        if (first && threadIdx.x == 0) {
+            ins->rho_slot[1] = rho; ins->beta = beta; ins->iter = 1; ins->err = err;
+        }
+    } else if (iter == 1) {
         const double bb = pin_nodes ? reduce2<TB>(pin, rg.b0, rg.nb, 0, 0, sh)
                                     : rg.nf > 0 ? reduce2<TB>(pin, rg.f0, rg.nf, 0, 0, sh)
                                                 : reduce2<TB>(pin, rg.t0, rg.nt, 0, 0, sh);
@@ -415,7 +427,7 @@ __global__ __launch_bounds__(TB) void k_tile_apply(TileArgs<R, NR> a, int iter, 
         const int j = threadIdx.x + m * TB;
         if (j < td.ln_count) {
             VT pp[3] = {rr_[m][0], rr_[m][1], rr_[m][2]};
-            if (iter > 1) {
+            if (iter > 1 || cont) {
 #pragma unroll
                 for (int i = 0; i < 3; i++) { VT t = po_[m][i] * vb; t += pp[i]; pp[i] = t; }
             }
@@ -577,12 +589,13 @@ __global__ __launch_bounds__(kBlock) void k_update(const NodeBlk* __restrict__ n
             if (shk[k] < 0) {
                 ld3(p, n, pv[k]);
                 ld3(q, n, qv[k]);
-            } else if (iter > 1) {
+            } else if (iter > 1 || (ins->keep_dir && ins->sweep > 1)) {
                 ld3(p, n, pv[k]);
             }
         }
     }
     if (ins->stopped) return;
+    const bool cont = iter == 1 && ins->keep_dir && ins->sweep > 1;
     const Ranges rg = ranges[nb.scene];
     const bool first = nb.first != 0;
     const double den = reduce2(pden, rg.t0, rg.nt, 0, 0, sh);
@@ -605,7 +618,7 @@ __global__ __launch_bounds__(kBlock) void k_update(const NodeBlk* __restrict__ n
         if (n < nb.end) {
             if (shk[k] >= 0) {
                 VT pp[3] = {rv[k][0], rv[k][1], rv[k][2]};
-                if (iter > 1) {
+                if (iter > 1 || cont) {
 #pragma unroll
                     for (int i = 0; i < 3; i++) { VT t = pv[k][i] * vb; t += pp[i]; pp[i] = t; }
                 }

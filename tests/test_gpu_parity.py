@@ -458,12 +458,14 @@ def converged(c, iters=400, tol=1e-12):
     return d
 
 
+@pytest.mark.parametrize("keep", [False, True])
 @pytest.mark.parametrize("which", ["cfg0", "meshmatrix"])
-def test_mixed_matches_converged_fp64(which):
+def test_mixed_matches_converged_fp64(which, keep):
     """MIXED with the refinement run to |r|/|b| <= 1e-10 stays within 1e-6 relative (positions
-    and velocities) of the FP64 oracle whose CG is run to 1e-12, after 100 (cfg0) / 40 steps."""
+    and velocities) of the FP64 oracle whose CG is run to 1e-12, after 100 (cfg0) / 40 steps --
+    restarting CG each sweep, or continuing its direction from the replaced residual."""
     capi = _capi()
-    print(f"case: mixed vs converged fp64 oracle, {which}", flush=True)
+    print(f"case: mixed vs converged fp64 oracle, {which}, keep_direction={keep}", flush=True)
     if which == "cfg0":
         c = beam_cfg0()
         c.update(tol=1e-10)
@@ -478,7 +480,10 @@ def test_mixed_matches_converged_fp64(which):
         eng = capi.Engine(0, capi.MIXED)
         sid = gpu_mm(eng, c)
         o = oracle_mm(converged(c))
-    eng.set_refinement(sid, inner_iterations=30, inner_tolerance=1e-6, max_sweeps=8)
+    if keep:
+        eng.set_refinement(sid, inner_iterations=30, inner_tolerance=1e-10, max_sweeps=8, keep_direction=True)
+    else:
+        eng.set_refinement(sid, inner_iterations=30, inner_tolerance=1e-6, max_sweeps=8)
     eng.commit()
     eng.step([sid], nsteps)
     x, v, f = eng.read(sid)
