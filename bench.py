@@ -107,11 +107,14 @@ def apply_roofline(eng, reps, pe, pv, nrec, E, N):
     """The dominant kernel: the tiled CG operator apply k_tile_apply (p = r + beta p staged in LDS,
     element rotations recomputed from the positions, element K p, per-node sums in LDS, q of
     tile-owned nodes, partials of shared nodes) over every scene of the engine in one launch,
-    timed with HIP events on its stream.
-    Algorithmic bytes per launch: E (16 + nrec p_e) + N 15 p_v
+    timed with HIP events on its stream.  The element geometry (rotation, rest coordinates) is
+    recomputed from the node positions.
+    Algorithmic bytes per launch: E (16 + nrec p_e) + N 18 p_v
       per element: 16 B of indices (4 local node ids + 4 incidence slots, u16 each) and nrec
-                   reals (rest coordinates 6 (FP32) or 9 (FP64), stiffness 3);
-      per node: x, r and p read, p and q written."""
+                   reals: the stiffness entries K00, K01, K33 (3) where elements have their own
+                   materials; none for a scene of one material (each beam here), whose K the
+                   kernel derives from the rest volume;
+      per node: x, x0, r and p read, p and q written."""
     import torch
     stream = bench_stream()
     sp = stream.cuda_stream
@@ -124,7 +127,7 @@ def apply_roofline(eng, reps, pe, pv, nrec, E, N):
     e1.record(stream)
     torch.cuda.synchronize()
     us = e0.elapsed_time(e1) * 1e3 / reps
-    bytes_alg = E * (16 + nrec * pe) + N * 15 * pv
+    bytes_alg = E * (16 + nrec * pe) + N * 18 * pv
     ach = bytes_alg / (us * 1e-6) / 1e9
     return {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
             "traffic": None, "us_per_launch": us, "algorithmic_bytes": bytes_alg}
@@ -342,7 +345,7 @@ def main():
     eng, ids = make_engine(specs, capi.FP32, graphs=True)
     total_ms, lat = time_merged(eng, ids, args.steps, args.warmup)
     value = E * args.steps / (total_ms / 1e3)
-    roof = apply_roofline(eng, args.apply_reps, 4, 4, 9, E, N)
+    roof = apply_roofline(eng, args.apply_reps, 4, 4, 0, E, N)
     roof["kernel"] = "k_tile_apply<float> over the 256-beam mix (CG operator: addDForce + mass + projection + p update)"
     tr = measured_traffic()
     if tr is not None:
@@ -395,7 +398,7 @@ def main():
         c1 = workloads.cfg1()
         e1, i1 = make_engine([c1], capi.FP32, graphs=True)
         t1, l1 = time_merged(e1, i1, 20, 3)
-        r1 = apply_roofline(e1, args.apply_reps, 4, 4, 9, len(c1["tets"]), len(c1["pos"]))
+        r1 = apply_roofline(e1, args.apply_reps, 4, 4, 0, len(c1["tets"]), len(c1["pos"]))
         e1.close()
         e64, i64 = make_engine([c1], capi.FP64, graphs=True)
         t64, _ = time_merged(e64, i64, 5, 1)

@@ -95,6 +95,18 @@ struct ElemInit {
     double vol;
 };
 
+// computeMaterialStiffness before the division by 36 V: K00, K01, K33 times 36 V, in
+// init_element's operation order
+static void material_factors(double E, double nu, double ks[3]) {
+#pragma clang fp contract(off)
+    const double young = 1.0f * E;
+    const double k00 = 1, k01 = nu / (1 - nu), k33 = (1 - 2 * nu) / (2 * (1 - nu));
+    const double s = (young * (1 - nu)) / ((1 + nu) * (1 - 2 * nu));
+    ks[0] = k00 * s;
+    ks[1] = k01 * s;
+    ks[2] = k33 * s;
+}
+
 static bool init_element(const std::vector<V3>& P, const int32_t* t, double E, double nu, ElemInit& out) {
 #pragma clang fp contract(off)
     const V3 &pa = P[t[0]], &pb = P[t[1]], &pc = P[t[2]], &pd = P[t[3]];
@@ -281,6 +293,7 @@ struct EngineT : EngineBase {
     int64_t Ntot = 0, Etot = 0;     // node / element capacity (the SoA stride of element arrays)
     // node vectors (AoS xyz)
     VT *x = nullptr, *v = nullptr, *f = nullptr, *b = nullptr, *r = nullptr, *p = nullptr, *q = nullptr, *xcg = nullptr;
+    VT* x0 = nullptr;                              // rest positions (tiled path)
     double* mass = nullptr;
     int8_t* fixed = nullptr;
     int *inc_ptr = nullptr, *inc = nullptr;        // node -> (position*4 + vertex), ascending element id
@@ -352,7 +365,7 @@ struct EngineT : EngineBase {
                         inst, partial, tiles_d, ranges_d, ltet, ln_gid, ln_inc_ptr, ln_pslot, sh_gid, sh_ptr,
                         ln_inc, sp0, sp1, pbb, pden, prho, p2, em_ptr, em_nbr, em_val, emass, osc_node,
                         osc_mean, osc_amp, osc_w, osc_phase, rot32, rest32, kmat32, emass32, r32, p32, q32,
-                        d32, sp32, pden32, prho32, prr, inst_in, sh_of, nd};
+                        d32, sp32, pden32, prho32, prr, inst_in, sh_of, nd, x0};
         for (void* ptr : ptrs)
             if (ptr) (void)hipFree(ptr);
     }
@@ -504,6 +517,14 @@ struct EngineT : EngineBase {
         }
     }
 
+    // one E and one nu for every element (BaseLinearElasticityFEMForceField.inl:113-144 reads
+    // entry 0 for elements beyond the vectors' length)
+    static bool one_material(const SceneHost& s) {
+        for (double y : s.young) if (y != s.young[0]) return false;
+        for (double p : s.poisson) if (p != s.poisson[0]) return false;
+        return true;
+    }
+
     // the scene record: EulerImplicitSolver / CGLinearSolver parameters and the factors of the
     // mechanical operations (EulerImplicitSolver.cpp:147-171)
     InstDev make_inst(int si) const {
@@ -531,6 +552,7 @@ struct EngineT : EngineBase {
         in.first_solve = 1;
         in.stopped = tiled ? 0 : 1;
         in.max_sweeps = s.max_sweeps;
+        if (one_material(s)) host::material_factors(s.young[0], s.poisson[0], in.ks);
         return in;
     }
     // the inner FP32 CG's record: the same operator, FP32 CG limits, no threshold test (the
@@ -714,7 +736,7 @@ struct EngineT : EngineBase {
         sets.clear();
         cur = nullptr;
         free_device();
-        x = v = f = b = r = p = q = xcg = nullptr; mass = nullptr; fixed = nullptr; inc_ptr = inc = nullptr;
+        x = v = f = b = r = p = q = xcg = x0 = nullptr; mass = nullptr; fixed = nullptr; inc_ptr = inc = nullptr;
         tet = nullptr; rot = rest = kmat = nullptr; ef = ef2 = nullptr; p2 = nullptr; em_ptr = em_nbr = nullptr;
         em_val = nullptr; emass = nullptr; osc_node = nullptr; osc_mean = osc_amp = osc_w = osc_phase = nullptr;
         inst = nullptr; partial = nullptr; tiles_d = nullptr; ranges_d = nullptr; ltet = nullptr;
@@ -728,7 +750,7 @@ struct EngineT : EngineBase {
         for (int k = 0; k < 11; k++) arenas(k)->reset(field(cap, k));
         Ntot = cap.node; Etot = cap.elem;
         const size_t N = std::max<int64_t>(1, cap.node), E = std::max<int64_t>(1, cap.elem);
-        for (VT** vec : {&x, &v, &f, &b, &r, &p, &q, &xcg}) {
+        for (VT** vec : {&x, &v, &f, &b, &r, &p, &q, &xcg, &x0}) {
             *vec = dalloc<VT>(3 * N);
             HIPCHK(hipMemset(*vec, 0, 3 * N * sizeof(VT)));
         }
@@ -800,6 +822,7 @@ struct EngineT : EngineBase {
         std::vector<VT> xv(3 * (size_t)(n + 1), (VT)0), zero(3 * (size_t)(n + 1), (VT)0);
         for (int i = 0; i < 3 * n; i++) xv[i] = (VT)s.rest[i];
         put_at(x, 3 * (int64_t)nb0, xv);
+        put_at(x0, 3 * (int64_t)nb0, xv);
         for (VT* vec : {v, f, b, r, p, q, xcg}) put_at(vec, 3 * (int64_t)nb0, zero);
         std::vector<double> mh(th.eh.mass);
         mh.push_back(0.0);
@@ -846,7 +869,8 @@ struct EngineT : EngineBase {
             td.e_count = P.t_ecount[t];
             td.ln_begin = (int)pl.at.lnode + (int)lg.size();
             td.ln_count = P.t_lncount[t];
-            td.flags = (t == 0 ? kTileFirst : 0) | (s.mass_kind == SOFA_AMD_MASS_MESHMATRIX ? kTileConsistent : 0);
+            td.flags = (t == 0 ? kTileFirst : 0) | (s.mass_kind == SOFA_AMD_MASS_MESHMATRIX ? kTileConsistent : 0) |
+                       (one_material(s) ? kTileOneMaterial : 0);
             td.pslot = (int)pl.at.tile + t;
             td.inc_begin = (int)ib0 + P.ln_inc_ptr[P.t_lnbegin[t]];
             for (int j = 0; j <= P.t_lncount[t]; j++) {
@@ -1095,7 +1119,7 @@ struct EngineT : EngineBase {
         if (cur->nb > 0) k_fin_rhs<VT><<<cur->nb, kBlock, 0, s>>>(fb, inst, ranges_d, sn, mass, fixed, v, sp0, sp1, f, b, r, xcg, pbb);
         for (int it = 1; it <= cur->maxit; it++) {
             if (cur->na > 0)
-                k_tile_apply<R, NR, VT, kTileThreads, true><<<cur->na, kTileThreads, 0, s>>>(ta, it, r, p, q, sp0, it == 1 ? pbb : prho, pden, 0, x);
+                k_tile_apply<R, NR, VT, kTileThreads, true><<<cur->na, kTileThreads, 0, s>>>(ta, it, r, p, q, sp0, it == 1 ? pbb : prho, pden, 0, x, x0);
             if (cur->nc > 0)
                 k_update<VT><<<cur->nc, kBlock, 0, s>>>(nb, inst, ranges_d, it, sn, sp0, mass, fixed, p, q, xcg, r, pden, prho);
         }
@@ -1133,7 +1157,7 @@ struct EngineT : EngineBase {
         for (int sw = 1; sw <= cur->maxsw + 1; sw++) {
             if (sw > 1) {   // q = A x, FP64
                 if (cur->na > 0)
-                    k_tile_apply<R, NR, VT, kTileThreads, true><<<cur->na, kTileThreads, 0, s>>>(ta, 0, xcg, nullptr, q, sp0, nullptr, pden, 0, x);
+                    k_tile_apply<R, NR, VT, kTileThreads, true><<<cur->na, kTileThreads, 0, s>>>(ta, 0, xcg, nullptr, q, sp0, nullptr, pden, 0, x, x0);
                 if (cur->nb > 0)
                     k_fin_apply<VT><<<cur->nb, kBlock, 0, s>>>(fb, inst, ranges_d, sn, mass, fixed, 0, xcg, nullptr, q, sp0, pden);
             }
@@ -1192,7 +1216,7 @@ struct EngineT : EngineBase {
         prepare_set(ids.data(), (int)ids.size(), s);
         const TileArgs<R, NR> ta = tile_args();
         for (int i = 0; i < reps; i++)
-            if (cur->na > 0) k_tile_apply<R, NR, VT, kTileThreads, true><<<cur->na, kTileThreads, 0, s>>>(ta, 2, r, p, q, sp0, prho, pden, 0, x);
+            if (cur->na > 0) k_tile_apply<R, NR, VT, kTileThreads, true><<<cur->na, kTileThreads, 0, s>>>(ta, 2, r, p, q, sp0, prho, pden, 0, x, x0);
         HIPCHK(hipGetLastError());
     }
     // tiles, tile-local nodes, nodes finished by the finalize kernel, shared partial slots,
@@ -1319,7 +1343,7 @@ void launch_inner_cg(const InnerCg& c, hipStream_t s) {
     for (int it = 1; it <= c.maxit; it++) {
         if (c.na > 0)
             k_tile_apply<float, 6, float, kTileThreads, false><<<c.na, kTileThreads, 0, s>>>(c.ta, it, c.r, c.p, c.q, c.sp,
-                                                                                        c.prho, c.pden, 1, nullptr);
+                                                                                        c.prho, c.pden, 1, nullptr, nullptr);
         if (c.nnb > 0)
             k_update<float><<<c.nnb, kBlock, 0, s>>>(c.nb, c.ta.inst, c.ta.ranges, it, c.sn, c.sp, c.ta.mass, c.ta.fixed,
                                                       c.p, c.q, c.d, c.r, c.pden, c.prho);

@@ -63,6 +63,8 @@ struct InstDev {
     int sweep;                        // inner record: current sweep (1-based)
     int pad3_;
     double rho_last;                  // inner record: r.r that built the current direction
+    double ks[3];                     // one material for the whole scene: K00, K01, K33 times
+                                      // 36 V (computeMaterialStiffness before the volume division)
 };
 
 struct Blk { int inst, begin, end, pad; };   // a block's scene and [begin, end) range
@@ -228,6 +230,59 @@ __device__ __forceinline__ void rotation_large(const R p0[3], const R p1[3], con
     rot[0] = ex0; rot[1] = ey0; rot[2] = ez0;
     rot[3] = ex1; rot[4] = ey1; rot[5] = ez1;
     rot[6] = ex2; rot[7] = ey2; rot[8] = ez2;
+}
+
+// initLarge (TetrahedronFEMForceField.inl:835-868): the rest frame R_0_1 from the rest positions
+// of vertices a, b, c, and the rest shape in it relative to a, in the element set-up's order
+// (each coordinate a three-term dot product, then the difference to vertex a).  out: NR = 9,
+// b, c and d with their rounding residue; NR = 6, b0, c0, c1, d0, d1, d2.
+template <typename R, int NR>
+__device__ __forceinline__ void rest_frame(const R p0[4][3], R* out) {
+#pragma clang fp contract(off)
+    R rot[9];
+    rotation_large(p0[0], p0[1], p0[2], rot);      // rot[3 i + j] = rows[j][i]
+    R ri[4][3];
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            R r = rot[i] * p0[k][0];
+            r += rot[3 + i] * p0[k][1];
+            r += rot[6 + i] * p0[k][2];
+            ri[k][i] = r;
+        }
+#pragma unroll
+    for (int k = 1; k < 4; k++)
+#pragma unroll
+        for (int i = 0; i < 3; i++) ri[k][i] -= ri[0][i];
+    if (NR == 9) {
+#pragma unroll
+        for (int k = 1; k < 4; k++)
+#pragma unroll
+            for (int i = 0; i < 3; i++) out[3 * (k - 1) + i] = ri[k][i];
+    } else {
+        out[0] = ri[1][0];
+        out[1] = ri[2][0]; out[2] = ri[2][1];
+        out[3] = ri[3][0]; out[4] = ri[3][1]; out[5] = ri[3][2];
+    }
+}
+
+// computeMaterialStiffness's last step for a scene of one material: K = ks / (36 V), V the
+// absolute volume of the rest tetrahedron (Tetrahedron.h:55-58), in the element set-up's order
+template <typename R>
+__device__ __forceinline__ void stiffness_from_volume(const R p0[4][3], const double ks[3], R k[3]) {
+#pragma clang fp contract(off)
+    const R a0 = p0[1][0] - p0[0][0], a1 = p0[1][1] - p0[0][1], a2 = p0[1][2] - p0[0][2];
+    const R b0 = p0[2][0] - p0[0][0], b1 = p0[2][1] - p0[0][1], b2 = p0[2][2] - p0[0][2];
+    const R c0 = p0[3][0] - p0[0][0], c1 = p0[3][1] - p0[0][1], c2 = p0[3][2] - p0[0][2];
+    const R x0 = a1 * b2 - a2 * b1, x1 = a2 * b0 - a0 * b2, x2 = a0 * b1 - a1 * b0;
+    R d = x0 * c0;
+    d += x1 * c1;
+    d += x2 * c2;
+    const R vol = fabs(d / (R)6);
+    const R div = vol * (R)36;
+#pragma unroll
+    for (int i = 0; i < 3; i++) k[i] = (R)ks[i] / div;
 }
 
 template <typename R, int NR>

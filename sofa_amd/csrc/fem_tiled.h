@@ -43,6 +43,7 @@ struct TileDesc {
     int inc_begin;            // first entry of the tile's incidence lists (ln_inc)
 };
 constexpr int kTileFirst = 1, kTileConsistent = 2;
+constexpr int kTileOneMaterial = 4;   // the scene has one E and nu: K follows from the rest volume
 
 struct FinBlk { int scene, begin, end, pslot, bi, pad0, pad1, pad2; };  // shared-node range; bi = index in scene
 struct NodeBlk { int scene, begin, end, pslot, first, pad0, pad1, pad2; };  // node range
@@ -312,19 +313,22 @@ __global__ __launch_bounds__(kBlock) void k_fin_rhs(const FinBlk* __restrict__ f
 // iter = 0 is a plain operator application q = A r (no CG scalars, p untouched), used for
 // the FP64 residual of iterative refinement.
 // ---------------------------------------------------------------------------------------
-// ROT: the element rotations are recomputed from the positions x (the ones the force pass used:
-// x changes only after the solve), bit for bit the stored ones, instead of read back -- 9 reals
-// less per element and launch, 3 more per tile-local node.  Without ROT they are loaded.
-template <typename R, int NR, typename VT, int TB = kBlock, bool ROT = false>
+// GEO: the element geometry is recomputed from the node positions instead of read back: the
+// rotation from the current positions x (the ones the force pass used: x changes only after the
+// solve) and the rotated rest coordinates from the rest positions x0 (initLarge,
+// TetrahedronFEMForceField.inl:835-868), with the functions and operation order of the element
+// set-up and no fused multiply-adds -- 9 + NR reals less per element and launch, 6 more per
+// tile-local node.  Without GEO both are loaded.
+template <typename R, int NR, typename VT, int TB = kBlock, bool GEO = false>
 __global__ __launch_bounds__(TB) void k_tile_apply(TileArgs<R, NR> a, int iter, const VT* __restrict__ r,
                                                     VT* __restrict__ p, VT* __restrict__ q, VT* __restrict__ sp,
                                                     const double* __restrict__ pin,   // |b|^2 (iter 1) or r.r partials
                                                     double* __restrict__ pden,
                                                     int pin_nodes,     // 1: iter-1 |b|^2 partials per node block
-                                                    const VT* __restrict__ x) {
+                                                    const VT* __restrict__ x, const VT* __restrict__ x0) {
     constexpr int EPT = kTileElems / TB;                   // elements per thread
     constexpr int NPT = (kTileNodes + TB - 1) / TB;        // tile-local nodes per thread
-    static_assert(kTileElems * 12 >= kTileNodes * 3, "positions are staged in the contribution buffer");
+    static_assert(kTileElems * 12 >= kTileNodes * 6, "positions are staged in the contribution buffer");
     __shared__ R lu[kTileNodes * 3];
     __shared__ R lc[kTileElems * 12];
     __shared__ uint16_t linc[kTileElems * 4];
@@ -332,6 +336,7 @@ __global__ __launch_bounds__(TB) void k_tile_apply(TileArgs<R, NR> a, int iter, 
     __shared__ double sh[TB / 64];
     const TileDesc td = a.tiles[blockIdx.x];
     const bool consistent = (td.flags & kTileConsistent) != 0;
+    const bool kvol = GEO && (td.flags & kTileOneMaterial) != 0;   // K recomputed, not loaded
     // ---- 1. issue every load that depends on the tile descriptor only: the element records of
     //         this thread (kept in registers), the tile-local node ids / slots / masks, the
     //         incidence tables, the scene record
@@ -343,14 +348,16 @@ __global__ __launch_bounds__(TB) void k_tile_apply(TileArgs<R, NR> a, int iter, 
         if (k < td.e_count) {
             const int64_t e = (int64_t)td.e_begin + k;
             lt[m] = a.ltet[e];
-            if (!ROT) {
+            if (!GEO) {
 #pragma unroll
                 for (int i = 0; i < 9; i++) rot[m][i] = a.ev.rot[i * a.ev.E + e];
+#pragma unroll
+                for (int i = 0; i < NR; i++) rin[m][i] = a.ev.rest[i * a.ev.E + e];
             }
+            if (!kvol) {
 #pragma unroll
-            for (int i = 0; i < NR; i++) rin[m][i] = a.ev.rest[i * a.ev.E + e];
-#pragma unroll
-            for (int i = 0; i < 3; i++) kk[m][i] = a.ev.kmat[i * a.ev.E + e];
+                for (int i = 0; i < 3; i++) kk[m][i] = a.ev.kmat[i * a.ev.E + e];
+            }
             em[m] = consistent ? a.emass[e] : (R)0;
         }
     }
@@ -373,14 +380,14 @@ __global__ __launch_bounds__(TB) void k_tile_apply(TileArgs<R, NR> a, int iter, 
     for (int j = threadIdx.x; j <= td.ln_count; j += TB) lptr[j] = a.tn.inc_ptr[td.ln_begin + j] - td.inc_begin;
     if (ins->stopped) return;
     // ---- 2. the tile's nodes: r and p (dependent on the node ids), masks
-    VT rr_[NPT][3], po_[NPT][3], xx_[NPT][3];
+    VT rr_[NPT][3], po_[NPT][3], xx_[NPT][3], x0_[NPT][3];
 #pragma unroll
     for (int m = 0; m < NPT; m++) {
         const int j = threadIdx.x + m * TB;
         if (j < td.ln_count) {
             ld3(r, gid[m], rr_[m]);
             if (iter > 1 || cont) ld3(p, gid[m], po_[m]);
-            if (ROT) ld3(x, gid[m], xx_[m]);
+            if (GEO) { ld3(x, gid[m], xx_[m]); ld3(x0, gid[m], x0_[m]); }
             fx[m] = a.fixed[gid[m]];
         }
     }
@@ -433,21 +440,32 @@ __global__ __launch_bounds__(TB) void k_tile_apply(TileArgs<R, NR> a, int iter, 
             }
             if (iter > 0 && psl[m] < 0) st3(p, gid[m], pp);
             lu[3 * j] = (R)pp[0]; lu[3 * j + 1] = (R)pp[1]; lu[3 * j + 2] = (R)pp[2];
-            if (ROT) { lc[3 * j] = (R)xx_[m][0]; lc[3 * j + 1] = (R)xx_[m][1]; lc[3 * j + 2] = (R)xx_[m][2]; }
+            if (GEO) {
+                R* lx = lc;
+                R* lx0 = lc + 3 * kTileNodes;
+#pragma unroll
+                for (int i = 0; i < 3; i++) { lx[3 * j + i] = (R)xx_[m][i]; lx0[3 * j + i] = (R)x0_[m][i]; }
+            }
         }
     }
     __syncthreads();
-    if (ROT) {   // rotations from the staged positions, then the buffer is free again
+    if (GEO) {   // geometry from the staged positions, then the buffer is free again
+        const R* lx = lc;
+        const R* lx0 = lc + 3 * kTileNodes;
 #pragma unroll
         for (int m = 0; m < EPT; m++) {
             const int k = threadIdx.x + m * TB;
             if (k < td.e_count) {
                 int l[4];
                 unpack_ltet(lt[m], l);
-                const R p0[3] = {lc[3 * l[0]], lc[3 * l[0] + 1], lc[3 * l[0] + 2]};
-                const R p1[3] = {lc[3 * l[1]], lc[3 * l[1] + 1], lc[3 * l[1] + 2]};
-                const R p2[3] = {lc[3 * l[2]], lc[3 * l[2] + 1], lc[3 * l[2] + 2]};
-                rotation_large(p0, p1, p2, rot[m]);
+                R p[4][3], p0[4][3];
+#pragma unroll
+                for (int v = 0; v < 4; v++)
+#pragma unroll
+                    for (int i = 0; i < 3; i++) { p[v][i] = lx[3 * l[v] + i]; p0[v][i] = lx0[3 * l[v] + i]; }
+                rotation_large(p[0], p[1], p[2], rot[m]);
+                rest_frame<R, NR>(p0, rin[m]);
+                if (kvol) stiffness_from_volume(p0, ins->ks, kk[m]);
             }
         }
         __syncthreads();

@@ -60,7 +60,8 @@ class Server:
         self.pool = cf.ThreadPoolExecutor(build_workers)
 
     def close(self):
-        self.pool.shutdown(wait=False, cancel_futures=True)
+        # queued preparations are dropped, running ones finish before the engine they read goes
+        self.pool.shutdown(wait=True, cancel_futures=True)
         self.eng.close()
 
     def run(self, jobs, wall_limit=30.0, stream=None, build=workloads.build_job, log=None, on_finish=None,
