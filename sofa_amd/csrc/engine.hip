@@ -690,18 +690,11 @@ struct EngineT : EngineBase {
     // on another host thread than the one stepping the engine
     void build_tiled(const SceneHost& s, int si, TiledHost& th) const {
         const int nE = (int)(s.tets.size() / 4), n = s.n_nodes;
-        if (nE > 0) {
-            th.plan = plan_tiles(n, s.rest.data(), s.tets.data(), nE, kTileElems, kTileNodes);
-            th.iperm = th.plan.iperm;
-        }
+        th.plan = plan_tiles(n, s.rest.data(), s.tets.data(), nE, kTileElems, kTileNodes);
+        th.iperm = th.plan.iperm;
         setup_scene(s, si, th.iperm, th.eh);
         th.sh_gid = th.plan.sh_gid;
         th.sh_ptr = th.plan.sh_ptr;
-        if (nE == 0) {   // no elements: every node is finished by the shared-node kernel
-            th.sh_gid.resize(n);
-            for (int i = 0; i < n; i++) th.sh_gid[i] = i;
-            th.sh_ptr.assign(n + 1, 0);
-        }
         if (th.sh_ptr.empty()) th.sh_ptr.push_back(0);
         Extent& e = th.need;
         const int64_t T = (int64_t)th.plan.t_ebegin.size();
@@ -1119,7 +1112,7 @@ struct EngineT : EngineBase {
         if (cur->nb > 0) k_fin_rhs<VT><<<cur->nb, kBlock, 0, s>>>(fb, inst, ranges_d, sn, mass, fixed, v, sp0, sp1, f, b, r, xcg, pbb);
         for (int it = 1; it <= cur->maxit; it++) {
             if (cur->na > 0)
-                k_tile_apply<R, NR, VT, kTileThreads, true><<<cur->na, kTileThreads, 0, s>>>(ta, it, r, p, q, sp0, it == 1 ? pbb : prho, pden, 0, x, x0);
+                k_tile_apply<R, NR, VT, kTileThreads, kApplyGeometry><<<cur->na, kTileThreads, 0, s>>>(ta, it, r, p, q, sp0, it == 1 ? pbb : prho, pden, 0, x, x0);
             if (cur->nc > 0)
                 k_update<VT><<<cur->nc, kBlock, 0, s>>>(nb, inst, ranges_d, it, sn, sp0, mass, fixed, p, q, xcg, r, pden, prho);
         }
@@ -1157,7 +1150,7 @@ struct EngineT : EngineBase {
         for (int sw = 1; sw <= cur->maxsw + 1; sw++) {
             if (sw > 1) {   // q = A x, FP64
                 if (cur->na > 0)
-                    k_tile_apply<R, NR, VT, kTileThreads, true><<<cur->na, kTileThreads, 0, s>>>(ta, 0, xcg, nullptr, q, sp0, nullptr, pden, 0, x, x0);
+                    k_tile_apply<R, NR, VT, kTileThreads, kApplyGeometry><<<cur->na, kTileThreads, 0, s>>>(ta, 0, xcg, nullptr, q, sp0, nullptr, pden, 0, x, x0);
                 if (cur->nb > 0)
                     k_fin_apply<VT><<<cur->nb, kBlock, 0, s>>>(fb, inst, ranges_d, sn, mass, fixed, 0, xcg, nullptr, q, sp0, pden);
             }
@@ -1216,7 +1209,7 @@ struct EngineT : EngineBase {
         prepare_set(ids.data(), (int)ids.size(), s);
         const TileArgs<R, NR> ta = tile_args();
         for (int i = 0; i < reps; i++)
-            if (cur->na > 0) k_tile_apply<R, NR, VT, kTileThreads, true><<<cur->na, kTileThreads, 0, s>>>(ta, 2, r, p, q, sp0, prho, pden, 0, x, x0);
+            if (cur->na > 0) k_tile_apply<R, NR, VT, kTileThreads, kApplyGeometry><<<cur->na, kTileThreads, 0, s>>>(ta, 2, r, p, q, sp0, prho, pden, 0, x, x0);
         HIPCHK(hipGetLastError());
     }
     // tiles, tile-local nodes, nodes finished by the finalize kernel, shared partial slots,
@@ -1342,7 +1335,7 @@ EngineBase* make_engine_fp32() {
 void launch_inner_cg(const InnerCg& c, hipStream_t s) {
     for (int it = 1; it <= c.maxit; it++) {
         if (c.na > 0)
-            k_tile_apply<float, 6, float, kTileThreads, false><<<c.na, kTileThreads, 0, s>>>(c.ta, it, c.r, c.p, c.q, c.sp,
+            k_tile_apply<float, 6, float, kTileThreads, 0><<<c.na, kTileThreads, 0, s>>>(c.ta, it, c.r, c.p, c.q, c.sp,
                                                                                         c.prho, c.pden, 1, nullptr, nullptr);
         if (c.nnb > 0)
             k_update<float><<<c.nnb, kBlock, 0, s>>>(c.nb, c.ta.inst, c.ta.ranges, it, c.sn, c.sp, c.ta.mass, c.ta.fixed,

@@ -473,7 +473,7 @@ __device__ __forceinline__ bool last_block_reduce(double blockval, double* parti
 // the same hand-off with the reference's summation order: every thread has stored its node's
 // value in nd; the last-arriving block adds nd[node_begin], nd[node_begin + 1], ... one after
 // the other from 0.0, as MechanicalObject::vDot accumulates (MechanicalObject.inl:2334-2356).
-// One wave does the adds (each lane of a 64-value chunk handed to every lane in order), so the
+// One wave does the adds (each lane of a 64-value chunk read by every lane in order), so the
 // cost is one dependent add per node of the scene.
 __device__ __forceinline__ bool last_block_sequential(const double* nd, InstDev* ins, double& total) {
     __shared__ int am_last;
@@ -493,7 +493,13 @@ __device__ __forceinline__ bool last_block_sequential(const double* nd, InstDev*
             const int i = base + (int)threadIdx.x;
             const double v = i < nn ? nd[b0 + i] : 0.0;
             const int cnt = min(64, nn - base);
-            for (int k = 0; k < cnt; k++) s = s + __shfl(v, k, 64);
+            // lane k's value read into a scalar register (constant lane index), added in order
+            const int lo = __double2loint(v), hi = __double2hiint(v);
+#pragma unroll
+            for (int k = 0; k < 64; k++) {
+                if (k < cnt)
+                    s = s + __hiloint2double(__builtin_amdgcn_readlane(hi, k), __builtin_amdgcn_readlane(lo, k));
+            }
         }
     }
     total = s;                        // valid in the first wave

@@ -27,11 +27,18 @@ namespace sofa_amd {
 #endif
 constexpr int kTileElems = SOFA_AMD_TILE_ELEMS;   // max elements per tile
 constexpr int kTileNodes = SOFA_AMD_TILE_ELEMS;   // max distinct nodes per tile
-constexpr int kNodesPerThread = 4;  // node kernels: 256 threads x 4 nodes per block
+#ifndef SOFA_AMD_NODES_PER_THREAD
+#define SOFA_AMD_NODES_PER_THREAD 4
+#endif
+constexpr int kNodesPerThread = SOFA_AMD_NODES_PER_THREAD;   // node kernels: 256 threads x this
 #ifndef SOFA_AMD_TILE_THREADS
-#define SOFA_AMD_TILE_THREADS 256
+#define SOFA_AMD_TILE_THREADS 512
 #endif
 constexpr int kTileThreads = SOFA_AMD_TILE_THREADS;   // threads of the tile apply kernel
+#ifndef SOFA_AMD_APPLY_GEOMETRY
+#define SOFA_AMD_APPLY_GEOMETRY 2
+#endif
+constexpr int kApplyGeometry = SOFA_AMD_APPLY_GEOMETRY;   // geometry the tile apply recomputes
 
 struct TileDesc {
     int scene;
@@ -313,13 +320,14 @@ __global__ __launch_bounds__(kBlock) void k_fin_rhs(const FinBlk* __restrict__ f
 // iter = 0 is a plain operator application q = A r (no CG scalars, p untouched), used for
 // the FP64 residual of iterative refinement.
 // ---------------------------------------------------------------------------------------
-// GEO: the element geometry is recomputed from the node positions instead of read back: the
-// rotation from the current positions x (the ones the force pass used: x changes only after the
-// solve) and the rotated rest coordinates from the rest positions x0 (initLarge,
-// TetrahedronFEMForceField.inl:835-868), with the functions and operation order of the element
-// set-up and no fused multiply-adds -- 9 + NR reals less per element and launch, 6 more per
-// tile-local node.  Without GEO both are loaded.
-template <typename R, int NR, typename VT, int TB = kBlock, bool GEO = false>
+// GEO: how much element geometry is recomputed from node positions instead of read back, with
+// the functions and operation order of the element set-up and no fused multiply-adds (so the
+// same bits).  0: none (rotation, rest coordinates, K loaded).  1: the rotation, from the
+// current positions x (the ones the force pass used: x changes only after the solve) -- 9 reals
+// less per element, 3 more per tile-local node.  2: also the rotated rest coordinates from the
+// rest positions x0 (initLarge, TetrahedronFEMForceField.inl:835-868) and, for a scene of one
+// material, K from the rest volume -- NR (+ 3) reals less per element, 3 more per local node.
+template <typename R, int NR, typename VT, int TB = kBlock, int GEO = 0>
 __global__ __launch_bounds__(TB) void k_tile_apply(TileArgs<R, NR> a, int iter, const VT* __restrict__ r,
                                                     VT* __restrict__ p, VT* __restrict__ q, VT* __restrict__ sp,
                                                     const double* __restrict__ pin,   // |b|^2 (iter 1) or r.r partials
@@ -336,7 +344,7 @@ __global__ __launch_bounds__(TB) void k_tile_apply(TileArgs<R, NR> a, int iter, 
     __shared__ double sh[TB / 64];
     const TileDesc td = a.tiles[blockIdx.x];
     const bool consistent = (td.flags & kTileConsistent) != 0;
-    const bool kvol = GEO && (td.flags & kTileOneMaterial) != 0;   // K recomputed, not loaded
+    const bool kvol = GEO == 2 && (td.flags & kTileOneMaterial) != 0;   // K recomputed, not loaded
     // ---- 1. issue every load that depends on the tile descriptor only: the element records of
     //         this thread (kept in registers), the tile-local node ids / slots / masks, the
     //         incidence tables, the scene record
@@ -348,9 +356,11 @@ __global__ __launch_bounds__(TB) void k_tile_apply(TileArgs<R, NR> a, int iter, 
         if (k < td.e_count) {
             const int64_t e = (int64_t)td.e_begin + k;
             lt[m] = a.ltet[e];
-            if (!GEO) {
+            if (GEO == 0) {
 #pragma unroll
                 for (int i = 0; i < 9; i++) rot[m][i] = a.ev.rot[i * a.ev.E + e];
+            }
+            if (GEO < 2) {
 #pragma unroll
                 for (int i = 0; i < NR; i++) rin[m][i] = a.ev.rest[i * a.ev.E + e];
             }
@@ -387,7 +397,8 @@ __global__ __launch_bounds__(TB) void k_tile_apply(TileArgs<R, NR> a, int iter, 
         if (j < td.ln_count) {
             ld3(r, gid[m], rr_[m]);
             if (iter > 1 || cont) ld3(p, gid[m], po_[m]);
-            if (GEO) { ld3(x, gid[m], xx_[m]); ld3(x0, gid[m], x0_[m]); }
+            if (GEO >= 1) ld3(x, gid[m], xx_[m]);
+            if (GEO == 2) ld3(x0, gid[m], x0_[m]);
             fx[m] = a.fixed[gid[m]];
         }
     }
@@ -440,16 +451,18 @@ __global__ __launch_bounds__(TB) void k_tile_apply(TileArgs<R, NR> a, int iter, 
             }
             if (iter > 0 && psl[m] < 0) st3(p, gid[m], pp);
             lu[3 * j] = (R)pp[0]; lu[3 * j + 1] = (R)pp[1]; lu[3 * j + 2] = (R)pp[2];
-            if (GEO) {
-                R* lx = lc;
-                R* lx0 = lc + 3 * kTileNodes;
+            if (GEO >= 1) {
 #pragma unroll
-                for (int i = 0; i < 3; i++) { lx[3 * j + i] = (R)xx_[m][i]; lx0[3 * j + i] = (R)x0_[m][i]; }
+                for (int i = 0; i < 3; i++) lc[3 * j + i] = (R)xx_[m][i];
+            }
+            if (GEO == 2) {
+#pragma unroll
+                for (int i = 0; i < 3; i++) lc[3 * kTileNodes + 3 * j + i] = (R)x0_[m][i];
             }
         }
     }
     __syncthreads();
-    if (GEO) {   // geometry from the staged positions, then the buffer is free again
+    if (GEO >= 1) {   // geometry from the staged positions, then the buffer is free again
         const R* lx = lc;
         const R* lx0 = lc + 3 * kTileNodes;
 #pragma unroll
@@ -458,14 +471,21 @@ __global__ __launch_bounds__(TB) void k_tile_apply(TileArgs<R, NR> a, int iter, 
             if (k < td.e_count) {
                 int l[4];
                 unpack_ltet(lt[m], l);
-                R p[4][3], p0[4][3];
+                R p[3][3];
 #pragma unroll
-                for (int v = 0; v < 4; v++)
+                for (int v = 0; v < 3; v++)
 #pragma unroll
-                    for (int i = 0; i < 3; i++) { p[v][i] = lx[3 * l[v] + i]; p0[v][i] = lx0[3 * l[v] + i]; }
+                    for (int i = 0; i < 3; i++) p[v][i] = lx[3 * l[v] + i];
                 rotation_large(p[0], p[1], p[2], rot[m]);
-                rest_frame<R, NR>(p0, rin[m]);
-                if (kvol) stiffness_from_volume(p0, ins->ks, kk[m]);
+                if (GEO == 2) {
+                    R p0[4][3];
+#pragma unroll
+                    for (int v = 0; v < 4; v++)
+#pragma unroll
+                        for (int i = 0; i < 3; i++) p0[v][i] = lx0[3 * l[v] + i];
+                    rest_frame<R, NR>(p0, rin[m]);
+                    if (kvol) stiffness_from_volume(p0, ins->ks, kk[m]);
+                }
             }
         }
         __syncthreads();

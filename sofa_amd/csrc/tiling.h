@@ -171,11 +171,31 @@ static inline TilePlan plan_tiles(int n_nodes, const double* rest, const int32_t
             P.ln_gid.push_back(nodes[j]);
         }
     }
-    // shared nodes (in >1 tile) and element-free nodes go to the finalize kernel
+    // nodes of no element get tiles of their own (no elements, up to max_nodes nodes), where
+    // they are finished like any node owned by one tile (the mass term only)
+    {
+        std::vector<int> free_nodes;
+        for (int n = 0; n < n_nodes; n++)
+            if (tiles_of[n] == 0) free_nodes.push_back(n);
+        for (size_t j0 = 0; j0 < free_nodes.size(); j0 += (size_t)max_nodes) {
+            const size_t j1 = std::min(free_nodes.size(), j0 + (size_t)max_nodes);
+            P.t_ebegin.push_back(E);
+            P.t_ecount.push_back(0);
+            P.t_lnbegin.push_back((int)P.ln_gid.size());
+            P.t_lncount.push_back((int)(j1 - j0));
+            for (size_t j = j0; j < j1; j++) {
+                P.ln_gid.push_back(free_nodes[j]);
+                P.ln_inc_ptr.push_back((int)P.ln_inc.size());
+                tiles_of[free_nodes[j]] = 1;
+            }
+        }
+    }
+    const int TT = (int)P.t_ebegin.size();
+    // shared nodes (in >1 tile) go to the finalize kernel
     std::vector<int> shidx(n_nodes, -1), fill(n_nodes, 0);
     P.sh_ptr.push_back(0);
     for (int n = 0; n < n_nodes; n++) {
-        if (tiles_of[n] != 1) {
+        if (tiles_of[n] > 1) {
             shidx[n] = (int)P.sh_gid.size();
             P.sh_gid.push_back(n);
             P.sh_ptr.push_back(P.sh_ptr.back() + tiles_of[n]);
@@ -183,7 +203,7 @@ static inline TilePlan plan_tiles(int n_nodes, const double* rest, const int32_t
     }
     P.n_slots = P.sh_ptr.back();
     P.ln_pslot.assign(P.ln_gid.size(), -1);
-    for (int ti = 0; ti < T; ti++) {
+    for (int ti = 0; ti < TT; ti++) {
         for (int j = 0; j < P.t_lncount[ti]; j++) {
             const int li = P.t_lnbegin[ti] + j;
             const int n = P.ln_gid[li];

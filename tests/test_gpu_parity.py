@@ -668,3 +668,84 @@ def test_fp64_reference_dot_order_bitexact(which):
     st, so = eng.stats(sid), o.stats()
     assert st["iterations"] == so["iterations"] and st["end_reason"] == so["endcond"]
     eng.close()
+
+
+# ------------------------------------------------------------------------------------------
+# Error paths and edge cases (the C ABI's status codes)
+# ------------------------------------------------------------------------------------------
+def test_error_codes():
+    """Invalid input is refused with the documented status and a message, nothing faults:
+    out-of-range tetrahedron / fixed indices (EINVAL), a zero-volume tetrahedron at commit
+    (EDEGENERATE), calls out of order (ESTATE), duplicate or unknown scene ids (EINVAL)."""
+    capi = _capi()
+    print("case: error codes", flush=True)
+    c = beam_cfg0()
+    for prec in ("FP64", "FP32"):
+        eng = capi.Engine(0, getattr(capi, prec))
+        bad = c["tets"].copy()
+        bad[3, 2] = len(c["pos"])
+        with pytest.raises(capi.SofaAmdError) as ei:
+            eng.add_scene(c["pos"], bad, [c["E"]], [c["nu"]], c["fixed"])
+        assert ei.value.code == capi.EINVAL
+        sid = add_gpu_scene(eng, c)
+        with pytest.raises(capi.SofaAmdError) as ei:
+            eng.step([sid], 1)                              # before commit
+        assert ei.value.code == capi.ESTATE
+        eng.commit()
+        with pytest.raises(capi.SofaAmdError) as ei:
+            eng.step([sid, sid], 1)
+        assert ei.value.code == capi.EINVAL
+        with pytest.raises(capi.SofaAmdError) as ei:
+            eng.step([sid + 7], 1)
+        assert ei.value.code == capi.EINVAL
+        with pytest.raises(capi.SofaAmdError) as ei:
+            eng.commit()                                     # twice
+        assert ei.value.code == capi.ESTATE
+        eng.close()
+        # a flat tetrahedron: its four vertices in one plane
+        eng = capi.Engine(0, getattr(capi, prec))
+        pos = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [1, 1, 0], [0, 0, 1]], dtype=np.float64)
+        tets = np.array([[0, 1, 2, 4], [0, 1, 2, 3]], dtype=np.int32)
+        eng.add_scene(pos, tets, [1e5], [0.3], [0])
+        with pytest.raises(capi.SofaAmdError) as ei:
+            eng.commit()
+        assert ei.value.code == capi.EDEGENERATE and "degenerate" in str(ei.value)
+        eng.close()
+
+
+@pytest.mark.parametrize("prec", ["FP64", "FP32", "FP64_TILED"])
+def test_nodes_without_elements(prec):
+    """A scene of free nodes (no tetrahedra): gravity only, the implicit step reduces to
+    v += h g, x += h v, as the oracle computes it; a scene with a single tetrahedron next to it in
+    the same launch is unaffected."""
+    capi = _capi()
+    print(f"case: element-free scene {prec}", flush=True)
+    pos = np.array([[0.0, 0.0, 0.0], [1.0, 0.0, 0.0], [0.0, 2.0, 1.0]])
+    eng = capi.Engine(0, getattr(capi, prec))
+    s0 = eng.add_scene(pos, np.zeros((0, 4), np.int32), [1e5], [0.3], np.array([0], np.int32),
+                       mass_kind=capi.MASS_UNIFORM, mass_value=2.0, gravity=(0.0, -9.81, 0.0), dt=0.01,
+                       rayleigh_stiffness=0.0, rayleigh_mass=0.0, cg_iterations=10, cg_tolerance=1e-12,
+                       cg_threshold=1e-20)
+    c = beam_cfg0()
+    s1 = add_gpu_scene(eng, c)
+    eng.commit()
+    eng.step([s0, s1], 10)
+    x, v, _ = eng.read(s0)
+    from oracle import OracleScene
+    o = OracleScene(pos)
+    o.set_gravity((0.0, -9.81, 0.0))
+    o.set_fem(np.zeros((0, 4), np.int32), [1e5], [0.3], "large")
+    o.set_uniform_mass(vertex_mass=2.0)
+    o.set_fixed([0])
+    o.set_solver(0.0, 0.0, 10, 1e-12, 1e-20)
+    o.step(0.01, 10)
+    tol = 1e-6 if prec == "FP32" else 1e-13
+    assert np.max(np.abs(x - o.get("x"))) <= tol and np.max(np.abs(v - o.get("v"))) <= tol
+    assert np.all(x[0] == pos[0])                            # the fixed node
+    alone = capi.Engine(0, getattr(capi, prec))
+    a1 = add_gpu_scene(alone, c)
+    alone.commit()
+    alone.step([a1], 10)
+    assert np.array_equal(alone.read(a1)[0], eng.read(s1)[0])
+    alone.close()
+    eng.close()
