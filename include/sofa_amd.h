@@ -176,6 +176,17 @@ int sofa_amd_engine_set_graphs(sofa_amd_engine eng, int enable);
  * scene; costs one dependent add per node and dot. */
 int sofa_amd_engine_set_dot_order(sofa_amd_engine eng, int order);
 
+/* Tiled engines: the CG operator kernel's variant -- threads per element and how much element
+ * geometry it recomputes from positions: 0: 1 thread per element (512 per tile), rotation and
+ * rest coordinates recomputed (the default); 1: 512 threads, rotation only; 2: 256 threads (2
+ * elements each), rotation and rest coordinates; 3: 256 threads, rotation only.  Variants differ
+ * in rounding only (the order of block sums, geometry recomputed in the engine's precision).
+ * sofa_amd_engine_autotune times all of them on the committed scenes (`reps` launches each, one
+ * launch over every scene) and keeps the fastest; it must run before the first step.  `chosen`
+ * and `us4` (the 4 timings in microseconds per launch) may be NULL.  No reference counterpart. */
+int sofa_amd_engine_autotune(sofa_amd_engine eng, int reps, int* chosen, float* us4);
+int sofa_amd_engine_set_apply_variant(sofa_amd_engine eng, int variant);
+
 /* Blocks until the engine's work is done. */
 int sofa_amd_engine_sync(sofa_amd_engine eng);
 

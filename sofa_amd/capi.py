@@ -29,7 +29,7 @@ EXPORTS = [
     "sofa_amd_engine_set_graphs", "sofa_amd_scene_set_mass_density", "sofa_amd_scene_add_oscillators",
     "sofa_amd_scene_set_refinement", "sofa_amd_engine_reserve", "sofa_amd_scene_commit", "sofa_amd_scene_release",
     "sofa_amd_engine_usage", "sofa_amd_scene_set_fix_all", "sofa_amd_engine_set_dot_order",
-    "sofa_amd_scene_prepare",
+    "sofa_amd_scene_prepare", "sofa_amd_engine_autotune", "sofa_amd_engine_set_apply_variant",
 ]
 
 
@@ -96,6 +96,8 @@ def lib():
         "sofa_amd_scene_set_fix_all": (c_int, [_vp, c_int, c_int]),
         "sofa_amd_engine_set_dot_order": (c_int, [_vp, c_int]),
         "sofa_amd_scene_prepare": (c_int, [_vp, c_int]),
+        "sofa_amd_engine_autotune": (c_int, [_vp, c_int, ctypes.POINTER(c_int), ctypes.POINTER(ctypes.c_float)]),
+        "sofa_amd_engine_set_apply_variant": (c_int, [_vp, c_int]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -237,6 +239,17 @@ class Engine:
         """FP64 engines: dot products in the reference's sequential node order (bit-exact
         trajectories) instead of the parallel blocked order."""
         check(lib().sofa_amd_engine_set_dot_order(self._h, 1 if sequential else 0))
+
+    def autotune(self, reps=5):
+        """Tiled engines, before the first step: time the operator variants on the committed
+        scenes and keep the fastest.  Returns (chosen variant, microseconds per launch of each)."""
+        ch = ctypes.c_int()
+        us = (ctypes.c_float * 4)()
+        check(lib().sofa_amd_engine_autotune(self._h, int(reps), ctypes.byref(ch), us))
+        return ch.value, [float(u) for u in us]
+
+    def set_apply_variant(self, variant):
+        check(lib().sofa_amd_engine_set_apply_variant(self._h, int(variant)))
 
     def set_graphs(self, enable=True):
         check(lib().sofa_amd_engine_set_graphs(self._h, 1 if enable else 0))

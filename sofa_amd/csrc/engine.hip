@@ -263,6 +263,9 @@ struct EngineBase {
     virtual void usage(int64_t* out22) const = 0;
     bool live_mode = false;           // reserved: scenes are committed and released one by one
     bool seq_dots = false;            // exact path: dot products in the reference's sequential order
+    bool stepped = false;             // a step has been enqueued
+    virtual int autotune(int reps, hipStream_t s, float* us_out) = 0;
+    virtual void set_apply_variant(int v) = 0;
 };
 
 template <typename T>
@@ -1088,6 +1091,64 @@ struct EngineT : EngineBase {
         k_finalize<<<(n + 63) / 64, 64, 0, s>>>(cur->scenes, n, inst);
     }
 
+    // the tile operator's launch shape and how much geometry it recomputes (results of the
+    // variants differ in rounding only: block sums and geometry recomputed in R instead of
+    // rounded from the host's double); picked by sofa_amd_engine_autotune or set explicitly
+    static constexpr int kApplyVariants = 4;
+    static int variant_threads(int v) { return v < 2 ? 512 : 256; }
+    static int variant_geometry(int v) { return (v & 1) ? 1 : 2; }
+    int apply_variant = 0;
+    void launch_apply(int na, hipStream_t s, const TileArgs<R, NR>& ta, int it, const VT* rr, VT* pp, VT* qq, VT* sp,
+                      const double* pin, double* pd, int pin_nodes) {
+        switch (apply_variant) {
+        case 1: k_tile_apply<R, NR, VT, 512, 1><<<na, 512, 0, s>>>(ta, it, rr, pp, qq, sp, pin, pd, pin_nodes, x, x0); break;
+        case 2: k_tile_apply<R, NR, VT, 256, 2><<<na, 256, 0, s>>>(ta, it, rr, pp, qq, sp, pin, pd, pin_nodes, x, x0); break;
+        case 3: k_tile_apply<R, NR, VT, 256, 1><<<na, 256, 0, s>>>(ta, it, rr, pp, qq, sp, pin, pd, pin_nodes, x, x0); break;
+        default: k_tile_apply<R, NR, VT, 512, 2><<<na, 512, 0, s>>>(ta, it, rr, pp, qq, sp, pin, pd, pin_nodes, x, x0); break;
+        }
+    }
+    // times every variant on the engine's scenes (all of them in one launch, `reps` launches
+    // each, events on `s`) and keeps the fastest; before the first step only (the timing runs
+    // the operator on the CG buffers)
+    int autotune(int reps, hipStream_t s, float* us_out) override {
+        if (!tiled) throw std::invalid_argument("autotune needs a tiled engine");
+        if (stepped) throw std::logic_error("autotune before the first step");
+        std::vector<int32_t> ids;
+        for (int i = 0; i < (int)scenes.size(); i++) if (placed(i)) ids.push_back(i);
+        if (ids.empty()) throw std::invalid_argument("no committed scene");
+        prepare_set(ids.data(), (int)ids.size(), s);
+        const TileArgs<R, NR> ta = tile_args();
+        hipEvent_t e0, e1;
+        HIPCHK(hipEventCreate(&e0));
+        HIPCHK(hipEventCreate(&e1));
+        int best = 0;
+        float best_us = 0;
+        for (int v = 0; v < kApplyVariants; v++) {
+            apply_variant = v;
+            for (int i = 0; i < 2; i++) launch_apply(cur->na, s, ta, 2, r, p, q, sp0, prho, pden, 0);
+            HIPCHK(hipEventRecord(e0, s));
+            for (int i = 0; i < reps; i++) launch_apply(cur->na, s, ta, 2, r, p, q, sp0, prho, pden, 0);
+            HIPCHK(hipEventRecord(e1, s));
+            HIPCHK(hipEventSynchronize(e1));
+            float ms = 0;
+            HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+            const float us = 1e3f * ms / (float)std::max(1, reps);
+            if (us_out) us_out[v] = us;
+            if (v == 0 || us < best_us) { best = v; best_us = us; }
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        apply_variant = best;
+        reset_graph();
+        return best;
+    }
+    void set_apply_variant(int v) override {
+        if (!tiled || v < 0 || v >= kApplyVariants) throw std::invalid_argument("invalid operator variant");
+        HIPCHK(hipDeviceSynchronize());
+        apply_variant = v;
+        reset_graph();
+    }
+
     TileArgs<R, NR> tile_args() const {
         TileArgs<R, NR> ta;
         ta.tiles = (const TileDesc*)cur->a;
@@ -1112,7 +1173,7 @@ struct EngineT : EngineBase {
         if (cur->nb > 0) k_fin_rhs<VT><<<cur->nb, kBlock, 0, s>>>(fb, inst, ranges_d, sn, mass, fixed, v, sp0, sp1, f, b, r, xcg, pbb);
         for (int it = 1; it <= cur->maxit; it++) {
             if (cur->na > 0)
-                k_tile_apply<R, NR, VT, kTileThreads, kApplyGeometry><<<cur->na, kTileThreads, 0, s>>>(ta, it, r, p, q, sp0, it == 1 ? pbb : prho, pden, 0, x, x0);
+                launch_apply(cur->na, s, ta, it, r, p, q, sp0, it == 1 ? pbb : prho, pden, 0);
             if (cur->nc > 0)
                 k_update<VT><<<cur->nc, kBlock, 0, s>>>(nb, inst, ranges_d, it, sn, sp0, mass, fixed, p, q, xcg, r, pden, prho);
         }
@@ -1150,7 +1211,7 @@ struct EngineT : EngineBase {
         for (int sw = 1; sw <= cur->maxsw + 1; sw++) {
             if (sw > 1) {   // q = A x, FP64
                 if (cur->na > 0)
-                    k_tile_apply<R, NR, VT, kTileThreads, kApplyGeometry><<<cur->na, kTileThreads, 0, s>>>(ta, 0, xcg, nullptr, q, sp0, nullptr, pden, 0, x, x0);
+                    launch_apply(cur->na, s, ta, 0, xcg, nullptr, q, sp0, nullptr, pden, 0);
                 if (cur->nb > 0)
                     k_fin_apply<VT><<<cur->nb, kBlock, 0, s>>>(fb, inst, ranges_d, sn, mass, fixed, 0, xcg, nullptr, q, sp0, pden);
             }
@@ -1176,6 +1237,7 @@ struct EngineT : EngineBase {
 
     void step(const int32_t* ids, int n, int nsteps, hipStream_t s) override {
         prepare_set(ids, n, s);
+        if (nsteps > 0) stepped = true;
         if (!use_graphs) {
             for (int st = 0; st < nsteps; st++) launch_step(n, s);
             HIPCHK(hipGetLastError());
@@ -1209,7 +1271,7 @@ struct EngineT : EngineBase {
         prepare_set(ids.data(), (int)ids.size(), s);
         const TileArgs<R, NR> ta = tile_args();
         for (int i = 0; i < reps; i++)
-            if (cur->na > 0) k_tile_apply<R, NR, VT, kTileThreads, kApplyGeometry><<<cur->na, kTileThreads, 0, s>>>(ta, 2, r, p, q, sp0, prho, pden, 0, x, x0);
+            if (cur->na > 0) launch_apply(cur->na, s, ta, 2, r, p, q, sp0, prho, pden, 0);
         HIPCHK(hipGetLastError());
     }
     // tiles, tile-local nodes, nodes finished by the finalize kernel, shared partial slots,
@@ -1778,6 +1840,27 @@ int sofa_amd_engine_set_dot_order(sofa_amd_engine eng, int order) {
     HIPCHK(hipDeviceSynchronize());
     eng->impl->seq_dots = order == 1;
     eng->impl->reset_graph();
+    return SOFA_AMD_OK;
+    GUARD_END
+}
+
+int sofa_amd_engine_autotune(sofa_amd_engine eng, int reps, int* chosen, float* us4) {
+    GUARD_BEGIN
+    if (!eng || !eng->impl || reps < 1) return fail(SOFA_AMD_EINVAL, "invalid argument");
+    if (!eng->impl->committed) return fail(SOFA_AMD_ESTATE, "engine not committed");
+    if (eng->impl->stepped) return fail(SOFA_AMD_ESTATE, "autotune before the first step");
+    HIPCHK(hipSetDevice(eng->impl->device));
+    const int v = eng->impl->autotune(reps, eng->impl->stream, us4);
+    if (chosen) *chosen = v;
+    return SOFA_AMD_OK;
+    GUARD_END
+}
+
+int sofa_amd_engine_set_apply_variant(sofa_amd_engine eng, int variant) {
+    GUARD_BEGIN
+    if (!eng || !eng->impl) return fail(SOFA_AMD_EINVAL, "null engine");
+    HIPCHK(hipSetDevice(eng->impl->device));
+    eng->impl->set_apply_variant(variant);
     return SOFA_AMD_OK;
     GUARD_END
 }

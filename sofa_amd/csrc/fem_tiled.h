@@ -34,11 +34,8 @@ constexpr int kNodesPerThread = SOFA_AMD_NODES_PER_THREAD;   // node kernels: 25
 #ifndef SOFA_AMD_TILE_THREADS
 #define SOFA_AMD_TILE_THREADS 512
 #endif
-constexpr int kTileThreads = SOFA_AMD_TILE_THREADS;   // threads of the tile apply kernel
-#ifndef SOFA_AMD_APPLY_GEOMETRY
-#define SOFA_AMD_APPLY_GEOMETRY 2
-#endif
-constexpr int kApplyGeometry = SOFA_AMD_APPLY_GEOMETRY;   // geometry the tile apply recomputes
+constexpr int kTileThreads = SOFA_AMD_TILE_THREADS;   // threads of the MIXED inner operator kernel
+
 
 struct TileDesc {
     int scene;

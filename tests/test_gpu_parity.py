@@ -749,3 +749,36 @@ def test_nodes_without_elements(prec):
     assert np.array_equal(alone.read(a1)[0], eng.read(s1)[0])
     alone.close()
     eng.close()
+
+
+def test_operator_variants_and_autotune():
+    """Every variant of the tiled operator (threads per element, geometry recomputed) keeps FP64
+    tiled trajectories within 1e-10 of the oracle; the autotuner picks one before the first step
+    and refuses to run after it."""
+    capi = _capi()
+    print("case: operator variants / autotune", flush=True)
+    c = beam_cfg0()
+    o = make_oracle(c, 0)
+    o.step(c["dt"], 30)
+    xo = o.get("x")
+    for v in range(4):
+        eng = capi.Engine(0, capi.FP64_TILED)
+        sid = add_gpu_scene(eng, c)
+        eng.commit()
+        eng.set_apply_variant(v)
+        eng.step([sid], 30)
+        assert rel_err(eng.read(sid)[0], xo) <= 1e-10, v
+        eng.close()
+    eng = capi.Engine(0, capi.FP32)
+    ids = [add_gpu_scene(eng, c), add_gpu_scene(eng, beam_medium(3e5))]
+    eng.commit()
+    chosen, us = eng.autotune(3)
+    assert 0 <= chosen < 4 and all(u > 0 for u in us) and us[chosen] == min(us)
+    eng.step(ids, 20)
+    o20 = make_oracle(c, 0)
+    o20.step(c["dt"], 20)
+    assert rel_err(eng.read(ids[0])[0], o20.get("x")) <= 1e-5
+    with pytest.raises(capi.SofaAmdError) as ei:
+        eng.autotune(3)
+    assert ei.value.code == capi.ESTATE
+    eng.close()
