@@ -22,6 +22,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <float.h>
+#include <type_traits>
 
 namespace sofa_amd {
 
@@ -91,6 +92,22 @@ __device__ __forceinline__ void normalize3(R& x, R& y, R& z) {
     n2 += z * z;
     const R n = sqrt(n2);
     if (n > Eps<R>::v) { x /= n; y /= n; z /= n; }
+}
+
+// FP32: one hardware reciprocal square root (v_rsq_f32, 1 ulp) and three multiplies instead of a
+// correctly rounded square root and three divisions.  The FP32 engine's error budget is far above
+// the ulp this costs; every FP32 kernel calls this same function, so a rotation recomputed from
+// the same positions still has the same bits.  The FP64 engines keep the divisions.
+template <>
+__device__ __forceinline__ void normalize3<float>(float& x, float& y, float& z) {
+#pragma clang fp contract(off)
+    float n2 = x * x;
+    n2 += y * y;
+    n2 += z * z;
+    if (n2 > Eps<float>::v * Eps<float>::v) {
+        const float s = __builtin_amdgcn_rsqf(n2);
+        x *= s; y *= s; z *= s;
+    }
 }
 
 template <typename T> __device__ __forceinline__ void load3(const T* __restrict__ v, int n, T& a, T& b, T& c) {
@@ -279,10 +296,17 @@ __device__ __forceinline__ void stiffness_from_volume(const R p0[4][3], const do
     R d = x0 * c0;
     d += x1 * c1;
     d += x2 * c2;
-    const R vol = fabs(d / (R)6);
-    const R div = vol * (R)36;
+    if constexpr (std::is_same<R, float>::value) {
+        // FP32: 36 V = 6 |d|, one hardware reciprocal (1 ulp) for the three entries
+        const float inv = __builtin_amdgcn_rcpf(fabsf(d) * 6.0f);
 #pragma unroll
-    for (int i = 0; i < 3; i++) k[i] = (R)ks[i] / div;
+        for (int i = 0; i < 3; i++) k[i] = (float)ks[i] * inv;
+    } else {
+        const R vol = fabs(d / (R)6);
+        const R div = vol * (R)36;
+#pragma unroll
+        for (int i = 0; i < 3; i++) k[i] = (R)ks[i] / div;
+    }
 }
 
 template <typename R, int NR>

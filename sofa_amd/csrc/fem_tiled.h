@@ -27,8 +27,10 @@ namespace sofa_amd {
 #endif
 constexpr int kTileElems = SOFA_AMD_TILE_ELEMS;   // max elements per tile
 constexpr int kTileNodes = SOFA_AMD_TILE_ELEMS;   // max distinct nodes per tile
+// incidences are uint16 contribution slots (row 3 v + c of 12 rows of kTileElems)
+static_assert(12 * kTileElems <= 65536, "contribution slots must fit in 16 bits");
 #ifndef SOFA_AMD_NODES_PER_THREAD
-#define SOFA_AMD_NODES_PER_THREAD 4
+#define SOFA_AMD_NODES_PER_THREAD 2
 #endif
 constexpr int kNodesPerThread = SOFA_AMD_NODES_PER_THREAD;   // node kernels: 256 threads x this
 #ifndef SOFA_AMD_TILE_THREADS
@@ -207,7 +209,7 @@ __global__ __launch_bounds__(kBlock) void k_tile_rhs(TileArgs<R, NR> a, const VT
             for (int i = 0; i < 9; i++) a.rot_lo[i * a.ev.E + e] = (float)rot[i];
         }
 #pragma unroll
-        for (int i = 0; i < 12; i++) cf[12 * k + i] = c[i];
+        for (int i = 0; i < 12; i++) cf[i * kTileElems + k] = c[i];
         if (in.k_rhs != 0.0) {
             elem_dforce<R, NR>(a.ev, e, rot, u, krhs, c);
         } else {
@@ -216,7 +218,7 @@ __global__ __launch_bounds__(kBlock) void k_tile_rhs(TileArgs<R, NR> a, const VT
         }
         if (in.mass_kind == kMassMatrix && in.m_rhs != 0.0) add_edge_mass((R)in.m_rhs * a.emass[e], u, c);
 #pragma unroll
-        for (int i = 0; i < 12; i++) cv[12 * k + i] = c[i];
+        for (int i = 0; i < 12; i++) cv[i * kTileElems + k] = c[i];
     }
     __syncthreads();
     double bb = 0.0;
@@ -232,7 +234,7 @@ __global__ __launch_bounds__(kBlock) void k_tile_rhs(TileArgs<R, NR> a, const VT
             for (int s = s0; s < s1; s++) {
                 const int sl = a.tn.inc[s];
 #pragma unroll
-                for (int i = 0; i < 3; i++) fn[i] += (VT)cf[3 * sl + i];
+                for (int i = 0; i < 3; i++) fn[i] += (VT)cf[sl + i * kTileElems];
             }
             st3(f, g, fn);
             VT bn[3] = {fn[0], fn[1], fn[2]};
@@ -243,7 +245,7 @@ __global__ __launch_bounds__(kBlock) void k_tile_rhs(TileArgs<R, NR> a, const VT
             for (int s = s0; s < s1; s++) {
                 const int sl = a.tn.inc[s];
 #pragma unroll
-                for (int i = 0; i < 3; i++) bn[i] -= (VT)cv[3 * sl + i];
+                for (int i = 0; i < 3; i++) bn[i] -= (VT)cv[sl + i * kTileElems];
             }
             bn[0] *= h; bn[1] *= h; bn[2] *= h;
             if (a.fixed[g]) { bn[0] = 0; bn[1] = 0; bn[2] = 0; }
@@ -257,7 +259,7 @@ __global__ __launch_bounds__(kBlock) void k_tile_rhs(TileArgs<R, NR> a, const VT
             for (int s = s0; s < s1; s++) {
                 const int sl = a.tn.inc[s];
 #pragma unroll
-                for (int i = 0; i < 3; i++) { pf[i] += (VT)cf[3 * sl + i]; pv[i] += (VT)cv[3 * sl + i]; }
+                for (int i = 0; i < 3; i++) { pf[i] += (VT)cf[sl + i * kTileElems]; pv[i] += (VT)cv[sl + i * kTileElems]; }
             }
             st3(spf, ps, pf);
             st3(spv, ps, pv);
@@ -504,7 +506,7 @@ __global__ __launch_bounds__(TB) void k_tile_apply(TileArgs<R, NR> a, int iter, 
             elem_dforce_r<R, NR>(rot[m], rin[m], kk[m], u, kcg, c);
             if (consistent) add_edge_mass(mcg * em[m], u, c);
 #pragma unroll
-            for (int i = 0; i < 12; i++) lc[12 * k + i] = c[i];
+            for (int i = 0; i < 12; i++) lc[i * kTileElems + k] = c[i];
         }
     }
     __syncthreads();
@@ -523,7 +525,7 @@ __global__ __launch_bounds__(TB) void k_tile_apply(TileArgs<R, NR> a, int iter, 
                 for (int s = s0; s < s1; s++) {
                     const int sl = linc[s];
 #pragma unroll
-                    for (int i = 0; i < 3; i++) qn[i] -= (VT)lc[3 * sl + i];
+                    for (int i = 0; i < 3; i++) qn[i] -= (VT)lc[sl + i * kTileElems];
                 }
                 if (fx[m]) { qn[0] = 0; qn[1] = 0; qn[2] = 0; }
                 st3(q, gid[m], qn);
@@ -533,7 +535,7 @@ __global__ __launch_bounds__(TB) void k_tile_apply(TileArgs<R, NR> a, int iter, 
                 for (int s = s0; s < s1; s++) {
                     const int sl = linc[s];
 #pragma unroll
-                    for (int i = 0; i < 3; i++) acc[i] -= (VT)lc[3 * sl + i];
+                    for (int i = 0; i < 3; i++) acc[i] -= (VT)lc[sl + i * kTileElems];
                 }
                 st3(sp, psl[m] & (kSlotOwner - 1), acc);
                 // p.q is linear in q: a shared node's term is split over its tiles (p . this

@@ -3,9 +3,12 @@
 // Elements are ordered by the Morton code of their rest centroid in element units (each axis
 // scaled by the median element extent along it; ties by element id) and cut
 // greedily into tiles of at most `max_elems` elements and `max_nodes` distinct nodes.  For
-// each tile-local node the incidence list (local element slot * 4 + vertex) is sorted by
-// ORIGINAL element id: the order in which the reference's element loop accumulates into that
-// node (TetrahedronFEMForceField.inl:1630-1634).
+// each tile-local node the incidence list is sorted by ORIGINAL element id: the order in which
+// the reference's element loop accumulates into that node (TetrahedronFEMForceField.inl:
+// 1630-1634).  An incidence is the LDS slot of the element's contribution to the node: the
+// kernels keep the 12 contribution components of a tile's elements as 12 rows of max_elems
+// (row 3 v + c, column = tile-local element), so vertex v of local element k is the slot
+// 3 v max_elems + k and its component c sits c max_elems further.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -25,7 +28,7 @@ struct TilePlan {
     std::vector<uint2> ltet;        // per permuted element: 4 x uint16 tile-local node index
     std::vector<int> ln_gid;        // tile-local node -> scene-local node id
     std::vector<int> ln_inc_ptr;    // size(ln_gid) + 1
-    std::vector<uint16_t> ln_inc;   // local slot * 4 + vertex
+    std::vector<uint16_t> ln_inc;   // contribution slot: 3 max_elems vertex + local element
     std::vector<int> ln_pslot;      // -1 interior, else scene-local shared-partial slot
     std::vector<int> sh_gid;        // nodes finished by the finalize kernel (ascending)
     std::vector<int> sh_ptr;        // size(sh_gid) + 1
@@ -137,8 +140,7 @@ static inline TilePlan plan_tiles(int n_nodes, const double* rest, const int32_t
     P.t_lnbegin.resize(T);
     P.t_lncount.resize(T);
     P.ln_inc_ptr.push_back(0);
-    std::vector<std::pair<int, int>> incl;   // (original element, slot*4+v)
-    for (int ti = 0; ti < T; ti++) {
+        for (int ti = 0; ti < T; ti++) {
         const int e0 = P.t_ebegin[ti], ec = P.t_ecount[ti];
         std::vector<int> nodes;
         nodes.reserve(4 * ec);
@@ -146,6 +148,23 @@ static inline TilePlan plan_tiles(int n_nodes, const double* rest, const int32_t
             for (int v = 0; v < 4; v++) nodes.push_back(tets[4 * P.perm[e0 + k] + v]);
         std::sort(nodes.begin(), nodes.end());
         nodes.erase(std::unique(nodes.begin(), nodes.end()), nodes.end());
+        // incidence count of each node in this tile, then the nodes by descending count (ties
+        // by node id): the per-node loops over incident elements run one node per thread, and
+        // a wave of nodes of like counts runs about as many iterations as its mean, not its max
+        std::vector<int> cnt(nodes.size(), 0);
+        for (int k = 0; k < ec; k++)
+            for (int v = 0; v < 4; v++) {
+                const int n = tets[4 * P.perm[e0 + k] + v];
+                cnt[std::lower_bound(nodes.begin(), nodes.end(), n) - nodes.begin()]++;
+            }
+        {
+            std::vector<int> order(nodes.size());
+            std::iota(order.begin(), order.end(), 0);
+            std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cnt[a] > cnt[b]; });
+            std::vector<int> by_count(nodes.size());
+            for (size_t j = 0; j < order.size(); j++) by_count[j] = nodes[order[j]];
+            nodes.swap(by_count);
+        }
         if ((int)nodes.size() > max_nodes) throw std::logic_error("tile node budget exceeded");
         P.t_lnbegin[ti] = (int)P.ln_gid.size();
         P.t_lncount[ti] = (int)nodes.size();
@@ -159,7 +178,7 @@ static inline TilePlan plan_tiles(int n_nodes, const double* rest, const int32_t
             uint32_t l[4];
             for (int v = 0; v < 4; v++) {
                 l[v] = (uint32_t)loc[tets[4 * e + v]];
-                per[l[v]].push_back({e, k * 4 + v});
+                per[l[v]].push_back({e, v * 3 * max_elems + k});
             }
             P.ltet[e0 + k] = make_uint2(l[0] | (l[1] << 16), l[2] | (l[3] << 16));
         }
