@@ -1169,7 +1169,7 @@ struct EngineT : EngineBase {
         const FinBlk* fb = (const FinBlk*)cur->b;
         const NodeBlk* nb = (const NodeBlk*)cur->c;
         const SharedNodes sn{sh_gid, sh_ptr, sh_of};
-        if (cur->na > 0) k_tile_rhs<R, NR, VT><<<cur->na, kBlock, 0, s>>>(ta, x, v, f, b, r, xcg, sp0, sp1, pbb);
+        if (cur->na > 0) k_tile_rhs<R, NR, VT, 512><<<cur->na, 512, 0, s>>>(ta, x, v, f, b, r, xcg, sp0, sp1, pbb);
         if (cur->nb > 0) k_fin_rhs<VT><<<cur->nb, kBlock, 0, s>>>(fb, inst, ranges_d, sn, mass, fixed, v, sp0, sp1, f, b, r, xcg, pbb);
         for (int it = 1; it <= cur->maxit; it++) {
             if (cur->na > 0)
@@ -1206,7 +1206,7 @@ struct EngineT : EngineBase {
         ic.sn = sn;
         ic.r = r32; ic.p = p32; ic.q = q32; ic.d = d32; ic.sp = sp32;
         ic.pden = pden32; ic.prho = prho32;
-        if (cur->na > 0) k_tile_rhs<R, NR, VT><<<cur->na, kBlock, 0, s>>>(ta, x, v, f, b, r, xcg, sp0, sp1, pbb);
+        if (cur->na > 0) k_tile_rhs<R, NR, VT, 512><<<cur->na, 512, 0, s>>>(ta, x, v, f, b, r, xcg, sp0, sp1, pbb);
         if (cur->nb > 0) k_fin_rhs<VT><<<cur->nb, kBlock, 0, s>>>(fb, inst, ranges_d, sn, mass, fixed, v, sp0, sp1, f, b, r, xcg, pbb);
         for (int sw = 1; sw <= cur->maxsw + 1; sw++) {
             if (sw > 1) {   // q = A x, FP64

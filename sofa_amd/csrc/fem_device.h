@@ -309,9 +309,12 @@ __device__ __forceinline__ void stiffness_from_volume(const R p0[4][3], const do
     }
 }
 
+// accumulateForceLarge's element part once the rest shape is known: rc the rest coordinates
+// b, c, d in the rest frame (zeros included), g their strain-displacement groups, k3 the
+// stiffness entries
 template <typename R, int NR>
-__device__ __forceinline__ void elem_force_p(const ElemView<R, NR>& ev, int64_t e, const R p[4][3], R rot[9],
-                                             R c[12], R g[12]) {
+__device__ __forceinline__ void elem_force_core(const R p[4][3], const R rc[9], const R g[12], const R k3[3],
+                                                R rot[9], R c[12]) {
     rotation_large(p[0], p[1], p[2], rot);
     // R_0_2 = transpose(rot)
     const R r02[3][3] = {{rot[0], rot[3], rot[6]}, {rot[1], rot[4], rot[7]}, {rot[2], rot[5], rot[8]}};
@@ -327,8 +330,6 @@ __device__ __forceinline__ void elem_force_p(const ElemView<R, NR>& ev, int64_t 
     df[3][0] -= df[0][0];
     df[3][1] -= df[0][1];
     df[3][2] -= df[0][2];
-    R rc[9];
-    jgroups<R, NR>(ev, e, g, rc);
     R D[12];
     D[0] = 0; D[1] = 0; D[2] = 0;
     D[3] = rc[0] - df[1][0];
@@ -341,10 +342,18 @@ __device__ __forceinline__ void elem_force_p(const ElemView<R, NR>& ev, int64_t 
     D[11] = rc[8] - df[3][2];
     R JtD[6], KJ[6], F[12];
     jt_mul<R, NR>(g, D, JtD);
-    const R k3[3] = {ev.kmat[e], ev.kmat[ev.E + e], ev.kmat[2 * ev.E + e]};
     k_mul(k3, JtD, KJ);
     j_mul(g, KJ, F);
     rot_apply(rot, F, c);
+}
+
+template <typename R, int NR>
+__device__ __forceinline__ void elem_force_p(const ElemView<R, NR>& ev, int64_t e, const R p[4][3], R rot[9],
+                                             R c[12], R g[12]) {
+    R rc[9];
+    jgroups<R, NR>(ev, e, g, rc);
+    const R k3[3] = {ev.kmat[e], ev.kmat[ev.E + e], ev.kmat[2 * ev.E + e]};
+    elem_force_core<R, NR>(p, rc, g, k3, rot, c);
 }
 
 template <typename R, int NR, typename VT>
@@ -422,6 +431,21 @@ __device__ __forceinline__ void jgroups_r(const R* rin, R g[12]) {
         g[10] = 0;
         g[11] = b0 * c1;
     }
+}
+
+// the element force with the element record in registers (rin: NR rest coordinates)
+template <typename R, int NR>
+__device__ __forceinline__ void elem_force_r(const R p[4][3], const R* rin, const R k3[3], R rot[9], R c[12]) {
+    R g[12], rc[9];
+    jgroups_r<R, NR>(rin, g);
+    if constexpr (NR == 9) {
+#pragma unroll
+        for (int k = 0; k < 9; k++) rc[k] = rin[k];
+    } else {
+        rc[0] = rin[0]; rc[1] = 0; rc[2] = 0; rc[3] = rin[1]; rc[4] = rin[2]; rc[5] = 0;
+        rc[6] = rin[3]; rc[7] = rin[4]; rc[8] = rin[5];
+    }
+    elem_force_core<R, NR>(p, rc, g, k3, rot, c);
 }
 
 // applyStiffnessCorotational with the element record in registers

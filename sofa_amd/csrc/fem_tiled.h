@@ -76,13 +76,43 @@ struct Ranges {
     int b0, nb;    // node-block partials (rho buffer)
 };
 
-// fixed-order block sum of the tiled kernels (TB threads): butterfly within each 64-lane wave
-// (both lanes of a pair add the same two values, so every lane ends with the same bits), then
-// the wave sums added in wave order; sh holds TB / 64 doubles
+// one DPP step of a 64-lane sum: v of the lane CTRL names in the rows ROWS enables, 0.0 where
+// there is none (a double moved as its two 32-bit halves)
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double dpp_from(double v) {
+    const int lo = __double2loint(v), hi = __double2hiint(v);
+    return __hiloint2double(__builtin_amdgcn_update_dpp(0, hi, CTRL, ROWS, 0xf, true),
+                            __builtin_amdgcn_update_dpp(0, lo, CTRL, ROWS, 0xf, true));
+}
+
+// sum over the 64 lanes of a wave in a fixed order: an inclusive scan within each row of 16
+// lanes (row_shr 1, 2, 4, 8), then row 0's total into row 1 and row 2's into row 3
+// (row_bcast 15), then the first half's total into the second half (row_bcast 31); lane 63 then
+// holds the sum, and every lane reads it from there (the same bits in every lane)
+__device__ __forceinline__ double wave_sum(double v) {
+    v = v + dpp_from<0x111, 0xf>(v);
+    v = v + dpp_from<0x112, 0xf>(v);
+    v = v + dpp_from<0x114, 0xf>(v);
+    v = v + dpp_from<0x118, 0xf>(v);
+    v = v + dpp_from<0x142, 0xa>(v);
+    v = v + dpp_from<0x143, 0xc>(v);
+    return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), 63),
+                            __builtin_amdgcn_readlane(__double2loint(v), 63));
+}
+
+// every wave of the block: the sum of buf[a0 .. a0 + na) (lane-strided partial sums, then
+// wave_sum) -- no LDS and no barrier, and the same bits in every wave and every block
+__device__ __forceinline__ double wave_reduce(const double* __restrict__ buf, int a0, int na) {
+    double acc = 0.0;
+    for (int k = (int)(threadIdx.x & 63); k < na; k += 64) acc = acc + buf[a0 + k];
+    return wave_sum(acc);
+}
+
+// fixed-order block sum of the tiled kernels (TB threads): wave_sum within each wave, then the
+// wave sums added in wave order; sh holds TB / 64 doubles
 template <int TB>
 __device__ __forceinline__ double wave_block_sum(double v, double* sh) {
-#pragma unroll
-    for (int m = 32; m > 0; m >>= 1) v = v + __shfl_xor(v, m, 64);
+    v = wave_sum(v);
     __syncthreads();                     // a previous call's readers are done with sh
     if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
     __syncthreads();
@@ -162,110 +192,198 @@ __device__ __forceinline__ void add_edge_mass(R mf, const R u[12], R c[12]) {
     }
 }
 
+// acc (+|-)= the node's element contributions lc[slot + c kTileElems] in incidence-list order.
+// Four incidences per round: their slots are read together, then their values, so a round
+// waits on LDS twice instead of eight times; the adds keep the list order (same bits).
+template <bool SUB, typename R, typename VT>
+__device__ __forceinline__ void sum_incidences(const uint16_t* linc, int s0, int s1, const R* lc, VT acc[3]) {
+    int s = s0;
+    for (; s + 4 <= s1; s += 4) {
+        int sl[4];
+#pragma unroll
+        for (int t = 0; t < 4; t++) sl[t] = linc[s + t];
+        R v[4][3];
+#pragma unroll
+        for (int t = 0; t < 4; t++)
+#pragma unroll
+            for (int i = 0; i < 3; i++) v[t][i] = lc[sl[t] + i * kTileElems];
+#pragma unroll
+        for (int t = 0; t < 4; t++)
+#pragma unroll
+            for (int i = 0; i < 3; i++) {
+                if (SUB) acc[i] -= (VT)v[t][i];
+                else acc[i] += (VT)v[t][i];
+            }
+    }
+    for (; s < s1; s++) {
+        const int sl = linc[s];
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            if (SUB) acc[i] -= (VT)lc[sl + i * kTileElems];
+            else acc[i] += (VT)lc[sl + i * kTileElems];
+        }
+    }
+}
+
 __device__ __forceinline__ void unpack_ltet(uint2 w, int l[4]) {
     l[0] = (int)(w.x & 0xffffu); l[1] = (int)(w.x >> 16); l[2] = (int)(w.y & 0xffffu); l[3] = (int)(w.y >> 16);
 }
 
 // ---------------------------------------------------------------------------------------
-// RHS: rotations, element force, K v; interior nodes finished, shared nodes -> partials
+// RHS: rotations, element force, K v; interior nodes finished, shared nodes -> partials.
+// TB threads, one element (and at most one tile-local node) per thread for TB = 512.  One LDS
+// contribution buffer serves both element passes: the force pass, the node sums of f, then the
+// K v pass (element record and rotation kept in registers) and the node sums of b.
 // ---------------------------------------------------------------------------------------
-template <typename R, int NR, typename VT>
-__global__ __launch_bounds__(kBlock) void k_tile_rhs(TileArgs<R, NR> a, const VT* __restrict__ x,
-                                                      const VT* __restrict__ v, VT* __restrict__ f,
-                                                      VT* __restrict__ b, VT* __restrict__ r, VT* __restrict__ xcg,
-                                                      VT* __restrict__ spf, VT* __restrict__ spv,
-                                                      double* __restrict__ pbb) {
+template <typename R, int NR, typename VT, int TB = 512>
+__global__ __launch_bounds__(TB) void k_tile_rhs(TileArgs<R, NR> a, const VT* __restrict__ x,
+                                                  const VT* __restrict__ v, VT* __restrict__ f,
+                                                  VT* __restrict__ b, VT* __restrict__ r, VT* __restrict__ xcg,
+                                                  VT* __restrict__ spf, VT* __restrict__ spv,
+                                                  double* __restrict__ pbb) {
+    constexpr int EPT = kTileElems / TB;
+    constexpr int NPT = (kTileNodes + TB - 1) / TB;
     __shared__ R lx[kTileNodes * 3];
     __shared__ R lv[kTileNodes * 3];
-    __shared__ R cf[kTileElems * 12];
-    __shared__ R cv[kTileElems * 12];
-    __shared__ double sh[kBlock / 64];
+    __shared__ R lc[kTileElems * 12];
+    __shared__ uint16_t linc[kTileElems * 4];
+    __shared__ int lptr[kTileNodes + 1];
+    __shared__ double sh[TB / 64];
     const TileDesc td = a.tiles[blockIdx.x];
     const InstDev& in = a.inst[td.scene];
-    for (int j = threadIdx.x; j < td.ln_count; j += kBlock) {
-        const int g = a.tn.gid[td.ln_begin + j];
-        VT t[3];
-        ld3(x, g, t);
-        lx[3 * j] = (R)t[0]; lx[3 * j + 1] = (R)t[1]; lx[3 * j + 2] = (R)t[2];
-        ld3(v, g, t);
-        lv[3 * j] = (R)t[0]; lv[3 * j + 1] = (R)t[1]; lv[3 * j + 2] = (R)t[2];
+    // element records of this thread's elements, in registers for both passes
+    R rin[EPT][NR], kk[EPT][3], em[EPT], rot[EPT][9];
+    uint2 lt[EPT];
+#pragma unroll
+    for (int m = 0; m < EPT; m++) {
+        const int k = threadIdx.x + m * TB;
+        if (k < td.e_count) {
+            const int64_t e = (int64_t)td.e_begin + k;
+            lt[m] = a.ltet[e];
+#pragma unroll
+            for (int i = 0; i < NR; i++) rin[m][i] = a.ev.rest[i * a.ev.E + e];
+#pragma unroll
+            for (int i = 0; i < 3; i++) kk[m][i] = a.ev.kmat[i * a.ev.E + e];
+            em[m] = in.mass_kind == kMassMatrix ? a.emass[e] : (R)0;
+        }
+    }
+    int gid[NPT], psl[NPT];
+#pragma unroll
+    for (int m = 0; m < NPT; m++) {
+        const int j = threadIdx.x + m * TB;
+        if (j < td.ln_count) {
+            const int li = td.ln_begin + j;
+            gid[m] = a.tn.gid[li];
+            psl[m] = a.tn.pslot[li];
+            VT t[3];
+            ld3(x, gid[m], t);
+            lx[3 * j] = (R)t[0]; lx[3 * j + 1] = (R)t[1]; lx[3 * j + 2] = (R)t[2];
+            ld3(v, gid[m], t);
+            lv[3 * j] = (R)t[0]; lv[3 * j + 1] = (R)t[1]; lv[3 * j + 2] = (R)t[2];
+        }
+    }
+    const int ninc = 4 * td.e_count;
+    for (int j = threadIdx.x; j < ninc; j += TB) linc[j] = a.tn.inc[td.inc_begin + j];
+    for (int j = threadIdx.x; j <= td.ln_count; j += TB) lptr[j] = a.tn.inc_ptr[td.ln_begin + j] - td.inc_begin;
+    __syncthreads();
+    // ---- element force (accumulateForceLarge); the rotations are stored for the operator
+#pragma unroll
+    for (int m = 0; m < EPT; m++) {
+        const int k = threadIdx.x + m * TB;
+        if (k < td.e_count) {
+            const int64_t e = (int64_t)td.e_begin + k;
+            int l[4];
+            unpack_ltet(lt[m], l);
+            R p[4][3], c[12];
+#pragma unroll
+            for (int n = 0; n < 4; n++)
+#pragma unroll
+                for (int i = 0; i < 3; i++) p[n][i] = lx[3 * l[n] + i];
+            elem_force_r<R, NR>(p, rin[m], kk[m], rot[m], c);
+#pragma unroll
+            for (int i = 0; i < 9; i++) a.ev.rot[i * a.ev.E + e] = rot[m][i];
+            if (a.rot_lo) {
+#pragma unroll
+                for (int i = 0; i < 9; i++) a.rot_lo[i * a.ev.E + e] = (float)rot[m][i];
+            }
+#pragma unroll
+            for (int i = 0; i < 12; i++) lc[i * kTileElems + k] = c[i];
+        }
     }
     __syncthreads();
+    // ---- node sums of f in ascending element id (interior nodes: with gravity)
+    VT fn[NPT][3];
+#pragma unroll
+    for (int m = 0; m < NPT; m++) {
+        const int j = threadIdx.x + m * TB;
+        if (j < td.ln_count) {
+            if (psl[m] < 0) mass_gravity(in, a.mass, gid[m], fn[m]);
+            else { fn[m][0] = 0; fn[m][1] = 0; fn[m][2] = 0; }
+            const int s0 = lptr[j], s1 = lptr[j + 1];
+            sum_incidences<false>(linc, s0, s1, lc, fn[m]);
+        }
+    }
+    __syncthreads();
+    // ---- K v (+ the consistent-mass edge terms) into the same buffer
     const R krhs = (R)in.k_rhs;
-    for (int k = threadIdx.x; k < td.e_count; k += kBlock) {
-        const int64_t e = (int64_t)td.e_begin + k;
-        int l[4];
-        unpack_ltet(a.ltet[e], l);
-        R p[4][3], u[12], rot[9], c[12], g[12];
+    const bool edge = in.mass_kind == kMassMatrix && in.m_rhs != 0.0;
 #pragma unroll
-        for (int n = 0; n < 4; n++)
+    for (int m = 0; m < EPT; m++) {
+        const int k = threadIdx.x + m * TB;
+        if (k < td.e_count) {
+            int l[4];
+            unpack_ltet(lt[m], l);
+            R u[12], c[12];
 #pragma unroll
-            for (int i = 0; i < 3; i++) { p[n][i] = lx[3 * l[n] + i]; u[3 * n + i] = lv[3 * l[n] + i]; }
-        elem_force_p<R, NR>(a.ev, e, p, rot, c, g);
+            for (int n = 0; n < 4; n++)
 #pragma unroll
-        for (int i = 0; i < 9; i++) a.ev.rot[i * a.ev.E + e] = rot[i];
-        if (a.rot_lo) {
+                for (int i = 0; i < 3; i++) u[3 * n + i] = lv[3 * l[n] + i];
+            if (in.k_rhs != 0.0) {
+                elem_dforce_r<R, NR>(rot[m], rin[m], kk[m], u, krhs, c);
+            } else {
 #pragma unroll
-            for (int i = 0; i < 9; i++) a.rot_lo[i * a.ev.E + e] = (float)rot[i];
+                for (int i = 0; i < 12; i++) c[i] = 0;
+            }
+            if (edge) add_edge_mass((R)in.m_rhs * em[m], u, c);
+#pragma unroll
+            for (int i = 0; i < 12; i++) lc[i * kTileElems + k] = c[i];
         }
-#pragma unroll
-        for (int i = 0; i < 12; i++) cf[i * kTileElems + k] = c[i];
-        if (in.k_rhs != 0.0) {
-            elem_dforce<R, NR>(a.ev, e, rot, u, krhs, c);
-        } else {
-#pragma unroll
-            for (int i = 0; i < 12; i++) c[i] = 0;
-        }
-        if (in.mass_kind == kMassMatrix && in.m_rhs != 0.0) add_edge_mass((R)in.m_rhs * a.emass[e], u, c);
-#pragma unroll
-        for (int i = 0; i < 12; i++) cv[i * kTileElems + k] = c[i];
     }
     __syncthreads();
+    // ---- b = h (f - rm M v - (h + rs) K v), projected; shared nodes leave partials
     double bb = 0.0;
     const VT h = (VT)in.dt;
-    for (int j = threadIdx.x; j < td.ln_count; j += kBlock) {
-        const int li = td.ln_begin + j;
-        const int g = a.tn.gid[li];
-        const int s0 = a.tn.inc_ptr[li], s1 = a.tn.inc_ptr[li + 1];
-        const int ps = a.tn.pslot[li] < 0 ? -1 : (a.tn.pslot[li] & (kSlotOwner - 1));
-        if (ps < 0) {
-            VT fn[3];
-            mass_gravity(in, a.mass, g, fn);
-            for (int s = s0; s < s1; s++) {
-                const int sl = a.tn.inc[s];
 #pragma unroll
-                for (int i = 0; i < 3; i++) fn[i] += (VT)cf[sl + i * kTileElems];
+    for (int m = 0; m < NPT; m++) {
+        const int j = threadIdx.x + m * TB;
+        if (j < td.ln_count) {
+            const int g = gid[m];
+            const int s0 = lptr[j], s1 = lptr[j + 1];
+            if (psl[m] < 0) {
+                st3(f, g, fn[m]);
+                VT bn[3] = {fn[m][0], fn[m][1], fn[m][2]};
+                if (in.m_rhs != 0.0) {
+                    const VT vn[3] = {(VT)lv[3 * j], (VT)lv[3 * j + 1], (VT)lv[3 * j + 2]};
+                    mass_mdx(in, NodeArrays{nullptr, nullptr, a.mass, a.fixed}, g, in.m_rhs, vn, bn);
+                }
+                sum_incidences<true>(linc, s0, s1, lc, bn);
+                bn[0] *= h; bn[1] *= h; bn[2] *= h;
+                if (a.fixed[g]) { bn[0] = 0; bn[1] = 0; bn[2] = 0; }
+                st3(b, g, bn);
+                st3(r, g, bn);
+                const VT z[3] = {0, 0, 0};
+                st3(xcg, g, z);
+                bb += dot3d(bn, bn);
+            } else {
+                const int ps = psl[m] & (kSlotOwner - 1);
+                VT pv[3] = {0, 0, 0};
+                sum_incidences<false>(linc, s0, s1, lc, pv);
+                st3(spf, ps, fn[m]);
+                st3(spv, ps, pv);
             }
-            st3(f, g, fn);
-            VT bn[3] = {fn[0], fn[1], fn[2]};
-            if (in.m_rhs != 0.0) {
-                const VT vn[3] = {(VT)lv[3 * j], (VT)lv[3 * j + 1], (VT)lv[3 * j + 2]};
-                mass_mdx(in, NodeArrays{nullptr, nullptr, a.mass, a.fixed}, g, in.m_rhs, vn, bn);
-            }
-            for (int s = s0; s < s1; s++) {
-                const int sl = a.tn.inc[s];
-#pragma unroll
-                for (int i = 0; i < 3; i++) bn[i] -= (VT)cv[sl + i * kTileElems];
-            }
-            bn[0] *= h; bn[1] *= h; bn[2] *= h;
-            if (a.fixed[g]) { bn[0] = 0; bn[1] = 0; bn[2] = 0; }
-            st3(b, g, bn);
-            st3(r, g, bn);
-            const VT z[3] = {0, 0, 0};
-            st3(xcg, g, z);
-            bb += dot3d(bn, bn);
-        } else {
-            VT pf[3] = {0, 0, 0}, pv[3] = {0, 0, 0};
-            for (int s = s0; s < s1; s++) {
-                const int sl = a.tn.inc[s];
-#pragma unroll
-                for (int i = 0; i < 3; i++) { pf[i] += (VT)cf[sl + i * kTileElems]; pv[i] += (VT)cv[sl + i * kTileElems]; }
-            }
-            st3(spf, ps, pf);
-            st3(spv, ps, pv);
         }
     }
-    const double tot = wave_block_sum<kBlock>(bb, sh);
+    const double tot = wave_block_sum<TB>(bb, sh);
     if (threadIdx.x == 0) pbb[td.pslot] = tot;
 }
 
@@ -408,7 +526,7 @@ __global__ __launch_bounds__(TB) void k_tile_apply(TileArgs<R, NR> a, int iter, 
     if (cont) {
         // the refinement replaced r by the true residual: beta from its r.r and the r.r that
         // built p; the tolerance is relative to the first sweep's |r|
-        const double rho = reduce2<TB>(pin, rg.b0, rg.nb, 0, 0, sh);
+        const double rho = wave_reduce(pin, rg.b0, rg.nb);
         const double err = sqrt(rho) / ins->normb;
         if (rho == 0.0 || err <= ins->tol) { record_stop(ins, first, 1, 1, err); return; }
         beta = rho / ins->rho_last;
@@ -416,9 +534,8 @@ __global__ __launch_bounds__(TB) void k_tile_apply(TileArgs<R, NR> a, int iter, 
             ins->rho_slot[1] = rho; ins->beta = beta; ins->iter = 1; ins->err = err;
         }
     } else if (iter == 1) {
-        const double bb = pin_nodes ? reduce2<TB>(pin, rg.b0, rg.nb, 0, 0, sh)
-                                    : rg.nf > 0 ? reduce2<TB>(pin, rg.f0, rg.nf, 0, 0, sh)
-                                                : reduce2<TB>(pin, rg.t0, rg.nt, 0, 0, sh);
+        const double bb = pin_nodes ? wave_reduce(pin, rg.b0, rg.nb)
+                                    : rg.nf > 0 ? wave_reduce(pin, rg.f0, rg.nf) : wave_reduce(pin, rg.t0, rg.nt);
         const double normb = sqrt(bb);
         if (normb == 0.0) { record_stop(ins, first, 4, 0, 1.0); return; }
         const double err = sqrt(bb) / normb;
@@ -428,7 +545,7 @@ __global__ __launch_bounds__(TB) void k_tile_apply(TileArgs<R, NR> a, int iter, 
         }
     } else if (iter > 1) {
         if (iter > ins->max_iter) { record_stop(ins, first, 0, ins->max_iter, ins->err); return; }
-        const double rho = reduce2<TB>(pin, rg.b0, rg.nb, 0, 0, sh);
+        const double rho = wave_reduce(pin, rg.b0, rg.nb);
         const double err = sqrt(rho) / ins->normb;
         if (err <= ins->tol) { record_stop(ins, first, 1, iter, err); return; }
         const double rho_1 = ins->rho_slot[(iter - 1) & 1];
@@ -522,21 +639,13 @@ __global__ __launch_bounds__(TB) void k_tile_apply(TileArgs<R, NR> a, int iter, 
                 const VT pp[3] = {(VT)lu[3 * j], (VT)lu[3 * j + 1], (VT)lu[3 * j + 2]};
                 VT qn[3] = {0, 0, 0};
                 if (in.m_cg != 0.0) mass_mdx(in, NodeArrays{nullptr, nullptr, a.mass, a.fixed}, gid[m], in.m_cg, pp, qn);
-                for (int s = s0; s < s1; s++) {
-                    const int sl = linc[s];
-#pragma unroll
-                    for (int i = 0; i < 3; i++) qn[i] -= (VT)lc[sl + i * kTileElems];
-                }
+                sum_incidences<true>(linc, s0, s1, lc, qn);
                 if (fx[m]) { qn[0] = 0; qn[1] = 0; qn[2] = 0; }
                 st3(q, gid[m], qn);
                 pq += dot3d(pp, qn);
             } else {
                 VT acc[3] = {0, 0, 0};
-                for (int s = s0; s < s1; s++) {
-                    const int sl = linc[s];
-#pragma unroll
-                    for (int i = 0; i < 3; i++) acc[i] -= (VT)lc[sl + i * kTileElems];
-                }
+                sum_incidences<true>(linc, s0, s1, lc, acc);
                 st3(sp, psl[m] & (kSlotOwner - 1), acc);
                 // p.q is linear in q: a shared node's term is split over its tiles (p . this
                 // tile's partial), its mass term added by its first tile, so that the tile
