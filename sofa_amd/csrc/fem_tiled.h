@@ -744,7 +744,8 @@ __global__ __launch_bounds__(kBlock) void k_update(const NodeBlk* __restrict__ n
     const bool cont = iter == 1 && ins->keep_dir && ins->sweep > 1;
     const Ranges rg = ranges[nb.scene];
     const bool first = nb.first != 0;
-    const double den = reduce2(pden, rg.t0, rg.nt, 0, 0, sh);
+    // a few partials: every wave adds them all (no barrier); many: split over the block
+    const double den = rg.nt <= 256 ? wave_reduce(pden, rg.t0, rg.nt) : reduce2(pden, rg.t0, rg.nt, 0, 0, sh);
     const bool exempt = (iter == 1 && ins->first_solve);
     if (den == 0.0) { record_stop(ins, first, 3, iter, ins->err); if (first && threadIdx.x == 0) ins->den = den; return; }
     if (fabs(den) <= ins->thr && !exempt) {
