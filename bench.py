@@ -149,6 +149,23 @@ def measured_traffic(path="profiles/k_tile_apply_beam_mix_traffic.md"):
     return {"bytes": d["traffic_bytes_per_launch"], "source": path, "kernel": d["kernel"]}
 
 
+def measured_issue(path="profiles/k_tile_apply_beam_mix_issue.md"):
+    """Instruction-issue figures of the apply kernel on the beam mix (rocprofv3 SQ counters:
+    VALU instructions per wave, the fraction of cycles the VALUs issue, occupancy), from the
+    summary in the profiles folder; None when it is absent."""
+    import re
+    try:
+        text = open(os.path.join(ROOT, path)).read()
+    except OSError:
+        return None
+    m = re.search(r"```json\s*(\{.*?\})\s*```", text, re.S)
+    if not m:
+        return None
+    d = json.loads(m.group(1))
+    d["source"] = path
+    return d
+
+
 def copy_probe(mb=512, reps=5):
     """A plain device copy of known size (calibrates the HBM byte counters; STREAM-copy rate)."""
     import torch
@@ -352,6 +369,10 @@ def main():
         roof["traffic"] = tr["bytes"]
         roof["traffic_source"] = tr["source"] + " (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)"
         roof["traffic_over_algorithmic"] = tr["bytes"] / roof["algorithmic_bytes"]
+        roof["traffic_rate_GBs"] = tr["bytes"] / (roof["us_per_launch"] * 1e-6) / 1e9
+    iss = measured_issue()
+    if iss is not None:
+        roof["issue"] = iss
     log(f"batched: {value:.4g} element-steps/s, {total_ms / args.steps:.3f} ms/step; apply {roof['us_per_launch']:.2f} us "
         f"= {roof['achieved']:.0f} GB/s")
     tiling = {k: 0 for k in ("tiles", "tile_local_nodes", "finalize_nodes", "shared_slots")}
