@@ -113,11 +113,24 @@ int sofa_amd_scene_set_fixed(sofa_amd_engine eng, int scene, const int32_t* indi
  * is cleared. */
 int sofa_amd_scene_set_fix_all(sofa_amd_engine eng, int scene, int fix_all);
 
+/* FixedProjectiveConstraint activate_projectVelocity (FixedProjectiveConstraint.inl:237-258;
+ * default off): after each step's position update the fixed nodes' velocities (every node's
+ * with fixAll) are cleared -- DefaultAnimationLoop's projectPositionAndVelocity
+ * (DefaultAnimationLoop.cpp:228-234), applied before the oscillator's projection. */
+int sofa_amd_scene_set_project_velocity(sofa_amd_engine eng, int scene, int on);
+
 /* EulerImplicitSolver rayleighStiffness / rayleighMass (EulerImplicitSolver.cpp:40-51) and
  * CGLinearSolver iterations / tolerance / threshold (CGLinearSolver.inl:36-40), dt. */
 int sofa_amd_scene_set_solver(sofa_amd_engine eng, int scene, double dt, double rayleigh_stiffness,
                               double rayleigh_mass, int cg_iterations, double cg_tolerance,
                               double cg_threshold);
+
+/* EulerImplicitSolver firstOrder, trapezoidalScheme and vdamping (EulerImplicitSolver.cpp:40-51;
+ * used at :118-123, :136-173 and :246-299): firstOrder solves (M - h K) v = f and sets v = dv;
+ * trapezoidal uses h/2 in the stiffness and Rayleigh terms; vdamping multiplies the new
+ * velocity by exp(-h vdamping) after the position update.  Defaults: 0, 0, 0. */
+int sofa_amd_scene_set_integrator(sofa_amd_engine eng, int scene, int first_order, int trapezoidal,
+                                  double vdamping);
 
 /* SOFA_AMD_MIXED engines: the linear solve of each step is iterative refinement -- r = b - A x
  * in FP64, stop when |r| / |b| <= cg_tolerance (the CGLinearSolver tolerance of

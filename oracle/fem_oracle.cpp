@@ -656,9 +656,14 @@ struct Scene {
     std::vector<std::unique_ptr<ForceField>> extra;
     std::vector<int> fixed;
     bool fixAll = false;
+    bool projectVelocity = false;   // FixedProjectiveConstraint activate_projectVelocity (default false)
     std::vector<Oscillator> oscillators;
     Vec3 gravity;
     Real rayleighStiffness = 0, rayleighMass = 0;
+    // EulerImplicitSolver Data (EulerImplicitSolver.cpp:40-51): firstOrder, trapezoidalScheme,
+    // vdamping (all off by default)
+    bool firstOrder = false, trapezoidal = false;
+    Real velocityDamping = 0;
     unsigned cgIterations = 25;
     Real cgTolerance = 1e-5, cgThreshold = 1e-5;
     int dotMode = 0;           // 0: reference order; 1: GPU block order (256-node blocks)
@@ -801,40 +806,56 @@ struct Scene {
         lastEndCond = endcond;
     }
 
-    // EulerImplicitSolver::solve (EulerImplicitSolver.cpp:83-300), firstOrder = false,
-    // trapezoidalScheme = false, solveConstraint = false; then DefaultAnimationLoop.
+    // EulerImplicitSolver::solve (EulerImplicitSolver.cpp:83-300) with solveConstraint = false;
+    // then DefaultAnimationLoop.
     void step(Real dt) {
         const size_t N = n();
         const Real h = dt;
-        const Real tr = 1.0;
+        const Real tr = trapezoidal ? 0.5 : 1.0;   // :118-123
         // computeForce(f, clear, accumulate): resetForce then addForce of every force field
         f.assign(N, Vec3());
         for (ForceField* ff : forceFields()) ff->addForce(f, x, v);
-        // b = f; b += (-rm M + (h+rs) K) v (addMBKv, B = 0); b *= h; project
+        // b = f; unless firstOrder: b += (-rm M + (h tr + rs) K) v (addMBKv, B = 0), b *= h;
+        // then project (:136-162)
         std::vector<Vec3> b = f;
-        {
+        if (!firstOrder) {
             const Real mF = -rayleighMass, bF = 0, kF = h * tr + rayleighStiffness;
             for (ForceField* ff : forceFields()) ff->addMBKdx(b, v, mF, bF, kF);
+            for (auto& e : b)
+                for (int k = 0; k < 3; k++) e[k] *= h;
         }
-        for (auto& e : b)
-            for (int k = 0; k < 3; k++) e[k] *= h;
         projectResponse(b);
         // system factors (:167-173)
-        const Real mFact = 1 + tr * h * rayleighMass;
-        const Real bFact = -tr * h;
-        const Real kFact = -tr * h * (tr * h + rayleighStiffness);
+        const Real mFact = firstOrder ? 1 : 1 + tr * h * rayleighMass;
+        const Real bFact = firstOrder ? 0 : -tr * h;
+        const Real kFact = firstOrder ? -h * tr : -tr * h * (tr * h + rayleighStiffness);
         std::vector<Vec3> xs(N);
         cgSolve(xs, b, mFact, bFact, kFact);
-        // vMultiOp fast path (MechanicalObject.inl:2209-2230): v += x; pos += v*h
+        // the vMultiOp update (:246-268; MechanicalObject.inl:2209-2230): v = v + x (firstOrder:
+        // v = x), pos = pos + v h
         const Real fxv = (Real)h;
         for (size_t i = 0; i < N; i++)
             for (int k = 0; k < 3; k++) {
-                v[i][k] += xs[i][k];
+                v[i][k] = firstOrder ? xs[i][k] : v[i][k] + xs[i][k];
                 x[i][k] += v[i][k] * fxv;
             }
-        // projectPositionAndVelocity with the context time still at the step start
-        // (OscillatorProjectiveConstraint.inl:77-106; FixedProjectiveConstraint's position and
-        // velocity projections are no-ops by default)
+        // vdamping: v *= exp(-h vdamping) after the position update (:298-299)
+        if (velocityDamping != 0.0) {
+            const Real d = std::exp(-h * velocityDamping);
+            for (size_t i = 0; i < N; i++)
+                for (int k = 0; k < 3; k++) v[i][k] *= d;
+        }
+        // projectPositionAndVelocity (DefaultAnimationLoop.cpp:228-234) with the context time
+        // still at the step start, the fixed constraint before the oscillator: the fixed nodes'
+        // velocities cleared if activate_projectVelocity (FixedProjectiveConstraint.inl:237-258;
+        // its projectPosition is a no-op), then OscillatorProjectiveConstraint.inl:77-106
+        if (projectVelocity) {
+            if (fixAll) {
+                for (auto& e : v) e = Vec3();
+            } else {
+                for (int i : fixed) v[i] = Vec3();
+            }
+        }
         for (const auto& o : oscillators) {
             const Real t = time;
             for (int k = 0; k < 3; k++) {
@@ -922,6 +943,8 @@ void orc_set_fixed(void* h, const int* idx, int n, int fix_all) {
     s->fixAll = fix_all != 0;
 }
 
+void orc_set_project_velocity(void* h, int on) { static_cast<Scene*>(h)->projectVelocity = on != 0; }
+
 void orc_add_oscillator(void* h, int index, const double* mean, const double* amplitude, double pulsation, double phase) {
     Scene* s = static_cast<Scene*>(h);
     Oscillator o;
@@ -941,6 +964,13 @@ void orc_set_solver(void* h, double rayleigh_stiffness, double rayleigh_mass, in
     s->cgIterations = (unsigned)cg_iterations;
     s->cgTolerance = cg_tolerance < 0 ? 1e-5 : cg_tolerance;     // CGLinearSolver::init :49-69
     s->cgThreshold = cg_threshold < 0 ? 1e-5 : cg_threshold;
+}
+
+void orc_set_integrator(void* h, int first_order, int trapezoidal, double vdamping) {
+    Scene* s = static_cast<Scene*>(h);
+    s->firstOrder = first_order != 0;
+    s->trapezoidal = trapezoidal != 0;
+    s->velocityDamping = vdamping;
 }
 
 void orc_set_dot_mode(void* h, int mode) { static_cast<Scene*>(h)->dotMode = mode; }

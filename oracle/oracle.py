@@ -43,6 +43,8 @@ def lib():
         L.orc_add_oscillator.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp, ctypes.c_double, ctypes.c_double]
         L.orc_set_solver.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_double, ctypes.c_double]
         L.orc_set_dot_mode.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.orc_set_integrator.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_double]
+        L.orc_set_project_velocity.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.orc_set_threads.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.orc_step.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_int]
         L.orc_get_vec.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp]
@@ -134,6 +136,14 @@ class OracleScene:
     def set_solver(self, rayleigh_stiffness=0.0, rayleigh_mass=0.0, iterations=25, tolerance=1e-5, threshold=1e-5):
         lib().orc_set_solver(self._h, float(rayleigh_stiffness), float(rayleigh_mass), int(iterations),
                              float(tolerance), float(threshold))
+
+    def set_integrator(self, first_order=False, trapezoidal=False, vdamping=0.0):
+        """EulerImplicitSolver firstOrder, trapezoidalScheme and vdamping."""
+        lib().orc_set_integrator(self._h, 1 if first_order else 0, 1 if trapezoidal else 0, float(vdamping))
+
+    def set_project_velocity(self, on=True):
+        """FixedProjectiveConstraint activate_projectVelocity."""
+        lib().orc_set_project_velocity(self._h, 1 if on else 0)
 
     def set_dot_mode(self, mode):
         lib().orc_set_dot_mode(self._h, int(mode))

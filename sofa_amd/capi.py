@@ -28,7 +28,7 @@ EXPORTS = [
     "sofa_amd_fem_add_force", "sofa_amd_fem_add_dforce", "sofa_amd_engine_bench_apply", "sofa_amd_scene_tiling",
     "sofa_amd_engine_set_graphs", "sofa_amd_scene_set_mass_density", "sofa_amd_scene_add_oscillators",
     "sofa_amd_scene_set_refinement", "sofa_amd_engine_reserve", "sofa_amd_scene_commit", "sofa_amd_scene_release",
-    "sofa_amd_engine_usage", "sofa_amd_scene_set_fix_all", "sofa_amd_engine_set_dot_order",
+    "sofa_amd_engine_usage", "sofa_amd_scene_set_fix_all", "sofa_amd_scene_set_integrator", "sofa_amd_scene_set_project_velocity", "sofa_amd_engine_set_dot_order",
     "sofa_amd_scene_prepare", "sofa_amd_engine_autotune", "sofa_amd_engine_set_apply_variant",
 ]
 
@@ -94,6 +94,8 @@ def lib():
         "sofa_amd_scene_release": (c_int, [_vp, c_int]),
         "sofa_amd_engine_usage": (c_int, [_vp, ctypes.POINTER(ctypes.c_int64)]),
         "sofa_amd_scene_set_fix_all": (c_int, [_vp, c_int, c_int]),
+        "sofa_amd_scene_set_integrator": (c_int, [_vp, c_int, c_int, c_int, c_double]),
+        "sofa_amd_scene_set_project_velocity": (c_int, [_vp, c_int, c_int]),
         "sofa_amd_engine_set_dot_order": (c_int, [_vp, c_int]),
         "sofa_amd_scene_prepare": (c_int, [_vp, c_int]),
         "sofa_amd_engine_autotune": (c_int, [_vp, c_int, ctypes.POINTER(c_int), ctypes.POINTER(ctypes.c_float)]),
@@ -223,6 +225,15 @@ class Engine:
 
     def sync(self):
         check(lib().sofa_amd_engine_sync(self._h))
+
+    def set_project_velocity(self, scene, on=True):
+        """FixedProjectiveConstraint activate_projectVelocity."""
+        check(lib().sofa_amd_scene_set_project_velocity(self._h, int(scene), 1 if on else 0))
+
+    def set_integrator(self, scene, first_order=False, trapezoidal=False, vdamping=0.0):
+        """EulerImplicitSolver firstOrder, trapezoidalScheme, vdamping."""
+        check(lib().sofa_amd_scene_set_integrator(self._h, int(scene), 1 if first_order else 0,
+                                                  1 if trapezoidal else 0, float(vdamping)))
 
     def set_fix_all(self, scene, fix_all=True):
         """FixedProjectiveConstraint fixAll."""

@@ -158,11 +158,14 @@ struct SceneHost {
     double g[3] = {0, 0, 0};
     std::vector<int32_t> fixed{0};          // FixedProjectiveConstraint default indices = {0}
     bool fix_all = false;                   // FixedProjectiveConstraint fixAll
+    bool project_velocity = false;          // FixedProjectiveConstraint activate_projectVelocity
     std::vector<double> density{1.0};       // MeshMatrixMass massDensity (one or per element)
     // OscillatorProjectiveConstraint entries
     std::vector<int32_t> osc_idx;
     std::vector<double> osc_mean, osc_amp, osc_w, osc_phase;
     double dt = 0.01, rs = 0, rm = 0;
+    bool first_order = false, trapezoidal = false;   // EulerImplicitSolver firstOrder, trapezoidalScheme
+    double vdamping = 0;                    // EulerImplicitSolver vdamping
     int cg_iter = 25;
     double tol = 1e-5, thr = 1e-5;
     // MIXED: inner FP32 CG iterations / relative tolerance per sweep, sweeps per step at most
@@ -513,10 +516,11 @@ struct EngineT : EngineBase {
             if (i < 0 || i >= n) throw std::invalid_argument("fixed index out of range");
             h.fixed[i] = 1;
         }
-        // oscillators: projectResponse clears their nodes like a fixed constraint
+        // oscillators: projectResponse clears their nodes like a fixed constraint (mask 2: not
+        // a fixed node for projectVelocity)
         for (int32_t i : s.osc_idx) {
             if (i < 0 || i >= n) throw std::invalid_argument("oscillator index out of range");
-            h.fixed[i] = 1;
+            if (!h.fixed[i]) h.fixed[i] = 2;
         }
     }
 
@@ -545,11 +549,24 @@ struct EngineT : EngineBase {
             // UniformMass::addForce: mg = theGravity * m, in the state's precision
             in.mg[k] = (double)((VT)s.g[k] * (VT)in.vertex_mass);
         }
-        const double h = s.dt, tr = 1.0;
-        in.k_rhs = h * tr + s.rs;
-        in.m_rhs = -s.rm;
-        in.m_cg = 1 + tr * h * s.rm;
-        in.k_cg = -tr * h * (tr * h + s.rs);
+        // EulerImplicitSolver.cpp:118-123,136-173: trapezoidal tr = 1/2; firstOrder: b = f, A = M - h tr K
+        const double h = s.dt, tr = s.trapezoidal ? 0.5 : 1.0;
+        if (s.first_order) {
+            in.k_rhs = 0.0;
+            in.m_rhs = 0.0;
+            in.m_cg = 1.0;
+            in.k_cg = -h * tr;
+            in.b_scale = 1.0;
+        } else {
+            in.k_rhs = h * tr + s.rs;
+            in.m_rhs = -s.rm;
+            in.m_cg = 1 + tr * h * s.rm;
+            in.k_cg = -tr * h * (tr * h + s.rs);
+            in.b_scale = h;
+        }
+        in.first_order = s.first_order ? 1 : 0;
+        in.project_velocity = s.project_velocity ? 1 : 0;
+        in.vdamp = s.vdamping != 0.0 ? std::exp(-h * s.vdamping) : 1.0;
         in.tol = s.tol < 0 ? 1e-5 : s.tol;
         in.thr = s.thr < 0 ? 1e-5 : s.thr;
         in.first_solve = 1;
@@ -1086,7 +1103,7 @@ struct EngineT : EngineBase {
                 k_cg_update<VT><<<cur->nb, kBlock, 0, s>>>(nb, inst, p_new, q, xcg, r, partial, ndp);
             }
         }
-        if (cur->nb > 0) k_euler_update<VT><<<cur->nb, kBlock, 0, s>>>(nb, inst, xcg, v, x);
+        if (cur->nb > 0) k_euler_update<VT><<<cur->nb, kBlock, 0, s>>>(nb, inst, fixed, xcg, v, x);
         if (cur->no > 0) k_oscillate<VT><<<cur->no, kBlock, 0, s>>>((const Blk*)cur->o, inst, oscillators(), x, v);
         k_finalize<<<(n + 63) / 64, 64, 0, s>>>(cur->scenes, n, inst);
     }
@@ -1177,7 +1194,7 @@ struct EngineT : EngineBase {
             if (cur->nc > 0)
                 k_update<VT><<<cur->nc, kBlock, 0, s>>>(nb, inst, ranges_d, it, sn, sp0, mass, fixed, p, q, xcg, r, pden, prho);
         }
-        if (cur->nc > 0) k_integrate<VT><<<cur->nc, kBlock, 0, s>>>(nb, inst, xcg, v, x);
+        if (cur->nc > 0) k_integrate<VT><<<cur->nc, kBlock, 0, s>>>(nb, inst, fixed, xcg, v, x);
         if (cur->no > 0) k_oscillate<VT><<<cur->no, kBlock, 0, s>>>((const Blk*)cur->o, inst, oscillators(), x, v);
         k_step_end_tiled<<<(n + 63) / 64, 64, 0, s>>>(cur->scenes, n, inst);
     }
@@ -1222,7 +1239,7 @@ struct EngineT : EngineBase {
                 if (cur->nc > 0) k_ir_accumulate<<<cur->nc, kBlock, 0, s>>>(nb, inst, inst_in, d32, xcg);
             }
         }
-        if (cur->nc > 0) k_integrate<VT><<<cur->nc, kBlock, 0, s>>>(nb, inst, xcg, v, x);
+        if (cur->nc > 0) k_integrate<VT><<<cur->nc, kBlock, 0, s>>>(nb, inst, fixed, xcg, v, x);
         if (cur->no > 0) k_oscillate<VT><<<cur->no, kBlock, 0, s>>>((const Blk*)cur->o, inst, oscillators(), x, v);
         k_step_end_mixed<<<(n + 63) / 64, 64, 0, s>>>(cur->scenes, n, inst, inst_in);
     }
@@ -1620,6 +1637,24 @@ int sofa_amd_scene_set_solver(sofa_amd_engine eng, int scene, double dt, double 
     if (!s || !(dt > 0) || iters < 0) return fail(SOFA_AMD_EINVAL, "invalid solver parameters");
     if (!editable(eng, scene)) return fail(SOFA_AMD_ESTATE, "scene already committed");
     s->dt = dt; s->rs = rs; s->rm = rm; s->cg_iter = iters; s->tol = tol; s->thr = thr;
+    return SOFA_AMD_OK;
+}
+
+int sofa_amd_scene_set_project_velocity(sofa_amd_engine eng, int scene, int on) {
+    SceneHost* s = scene_of(eng, scene);
+    if (!s) return fail(SOFA_AMD_EINVAL, "invalid scene");
+    if (!editable(eng, scene)) return fail(SOFA_AMD_ESTATE, "scene already committed");
+    s->project_velocity = on != 0;
+    return SOFA_AMD_OK;
+}
+
+int sofa_amd_scene_set_integrator(sofa_amd_engine eng, int scene, int first_order, int trapezoidal, double vdamping) {
+    SceneHost* s = scene_of(eng, scene);
+    if (!s || !std::isfinite(vdamping)) return fail(SOFA_AMD_EINVAL, "invalid integrator parameters");
+    if (!editable(eng, scene)) return fail(SOFA_AMD_ESTATE, "scene already committed");
+    s->first_order = first_order != 0;
+    s->trapezoidal = trapezoidal != 0;
+    s->vdamping = vdamping;
     return SOFA_AMD_OK;
 }
 

@@ -66,6 +66,11 @@ struct InstDev {
     double rho_last;                  // inner record: r.r that built the current direction
     double ks[3];                     // one material for the whole scene: K00, K01, K33 times
                                       // 36 V (computeMaterialStiffness before the volume division)
+    // EulerImplicitSolver options (EulerImplicitSolver.cpp:40-51,136-173,246-299)
+    double b_scale;                   // the right-hand side's factor: h, or 1 with firstOrder
+    double vdamp;                     // exp(-h vdamping), 1 without damping
+    int first_order;                  // v = dv instead of v += dv
+    int project_velocity;             // FixedProjectiveConstraint activate_projectVelocity
 };
 
 struct Blk { int inst, begin, end, pad; };   // a block's scene and [begin, end) range

@@ -27,7 +27,8 @@ struct NodeArrays {
     const int* __restrict__ inc_ptr;   // per node [inc_ptr[n], inc_ptr[n+1]) in inc
     const int* __restrict__ inc;       // element*4 + local vertex, ascending element id
     const double* __restrict__ mass;   // per-node mass (DiagonalMass, MeshMatrixMass vertex mass)
-    const int8_t* __restrict__ fixed;  // 1 where a projective constraint clears the response
+    const int8_t* __restrict__ fixed;  // non-zero where a projective constraint clears the response:
+                                       // 1 a fixed node, 2 a node driven by an oscillator
     const int* __restrict__ em_ptr;    // MeshMatrixMass edges of each node, ascending edge id:
     const int* __restrict__ em_nbr;    //   the other end
     const double* __restrict__ em_val; //   the edge mass rho V / 20 summed over the edge's tets
@@ -176,7 +177,7 @@ __global__ __launch_bounds__(kBlock) void k_rhs_node(const Blk* __restrict__ nbl
             const int64_t slot = na.inc[s];
             bn[0] -= ef_v[3 * slot]; bn[1] -= ef_v[3 * slot + 1]; bn[2] -= ef_v[3 * slot + 2];
         }
-        const VT h = (VT)in.dt;
+        const VT h = (VT)in.b_scale;   // h (firstOrder: 1, b = f)
         bn[0] *= h; bn[1] *= h; bn[2] *= h;
         if (na.fixed[n]) { bn[0] = 0; bn[1] = 0; bn[2] = 0; }
         const int64_t o = 3 * (int64_t)n;
@@ -366,19 +367,27 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(const Blk* __restrict__ nb
 // ---------------------------------------------------------------------------------------
 template <typename VT>
 __global__ __launch_bounds__(kBlock) void k_euler_update(const Blk* __restrict__ nblk, const InstDev* __restrict__ inst,
+                                                          const int8_t* __restrict__ fixed,
                                                           const VT* __restrict__ xcg, VT* __restrict__ v,
                                                           VT* __restrict__ x) {
     const Blk bk = nblk[blockIdx.x];
     const int n = bk.begin + threadIdx.x;
     if (n >= bk.end) return;
-    // MechanicalObject::vMultiOp fast path (MechanicalObject.inl:2209-2230)
-    const VT h = (VT)inst[bk.inst].dt;
+    // MechanicalObject::vMultiOp fast path (MechanicalObject.inl:2209-2230): v += dv (firstOrder:
+    // v = dv), x += v h; then vdamping's v *= exp(-h d) (EulerImplicitSolver.cpp:246-299)
+    const InstDev& in = inst[bk.inst];
+    const VT h = (VT)in.dt;
     const int64_t o = 3 * (int64_t)n;
 #pragma unroll
     for (int k = 0; k < 3; k++) {
-        const VT vv = v[o + k] + xcg[o + k];
-        v[o + k] = vv;
+        const VT vv = in.first_order ? xcg[o + k] : v[o + k] + xcg[o + k];
         x[o + k] = x[o + k] + vv * h;
+        v[o + k] = in.vdamp != 1.0 ? vv * (VT)in.vdamp : vv;
+    }
+    // FixedProjectiveConstraint::projectVelocity (.inl:237-258), when activated
+    if (in.project_velocity && fixed[n] == 1) {
+#pragma unroll
+        for (int k = 0; k < 3; k++) v[o + k] = 0;
     }
 }
 

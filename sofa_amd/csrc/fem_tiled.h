@@ -352,7 +352,7 @@ __global__ __launch_bounds__(TB) void k_tile_rhs(TileArgs<R, NR> a, const VT* __
     __syncthreads();
     // ---- b = h (f - rm M v - (h + rs) K v), projected; shared nodes leave partials
     double bb = 0.0;
-    const VT h = (VT)in.dt;
+    const VT h = (VT)in.b_scale;   // h (firstOrder: 1, b = f)
 #pragma unroll
     for (int m = 0; m < NPT; m++) {
         const int j = threadIdx.x + m * TB;
@@ -419,7 +419,7 @@ __global__ __launch_bounds__(kBlock) void k_fin_rhs(const FinBlk* __restrict__ f
 #pragma unroll
             for (int i = 0; i < 3; i++) bn[i] -= spv[3 * (int64_t)s + i];
         }
-        const VT h = (VT)in.dt;
+        const VT h = (VT)in.b_scale;   // h (firstOrder: 1, b = f)
         bn[0] *= h; bn[1] *= h; bn[2] *= h;
         if (fixed[g]) { bn[0] = 0; bn[1] = 0; bn[2] = 0; }
         st3(b, g, bn);
@@ -798,11 +798,16 @@ __global__ __launch_bounds__(kBlock) void k_update(const NodeBlk* __restrict__ n
 
 template <typename VT>
 __global__ __launch_bounds__(kBlock) void k_integrate(const NodeBlk* __restrict__ nb_, InstDev* __restrict__ inst,
+                                                       const int8_t* __restrict__ fixed,
                                                        const VT* __restrict__ xcg, VT* __restrict__ v,
                                                        VT* __restrict__ x) {
     const NodeBlk nb = nb_[blockIdx.x];
-    InstDev* ins = &inst[nb.scene];
-    const VT h = (VT)ins->dt;
+    const InstDev& in = inst[nb.scene];
+    const VT h = (VT)in.dt;
+    // v += dv (firstOrder: v = dv), x += v h, then vdamping's v *= exp(-h d)
+    // (EulerImplicitSolver.cpp:246-299)
+    const bool first_order = in.first_order != 0, damp = in.vdamp != 1.0;
+    const VT vd = (VT)in.vdamp;
 #pragma unroll
     for (int k = 0; k < kNodesPerThread; k++) {
         const int n = nb.begin + k * kBlock + threadIdx.x;
@@ -810,9 +815,14 @@ __global__ __launch_bounds__(kBlock) void k_integrate(const NodeBlk* __restrict_
             const int64_t o = 3 * (int64_t)n;
 #pragma unroll
             for (int i = 0; i < 3; i++) {
-                const VT vv = v[o + i] + xcg[o + i];
-                v[o + i] = vv;
+                const VT vv = first_order ? xcg[o + i] : v[o + i] + xcg[o + i];
                 x[o + i] = x[o + i] + vv * h;
+                v[o + i] = damp ? vv * vd : vv;
+            }
+            // FixedProjectiveConstraint::projectVelocity (.inl:237-258), when activated
+            if (in.project_velocity && fixed[n] == 1) {
+#pragma unroll
+                for (int i = 0; i < 3; i++) v[o + i] = 0;
             }
         }
     }

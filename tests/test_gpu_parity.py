@@ -128,6 +128,57 @@ def test_fp64_trajectory_cfg0(nsteps):
     eng.close()
 
 
+@pytest.mark.parametrize("mode", ["first_order", "trapezoidal", "vdamping", "project_velocity"])
+def test_integrator_options(mode):
+    """EulerImplicitSolver firstOrder / trapezoidalScheme / vdamping and FixedProjectiveConstraint
+    activate_projectVelocity (fixed nodes started with a velocity) on the cfg0 beam, 30 steps:
+    FP64 bit-exact against the oracle in the GPU's dot order; FP64 tiled <= 1e-10 and FP32
+    <= 2e-5 relative against the oracle's reference order."""
+    capi = _capi()
+    print(f"case: integrator option {mode}, cfg0, 30 steps", flush=True)
+    c = beam_cfg0()
+    opts = dict(first_order=mode == "first_order", trapezoidal=mode == "trapezoidal",
+                vdamping=3.0 if mode == "vdamping" else 0.0)
+    project = mode == "project_velocity"
+    if mode == "first_order":
+        # (M - h K) dv = f is far stiffer than the second-order system: CG 25 stops at |r|/|b| ~ 0.1,
+        # where the two dot orders of the oracle itself differ by 1e-8; 300 iterations converge
+        # (228 used) and bring that to 1e-12
+        c = dict(c, iters=300)
+    v0 = np.zeros_like(c["pos"])
+    if project:
+        v0[c["fixed"]] = (0.02, 0.0, 0.05)
+    nsteps = 30
+    orcs = []
+    for dm in (1, 0):
+        o = make_oracle(c, dot_mode=dm)
+        o.set_integrator(**opts)
+        o.set_project_velocity(project)
+        o.set("v", v0)
+        o.step(c["dt"], nsteps)
+        orcs.append(o)
+    orc1, orc0 = orcs
+    x0 = orc0.get("x")
+    assert rel_err(x0, c["pos"]) > 1e-6          # the beam moved
+    for prec, bar in ((capi.FP64, None), (capi.FP64_TILED, 1e-10), (capi.FP32, 2e-5)):
+        eng = capi.Engine(0, prec)
+        sid = add_gpu_scene(eng, c)
+        eng.set_integrator(sid, **opts)
+        eng.set_project_velocity(sid, project)
+        eng.commit()
+        eng.write(sid, v=v0)
+        eng.step([sid], nsteps)
+        x, v, _ = eng.read(sid)
+        eng.close()
+        if bar is None:
+            assert np.array_equal(x, orc1.get("x")), rel_err(x, orc1.get("x"))
+            assert np.array_equal(v, orc1.get("v")), rel_err(v, orc1.get("v"))
+        else:
+            err = rel_err(x, x0)
+            print(f"  precision {prec}: rel err {err:.3g}", flush=True)
+            assert err <= bar, (prec, err)
+
+
 def test_fp64_golden_checkFEMValues():
     """The reference's checkFEMValues scene on the GPU: position[159] after 100 steps
     (BaseTetrahedronFEMForceField_test.h:379-431) within its 1e-4 tolerance."""

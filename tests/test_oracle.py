@@ -112,6 +112,72 @@ def test_euler_closed_form(case):
         s.step(h, 1)
 
 
+@pytest.mark.parametrize("mode", ["first_order", "trapezoidal", "vdamping"])
+def test_euler_options_closed_form(mode):
+    """firstOrder, trapezoidalScheme and vdamping (EulerImplicitSolver.cpp:118-123,136-173,246-299)
+    on the 1-DOF spring (f = -K (z - z0) - m g, df/dz = -K) against their closed forms:
+      firstOrder:   (m + h K) v' = f(z),                     z' = z + h v'
+      trapezoidal:  tr = 1/2: v' = v + h (f - rm m v - (h tr + rk) K v) / ((1 + h tr rm) m + h tr (h tr + rk) K)
+      vdamping:     the implicit step, z' = z + h v'', v' = v'' exp(-h d)."""
+    g = GOLD["eulerDynamic"]
+    h, K, m, rm, rk, d = 0.01, 100.0, 1.0, 0.1, 0.1, 2.0
+    s = OracleScene(np.array([[0.0, g["z0"], 0.0]]))
+    s.set_gravity((0.0, -g["g"], 0.0))
+    s.set_uniform_mass(vertex_mass=m)
+    s.add_axis_spring(0, 1, K, g["z0"])
+    s.set_fixed(np.zeros(0, np.int32))
+    s.set_solver(rk, rm, 25, 1e-12, 1e-30)
+    s.set_integrator(first_order=mode == "first_order", trapezoidal=mode == "trapezoidal",
+                     vdamping=d if mode == "vdamping" else 0.0)
+    z, v = g["z0"] + 0.05, 0.0
+    s.set("x", np.array([[0.0, z, 0.0]]))
+    for i in range(200):
+        f = -K * (z - g["z0"]) - m * g["g"]
+        if mode == "first_order":
+            v = f / (m + h * K)
+            z = z + h * v
+        elif mode == "trapezoidal":
+            tr = 0.5
+            v = v + h * (f - rm * m * v - (h * tr + rk) * K * v) / ((1 + h * tr * rm) * m + h * tr * (h * tr + rk) * K)
+            z = z + h * v
+        else:
+            v = v + h * (f - rm * m * v - (h + rk) * K * v) / ((1 + h * rm) * m + h * (h + rk) * K)
+            z = z + h * v
+            v = v * np.exp(-h * d)
+        s.step(h, 1)
+        y, vy = s.get("x")[0, 1], s.get("v")[0, 1]
+        assert abs(y - z) <= 1e-12 * max(1.0, abs(z)), (mode, i, y, z)
+        assert abs(vy - v) <= 1e-11 * max(1.0, abs(v)), (mode, i, vy, v)
+
+
+@pytest.mark.parametrize("project", [False, True])
+def test_project_velocity_closed_form(project):
+    """FixedProjectiveConstraint activate_projectVelocity (.inl:237-258) applied after the position
+    update (DefaultAnimationLoop.cpp:281-293): fixed nodes given a velocity v0 move by h v0 in the
+    first step and then stay (on), or keep drifting by h v0 per step (off, the response being
+    cleared so v never changes)."""
+    pos, tets, fixed = meshes.beam((3, 3, 5), (1.0, 1.0, 2.0))
+    s = OracleScene(pos)
+    s.set_gravity((0.0, -9.81, 0.0))
+    s.set_fem(tets, [1e5], [0.45], "large")
+    s.set_uniform_mass(total_mass=100.0)
+    s.set_fixed(fixed)
+    s.set_solver(0.1, 0.1, 25, 1e-9, 1e-9)
+    s.set_project_velocity(project)
+    v = np.zeros_like(pos)
+    v[fixed] = (0.0, 0.0, 0.1)
+    s.set("v", v)
+    h, n = 0.01, 5
+    s.step(h, n)
+    x, vv = s.get("x"), s.get("v")
+    moves = 1 if project else n
+    expect = pos[fixed].copy()
+    for _ in range(moves):
+        expect = expect + np.array([0.0, 0.0, 0.1]) * h
+    assert np.allclose(x[fixed], expect, rtol=0, atol=1e-15)
+    assert np.all(vv[fixed] == (0.0 if project else np.array([0.0, 0.0, 0.1])))
+
+
 # ------------------------------------------------------------------------------------------
 # mathematical invariants
 # ------------------------------------------------------------------------------------------
