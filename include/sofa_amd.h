@@ -203,9 +203,10 @@ int sofa_amd_engine_set_apply_variant(sofa_amd_engine eng, int variant);
 /* Blocks until the engine's work is done. */
 int sofa_amd_engine_sync(sofa_amd_engine eng);
 
-/* Host copies of a scene's position / velocity / force (n_nodes x 3 doubles each; NULL skips). */
+/* Host copies of a scene's position / velocity / force (n_nodes x 3 doubles each; NULL skips).
+ * The copies are made in any case; SOFA_AMD_ENAN when one of them holds a NaN or an Inf. */
 int sofa_amd_scene_read(sofa_amd_engine eng, int scene, double* x, double* v, double* f);
-/* Overwrites position / velocity (NULL skips). */
+/* Overwrites position / velocity (NULL skips); SOFA_AMD_ENAN (nothing written) for a NaN or Inf. */
 int sofa_amd_scene_write(sofa_amd_engine eng, int scene, const double* x, const double* v);
 int sofa_amd_scene_stats(sofa_amd_engine eng, int scene, sofa_amd_cg_stats* out);
 /* Device-side element data of one element as the reference stores it (for parity checks):

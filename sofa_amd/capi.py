@@ -161,6 +161,11 @@ class Engine:
         check(L.sofa_amd_scene_create(self._h, len(pos), pp, ctypes.byref(sid)))
         s = sid.value
         t, tp = _i(np.asarray(tets).reshape(-1))
+        # recorded at creation, so that the lists stay indexed by scene id even when a later
+        # call of this set-up is refused (the engine keeps the half-made scene's id)
+        assert s == len(self.n_nodes), (s, len(self.n_nodes))
+        self.n_nodes.append(len(pos))
+        self.n_tets.append(len(t) // 4)
         y, yp = _d(np.atleast_1d(young))
         nu, nup = _d(np.atleast_1d(poisson))
         check(L.sofa_amd_scene_set_fem(self._h, s, len(t) // 4, tp, yp, len(y), nup, len(nu)))
@@ -182,8 +187,6 @@ class Engine:
         check(L.sofa_amd_scene_set_fixed(self._h, s, fxp, len(fx)))
         check(L.sofa_amd_scene_set_solver(self._h, s, float(dt), float(rayleigh_stiffness), float(rayleigh_mass),
                                           int(cg_iterations), float(cg_tolerance), float(cg_threshold)))
-        self.n_nodes.append(len(pos))
-        self.n_tets.append(len(t) // 4)
         return s
 
     def commit(self):

@@ -288,6 +288,7 @@ struct InnerCg {
     int na, nfb, nnb, maxit;
     SharedNodes sn;
     float *r, *p, *q, *d, *sp;
+    const float *x, *x0;            // float positions and rest positions (geometry recomputed)
     double *pden, *prho;            // prho: |r|^2 partials per node block (iteration 1 included)
 };
 void launch_inner_cg(const InnerCg& c, hipStream_t s);
@@ -330,7 +331,8 @@ struct EngineT : EngineBase {
     VT *sp0 = nullptr, *sp1 = nullptr;
     double *pbb = nullptr, *pden = nullptr, *prho = nullptr;
     // MIXED: float element record and inner CG vectors, inner scene records, |r|^2 partials
-    float *rot32 = nullptr, *rest32 = nullptr, *kmat32 = nullptr, *emass32 = nullptr;
+    float *kmat32 = nullptr, *emass32 = nullptr;   // MIXED: float K entries and edge masses
+    float *x32 = nullptr, *x0_32 = nullptr;   // MIXED: float positions (per step) and rest positions
     float *r32 = nullptr, *p32 = nullptr, *q32 = nullptr, *d32 = nullptr, *sp32 = nullptr;
     double *pden32 = nullptr, *prho32 = nullptr, *prr = nullptr;
     InstDev* inst_in = nullptr;
@@ -370,8 +372,8 @@ struct EngineT : EngineBase {
         void* ptrs[] = {x, v, f, b, r, p, q, xcg, mass, fixed, inc_ptr, inc, tet, rot, rest, kmat, ef, ef2,
                         inst, partial, tiles_d, ranges_d, ltet, ln_gid, ln_inc_ptr, ln_pslot, sh_gid, sh_ptr,
                         ln_inc, sp0, sp1, pbb, pden, prho, p2, em_ptr, em_nbr, em_val, emass, osc_node,
-                        osc_mean, osc_amp, osc_w, osc_phase, rot32, rest32, kmat32, emass32, r32, p32, q32,
-                        d32, sp32, pden32, prho32, prr, inst_in, sh_of, nd, x0};
+                        osc_mean, osc_amp, osc_w, osc_phase, kmat32, emass32, r32, p32, q32,
+                        d32, sp32, pden32, prho32, prr, inst_in, sh_of, nd, x0, x32, x0_32};
         for (void* ptr : ptrs)
             if (ptr) (void)hipFree(ptr);
     }
@@ -448,7 +450,7 @@ struct EngineT : EngineBase {
         int nE = 0;
         std::vector<int4> tet;                       // placement order, scene-local node ids
         std::vector<R> rot, rest, kmat, emass;       // SoA with stride nE
-        std::vector<float> rest_lo, k_lo, em_lo;     // MIXED
+        std::vector<float> k_lo, em_lo;     // MIXED
         std::vector<double> mass;                    // per node (DiagonalMass / MeshMatrixMass)
         std::vector<int> inc_ptr, inc;               // n + 1, 4 nE: position * 4 + vertex
         std::vector<int8_t> fixed;
@@ -465,7 +467,7 @@ struct EngineT : EngineBase {
         h.tet.resize(nE);
         h.rot.assign(9 * (size_t)nE, (R)0); h.rest.assign(NR * (size_t)nE, (R)0); h.kmat.assign(3 * (size_t)nE, (R)0);
         h.emass.assign(nE, (R)0);
-        if (mixed) { h.rest_lo.assign(6 * (size_t)nE, 0.f); h.k_lo.assign(3 * (size_t)nE, 0.f); h.em_lo.assign(nE, 0.f); }
+        if (mixed) { h.k_lo.assign(3 * (size_t)nE, 0.f); h.em_lo.assign(nE, 0.f); }
         h.mass.assign(n, 0.0);
         std::vector<int> cnt(n + 1, 0);
         for (int e = 0; e < nE; e++) {
@@ -487,10 +489,8 @@ struct EngineT : EngineBase {
                 for (int k = 0; k < 6; k++) h.rest[k * (size_t)nE + pe] = (R)ei.rest[pick[k]];
             }
             for (int k = 0; k < 3; k++) h.kmat[k * (size_t)nE + pe] = (R)ei.k[k];
-            if (mixed) {
-                for (int k = 0; k < 6; k++) h.rest_lo[k * (size_t)nE + pe] = (float)ei.rest[pick[k]];
+            if (mixed)
                 for (int k = 0; k < 3; k++) h.k_lo[k * (size_t)nE + pe] = (float)ei.k[k];
-            }
             for (int k = 0; k < 4; k++) cnt[t[k] + 1]++;
             if (s.mass_kind == SOFA_AMD_MASS_DIAGONAL) {
                 const double m = (s.mass_value * ei.vol) / 4.0;     // DiagonalMass.inl:1075-1080
@@ -754,7 +754,7 @@ struct EngineT : EngineBase {
         em_val = nullptr; emass = nullptr; osc_node = nullptr; osc_mean = osc_amp = osc_w = osc_phase = nullptr;
         inst = nullptr; partial = nullptr; tiles_d = nullptr; ranges_d = nullptr; ltet = nullptr;
         ln_gid = ln_inc_ptr = ln_pslot = sh_gid = sh_ptr = nullptr; ln_inc = nullptr; sp0 = sp1 = nullptr;
-        pbb = pden = prho = nullptr; rot32 = rest32 = kmat32 = emass32 = nullptr;
+        pbb = pden = prho = nullptr; kmat32 = emass32 = nullptr; x32 = x0_32 = nullptr;
         r32 = p32 = q32 = d32 = sp32 = nullptr; pden32 = prho32 = prr = nullptr; inst_in = nullptr; sh_of = nullptr;
         cap = c;
         if (cap.node >= (int64_t)1 << 30 || cap.inc >= (int64_t)1 << 31 || cap.lnode >= (int64_t)1 << 31 ||
@@ -784,9 +784,9 @@ struct EngineT : EngineBase {
         for (double* pb : {pbb, pden}) HIPCHK(hipMemset(pb, 0, std::max<size_t>(1, np) * sizeof(double)));
         HIPCHK(hipMemset(prho, 0, std::max<int64_t>(1, cap.nblk) * sizeof(double)));
         if (mixed) {
-            rot32 = dalloc<float>(9 * E); rest32 = dalloc<float>(6 * E); kmat32 = dalloc<float>(3 * E);
+            kmat32 = dalloc<float>(3 * E);
             emass32 = dalloc<float>(E);
-            for (float** vec : {&r32, &p32, &q32, &d32}) {
+            for (float** vec : {&r32, &p32, &q32, &d32, &x32, &x0_32}) {
                 *vec = dalloc<float>(3 * N);
                 HIPCHK(hipMemset(*vec, 0, 3 * N * sizeof(float)));
             }
@@ -860,12 +860,12 @@ struct EngineT : EngineBase {
         put_at(ltet, eb0, P.ltet);
         put_at(ln_inc, ib0, P.ln_inc);
         if (mixed) {
-            for (int k = 0; k < 6; k++) put_at(rest32, k * Etot + eb0, th.eh.rest_lo.data() + k * (size_t)nE, nE);
             for (int k = 0; k < 3; k++) put_at(kmat32, k * Etot + eb0, th.eh.k_lo.data() + k * (size_t)nE, nE);
             put_at(emass32, eb0, th.eh.em_lo);
-            std::vector<float> rl(9 * (size_t)nE);
-            for (size_t j = 0; j < rl.size(); j++) rl[j] = (float)th.eh.rot[j];
-            for (int k = 0; k < 9; k++) put_at(rot32, k * Etot + eb0, rl.data() + k * (size_t)nE, nE);
+            std::vector<float> xf(3 * (size_t)(n + 1), 0.0f);
+            for (int i = 0; i < 3 * n; i++) xf[i] = (float)s.rest[i];
+            put_at(x32, 3 * (int64_t)nb0, xf);
+            put_at(x0_32, 3 * (int64_t)nb0, xf);
         }
         // tiles and their local nodes (each tile's list keeps one extra entry for its end)
         const int T = (int)P.t_ebegin.size();
@@ -1177,7 +1177,7 @@ struct EngineT : EngineBase {
         ta.mass = mass;
         ta.fixed = fixed;
         ta.emass = emass;
-        ta.rot_lo = nullptr;
+        ta.x_lo = nullptr;
         return ta;
     }
 
@@ -1203,7 +1203,7 @@ struct EngineT : EngineBase {
     // FP32 inner CG, FP64 update
     void launch_step_mixed(int n, hipStream_t s) {
         TileArgs<R, NR> ta = tile_args();
-        ta.rot_lo = rot32;
+        ta.x_lo = x32;                 // the inner operator recomputes the rotations from these
         const FinBlk* fb = (const FinBlk*)cur->b;
         const NodeBlk* nb = (const NodeBlk*)cur->c;
         const SharedNodes sn{sh_gid, sh_ptr, sh_of};
@@ -1211,13 +1211,15 @@ struct EngineT : EngineBase {
         ic.ta.tiles = ta.tiles;
         ic.ta.inst = inst_in;
         ic.ta.ranges = ranges_d;
-        ic.ta.ev = ElemView<float, 6>{tet, rot32, rest32, kmat32, Etot};
+        // rotations and rest coordinates are recomputed from x32 / x0_32: only K is read
+        ic.ta.ev = ElemView<float, 6>{tet, nullptr, nullptr, kmat32, Etot};
         ic.ta.ltet = ltet;
         ic.ta.tn = ta.tn;
         ic.ta.mass = mass;
         ic.ta.fixed = fixed;
         ic.ta.emass = emass32;
-        ic.ta.rot_lo = nullptr;
+        ic.ta.x_lo = nullptr;
+        ic.x = x32; ic.x0 = x0_32;
         ic.fb = fb; ic.nb = nb;
         ic.na = cur->na; ic.nfb = cur->nb; ic.nnb = cur->nc; ic.maxit = cur->maxit;
         ic.sn = sn;
@@ -1414,8 +1416,8 @@ EngineBase* make_engine_fp32() {
 void launch_inner_cg(const InnerCg& c, hipStream_t s) {
     for (int it = 1; it <= c.maxit; it++) {
         if (c.na > 0)
-            k_tile_apply<float, 6, float, kTileThreads, 0><<<c.na, kTileThreads, 0, s>>>(c.ta, it, c.r, c.p, c.q, c.sp,
-                                                                                        c.prho, c.pden, 1, nullptr, nullptr);
+            k_tile_apply<float, 6, float, kTileThreads, 2><<<c.na, kTileThreads, 0, s>>>(c.ta, it, c.r, c.p, c.q, c.sp,
+                                                                                        c.prho, c.pden, 1, c.x, c.x0);
         if (c.nnb > 0)
             k_update<float><<<c.nnb, kBlock, 0, s>>>(c.nb, c.ta.inst, c.ta.ranges, it, c.sn, c.sp, c.ta.mass, c.ta.fixed,
                                                       c.p, c.q, c.d, c.r, c.pden, c.prho);
@@ -1708,11 +1710,21 @@ int sofa_amd_engine_sync(sofa_amd_engine eng) {
     GUARD_END
 }
 
+static bool all_finite(const double* a, size_t n) {
+    for (size_t i = 0; i < n; i++)
+        if (!std::isfinite(a[i])) return false;
+    return true;
+}
+
 int sofa_amd_scene_read(sofa_amd_engine eng, int scene, double* x, double* v, double* f) {
     GUARD_BEGIN
     if (!scene_of(eng, scene)) return fail(SOFA_AMD_EINVAL, "invalid scene");
     if (!usable(eng, scene)) return fail(SOFA_AMD_ESTATE, "scene not committed");
     eng->impl->read(scene, x, v, f);
+    // the buffers are filled either way; a state that has left the finite numbers is reported
+    const size_t n3 = 3 * (size_t)scene_of(eng, scene)->n_nodes;
+    for (const double* a : {(const double*)x, (const double*)v, (const double*)f})
+        if (a && !all_finite(a, n3)) return fail(SOFA_AMD_ENAN, "non-finite state (NaN or Inf) in the scene read back");
     return SOFA_AMD_OK;
     GUARD_END
 }
@@ -1721,6 +1733,9 @@ int sofa_amd_scene_write(sofa_amd_engine eng, int scene, const double* x, const 
     GUARD_BEGIN
     if (!scene_of(eng, scene)) return fail(SOFA_AMD_EINVAL, "invalid scene");
     if (!usable(eng, scene)) return fail(SOFA_AMD_ESTATE, "scene not committed");
+    const size_t n3 = 3 * (size_t)scene_of(eng, scene)->n_nodes;
+    if ((x && !all_finite(x, n3)) || (v && !all_finite(v, n3)))
+        return fail(SOFA_AMD_ENAN, "non-finite position or velocity written (NaN or Inf)");
     eng->impl->write(scene, x, v);
     return SOFA_AMD_OK;
     GUARD_END

@@ -176,7 +176,7 @@ struct TileArgs {
     const double* __restrict__ mass;
     const int8_t* __restrict__ fixed;
     const R* __restrict__ emass;        // per element rho V / 20 (consistent MeshMatrixMass), else unused
-    float* __restrict__ rot_lo;         // MIXED engines: float copy of the rotations (written by the RHS)
+    float* __restrict__ x_lo;           // MIXED engines: float copy of the positions (written by the RHS)
 };
 
 // consistent-mass edge terms of one element, folded into its nodal contributions (which the
@@ -278,6 +278,10 @@ __global__ __launch_bounds__(TB) void k_tile_rhs(TileArgs<R, NR> a, const VT* __
             VT t[3];
             ld3(x, gid[m], t);
             lx[3 * j] = (R)t[0]; lx[3 * j + 1] = (R)t[1]; lx[3 * j + 2] = (R)t[2];
+            if (a.x_lo) {   // every tile of a shared node writes the same value
+                const float xf[3] = {(float)t[0], (float)t[1], (float)t[2]};
+                st3(a.x_lo, gid[m], xf);
+            }
             ld3(v, gid[m], t);
             lv[3 * j] = (R)t[0]; lv[3 * j + 1] = (R)t[1]; lv[3 * j + 2] = (R)t[2];
         }
@@ -302,10 +306,6 @@ __global__ __launch_bounds__(TB) void k_tile_rhs(TileArgs<R, NR> a, const VT* __
             elem_force_r<R, NR>(p, rin[m], kk[m], rot[m], c);
 #pragma unroll
             for (int i = 0; i < 9; i++) a.ev.rot[i * a.ev.E + e] = rot[m][i];
-            if (a.rot_lo) {
-#pragma unroll
-                for (int i = 0; i < 9; i++) a.rot_lo[i * a.ev.E + e] = (float)rot[m][i];
-            }
 #pragma unroll
             for (int i = 0; i < 12; i++) lc[i * kTileElems + k] = c[i];
         }

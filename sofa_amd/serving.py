@@ -35,8 +35,9 @@ def footprint_bytes(precision, nodes, tets):
     per_node = 8 * 3 * pv + 8 + 1 + 4 + 16 + 12 + 6 * pv
     per_tet = 24 + (9 + nr + 3 + 1) * pe + 12 * pv + 24
     if precision == capi.MIXED:
-        per_node += 4 * 3 * 4 + 3 * 4
-        per_tet += (9 + 6 + 3 + 1) * 4
+        # float r, p, q, d, x, x0 and shared partials; float K entries and edge masses
+        per_node += 6 * 3 * 4 + 3 * 4
+        per_tet += (3 + 1) * 4
     return per_node * nodes + per_tet * tets
 
 

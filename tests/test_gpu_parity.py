@@ -724,10 +724,49 @@ def test_fp64_reference_dot_order_bitexact(which):
 # ------------------------------------------------------------------------------------------
 # Error paths and edge cases (the C ABI's status codes)
 # ------------------------------------------------------------------------------------------
+@pytest.mark.parametrize("prec", ["FP64", "FP64_TILED", "FP32", "MIXED"])
+def test_edge_scenes(prec):
+    """The smallest scene (one tetrahedron, one fixed vertex, under gravity) against the oracle
+    (FP64: bit-exact in the GPU's dot order; FP64 tiled and MIXED <= 1e-10, FP32 <= 2e-5 against
+    the reference order), and a scene at rest with no load: b = 0, so CG stops before its
+    first iteration (CGLinearSolver.inl:125, end reason "null b") and nothing moves (FP32:
+    moves by rounding only)."""
+    capi = _capi()
+    print(f"case: edge scenes, {prec}", flush=True)
+    pos = np.array([[0.0, 0.0, 0.0], [1.0, 0.0, 0.0], [0.0, 1.0, 0.0], [0.0, 0.0, 1.0]])
+    tets = np.array([[0, 1, 2, 3]], dtype=np.int32)
+    c = dict(pos=pos, tets=tets, fixed=np.array([0], dtype=np.int32), E=1e4, nu=0.3, total_mass=4.0,
+             g=(0.0, -9.81, 0.0), rs=0.1, rm=0.1, dt=0.01, iters=25, tol=1e-12, thr=1e-30)
+    orc = make_oracle(c, dot_mode=1 if prec == "FP64" else 0)
+    orc.step(c["dt"], 20)
+    eng = capi.Engine(0, getattr(capi, prec))
+    sid = add_gpu_scene(eng, c)
+    rest = add_gpu_scene(eng, dict(beam_cfg0(), g=(0.0, 0.0, 0.0)))
+    eng.commit()
+    eng.step([sid, rest], 20)
+    x, v, _ = eng.read(sid)
+    xr, vr, _ = eng.read(rest)
+    st = eng.stats(rest)
+    eng.close()
+    if prec == "FP64":
+        assert np.array_equal(x, orc.get("x")) and np.array_equal(v, orc.get("v"))
+    else:
+        err = rel_err(x, orc.get("x"))
+        assert err <= (2e-5 if prec == "FP32" else 1e-10), err
+    assert rel_err(orc.get("x"), pos) > 1e-6
+    if prec == "FP32":
+        # the rest shape in float differs from the float positions by rounding: forces of ~1e-7
+        assert rel_err(xr, beam_cfg0()["pos"]) <= 1e-6
+    else:
+        assert np.array_equal(xr, beam_cfg0()["pos"]) and not np.any(vr)
+        assert st["end_reason"] == 4, st
+
+
 def test_error_codes():
     """Invalid input is refused with the documented status and a message, nothing faults:
     out-of-range tetrahedron / fixed indices (EINVAL), a zero-volume tetrahedron at commit
-    (EDEGENERATE), calls out of order (ESTATE), duplicate or unknown scene ids (EINVAL)."""
+    (EDEGENERATE), calls out of order (ESTATE), duplicate or unknown scene ids (EINVAL), a NaN
+    velocity written (ENAN, refused on the host)."""
     capi = _capi()
     print("case: error codes", flush=True)
     c = beam_cfg0()
@@ -746,6 +785,14 @@ def test_error_codes():
         with pytest.raises(capi.SofaAmdError) as ei:
             eng.step([sid, sid], 1)
         assert ei.value.code == capi.EINVAL
+        vbad = np.zeros_like(c["pos"])
+        vbad[5, 1] = np.nan
+        x0, v0, _ = eng.read(sid)
+        with pytest.raises(capi.SofaAmdError) as ei:     # refused on the host, nothing written
+            eng.write(sid, v=vbad)
+        assert ei.value.code == capi.ENAN
+        x, v, _ = eng.read(sid)
+        assert np.array_equal(v, v0) and np.array_equal(x, x0)
         with pytest.raises(capi.SofaAmdError) as ei:
             eng.step([sid + 7], 1)
         assert ei.value.code == capi.EINVAL
