@@ -87,6 +87,15 @@ int sofa_amd_scene_create(sofa_amd_engine eng, int n_nodes, const double* positi
 int sofa_amd_scene_set_fem(sofa_amd_engine eng, int scene, int n_tets, const int32_t* tets,
                            const double* young, int n_young, const double* poisson, int n_poisson);
 
+/* The same on hexahedra (n_hexa x 8 corners, RegularGridTopology order), tessellated as
+ * TetrahedronFEMForceField::init does when its topology has no tetrahedra
+ * (TetrahedronFEMForceField.inl:1318-1370): 6 tetrahedra per cube, corners mirrored by cube
+ * parity along x / y / z; nx, ny = cubes along x and y of the grid (the reference's defaults for
+ * a topology that is not a grid: 2 and 1). Element e of the scene is tetrahedron e % 6 of cube
+ * e / 6. */
+int sofa_amd_scene_set_fem_hexa(sofa_amd_engine eng, int scene, int n_hexa, const int32_t* hexa, int nx, int ny,
+                                const double* young, int n_young, const double* poisson, int n_poisson);
+
 /* UniformMass vertexMass (UniformMass.inl:403-496), DiagonalMass massDensity on the FEM's
  * tetrahedra (DiagonalMass.inl:1060-1080, 536-559, 1393-1413) or a uniform MeshMatrixMass
  * massDensity. */

@@ -28,7 +28,7 @@ EXPORTS = [
     "sofa_amd_fem_add_force", "sofa_amd_fem_add_dforce", "sofa_amd_engine_bench_apply", "sofa_amd_scene_tiling",
     "sofa_amd_engine_set_graphs", "sofa_amd_scene_set_mass_density", "sofa_amd_scene_add_oscillators",
     "sofa_amd_scene_set_refinement", "sofa_amd_engine_reserve", "sofa_amd_scene_commit", "sofa_amd_scene_release",
-    "sofa_amd_engine_usage", "sofa_amd_scene_set_fix_all", "sofa_amd_scene_set_integrator", "sofa_amd_scene_set_project_velocity", "sofa_amd_engine_set_dot_order",
+    "sofa_amd_engine_usage", "sofa_amd_scene_set_fix_all", "sofa_amd_scene_set_integrator", "sofa_amd_scene_set_fem_hexa", "sofa_amd_scene_set_project_velocity", "sofa_amd_engine_set_dot_order",
     "sofa_amd_scene_prepare", "sofa_amd_engine_autotune", "sofa_amd_engine_set_apply_variant",
 ]
 
@@ -95,6 +95,7 @@ def lib():
         "sofa_amd_engine_usage": (c_int, [_vp, ctypes.POINTER(ctypes.c_int64)]),
         "sofa_amd_scene_set_fix_all": (c_int, [_vp, c_int, c_int]),
         "sofa_amd_scene_set_integrator": (c_int, [_vp, c_int, c_int, c_int, c_double]),
+        "sofa_amd_scene_set_fem_hexa": (c_int, [_vp, c_int, c_int, _ip, c_int, c_int, _dp, c_int, _dp, c_int]),
         "sofa_amd_scene_set_project_velocity": (c_int, [_vp, c_int, c_int]),
         "sofa_amd_engine_set_dot_order": (c_int, [_vp, c_int]),
         "sofa_amd_scene_prepare": (c_int, [_vp, c_int]),
@@ -152,23 +153,30 @@ class Engine:
     # ---- scene construction ----
     def add_scene(self, positions, tets, young, poisson, fixed, mass_kind=MASS_UNIFORM, mass_value=1.0,
                   gravity=(0.0, 0.0, 0.0), dt=0.01, rayleigh_stiffness=0.0, rayleigh_mass=0.0,
-                  cg_iterations=25, cg_tolerance=1e-5, cg_threshold=1e-5, oscillators=None):
+                  cg_iterations=25, cg_tolerance=1e-5, cg_threshold=1e-5, oscillators=None, hexa_grid=None):
         """oscillators: None or dict(index, mean (n,3), amplitude (n,3), pulsation (n,), phase (n,));
-        mass_value may be an array of per-element densities for the MeshMatrixMass kinds."""
+        mass_value may be an array of per-element densities for the MeshMatrixMass kinds;
+        hexa_grid=(nx, ny): `tets` holds hexahedra (H,8), tessellated by the library as
+        TetrahedronFEMForceField::init does (6 per cube; nx, ny cubes along x and y)."""
         L = lib()
         pos, pp = _d(np.asarray(positions).reshape(-1, 3))
         sid = ctypes.c_int32()
         check(L.sofa_amd_scene_create(self._h, len(pos), pp, ctypes.byref(sid)))
         s = sid.value
         t, tp = _i(np.asarray(tets).reshape(-1))
+        per = 4 if hexa_grid is None else 8
         # recorded at creation, so that the lists stay indexed by scene id even when a later
         # call of this set-up is refused (the engine keeps the half-made scene's id)
         assert s == len(self.n_nodes), (s, len(self.n_nodes))
         self.n_nodes.append(len(pos))
-        self.n_tets.append(len(t) // 4)
+        self.n_tets.append(len(t) // per * (1 if hexa_grid is None else 6))
         y, yp = _d(np.atleast_1d(young))
         nu, nup = _d(np.atleast_1d(poisson))
-        check(L.sofa_amd_scene_set_fem(self._h, s, len(t) // 4, tp, yp, len(y), nup, len(nu)))
+        if hexa_grid is None:
+            check(L.sofa_amd_scene_set_fem(self._h, s, len(t) // 4, tp, yp, len(y), nup, len(nu)))
+        else:
+            check(L.sofa_amd_scene_set_fem_hexa(self._h, s, len(t) // 8, tp, int(hexa_grid[0]), int(hexa_grid[1]),
+                                                yp, len(y), nup, len(nu)))
         mv = np.atleast_1d(np.asarray(mass_value, dtype=np.float64))
         if mv.size > 1:
             d, dp = _d(mv)

@@ -1567,6 +1567,37 @@ int sofa_amd_scene_set_fem(sofa_amd_engine eng, int scene, int n_tets, const int
     GUARD_END
 }
 
+// TetrahedronFEMForceField::init on a topology with hexahedra and no tetrahedra
+// (TetrahedronFEMForceField.inl:1318-1370): every cube becomes 6 tetrahedra, its corners first
+// mirrored across x where (i mod nx) is even, across y where floor(i / nx) mod ny is odd and
+// across z where floor(i / (nx ny)) is odd (the Hexahedron x/y/z edge pairs,
+// sofa/geometry/Hexahedron.h:67-88), so that neighbouring cubes share their face diagonals.
+int sofa_amd_scene_set_fem_hexa(sofa_amd_engine eng, int scene, int n_hexa, const int32_t* hexa, int nx, int ny,
+                                const double* young, int n_young, const double* poisson, int n_poisson) {
+    GUARD_BEGIN
+    SceneHost* s = scene_of(eng, scene);
+    if (!s || n_hexa < 0 || (n_hexa > 0 && !hexa) || nx < 1 || ny < 1) return fail(SOFA_AMD_EINVAL, "invalid argument");
+    static const int xe[4][2] = {{0, 1}, {4, 5}, {3, 2}, {7, 6}};
+    static const int ye[4][2] = {{4, 7}, {5, 6}, {1, 2}, {0, 3}};
+    static const int ze[4][2] = {{4, 0}, {5, 1}, {6, 2}, {7, 3}};
+    static const int six[6][4] = {{0, 5, 1, 6}, {0, 1, 3, 6}, {1, 3, 6, 2}, {6, 3, 0, 7}, {6, 7, 0, 5}, {7, 5, 4, 0}};
+    std::vector<int32_t> t(24 * (size_t)n_hexa);
+    for (int i = 0; i < n_hexa; i++) {
+        int32_t c[8];
+        for (int k = 0; k < 8; k++) c[k] = hexa[8 * (size_t)i + k];
+        if (!((i % nx) & 1))
+            for (const auto& e : xe) std::swap(c[e[0]], c[e[1]]);
+        if (((i / nx) % ny) & 1)
+            for (const auto& e : ye) std::swap(c[e[0]], c[e[1]]);
+        if ((i / (nx * ny)) & 1)
+            for (const auto& e : ze) std::swap(c[e[0]], c[e[1]]);
+        for (int k = 0; k < 6; k++)
+            for (int v = 0; v < 4; v++) t[24 * (size_t)i + 4 * k + v] = c[six[k][v]];
+    }
+    return sofa_amd_scene_set_fem(eng, scene, 6 * n_hexa, t.data(), young, n_young, poisson, n_poisson);
+    GUARD_END
+}
+
 int sofa_amd_scene_set_mass(sofa_amd_engine eng, int scene, int kind, double value) {
     SceneHost* s = scene_of(eng, scene);
     if (!s || kind < SOFA_AMD_MASS_UNIFORM || kind > SOFA_AMD_MASS_MESHMATRIX_LUMPED || !(value > 0))

@@ -131,6 +131,13 @@ __device__ __forceinline__ double reduce2(const double* __restrict__ buf, int a0
     return wave_block_sum<TB>(acc, sh);
 }
 
+// a scene's sum of na partials, the same bits in every block of the scene: few partials are
+// added by every wave on its own (no barrier), many are split over the block (two barriers)
+template <int TB>
+__device__ __forceinline__ double scene_sum(const double* __restrict__ buf, int a0, int na, double* sh) {
+    return na <= 256 ? wave_reduce(buf, a0, na) : reduce2<TB>(buf, a0, na, 0, 0, sh);
+}
+
 template <typename VT>
 __device__ __forceinline__ void ld3(const VT* __restrict__ v, int n, VT o[3]) {
     const VT* q = v + 3 * (int64_t)n;
@@ -526,7 +533,7 @@ __global__ __launch_bounds__(TB) void k_tile_apply(TileArgs<R, NR> a, int iter, 
     if (cont) {
         // the refinement replaced r by the true residual: beta from its r.r and the r.r that
         // built p; the tolerance is relative to the first sweep's |r|
-        const double rho = wave_reduce(pin, rg.b0, rg.nb);
+        const double rho = scene_sum<TB>(pin, rg.b0, rg.nb, sh);
         const double err = sqrt(rho) / ins->normb;
         if (rho == 0.0 || err <= ins->tol) { record_stop(ins, first, 1, 1, err); return; }
         beta = rho / ins->rho_last;
@@ -534,8 +541,8 @@ __global__ __launch_bounds__(TB) void k_tile_apply(TileArgs<R, NR> a, int iter, 
             ins->rho_slot[1] = rho; ins->beta = beta; ins->iter = 1; ins->err = err;
         }
     } else if (iter == 1) {
-        const double bb = pin_nodes ? wave_reduce(pin, rg.b0, rg.nb)
-                                    : rg.nf > 0 ? wave_reduce(pin, rg.f0, rg.nf) : wave_reduce(pin, rg.t0, rg.nt);
+        const double bb = pin_nodes ? scene_sum<TB>(pin, rg.b0, rg.nb, sh)
+                                    : rg.nf > 0 ? scene_sum<TB>(pin, rg.f0, rg.nf, sh) : scene_sum<TB>(pin, rg.t0, rg.nt, sh);
         const double normb = sqrt(bb);
         if (normb == 0.0) { record_stop(ins, first, 4, 0, 1.0); return; }
         const double err = sqrt(bb) / normb;
@@ -545,7 +552,7 @@ __global__ __launch_bounds__(TB) void k_tile_apply(TileArgs<R, NR> a, int iter, 
         }
     } else if (iter > 1) {
         if (iter > ins->max_iter) { record_stop(ins, first, 0, ins->max_iter, ins->err); return; }
-        const double rho = wave_reduce(pin, rg.b0, rg.nb);
+        const double rho = scene_sum<TB>(pin, rg.b0, rg.nb, sh);
         const double err = sqrt(rho) / ins->normb;
         if (err <= ins->tol) { record_stop(ins, first, 1, iter, err); return; }
         const double rho_1 = ins->rho_slot[(iter - 1) & 1];
@@ -744,8 +751,7 @@ __global__ __launch_bounds__(kBlock) void k_update(const NodeBlk* __restrict__ n
     const bool cont = iter == 1 && ins->keep_dir && ins->sweep > 1;
     const Ranges rg = ranges[nb.scene];
     const bool first = nb.first != 0;
-    // a few partials: every wave adds them all (no barrier); many: split over the block
-    const double den = rg.nt <= 256 ? wave_reduce(pden, rg.t0, rg.nt) : reduce2(pden, rg.t0, rg.nt, 0, 0, sh);
+    const double den = scene_sum<kBlock>(pden, rg.t0, rg.nt, sh);
     const bool exempt = (iter == 1 && ins->first_solve);
     if (den == 0.0) { record_stop(ins, first, 3, iter, ins->err); if (first && threadIdx.x == 0) ins->den = den; return; }
     if (fabs(den) <= ins->thr && !exempt) {

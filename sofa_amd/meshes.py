@@ -65,6 +65,28 @@ def split_hexa_6(hexa):
     return np.ascontiguousarray(hexa[:, _SPLIT6].reshape(-1, 4).astype(np.int32))
 
 
+_XEDGES = ((0, 1), (4, 5), (3, 2), (7, 6))
+_YEDGES = ((4, 7), (5, 6), (1, 2), (0, 3))
+_ZEDGES = ((4, 0), (5, 1), (6, 2), (7, 3))
+
+
+def split_hexa_fem(hexa, nx=2, ny=1):
+    """(H,8) -> (6H,4): TetrahedronFEMForceField::init's own tessellation of a hexahedral
+    topology (TetrahedronFEMForceField.inl:1318-1370). Cube i's corners are mirrored across x
+    when (i mod nx) is even, across y when floor(i/nx) mod ny is odd, across z when
+    floor(i/(nx ny)) is odd (edge pairs of sofa/geometry/Hexahedron.h:67-88), then split with
+    the 6-tetrahedra pattern. nx, ny: cubes along x and y (2 and 1 when the topology is not a
+    grid, as in the reference)."""
+    h = np.array(hexa, dtype=np.int64, copy=True)
+    i = np.arange(len(h))
+    for mask, edges in ((((i % nx) & 1) == 0, _XEDGES), ((((i // nx) % ny) & 1) == 1, _YEDGES),
+                        (((i // (nx * ny)) & 1) == 1, _ZEDGES)):
+        for a, b in edges:
+            ca, cb = h[mask, a].copy(), h[mask, b].copy()
+            h[mask, a], h[mask, b] = cb, ca
+    return np.ascontiguousarray(h[:, _SPLIT6].reshape(-1, 4).astype(np.int32))
+
+
 def box_roi(pos, box):
     """Indices of the points inside the box (xmin,ymin,zmin,xmax,ymax,zmax), bounds inclusive."""
     box = np.asarray(box, dtype=np.float64)

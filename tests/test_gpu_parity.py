@@ -762,6 +762,37 @@ def test_edge_scenes(prec):
         assert st["end_reason"] == 4, st
 
 
+def test_hexa_topology_tessellation():
+    """A hexahedral grid handed to the library (sofa_amd_scene_set_fem_hexa: the tessellation of
+    TetrahedronFEMForceField::init, TetrahedronFEMForceField.inl:1318-1370) steps exactly like the
+    same grid tessellated on the host (meshes.split_hexa_fem), and both match the oracle bit for
+    bit in FP64 (the GPU's dot order)."""
+    capi = _capi()
+    print("case: hexahedral topology, FP64, 20 steps", flush=True)
+    n = (4, 4, 9)
+    pos, hexa = meshes.regular_grid(n, (0.0, 0.0, 0.0), (1.0, 1.0, 2.0))
+    tets = meshes.split_hexa_fem(hexa, n[0] - 1, n[1] - 1)
+    fixed = meshes.box_roi(pos, (-0.01, -0.01, -0.01, 1.01, 1.01, 0.01))
+    c = dict(pos=pos, tets=tets, fixed=fixed, E=2e4, nu=0.4, total_mass=200.0, g=(0.0, -9.81, 0.0),
+             rs=0.1, rm=0.1, dt=0.01, iters=25, tol=1e-9, thr=1e-9)
+    eng = capi.Engine(0, capi.FP64)
+    a = add_gpu_scene(eng, c)
+    b = eng.add_scene(pos, hexa, [c["E"]], [c["nu"]], fixed, mass_kind=capi.MASS_UNIFORM,
+                      mass_value=c["total_mass"] / len(pos), gravity=c["g"], dt=c["dt"],
+                      rayleigh_stiffness=c["rs"], rayleigh_mass=c["rm"], cg_iterations=c["iters"],
+                      cg_tolerance=c["tol"], cg_threshold=c["thr"], hexa_grid=(n[0] - 1, n[1] - 1))
+    eng.commit()
+    eng.step([a, b], 20)
+    xa, va, _ = eng.read(a)
+    xb, vb, _ = eng.read(b)
+    eng.close()
+    assert np.array_equal(xa, xb) and np.array_equal(va, vb)
+    orc = make_oracle(c, dot_mode=1)
+    orc.step(c["dt"], 20)
+    assert np.array_equal(xa, orc.get("x")), rel_err(xa, orc.get("x"))
+    assert rel_err(xa, pos) > 1e-6
+
+
 def test_error_codes():
     """Invalid input is refused with the documented status and a message, nothing faults:
     out-of-range tetrahedron / fixed indices (EINVAL), a zero-volume tetrahedron at commit

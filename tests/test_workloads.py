@@ -168,3 +168,62 @@ def test_serving_scheduler_plain_and_refusal():
     srv.close()
     assert res["jobs_rejected"] == 1 and res["jobs_done"] == 5
     assert all(len(t) == 1 for t in eng.log)
+
+
+def test_fem_hexa_tessellation():
+    """TetrahedronFEMForceField::init's 6-tetrahedra tessellation of a grid (cube corners mirrored
+    by cube parity): conforming (every interior triangle in exactly two tetrahedra, every boundary
+    triangle in one and on the box), filling the box exactly, and neighbouring cubes cut as
+    mirror images of each other across their shared face, along x, y and z alike -- which a wrong
+    parity, edge pair or pattern breaks."""
+    from collections import Counter
+    from sofa_amd import meshes
+    n = (5, 4, 4)
+    L = (4.0, 3.0, 3.0)
+    pos, hexa = meshes.regular_grid(n, (0, 0, 0), L)
+    nx, ny, nz = n[0] - 1, n[1] - 1, n[2] - 1
+    tets = meshes.split_hexa_fem(hexa, nx, ny)
+    assert tets.shape == (6 * len(hexa), 4)
+    A = pos[tets]
+    vol = np.einsum("ij,ij->i", np.cross(A[:, 1] - A[:, 0], A[:, 2] - A[:, 0]), A[:, 3] - A[:, 0]) / 6.0
+    assert np.all(np.abs(vol) > 1e-12) and abs(np.abs(vol).sum() - np.prod(L)) < 1e-9
+    cnt = Counter()
+    for t in tets:
+        for f in ((0, 1, 2), (0, 1, 3), (0, 2, 3), (1, 2, 3)):
+            cnt[tuple(sorted(t[list(f)]))] += 1
+    for f, c in cnt.items():
+        P = pos[list(f)]
+        on_box = any(np.all(P[:, d] == v) for d in range(3) for v in (0.0, L[d]))
+        assert (c == 2 and not on_box) or (c == 1 and on_box), (f, c)
+
+    def gidx(ix, iy, iz):
+        return ix + n[0] * (iy + n[1] * iz)
+
+    coords = np.stack(np.unravel_index(np.arange(len(pos)), (n[2], n[1], n[0]))[::-1], axis=1)   # (ix, iy, iz)
+    cube_tets = [set(frozenset(t) for t in tets[6 * c:6 * c + 6]) for c in range(len(hexa))]
+    checked = 0
+    for c in range(len(hexa)):
+        cx, cy, cz = c % nx, (c // nx) % ny, c // (nx * ny)
+        for d, step in ((0, 1), (1, nx), (2, nx * ny)):
+            if (cx, cy, cz)[d] + 1 >= (nx, ny, nz)[d]:
+                continue
+            plane = (cx, cy, cz)[d] + 1                     # shared face at grid index `plane`
+            mapped = set()
+            for t in cube_tets[c + step]:
+                m = []
+                for v in t:
+                    q = list(coords[v])
+                    q[d] = 2 * plane - q[d]
+                    m.append(gidx(*q))
+                mapped.add(frozenset(m))
+            assert mapped == cube_tets[c], (c, d)
+            checked += 1
+    assert checked == 75
+    # absolute parity: cube 1 (odd x, even y and z) is cut by the plain 6-tetrahedra pattern,
+    # cube 0 (even x) by that pattern reflected across its own mid-plane x = 1/2
+    plain = meshes.split_hexa_6(hexa)
+    assert cube_tets[1] == set(frozenset(t) for t in plain[6:12])
+    refl = set()
+    for t in plain[0:6]:
+        refl.add(frozenset(gidx(1 - coords[v][0], coords[v][1], coords[v][2]) for v in t))
+    assert cube_tets[0] == refl
