@@ -286,6 +286,7 @@ struct InnerCg {
     const FinBlk* fb;
     const NodeBlk* nb;
     int na, nfb, nnb, maxit;
+    bool wide;                      // some scene has more than kWavePartials partial sums
     SharedNodes sn;
     float *r, *p, *q, *d, *sp;
     const float *x, *x0;            // float positions and rest positions (geometry recomputed)
@@ -354,6 +355,7 @@ struct EngineT : EngineBase {
         void *a = nullptr, *b = nullptr, *c = nullptr, *o = nullptr;
         int* scenes = nullptr;          // scene record indices
         int na = 0, nb = 0, nc = 0, no = 0, maxit = 0, maxsw = 0, n = 0;
+        int maxpart = 0;                // most partial sums the operator reduces for one scene
         hipGraphExec_t gexec = nullptr;
         ~LaunchSet() {
             if (gexec) (void)hipGraphExecDestroy(gexec);
@@ -1059,6 +1061,10 @@ struct EngineT : EngineBase {
                 na.insert(na.end(), pl.nblks.begin(), pl.nblks.end());
                 maxit = std::max(maxit, mixed ? scenes[id].inner_iter : scenes[id].cg_iter);
                 ls->maxsw = std::max(ls->maxsw, scenes[id].max_sweeps);
+                // the partial sums the operator kernel reduces: node blocks (r.r), and for |b|^2 the
+                // finalize blocks, or the tiles of a scene without shared nodes
+                const int bparts = pl.fins.empty() ? (int)pl.tiles.size() : (int)pl.fins.size();
+                ls->maxpart = std::max({ls->maxpart, bparts, (int)pl.nblks.size()});
             }
             put(ls->a, ta); put(ls->b, fa); put(ls->c, na);
             ls->na = (int)ta.size(); ls->nb = (int)fa.size(); ls->nc = (int)na.size();
@@ -1117,11 +1123,17 @@ struct EngineT : EngineBase {
     int apply_variant = 0;
     void launch_apply(int na, hipStream_t s, const TileArgs<R, NR>& ta, int it, const VT* rr, VT* pp, VT* qq, VT* sp,
                       const double* pin, double* pd, int pin_nodes) {
+        if (cur->maxpart > kWavePartials) launch_apply_t<true>(na, s, ta, it, rr, pp, qq, sp, pin, pd, pin_nodes);
+        else launch_apply_t<false>(na, s, ta, it, rr, pp, qq, sp, pin, pd, pin_nodes);
+    }
+    template <bool WIDE>
+    void launch_apply_t(int na, hipStream_t s, const TileArgs<R, NR>& ta, int it, const VT* rr, VT* pp, VT* qq, VT* sp,
+                        const double* pin, double* pd, int pin_nodes) {
         switch (apply_variant) {
-        case 1: k_tile_apply<R, NR, VT, 512, 1><<<na, 512, 0, s>>>(ta, it, rr, pp, qq, sp, pin, pd, pin_nodes, x, x0); break;
-        case 2: k_tile_apply<R, NR, VT, 256, 2><<<na, 256, 0, s>>>(ta, it, rr, pp, qq, sp, pin, pd, pin_nodes, x, x0); break;
-        case 3: k_tile_apply<R, NR, VT, 256, 1><<<na, 256, 0, s>>>(ta, it, rr, pp, qq, sp, pin, pd, pin_nodes, x, x0); break;
-        default: k_tile_apply<R, NR, VT, 512, 2><<<na, 512, 0, s>>>(ta, it, rr, pp, qq, sp, pin, pd, pin_nodes, x, x0); break;
+        case 1: k_tile_apply<R, NR, VT, 512, 1, WIDE><<<na, 512, 0, s>>>(ta, it, rr, pp, qq, sp, pin, pd, pin_nodes, x, x0); break;
+        case 2: k_tile_apply<R, NR, VT, 256, 2, WIDE><<<na, 256, 0, s>>>(ta, it, rr, pp, qq, sp, pin, pd, pin_nodes, x, x0); break;
+        case 3: k_tile_apply<R, NR, VT, 256, 1, WIDE><<<na, 256, 0, s>>>(ta, it, rr, pp, qq, sp, pin, pd, pin_nodes, x, x0); break;
+        default: k_tile_apply<R, NR, VT, 512, 2, WIDE><<<na, 512, 0, s>>>(ta, it, rr, pp, qq, sp, pin, pd, pin_nodes, x, x0); break;
         }
     }
     // times every variant on the engine's scenes (all of them in one launch, `reps` launches
@@ -1222,6 +1234,7 @@ struct EngineT : EngineBase {
         ic.x = x32; ic.x0 = x0_32;
         ic.fb = fb; ic.nb = nb;
         ic.na = cur->na; ic.nfb = cur->nb; ic.nnb = cur->nc; ic.maxit = cur->maxit;
+        ic.wide = cur->maxpart > kWavePartials;
         ic.sn = sn;
         ic.r = r32; ic.p = p32; ic.q = q32; ic.d = d32; ic.sp = sp32;
         ic.pden = pden32; ic.prho = prho32;
@@ -1416,8 +1429,14 @@ EngineBase* make_engine_fp32() {
 void launch_inner_cg(const InnerCg& c, hipStream_t s) {
     for (int it = 1; it <= c.maxit; it++) {
         if (c.na > 0)
-            k_tile_apply<float, 6, float, kTileThreads, 2><<<c.na, kTileThreads, 0, s>>>(c.ta, it, c.r, c.p, c.q, c.sp,
-                                                                                        c.prho, c.pden, 1, c.x, c.x0);
+        {
+            if (c.wide)
+                k_tile_apply<float, 6, float, kTileThreads, 2, true><<<c.na, kTileThreads, 0, s>>>(
+                    c.ta, it, c.r, c.p, c.q, c.sp, c.prho, c.pden, 1, c.x, c.x0);
+            else
+                k_tile_apply<float, 6, float, kTileThreads, 2, false><<<c.na, kTileThreads, 0, s>>>(
+                    c.ta, it, c.r, c.p, c.q, c.sp, c.prho, c.pden, 1, c.x, c.x0);
+        }
         if (c.nnb > 0)
             k_update<float><<<c.nnb, kBlock, 0, s>>>(c.nb, c.ta.inst, c.ta.ranges, it, c.sn, c.sp, c.ta.mass, c.ta.fixed,
                                                       c.p, c.q, c.d, c.r, c.pden, c.prho);

@@ -132,10 +132,13 @@ __device__ __forceinline__ double reduce2(const double* __restrict__ buf, int a0
 }
 
 // a scene's sum of na partials, the same bits in every block of the scene: few partials are
-// added by every wave on its own (no barrier), many are split over the block (two barriers)
-template <int TB>
+// added by every wave on its own (no barrier), many are split over the block (two barriers).
+// WIDE = false: the launch has no scene of more than 256 partials (the block path compiled out)
+constexpr int kWavePartials = 256;
+template <int TB, bool WIDE = true>
 __device__ __forceinline__ double scene_sum(const double* __restrict__ buf, int a0, int na, double* sh) {
-    return na <= 256 ? wave_reduce(buf, a0, na) : reduce2<TB>(buf, a0, na, 0, 0, sh);
+    if (!WIDE || na <= kWavePartials) return wave_reduce(buf, a0, na);
+    return reduce2<TB>(buf, a0, na, 0, 0, sh);
 }
 
 template <typename VT>
@@ -451,7 +454,7 @@ __global__ __launch_bounds__(kBlock) void k_fin_rhs(const FinBlk* __restrict__ f
 // less per element, 3 more per tile-local node.  2: also the rotated rest coordinates from the
 // rest positions x0 (initLarge, TetrahedronFEMForceField.inl:835-868) and, for a scene of one
 // material, K from the rest volume -- NR (+ 3) reals less per element, 3 more per local node.
-template <typename R, int NR, typename VT, int TB = kBlock, int GEO = 0>
+template <typename R, int NR, typename VT, int TB = kBlock, int GEO = 0, bool WIDE = true>
 __global__ __launch_bounds__(TB) void k_tile_apply(TileArgs<R, NR> a, int iter, const VT* __restrict__ r,
                                                     VT* __restrict__ p, VT* __restrict__ q, VT* __restrict__ sp,
                                                     const double* __restrict__ pin,   // |b|^2 (iter 1) or r.r partials
@@ -533,7 +536,7 @@ __global__ __launch_bounds__(TB) void k_tile_apply(TileArgs<R, NR> a, int iter, 
     if (cont) {
         // the refinement replaced r by the true residual: beta from its r.r and the r.r that
         // built p; the tolerance is relative to the first sweep's |r|
-        const double rho = scene_sum<TB>(pin, rg.b0, rg.nb, sh);
+        const double rho = scene_sum<TB, WIDE>(pin, rg.b0, rg.nb, sh);
         const double err = sqrt(rho) / ins->normb;
         if (rho == 0.0 || err <= ins->tol) { record_stop(ins, first, 1, 1, err); return; }
         beta = rho / ins->rho_last;
@@ -541,8 +544,8 @@ __global__ __launch_bounds__(TB) void k_tile_apply(TileArgs<R, NR> a, int iter, 
             ins->rho_slot[1] = rho; ins->beta = beta; ins->iter = 1; ins->err = err;
         }
     } else if (iter == 1) {
-        const double bb = pin_nodes ? scene_sum<TB>(pin, rg.b0, rg.nb, sh)
-                                    : rg.nf > 0 ? scene_sum<TB>(pin, rg.f0, rg.nf, sh) : scene_sum<TB>(pin, rg.t0, rg.nt, sh);
+        const double bb = pin_nodes ? scene_sum<TB, WIDE>(pin, rg.b0, rg.nb, sh)
+                                    : rg.nf > 0 ? scene_sum<TB, WIDE>(pin, rg.f0, rg.nf, sh) : scene_sum<TB, WIDE>(pin, rg.t0, rg.nt, sh);
         const double normb = sqrt(bb);
         if (normb == 0.0) { record_stop(ins, first, 4, 0, 1.0); return; }
         const double err = sqrt(bb) / normb;
@@ -552,7 +555,7 @@ __global__ __launch_bounds__(TB) void k_tile_apply(TileArgs<R, NR> a, int iter, 
         }
     } else if (iter > 1) {
         if (iter > ins->max_iter) { record_stop(ins, first, 0, ins->max_iter, ins->err); return; }
-        const double rho = scene_sum<TB>(pin, rg.b0, rg.nb, sh);
+        const double rho = scene_sum<TB, WIDE>(pin, rg.b0, rg.nb, sh);
         const double err = sqrt(rho) / ins->normb;
         if (err <= ins->tol) { record_stop(ins, first, 1, iter, err); return; }
         const double rho_1 = ins->rho_slot[(iter - 1) & 1];
