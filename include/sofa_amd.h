@@ -44,6 +44,10 @@ extern "C" {
                                                vertex and edge terms (MeshMatrixMass.inl:547-657,
                                                1987-2092)                                          */
 #define SOFA_AMD_MASS_MESHMATRIX_LUMPED  3  /* MeshMatrixMass, lumping = true (coefficient 2.5)      */
+#define SOFA_AMD_MASS_DIAGONAL_TOTAL     4  /* DiagonalMass given its totalMass: vertex masses for
+                                               density 1, then scaled by totalMass / their sum
+                                               (DiagonalMass.inl:1235-1257); the mass value is the
+                                               total mass                                          */
 
 typedef struct sofa_amd_engine_s* sofa_amd_engine;
 
@@ -127,6 +131,15 @@ int sofa_amd_scene_set_fix_all(sofa_amd_engine eng, int scene, int fix_all);
  * with fixAll) are cleared -- DefaultAnimationLoop's projectPositionAndVelocity
  * (DefaultAnimationLoop.cpp:228-234), applied before the oscillator's projection. */
 int sofa_amd_scene_set_project_velocity(sofa_amd_engine eng, int scene, int on);
+
+/* PlaneForceField (PlaneForceField.inl:86-224), applied after the FEM in scene order: every node
+ * below the plane x.normal = d (either side if bilateral) gets the force
+ * normal (-stiffness dist) + v (damping dist), dist = x.normal - d, clamped to max_force when
+ * max_force > 0; its stiffness -stiffness normal normal^T enters the implicit system for the
+ * nodes in contact at the step's force evaluation (the damping does not). One plane per scene;
+ * the reference's defaults: normal (0, 1, 0), d 0, stiffness 500, damping 5, max_force 0. */
+int sofa_amd_scene_add_plane(sofa_amd_engine eng, int scene, const double* normal, double d, double stiffness,
+                             double damping, double max_force, int bilateral);
 
 /* EulerImplicitSolver rayleighStiffness / rayleighMass (EulerImplicitSolver.cpp:40-51) and
  * CGLinearSolver iterations / tolerance / threshold (CGLinearSolver.inl:36-40), dt. */

@@ -535,12 +535,23 @@ struct UniformMass : public MassBase {
 // DiagonalMass<Vec3d> with a tetrahedral topology (DiagonalMass.inl)
 struct DiagonalMass : public MassBase {
     std::vector<Real> masses;
-    void init(const std::vector<Vec3>& pos, const std::vector<Tetra>& tets, Real density) {
-        masses.assign(pos.size(), 0.0);                  // :1060-1080
+    // computeVertexMass (:1060-1080); with total = true, `value` is totalMass and the masses are
+    // computed for density 1 then scaled by totalMass / their sum (initFromTotalMass, :1235-1257)
+    void init(const std::vector<Vec3>& pos, const std::vector<Tetra>& tets, Real value, bool total = false) {
+        masses.assign(pos.size(), 0.0);
+        const Real density = total ? Real(1.0) : value;
+        Real sum = 0;
         for (const Tetra& t : tets) {
             const Real vol = tet_volume(pos[t[0]], pos[t[1]], pos[t[2]], pos[t[3]]);
             const Real mass = (density * vol) / Real(4.0);
-            for (int j = 0; j < 4; j++) masses[t[j]] += mass;
+            for (int j = 0; j < 4; j++) {
+                masses[t[j]] += mass;
+                sum += mass;
+            }
+        }
+        if (total) {
+            const Real d = sum < std::numeric_limits<Real>::epsilon() ? Real(1.0) : Real(value / sum);
+            for (Real& m : masses) m *= d;
         }
     }
     void addMDx(std::vector<Vec3>& res, const std::vector<Vec3>& dx, Real factor) override {
@@ -640,6 +651,45 @@ struct AxisSpring : public ForceField {
     }
     void addMBKdx(std::vector<Vec3>& df, const std::vector<Vec3>& dx, Real, Real, Real kFact) override {
         if (kFact != 0.0) df[node][axis] += -stiffness * dx[node][axis] * kFact;
+    }
+};
+
+// PlaneForceField<Vec3d> (PlaneForceField.inl:158-224): a penalty force on the nodes below the
+// plane x.n = d (both sides if bilateral), recorded as contacts for addDForce
+struct PlaneForceField : public ForceField {
+    Vec3 normal = Vec3(0, 1, 0);
+    Real d = 0, stiffness = 500, damping = 5, maxForce = 0;
+    bool bilateral = false;
+    std::vector<int> contacts;
+    void addForce(std::vector<Vec3>& f, const std::vector<Vec3>& x, const std::vector<Vec3>& v) override {
+        contacts.clear();
+        Real limit = maxForce;
+        limit *= limit;
+        for (size_t i = 0; i < x.size(); i++) {
+            const Real dist = dot3(x[i], normal) - d;
+            if (bilateral || dist < 0) {
+                const Real forceIntensity = -stiffness * dist;
+                const Real dampingIntensity = -damping * dist;
+                Vec3 force;
+                for (int k = 0; k < 3; k++) force[k] = normal[k] * forceIntensity - v[i][k] * dampingIntensity;
+                const Real amplitude = dot3(force, force);
+                if (limit > 0 && amplitude > limit) {
+                    const Real sc = std::sqrt(limit / amplitude);
+                    for (int k = 0; k < 3; k++) force[k] *= sc;
+                }
+                for (int k = 0; k < 3; k++) f[i][k] += force[k];
+                contacts.push_back((int)i);
+            }
+        }
+    }
+    void addMBKdx(std::vector<Vec3>& df, const std::vector<Vec3>& dx, Real, Real bFact, Real kFact) override {
+        // BaseForceField.cpp:38-47 (rayleighStiffness of the component: 0), then addDForce
+        if (kFact == 0.0 && bFact == 0.0) return;
+        const Real fact = -stiffness * kFact;
+        for (int p : contacts) {
+            const Real a = fact * dot3(dx[p], normal);
+            for (int k = 0; k < 3; k++) df[p][k] = df[p][k] + normal[k] * a;
+        }
     }
 };
 
@@ -926,6 +976,24 @@ void orc_set_diagonal_mass(void* h, double density) {
     Scene* s = static_cast<Scene*>(h);
     DiagonalMass* m = new DiagonalMass();
     m->init(s->x0, s->fem ? s->fem->elems : std::vector<Tetra>(), density);
+    m->gravity = s->gravity;
+    s->mass.reset(m);
+}
+
+void orc_add_plane(void* h, const double* normal, double d, double stiffness, double damping, double max_force,
+                   int bilateral) {
+    Scene* s = static_cast<Scene*>(h);
+    PlaneForceField* pf = new PlaneForceField();
+    pf->normal = Vec3(normal[0], normal[1], normal[2]);
+    pf->d = d; pf->stiffness = stiffness; pf->damping = damping; pf->maxForce = max_force;
+    pf->bilateral = bilateral != 0;
+    s->extra.emplace_back(pf);
+}
+
+void orc_set_diagonal_total_mass(void* h, double total_mass) {
+    Scene* s = static_cast<Scene*>(h);
+    DiagonalMass* m = new DiagonalMass();
+    m->init(s->x0, s->fem ? s->fem->elems : std::vector<Tetra>(), total_mass, true);
     m->gravity = s->gravity;
     s->mass.reset(m);
 }

@@ -38,6 +38,8 @@ def lib():
         L.orc_set_uniform_mass.argtypes = [ctypes.c_void_p, ctypes.c_double]
         L.orc_set_uniform_total_mass.argtypes = [ctypes.c_void_p, ctypes.c_double]
         L.orc_set_diagonal_mass.argtypes = [ctypes.c_void_p, ctypes.c_double]
+        L.orc_set_diagonal_total_mass.argtypes = [ctypes.c_void_p, ctypes.c_double]
+        L.orc_add_plane.argtypes = [ctypes.c_void_p, _dp, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int]
         L.orc_add_axis_spring.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double]
         L.orc_set_fixed.argtypes = [ctypes.c_void_p, _ip, ctypes.c_int, ctypes.c_int]
         L.orc_add_oscillator.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp, ctypes.c_double, ctypes.c_double]
@@ -109,6 +111,16 @@ class OracleScene:
 
     def set_diagonal_mass(self, density):
         lib().orc_set_diagonal_mass(self._h, float(density))
+
+    def add_plane(self, normal=(0.0, 1.0, 0.0), d=0.0, stiffness=500.0, damping=5.0, max_force=0.0, bilateral=False):
+        """PlaneForceField (after the FEM in scene order)."""
+        n, np_ = _d(np.asarray(normal, dtype=np.float64))
+        lib().orc_add_plane(self._h, np_, float(d), float(stiffness), float(damping), float(max_force),
+                            1 if bilateral else 0)
+
+    def set_diagonal_total_mass(self, total_mass):
+        """DiagonalMass given totalMass (masses for density 1, scaled to the total)."""
+        lib().orc_set_diagonal_total_mass(self._h, float(total_mass))
 
     def set_meshmatrix_mass(self, density, lumping=False):
         d, dp = _d(np.atleast_1d(density))

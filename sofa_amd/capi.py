@@ -15,7 +15,7 @@ LIB_PATH = os.path.join(HERE, "libsofa_amd.so")
 
 OK, EINVAL, EDEVICE, ENOMEM, EDEGENERATE, ESTATE, ENAN = 0, -1, -2, -3, -4, -5, -6
 FP64, FP32, MIXED, FP64_TILED = 0, 1, 2, 3
-MASS_UNIFORM, MASS_DIAGONAL, MASS_MESHMATRIX, MASS_MESHMATRIX_LUMPED = 0, 1, 2, 3
+MASS_UNIFORM, MASS_DIAGONAL, MASS_MESHMATRIX, MASS_MESHMATRIX_LUMPED, MASS_DIAGONAL_TOTAL = 0, 1, 2, 3, 4
 END_REASONS = {0: "iterations", 1: "tolerance", 2: "threshold", 3: "den==0", 4: "null norm of vector b"}
 
 # every symbol include/sofa_amd.h declares
@@ -28,7 +28,7 @@ EXPORTS = [
     "sofa_amd_fem_add_force", "sofa_amd_fem_add_dforce", "sofa_amd_engine_bench_apply", "sofa_amd_scene_tiling",
     "sofa_amd_engine_set_graphs", "sofa_amd_scene_set_mass_density", "sofa_amd_scene_add_oscillators",
     "sofa_amd_scene_set_refinement", "sofa_amd_engine_reserve", "sofa_amd_scene_commit", "sofa_amd_scene_release",
-    "sofa_amd_engine_usage", "sofa_amd_scene_set_fix_all", "sofa_amd_scene_set_integrator", "sofa_amd_scene_set_fem_hexa", "sofa_amd_scene_set_project_velocity", "sofa_amd_engine_set_dot_order",
+    "sofa_amd_engine_usage", "sofa_amd_scene_set_fix_all", "sofa_amd_scene_set_integrator", "sofa_amd_scene_set_fem_hexa", "sofa_amd_scene_add_plane", "sofa_amd_scene_set_project_velocity", "sofa_amd_engine_set_dot_order",
     "sofa_amd_scene_prepare", "sofa_amd_engine_autotune", "sofa_amd_engine_set_apply_variant",
 ]
 
@@ -96,6 +96,7 @@ def lib():
         "sofa_amd_scene_set_fix_all": (c_int, [_vp, c_int, c_int]),
         "sofa_amd_scene_set_integrator": (c_int, [_vp, c_int, c_int, c_int, c_double]),
         "sofa_amd_scene_set_fem_hexa": (c_int, [_vp, c_int, c_int, _ip, c_int, c_int, _dp, c_int, _dp, c_int]),
+        "sofa_amd_scene_add_plane": (c_int, [_vp, c_int, _dp, c_double, c_double, c_double, c_double, c_int]),
         "sofa_amd_scene_set_project_velocity": (c_int, [_vp, c_int, c_int]),
         "sofa_amd_engine_set_dot_order": (c_int, [_vp, c_int]),
         "sofa_amd_scene_prepare": (c_int, [_vp, c_int]),
@@ -236,6 +237,13 @@ class Engine:
 
     def sync(self):
         check(lib().sofa_amd_engine_sync(self._h))
+
+    def add_plane(self, scene, normal=(0.0, 1.0, 0.0), d=0.0, stiffness=500.0, damping=5.0, max_force=0.0,
+                  bilateral=False):
+        """PlaneForceField (one per scene)."""
+        n, np_ = _d(np.asarray(normal, dtype=np.float64))
+        check(lib().sofa_amd_scene_add_plane(self._h, int(scene), np_, float(d), float(stiffness), float(damping),
+                                             float(max_force), 1 if bilateral else 0))
 
     def set_project_velocity(self, scene, on=True):
         """FixedProjectiveConstraint activate_projectVelocity."""

@@ -154,11 +154,14 @@ struct SceneHost {
     std::vector<int32_t> tets;              // 4E
     std::vector<double> young{5000.0}, poisson{0.45};
     int mass_kind = SOFA_AMD_MASS_UNIFORM;
+    bool mass_total = false;                // DiagonalMass: mass_value is the total mass
     double mass_value = 1.0;                // vertexMass or massDensity
     double g[3] = {0, 0, 0};
     std::vector<int32_t> fixed{0};          // FixedProjectiveConstraint default indices = {0}
     bool fix_all = false;                   // FixedProjectiveConstraint fixAll
     bool project_velocity = false;          // FixedProjectiveConstraint activate_projectVelocity
+    bool plane_on = false, plane_bilateral = false;   // PlaneForceField
+    double plane_n[3] = {0, 1, 0}, plane_d = 0, plane_k = 500, plane_damp = 5, plane_max = 0;
     std::vector<double> density{1.0};       // MeshMatrixMass massDensity (one or per element)
     // OscillatorProjectiveConstraint entries
     std::vector<int32_t> osc_idx;
@@ -287,6 +290,7 @@ struct InnerCg {
     const NodeBlk* nb;
     int na, nfb, nnb, maxit;
     bool wide;                      // some scene has more than kWavePartials partial sums
+    bool plane;                     // some scene has a PlaneForceField
     SharedNodes sn;
     float *r, *p, *q, *d, *sp;
     const float *x, *x0;            // float positions and rest positions (geometry recomputed)
@@ -304,6 +308,7 @@ struct EngineT : EngineBase {
     VT* x0 = nullptr;                              // rest positions (tiled path)
     double* mass = nullptr;
     int8_t* fixed = nullptr;
+    int8_t* contact = nullptr;                     // PlaneForceField contacts per node
     int *inc_ptr = nullptr, *inc = nullptr;        // node -> (position*4 + vertex), ascending element id
     // element record (SoA, in tile order on the tiled path, else in element order)
     int4* tet = nullptr;
@@ -356,6 +361,7 @@ struct EngineT : EngineBase {
         int* scenes = nullptr;          // scene record indices
         int na = 0, nb = 0, nc = 0, no = 0, maxit = 0, maxsw = 0, n = 0;
         int maxpart = 0;                // most partial sums the operator reduces for one scene
+        bool plane = false;             // some scene has a PlaneForceField
         hipGraphExec_t gexec = nullptr;
         ~LaunchSet() {
             if (gexec) (void)hipGraphExecDestroy(gexec);
@@ -375,7 +381,7 @@ struct EngineT : EngineBase {
                         inst, partial, tiles_d, ranges_d, ltet, ln_gid, ln_inc_ptr, ln_pslot, sh_gid, sh_ptr,
                         ln_inc, sp0, sp1, pbb, pden, prho, p2, em_ptr, em_nbr, em_val, emass, osc_node,
                         osc_mean, osc_amp, osc_w, osc_phase, kmat32, emass32, r32, p32, q32,
-                        d32, sp32, pden32, prho32, prr, inst_in, sh_of, nd, x0, x32, x0_32};
+                        d32, sp32, pden32, prho32, prr, inst_in, sh_of, nd, x0, x32, x0_32, contact};
         for (void* ptr : ptrs)
             if (ptr) (void)hipFree(ptr);
     }
@@ -383,7 +389,7 @@ struct EngineT : EngineBase {
     bool live() const override { return tiled; }
 
     ElemView<R, NR> view() const { return ElemView<R, NR>{tet, rot, rest, kmat, Etot}; }
-    NodeArrays nodes() const { return NodeArrays{inc_ptr, inc, mass, fixed, em_ptr, em_nbr, em_val}; }
+    NodeArrays nodes() const { return NodeArrays{inc_ptr, inc, mass, fixed, em_ptr, em_nbr, em_val, contact}; }
     Oscillators oscillators() const { return Oscillators{osc_node, osc_mean, osc_amp, osc_w, osc_phase}; }
     int node_begin(int scene) const { return (int)place[scene].at.node; }
     int elem_begin(int scene) const { return (int)place[scene].at.elem; }
@@ -472,6 +478,7 @@ struct EngineT : EngineBase {
         if (mixed) { h.k_lo.assign(3 * (size_t)nE, 0.f); h.em_lo.assign(nE, 0.f); }
         h.mass.assign(n, 0.0);
         std::vector<int> cnt(n + 1, 0);
+        double mass_sum = 0.0;                       // DiagonalMass computeVertexMass total
         for (int e = 0; e < nE; e++) {
             const int32_t* t = &s.tets[4 * (size_t)e];
             for (int k = 0; k < 4; k++)
@@ -495,8 +502,12 @@ struct EngineT : EngineBase {
                 for (int k = 0; k < 3; k++) h.k_lo[k * (size_t)nE + pe] = (float)ei.k[k];
             for (int k = 0; k < 4; k++) cnt[t[k] + 1]++;
             if (s.mass_kind == SOFA_AMD_MASS_DIAGONAL) {
-                const double m = (s.mass_value * ei.vol) / 4.0;     // DiagonalMass.inl:1075-1080
-                for (int k = 0; k < 4; k++) h.mass[t[k]] += m;
+                // DiagonalMass.inl:1075-1080; given the total mass, first with density 1
+                const double m = ((s.mass_total ? 1.0 : s.mass_value) * ei.vol) / 4.0;
+                for (int k = 0; k < 4; k++) {
+                    h.mass[t[k]] += m;
+                    mass_sum += m;
+                }
             } else if (s.mass_kind == SOFA_AMD_MASS_MESHMATRIX || s.mass_kind == SOFA_AMD_MASS_MESHMATRIX_LUMPED) {
                 const double rho = s.density.size() > (size_t)e ? s.density[e] : s.density[0];
                 const double m = (rho * ei.vol) / 10.0;              // MeshMatrixMass.inl:584
@@ -506,6 +517,12 @@ struct EngineT : EngineBase {
                     if (mixed) h.em_lo[pe] = (float)((rho * ei.vol) / 20.0);
                 }
             }
+        }
+        // DiagonalMass::initFromTotalMass (DiagonalMass.inl:1235-1257): density = total / sum
+        // (1 when the sum is below epsilon), every vertex mass times it
+        if (s.mass_kind == SOFA_AMD_MASS_DIAGONAL && s.mass_total) {
+            const double density = mass_sum < std::numeric_limits<double>::epsilon() ? 1.0 : s.mass_value / mass_sum;
+            for (int i = 0; i < n; i++) h.mass[i] *= density;
         }
         for (int i = 0; i < n; i++) cnt[i + 1] += cnt[i];
         h.inc_ptr = cnt;
@@ -568,6 +585,10 @@ struct EngineT : EngineBase {
         }
         in.first_order = s.first_order ? 1 : 0;
         in.project_velocity = s.project_velocity ? 1 : 0;
+        in.plane_on = s.plane_on ? 1 : 0;
+        in.plane_bilateral = s.plane_bilateral ? 1 : 0;
+        for (int k = 0; k < 3; k++) in.plane_n[k] = s.plane_n[k];
+        in.plane_d = s.plane_d; in.plane_k = s.plane_k; in.plane_damp = s.plane_damp; in.plane_max = s.plane_max;
         in.vdamp = s.vdamping != 0.0 ? std::exp(-h * s.vdamping) : 1.0;
         in.tol = s.tol < 0 ? 1e-5 : s.tol;
         in.thr = s.thr < 0 ? 1e-5 : s.thr;
@@ -684,6 +705,8 @@ struct EngineT : EngineBase {
         for (int64_t i = 0; i < 3 * Ntot; i++) xv[i] = (VT)xh[i];
         HIPCHK(hipMemcpy(x, xv.data(), xv.size() * sizeof(VT), hipMemcpyHostToDevice));
         for (VT* vec : {v, f, b, r, p, q, xcg}) HIPCHK(hipMemset(vec, 0, 3 * Ntot * sizeof(VT)));
+        contact = dalloc<int8_t>(Ntot);
+        HIPCHK(hipMemset(contact, 0, std::max<int64_t>(1, Ntot)));
         upload(mass, massh); upload(fixed, fixh);
         upload(inc_ptr, cnt); upload(inc, inch);
         upload(tet, teth); upload(rot, roth); upload(rest, resth); upload(kmat, kh);
@@ -751,7 +774,7 @@ struct EngineT : EngineBase {
         sets.clear();
         cur = nullptr;
         free_device();
-        x = v = f = b = r = p = q = xcg = x0 = nullptr; mass = nullptr; fixed = nullptr; inc_ptr = inc = nullptr;
+        x = v = f = b = r = p = q = xcg = x0 = nullptr; mass = nullptr; fixed = nullptr; contact = nullptr; inc_ptr = inc = nullptr;
         tet = nullptr; rot = rest = kmat = nullptr; ef = ef2 = nullptr; p2 = nullptr; em_ptr = em_nbr = nullptr;
         em_val = nullptr; emass = nullptr; osc_node = nullptr; osc_mean = osc_amp = osc_w = osc_phase = nullptr;
         inst = nullptr; partial = nullptr; tiles_d = nullptr; ranges_d = nullptr; ltet = nullptr;
@@ -770,6 +793,8 @@ struct EngineT : EngineBase {
             HIPCHK(hipMemset(*vec, 0, 3 * N * sizeof(VT)));
         }
         mass = dalloc<double>(N); fixed = dalloc<int8_t>(N); inc_ptr = dalloc<int>(N); sh_of = dalloc<int>(N);
+        contact = dalloc<int8_t>(N);
+        HIPCHK(hipMemset(contact, 0, std::max<int64_t>(1, N)));
         tet = dalloc<int4>(E); rot = dalloc<R>(9 * E); rest = dalloc<R>(NR * E); kmat = dalloc<R>(3 * E);
         emass = dalloc<R>(E); ltet = dalloc<uint2>(E); ef = dalloc<VT>(12 * E);
         inc = dalloc<int>(cap.inc); ln_inc = dalloc<uint16_t>(cap.inc);
@@ -1065,6 +1090,7 @@ struct EngineT : EngineBase {
                 // finalize blocks, or the tiles of a scene without shared nodes
                 const int bparts = pl.fins.empty() ? (int)pl.tiles.size() : (int)pl.fins.size();
                 ls->maxpart = std::max({ls->maxpart, bparts, (int)pl.nblks.size()});
+                ls->plane = ls->plane || scenes[id].plane_on;
             }
             put(ls->a, ta); put(ls->b, fa); put(ls->c, na);
             ls->na = (int)ta.size(); ls->nb = (int)fa.size(); ls->nc = (int)na.size();
@@ -1098,7 +1124,7 @@ struct EngineT : EngineBase {
         const Blk* nb = (const Blk*)cur->b;
         if (cur->na > 0) k_rhs_elem<R, NR, VT><<<cur->na, kBlock, 0, s>>>(ev, eb, inst, x, v, ef, ef2);
         double* ndp = seq_dots ? nd : nullptr;
-        if (cur->nb > 0) k_rhs_node<VT><<<cur->nb, kBlock, 0, s>>>(nb, inst, na, ef, ef2, v, f, b, r, xcg, partial, ndp);
+        if (cur->nb > 0) k_rhs_node<VT><<<cur->nb, kBlock, 0, s>>>(nb, inst, na, ef, ef2, v, f, b, r, xcg, partial, ndp, x);
         for (int it = 1; it <= cur->maxit; it++) {
             // direction of iteration it in buffer it & 1, the previous one in the other
             VT* p_new = (it & 1) ? p : p2;
@@ -1123,17 +1149,23 @@ struct EngineT : EngineBase {
     int apply_variant = 0;
     void launch_apply(int na, hipStream_t s, const TileArgs<R, NR>& ta, int it, const VT* rr, VT* pp, VT* qq, VT* sp,
                       const double* pin, double* pd, int pin_nodes) {
-        if (cur->maxpart > kWavePartials) launch_apply_t<true>(na, s, ta, it, rr, pp, qq, sp, pin, pd, pin_nodes);
-        else launch_apply_t<false>(na, s, ta, it, rr, pp, qq, sp, pin, pd, pin_nodes);
+        const bool wide = cur->maxpart > kWavePartials;
+        if (cur->plane) {
+            if (wide) launch_apply_t<true, true>(na, s, ta, it, rr, pp, qq, sp, pin, pd, pin_nodes);
+            else launch_apply_t<false, true>(na, s, ta, it, rr, pp, qq, sp, pin, pd, pin_nodes);
+        } else {
+            if (wide) launch_apply_t<true, false>(na, s, ta, it, rr, pp, qq, sp, pin, pd, pin_nodes);
+            else launch_apply_t<false, false>(na, s, ta, it, rr, pp, qq, sp, pin, pd, pin_nodes);
+        }
     }
-    template <bool WIDE>
+    template <bool WIDE, bool PLANE>
     void launch_apply_t(int na, hipStream_t s, const TileArgs<R, NR>& ta, int it, const VT* rr, VT* pp, VT* qq, VT* sp,
                         const double* pin, double* pd, int pin_nodes) {
         switch (apply_variant) {
-        case 1: k_tile_apply<R, NR, VT, 512, 1, WIDE><<<na, 512, 0, s>>>(ta, it, rr, pp, qq, sp, pin, pd, pin_nodes, x, x0); break;
-        case 2: k_tile_apply<R, NR, VT, 256, 2, WIDE><<<na, 256, 0, s>>>(ta, it, rr, pp, qq, sp, pin, pd, pin_nodes, x, x0); break;
-        case 3: k_tile_apply<R, NR, VT, 256, 1, WIDE><<<na, 256, 0, s>>>(ta, it, rr, pp, qq, sp, pin, pd, pin_nodes, x, x0); break;
-        default: k_tile_apply<R, NR, VT, 512, 2, WIDE><<<na, 512, 0, s>>>(ta, it, rr, pp, qq, sp, pin, pd, pin_nodes, x, x0); break;
+        case 1: k_tile_apply<R, NR, VT, 512, 1, WIDE, PLANE><<<na, 512, 0, s>>>(ta, it, rr, pp, qq, sp, pin, pd, pin_nodes, x, x0); break;
+        case 2: k_tile_apply<R, NR, VT, 256, 2, WIDE, PLANE><<<na, 256, 0, s>>>(ta, it, rr, pp, qq, sp, pin, pd, pin_nodes, x, x0); break;
+        case 3: k_tile_apply<R, NR, VT, 256, 1, WIDE, PLANE><<<na, 256, 0, s>>>(ta, it, rr, pp, qq, sp, pin, pd, pin_nodes, x, x0); break;
+        default: k_tile_apply<R, NR, VT, 512, 2, WIDE, PLANE><<<na, 512, 0, s>>>(ta, it, rr, pp, qq, sp, pin, pd, pin_nodes, x, x0); break;
         }
     }
     // times every variant on the engine's scenes (all of them in one launch, `reps` launches
@@ -1188,6 +1220,7 @@ struct EngineT : EngineBase {
         ta.tn = TileNodes{ln_gid, ln_inc_ptr, ln_inc, ln_pslot};
         ta.mass = mass;
         ta.fixed = fixed;
+        ta.contact = contact;
         ta.emass = emass;
         ta.x_lo = nullptr;
         return ta;
@@ -1199,12 +1232,12 @@ struct EngineT : EngineBase {
         const NodeBlk* nb = (const NodeBlk*)cur->c;
         const SharedNodes sn{sh_gid, sh_ptr, sh_of};
         if (cur->na > 0) k_tile_rhs<R, NR, VT, 512><<<cur->na, 512, 0, s>>>(ta, x, v, f, b, r, xcg, sp0, sp1, pbb);
-        if (cur->nb > 0) k_fin_rhs<VT><<<cur->nb, kBlock, 0, s>>>(fb, inst, ranges_d, sn, mass, fixed, v, sp0, sp1, f, b, r, xcg, pbb);
+        if (cur->nb > 0) k_fin_rhs<VT><<<cur->nb, kBlock, 0, s>>>(fb, inst, ranges_d, sn, mass, fixed, v, sp0, sp1, f, b, r, xcg, pbb, x, contact);
         for (int it = 1; it <= cur->maxit; it++) {
             if (cur->na > 0)
                 launch_apply(cur->na, s, ta, it, r, p, q, sp0, it == 1 ? pbb : prho, pden, 0);
             if (cur->nc > 0)
-                k_update<VT><<<cur->nc, kBlock, 0, s>>>(nb, inst, ranges_d, it, sn, sp0, mass, fixed, p, q, xcg, r, pden, prho);
+                k_update<VT><<<cur->nc, kBlock, 0, s>>>(nb, inst, ranges_d, it, sn, sp0, mass, fixed, p, q, xcg, r, pden, prho, contact);
         }
         if (cur->nc > 0) k_integrate<VT><<<cur->nc, kBlock, 0, s>>>(nb, inst, fixed, xcg, v, x);
         if (cur->no > 0) k_oscillate<VT><<<cur->no, kBlock, 0, s>>>((const Blk*)cur->o, inst, oscillators(), x, v);
@@ -1229,23 +1262,25 @@ struct EngineT : EngineBase {
         ic.ta.tn = ta.tn;
         ic.ta.mass = mass;
         ic.ta.fixed = fixed;
+        ic.ta.contact = contact;
         ic.ta.emass = emass32;
         ic.ta.x_lo = nullptr;
         ic.x = x32; ic.x0 = x0_32;
         ic.fb = fb; ic.nb = nb;
         ic.na = cur->na; ic.nfb = cur->nb; ic.nnb = cur->nc; ic.maxit = cur->maxit;
         ic.wide = cur->maxpart > kWavePartials;
+        ic.plane = cur->plane;
         ic.sn = sn;
         ic.r = r32; ic.p = p32; ic.q = q32; ic.d = d32; ic.sp = sp32;
         ic.pden = pden32; ic.prho = prho32;
         if (cur->na > 0) k_tile_rhs<R, NR, VT, 512><<<cur->na, 512, 0, s>>>(ta, x, v, f, b, r, xcg, sp0, sp1, pbb);
-        if (cur->nb > 0) k_fin_rhs<VT><<<cur->nb, kBlock, 0, s>>>(fb, inst, ranges_d, sn, mass, fixed, v, sp0, sp1, f, b, r, xcg, pbb);
+        if (cur->nb > 0) k_fin_rhs<VT><<<cur->nb, kBlock, 0, s>>>(fb, inst, ranges_d, sn, mass, fixed, v, sp0, sp1, f, b, r, xcg, pbb, x, contact);
         for (int sw = 1; sw <= cur->maxsw + 1; sw++) {
             if (sw > 1) {   // q = A x, FP64
                 if (cur->na > 0)
                     launch_apply(cur->na, s, ta, 0, xcg, nullptr, q, sp0, nullptr, pden, 0);
                 if (cur->nb > 0)
-                    k_fin_apply<VT><<<cur->nb, kBlock, 0, s>>>(fb, inst, ranges_d, sn, mass, fixed, 0, xcg, nullptr, q, sp0, pden);
+                    k_fin_apply<VT><<<cur->nb, kBlock, 0, s>>>(fb, inst, ranges_d, sn, mass, fixed, 0, xcg, nullptr, q, sp0, pden, contact);
             }
             if (cur->nc > 0) k_ir_residual<<<cur->nc, kBlock, 0, s>>>(nb, inst, sw, b, q, r32, d32, prr, prho32);
             k_ir_control<<<n, kBlock, 0, s>>>(cur->scenes, inst, inst_in, ranges_d, sw, prr);
@@ -1431,15 +1466,18 @@ void launch_inner_cg(const InnerCg& c, hipStream_t s) {
         if (c.na > 0)
         {
             if (c.wide)
-                k_tile_apply<float, 6, float, kTileThreads, 2, true><<<c.na, kTileThreads, 0, s>>>(
+                k_tile_apply<float, 6, float, kTileThreads, 2, true, true><<<c.na, kTileThreads, 0, s>>>(
+                    c.ta, it, c.r, c.p, c.q, c.sp, c.prho, c.pden, 1, c.x, c.x0);
+            else if (c.plane)
+                k_tile_apply<float, 6, float, kTileThreads, 2, false, true><<<c.na, kTileThreads, 0, s>>>(
                     c.ta, it, c.r, c.p, c.q, c.sp, c.prho, c.pden, 1, c.x, c.x0);
             else
-                k_tile_apply<float, 6, float, kTileThreads, 2, false><<<c.na, kTileThreads, 0, s>>>(
+                k_tile_apply<float, 6, float, kTileThreads, 2, false, false><<<c.na, kTileThreads, 0, s>>>(
                     c.ta, it, c.r, c.p, c.q, c.sp, c.prho, c.pden, 1, c.x, c.x0);
         }
         if (c.nnb > 0)
             k_update<float><<<c.nnb, kBlock, 0, s>>>(c.nb, c.ta.inst, c.ta.ranges, it, c.sn, c.sp, c.ta.mass, c.ta.fixed,
-                                                      c.p, c.q, c.d, c.r, c.pden, c.prho);
+                                                      c.p, c.q, c.d, c.r, c.pden, c.prho, c.ta.contact);
     }
 }
 #else
@@ -1619,9 +1657,11 @@ int sofa_amd_scene_set_fem_hexa(sofa_amd_engine eng, int scene, int n_hexa, cons
 
 int sofa_amd_scene_set_mass(sofa_amd_engine eng, int scene, int kind, double value) {
     SceneHost* s = scene_of(eng, scene);
-    if (!s || kind < SOFA_AMD_MASS_UNIFORM || kind > SOFA_AMD_MASS_MESHMATRIX_LUMPED || !(value > 0))
+    if (!s || kind < SOFA_AMD_MASS_UNIFORM || kind > SOFA_AMD_MASS_DIAGONAL_TOTAL || !(value > 0))
         return fail(SOFA_AMD_EINVAL, "invalid mass");
     if (!editable(eng, scene)) return fail(SOFA_AMD_ESTATE, "scene already committed");
+    s->mass_total = kind == SOFA_AMD_MASS_DIAGONAL_TOTAL;
+    if (s->mass_total) kind = SOFA_AMD_MASS_DIAGONAL;
     s->mass_kind = kind;
     s->mass_value = value;
     s->density.assign(1, value);
@@ -1689,6 +1729,21 @@ int sofa_amd_scene_set_solver(sofa_amd_engine eng, int scene, double dt, double 
     if (!s || !(dt > 0) || iters < 0) return fail(SOFA_AMD_EINVAL, "invalid solver parameters");
     if (!editable(eng, scene)) return fail(SOFA_AMD_ESTATE, "scene already committed");
     s->dt = dt; s->rs = rs; s->rm = rm; s->cg_iter = iters; s->tol = tol; s->thr = thr;
+    return SOFA_AMD_OK;
+}
+
+int sofa_amd_scene_add_plane(sofa_amd_engine eng, int scene, const double* normal, double d, double stiffness,
+                             double damping, double max_force, int bilateral) {
+    SceneHost* s = scene_of(eng, scene);
+    if (!s || !normal) return fail(SOFA_AMD_EINVAL, "invalid argument");
+    for (double v : {normal[0], normal[1], normal[2], d, stiffness, damping, max_force})
+        if (!std::isfinite(v)) return fail(SOFA_AMD_EINVAL, "invalid plane parameters");
+    if (!editable(eng, scene)) return fail(SOFA_AMD_ESTATE, "scene already committed");
+    if (s->plane_on) return fail(SOFA_AMD_EINVAL, "one PlaneForceField per scene");
+    s->plane_on = true;
+    s->plane_bilateral = bilateral != 0;
+    for (int k = 0; k < 3; k++) s->plane_n[k] = normal[k];
+    s->plane_d = d; s->plane_k = stiffness; s->plane_damp = damping; s->plane_max = max_force;
     return SOFA_AMD_OK;
 }
 

@@ -71,6 +71,9 @@ struct InstDev {
     double vdamp;                     // exp(-h vdamping), 1 without damping
     int first_order;                  // v = dv instead of v += dv
     int project_velocity;             // FixedProjectiveConstraint activate_projectVelocity
+    // PlaneForceField (PlaneForceField.inl:86-98), after the FEM in scene order
+    double plane_n[3], plane_d, plane_k, plane_damp, plane_max;
+    int plane_on, plane_bilateral;
 };
 
 struct Blk { int inst, begin, end, pad; };   // a block's scene and [begin, end) range
@@ -113,6 +116,52 @@ __device__ __forceinline__ void normalize3<float>(float& x, float& y, float& z) 
         const float s = __builtin_amdgcn_rsqf(n2);
         x *= s; y *= s; z *= s;
     }
+}
+
+// PlaneForceField::addForce for one node (PlaneForceField.inl:158-205): adds the penalty force
+// of a node below the plane (either side if bilateral) to f; returns whether it is in contact
+template <typename VT>
+__device__ __forceinline__ bool plane_force(const InstDev& in, const VT x[3], const VT v[3], VT f[3]) {
+#pragma clang fp contract(off)
+    const VT n[3] = {(VT)in.plane_n[0], (VT)in.plane_n[1], (VT)in.plane_n[2]};
+    VT dist = x[0] * n[0];
+    dist += x[1] * n[1];
+    dist += x[2] * n[2];
+    dist = dist - (VT)in.plane_d;
+    if (!(in.plane_bilateral || dist < 0)) return false;
+    const VT fi = -(VT)in.plane_k * dist;
+    const VT di = -(VT)in.plane_damp * dist;
+    VT force[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) force[k] = n[k] * fi - v[k] * di;
+    VT limit = (VT)in.plane_max;
+    limit *= limit;
+    VT amp = force[0] * force[0];
+    amp += force[1] * force[1];
+    amp += force[2] * force[2];
+    if (limit > 0 && amp > limit) {
+        const VT sc = sqrt(limit / amp);
+#pragma unroll
+        for (int k = 0; k < 3; k++) force[k] *= sc;
+    }
+#pragma unroll
+    for (int k = 0; k < 3; k++) f[k] += force[k];
+    return true;
+}
+
+// PlaneForceField::addDForce for a node in contact (PlaneForceField.inl:208-224):
+// df += n (fact (dx . n)), fact = -stiffness kFactor
+template <typename VT>
+__device__ __forceinline__ void plane_dforce(const InstDev& in, double kfactor, const VT dx[3], VT df[3]) {
+#pragma clang fp contract(off)
+    const VT n[3] = {(VT)in.plane_n[0], (VT)in.plane_n[1], (VT)in.plane_n[2]};
+    const VT fact = (VT)(-in.plane_k * kfactor);
+    VT dn = dx[0] * n[0];
+    dn += dx[1] * n[1];
+    dn += dx[2] * n[2];
+    const VT a = fact * dn;
+#pragma unroll
+    for (int k = 0; k < 3; k++) df[k] = df[k] + n[k] * a;
 }
 
 template <typename T> __device__ __forceinline__ void load3(const T* __restrict__ v, int n, T& a, T& b, T& c) {

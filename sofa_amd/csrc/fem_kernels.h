@@ -32,6 +32,8 @@ struct NodeArrays {
     const int* __restrict__ em_ptr;    // MeshMatrixMass edges of each node, ascending edge id:
     const int* __restrict__ em_nbr;    //   the other end
     const double* __restrict__ em_val; //   the edge mass rho V / 20 summed over the edge's tets
+    int8_t* __restrict__ contact;      // PlaneForceField: 1 where the node touched the plane at the
+                                       // step's force evaluation (its addDForce contact list)
 };
 
 // mass kinds of a scene
@@ -147,7 +149,8 @@ __global__ __launch_bounds__(kBlock) void k_rhs_node(const Blk* __restrict__ nbl
                                                       const VT* __restrict__ ef_v, const VT* __restrict__ v,
                                                       VT* __restrict__ f, VT* __restrict__ b, VT* __restrict__ r,
                                                       VT* __restrict__ xcg, double* __restrict__ partial,
-                                                      double* __restrict__ nd) {   // non-null: reference dot order
+                                                      double* __restrict__ nd,     // non-null: reference dot order
+                                                      const VT* __restrict__ x) {
     __shared__ double sh[kBlock];
     const Blk bk = nblk[blockIdx.x];
     InstDev* ins = &inst[bk.inst];
@@ -155,6 +158,11 @@ __global__ __launch_bounds__(kBlock) void k_rhs_node(const Blk* __restrict__ nbl
     double bb = 0.0;
     if (n < bk.end) {
         const InstDev& in = *ins;
+        VT xn[3] = {0, 0, 0}, vn[3] = {0, 0, 0};
+        if (in.plane_on) {
+            load3(x, n, xn[0], xn[1], xn[2]);
+            load3(v, n, vn[0], vn[1], vn[2]);
+        }
         // computeForce: f = 0; mass addForce (f += m g); FEM addForce (f += R F, element order)
         VT fn[3];
         mass_gravity(in, na.mass, n, fn);
@@ -163,13 +171,16 @@ __global__ __launch_bounds__(kBlock) void k_rhs_node(const Blk* __restrict__ nbl
             const int64_t slot = na.inc[s];
             fn[0] += ef_f[3 * slot]; fn[1] += ef_f[3 * slot + 1]; fn[2] += ef_f[3 * slot + 2];
         }
+        // PlaneForceField after the FEM (scene order); its contacts are kept for addDForce
+        const bool touch = in.plane_on && plane_force(in, xn, vn, fn);
+        if (in.plane_on) na.contact[n] = touch ? 1 : 0;
         f[3 * (int64_t)n] = fn[0]; f[3 * (int64_t)n + 1] = fn[1]; f[3 * (int64_t)n + 2] = fn[2];
         // b = f; b += (-rm) M v (mass first); b -= (h+rs) K v (FEM, element order); b *= h
         VT bn[3] = {fn[0], fn[1], fn[2]};
         if (in.m_rhs != 0.0) {
-            VT vn[3];
-            load3(v, n, vn[0], vn[1], vn[2]);
-            mass_mdx(in, na, n, in.m_rhs, vn, bn);
+            VT vm[3];
+            load3(v, n, vm[0], vm[1], vm[2]);
+            mass_mdx(in, na, n, in.m_rhs, vm, bn);
             if (in.mass_kind == kMassMatrix)
                 mass_edges(na, n, in.m_rhs, [&](int j, VT d[3]) { load3(v, j, d[0], d[1], d[2]); }, bn);
         }
@@ -177,6 +188,7 @@ __global__ __launch_bounds__(kBlock) void k_rhs_node(const Blk* __restrict__ nbl
             const int64_t slot = na.inc[s];
             bn[0] -= ef_v[3 * slot]; bn[1] -= ef_v[3 * slot + 1]; bn[2] -= ef_v[3 * slot + 2];
         }
+        if (touch && in.k_rhs != 0.0) plane_dforce(in, in.k_rhs, vn, bn);
         const VT h = (VT)in.b_scale;   // h (firstOrder: 1, b = f)
         bn[0] *= h; bn[1] *= h; bn[2] *= h;
         if (na.fixed[n]) { bn[0] = 0; bn[1] = 0; bn[2] = 0; }
@@ -290,6 +302,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_node(const Blk* __restrict__ nblk
             const int64_t slot = na.inc[s];
             qn[0] -= ef[3 * slot]; qn[1] -= ef[3 * slot + 1]; qn[2] -= ef[3 * slot + 2];
         }
+        if (in.plane_on && na.contact[n]) plane_dforce(in, in.k_cg, pn, qn);
         if (na.fixed[n]) { qn[0] = 0; qn[1] = 0; qn[2] = 0; }
         q[o] = qn[0]; q[o + 1] = qn[1]; q[o + 2] = qn[2];
         pq = dot3d(pn, qn);

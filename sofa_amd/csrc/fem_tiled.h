@@ -185,6 +185,7 @@ struct TileArgs {
     TileNodes tn;
     const double* __restrict__ mass;
     const int8_t* __restrict__ fixed;
+    int8_t* __restrict__ contact;        // PlaneForceField contacts of the step (per node)
     const R* __restrict__ emass;        // per element rho V / 20 (consistent MeshMatrixMass), else unused
     float* __restrict__ x_lo;           // MIXED engines: float copy of the positions (written by the RHS)
 };
@@ -370,13 +371,19 @@ __global__ __launch_bounds__(TB) void k_tile_rhs(TileArgs<R, NR> a, const VT* __
             const int g = gid[m];
             const int s0 = lptr[j], s1 = lptr[j + 1];
             if (psl[m] < 0) {
+                const VT vn[3] = {(VT)lv[3 * j], (VT)lv[3 * j + 1], (VT)lv[3 * j + 2]};
+                // PlaneForceField after the FEM (scene order); the contact is kept for addDForce
+                bool touch = false;
+                if (in.plane_on) {
+                    const VT xn[3] = {(VT)lx[3 * j], (VT)lx[3 * j + 1], (VT)lx[3 * j + 2]};
+                    touch = plane_force(in, xn, vn, fn[m]);
+                    a.contact[g] = touch ? 1 : 0;
+                }
                 st3(f, g, fn[m]);
                 VT bn[3] = {fn[m][0], fn[m][1], fn[m][2]};
-                if (in.m_rhs != 0.0) {
-                    const VT vn[3] = {(VT)lv[3 * j], (VT)lv[3 * j + 1], (VT)lv[3 * j + 2]};
-                    mass_mdx(in, NodeArrays{nullptr, nullptr, a.mass, a.fixed}, g, in.m_rhs, vn, bn);
-                }
+                if (in.m_rhs != 0.0) mass_mdx(in, NodeArrays{nullptr, nullptr, a.mass, a.fixed}, g, in.m_rhs, vn, bn);
                 sum_incidences<true>(linc, s0, s1, lc, bn);
+                if (touch && in.k_rhs != 0.0) plane_dforce(in, in.k_rhs, vn, bn);
                 bn[0] *= h; bn[1] *= h; bn[2] *= h;
                 if (a.fixed[g]) { bn[0] = 0; bn[1] = 0; bn[2] = 0; }
                 st3(b, g, bn);
@@ -403,7 +410,8 @@ __global__ __launch_bounds__(kBlock) void k_fin_rhs(const FinBlk* __restrict__ f
                                                      const int8_t* __restrict__ fixed, const VT* __restrict__ v,
                                                      const VT* __restrict__ spf, const VT* __restrict__ spv,
                                                      VT* __restrict__ f, VT* __restrict__ b, VT* __restrict__ r,
-                                                     VT* __restrict__ xcg, double* __restrict__ pbb) {
+                                                     VT* __restrict__ xcg, double* __restrict__ pbb,
+                                                     const VT* __restrict__ x, int8_t* __restrict__ contact) {
     __shared__ double sh[kBlock / 64];
     const FinBlk fb = fb_[blockIdx.x];
     const InstDev& in = inst[fb.scene];
@@ -411,24 +419,30 @@ __global__ __launch_bounds__(kBlock) void k_fin_rhs(const FinBlk* __restrict__ f
     double bb = 0.0;
     if (j < fb.end) {
         const int g = sn.gid[j];
-        VT fn[3];
+        VT fn[3], vn[3];
+        ld3(v, g, vn);
         mass_gravity(in, mass, g, fn);
         const int s0 = sn.ptr[j], s1 = sn.ptr[j + 1];
         for (int s = s0; s < s1; s++) {
 #pragma unroll
             for (int i = 0; i < 3; i++) fn[i] += spf[3 * (int64_t)s + i];
         }
+        // PlaneForceField after the FEM (scene order); the contact is kept for addDForce
+        bool touch = false;
+        if (in.plane_on) {
+            VT xn[3];
+            ld3(x, g, xn);
+            touch = plane_force(in, xn, vn, fn);
+            contact[g] = touch ? 1 : 0;
+        }
         st3(f, g, fn);
         VT bn[3] = {fn[0], fn[1], fn[2]};
-        if (in.m_rhs != 0.0) {
-            VT vn[3];
-            ld3(v, g, vn);
-            mass_mdx(in, NodeArrays{nullptr, nullptr, mass, fixed}, g, in.m_rhs, vn, bn);
-        }
+        if (in.m_rhs != 0.0) mass_mdx(in, NodeArrays{nullptr, nullptr, mass, fixed}, g, in.m_rhs, vn, bn);
         for (int s = s0; s < s1; s++) {
 #pragma unroll
             for (int i = 0; i < 3; i++) bn[i] -= spv[3 * (int64_t)s + i];
         }
+        if (touch && in.k_rhs != 0.0) plane_dforce(in, in.k_rhs, vn, bn);
         const VT h = (VT)in.b_scale;   // h (firstOrder: 1, b = f)
         bn[0] *= h; bn[1] *= h; bn[2] *= h;
         if (fixed[g]) { bn[0] = 0; bn[1] = 0; bn[2] = 0; }
@@ -447,6 +461,7 @@ __global__ __launch_bounds__(kBlock) void k_fin_rhs(const FinBlk* __restrict__ f
 // iter = 0 is a plain operator application q = A r (no CG scalars, p untouched), used for
 // the FP64 residual of iterative refinement.
 // ---------------------------------------------------------------------------------------
+// PLANE = false: no scene of the launch has a PlaneForceField (its per-node terms compiled out).
 // GEO: how much element geometry is recomputed from node positions instead of read back, with
 // the functions and operation order of the element set-up and no fused multiply-adds (so the
 // same bits).  0: none (rotation, rest coordinates, K loaded).  1: the rotation, from the
@@ -454,7 +469,7 @@ __global__ __launch_bounds__(kBlock) void k_fin_rhs(const FinBlk* __restrict__ f
 // less per element, 3 more per tile-local node.  2: also the rotated rest coordinates from the
 // rest positions x0 (initLarge, TetrahedronFEMForceField.inl:835-868) and, for a scene of one
 // material, K from the rest volume -- NR (+ 3) reals less per element, 3 more per local node.
-template <typename R, int NR, typename VT, int TB = kBlock, int GEO = 0, bool WIDE = true>
+template <typename R, int NR, typename VT, int TB = kBlock, int GEO = 0, bool WIDE = true, bool PLANE = true>
 __global__ __launch_bounds__(TB) void k_tile_apply(TileArgs<R, NR> a, int iter, const VT* __restrict__ r,
                                                     VT* __restrict__ p, VT* __restrict__ q, VT* __restrict__ sp,
                                                     const double* __restrict__ pin,   // |b|^2 (iter 1) or r.r partials
@@ -499,7 +514,7 @@ __global__ __launch_bounds__(TB) void k_tile_apply(TileArgs<R, NR> a, int iter, 
         }
     }
     int gid[NPT], psl[NPT];
-    int8_t fx[NPT];
+    int8_t fx[NPT], ct[NPT];
 #pragma unroll
     for (int m = 0; m < NPT; m++) {
         const int j = threadIdx.x + m * TB;
@@ -527,6 +542,7 @@ __global__ __launch_bounds__(TB) void k_tile_apply(TileArgs<R, NR> a, int iter, 
             if (GEO >= 1) ld3(x, gid[m], xx_[m]);
             if (GEO == 2) ld3(x0, gid[m], x0_[m]);
             fx[m] = a.fixed[gid[m]];
+            ct[m] = PLANE && ins->plane_on ? a.contact[gid[m]] : 0;
         }
     }
     const Ranges rg = a.ranges[td.scene];
@@ -650,6 +666,7 @@ __global__ __launch_bounds__(TB) void k_tile_apply(TileArgs<R, NR> a, int iter, 
                 VT qn[3] = {0, 0, 0};
                 if (in.m_cg != 0.0) mass_mdx(in, NodeArrays{nullptr, nullptr, a.mass, a.fixed}, gid[m], in.m_cg, pp, qn);
                 sum_incidences<true>(linc, s0, s1, lc, qn);
+                if (PLANE && ct[m]) plane_dforce(in, in.k_cg, pp, qn);
                 if (fx[m]) { qn[0] = 0; qn[1] = 0; qn[2] = 0; }
                 st3(q, gid[m], qn);
                 pq += dot3d(pp, qn);
@@ -658,14 +675,16 @@ __global__ __launch_bounds__(TB) void k_tile_apply(TileArgs<R, NR> a, int iter, 
                 sum_incidences<true>(linc, s0, s1, lc, acc);
                 st3(sp, psl[m] & (kSlotOwner - 1), acc);
                 // p.q is linear in q: a shared node's term is split over its tiles (p . this
-                // tile's partial), its mass term added by its first tile, so that the tile
-                // partials hold all of p.q and no later kernel has to add to it
+                // tile's partial), its mass and plane terms added by its first tile, so that the
+                // tile partials hold all of p.q and no later kernel has to add to it
                 if (!fx[m]) {
                     const VT pp[3] = {(VT)lu[3 * j], (VT)lu[3 * j + 1], (VT)lu[3 * j + 2]};
                     pq += dot3d(pp, acc);
-                    if ((psl[m] & kSlotOwner) && in.m_cg != 0.0) {
+                    if ((psl[m] & kSlotOwner) && (in.m_cg != 0.0 || (PLANE && ct[m]))) {
                         VT mq[3] = {0, 0, 0};
-                        mass_mdx(in, NodeArrays{nullptr, nullptr, a.mass, a.fixed}, gid[m], in.m_cg, pp, mq);
+                        if (in.m_cg != 0.0)
+                            mass_mdx(in, NodeArrays{nullptr, nullptr, a.mass, a.fixed}, gid[m], in.m_cg, pp, mq);
+                        if (PLANE && ct[m]) plane_dforce(in, in.k_cg, pp, mq);
                         pq += dot3d(pp, mq);
                     }
                 }
@@ -683,7 +702,8 @@ __global__ __launch_bounds__(kBlock) void k_fin_apply(const FinBlk* __restrict__
                                                        const int8_t* __restrict__ fixed, int iter,
                                                        const VT* __restrict__ r, VT* __restrict__ p,
                                                        VT* __restrict__ q, const VT* __restrict__ sp,
-                                                       double* __restrict__ pden) {
+                                                       double* __restrict__ pden,
+                                                       const int8_t* __restrict__ contact) {
     __shared__ double sh[kBlock / 64];
     const FinBlk fb = fb_[blockIdx.x];
     const InstDev& in = inst[fb.scene];
@@ -709,6 +729,7 @@ __global__ __launch_bounds__(kBlock) void k_fin_apply(const FinBlk* __restrict__
 #pragma unroll
             for (int i = 0; i < 3; i++) qn[i] += sp[3 * (int64_t)s + i];
         }
+        if (in.plane_on && contact[g]) plane_dforce(in, in.k_cg, pp, qn);
         if (fixed[g]) { qn[0] = 0; qn[1] = 0; qn[2] = 0; }
         st3(q, g, qn);
         pq = dot3d(pp, qn);
@@ -728,7 +749,8 @@ __global__ __launch_bounds__(kBlock) void k_update(const NodeBlk* __restrict__ n
                                                     const VT* __restrict__ sp, const double* __restrict__ mass,
                                                     const int8_t* __restrict__ fixed, VT* __restrict__ p,
                                                     const VT* __restrict__ q, VT* __restrict__ xcg, VT* __restrict__ r,
-                                                    const double* __restrict__ pden, double* __restrict__ prho) {
+                                                    const double* __restrict__ pden, double* __restrict__ prho,
+                                                    const int8_t* __restrict__ contact) {
     __shared__ double sh[kBlock / 64];
     const NodeBlk nb = nb_[blockIdx.x];
     InstDev* ins = &inst[nb.scene];
@@ -786,6 +808,7 @@ __global__ __launch_bounds__(kBlock) void k_update(const NodeBlk* __restrict__ n
 #pragma unroll
                     for (int i = 0; i < 3; i++) qn[i] += sp[3 * (int64_t)s + i];
                 }
+                if (in.plane_on && contact[n]) plane_dforce(in, in.k_cg, pp, qn);
                 if (fixed[n]) { qn[0] = 0; qn[1] = 0; qn[2] = 0; }
 #pragma unroll
                 for (int i = 0; i < 3; i++) { pv[k][i] = pp[i]; qv[k][i] = qn[i]; }

@@ -793,6 +793,52 @@ def test_hexa_topology_tessellation():
     assert rel_err(xa, pos) > 1e-6
 
 
+def _plane_scene():
+    # a small version of the SofaCUDA beam benchmark scenes: a 6-tetrahedra grid beam under
+    # gravity, clamped at x = 0, DiagonalMass given its totalMass, sagging onto a
+    # PlaneForceField floor (normal +y) placed just below it
+    pos, hexa = meshes.regular_grid((12, 4, 4), (0.0, 6.0, -2.0), (8.0, 8.0, 0.0))
+    tets = meshes.split_hexa_6(hexa)
+    fixed = meshes.box_roi(pos, (-0.1, 5.0, -3.0, 0.1, 11.0, 3.0))
+    return dict(pos=pos, tets=tets, fixed=fixed, E=1000.0, nu=0.3, total_mass=50.0, g=(0.0, -9.0, 0.0),
+                rs=0.1, rm=0.1, dt=0.04, iters=10, tol=1e-6, thr=1e-6,
+                plane=dict(normal=(0.0, 1.0, 0.0), d=5.2, stiffness=10000.0, damping=5.0))
+
+
+@pytest.mark.parametrize("prec,bar", [("FP64", None), ("FP64_TILED", 1e-8), ("FP32", 1e-4)])
+def test_plane_force_field_and_total_mass(prec, bar):
+    """PlaneForceField and DiagonalMass(totalMass) -- the components of the SofaCUDA benchmark
+    scenes -- against the oracle after 60 steps: FP64 bit-exact in the GPU's dot order, the tiled
+    paths within the stated bar of the reference order (a node whose contact flips between
+    precisions right at the plane moves the result by more than rounding)."""
+    capi = _capi()
+    print(f"case: plane force field + total mass, {prec}", flush=True)
+    c = _plane_scene()
+    orc = make_oracle(c, dot_mode=1 if prec == "FP64" else 0)
+    orc.set_diagonal_total_mass(c["total_mass"])
+    orc.add_plane(**c["plane"])
+    orc.step(c["dt"], 60)
+    xo = orc.get("x")
+    assert xo[:, 1].min() < c["plane"]["d"]                    # the beam reached the floor
+    eng = capi.Engine(0, getattr(capi, prec))
+    sid = eng.add_scene(c["pos"], c["tets"], [c["E"]], [c["nu"]], c["fixed"], mass_kind=capi.MASS_DIAGONAL_TOTAL,
+                        mass_value=c["total_mass"], gravity=c["g"], dt=c["dt"], rayleigh_stiffness=c["rs"],
+                        rayleigh_mass=c["rm"], cg_iterations=c["iters"], cg_tolerance=c["tol"],
+                        cg_threshold=c["thr"])
+    eng.add_plane(sid, **c["plane"])
+    eng.commit()
+    eng.step([sid], 60)
+    x, v, _ = eng.read(sid)
+    eng.close()
+    if bar is None:
+        assert np.array_equal(x, xo), rel_err(x, xo)
+        assert np.array_equal(v, orc.get("v"))
+    else:
+        err = rel_err(x, xo)
+        print(f"  rel err {err:.3g}", flush=True)
+        assert err <= bar, err
+
+
 def test_error_codes():
     """Invalid input is refused with the documented status and a message, nothing faults:
     out-of-range tetrahedron / fixed indices (EINVAL), a zero-volume tetrahedron at commit

@@ -178,6 +178,63 @@ def test_project_velocity_closed_form(project):
     assert np.all(vv[fixed] == (0.0 if project else np.array([0.0, 0.0, 0.1])))
 
 
+def test_diagonal_total_mass():
+    """DiagonalMass given totalMass (DiagonalMass.inl:1235-1257): the vertex masses sum to the
+    total and are proportional to the density-1 masses (each vertex V/4 of each incident
+    tetrahedron), checked through the gravity force m_i g of a scene at rest."""
+    pos, tets, _ = meshes.beam((4, 3, 6), (1.0, 0.7, 2.0), split=6)
+    g = 9.81
+    A = pos[tets]
+    vol = np.abs(np.einsum("ij,ij->i", np.cross(A[:, 1] - A[:, 0], A[:, 2] - A[:, 0]), A[:, 3] - A[:, 0])) / 6.0
+    m1 = np.zeros(len(pos))
+    for t, v in zip(tets, vol):
+        m1[t] += v / 4.0
+    for total in (50.0, 3.5):
+        s = OracleScene(pos)
+        s.set_gravity((0.0, -g, 0.0))
+        s.set_fem(tets, [1e3], [0.3], "large")
+        s.set_diagonal_total_mass(total)
+        s.set_fixed(np.zeros(0, np.int32))
+        s.set_solver(0.0, 0.0, 0, 1e-5, 1e-5)
+        s.step(1e-3, 1)                                   # CG 0 iterations: f is the gravity load
+        m = -s.get("f")[:, 1] / g
+        assert abs(m.sum() - total) <= 1e-12 * total
+        assert np.allclose(m, m1 * (total / m1.sum()), rtol=1e-13, atol=0)
+
+
+@pytest.mark.parametrize("max_force", [0.0, 30.0])
+def test_plane_force_field_closed_form(max_force):
+    """PlaneForceField (PlaneForceField.inl:158-224) under EulerImplicitSolver + CG on one node
+    falling onto the plane y = D: with c = [y < D] at the step start and dist = y - D,
+      f  = -m g + c (-k dist + damp dist v)            (force clamped to maxForce when set)
+      dv = h (f - rm m v - (h + rs) k c v) / ((1 + h rm) m + h (h + rs) k c),  y' = y + h (v + dv)
+    -- the stiffness enters the system matrix, the damping only the force."""
+    m, g, k, damp, D, h, rm, rs = 2.0, 9.81, 4000.0, 20.0, 0.3, 0.005, 0.1, 0.05
+    s = OracleScene(np.array([[0.0, 1.0, 0.0]]))
+    s.set_gravity((0.0, -g, 0.0))
+    s.set_uniform_mass(vertex_mass=m)
+    s.add_plane((0.0, 1.0, 0.0), D, k, damp, max_force)
+    s.set_fixed(np.zeros(0, np.int32))
+    s.set_solver(rs, rm, 25, 1e-14, 1e-30)
+    y, v = 1.0, 0.0
+    contacts = 0
+    for i in range(600):
+        dist = y - D
+        c = 1.0 if dist < 0 else 0.0
+        fc = c * (-k * dist + damp * dist * v)
+        if max_force > 0 and fc * fc > max_force ** 2:
+            fc = fc * np.sqrt(max_force ** 2 / (fc * fc))
+        f = -m * g + fc
+        dv = h * (f - rm * m * v - (h + rs) * k * c * v) / ((1 + h * rm) * m + h * (h + rs) * k * c)
+        v = v + dv
+        y = y + h * v
+        contacts += c > 0
+        s.step(h, 1)
+        ys, vs = s.get("x")[0, 1], s.get("v")[0, 1]
+        assert abs(ys - y) <= 1e-11 and abs(vs - v) <= 1e-9 * max(1.0, abs(v)), (i, ys, y, vs, v)
+    assert contacts > 50                      # the node reached the plane and stayed in contact
+
+
 # ------------------------------------------------------------------------------------------
 # mathematical invariants
 # ------------------------------------------------------------------------------------------
