@@ -166,6 +166,40 @@ def measured_issue(path="profiles/k_tile_apply_beam_mix_issue.md"):
     return d
 
 
+# SofaCUDA's own benchmark of this component stack (applications/plugins/SofaCUDA/doc/
+# SofaCUDA_benchmarks.csv:15-22, hardware at :37-39): mean seconds per 1000 FP32 steps,
+# as element-steps/s (tetrahedra x 1000 / seconds)
+SOFACUDA_PUBLISHED = {
+    "beam10x10x40": {"tets": 18954, "gpu_rtx2070": 18954 * 1000 / 2.798994, "gpu_gtx1060": 18954 * 1000 / 3.824197,
+                     "cpu_i7_10750h": 18954 * 1000 / 33.56943, "cpu_ryzen7_3800x": 18954 * 1000 / 32.29484},
+    "beam16x16x76": {"tets": 101250, "gpu_rtx2070": 101250 * 1000 / 2.595726, "gpu_gtx1060": 101250 * 1000 / 3.432895,
+                     "cpu_i7_10750h": 101250 * 1000 / 109.0122, "cpu_ryzen7_3800x": 101250 * 1000 / 114.5244},
+}
+
+
+def sofacuda_scenes(steps):
+    """SofaCUDA's TetrahedronFEMForceField benchmark scenes (beam10x10x40; beam16x16x76 with its
+    PlaneForceField floor), read from SOFA scene files by sofa_amd.scene_xml, each stepped alone
+    `steps` times (the reference's runs: 1000 steps in batch mode), FP32 (the reference's
+    CudaVec3f) and FP64 (the parity path); element-steps/s beside the published figures."""
+    from sofa_amd import capi, scene_xml
+    out = {}
+    for name, pub in SOFACUDA_PUBLISHED.items():
+        (spec,), _ = scene_xml.load_scene(os.path.join(ROOT, "sofa_amd", "scenes", name + ".scn"))
+        row = {"tets": len(spec["tets"]), "nodes": len(spec["pos"]), "steps": steps,
+               "cg_iterations": spec["iters"], "plane": "plane" in spec}
+        for label, prec in (("fp32", capi.FP32), ("fp64_parity", capi.FP64)):
+            eng, ids = make_engine([spec], prec, graphs=True)
+            t, lat = time_merged(eng, ids, steps, 10)
+            eng.close()
+            row[label] = {"value": len(spec["tets"]) * steps / (t / 1e3), "ms_per_step": t / steps,
+                          "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99))}
+        row["published"] = {k: v for k, v in pub.items() if k != "tets"}
+        row["fp32_over_published_gpu_rtx2070"] = row["fp32"]["value"] / pub["gpu_rtx2070"]
+        out[name] = row
+    return out
+
+
 def copy_probe(mb=512, reps=5):
     """A plain device copy of known size (calibrates the HBM byte counters; STREAM-copy rate)."""
     import torch
@@ -340,6 +374,8 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-fp64-ref", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-sofacuda", action="store_true")
+    ap.add_argument("--sofacuda-steps", type=int, default=1000)
     ap.add_argument("--serve-seconds", type=float, default=15.0)
     ap.add_argument("--serve-arrivals", type=float, default=5.0)
     ap.add_argument("--serve-budget-gb", type=float, default=8.0)
@@ -436,6 +472,9 @@ def main():
     served = None if args.no_serve else serve_cfg4(args.serve_seconds, args.serve_arrivals, args.serve_budget_gb)
     log(f"cfg4: {served}")
 
+    scn = None if args.no_sofacuda else sofacuda_scenes(args.sofacuda_steps)
+    log(f"sofacuda scenes: {scn}")
+
     cpu = None if args.no_cpu_baseline else cpu_baseline(specs, args.cpu_steps, args.cpu_stride)
     log(f"cpu baseline: {cpu}")
 
@@ -467,6 +506,7 @@ def main():
         "cfg1_single_beam": cfg1,
         "cfg2_ellipsoid_mixed": cfg2,
         "cfg4_served": served,
+        "sofacuda_benchmark_scenes": scn,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -176,11 +176,19 @@ def add_to_engine(eng, spec):
     from . import capi
     kind, val = spec["mass"]
     mk = {"uniform": capi.MASS_UNIFORM, "diagonal": capi.MASS_DIAGONAL, "meshmatrix": capi.MASS_MESHMATRIX,
-          "meshmatrix_lumped": capi.MASS_MESHMATRIX_LUMPED}[kind]
+          "meshmatrix_lumped": capi.MASS_MESHMATRIX_LUMPED, "diagonal_total": capi.MASS_DIAGONAL_TOTAL}[kind]
     s = eng.add_scene(spec["pos"], spec["tets"], np.atleast_1d(spec["E"]), np.atleast_1d(spec["nu"]),
                       spec["fixed"], mass_kind=mk, mass_value=val, gravity=spec["g"], dt=spec["dt"],
                       rayleigh_stiffness=spec["rs"], rayleigh_mass=spec["rm"], cg_iterations=spec["iters"],
                       cg_tolerance=spec["tol"], cg_threshold=spec["thr"], oscillators=spec.get("oscillators"))
     if "refinement" in spec:
         eng.set_refinement(s, **spec["refinement"])
+    if spec.get("fix_all"):
+        eng.set_fix_all(s, True)
+    if spec.get("project_velocity"):
+        eng.set_project_velocity(s, True)
+    if "integrator" in spec:
+        eng.set_integrator(s, **spec["integrator"])
+    if "plane" in spec:
+        eng.add_plane(s, **spec["plane"])
     return s

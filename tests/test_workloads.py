@@ -227,3 +227,50 @@ def test_fem_hexa_tessellation():
     for t in plain[0:6]:
         refl.add(frozenset(gidx(1 - coords[v][0], coords[v][1], coords[v][2]) for v in t))
     assert cube_tets[0] == refl
+
+
+def test_scene_reader_benchmark_scenes():
+    """The SOFA scene reader on the SofaCUDA TetrahedronFEMForceField benchmark configurations:
+    grid sizes, the 6-tetrahedra split, the BoxROI-clamped face, DiagonalMass totalMass, the solver
+    Data and the PlaneForceField floor come out as the scenes state them. Where the reference tree
+    is present, its own scene files read into the same bodies (their visual and collision
+    components listed as ignored)."""
+    import os
+    from sofa_amd import scene_xml
+    here = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "sofa_amd", "scenes")
+    expect = {"beam10x10x40": (4000, 18954, 100, 20, 0.01, 2000.0, None),
+              "beam16x16x76": (19456, 101250, 256, 10, 0.04, 1000.0, 2.0)}
+    ours = {}
+    for name, (n, e, nf, iters, dt, E, plane_d) in expect.items():
+        specs, info = scene_xml.load_scene(os.path.join(here, name + ".scn"))
+        assert len(specs) == 1 and info["ignored"] == []
+        s = specs[0]
+        assert (len(s["pos"]), len(s["tets"]), len(s["fixed"])) == (n, e, nf)
+        assert s["iters"] == iters and s["dt"] == dt and s["E"][0] == E and s["nu"][0] == 0.3
+        assert s["tol"] == 1e-6 and s["thr"] == 1e-6 and s["rs"] == 0.1 and s["rm"] == 0.1
+        assert s["mass"] == ("diagonal_total", 50.0) and s["g"] == (0.0, -9.0, 0.0)
+        assert np.all(s["pos"][s["fixed"], 0] == 0.0)                        # the clamped face x = 0
+        assert np.array_equal(s["tets"], meshes_split6(s["pos"], name))
+        assert (s.get("plane") or {}).get("d") == plane_d
+        ours[name] = s
+    ref = "/root/reference/applications/plugins/SofaCUDA/scenes/benchmarks"
+    if os.path.isdir(ref):
+        for name, s in ours.items():
+            for dev in ("cpu", "gpu"):
+                specs, info = scene_xml.load_scene(os.path.join(ref, f"TetrahedronFEMForceField_{name}_{dev}.scn"))
+                assert len(specs) == 1 and "OglModel" in info["ignored"]
+                r = specs[0]
+                for k in ("pos", "tets", "fixed"):
+                    assert np.array_equal(r[k], s[k]), (name, dev, k)
+                for k in ("iters", "tol", "thr", "rs", "rm", "dt", "mass", "g", "plane"):
+                    assert r.get(k) == s.get(k), (name, dev, k)
+                assert np.array_equal(r["E"], s["E"]) and np.array_equal(r["nu"], s["nu"])
+
+
+def meshes_split6(pos, name):
+    from sofa_amd import meshes
+    n = {"beam10x10x40": (40, 10, 10), "beam16x16x76": (76, 16, 16)}[name]
+    lo, hi = (0.0, 6.0, -2.0), ((16.0, 10.0, 2.0) if name == "beam10x10x40" else (19.0, 10.0, 2.0))
+    p, hexa = meshes.regular_grid(n, lo, hi)
+    assert np.array_equal(p, pos)
+    return meshes.split_hexa_6(hexa)
