@@ -11,7 +11,10 @@ unchanged. Read, with their Data names and defaults:
 * ``TetrahedronSetTopologyContainer``: ``tetrahedra``/``position`` inline or ``src`` = a link to
   another container; ``Hexa2TetraTopologicalMapping`` (``input`` grid, ``output`` container,
   ``swapping`` false: the 6-tetrahedra split of Hexa2TetraTopologicalMapping.cpp:103-197);
-* ``MechanicalObject``: ``position`` inline or a link such as ``@../Beam/grid.position``;
+* ``MechanicalObject``: ``position`` inline, a link such as ``@../Beam/grid.position``, or
+  ``src`` = a mesh loader;
+* ``MeshGmshLoader`` / ``MeshVTKLoader``: ``filename`` (Gmsh 1 / 2.2 / 4.1 or legacy VTK ASCII,
+  ``sofa_amd.mesh_io``), ``scale3d``, ``translation``;
 * ``TetrahedronFEMForceField``: ``youngModulus``, ``poissonRatio`` (lists), ``method`` (only
   ``large``); a topology with hexahedra only is tessellated as its init does;
 * ``UniformMass`` (``vertexMass`` or ``totalMass``), ``DiagonalMass`` (``massDensity`` or
@@ -24,8 +27,8 @@ unchanged. Read, with their Data names and defaults:
   ``threshold``);
 * ``PlaneForceField`` (``normal``, ``d``, ``stiffness``, ``damping``, ``maxForce``, ``bilateral``).
 
-Components off the path (visual models, collision pipeline, plugins, surface mappings) are
-listed in ``ignored``; a component on the path with an option this build does not run
+Every other component (visual models, collision pipeline, plugins, surface or visual mappings)
+is listed in ``ignored`` and plays no part in the bodies; a component on the path with an option this build does not run
 (``method`` other than large, CG ``warmStart``, ``solveConstraint``) raises ``ValueError``.
 """
 import xml.etree.ElementTree as ET
@@ -34,13 +37,6 @@ import numpy as np
 
 from . import meshes
 
-IGNORED = {
-    "RequiredPlugin", "VisualStyle", "DefaultAnimationLoop", "DefaultVisualManagerLoop", "CollisionPipeline",
-    "BruteForceBroadPhase", "BVHNarrowPhase", "CollisionResponse", "DiscreteIntersection", "OglModel",
-    "IdentityMapping", "TriangleSetTopologyContainer", "TriangleSetTopologyModifier", "TriangleSetGeometryAlgorithms",
-    "Tetra2TriangleTopologicalMapping", "TetrahedronSetTopologyModifier", "TetrahedronSetGeometryAlgorithms",
-    "TriangleCollisionModel", "LineCollisionModel", "PointCollisionModel", "VisualModel", "LightManager",
-}
 
 
 def _floats(s):
@@ -78,13 +74,21 @@ class _Node:
         return None
 
 
+HANDLED = {
+    "RegularGridTopology", "GridTopology", "TetrahedronSetTopologyContainer", "Hexa2TetraTopologicalMapping",
+    "MechanicalObject", "TetrahedronFEMForceField", "UniformMass", "DiagonalMass", "MeshMatrixMass", "BoxROI",
+    "FixedProjectiveConstraint", "FixedConstraint", "OscillatorProjectiveConstraint", "EulerImplicitSolver",
+    "CGLinearSolver", "PlaneForceField", "MeshGmshLoader", "MeshVTKLoader",
+}
+
+
 def _walk(elem, parent, ignored):
     node = _Node(elem, parent)
     for child in elem:
         if child.tag == "Node":
             node.children.append(_walk(child, node, ignored))
         else:
-            if child.tag in IGNORED:
+            if child.tag not in HANDLED:
                 ignored.append(child.tag)
             node.objects.append(_Obj(child, node))
     return node
@@ -132,7 +136,27 @@ def _grid(obj):
     return obj
 
 
-def _evaluate(root):
+def _load_mesh(obj, search):
+    """MeshGmshLoader / MeshVTKLoader: points and tetrahedra of `filename` (looked up beside the
+    scene, then in the search paths), then MeshLoader's scale3d and translation (TRSx,
+    MeshLoader.cpp:206-234; a rotation is not supported)."""
+    import os
+    from . import mesh_io
+    fn = obj.attr["filename"]
+    cands = [fn] if os.path.isabs(fn) else [os.path.join(d, fn) for d in search]
+    path = next((c for c in cands if os.path.exists(c)), None)
+    if path is None:
+        raise ValueError(f"{obj.tag}: {fn} not found in {search}")
+    pos, tets = mesh_io.read_mesh(path)
+    if any(v != 0.0 for v in _floats(obj.attr.get("rotation", "0 0 0"))):
+        raise ValueError(f"{obj.tag}: rotation is not supported")
+    sc = np.asarray(_floats(obj.attr.get("scale3d", "1 1 1")))
+    tr = np.asarray(_floats(obj.attr.get("translation", "0 0 0")))
+    pos = pos * sc + tr
+    obj.data.update(position=pos, tetrahedra=tets)
+
+
+def _evaluate(root, search=(".",)):
     """Fills obj.data for topology objects and mappings (in document order, links resolved)."""
     order = []
 
@@ -145,6 +169,8 @@ def _evaluate(root):
     for o in order:
         if o.tag in ("RegularGridTopology", "GridTopology"):
             _grid(o)
+        elif o.tag in ("MeshGmshLoader", "MeshVTKLoader"):
+            _load_mesh(o, search)
     for _ in range(3):                      # containers fed by mappings or links to other containers
         for o in order:
             if o.tag == "Hexa2TetraTopologicalMapping":
@@ -179,17 +205,22 @@ def _vec(o, key):
     return v
 
 
-def load_scene(path_or_text):
+def load_scene(path_or_text, search_paths=()):
     """Parses a scene (a path, or the XML text itself); returns (specs, info) where specs is a
     list of body specs for ``workloads.add_to_engine`` and info names the bodies and the
-    components that were ignored."""
+    components that were ignored. Mesh files are looked up beside the scene, then in
+    `search_paths` (SOFA's data directories, e.g. a checkout's ``share``)."""
+    import os
     text = path_or_text
+    search = [os.getcwd()]
     if not path_or_text.lstrip().startswith("<"):
         with open(path_or_text) as fh:
             text = fh.read()
+        search = [os.path.dirname(os.path.abspath(path_or_text))]
+    search += list(search_paths)
     ignored = []
     root = _walk(ET.fromstring(text), None, ignored)
-    order = _evaluate(root)
+    order = _evaluate(root, search)
     g = _floats(root.attr.get("gravity", "0 -9.81 0"))
     dt = float(root.attr.get("dt", "0.01"))
     # BoxROI indices (on the rest positions of their node's mechanical object or topology)
@@ -199,6 +230,8 @@ def load_scene(path_or_text):
         mo = node.first("MechanicalObject")
         if mo is not None:
             p = _vec(mo, "position")
+            if p is None and "src" in mo.attr:
+                p = _resolve(mo.node, mo.attr["src"])[0].data.get("position")
             if p is not None:
                 return np.asarray(_floats(p) if isinstance(p, str) else p, dtype=np.float64).reshape(-1, 3)
         for o in node.objects:

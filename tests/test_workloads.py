@@ -274,3 +274,79 @@ def meshes_split6(pos, name):
     p, hexa = meshes.regular_grid(n, lo, hi)
     assert np.array_equal(p, pos)
     return meshes.split_hexa_6(hexa)
+
+
+def _write_mesh_files(tmp, pos, tets):
+    """The same mesh in Gmsh 1, 2.2 and 4.1 and legacy VTK, node numbers offset in the Gmsh files
+    (1-based, with a gap) and one triangle element mixed in, as real files have."""
+    import os
+    num = [i + 1 + (7 if i >= 3 else 0) for i in range(len(pos))]
+    f1 = os.path.join(tmp, "m1.msh")
+    with open(f1, "w") as fh:
+        fh.write(f"$NOD\n{len(pos)}\n")
+        for i, p in enumerate(pos):
+            fh.write(f"{num[i]} {float(p[0])!r} {float(p[1])!r} {float(p[2])!r}\n")
+        fh.write(f"$ENDNOD\n$ELM\n{len(tets) + 1}\n1 2 0 0 3 {num[0]} {num[1]} {num[2]}\n")
+        for k, t in enumerate(tets):
+            fh.write(f"{k + 2} 4 1 1 4 " + " ".join(str(num[v]) for v in t) + "\n")
+        fh.write("$ENDELM\n")
+    f2 = os.path.join(tmp, "m2.msh")
+    with open(f2, "w") as fh:
+        fh.write(f"$MeshFormat\n2.2 0 8\n$EndMeshFormat\n$Nodes\n{len(pos)}\n")
+        for i, p in enumerate(pos):
+            fh.write(f"{num[i]} {float(p[0])!r} {float(p[1])!r} {float(p[2])!r}\n")
+        fh.write(f"$EndNodes\n$Elements\n{len(tets) + 1}\n1 2 2 0 1 {num[0]} {num[1]} {num[2]}\n")
+        for k, t in enumerate(tets):
+            fh.write(f"{k + 2} 4 2 0 1 " + " ".join(str(num[v]) for v in t) + "\n")
+        fh.write("$EndElements\n")
+    f4 = os.path.join(tmp, "m4.msh")
+    with open(f4, "w") as fh:
+        fh.write(f"$MeshFormat\n4.1 0 8\n$EndMeshFormat\n$Nodes\n1 {len(pos)} 1 {max(num)}\n3 1 0 {len(pos)}\n")
+        fh.write("".join(f"{v}\n" for v in num))
+        fh.write("".join(f"{float(p[0])!r} {float(p[1])!r} {float(p[2])!r}\n" for p in pos))
+        fh.write(f"$EndNodes\n$Elements\n2 {len(tets) + 1} 1 {len(tets) + 1}\n2 1 2 1\n1 {num[0]} {num[1]} {num[2]}\n")
+        fh.write(f"3 1 4 {len(tets)}\n")
+        for k, t in enumerate(tets):
+            fh.write(f"{k + 2} " + " ".join(str(num[v]) for v in t) + "\n")
+        fh.write("$EndElements\n")
+    fv = os.path.join(tmp, "m.vtk")
+    with open(fv, "w") as fh:
+        fh.write("# vtk DataFile Version 3.0\nmesh\nASCII\nDATASET UNSTRUCTURED_GRID\n")
+        fh.write(f"POINTS {len(pos)} double\n" + "".join(f"{float(p[0])!r} {float(p[1])!r} {float(p[2])!r}\n" for p in pos))
+        fh.write(f"CELLS {len(tets) + 1} {5 * len(tets) + 4}\n3 0 1 2\n" + "".join("4 " + " ".join(map(str, t)) + "\n" for t in tets))
+        fh.write(f"CELL_TYPES {len(tets) + 1}\n5\n" + "10\n" * len(tets))
+    return f1, f2, f4, fv
+
+
+def test_mesh_files(tmp_path):
+    """Gmsh 1 / 2.2 / 4.1 and legacy VTK: points in file order (Gmsh numbers mapped), the
+    tetrahedra in file order, other element types skipped; a scene file naming the mesh
+    (MeshGmshLoader with scale3d and translation) reads into the same body. Where the reference
+    tree is present, its liver scene (SofaCUDA examples) with its Gmsh 1 mesh reads too."""
+    import os
+    from sofa_amd import mesh_io, meshes, scene_xml
+    pos, tets, fixed = meshes.beam((3, 3, 4), (1.0, 1.0, 1.5), split=6)
+    for f in _write_mesh_files(str(tmp_path), pos, tets):
+        p, t = mesh_io.read_mesh(f)
+        assert np.array_equal(p, pos) and np.array_equal(t, tets), f
+    scn = f"""<Node name="root" gravity="0 -9.81 0" dt="0.01"><Node name="B">
+      <EulerImplicitSolver rayleighStiffness="0.1" rayleighMass="0.1"/><CGLinearSolver iterations="30"/>
+      <MeshGmshLoader name="loader" filename="m2.msh" scale3d="2 2 2" translation="1 0 -1"/>
+      <MechanicalObject src="@loader"/><TetrahedronSetTopologyContainer src="@loader"/>
+      <UniformMass totalMass="12"/><FixedProjectiveConstraint indices="0 1 2"/>
+      <TetrahedronFEMForceField youngModulus="3000" poissonRatio="0.3" method="large"/></Node></Node>"""
+    path = os.path.join(str(tmp_path), "s.scn")
+    with open(path, "w") as fh:
+        fh.write(scn)
+    (s,), info = scene_xml.load_scene(path)
+    assert np.array_equal(s["pos"], pos * 2.0 + np.array([1.0, 0.0, -1.0])) and np.array_equal(s["tets"], tets)
+    assert s["mass"] == ("uniform", 12.0 / len(pos)) and list(s["fixed"]) == [0, 1, 2] and s["iters"] == 30
+    ref = "/root/reference"
+    if os.path.isdir(ref):
+        p, t = mesh_io.read_gmsh(os.path.join(ref, "share/mesh/liver.msh"))
+        assert p.shape == (181, 3) and t.shape == (596, 4) and t.min() == 0 and t.max() == 180
+        (s,), info = scene_xml.load_scene(os.path.join(ref, "applications/plugins/SofaCUDA/examples/liver-CUDA.scn"),
+                                          search_paths=[os.path.join(ref, "share")])
+        assert np.array_equal(s["pos"], p) and np.array_equal(s["tets"], t) and list(s["fixed"]) == [3, 39, 64]
+        assert s["mass"] == ("diagonal", 1.0) and s["E"][0] == 3000.0 and s["iters"] == 25
+        assert "SphereCollisionModel" in info["ignored"] and "BarycentricMapping" in info["ignored"]
