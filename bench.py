@@ -396,10 +396,17 @@ def main():
     E = sum(len(s["tets"]) for s in specs)
     N = sum(len(s["pos"]) for s in specs)
     eng, ids = make_engine(specs, capi.FP32, graphs=True)
+    # operator variant (threads per tile, geometry recomputed) picked by the library's autotuner
+    # on these scenes, before the first step
+    variant, variant_us = eng.autotune(10)
+    log(f"autotune: variant {variant}, us per launch {variant_us}")
     total_ms, lat = time_merged(eng, ids, args.steps, args.warmup)
     value = E * args.steps / (total_ms / 1e3)
     roof = apply_roofline(eng, args.apply_reps, 4, 4, 0, E, N)
     roof["kernel"] = "k_tile_apply<float> over the 256-beam mix (CG operator: addDForce + mass + projection + p update)"
+    roof["autotuned_variant"] = {"chosen": variant, "us_per_launch": variant_us,
+                                 "variants": ["512 threads, geometry and K recomputed", "512 threads, rotation recomputed",
+                                              "256 threads, geometry and K recomputed", "256 threads, rotation recomputed"]}
     tr = measured_traffic()
     if tr is not None:
         roof["traffic"] = tr["bytes"]
