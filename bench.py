@@ -375,6 +375,8 @@ def main():
     ap.add_argument("--no-fp64-ref", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-sofacuda", action="store_true")
+    ap.add_argument("--variant", type=int, default=-1,
+                    help="operator variant for the headline engine (0-3); -1 lets the autotuner pick")
     ap.add_argument("--sofacuda-steps", type=int, default=1000)
     ap.add_argument("--serve-seconds", type=float, default=15.0)
     ap.add_argument("--serve-arrivals", type=float, default=5.0)
@@ -398,7 +400,11 @@ def main():
     eng, ids = make_engine(specs, capi.FP32, graphs=True)
     # operator variant (threads per tile, geometry recomputed) picked by the library's autotuner
     # on these scenes, before the first step
-    variant, variant_us = eng.autotune(10)
+    if args.variant < 0:
+        variant, variant_us = eng.autotune(10)
+    else:
+        eng.set_apply_variant(args.variant)
+        variant, variant_us = args.variant, None
     log(f"autotune: variant {variant}, us per launch {variant_us}")
     total_ms, lat = time_merged(eng, ids, args.steps, args.warmup)
     value = E * args.steps / (total_ms / 1e3)
