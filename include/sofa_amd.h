@@ -231,6 +231,16 @@ int sofa_amd_scene_read(sofa_amd_engine eng, int scene, double* x, double* v, do
 /* Overwrites position / velocity (NULL skips); SOFA_AMD_ENAN (nothing written) for a NaN or Inf. */
 int sofa_amd_scene_write(sofa_amd_engine eng, int scene, const double* x, const double* v);
 int sofa_amd_scene_stats(sofa_amd_engine eng, int scene, sofa_amd_cg_stats* out);
+/* TetrahedronFEMForceField::computeVonMisesStress (TetrahedronFEMForceField.inl:2197-2347, the
+ * Data vonMisesPerElement / vonMisesPerNode it fills at each AnimateEndEvent) from the scene's
+ * current positions: mode 1 corotational strain, mode 2 full Green strain (the reference's
+ * computeVonMisesStress values; 0 is "no computation" there and SOFA_AMD_EINVAL here).
+ * per_element: n_tets values in element order, per_node: n_nodes values (the mean over the
+ * node's tetrahedra; NULL skips either).  The element shape functions are built on the first
+ * call for a scene and kept on the device (112 B per element + 32 B per node).  Mode 1 also
+ * stores the element rotations of the current positions, as the reference does.  Blocks until
+ * done; SOFA_AMD_EDEGENERATE when a rest tetrahedron's matrix cannot be inverted. */
+int sofa_amd_scene_von_mises(sofa_amd_engine eng, int scene, int mode, double* per_element, double* per_node);
 /* Device-side element data of one element as the reference stores it (for parity checks):
  * rotations[e] (3x3), K diagonal/off-diagonal/shear (3), rotated rest coordinates (4x3). */
 int sofa_amd_scene_element(sofa_amd_engine eng, int scene, int elem, double* rot9, double* k3,

@@ -163,6 +163,161 @@ struct TetraFEM : public ForceField {
         contrib.assign(12 * elems.size(), 0.0);
     }
 
+    // computeVonMisesStress state (TetrahedronFEMForceField.h: elemLambda, elemMu, elemShapeFun)
+    std::vector<Real> elemLambda, elemMu;
+    std::vector<std::array<Real, 16>> elemShapeFun;
+
+    // sofa/type/Mat.h:1104-1168, the general inverse: Gauss-Jordan with full pivoting (first
+    // strictly largest |m1(i,j)| over unused rows i and columns j, scanned row by row), the pivot
+    // row divided by the pivot, every other row reduced by it, rows permuted back at the end
+    static bool invertMatrix4(const Real from[16], Real dest[16]) {
+        Real m1[4][4], m2[4][4];
+        int r[4] = {0, 0, 0, 0}, c[4] = {0, 0, 0, 0}, row[4] = {0, 0, 0, 0}, col[4] = {0, 0, 0, 0};
+        for (int i = 0; i < 4; i++)
+            for (int j = 0; j < 4; j++) {
+                m1[i][j] = from[4 * i + j];
+                m2[i][j] = (i == j) ? 1.0 : 0.0;
+            }
+        for (int k = 0; k < 4; k++) {
+            Real pivot = 0;
+            for (int i = 0; i < 4; i++) {
+                if (row[i]) continue;
+                for (int j = 0; j < 4; j++) {
+                    if (col[j]) continue;
+                    Real t = m1[i][j];
+                    if (t < 0) t = -t;
+                    if (t > pivot) {
+                        pivot = t;
+                        r[k] = i;
+                        c[k] = j;
+                    }
+                }
+            }
+            if (std::fabs(pivot) <= std::numeric_limits<Real>::epsilon()) return false;   // equalsZero
+            row[r[k]] = col[c[k]] = 1;
+            pivot = m1[r[k]][c[k]];
+            for (int j = 0; j < 4; j++) m1[r[k]][j] /= pivot;
+            m1[r[k]][c[k]] = 1;
+            for (int j = 0; j < 4; j++) m2[r[k]][j] /= pivot;
+            for (int i = 0; i < 4; i++) {
+                if (i == r[k]) continue;
+                const Real f = m1[i][c[k]];
+                Real t1[4], t2[4];
+                for (int j = 0; j < 4; j++) t1[j] = m1[r[k]][j] * f;
+                for (int j = 0; j < 4; j++) m1[i][j] -= t1[j];
+                m1[i][c[k]] = 0;
+                for (int j = 0; j < 4; j++) t2[j] = m2[r[k]][j] * f;
+                for (int j = 0; j < 4; j++) m2[i][j] -= t2[j];
+            }
+        }
+        for (int i = 0; i < 4; i++)
+            for (int j = 0; j < 4; j++)
+                if (c[j] == i) row[i] = r[j];
+        for (int i = 0; i < 4; i++)
+            for (int j = 0; j < 4; j++) dest[4 * i + j] = m2[row[i]][j];
+        return true;
+    }
+
+    // reinit's von Mises set-up (:1521-1542): matVert rows (1, X0[vertex k])
+    bool initVonMises() {
+        elemShapeFun.assign(elems.size(), {});
+        for (size_t i = 0; i < elems.size(); i++) {
+            Real mv[16];
+            for (int k = 0; k < 4; k++) {
+                mv[4 * k] = 1.0;
+                for (int l = 1; l < 4; l++) mv[4 * k + l] = initialPoints[elems[i][k]][l - 1];
+            }
+            if (!invertMatrix4(mv, elemShapeFun[i].data())) return false;
+        }
+        return true;
+    }
+
+    // computeVonMisesStress (:2197-2347) at positions X; mode 1 corotational strain (method
+    // large), 2 full Green strain; vME per element, vMN per node (mean over the node's tetrahedra)
+    void computeVonMisesStress(int mode, const std::vector<Vec3>& X, std::vector<Real>& vME, std::vector<Real>& vMN) {
+        const size_t N = X.size();
+        std::vector<Vec3> U(N);
+        for (size_t i = 0; i < initialPoints.size(); i++) U[i] = sub(X[i], initialPoints[i]);
+        vME.assign(elems.size(), 0.0);
+        for (size_t el = 0; el < elems.size(); el++) {
+            const Tetra& idx = elems[el];
+            const auto& shf = elemShapeFun[el];
+            Real vStrain[6];
+            Real gradU[3][3];
+            if (mode == 2) {
+                for (int k = 0; k < 3; k++)
+                    for (int l = 0; l < 3; l++) {
+                        gradU[k][l] = 0.0;
+                        for (int m = 0; m < 4; m++) gradU[k][l] += shf[4 * (l + 1) + m] * U[idx[m]][k];
+                    }
+                // 0.5 (gradU + gradU^T + gradU^T gradU): Mat sums, the 3x3 product (Mat.h:1444-1470)
+                Real strain[3][3];
+                for (int i = 0; i < 3; i++)
+                    for (int j = 0; j < 3; j++) {
+                        const Real sum = gradU[i][j] + gradU[j][i];
+                        const Real prod = gradU[0][i] * gradU[0][j] + gradU[1][i] * gradU[1][j] + gradU[2][i] * gradU[2][j];
+                        strain[i][j] = (Real)0.5 * (sum + prod);
+                    }
+                for (int i = 0; i < 3; i++) vStrain[i] = strain[i][i];
+                vStrain[3] = strain[1][2];
+                vStrain[4] = strain[0][2];
+                vStrain[5] = strain[0][1];
+            } else {
+                // method large: the element frame of X, rotations[el] updated as a side effect
+                const Mat3 R02 = computeRotationLarge(X, idx[0], idx[1], idx[2]);
+                rotations[el] = transpose(R02);
+                Vec3 deforme[4];
+                for (int i = 0; i < 4; i++) deforme[i] = mul(R02, X[idx[i]]);
+                deforme[1][0] -= deforme[0][0];
+                deforme[2][0] -= deforme[0][0];
+                deforme[2][1] -= deforme[0][1];
+                deforme[3] = sub(deforme[3], deforme[0]);
+                const auto& ri = rotatedInitial[el];
+                Real D[12];
+                D[0] = 0; D[1] = 0; D[2] = 0;
+                D[3] = ri[1][0] - deforme[1][0];
+                D[4] = 0; D[5] = 0;
+                D[6] = ri[2][0] - deforme[2][0];
+                D[7] = ri[2][1] - deforme[2][1];
+                D[8] = 0;
+                D[9] = ri[3][0] - deforme[3][0];
+                D[10] = ri[3][1] - deforme[3][1];
+                D[11] = ri[3][2] - deforme[3][2];
+                for (int k = 0; k < 3; k++)
+                    for (int l = 0; l < 3; l++) {
+                        gradU[k][l] = 0.0;
+                        for (int m = 0; m < 4; m++) gradU[k][l] += shf[4 * (l + 1) + m] * D[3 * m + k];
+                    }
+                Real strain[3][3];
+                for (int i = 0; i < 3; i++)
+                    for (int j = 0; j < 3; j++) strain[i][j] = Real(0.5) * (gradU[i][j] + gradU[j][i]);
+                for (int i = 0; i < 3; i++) vStrain[i] = strain[i][i];
+                vStrain[3] = strain[1][2];
+                vStrain[4] = strain[0][2];
+                vStrain[5] = strain[0][1];
+            }
+            const Real lambda = elemLambda[el], mu = elemMu[el];
+            Real sv[6];
+            Real traceStrain = 0.0;
+            for (int k = 0; k < 3; k++) {
+                traceStrain += vStrain[k];
+                sv[k] = vStrain[k] * 2 * mu;
+            }
+            for (int k = 3; k < 6; k++) sv[k] = vStrain[k] * 2 * mu;
+            for (int k = 0; k < 3; k++) sv[k] += lambda * traceStrain;
+            vME[el] = std::sqrt(sv[0] * sv[0] + sv[1] * sv[1] + sv[2] * sv[2] - sv[0] * sv[1] - sv[1] * sv[2] - sv[2] * sv[0] +
+                                3 * sv[3] * sv[3] + 3 * sv[4] * sv[4] + 3 * sv[5] * sv[5]);
+            if (vME[el] < 1e-10) vME[el] = 0.0;
+        }
+        ensureIncidence(N);
+        vMN.assign(N, 0.0);
+        for (size_t dof = 0; dof < N; dof++) {
+            const int s0 = incPtr[dof], s1 = incPtr[dof + 1];
+            for (int q = s0; q < s1; q++) vMN[dof] += vME[incIdx[q] / 4];
+            if (s1 > s0) vMN[dof] /= Real(s1 - s0);
+        }
+    }
+
     Real& Kel(size_t i, int r, int c) { return K[i][r * 6 + c]; }
     Real& Jel(size_t i, int r, int c) { return J[i][r * 6 + c]; }
 
@@ -228,6 +383,9 @@ struct TetraFEM : public ForceField {
         Km[3 * 6 + 3] = Km[4 * 6 + 4] = Km[5 * 6 + 5] = (1 - 2 * poissonRatio) / (2 * (1 - poissonRatio));
         const Real s = (youngModulus * (1 - poissonRatio)) / ((1 + poissonRatio) * (1 - 2 * poissonRatio));
         for (auto& k : Km) k *= s;
+        // :278-282 (computeVonMisesStress set): the Lame coefficients before the volume division
+        elemLambda[i] = Km[0 * 6 + 1];
+        elemMu[i] = Km[3 * 6 + 3];
         const Real vol = tet_volume(initialPoints[a], initialPoints[b], initialPoints[c], initialPoints[d]);
         restVolume += vol;
         const Real div = vol * 36;
@@ -283,6 +441,9 @@ struct TetraFEM : public ForceField {
         const size_t ne = elems.size();
         K.assign(ne, {});
         J.assign(ne, {});
+        elemLambda.assign(ne, 0.0);
+        elemMu.assign(ne, 0.0);
+        elemShapeFun.clear();
         restVolume = 0;
         if (method == LARGE) {
             rotations.assign(ne, Mat3());
@@ -1129,6 +1290,21 @@ void orc_fem_get_element(void* h, int e, double* R0, double* R, double* K, doubl
 }
 
 double orc_fem_rest_volume(void* h) { return static_cast<Scene*>(h)->fem->restVolume; }
+
+// computeVonMisesStress from the current positions (mode 1 corotational, 2 Green strain):
+// per element (n_tets) and per node (n_nodes); returns 0, or -1 when a rest element's matrix
+// has no inverse or the mode / method does not allow it
+int orc_von_mises(void* h, int mode, double* per_elem, double* per_node) {
+    Scene* s = static_cast<Scene*>(h);
+    TetraFEM* fem = s->fem.get();
+    if (!fem || (mode != 1 && mode != 2) || (mode == 1 && fem->method != TetraFEM::LARGE)) return -1;
+    if (fem->elemShapeFun.size() != fem->elems.size() && !fem->initVonMises()) return -1;
+    std::vector<Real> vme, vmn;
+    fem->computeVonMisesStress(mode, s->x, vme, vmn);
+    for (size_t e = 0; e < vme.size(); e++) per_elem[e] = vme[e];
+    for (size_t n = 0; n < vmn.size(); n++) per_node[n] = vmn[n];
+    return 0;
+}
 
 void orc_get_masses(void* h, double* out) {
     Scene* s = static_cast<Scene*>(h);

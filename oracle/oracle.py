@@ -55,6 +55,8 @@ def lib():
         L.orc_fem_add_force.argtypes = [ctypes.c_void_p, _dp, _dp]
         L.orc_fem_add_dforce.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_double]
         L.orc_fem_get_element.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp, _dp, _dp, _dp]
+        L.orc_von_mises.restype = ctypes.c_int
+        L.orc_von_mises.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp]
         L.orc_fem_rest_volume.restype = ctypes.c_double
         L.orc_fem_rest_volume.argtypes = [ctypes.c_void_p]
         L.orc_get_masses.argtypes = [ctypes.c_void_p, _dp]
@@ -102,6 +104,7 @@ class OracleScene:
         rc = lib().orc_set_fem(self._h, len(t) // 4, tp, yp, len(y), pp, len(p), m)
         if rc != 0:
             raise ValueError("invalid tetrahedron index")
+        self.n_tets = len(t) // 4
 
     def set_uniform_mass(self, vertex_mass=None, total_mass=None):
         if total_mass is not None:
@@ -201,6 +204,14 @@ class OracleScene:
                                   K.ctypes.data_as(_dp), J.ctypes.data_as(_dp), rest.ctypes.data_as(_dp))
         return {"R0": R0.reshape(3, 3), "R": R.reshape(3, 3), "K": K.reshape(6, 6), "J": J.reshape(12, 6),
                 "rest": rest.reshape(4, 3)}
+
+    def von_mises(self, mode=1):
+        """computeVonMisesStress (TetrahedronFEMForceField.inl:2197-2347) at the current positions:
+        (per element, per node)."""
+        pe = np.zeros(self.n_tets); pn = np.zeros(self.n)
+        if lib().orc_von_mises(self._h, int(mode), pe.ctypes.data_as(_dp), pn.ctypes.data_as(_dp)) != 0:
+            raise ValueError("von Mises stress not available (mode, method or a singular rest element)")
+        return pe, pn
 
     def rest_volume(self):
         return lib().orc_fem_rest_volume(self._h)

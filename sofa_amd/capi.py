@@ -25,6 +25,7 @@ EXPORTS = [
     "sofa_amd_scene_set_mass", "sofa_amd_scene_set_gravity", "sofa_amd_scene_set_fixed",
     "sofa_amd_scene_set_solver", "sofa_amd_engine_commit", "sofa_amd_engine_step", "sofa_amd_engine_sync",
     "sofa_amd_scene_read", "sofa_amd_scene_write", "sofa_amd_scene_stats", "sofa_amd_scene_element",
+    "sofa_amd_scene_von_mises",
     "sofa_amd_fem_add_force", "sofa_amd_fem_add_dforce", "sofa_amd_engine_bench_apply", "sofa_amd_scene_tiling",
     "sofa_amd_engine_set_graphs", "sofa_amd_scene_set_mass_density", "sofa_amd_scene_add_oscillators",
     "sofa_amd_scene_set_refinement", "sofa_amd_engine_reserve", "sofa_amd_scene_commit", "sofa_amd_scene_release",
@@ -81,6 +82,7 @@ def lib():
         "sofa_amd_scene_write": (c_int, [_vp, c_int, _dp, _dp]),
         "sofa_amd_scene_stats": (c_int, [_vp, c_int, ctypes.POINTER(CGStats)]),
         "sofa_amd_scene_element": (c_int, [_vp, c_int, c_int, _dp, _dp, _dp]),
+        "sofa_amd_scene_von_mises": (c_int, [_vp, c_int, c_int, _dp, _dp]),
         "sofa_amd_fem_add_force": (c_int, [_vp, c_int, _vp, _vp, _vp]),
         "sofa_amd_fem_add_dforce": (c_int, [_vp, c_int, _vp, _vp, c_double, _vp]),
         "sofa_amd_engine_bench_apply": (c_int, [_vp, c_int, c_int, _vp]),
@@ -309,6 +311,14 @@ class Engine:
         check(lib().sofa_amd_scene_element(self._h, int(scene), int(e), rot.ctypes.data_as(_dp),
                                            k.ctypes.data_as(_dp), rest.ctypes.data_as(_dp)))
         return rot.reshape(3, 3), k, rest.reshape(4, 3)
+
+    def von_mises(self, scene, mode=1):
+        """TetrahedronFEMForceField computeVonMisesStress (1: corotational strain, 2: Green strain)
+        from the scene's current positions: (per element, per node)."""
+        pe = np.zeros(self.n_tets[scene]); pn = np.zeros(self.n_nodes[scene])
+        check(lib().sofa_amd_scene_von_mises(self._h, int(scene), int(mode), pe.ctypes.data_as(_dp),
+                                             pn.ctypes.data_as(_dp)))
+        return pe, pn
 
     # ---- component-level calls on device pointers ----
     def fem_add_force(self, scene, x_dev_ptr, f_dev_ptr, stream=None):

@@ -143,6 +143,55 @@ static bool init_element(const std::vector<V3>& P, const int32_t* t, double E, d
         for (int i = 0; i < 3; i++) out.rest[3 * (k - 1) + i] = ri[k].v[i];
     return vol > 0.0 && std::isfinite(out.k[0]);
 }
+
+// the reference's general matrix inverse (sofa/type/Mat.h:1104-1168), used for the shape
+// functions of computeVonMisesStress: Gauss-Jordan elimination with full pivoting, the pivot the
+// first strictly largest |entry| in row-major order over unused rows and columns, the pivot row
+// divided, every other row reduced, rows permuted back at the end; false when a pivot is
+// within the type's epsilon of zero
+static bool invert4(const double a[4][4], double out[4][4]) {
+#pragma clang fp contract(off)
+    double m1[4][4], m2[4][4];
+    int r[4] = {0, 0, 0, 0}, c[4] = {0, 0, 0, 0}, row[4] = {0, 0, 0, 0}, col[4] = {0, 0, 0, 0};
+    for (int i = 0; i < 4; i++)
+        for (int j = 0; j < 4; j++) {
+            m1[i][j] = a[i][j];
+            m2[i][j] = i == j ? 1.0 : 0.0;
+        }
+    for (int k = 0; k < 4; k++) {
+        double pivot = 0;
+        for (int i = 0; i < 4; i++) {
+            if (row[i]) continue;
+            for (int j = 0; j < 4; j++) {
+                if (col[j]) continue;
+                double t = m1[i][j];
+                if (t < 0) t = -t;
+                if (t > pivot) { pivot = t; r[k] = i; c[k] = j; }
+            }
+        }
+        if (std::abs(pivot) <= std::numeric_limits<double>::epsilon()) return false;
+        const int rk = r[k], ck = c[k];
+        row[rk] = col[ck] = 1;
+        pivot = m1[rk][ck];
+        for (int j = 0; j < 4; j++) m1[rk][j] /= pivot;
+        m1[rk][ck] = 1;
+        for (int j = 0; j < 4; j++) m2[rk][j] /= pivot;
+        for (int i = 0; i < 4; i++) {
+            if (i == rk) continue;
+            const double f = m1[i][ck];
+            for (int j = 0; j < 4; j++) m1[i][j] -= m1[rk][j] * f;
+            m1[i][ck] = 0;
+            for (int j = 0; j < 4; j++) m2[i][j] -= m2[rk][j] * f;
+        }
+    }
+    for (int i = 0; i < 4; i++)
+        for (int j = 0; j < 4; j++)
+            if (c[j] == i) row[i] = r[j];
+    for (int i = 0; i < 4; i++)
+        for (int j = 0; j < 4; j++) out[i][j] = m2[row[i]][j];
+    return true;
+}
+
 }  // namespace host
 
 // ---------------------------------------------------------------------------------------
@@ -272,6 +321,7 @@ struct EngineBase {
     bool stepped = false;             // a step has been enqueued
     virtual int autotune(int reps, hipStream_t s, float* us_out) = 0;
     virtual void set_apply_variant(int v) = 0;
+    virtual void von_mises(int scene, int mode, double* per_elem, double* per_node) = 0;
 };
 
 template <typename T>
@@ -374,6 +424,7 @@ struct EngineT : EngineBase {
 
     ~EngineT() override {
         sets.clear();
+        while (!vmd.empty()) free_vm(vmd.begin()->first);
         free_device();
     }
     void free_device() {
@@ -1032,6 +1083,7 @@ struct EngineT : EngineBase {
         sets.clear();                     // launch sets holding the scene's blocks
         cur = nullptr;
         release_ranges(place[si]);
+        free_vm(si);
         iperm[si].clear();
         iperm[si].shrink_to_fit();
         std::lock_guard<std::mutex> g(prep_mu);
@@ -1452,6 +1504,85 @@ struct EngineT : EngineBase {
         if (gn > 0) k_gather<VT><<<gn, kBlock, 0, s>>>(inc_ptr, inc, nb0, nb0 + sc.n_nodes, ef, (VT*)dfd, 1);
         HIPCHK(hipGetLastError());
     }
+
+    // computeVonMisesStress (TetrahedronFEMForceField.inl:2197-2347) from the scene's current
+    // positions: per scene, built on first use, rows 1..3 of the inverse of each element's rest
+    // matrix [1 x y z] (init, :1521-1542) and its Lame coefficients (computeMaterialStiffness:
+    // K(0,1) and K(3,3) before the division by 36 V, :278-282), in the engine's element order;
+    // the rest positions (mode 2); the output buffers
+    struct VmData {
+        double *shf = nullptr, *lm = nullptr, *x0 = nullptr, *vme = nullptr, *vmn = nullptr;
+    };
+    std::map<int, VmData> vmd;
+    void free_vm(int si) {
+        auto it = vmd.find(si);
+        if (it == vmd.end()) return;
+        for (double* ptr : {it->second.shf, it->second.lm, it->second.x0, it->second.vme, it->second.vmn})
+            if (ptr) (void)hipFree(ptr);
+        vmd.erase(it);
+    }
+    void von_mises(int scene, int mode, double* pe, double* pn) override {
+        HIPCHK(hipSetDevice(device));
+        HIPCHK(hipDeviceSynchronize());
+        const SceneHost& sc = scenes[scene];
+        const int64_t nE = (int64_t)sc.tets.size() / 4;
+        const int nN = sc.n_nodes;
+        const std::vector<int>& ip = iperm[scene];
+        auto it = vmd.find(scene);
+        if (it == vmd.end()) {
+            std::vector<double> shf(12 * (size_t)nE), lm(2 * (size_t)nE);
+            for (int64_t e = 0; e < nE; e++) {
+                const int32_t* t = &sc.tets[4 * (size_t)e];
+                double M[4][4], inv[4][4];
+                for (int k = 0; k < 4; k++) {
+                    M[k][0] = 1.0;
+                    for (int l = 1; l < 4; l++) M[k][l] = sc.rest[3 * (size_t)t[k] + l - 1];
+                }
+                if (!host::invert4(M, inv))
+                    throw std::domain_error("rest tetrahedron " + std::to_string(e) + " has no shape functions");
+                const size_t pos = (size_t)ip[e];
+                for (int l = 0; l < 3; l++)
+                    for (int m = 0; m < 4; m++) shf[12 * pos + 4 * l + m] = inv[l + 1][m];
+                const double E = sc.young.size() > (size_t)e ? sc.young[e] : sc.young[0];
+                const double nu = sc.poisson.size() > (size_t)e ? sc.poisson[e] : sc.poisson[0];
+                double ks[3];
+                host::material_factors(E, nu, ks);
+                lm[2 * pos] = ks[1];
+                lm[2 * pos + 1] = ks[2];
+            }
+            VmData d;
+            d.shf = dalloc<double>(shf.size());
+            d.lm = dalloc<double>(lm.size());
+            d.x0 = dalloc<double>(3 * (size_t)nN);
+            d.vme = dalloc<double>((size_t)nE);
+            d.vmn = dalloc<double>((size_t)nN);
+            it = vmd.emplace(scene, d).first;
+            if (nE > 0) {
+                HIPCHK(hipMemcpy(d.shf, shf.data(), shf.size() * sizeof(double), hipMemcpyHostToDevice));
+                HIPCHK(hipMemcpy(d.lm, lm.data(), lm.size() * sizeof(double), hipMemcpyHostToDevice));
+            }
+            if (nN > 0) HIPCHK(hipMemcpy(d.x0, sc.rest.data(), 3 * (size_t)nN * sizeof(double), hipMemcpyHostToDevice));
+        }
+        const VmData& d = it->second;
+        const int nb0 = node_begin(scene);
+        const int64_t eb0 = elem_begin(scene);
+        if (nE > 0) {
+            const int grid = (int)((nE + kBlock - 1) / kBlock);
+            // x of the scene (node ids of the element record minus nb0 index it), as add_force's
+            k_von_mises_elem<R, NR, VT><<<grid, kBlock, 0, stream>>>(view(), eb0, eb0 + nE, nb0, x + 3 * (int64_t)nb0,
+                                                                     d.x0, d.shf, d.lm, mode, d.vme);
+        }
+        const int gn = (nN + kBlock - 1) / kBlock;
+        if (gn > 0) k_von_mises_node<<<gn, kBlock, 0, stream>>>(inc_ptr, inc, nb0, nb0 + nN, eb0, d.vme, d.vmn);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(stream));
+        if (pe && nE > 0) {
+            std::vector<double> tmp((size_t)nE);
+            HIPCHK(hipMemcpy(tmp.data(), d.vme, tmp.size() * sizeof(double), hipMemcpyDeviceToHost));
+            for (int64_t e = 0; e < nE; e++) pe[e] = tmp[(size_t)ip[e]];
+        }
+        if (pn && nN > 0) HIPCHK(hipMemcpy(pn, d.vmn, (size_t)nN * sizeof(double), hipMemcpyDeviceToHost));
+    }
 };
 
 #if defined(SOFA_AMD_TU_FP32)
@@ -1862,6 +1993,16 @@ int sofa_amd_scene_element(sofa_amd_engine eng, int scene, int elem, double* rot
         return fail(SOFA_AMD_EINVAL, "invalid argument");
     if (!usable(eng, scene)) return fail(SOFA_AMD_ESTATE, "scene not committed");
     eng->impl->element(scene, elem, rot9, k3, rest12);
+    return SOFA_AMD_OK;
+    GUARD_END
+}
+
+int sofa_amd_scene_von_mises(sofa_amd_engine eng, int scene, int mode, double* per_element, double* per_node) {
+    GUARD_BEGIN
+    SceneHost* s = scene_of(eng, scene);
+    if (!s || (mode != 1 && mode != 2)) return fail(SOFA_AMD_EINVAL, "invalid argument (mode must be 1 or 2)");
+    if (!usable(eng, scene)) return fail(SOFA_AMD_ESTATE, "scene not committed");
+    eng->impl->von_mises(scene, mode, per_element, per_node);
     return SOFA_AMD_OK;
     GUARD_END
 }

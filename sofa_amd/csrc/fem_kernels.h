@@ -509,4 +509,143 @@ __global__ __launch_bounds__(kBlock) void k_gather(const int* __restrict__ inc_p
     out[o] = a[0]; out[o + 1] = a[1]; out[o + 2] = a[2];
 }
 
+// ---------------------------------------------------------------------------------------
+// von Mises stress (TetrahedronFEMForceField::computeVonMisesStress, .inl:2197-2347)
+// ---------------------------------------------------------------------------------------
+// Per element, from the scene's current positions x, in double whatever the engine's precision
+// (a post-process read once per step at most): the displacement gradient
+// gradU(k,l) = sum_m shf(l+1,m) u_m[k] with shf the inverse of the rest matrix [1 x y z] per
+// vertex and u_m either the corotational displacement of vertex m in the element frame
+// (mode 1; rest minus current, as the reference's D) or x - x0 (mode 2); the strain
+// (gradU + gradU^T)/2 (mode 1) or (gradU + gradU^T + gradU^T gradU)/2 (mode 2); the stress
+// s = 2 mu strain + lambda tr(strain) I in Voigt order (xx, yy, zz, yz, xz, xy); von Mises
+// sqrt(s0^2 + s1^2 + s2^2 - s0 s1 - s1 s2 - s2 s0 + 3 (s3^2 + s4^2 + s5^2)), values below 1e-10
+// set to 0.  Mode 1 also stores the element rotation of x, as the reference does.  Every
+// operation in the reference's order, no contraction where the translation unit honours the
+// pragma (the FP64 engines': they match the reference bit for bit).
+// shf: 12 values per element (rows 1..3 of the inverse, row-major), lm: lambda, mu per element,
+// both in the engine's element order of the scene; vme in that order.  x and x0 are the scene's
+// (node ids of the element record minus nb index them).
+template <typename R, int NR, typename VT>
+__global__ __launch_bounds__(kBlock) void k_von_mises_elem(ElemView<R, NR> ev, int64_t e0, int64_t e1, int nb,
+                                                           const VT* __restrict__ x,
+                                                           const double* __restrict__ x0,
+                                                           const double* __restrict__ shf,
+                                                           const double* __restrict__ lm, int mode,
+                                                           double* __restrict__ vme) {
+#pragma clang fp contract(off)
+    const int64_t e = e0 + (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (e >= e1) return;
+    const int64_t le = e - e0;
+    const int4 t = ev.tet[e];
+    const int nid[4] = {t.x - nb, t.y - nb, t.z - nb, t.w - nb};
+    double p[4][3];
+#pragma unroll
+    for (int m = 0; m < 4; m++) {
+        VT a, b, c;
+        load3(x, nid[m], a, b, c);
+        p[m][0] = (double)a; p[m][1] = (double)b; p[m][2] = (double)c;
+    }
+    double D[12];
+    if (mode == 2) {
+#pragma unroll
+        for (int m = 0; m < 4; m++)
+#pragma unroll
+            for (int k = 0; k < 3; k++) D[3 * m + k] = p[m][k] - x0[3 * (int64_t)nid[m] + k];
+    } else {
+        R gr[12], rcr[9];
+        jgroups<R, NR>(ev, e, gr, rcr);
+        double rc[9];
+#pragma unroll
+        for (int k = 0; k < 9; k++) rc[k] = (double)rcr[k];
+        double rot[9];
+        rotation_large<double>(p[0], p[1], p[2], rot);
+#pragma unroll
+        for (int k = 0; k < 9; k++) ev.rot[k * ev.E + e] = (R)rot[k];
+        // deforme[k] = R_0_2 p_k, R_0_2 = transpose(rot); then relative to deforme[0]
+        double df[4][3];
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+#pragma unroll
+            for (int i = 0; i < 3; i++) {
+                double v = rot[i] * p[k][0];
+                v = v + rot[3 + i] * p[k][1];
+                df[k][i] = v + rot[6 + i] * p[k][2];
+            }
+        df[1][0] -= df[0][0];
+        df[2][0] -= df[0][0];
+        df[2][1] -= df[0][1];
+#pragma unroll
+        for (int i = 0; i < 3; i++) df[3][i] -= df[0][i];
+        D[0] = 0; D[1] = 0; D[2] = 0;
+        D[3] = rc[0] - df[1][0]; D[4] = 0; D[5] = 0;
+        D[6] = rc[3] - df[2][0]; D[7] = rc[4] - df[2][1]; D[8] = 0;
+        D[9] = rc[6] - df[3][0]; D[10] = rc[7] - df[3][1]; D[11] = rc[8] - df[3][2];
+    }
+    const double* sf = shf + 12 * le;
+    double gu[3][3];
+#pragma unroll
+    for (int k = 0; k < 3; k++)
+#pragma unroll
+        for (int l = 0; l < 3; l++) {
+            double a = 0.0;
+#pragma unroll
+            for (int m = 0; m < 4; m++) a += sf[4 * l + m] * D[3 * m + k];
+            gu[k][l] = a;
+        }
+    double st[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            double a = gu[i][j] + gu[j][i];
+            if (mode == 2) {
+                // (gradU^T gradU)(i,j), the 3x3 product's three-term order
+                double g = gu[0][i] * gu[0][j] + gu[1][i] * gu[1][j];
+                g = g + gu[2][i] * gu[2][j];
+                a = a + g;
+            }
+            st[i][j] = a * 0.5;
+        }
+    const double vs[6] = {st[0][0], st[1][1], st[2][2], st[1][2], st[0][2], st[0][1]};
+    const double lambda = lm[2 * le], mu = lm[2 * le + 1];
+    double s[6], tr = 0.0;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        tr += vs[k];
+        s[k] = vs[k] * 2 * mu;
+    }
+#pragma unroll
+    for (int k = 3; k < 6; k++) s[k] = vs[k] * 2 * mu;
+#pragma unroll
+    for (int k = 0; k < 3; k++) s[k] += lambda * tr;
+    double q = s[0] * s[0] + s[1] * s[1];
+    q = q + s[2] * s[2];
+    q = q - s[0] * s[1];
+    q = q - s[1] * s[2];
+    q = q - s[2] * s[0];
+    q = q + 3 * s[3] * s[3];
+    q = q + 3 * s[4] * s[4];
+    q = q + 3 * s[5] * s[5];
+    double v = sqrt(q);
+    if (v < 1e-10) v = 0.0;
+    vme[le] = v;
+}
+
+// vonMisesPerNode: the mean of the element values around each node, summed over its incidence
+// list (ascending element id, getTetrahedraAroundVertex's order); 0 for a node of no element
+static __global__ __launch_bounds__(kBlock) void k_von_mises_node(const int* __restrict__ inc_ptr, const int* __restrict__ inc,
+                                                           int n0, int n1, int64_t e0,
+                                                           const double* __restrict__ vme,
+                                                           double* __restrict__ vmn) {
+#pragma clang fp contract(off)
+    const int n = n0 + blockIdx.x * kBlock + threadIdx.x;
+    if (n >= n1) return;
+    const int s0 = inc_ptr[n], s1 = inc_ptr[n + 1];
+    double a = 0.0;
+    for (int s = s0; s < s1; s++) a += vme[(int64_t)(inc[s] >> 2) - e0];
+    if (s1 > s0) a /= (double)(s1 - s0);
+    vmn[n - n0] = a;
+}
+
 }  // namespace sofa_amd

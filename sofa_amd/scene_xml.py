@@ -16,7 +16,8 @@ unchanged. Read, with their Data names and defaults:
 * ``MeshGmshLoader`` / ``MeshVTKLoader``: ``filename`` (Gmsh 1 / 2.2 / 4.1 or legacy VTK ASCII,
   ``sofa_amd.mesh_io``), ``scale3d``, ``translation``;
 * ``TetrahedronFEMForceField``: ``youngModulus``, ``poissonRatio`` (lists), ``method`` (only
-  ``large``); a topology with hexahedra only is tessellated as its init does;
+  ``large``), ``computeVonMisesStress`` (kept as the spec's ``von_mises`` mode, 1 or 2, for
+  ``Engine.von_mises``); a topology with hexahedra only is tessellated as its init does;
 * ``UniformMass`` (``vertexMass`` or ``totalMass``), ``DiagonalMass`` (``massDensity`` or
   ``totalMass``), ``MeshMatrixMass`` (``massDensity``, ``lumping``);
 * ``BoxROI`` (``box``: one or more 6-tuples, bounds inclusive), ``FixedProjectiveConstraint``
@@ -246,20 +247,23 @@ def load_scene(path_or_text, search_paths=()):
                 raise ValueError("BoxROI without positions")
             o.data["indices"] = box_indices(p, o.attr.get("box", "0 0 0 1 1 1"))
 
-    def visit(node):
-        solver = node.first("EulerImplicitSolver")
+    # solvers govern their node's subtree (a context lookup searches up the graph): the nearest
+    # EulerImplicitSolver / CGLinearSolver at or above a body's node is its solver
+    def visit(node, solver_up, cg_up):
+        solver = node.first("EulerImplicitSolver") or solver_up
+        cg = node.first("CGLinearSolver") or cg_up
         fem = node.first("TetrahedronFEMForceField")
         mo = node.first("MechanicalObject")
         if solver is not None and fem is not None and mo is not None:
-            bodies.append(_body(node, solver, fem, mo, g, dt, node_positions))
+            bodies.append(_body(node, solver, cg, fem, mo, g, dt, node_positions))
             names.append(node.name)
         for c in node.children:
-            visit(c)
-    visit(root)
+            visit(c, solver, cg)
+    visit(root, None, None)
     return bodies, {"bodies": names, "ignored": sorted(set(ignored))}
 
 
-def _body(node, solver, fem, mo, g, dt, node_positions):
+def _body(node, solver, cg, fem, mo, g, dt, node_positions):
     pos = node_positions(node)
     if pos is None:
         raise ValueError(f"node {node.name}: no positions")
@@ -283,6 +287,9 @@ def _body(node, solver, fem, mo, g, dt, node_positions):
     n = len(pos)
     spec = dict(pos=pos, tets=np.ascontiguousarray(tets, dtype=np.int32), E=np.array(E), nu=np.array(nu),
                 g=tuple(g), dt=dt)
+    vm = int(fem.attr.get("computeVonMisesStress", "0"))
+    if vm in (1, 2):                      # TetrahedronFEMForceField.inl:57, 0 = no computation
+        spec["von_mises"] = vm
     # mass
     um, dm, mm = node.first("UniformMass"), node.first("DiagonalMass"), node.first("MeshMatrixMass")
     if um is not None:
@@ -330,7 +337,6 @@ def _body(node, solver, fem, mo, g, dt, node_positions):
     spec["integrator"] = dict(first_order=_bool(a.get("firstOrder", "false")),
                               trapezoidal=_bool(a.get("trapezoidalScheme", "false")),
                               vdamping=float(a.get("vdamping", "0")))
-    cg = node.first("CGLinearSolver")
     c = cg.attr if cg is not None else {}
     if _bool(c.get("warmStart", "false")):
         raise ValueError("CGLinearSolver warmStart is not on this path")

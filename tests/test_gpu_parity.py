@@ -1015,3 +1015,64 @@ def test_operator_variants_and_autotune():
         eng.autotune(3)
     assert ei.value.code == capi.ESTATE
     eng.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec", ["FP64", "FP64_TILED", "FP32", "MIXED"])
+def test_von_mises(prec):
+    """computeVonMisesStress (TetrahedronFEMForceField.inl:2197-2347) through
+    sofa_amd_scene_von_mises against the oracle at the same positions: a jittered beam with
+    per-element E and nu, stepped 30 steps in the oracle, its positions (FP32 engines: rounded to
+    float, in both) written to the engine, which holds another scene before it.  Both modes, per
+    element and per node.  FP64 engines: bit for bit, and mode 1 leaves the same element rotations
+    as the reference's.  FP32 engine (its kernels built with multiply-adds contracted): mode 2
+    within 1e-12 of the largest value (double arithmetic on the same float positions), mode 1
+    within 1e-4 (float rest coordinates: 6e-8 relative error in a coordinate difference that the
+    strain makes 1e-3 of the element size or less)."""
+    capi = _capi()
+    from oracle import OracleScene
+    print(f"case: von Mises, {prec}", flush=True)
+    rng = np.random.default_rng(8)
+    c = beam_cfg0()
+    pos = c["pos"] + rng.uniform(-0.03, 0.03, c["pos"].shape)
+    ne = len(c["tets"])
+    E = rng.uniform(3e4, 3e5, ne)
+    nu = rng.uniform(0.3, 0.45, ne)
+    orc = OracleScene(pos)
+    orc.set_gravity(c["g"])
+    orc.set_fem(c["tets"], E, nu, "large")
+    orc.set_uniform_mass(total_mass=c["total_mass"])
+    orc.set_fixed(c["fixed"])
+    orc.set_solver(c["rs"], c["rm"], 100, 1e-12, 1e-30)
+    orc.step(c["dt"], 30)
+    x = orc.get("x")
+    if prec in ("FP32",):
+        x = x.astype(np.float32).astype(np.float64)
+        orc.set("x", x)
+    eng = capi.Engine(0, getattr(capi, prec))
+    other = add_gpu_scene(eng, beam_medium())
+    sid = eng.add_scene(pos, c["tets"], E, nu, c["fixed"], mass_kind=capi.MASS_UNIFORM,
+                        mass_value=c["total_mass"] / len(pos), gravity=c["g"], dt=c["dt"])
+    eng.commit()
+    eng.step([other], 2)
+    eng.write(sid, x, None)
+    try:
+        for mode in (2, 1):
+            pe, pn = eng.von_mises(sid, mode)
+            oe, on = orc.von_mises(mode)
+            assert np.max(oe) > 0
+            if prec != "FP32":
+                assert np.array_equal(pe, oe), (mode, np.max(np.abs(pe - oe)))
+                assert np.array_equal(pn, on), (mode, np.max(np.abs(pn - on)))
+            else:
+                bar = 1e-12 if mode == 2 else 1e-4
+                assert np.max(np.abs(pe - oe)) <= bar * np.max(oe), (mode, np.max(np.abs(pe - oe)) / np.max(oe))
+                assert np.max(np.abs(pn - on)) <= bar * np.max(on)
+        if prec != "FP32":
+            for e in (0, ne // 2, ne - 1):
+                rot, _, _ = eng.element(sid, e)
+                assert np.array_equal(rot, orc.fem_element(e)["R"]), e
+        with pytest.raises(capi.SofaAmdError):
+            eng.von_mises(sid, 0)
+    finally:
+        eng.close()

@@ -452,3 +452,79 @@ def test_threaded_oracle_bit_identical():
         out.append((s.get("x"), s.get("v"), s.stats()))
     assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
     assert out[0][2] == out[1][2]
+
+
+# ------------------------------------------------------------------------------------------
+# von Mises stress (TetrahedronFEMForceField::computeVonMisesStress, .inl:2197-2347)
+# ------------------------------------------------------------------------------------------
+def _von_mises_closed(strain, E, nu):
+    """von Mises stress of a 3x3 strain tensor under isotropic linear elasticity (Lame lambda,
+    mu from E, nu): sqrt(((s1-s2)^2 + (s2-s3)^2 + (s3-s1)^2) / 2) from the principal stresses."""
+    lam = E * nu / ((1 + nu) * (1 - 2 * nu))
+    mu = E / (2 * (1 + nu))
+    sig = 2 * mu * strain + lam * np.trace(strain) * np.eye(3)
+    p = np.linalg.eigvalsh(0.5 * (sig + sig.T))
+    return np.sqrt(0.5 * ((p[0] - p[1]) ** 2 + (p[1] - p[2]) ** 2 + (p[2] - p[0]) ** 2))
+
+
+def _rotation(rng):
+    q, _ = np.linalg.qr(rng.normal(size=(3, 3)))
+    return q * np.sign(np.linalg.det(q))
+
+
+def test_von_mises_green_strain_affine():
+    """Mode 2 (full Green strain) under x = Q F x0 + t on a jittered beam with per-element E and
+    nu: the shape functions reproduce an affine field exactly, so every element's strain is
+    (F^T F - I) / 2 and its value the principal-stress von Mises of that strain; node values are
+    the mean over the node's tetrahedra.  At rest both modes give exactly 0."""
+    rng = np.random.default_rng(5)
+    pos, tets, _ = meshes.beam((4, 4, 7), (1.0, 1.0, 2.0))
+    pos = pos + rng.uniform(-0.04, 0.04, pos.shape)
+    ne = len(tets)
+    E = rng.uniform(1e4, 1e6, ne)
+    nu = rng.uniform(0.2, 0.45, ne)
+    s = OracleScene(pos)
+    s.set_fem(tets, E, nu, "large")
+    for mode in (1, 2):
+        pe, pn = s.von_mises(mode)
+        assert not np.any(pe) and not np.any(pn), mode
+    F = np.eye(3) + rng.uniform(-0.05, 0.05, (3, 3))
+    Q = _rotation(rng)
+    x = pos @ (Q @ F).T + np.array([0.3, -1.0, 2.0])
+    s.set("x", x)
+    pe, pn = s.von_mises(2)
+    strain = 0.5 * (F.T @ F - np.eye(3))
+    want = np.array([_von_mises_closed(strain, E[e], nu[e]) for e in range(ne)])
+    assert np.max(np.abs(pe - want)) <= 1e-9 * np.max(want), np.max(np.abs(pe - want))
+    acc = np.zeros(len(pos)); cnt = np.zeros(len(pos))
+    np.add.at(acc, tets.reshape(-1), np.repeat(pe, 4)); np.add.at(cnt, tets.reshape(-1), 1)
+    assert np.allclose(pn, acc / cnt, rtol=1e-13, atol=0)
+
+
+def test_von_mises_corotational_aligned_elements():
+    """Mode 1 (corotational strain) on single tetrahedra whose rest frame is the world frame
+    (a at the origin, b on x, c in the xy plane) deformed by an upper-triangular F (the element
+    frame stays the world frame), then moved rigidly by Q, t: the displacement in the element
+    frame is -(F - I) x0, so the strain is the small strain (G + G^T) / 2 of G = F - I, whatever Q;
+    the stored rotation is Q's transpose.  A rigid motion alone gives values at the level of
+    rounding."""
+    rng = np.random.default_rng(6)
+    for trial in range(6):
+        b0, c0, c1, d0, d1, d2 = rng.uniform(0.2, 1.0, 6) * np.array([1, 0.6, 1, 0.5, 0.5, 1])
+        x0 = np.array([[0, 0, 0], [b0, 0, 0], [c0, c1, 0], [d0, d1, d2]], dtype=np.float64)
+        E, nu = rng.uniform(1e3, 1e6), rng.uniform(0.1, 0.49)
+        s = OracleScene(x0)
+        s.set_fem(np.array([[0, 1, 2, 3]], np.int32), [E], [nu], "large")
+        G = np.triu(rng.uniform(-0.08, 0.08, (3, 3)))
+        Q = _rotation(rng)
+        t = rng.uniform(-1, 1, 3)
+        s.set("x", x0 @ (Q @ (np.eye(3) + G)).T + t)
+        pe, pn = s.von_mises(1)
+        want = _von_mises_closed(0.5 * (G + G.T), E, nu)
+        assert abs(pe[0] - want) <= 1e-9 * want, (trial, pe[0], want)
+        assert np.all(pn == pe[0])
+        R = s.fem_element(0)["R"]
+        assert np.max(np.abs(R - Q)) <= 1e-12, trial      # rotations[e] = R_0_2^T = Q
+        s.set("x", x0 @ Q.T + t)
+        pe, _ = s.von_mises(1)
+        assert pe[0] <= 1e-7 * E, (trial, pe[0])

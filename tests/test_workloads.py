@@ -350,3 +350,22 @@ def test_mesh_files(tmp_path):
         assert np.array_equal(s["pos"], p) and np.array_equal(s["tets"], t) and list(s["fixed"]) == [3, 39, 64]
         assert s["mass"] == ("diagonal", 1.0) and s["E"][0] == 3000.0 and s["iters"] == 25
         assert "SphereCollisionModel" in info["ignored"] and "BarycentricMapping" in info["ignored"]
+
+
+def test_scene_reader_von_mises_attribute():
+    """TetrahedronFEMForceField computeVonMisesStress (TetrahedronFEMForceField.inl:57: 0 none,
+    1 corotational, 2 Green strain) is kept as the spec's von_mises mode."""
+    from sofa_amd import scene_xml
+    text = """<Node name="root" dt="0.01" gravity="0 -9.81 0">
+      <EulerImplicitSolver/><CGLinearSolver iterations="20" tolerance="1e-9" threshold="1e-9"/>
+      <Node name="beam">
+        <RegularGridTopology name="grid" n="3 3 5" min="0 0 0" max="1 1 2"/>
+        <MechanicalObject/>
+        <TetrahedronFEMForceField youngModulus="1e5" poissonRatio="0.3" method="large" computeVonMisesStress="%s"/>
+        <UniformMass totalMass="10"/>
+        <BoxROI name="roi" box="-0.1 -0.1 -0.1 1.1 1.1 0.1"/>
+        <FixedProjectiveConstraint indices="@roi.indices"/>
+      </Node></Node>"""
+    for mode, want in (("0", None), ("1", 1), ("2", 2)):
+        specs, info = scene_xml.load_scene(text % mode)
+        assert len(specs) == 1 and specs[0].get("von_mises") == want, (mode, specs[0].get("von_mises"))
