@@ -241,6 +241,18 @@ int sofa_amd_scene_stats(sofa_amd_engine eng, int scene, sofa_amd_cg_stats* out)
  * stores the element rotations of the current positions, as the reference does.  Blocks until
  * done; SOFA_AMD_EDEGENERATE when a rest tetrahedron's matrix cannot be inverted. */
 int sofa_amd_scene_von_mises(sofa_amd_engine eng, int scene, int mode, double* per_element, double* per_node);
+/* TetrahedronFEMForceField::addKToMatrix / buildStiffnessMatrix (TetrahedronFEMForceField.inl:
+ * 1921-1999, computeStiffnessMatrix :212-232): the scene's assembled stiffness
+ * -k_factor * sum_e R_e J_e K_e J_e^T R_e^T with the rotations of the last step or addForce, as a
+ * block-sparse matrix of 3x3 blocks over the scene's nodes: block (a, b) exists when nodes a and
+ * b share an element; row_ptr (n_nodes + 1) and cols (*nnzb, ascending in a row) give the
+ * pattern, values (9 * *nnzb, each block row-major) the entries, every block summed over its
+ * elements in ascending element id.  NULL arrays are skipped: a first call with cols and values
+ * NULL returns *nnzb.  The pattern and each block's element list are built on the first call
+ * for a scene and kept on the device (about 64 B per element + 76 B per block).  Blocks until
+ * done. */
+int sofa_amd_scene_stiffness(sofa_amd_engine eng, int scene, double k_factor, int32_t* row_ptr, int32_t* cols,
+                             double* values, int64_t* nnzb);
 /* Device-side element data of one element as the reference stores it (for parity checks):
  * rotations[e] (3x3), K diagonal/off-diagonal/shear (3), rotated rest coordinates (4x3). */
 int sofa_amd_scene_element(sofa_amd_engine eng, int scene, int elem, double* rot9, double* k3,

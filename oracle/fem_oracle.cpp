@@ -318,6 +318,54 @@ struct TetraFEM : public ForceField {
         }
     }
 
+    // the general Mat product (sofa/type/Mat.h:1427-1441): r = 0, then r(i,j) += m1(i,k) m2(k,j)
+    // in the loop order i, k, j
+    static void matmul(const Real* a, const Real* b, Real* r, int L, int C, int P) {
+        for (int i = 0; i < L * P; i++) r[i] = 0;
+        for (int i = 0; i < L; i++)
+            for (int k = 0; k < C; k++) {
+                const Real aik = a[i * C + k];
+                for (int j = 0; j < P; j++) r[i * P + j] += aik * b[k * P + j];
+            }
+    }
+
+    // computeStiffnessMatrix (:212-232): JKJt = J K J^T, S = RR JKJt, SR = S RR^T with RR the
+    // 12x12 block diagonal of Rot
+    void computeStiffnessMatrix(size_t i, const Mat3& Rot, Real SR[144]) {
+        Real Jt[72], JKm[72], JKJt[144], RR[144], RRt[144], Sm[144];
+        for (int r = 0; r < 12; r++)
+            for (int c = 0; c < 6; c++) Jt[c * 12 + r] = J[i][r * 6 + c];
+        matmul(J[i].data(), K[i].data(), JKm, 12, 6, 6);
+        matmul(JKm, Jt, JKJt, 12, 6, 12);
+        for (int q = 0; q < 144; q++) RR[q] = RRt[q] = 0;
+        for (int a = 0; a < 3; a++)
+            for (int b = 0; b < 3; b++)
+                for (int blk = 0; blk < 4; blk++) {
+                    RR[(a + 3 * blk) * 12 + b + 3 * blk] = Rot.m[a][b];
+                    RRt[(a + 3 * blk) * 12 + b + 3 * blk] = Rot.m[b][a];
+                }
+        matmul(RR, JKJt, Sm, 12, 12, 12);
+        matmul(Sm, RRt, SR, 12, 12, 12);
+    }
+
+    // addKToMatrix (:1921-1957) into a dense 3N x 3N matrix: block (n1, n2) of element IT adds
+    // -tmp(3 n1 + i, 3 n2 + j) * k at the rows / columns of its nodes, elements in order
+    void addKToMatrix(Real* mat, size_t nNodes, Real k) {
+        const size_t W = 3 * nNodes;
+        Mat3 ident;
+        for (int a = 0; a < 3; a++) ident.m[a][a] = 1;
+        Real tmp[144];
+        for (size_t it = 0; it < elems.size(); it++) {
+            computeStiffnessMatrix(it, method == SMALL ? ident : rotations[it], tmp);
+            for (int n1 = 0; n1 < 4; n1++)
+                for (int n2 = 0; n2 < 4; n2++)
+                    for (int a = 0; a < 3; a++)
+                        for (int b = 0; b < 3; b++)
+                            mat[(3 * (size_t)elems[it][n1] + a) * W + 3 * (size_t)elems[it][n2] + b] +=
+                                -tmp[(3 * n1 + a) * 12 + 3 * n2 + b] * k;
+        }
+    }
+
     Real& Kel(size_t i, int r, int c) { return K[i][r * 6 + c]; }
     Real& Jel(size_t i, int r, int c) { return J[i][r * 6 + c]; }
 
@@ -1290,6 +1338,13 @@ void orc_fem_get_element(void* h, int e, double* R0, double* R, double* K, doubl
 }
 
 double orc_fem_rest_volume(void* h) { return static_cast<Scene*>(h)->fem->restVolume; }
+
+// addKToMatrix with the current rotations (those of the last addForce or step) into a dense
+// row-major 3N x 3N matrix that the caller zeroed
+void orc_stiffness_dense(void* h, double k, double* mat) {
+    Scene* s = static_cast<Scene*>(h);
+    s->fem->addKToMatrix(mat, s->n(), (Real)k);
+}
 
 // computeVonMisesStress from the current positions (mode 1 corotational, 2 Green strain):
 // per element (n_tets) and per node (n_nodes); returns 0, or -1 when a rest element's matrix

@@ -55,6 +55,7 @@ def lib():
         L.orc_fem_add_force.argtypes = [ctypes.c_void_p, _dp, _dp]
         L.orc_fem_add_dforce.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_double]
         L.orc_fem_get_element.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp, _dp, _dp, _dp]
+        L.orc_stiffness_dense.argtypes = [ctypes.c_void_p, ctypes.c_double, _dp]
         L.orc_von_mises.restype = ctypes.c_int
         L.orc_von_mises.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp]
         L.orc_fem_rest_volume.restype = ctypes.c_double
@@ -204,6 +205,13 @@ class OracleScene:
                                   K.ctypes.data_as(_dp), J.ctypes.data_as(_dp), rest.ctypes.data_as(_dp))
         return {"R0": R0.reshape(3, 3), "R": R.reshape(3, 3), "K": K.reshape(6, 6), "J": J.reshape(12, 6),
                 "rest": rest.reshape(4, 3)}
+
+    def stiffness_dense(self, k=1.0):
+        """addKToMatrix (TetrahedronFEMForceField.inl:1921-1957) with the current rotations: the
+        dense 3N x 3N matrix -k sum_e R J K J^T R^T."""
+        m = np.zeros((3 * self.n, 3 * self.n))
+        lib().orc_stiffness_dense(self._h, float(k), m.ctypes.data_as(_dp))
+        return m
 
     def von_mises(self, mode=1):
         """computeVonMisesStress (TetrahedronFEMForceField.inl:2197-2347) at the current positions:

@@ -25,7 +25,7 @@ EXPORTS = [
     "sofa_amd_scene_set_mass", "sofa_amd_scene_set_gravity", "sofa_amd_scene_set_fixed",
     "sofa_amd_scene_set_solver", "sofa_amd_engine_commit", "sofa_amd_engine_step", "sofa_amd_engine_sync",
     "sofa_amd_scene_read", "sofa_amd_scene_write", "sofa_amd_scene_stats", "sofa_amd_scene_element",
-    "sofa_amd_scene_von_mises",
+    "sofa_amd_scene_von_mises", "sofa_amd_scene_stiffness",
     "sofa_amd_fem_add_force", "sofa_amd_fem_add_dforce", "sofa_amd_engine_bench_apply", "sofa_amd_scene_tiling",
     "sofa_amd_engine_set_graphs", "sofa_amd_scene_set_mass_density", "sofa_amd_scene_add_oscillators",
     "sofa_amd_scene_set_refinement", "sofa_amd_engine_reserve", "sofa_amd_scene_commit", "sofa_amd_scene_release",
@@ -60,6 +60,9 @@ def lib():
     if not os.path.exists(LIB_PATH):
         raise ImportError(f"{LIB_PATH} is missing: build it with `python -m sofa_amd.build` "
                           "(or __graft_entry__.build()); there is no CPU fallback")
+    # torch first: the process then binds the HIP runtime torch was built for, which the library
+    # shares; loaded the other way round, torch finds no GPU in the same process
+    import torch  # noqa: F401
     L = ctypes.CDLL(LIB_PATH)
     c_int, c_double = ctypes.c_int, ctypes.c_double
     sig = {
@@ -83,6 +86,7 @@ def lib():
         "sofa_amd_scene_stats": (c_int, [_vp, c_int, ctypes.POINTER(CGStats)]),
         "sofa_amd_scene_element": (c_int, [_vp, c_int, c_int, _dp, _dp, _dp]),
         "sofa_amd_scene_von_mises": (c_int, [_vp, c_int, c_int, _dp, _dp]),
+        "sofa_amd_scene_stiffness": (c_int, [_vp, c_int, ctypes.c_double, _ip, _ip, _dp, ctypes.POINTER(ctypes.c_int64)]),
         "sofa_amd_fem_add_force": (c_int, [_vp, c_int, _vp, _vp, _vp]),
         "sofa_amd_fem_add_dforce": (c_int, [_vp, c_int, _vp, _vp, c_double, _vp]),
         "sofa_amd_engine_bench_apply": (c_int, [_vp, c_int, c_int, _vp]),
@@ -319,6 +323,18 @@ class Engine:
         check(lib().sofa_amd_scene_von_mises(self._h, int(scene), int(mode), pe.ctypes.data_as(_dp),
                                              pn.ctypes.data_as(_dp)))
         return pe, pn
+
+    def stiffness(self, scene, k_factor=1.0):
+        """addKToMatrix / buildStiffnessMatrix: the scene's assembled stiffness (-k sum R J K J^T R^T
+        with the last step's or addForce's rotations) as (row_ptr, cols, blocks (nnzb, 3, 3))."""
+        nnzb = ctypes.c_int64()
+        check(lib().sofa_amd_scene_stiffness(self._h, int(scene), float(k_factor), None, None, None, ctypes.byref(nnzb)))
+        rp = np.zeros(self.n_nodes[scene] + 1, dtype=np.int32)
+        cols = np.zeros(nnzb.value, dtype=np.int32)
+        vals = np.zeros((nnzb.value, 3, 3))
+        check(lib().sofa_amd_scene_stiffness(self._h, int(scene), float(k_factor), rp.ctypes.data_as(_ip),
+                                             cols.ctypes.data_as(_ip), vals.ctypes.data_as(_dp), ctypes.byref(nnzb)))
+        return rp, cols, vals
 
     # ---- component-level calls on device pointers ----
     def fem_add_force(self, scene, x_dev_ptr, f_dev_ptr, stream=None):

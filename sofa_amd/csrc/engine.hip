@@ -322,6 +322,7 @@ struct EngineBase {
     virtual int autotune(int reps, hipStream_t s, float* us_out) = 0;
     virtual void set_apply_variant(int v) = 0;
     virtual void von_mises(int scene, int mode, double* per_elem, double* per_node) = 0;
+    virtual int64_t stiffness(int scene, double kf, int32_t* row_ptr, int32_t* cols, double* values) = 0;
 };
 
 template <typename T>
@@ -425,6 +426,7 @@ struct EngineT : EngineBase {
     ~EngineT() override {
         sets.clear();
         while (!vmd.empty()) free_vm(vmd.begin()->first);
+        while (!kd.empty()) free_k(kd.begin()->first);
         free_device();
     }
     void free_device() {
@@ -1084,6 +1086,7 @@ struct EngineT : EngineBase {
         cur = nullptr;
         release_ranges(place[si]);
         free_vm(si);
+        free_k(si);
         iperm[si].clear();
         iperm[si].shrink_to_fit();
         std::lock_guard<std::mutex> g(prep_mu);
@@ -1520,6 +1523,89 @@ struct EngineT : EngineBase {
         for (double* ptr : {it->second.shf, it->second.lm, it->second.x0, it->second.vme, it->second.vmn})
             if (ptr) (void)hipFree(ptr);
         vmd.erase(it);
+    }
+    // addKToMatrix / buildStiffnessMatrix: per scene, built on first use, the block pattern (node
+    // pairs that share an element; rows ascending, columns ascending in a row) and, per block,
+    // its (element, n1, n2) contributions in ascending element id
+    struct KData {
+        std::vector<int32_t> row_ptr, cols;
+        int* slot_ptr = nullptr;
+        int* slots = nullptr;
+        double* vals = nullptr;
+    };
+    std::map<int, KData> kd;
+    void free_k(int si) {
+        auto it = kd.find(si);
+        if (it == kd.end()) return;
+        for (void* ptr : {(void*)it->second.slot_ptr, (void*)it->second.slots, (void*)it->second.vals})
+            if (ptr) (void)hipFree(ptr);
+        kd.erase(it);
+    }
+    int64_t stiffness(int scene, double kf, int32_t* row_ptr, int32_t* cols, double* values) override {
+        HIPCHK(hipSetDevice(device));
+        const SceneHost& sc = scenes[scene];
+        const int64_t nE = (int64_t)sc.tets.size() / 4;
+        const int nN = sc.n_nodes;
+        auto it = kd.find(scene);
+        if (it == kd.end()) {
+            if (elem_begin(scene) + nE > ((int64_t)1 << 27)) throw std::invalid_argument("too many elements for the block slots");
+            KData d;
+            std::vector<std::vector<int32_t>> nbr((size_t)nN);
+            for (int64_t e = 0; e < nE; e++) {
+                const int32_t* t = &sc.tets[4 * (size_t)e];
+                for (int a = 0; a < 4; a++)
+                    for (int b = 0; b < 4; b++) nbr[t[a]].push_back(t[b]);
+            }
+            d.row_ptr.assign((size_t)nN + 1, 0);
+            for (int n = 0; n < nN; n++) {
+                auto& v = nbr[n];
+                std::sort(v.begin(), v.end());
+                v.erase(std::unique(v.begin(), v.end()), v.end());
+                d.row_ptr[n + 1] = d.row_ptr[n] + (int32_t)v.size();
+            }
+            d.cols.reserve((size_t)d.row_ptr[nN]);
+            for (int n = 0; n < nN; n++) d.cols.insert(d.cols.end(), nbr[n].begin(), nbr[n].end());
+            nbr.clear();
+            const int64_t nb = d.row_ptr[nN];
+            std::vector<int> bid(16 * (size_t)nE), sp((size_t)nb + 1, 0), sl(16 * (size_t)nE);
+            for (int64_t e = 0; e < nE; e++) {
+                const int32_t* t = &sc.tets[4 * (size_t)e];
+                for (int a = 0; a < 4; a++)
+                    for (int b = 0; b < 4; b++) {
+                        const int32_t* c0 = d.cols.data() + d.row_ptr[t[a]];
+                        const int32_t* c1 = d.cols.data() + d.row_ptr[t[a] + 1];
+                        const int k = (int)(std::lower_bound(c0, c1, t[b]) - d.cols.data());
+                        bid[16 * (size_t)e + 4 * a + b] = k;
+                        sp[k + 1]++;
+                    }
+            }
+            for (int64_t k = 0; k < nb; k++) sp[k + 1] += sp[k];
+            std::vector<int> fill(sp.begin(), sp.end() - 1);
+            const std::vector<int>& ip = iperm[scene];
+            const int64_t eb0 = elem_begin(scene);
+            for (int64_t e = 0; e < nE; e++)
+                for (int j = 0; j < 16; j++)
+                    sl[fill[bid[16 * (size_t)e + j]]++] = (int)((eb0 + ip[e]) * 16 + j);
+            d.slot_ptr = dalloc<int>(sp.size());
+            d.slots = dalloc<int>(sl.size());
+            d.vals = dalloc<double>(9 * (size_t)nb);
+            it = kd.emplace(scene, std::move(d)).first;
+            HIPCHK(hipMemcpy(it->second.slot_ptr, sp.data(), sp.size() * sizeof(int), hipMemcpyHostToDevice));
+            if (!sl.empty()) HIPCHK(hipMemcpy(it->second.slots, sl.data(), sl.size() * sizeof(int), hipMemcpyHostToDevice));
+        }
+        KData& d = it->second;
+        const int nb = d.row_ptr[nN];
+        if (row_ptr) std::copy(d.row_ptr.begin(), d.row_ptr.end(), row_ptr);
+        if (cols) std::copy(d.cols.begin(), d.cols.end(), cols);
+        if (values && nb > 0) {
+            HIPCHK(hipDeviceSynchronize());     // the rotations of the last step / addForce
+            const int grid = (nb + kBlock - 1) / kBlock;
+            k_stiffness_blocks<R, NR><<<grid, kBlock, 0, stream>>>(view(), d.slot_ptr, d.slots, nb, kf, d.vals);
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipStreamSynchronize(stream));
+            HIPCHK(hipMemcpy(values, d.vals, 9 * (size_t)nb * sizeof(double), hipMemcpyDeviceToHost));
+        }
+        return nb;
     }
     void von_mises(int scene, int mode, double* pe, double* pn) override {
         HIPCHK(hipSetDevice(device));
@@ -2003,6 +2089,18 @@ int sofa_amd_scene_von_mises(sofa_amd_engine eng, int scene, int mode, double* p
     if (!s || (mode != 1 && mode != 2)) return fail(SOFA_AMD_EINVAL, "invalid argument (mode must be 1 or 2)");
     if (!usable(eng, scene)) return fail(SOFA_AMD_ESTATE, "scene not committed");
     eng->impl->von_mises(scene, mode, per_element, per_node);
+    return SOFA_AMD_OK;
+    GUARD_END
+}
+
+int sofa_amd_scene_stiffness(sofa_amd_engine eng, int scene, double k_factor, int32_t* row_ptr, int32_t* cols,
+                             double* values, int64_t* nnzb) {
+    GUARD_BEGIN
+    SceneHost* s = scene_of(eng, scene);
+    if (!s || !nnzb) return fail(SOFA_AMD_EINVAL, "invalid argument");
+    if (!usable(eng, scene)) return fail(SOFA_AMD_ESTATE, "scene not committed");
+    if (!std::isfinite(k_factor)) return fail(SOFA_AMD_ENAN, "k_factor is not finite");
+    *nnzb = eng->impl->stiffness(scene, k_factor, row_ptr, cols, values);
     return SOFA_AMD_OK;
     GUARD_END
 }

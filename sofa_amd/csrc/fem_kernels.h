@@ -648,4 +648,102 @@ static __global__ __launch_bounds__(kBlock) void k_von_mises_node(const int* __r
     vmn[n - n0] = a;
 }
 
+// ---------------------------------------------------------------------------------------
+// assembled stiffness (TetrahedronFEMForceField::addKToMatrix / buildStiffnessMatrix,
+// .inl:1921-1999, with computeStiffnessMatrix, :212-232)
+// ---------------------------------------------------------------------------------------
+// One thread per 3x3 block (a, b) of the scene's matrix: the block is the sum, in ascending
+// element id (the element loop's order), of -(R J K J^T R^T)(n1, n2) * k over the elements and
+// vertex pairs (n1, n2) with node a at n1 and b at n2 -- slots[] lists them per block as
+// device element * 16 + n1 * 4 + n2.  The element block needs only J's rows of n1 and n2: with
+// R = rotations[e] (the last addForce's), JK = J(n1 rows) K, B = JK J(n2 rows)^T, S = R B,
+// block = S R^T, every sum in the reference's dense-product order (zero terms of the reference's
+// 12x12 products leave its sums unchanged).  Computed in double, no contraction where the
+// translation unit honours the pragma.
+template <typename R, int NR>
+__global__ __launch_bounds__(kBlock) void k_stiffness_blocks(ElemView<R, NR> ev, const int* __restrict__ slot_ptr,
+                                                             const int* __restrict__ slots, int nblk, double kf,
+                                                             double* __restrict__ out) {
+#pragma clang fp contract(off)
+    const int b = blockIdx.x * kBlock + threadIdx.x;
+    if (b >= nblk) return;
+    double acc[9];
+#pragma unroll
+    for (int i = 0; i < 9; i++) acc[i] = 0.0;
+    for (int s = slot_ptr[b]; s < slot_ptr[b + 1]; s++) {
+        const int slot = slots[s];
+        const int64_t e = slot >> 4;
+        const int n1 = (slot >> 2) & 3, n2 = slot & 3;
+        R gr[12], rcr[9];
+        jgroups<R, NR>(ev, e, gr, rcr);
+        double rot[9];
+#pragma unroll
+        for (int k = 0; k < 9; k++) rot[k] = (double)ev.rot[k * ev.E + e];
+        const double k00 = (double)ev.kmat[e], k01 = (double)ev.kmat[ev.E + e], k33 = (double)ev.kmat[2 * ev.E + e];
+        double Km[6][6];
+#pragma unroll
+        for (int l = 0; l < 6; l++)
+#pragma unroll
+            for (int m = 0; m < 6; m++)
+                Km[l][m] = (l < 3 && m < 3) ? (l == m ? k00 : k01) : ((l == m) ? k33 : 0.0);
+        // J rows of a vertex (computeStrainDisplacement's placement of its three groups)
+        double J1[3][6], J2[3][6];
+#pragma unroll
+        for (int q = 0; q < 3; q++)
+#pragma unroll
+            for (int l = 0; l < 6; l++) { J1[q][l] = 0.0; J2[q][l] = 0.0; }
+        {
+            const double a0 = (double)gr[3 * n1], a1 = (double)gr[3 * n1 + 1], a2 = (double)gr[3 * n1 + 2];
+            J1[0][0] = a0; J1[0][3] = a1; J1[0][5] = a2;
+            J1[1][3] = a0; J1[1][1] = a1; J1[1][4] = a2;
+            J1[2][5] = a0; J1[2][4] = a1; J1[2][2] = a2;
+            const double b0 = (double)gr[3 * n2], b1 = (double)gr[3 * n2 + 1], b2 = (double)gr[3 * n2 + 2];
+            J2[0][0] = b0; J2[0][3] = b1; J2[0][5] = b2;
+            J2[1][3] = b0; J2[1][1] = b1; J2[1][4] = b2;
+            J2[2][5] = b0; J2[2][4] = b1; J2[2][2] = b2;
+        }
+        double JK[3][6];
+#pragma unroll
+        for (int q = 0; q < 3; q++)
+#pragma unroll
+            for (int m = 0; m < 6; m++) {
+                double a = 0.0;
+#pragma unroll
+                for (int l = 0; l < 6; l++) a += J1[q][l] * Km[l][m];
+                JK[q][m] = a;
+            }
+        double B[3][3];
+#pragma unroll
+        for (int q1 = 0; q1 < 3; q1++)
+#pragma unroll
+            for (int q2 = 0; q2 < 3; q2++) {
+                double a = 0.0;
+#pragma unroll
+                for (int m = 0; m < 6; m++) a += JK[q1][m] * J2[q2][m];
+                B[q1][q2] = a;
+            }
+        double S[3][3];
+#pragma unroll
+        for (int q1 = 0; q1 < 3; q1++)
+#pragma unroll
+            for (int p = 0; p < 3; p++) {
+                double a = 0.0;
+#pragma unroll
+                for (int r = 0; r < 3; r++) a += rot[3 * q1 + r] * B[r][p];
+                S[q1][p] = a;
+            }
+#pragma unroll
+        for (int q1 = 0; q1 < 3; q1++)
+#pragma unroll
+            for (int q2 = 0; q2 < 3; q2++) {
+                double a = 0.0;
+#pragma unroll
+                for (int p = 0; p < 3; p++) a += S[q1][p] * rot[3 * q2 + p];
+                acc[3 * q1 + q2] += (-a) * kf;
+            }
+    }
+#pragma unroll
+    for (int i = 0; i < 9; i++) out[9 * (int64_t)b + i] = acc[i];
+}
+
 }  // namespace sofa_amd

@@ -1076,3 +1076,66 @@ def test_von_mises(prec):
             eng.von_mises(sid, 0)
     finally:
         eng.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec", ["FP64", "FP64_TILED", "FP32", "MIXED"])
+def test_assembled_stiffness(prec):
+    """addKToMatrix / buildStiffnessMatrix (TetrahedronFEMForceField.inl:1921-1999) through
+    sofa_amd_scene_stiffness after an addForce at deformed, rotated positions (the rotations the
+    matrix uses), on a jittered beam with per-element E and nu placed after another scene: the
+    pattern is the node pairs that share an element (columns ascending), and the blocks equal the
+    oracle's dense assembly -- bit for bit in FP64, FP64 tiled and MIXED; FP32 (float element
+    record and positions) within 1e-5 of the largest entry.  Applied to dx the matrix gives
+    addDForce's df."""
+    import torch
+    capi = _capi()
+    from oracle import OracleScene
+    print(f"case: assembled stiffness, {prec}", flush=True)
+    rng = np.random.default_rng(10)
+    pos, tets, fixed = meshes.beam((4, 4, 9), (1.0, 1.0, 2.0))
+    pos = pos + rng.uniform(-0.03, 0.03, pos.shape)
+    ne, n = len(tets), len(pos)
+    E = rng.uniform(1e4, 1e6, ne)
+    nu = rng.uniform(0.2, 0.45, ne)
+    Q, _ = np.linalg.qr(rng.normal(size=(3, 3)))
+    x = (pos + 0.05 * np.sin(3 * pos[:, [2, 0, 1]])) @ Q.T
+    vt = torch.float32 if prec == "FP32" else torch.float64
+    if prec == "FP32":
+        x = x.astype(np.float32).astype(np.float64)
+    orc = OracleScene(pos)
+    orc.set_fem(tets, E, nu, "large")
+    orc.fem_add_force(x, np.zeros((n, 3)))
+    k = 0.25
+    M = orc.stiffness_dense(k)
+    eng = capi.Engine(0, getattr(capi, prec))
+    other = add_gpu_scene(eng, beam_cfg0())
+    sid = eng.add_scene(pos, tets, E, nu, fixed, mass_kind=capi.MASS_UNIFORM, mass_value=1.0)
+    eng.commit()
+    try:
+        eng.step([other], 1)
+        xd = torch.tensor(x, dtype=vt, device="cuda")
+        fd = torch.zeros((n, 3), dtype=vt, device="cuda")
+        torch.cuda.synchronize()
+        eng.fem_add_force(sid, xd.data_ptr(), fd.data_ptr())
+        eng.sync()
+        rp, cols, blocks = eng.stiffness(sid, k)
+        share = [set() for _ in range(n)]
+        for t4 in tets:
+            for a in t4:
+                share[a].update(int(b) for b in t4)
+        assert np.array_equal(rp, np.concatenate([[0], np.cumsum([len(s_) for s_ in share])]))
+        assert np.array_equal(cols, np.concatenate([sorted(s_) for s_ in share]))
+        D = np.zeros((3 * n, 3 * n))
+        for a in range(n):
+            for q in range(rp[a], rp[a + 1]):
+                D[3 * a:3 * a + 3, 3 * cols[q]:3 * cols[q] + 3] = blocks[q]
+        if prec == "FP32":
+            assert np.max(np.abs(D - M)) <= 1e-5 * np.max(np.abs(M)), np.max(np.abs(D - M)) / np.max(np.abs(M))
+        else:
+            assert np.array_equal(D, M), np.max(np.abs(D - M))
+        dx = rng.normal(size=(n, 3))
+        df = orc.fem_add_dforce(dx, np.zeros((n, 3)), k)
+        assert np.max(np.abs(D @ dx.reshape(-1) - df.reshape(-1))) <= 1e-5 * np.max(np.abs(df))
+    finally:
+        eng.close()

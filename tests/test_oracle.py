@@ -528,3 +528,39 @@ def test_von_mises_corotational_aligned_elements():
         s.set("x", x0 @ Q.T + t)
         pe, _ = s.von_mises(1)
         assert pe[0] <= 1e-7 * E, (trial, pe[0])
+
+
+# ------------------------------------------------------------------------------------------
+# assembled stiffness (TetrahedronFEMForceField::addKToMatrix, .inl:1921-1957)
+# ------------------------------------------------------------------------------------------
+def test_assembled_stiffness_matches_matrix_free():
+    """The assembled -k sum_e R J K J^T R^T with the rotations of an addForce at deformed,
+    rotated positions: (i) applied to dx it gives addDForce's df (the matrix-free corotational
+    product, applyStiffnessCorotational) to rounding; (ii) it is symmetric; (iii) rigid
+    translations are in its null space; (iv) its non-zero blocks are node pairs that share an
+    element; (v) it scales linearly in k."""
+    rng = np.random.default_rng(9)
+    pos, tets, _ = meshes.beam((3, 4, 6), (1.0, 1.2, 2.0))
+    pos = pos + rng.uniform(-0.03, 0.03, pos.shape)
+    ne, n = len(tets), len(pos)
+    s = OracleScene(pos)
+    s.set_fem(tets, rng.uniform(1e4, 1e6, ne), rng.uniform(0.2, 0.45, ne), "large")
+    Q = _rotation(rng)
+    x = (pos + 0.05 * np.sin(3 * pos[:, [2, 0, 1]])) @ Q.T
+    s.fem_add_force(x, np.zeros((n, 3)))                # rotations of x
+    k = 0.37
+    M = s.stiffness_dense(k)
+    dx = rng.normal(size=(n, 3))
+    df = s.fem_add_dforce(dx, np.zeros((n, 3)), k)
+    scale = np.max(np.abs(M))
+    assert np.max(np.abs(M @ dx.reshape(-1) - df.reshape(-1))) <= 1e-12 * scale * np.max(np.abs(dx)) * 30
+    assert np.max(np.abs(M - M.T)) <= 1e-13 * scale
+    for axis in range(3):
+        t = np.zeros((n, 3)); t[:, axis] = 1.0
+        assert np.max(np.abs(M @ t.reshape(-1))) <= 1e-12 * scale
+    share = np.zeros((n, n), bool)
+    for t4 in tets:
+        share[np.ix_(t4, t4)] = True
+    blocks = np.abs(M).reshape(n, 3, n, 3).max(axis=(1, 3)) > 0
+    assert np.array_equal(blocks, share)
+    assert np.allclose(s.stiffness_dense(2 * k), 2 * M, rtol=1e-15, atol=0)
