@@ -1682,8 +1682,11 @@ void launch_inner_cg(const InnerCg& c, hipStream_t s) {
     for (int it = 1; it <= c.maxit; it++) {
         if (c.na > 0)
         {
-            if (c.wide)
+            if (c.wide && c.plane)
                 k_tile_apply<float, 6, float, kTileThreads, 2, true, true><<<c.na, kTileThreads, 0, s>>>(
+                    c.ta, it, c.r, c.p, c.q, c.sp, c.prho, c.pden, 1, c.x, c.x0);
+            else if (c.wide)
+                k_tile_apply<float, 6, float, kTileThreads, 2, true, false><<<c.na, kTileThreads, 0, s>>>(
                     c.ta, it, c.r, c.p, c.q, c.sp, c.prho, c.pden, 1, c.x, c.x0);
             else if (c.plane)
                 k_tile_apply<float, 6, float, kTileThreads, 2, false, true><<<c.na, kTileThreads, 0, s>>>(
