@@ -472,12 +472,26 @@ def main():
         e1.close()
         e64, i64 = make_engine([c1], capi.FP64, graphs=True)
         t64, _ = time_merged(e64, i64, 5, 1)
+        # post-processing calls on the stepped FP64 scene (wall time of one call after the first,
+        # which builds the per-scene tables; host copies of the results included)
+        post = {}
+        for mode in (1, 2):
+            e64.von_mises(i64[0], mode)
+            t0 = time.perf_counter()
+            e64.von_mises(i64[0], mode)
+            post[f"von_mises_mode{mode}_ms"] = (time.perf_counter() - t0) * 1e3
+        e64.stiffness(i64[0], 1.0)
+        t0 = time.perf_counter()
+        _, kcols, _ = e64.stiffness(i64[0], 1.0)
+        post["stiffness_ms"] = (time.perf_counter() - t0) * 1e3
+        post["stiffness_blocks"] = int(len(kcols))
         e64.close()
         cfg1 = {"workload": "beam 33x33x257 nodes, 1x1x8 m: 1,310,720 tets, 279,873 nodes, CG 50, dt 0.005",
                 "fp32": {"value": len(c1["tets"]) * 20 / (t1 / 1e3), "ms_per_step": t1 / 20,
                          "p50_ms": float(np.percentile(l1, 50)), "p99_ms": float(np.percentile(l1, 99)),
                          "apply_us": r1["us_per_launch"], "apply_frac": r1["frac"]},
-                "fp64_parity": {"value": len(c1["tets"]) * 5 / (t64 / 1e3), "ms_per_step": t64 / 5}}
+                "fp64_parity": {"value": len(c1["tets"]) * 5 / (t64 / 1e3), "ms_per_step": t64 / 5},
+                "fp64_postprocess": post}
         log(f"cfg1: {cfg1}")
 
     cfg2 = None if args.no_cfg2 else cfg2_mixed(args.cfg2_steps, args.cfg2_error_steps)
