@@ -237,7 +237,7 @@ int sofa_amd_scene_stats(sofa_amd_engine eng, int scene, sofa_amd_cg_stats* out)
  * computeVonMisesStress values; 0 is "no computation" there and SOFA_AMD_EINVAL here).
  * per_element: n_tets values in element order, per_node: n_nodes values (the mean over the
  * node's tetrahedra; NULL skips either).  The element shape functions are built on the first
- * call for a scene and kept on the device (112 B per element + 32 B per node).  Mode 1 also
+ * call for a scene and kept on the device (120 B per element + 32 B per node).  Mode 1 also
  * stores the element rotations of the current positions, as the reference does.  Blocks until
  * done; SOFA_AMD_EDEGENERATE when a rest tetrahedron's matrix cannot be inverted. */
 int sofa_amd_scene_von_mises(sofa_amd_engine eng, int scene, int mode, double* per_element, double* per_node);
