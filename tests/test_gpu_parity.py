@@ -1139,3 +1139,46 @@ def test_assembled_stiffness(prec):
         assert np.max(np.abs(D @ dx.reshape(-1) - df.reshape(-1))) <= 1e-5 * np.max(np.abs(df))
     finally:
         eng.close()
+
+
+@pytest.mark.gpu
+def test_postprocess_tables_follow_live_scenes():
+    """Live (reserved) FP64 tiled engine: von Mises and stiffness tables built for a scene, the
+    scene released and its ranges reused by another one; the new scene's von Mises values and
+    assembled blocks equal those of the same scene in a statically committed engine, bit for bit,
+    and the released scene's calls are refused."""
+    capi = _capi()
+    print("case: post-processing tables with live scenes, FP64_TILED", flush=True)
+    a_cfg = beam_cfg0()
+    rng = np.random.default_rng(12)
+    c_cfg = dict(beam_cfg0(), pos=beam_cfg0()["pos"] + rng.uniform(-0.02, 0.02, beam_cfg0()["pos"].shape))
+    live = capi.Engine(0, capi.FP64_TILED)
+    live.reserve(len(a_cfg["pos"]) + 16, len(a_cfg["tets"]), 2)
+    try:
+        a = add_gpu_scene(live, a_cfg)
+        assert live.commit_scene(a)
+        live.step([a], 3)
+        live.von_mises(a, 1)
+        live.stiffness(a, 1.0)
+        live.release_scene(a)
+        with pytest.raises(capi.SofaAmdError):
+            live.von_mises(a, 1)
+        c = add_gpu_scene(live, c_cfg)
+        assert live.commit_scene(c)
+        live.step([c], 3)
+        ref = capi.Engine(0, capi.FP64_TILED)
+        rid = add_gpu_scene(ref, c_cfg)
+        ref.commit()
+        ref.step([rid], 3)
+        try:
+            for mode in (1, 2):
+                pl, nl = live.von_mises(c, mode)
+                pr, nr = ref.von_mises(rid, mode)
+                assert np.max(pr) > 0 and np.array_equal(pl, pr) and np.array_equal(nl, nr), mode
+            rl, cl, bl = live.stiffness(c, 0.5)
+            rr, cr, br = ref.stiffness(rid, 0.5)
+            assert np.array_equal(rl, rr) and np.array_equal(cl, cr) and np.array_equal(bl, br)
+        finally:
+            ref.close()
+    finally:
+        live.close()
