@@ -35,7 +35,13 @@ extern "C" {
 #define SOFA_AMD_MIXED       2   /* FP32 element arithmetic + FP32 inner CG, FP64 state and
                                     iterative refinement (r = b - A x in FP64)                   */
 #define SOFA_AMD_FP64_TILED  3   /* double on the tiled kernels: nodes shared by several tiles
-                                    are summed tile by tile (deterministic, not reference order) */
+                                    are summed tile by tile (deterministic, not reference order);
+                                    with sofa_amd_engine_set_dot_order(eng, 1), set before the
+                                    scenes are placed, every sum in the reference's order: the
+                                    FP64 parity trajectories bit for bit, on the tiled kernels   */
+#define SOFA_AMD_SPLIT       4   /* FP32 element record and element arithmetic (tiled kernels);
+                                    FP64 node vectors (state, CG vectors), per-node sums, CG dot
+                                    products and scalars                                         */
 
 /* mass kinds */
 #define SOFA_AMD_MASS_UNIFORM            0  /* UniformMass (UniformMass.inl:403-496)                  */
@@ -204,11 +210,15 @@ int sofa_amd_engine_usage(sofa_amd_engine eng, int64_t* out22);
  * reference counterpart: SOFA has no device graphs. */
 int sofa_amd_engine_set_graphs(sofa_amd_engine eng, int enable);
 
-/* SOFA_AMD_FP64 engines: order of the CG dot products (|b|^2, p.q, r.r).  0 (default): per
- * 256-node block sums combined in a fixed order (parallel, deterministic).  1: the reference's
- * sequential order, node after node from 0 (MechanicalObject::vDot, MechanicalObject.inl
- * :2334-2356), so that trajectories equal the reference's bit for bit however sensitive the
- * scene; costs one dependent add per node and dot. */
+/* SOFA_AMD_FP64 and SOFA_AMD_FP64_TILED engines: order of the sums.  0 (default): CG dot
+ * products (|b|^2, p.q, r.r) as per-block sums combined in a fixed order (parallel,
+ * deterministic); on the tiled engine, nodes shared by several tiles summed tile by tile.  1:
+ * the reference's order -- dot products node after node from 0 (MechanicalObject::vDot,
+ * MechanicalObject.inl:2334-2356), and on the tiled engine every node's element contributions
+ * one by one in ascending element id (TetrahedronFEMForceField.inl:1630-1634) -- so that
+ * trajectories equal the reference's bit for bit however sensitive the scene; costs one dependent
+ * add per node and dot.  A tiled engine takes it before its scenes are placed (commit / reserve)
+ * and then refuses a consistent MeshMatrixMass (SOFA_AMD_EINVAL at commit). */
 int sofa_amd_engine_set_dot_order(sofa_amd_engine eng, int order);
 
 /* Tiled engines: the CG operator kernel's variant -- threads per element and how much element
@@ -272,6 +282,61 @@ int sofa_amd_fem_add_force(sofa_amd_engine eng, int scene, const void* x_dev, vo
  * sofa_amd_fem_add_force. */
 int sofa_amd_fem_add_dforce(sofa_amd_engine eng, int scene, const void* dx_dev, void* df_dev,
                             double k_factor, void* stream);
+
+/* ---- component-level calls of the other components on the path ---------------------------
+ * Caller vectors are device pointers to the scene's n_nodes x 3 AoS values in the engine's state
+ * type (double for SOFA_AMD_FP64, FP64_TILED, MIXED and SPLIT; float for SOFA_AMD_FP32), enqueued on
+ * `stream` (NULL = the engine's stream) unless stated otherwise. */
+
+/* Mass::addMDx (Mass.h:69): f += factor M dx -- UniformMass.inl:403-419, DiagonalMass.inl:536-559,
+ * MeshMatrixMass.inl:1987-2047 (the consistent mass's edge terms after the vertex term, in edge
+ * order; on the FP64 engine only: SOFA_AMD_EINVAL on the tiled engines). */
+int sofa_amd_mass_add_mdx(sofa_amd_engine eng, int scene, const void* dx_dev, void* f_dev, double factor,
+                          void* stream);
+/* Mass::accFromF (Mass.h:77): a = M^-1 f -- UniformMass.inl:423-436, DiagonalMass.inl:564-576,
+ * MeshMatrixMass.inl:2051-2069 (lumped only; SOFA_AMD_EINVAL for the consistent mass, which the
+ * reference refuses with msg_error). */
+int sofa_amd_mass_acc_from_f(sofa_amd_engine eng, int scene, const void* f_dev, void* a_dev, void* stream);
+/* Mass::addForce (Mass.h:66): f += the weight under the scene's gravity -- UniformMass.inl:471-496,
+ * DiagonalMass.inl:1393-1413, MeshMatrixMass.inl:2073-2092. */
+int sofa_amd_mass_add_force(sofa_amd_engine eng, int scene, void* f_dev, void* stream);
+/* ProjectiveConstraintSet::projectResponse (ProjectiveConstraintSet.h:112-116) of the scene's
+ * constraints: FixedProjectiveConstraint (FixedProjectiveConstraint.inl:184-207, fixAll) and
+ * OscillatorProjectiveConstraint (OscillatorProjectiveConstraint.inl:65-70) clear their nodes. */
+int sofa_amd_project_response(sofa_amd_engine eng, int scene, void* dx_dev, void* stream);
+/* ProjectiveConstraintSet::projectVelocity (ProjectiveConstraintSet.h:118-120) at context time
+ * `time`: FixedProjectiveConstraint clears its nodes when activate_projectVelocity is set
+ * (FixedProjectiveConstraint.inl:237-258), then OscillatorProjectiveConstraint sets
+ * v = a w cos(w t + phi) (OscillatorProjectiveConstraint.inl:72-87). */
+int sofa_amd_project_velocity(sofa_amd_engine eng, int scene, void* v_dev, double time, void* stream);
+/* ProjectiveConstraintSet::projectPosition (ProjectiveConstraintSet.h:122-123): a no-op for
+ * FixedProjectiveConstraint (.inl:262-265); OscillatorProjectiveConstraint sets
+ * x = mean + a sin(w t + phi) (OscillatorProjectiveConstraint.inl:89-106). */
+int sofa_amd_project_position(sofa_amd_engine eng, int scene, void* x_dev, double time, void* stream);
+/* BaseMechanicalState::vDot (BaseMechanicalState.h:147; MechanicalObject::vDot,
+ * MechanicalObject.inl:2334-2356) of two caller vectors over the scene's nodes, into
+ * *result_dev (one double in device memory).  order 1: the reference's sequential sum, node after
+ * node from 0, bit for bit; order 0: the FP64 engine's parallel blocked order (256-node block
+ * trees, then the block partials), whose per-block partials live in a buffer the library
+ * allocates on the engine and grows to the largest scene asked for. */
+int sofa_amd_vdot(sofa_amd_engine eng, int scene, const void* a_dev, const void* b_dev, int order,
+                  double* result_dev, void* stream);
+/* CGLinearSolver::solve (CGLinearSolver.h:68, CGLinearSolver.inl:71-315) of
+ * (m_factor M + b_factor B + k_factor K) x = b on the scene, through MatrixLinearSolver::solveSystem
+ * (MatrixLinearSolver.inl:218-233) and GraphScatteredMatrix::apply (GraphScatteredTypes.cpp:33-45):
+ * mass addMDx, FEM addDForce with the rotations of the last step or addForce, PlaneForceField
+ * contacts, projectResponse.  The CG Data are the scene's (sofa_amd_scene_set_solver; x starts at
+ * 0: warmStart false), its dot products in the engine's order (sofa_amd_engine_set_dot_order).  B
+ * is the components' own Rayleigh damping, which this path's components do not have (their
+ * rayleighMass / rayleighStiffness Data are 0), so b_factor adds nothing.  Counts as one solve of the
+ * scene's CGLinearSolver (timeStepCount).  Not on MIXED engines.  Blocks until x_dev holds the
+ * solution; stats may be NULL. */
+int sofa_amd_cg_solve(sofa_amd_engine eng, int scene, double m_factor, double b_factor, double k_factor,
+                      const void* b_dev, void* x_dev, sofa_amd_cg_stats* stats, void* stream);
+/* ForceField::getPotentialEnergy (ForceField.h:121) of TetrahedronFEMForceField: the reference
+ * sums it for method "small" only and returns 0 for "large" (TetrahedronFEMForceField.inl:625-717),
+ * which is what this gives (synchronous). */
+int sofa_amd_fem_potential_energy(sofa_amd_engine eng, int scene, const void* x_dev, double* energy);
 
 /* ---- measurement and introspection (no reference counterpart) ---------------------------- */
 

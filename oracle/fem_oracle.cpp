@@ -718,6 +718,8 @@ struct MassBase : public ForceField {
     Vec3 gravity;
     Real rayleighMass = 0;   // Mass.h rayleighMass Data, default 0
     virtual void addMDx(std::vector<Vec3>& res, const std::vector<Vec3>& dx, Real factor) = 0;
+    // Mass::accFromF: a = M^-1 f; false where the reference refuses (a consistent MeshMatrixMass)
+    virtual bool accFromF(std::vector<Vec3>& a, const std::vector<Vec3>& f) = 0;
     void addMBKdx(std::vector<Vec3>& df, const std::vector<Vec3>& dx, Real mFact, Real bFact, Real) override {
         // Mass.inl:94-104: ForceField::addMBKdx calls Mass::addDForce, a no-op; then addMDx
         const Real m = mFact - bFact * rayleighMass;
@@ -733,6 +735,12 @@ struct UniformMass : public MassBase {
         if (factor != 1.0) m *= Real(factor);
         for (size_t i = 0; i < res.size(); i++)
             for (int q = 0; q < 3; q++) res[i][q] += dx[i][q] * m;
+    }
+    bool accFromF(std::vector<Vec3>& a, const std::vector<Vec3>& f) override {
+        const Real m = vertexMass;                       // :423-436
+        for (size_t i = 0; i < f.size(); i++)
+            for (int q = 0; q < 3; q++) a[i][q] = f[i][q] / m;
+        return true;
     }
     void addForce(std::vector<Vec3>& f, const std::vector<Vec3>&, const std::vector<Vec3>&) override {
         Vec3 mg(gravity[0] * vertexMass, gravity[1] * vertexMass, gravity[2] * vertexMass);   // :471-496
@@ -772,6 +780,11 @@ struct DiagonalMass : public MassBase {
             for (size_t i = 0; i < n; i++)
                 for (int q = 0; q < 3; q++) res[i][q] += (dx[i][q] * masses[i]) * Real(factor);
         }
+    }
+    bool accFromF(std::vector<Vec3>& a, const std::vector<Vec3>& f) override {
+        for (size_t i = 0; i < masses.size(); i++)      // :564-576
+            for (int q = 0; q < 3; q++) a[i][q] = f[i][q] / masses[i];
+        return true;
     }
     void addForce(std::vector<Vec3>& f, const std::vector<Vec3>&, const std::vector<Vec3>&) override {
         for (size_t i = 0; i < masses.size(); i++)      // :1393-1413
@@ -842,6 +855,13 @@ struct MeshMatrixMass : public MassBase {
                 for (int k = 0; k < 3; k++) res[e1][k] += dx[e0][k] * tempMass;
             }
         }
+    }
+    // :2051-2069: lumped only (the reference refuses the sparse mass)
+    bool accFromF(std::vector<Vec3>& a, const std::vector<Vec3>& f) override {
+        if (!lumping) return false;
+        for (size_t i = 0; i < vertexMass.size(); i++)
+            for (int k = 0; k < 3; k++) a[i][k] = f[i][k] / (vertexMass[i] * lumpCoeff);
+        return true;
     }
     // :2073-2092 (gravity always with the lumped vertex mass)
     void addForce(std::vector<Vec3>& f, const std::vector<Vec3>&, const std::vector<Vec3>&) override {
@@ -986,6 +1006,22 @@ struct Scene {
         if (fixAll) { for (auto& e : r) e = Vec3(); }
         else for (int i : fixed) r[i] = Vec3();
         for (const auto& o : oscillators) r[o.index] = Vec3();
+    }
+
+    // projectVelocity at context time t: FixedProjectiveConstraint (.inl:237-258, when
+    // activate_projectVelocity) then OscillatorProjectiveConstraint (.inl:72-87)
+    void projectVelocityAt(std::vector<Vec3>& vv, Real t) {
+        if (projectVelocity) {
+            if (fixAll) { for (auto& e : vv) e = Vec3(); }
+            else for (int i : fixed) vv[i] = Vec3();
+        }
+        for (const auto& o : oscillators)
+            for (int k = 0; k < 3; k++) vv[o.index][k] = o.amplitude[k] * o.pulsation * std::cos(o.pulsation * t + o.phase);
+    }
+    // projectPosition at context time t: Fixed is a no-op (.inl:262-265); Oscillator (:89-106)
+    void projectPositionAt(std::vector<Vec3>& xx, Real t) {
+        for (const auto& o : oscillators)
+            for (int k = 0; k < 3; k++) xx[o.index][k] = o.mean[k] + o.amplitude[k] * std::sin(o.pulsation * t + o.phase);
     }
 
     // GraphScatteredMatrix::apply (GraphScatteredTypes.cpp:33-45)
@@ -1379,6 +1415,67 @@ void orc_set_meshmatrix_mass(void* h, const double* density, int n_density, int 
             lumping != 0);
     m->gravity = s->gravity;
     s->mass.reset(m);
+}
+
+// component-level calls on caller vectors (n x 3): Mass::addMDx / accFromF / addForce, the
+// projective constraints, vDot, CGLinearSolver::solve
+static std::vector<Vec3> to_vec(const double* a, size_t n) {
+    std::vector<Vec3> r(n);
+    for (size_t i = 0; i < n; i++)
+        for (int k = 0; k < 3; k++) r[i][k] = a[3 * i + k];
+    return r;
+}
+static void from_vec(const std::vector<Vec3>& r, double* a) {
+    for (size_t i = 0; i < r.size(); i++)
+        for (int k = 0; k < 3; k++) a[3 * i + k] = r[i][k];
+}
+
+void orc_mass_add_mdx(void* h, const double* dx, double* res, double factor) {
+    Scene* s = static_cast<Scene*>(h);
+    std::vector<Vec3> r = to_vec(res, s->n());
+    s->mass->addMDx(r, to_vec(dx, s->n()), factor);
+    from_vec(r, res);
+}
+
+int orc_mass_acc_from_f(void* h, const double* f, double* a) {
+    Scene* s = static_cast<Scene*>(h);
+    std::vector<Vec3> r(s->n());
+    if (!s->mass->accFromF(r, to_vec(f, s->n()))) return -1;
+    from_vec(r, a);
+    return 0;
+}
+
+void orc_mass_add_force(void* h, double* f) {
+    Scene* s = static_cast<Scene*>(h);
+    std::vector<Vec3> r = to_vec(f, s->n());
+    s->mass->addForce(r, s->x, s->v);
+    from_vec(r, f);
+}
+
+// what: 0 projectResponse, 1 projectVelocity at t, 2 projectPosition at t
+void orc_project(void* h, int what, double* vec, double t) {
+    Scene* s = static_cast<Scene*>(h);
+    std::vector<Vec3> r = to_vec(vec, s->n());
+    if (what == 0) s->projectResponse(r);
+    else if (what == 1) s->projectVelocityAt(r, t);
+    else s->projectPositionAt(r, t);
+    from_vec(r, vec);
+}
+
+// mode 0: the reference's sequential order; 1: the GPU's blocked order
+double orc_vdot(void* h, const double* a, const double* b, int mode) {
+    Scene* s = static_cast<Scene*>(h);
+    const std::vector<Vec3> va = to_vec(a, s->n()), vb = to_vec(b, s->n());
+    return mode == 1 ? s->dot_blocked(va, vb) : s->dot_ref(va, vb);
+}
+
+// CGLinearSolver::solve of (mF M + bF B + kF K) x = b with the scene's CG Data; the rotations of
+// the last addForce (orc_fem_add_force or a step)
+void orc_cg_solve(void* h, const double* b, double* x, double mF, double bF, double kF) {
+    Scene* s = static_cast<Scene*>(h);
+    std::vector<Vec3> xs(s->n());
+    s->cgSolve(xs, to_vec(b, s->n()), mF, bF, kF);
+    from_vec(xs, x);
 }
 
 int orc_get_edge_count(void* h) {

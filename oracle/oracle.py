@@ -65,6 +65,14 @@ def lib():
         L.orc_get_edge_count.restype = ctypes.c_int
         L.orc_get_edge_count.argtypes = [ctypes.c_void_p]
         L.orc_get_edges.argtypes = [ctypes.c_void_p, _ip, _dp]
+        L.orc_mass_add_mdx.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_double]
+        L.orc_mass_acc_from_f.restype = ctypes.c_int
+        L.orc_mass_acc_from_f.argtypes = [ctypes.c_void_p, _dp, _dp]
+        L.orc_mass_add_force.argtypes = [ctypes.c_void_p, _dp]
+        L.orc_project.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, ctypes.c_double]
+        L.orc_vdot.restype = ctypes.c_double
+        L.orc_vdot.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_int]
+        L.orc_cg_solve.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_double, ctypes.c_double, ctypes.c_double]
         _lib = L
     return _lib
 
@@ -220,6 +228,48 @@ class OracleScene:
         if lib().orc_von_mises(self._h, int(mode), pe.ctypes.data_as(_dp), pn.ctypes.data_as(_dp)) != 0:
             raise ValueError("von Mises stress not available (mode, method or a singular rest element)")
         return pe, pn
+
+    # ---- component-level calls on (n, 3) arrays (the inputs are not modified) ----
+    def mass_add_mdx(self, dx, res, factor=1.0):
+        """Mass::addMDx: res + factor M dx."""
+        dx, dxp = _d(dx)
+        out = np.array(res, dtype=np.float64, copy=True)
+        lib().orc_mass_add_mdx(self._h, dxp, out.ctypes.data_as(_dp), float(factor))
+        return out
+
+    def mass_acc_from_f(self, f):
+        """Mass::accFromF: M^-1 f; ValueError where the reference refuses (consistent mass)."""
+        f, fp = _d(f)
+        a = np.zeros((self.n, 3))
+        if lib().orc_mass_acc_from_f(self._h, fp, a.ctypes.data_as(_dp)) != 0:
+            raise ValueError("accFromF needs a lumped mass")
+        return a
+
+    def mass_add_force(self, f):
+        """Mass::addForce: f + the weight."""
+        out = np.array(f, dtype=np.float64, copy=True)
+        lib().orc_mass_add_force(self._h, out.ctypes.data_as(_dp))
+        return out
+
+    def project(self, what, vec, t=0.0):
+        """0 projectResponse, 1 projectVelocity at time t, 2 projectPosition at time t."""
+        out = np.array(vec, dtype=np.float64, copy=True)
+        lib().orc_project(self._h, int(what), out.ctypes.data_as(_dp), float(t))
+        return out
+
+    def vdot(self, a, b, mode=0):
+        """vDot: mode 0 the reference's sequential order, 1 the GPU's blocked order."""
+        a, ap = _d(a)
+        b, bp = _d(b)
+        return lib().orc_vdot(self._h, ap, bp, int(mode))
+
+    def cg_solve(self, b, m_factor, b_factor, k_factor):
+        """CGLinearSolver::solve of (m M + b B + k K) x = b with the scene's CG Data and the
+        rotations of the last addForce; returns x (stats() gives the solve's statistics)."""
+        b, bp = _d(b)
+        x = np.zeros((self.n, 3))
+        lib().orc_cg_solve(self._h, bp, x.ctypes.data_as(_dp), float(m_factor), float(b_factor), float(k_factor))
+        return x
 
     def rest_volume(self):
         return lib().orc_fem_rest_volume(self._h)

@@ -14,7 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libsofa_amd.so")
 
 OK, EINVAL, EDEVICE, ENOMEM, EDEGENERATE, ESTATE, ENAN = 0, -1, -2, -3, -4, -5, -6
-FP64, FP32, MIXED, FP64_TILED = 0, 1, 2, 3
+FP64, FP32, MIXED, FP64_TILED, SPLIT = 0, 1, 2, 3, 4
 MASS_UNIFORM, MASS_DIAGONAL, MASS_MESHMATRIX, MASS_MESHMATRIX_LUMPED, MASS_DIAGONAL_TOTAL = 0, 1, 2, 3, 4
 END_REASONS = {0: "iterations", 1: "tolerance", 2: "threshold", 3: "den==0", 4: "null norm of vector b"}
 
@@ -31,6 +31,9 @@ EXPORTS = [
     "sofa_amd_scene_set_refinement", "sofa_amd_engine_reserve", "sofa_amd_scene_commit", "sofa_amd_scene_release",
     "sofa_amd_engine_usage", "sofa_amd_scene_set_fix_all", "sofa_amd_scene_set_integrator", "sofa_amd_scene_set_fem_hexa", "sofa_amd_scene_add_plane", "sofa_amd_scene_set_project_velocity", "sofa_amd_engine_set_dot_order",
     "sofa_amd_scene_prepare", "sofa_amd_engine_autotune", "sofa_amd_engine_set_apply_variant",
+    "sofa_amd_mass_add_mdx", "sofa_amd_mass_acc_from_f", "sofa_amd_mass_add_force", "sofa_amd_project_response",
+    "sofa_amd_project_velocity", "sofa_amd_project_position", "sofa_amd_vdot", "sofa_amd_cg_solve",
+    "sofa_amd_fem_potential_energy",
 ]
 
 
@@ -108,6 +111,16 @@ def lib():
         "sofa_amd_scene_prepare": (c_int, [_vp, c_int]),
         "sofa_amd_engine_autotune": (c_int, [_vp, c_int, ctypes.POINTER(c_int), ctypes.POINTER(ctypes.c_float)]),
         "sofa_amd_engine_set_apply_variant": (c_int, [_vp, c_int]),
+        "sofa_amd_mass_add_mdx": (c_int, [_vp, c_int, _vp, _vp, c_double, _vp]),
+        "sofa_amd_mass_acc_from_f": (c_int, [_vp, c_int, _vp, _vp, _vp]),
+        "sofa_amd_mass_add_force": (c_int, [_vp, c_int, _vp, _vp]),
+        "sofa_amd_project_response": (c_int, [_vp, c_int, _vp, _vp]),
+        "sofa_amd_project_velocity": (c_int, [_vp, c_int, _vp, c_double, _vp]),
+        "sofa_amd_project_position": (c_int, [_vp, c_int, _vp, c_double, _vp]),
+        "sofa_amd_vdot": (c_int, [_vp, c_int, _vp, _vp, c_int, _vp, _vp]),
+        "sofa_amd_cg_solve": (c_int, [_vp, c_int, c_double, c_double, c_double, _vp, _vp,
+                                      ctypes.POINTER(CGStats), _vp]),
+        "sofa_amd_fem_potential_energy": (c_int, [_vp, c_int, _vp, _dp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -342,6 +355,42 @@ class Engine:
 
     def fem_add_dforce(self, scene, dx_dev_ptr, df_dev_ptr, k_factor, stream=None):
         check(lib().sofa_amd_fem_add_dforce(self._h, int(scene), dx_dev_ptr, df_dev_ptr, float(k_factor), stream))
+
+    # ---- component-level calls of the masses, constraints, state and CG on device pointers ----
+    def mass_add_mdx(self, scene, dx_dev_ptr, f_dev_ptr, factor=1.0, stream=None):
+        check(lib().sofa_amd_mass_add_mdx(self._h, int(scene), dx_dev_ptr, f_dev_ptr, float(factor), stream))
+
+    def mass_acc_from_f(self, scene, f_dev_ptr, a_dev_ptr, stream=None):
+        check(lib().sofa_amd_mass_acc_from_f(self._h, int(scene), f_dev_ptr, a_dev_ptr, stream))
+
+    def mass_add_force(self, scene, f_dev_ptr, stream=None):
+        check(lib().sofa_amd_mass_add_force(self._h, int(scene), f_dev_ptr, stream))
+
+    def project_response(self, scene, dx_dev_ptr, stream=None):
+        check(lib().sofa_amd_project_response(self._h, int(scene), dx_dev_ptr, stream))
+
+    def project_velocity(self, scene, v_dev_ptr, time, stream=None):
+        check(lib().sofa_amd_project_velocity(self._h, int(scene), v_dev_ptr, float(time), stream))
+
+    def project_position(self, scene, x_dev_ptr, time, stream=None):
+        check(lib().sofa_amd_project_position(self._h, int(scene), x_dev_ptr, float(time), stream))
+
+    def vdot(self, scene, a_dev_ptr, b_dev_ptr, order, result_dev_ptr, stream=None):
+        """vDot into one double at result_dev_ptr (order 1: the reference's sequential sum)."""
+        check(lib().sofa_amd_vdot(self._h, int(scene), a_dev_ptr, b_dev_ptr, int(order), result_dev_ptr, stream))
+
+    def cg_solve(self, scene, m_factor, b_factor, k_factor, b_dev_ptr, x_dev_ptr, stream=None):
+        """CGLinearSolver::solve of (m M + b B + k K) x = b; returns the statistics."""
+        st = CGStats()
+        check(lib().sofa_amd_cg_solve(self._h, int(scene), float(m_factor), float(b_factor), float(k_factor),
+                                      b_dev_ptr, x_dev_ptr, ctypes.byref(st), stream))
+        return {"iterations": st.iterations, "end_reason": st.end_reason, "step_count": st.step_count,
+                "err": st.err, "den": st.den}
+
+    def fem_potential_energy(self, scene, x_dev_ptr):
+        e = ctypes.c_double()
+        check(lib().sofa_amd_fem_potential_energy(self._h, int(scene), x_dev_ptr, ctypes.byref(e)))
+        return e.value
 
     # ---- measurement ----
     def bench_apply(self, scene, reps, stream=None):

@@ -296,7 +296,9 @@ struct EngineBase {
     std::mutex scenes_mu;             // guards the container against a concurrent scene_prepare
     bool committed = false;
     bool use_graphs = false;
-    virtual ~EngineBase() {}
+    virtual ~EngineBase() {
+        if (vdot_part) (void)hipFree(vdot_part);
+    }
     virtual void commit() = 0;
     virtual void step(const int32_t* ids, int n, int nsteps, hipStream_t s) = 0;
     virtual void read(int scene, double* x, double* v, double* f) = 0;
@@ -319,10 +321,22 @@ struct EngineBase {
     bool live_mode = false;           // reserved: scenes are committed and released one by one
     bool seq_dots = false;            // exact path: dot products in the reference's sequential order
     bool stepped = false;             // a step has been enqueued
+    double* vdot_part = nullptr;      // vDot's per-block partials (grown on demand)
+    int64_t vdot_len = 0;
     virtual int autotune(int reps, hipStream_t s, float* us_out) = 0;
     virtual void set_apply_variant(int v) = 0;
     virtual void von_mises(int scene, int mode, double* per_elem, double* per_node) = 0;
     virtual int64_t stiffness(int scene, double kf, int32_t* row_ptr, int32_t* cols, double* values) = 0;
+    // component-level calls of the masses (0 addMDx, 1 accFromF, 2 addForce), the projective
+    // constraints (0 projectResponse, 1 projectVelocity, 2 projectPosition at time t), vDot and
+    // CGLinearSolver::solve on caller vectors of one scene
+    virtual void mass_call(int scene, int what, const void* in, void* out, double factor, hipStream_t s) = 0;
+    virtual void project(int scene, int what, void* vec, double t, hipStream_t s) = 0;
+    virtual void vdot(int scene, const void* a, const void* b, int order, double* out, double* scratch,
+                      int64_t scratch_len, hipStream_t s) = 0;
+    virtual void cg_solve(int scene, double mf, double bf, double kf, const void* bvec, void* xvec,
+                          sofa_amd_cg_stats* st, hipStream_t s) = 0;
+    virtual int scene_nodes(int scene) const = 0;
 };
 
 template <typename T>
@@ -384,6 +398,7 @@ struct EngineT : EngineBase {
     uint2* ltet = nullptr;
     int *ln_gid = nullptr, *ln_inc_ptr = nullptr, *ln_pslot = nullptr, *sh_gid = nullptr, *sh_ptr = nullptr;
     int* sh_of = nullptr;                          // per node: index into sh_gid / sh_ptr, or -1
+    int* sh_ord = nullptr;                         // reference order: shared-node slots in element order
     uint16_t* ln_inc = nullptr;
     VT *sp0 = nullptr, *sp1 = nullptr;
     double *pbb = nullptr, *pden = nullptr, *prho = nullptr;
@@ -434,7 +449,7 @@ struct EngineT : EngineBase {
                         inst, partial, tiles_d, ranges_d, ltet, ln_gid, ln_inc_ptr, ln_pslot, sh_gid, sh_ptr,
                         ln_inc, sp0, sp1, pbb, pden, prho, p2, em_ptr, em_nbr, em_val, emass, osc_node,
                         osc_mean, osc_amp, osc_w, osc_phase, kmat32, emass32, r32, p32, q32,
-                        d32, sp32, pden32, prho32, prr, inst_in, sh_of, nd, x0, x32, x0_32, contact};
+                        d32, sp32, pden32, prho32, prr, inst_in, sh_of, nd, x0, x32, x0_32, contact, sh_ord};
         for (void* ptr : ptrs)
             if (ptr) (void)hipFree(ptr);
     }
@@ -789,6 +804,14 @@ struct EngineT : EngineBase {
     void build_tiled(const SceneHost& s, int si, TiledHost& th) const {
         const int nE = (int)(s.tets.size() / 4), n = s.n_nodes;
         th.plan = plan_tiles(n, s.rest.data(), s.tets.data(), nE, kTileElems, kTileNodes);
+        if (exact_tiled()) {
+            // the consistent mass's edge terms ride on the element contributions in the tiles,
+            // which is not the reference's order (its edge loop runs before the FEM)
+            if (s.mass_kind == SOFA_AMD_MASS_MESHMATRIX)
+                throw std::invalid_argument("the reference summation order on a tiled engine needs a lumped mass "
+                                            "(UniformMass, DiagonalMass, lumped MeshMatrixMass)");
+            plan_exact(th.plan, n, kTileElems);
+        }
         th.iperm = th.plan.iperm;
         setup_scene(s, si, th.iperm, th.eh);
         th.sh_gid = th.plan.sh_gid;
@@ -846,6 +869,9 @@ struct EngineT : EngineBase {
             HIPCHK(hipMemset(*vec, 0, 3 * N * sizeof(VT)));
         }
         mass = dalloc<double>(N); fixed = dalloc<int8_t>(N); inc_ptr = dalloc<int>(N); sh_of = dalloc<int>(N);
+        nd = dalloc<double>(N);
+        HIPCHK(hipMemset(nd, 0, N * sizeof(double)));
+        sh_ord = dalloc<int>(std::max<int64_t>(1, cap.slot));
         contact = dalloc<int8_t>(N);
         HIPCHK(hipMemset(contact, 0, std::max<int64_t>(1, N)));
         tet = dalloc<int4>(E); rot = dalloc<R>(9 * E); rest = dalloc<R>(NR * E); kmat = dalloc<R>(3 * E);
@@ -974,7 +1000,7 @@ struct EngineT : EngineBase {
                 if (end || P.ln_pslot[li] < 0) {
                     lps.push_back(-1);
                 } else {
-                    const int owner = P.ln_pslot[li] == first_slot[P.ln_gid[li]] ? kSlotOwner : 0;
+                    const int owner = !exact_tiled() && P.ln_pslot[li] == first_slot[P.ln_gid[li]] ? kSlotOwner : 0;
                     lps.push_back((P.ln_pslot[li] + (int)pl.at.slot) | owner);
                 }
             }
@@ -989,6 +1015,12 @@ struct EngineT : EngineBase {
         sg[Sn] = nb0;
         for (int j = 0; j <= Sn; j++) spp[j] = (int)pl.at.slot + th.sh_ptr[j];
         put_at(sh_gid, pl.at.shn, sg); put_at(sh_ptr, pl.at.shn, spp);
+        if (exact_tiled()) {
+            // the node's slots in element order; the list shares the slot arena's range
+            std::vector<int> ordv(P.sh_ord);
+            for (int& o : ordv) o += (int)pl.at.slot;
+            put_at(sh_ord, pl.at.slot, ordv);
+        }
         std::vector<int> of(n + 1, -1);
         for (int j = 0; j < Sn; j++) of[th.sh_gid[j]] = (int)pl.at.shn + j;
         put_at(sh_of, nb0, of);
@@ -1195,6 +1227,40 @@ struct EngineT : EngineBase {
         k_finalize<<<(n + 63) / 64, 64, 0, s>>>(cur->scenes, n, inst);
     }
 
+    // the reference's summation order on the tiled kernels (FP64 tiled engines, chosen before
+    // the scenes are placed): shared nodes summed incidence by incidence in element order, every
+    // CG dot product node after node (k_seqdot); the trajectories are then the FP64 parity
+    // path's in its sequential dot order, bit for bit
+    bool exact_tiled() const { return tiled && !mixed && seq_dots && std::is_same<R, double>::value; }
+
+    // one step in the reference's summation order (FP64 tiled engines): per CG iteration the tile
+    // apply (interior nodes finished, shared nodes' contributions per incidence), the shared
+    // nodes' finalize, p.q node after node, the update, r.r node after node
+    void launch_step_exact(int n, hipStream_t s) {
+        const TileArgs<R, NR> ta = tile_args();
+        const FinBlk* fb = (const FinBlk*)cur->b;
+        const NodeBlk* nb = (const NodeBlk*)cur->c;
+        const SharedNodes sn{sh_gid, sh_ptr, sh_of, sh_ord};
+        if (cur->na > 0) k_tile_rhs<R, NR, VT, 512, true><<<cur->na, 512, 0, s>>>(ta, x, v, f, b, r, xcg, sp0, sp1, pbb);
+        if (cur->nb > 0)
+            k_fin_rhs<VT, true><<<cur->nb, kBlock, 0, s>>>(fb, inst, ranges_d, sn, mass, fixed, v, sp0, sp1, f, b, r, xcg,
+                                                             pbb, x, contact, nd);
+        k_seqdot<<<n, 64, 0, s>>>(cur->scenes, inst, nd, kDotBB);
+        for (int it = 1; it <= cur->maxit; it++) {
+            if (cur->na > 0) launch_apply(cur->na, s, ta, it, r, p, q, sp0, nullptr, pden, 0);
+            if (cur->nb > 0)
+                k_fin_exact<VT><<<cur->nb, kBlock, 0, s>>>(fb, inst, sn, mass, fixed, it, r, p, q, sp0, nd, contact);
+            k_seqdot<<<n, 64, 0, s>>>(cur->scenes, inst, nd, kDotPQ);
+            if (cur->nc > 0)
+                k_update<VT, true><<<cur->nc, kBlock, 0, s>>>(nb, inst, ranges_d, it, sn, sp0, mass, fixed, p, q, xcg, r,
+                                                              pden, prho, contact, nd);
+            k_seqdot<<<n, 64, 0, s>>>(cur->scenes, inst, nd, kDotRR);
+        }
+        if (cur->nc > 0) k_integrate<VT><<<cur->nc, kBlock, 0, s>>>(nb, inst, fixed, xcg, v, x);
+        if (cur->no > 0) k_oscillate<VT><<<cur->no, kBlock, 0, s>>>((const Blk*)cur->o, inst, oscillators(), x, v);
+        k_step_end_tiled<<<(n + 63) / 64, 64, 0, s>>>(cur->scenes, n, inst);
+    }
+
     // the tile operator's launch shape and how much geometry it recomputes (results of the
     // variants differ in rounding only: block sums and geometry recomputed in R instead of
     // rounded from the host's double); picked by sofa_amd_engine_autotune or set explicitly
@@ -1204,6 +1270,13 @@ struct EngineT : EngineBase {
     int apply_variant = 0;
     void launch_apply(int na, hipStream_t s, const TileArgs<R, NR>& ta, int it, const VT* rr, VT* pp, VT* qq, VT* sp,
                       const double* pin, double* pd, int pin_nodes) {
+        if constexpr (std::is_same<R, double>::value) {
+            if (exact_tiled()) {   // scalars from k_seqdot: no partial sums reduced
+                if (cur->plane) launch_apply_t<false, true, true>(na, s, ta, it, rr, pp, qq, sp, pin, pd, pin_nodes);
+                else launch_apply_t<false, false, true>(na, s, ta, it, rr, pp, qq, sp, pin, pd, pin_nodes);
+                return;
+            }
+        }
         const bool wide = cur->maxpart > kWavePartials;
         if (cur->plane) {
             if (wide) launch_apply_t<true, true>(na, s, ta, it, rr, pp, qq, sp, pin, pd, pin_nodes);
@@ -1213,14 +1286,14 @@ struct EngineT : EngineBase {
             else launch_apply_t<false, false>(na, s, ta, it, rr, pp, qq, sp, pin, pd, pin_nodes);
         }
     }
-    template <bool WIDE, bool PLANE>
+    template <bool WIDE, bool PLANE, bool EXACT = false>
     void launch_apply_t(int na, hipStream_t s, const TileArgs<R, NR>& ta, int it, const VT* rr, VT* pp, VT* qq, VT* sp,
                         const double* pin, double* pd, int pin_nodes) {
         switch (apply_variant) {
-        case 1: k_tile_apply<R, NR, VT, 512, 1, WIDE, PLANE><<<na, 512, 0, s>>>(ta, it, rr, pp, qq, sp, pin, pd, pin_nodes, x, x0); break;
-        case 2: k_tile_apply<R, NR, VT, 256, 2, WIDE, PLANE><<<na, 256, 0, s>>>(ta, it, rr, pp, qq, sp, pin, pd, pin_nodes, x, x0); break;
-        case 3: k_tile_apply<R, NR, VT, 256, 1, WIDE, PLANE><<<na, 256, 0, s>>>(ta, it, rr, pp, qq, sp, pin, pd, pin_nodes, x, x0); break;
-        default: k_tile_apply<R, NR, VT, 512, 2, WIDE, PLANE><<<na, 512, 0, s>>>(ta, it, rr, pp, qq, sp, pin, pd, pin_nodes, x, x0); break;
+        case 1: k_tile_apply<R, NR, VT, 512, 1, WIDE, PLANE, EXACT><<<na, 512, 0, s>>>(ta, it, rr, pp, qq, sp, pin, pd, pin_nodes, x, x0); break;
+        case 2: k_tile_apply<R, NR, VT, 256, 2, WIDE, PLANE, EXACT><<<na, 256, 0, s>>>(ta, it, rr, pp, qq, sp, pin, pd, pin_nodes, x, x0); break;
+        case 3: k_tile_apply<R, NR, VT, 256, 1, WIDE, PLANE, EXACT><<<na, 256, 0, s>>>(ta, it, rr, pp, qq, sp, pin, pd, pin_nodes, x, x0); break;
+        default: k_tile_apply<R, NR, VT, 512, 2, WIDE, PLANE, EXACT><<<na, 512, 0, s>>>(ta, it, rr, pp, qq, sp, pin, pd, pin_nodes, x, x0); break;
         }
     }
     // times every variant on the engine's scenes (all of them in one launch, `reps` launches
@@ -1278,6 +1351,7 @@ struct EngineT : EngineBase {
         ta.contact = contact;
         ta.emass = emass;
         ta.x_lo = nullptr;
+        ta.nd = nd;
         return ta;
     }
 
@@ -1285,7 +1359,10 @@ struct EngineT : EngineBase {
         const TileArgs<R, NR> ta = tile_args();
         const FinBlk* fb = (const FinBlk*)cur->b;
         const NodeBlk* nb = (const NodeBlk*)cur->c;
-        const SharedNodes sn{sh_gid, sh_ptr, sh_of};
+        const SharedNodes sn{sh_gid, sh_ptr, sh_of, sh_ord};
+        if constexpr (std::is_same<R, double>::value) {
+            if (exact_tiled()) { launch_step_exact(n, s); return; }
+        }
         if (cur->na > 0) k_tile_rhs<R, NR, VT, 512><<<cur->na, 512, 0, s>>>(ta, x, v, f, b, r, xcg, sp0, sp1, pbb);
         if (cur->nb > 0) k_fin_rhs<VT><<<cur->nb, kBlock, 0, s>>>(fb, inst, ranges_d, sn, mass, fixed, v, sp0, sp1, f, b, r, xcg, pbb, x, contact);
         for (int it = 1; it <= cur->maxit; it++) {
@@ -1306,7 +1383,7 @@ struct EngineT : EngineBase {
         ta.x_lo = x32;                 // the inner operator recomputes the rotations from these
         const FinBlk* fb = (const FinBlk*)cur->b;
         const NodeBlk* nb = (const NodeBlk*)cur->c;
-        const SharedNodes sn{sh_gid, sh_ptr, sh_of};
+        const SharedNodes sn{sh_gid, sh_ptr, sh_of, nullptr};
         InnerCg ic;
         ic.ta.tiles = ta.tiles;
         ic.ta.inst = inst_in;
@@ -1508,6 +1585,154 @@ struct EngineT : EngineBase {
         HIPCHK(hipGetLastError());
     }
 
+    int scene_nodes(int scene) const override { return scenes[scene].n_nodes; }
+
+    // Mass::addMDx / accFromF / addForce on the caller's vectors (state type VT, scene-local)
+    void mass_call(int scene, int what, const void* in, void* out, double factor, hipStream_t s) override {
+        const SceneHost& sc = scenes[scene];
+        if (sc.mass_kind == SOFA_AMD_MASS_MESHMATRIX) {
+            if (what == 1)   // MeshMatrixMass.inl:2055-2060: the sparse mass is not inverted
+                throw std::invalid_argument("accFromF needs a lumped mass (MeshMatrixMass lumping = false)");
+            if (what == 0 && tiled)   // the tiled engines keep the edge masses per element only
+                throw std::invalid_argument("addMDx of a consistent MeshMatrixMass runs on the FP64 engine");
+        }
+        const int n = sc.n_nodes, nb0 = node_begin(scene), sid = sid_of(scene);
+        const int grid = (n + kBlock - 1) / kBlock;
+        if (grid == 0) return;
+        if (what == 0)
+            k_mass_mdx<VT><<<grid, kBlock, 0, s>>>(inst, sid, nodes(), nb0, n, (const VT*)in, (VT*)out, factor);
+        else if (what == 1)
+            k_mass_acc<VT><<<grid, kBlock, 0, s>>>(inst, sid, mass, nb0, n, (const VT*)in, (VT*)out);
+        else
+            k_mass_add_force<VT><<<grid, kBlock, 0, s>>>(inst, sid, mass, nb0, n, (VT*)out);
+        HIPCHK(hipGetLastError());
+    }
+
+    // the scene's oscillator entries [o0, o1)
+    void osc_range(int scene, int& o0, int& o1) const {
+        const auto& ob = place[scene].oblks;
+        o0 = ob.empty() ? 0 : ob.front().begin;
+        o1 = ob.empty() ? 0 : ob.back().end;
+    }
+    // FixedProjectiveConstraint then OscillatorProjectiveConstraint (scene order) on a caller vector
+    void project(int scene, int what, void* vec, double t, hipStream_t s) override {
+        const int n = scenes[scene].n_nodes, nb0 = node_begin(scene), sid = sid_of(scene);
+        const int grid = (n + kBlock - 1) / kBlock;
+        if (what < 2 && grid > 0) k_project<VT><<<grid, kBlock, 0, s>>>(inst, sid, fixed, nb0, n, (VT*)vec, what);
+        int o0, o1;
+        osc_range(scene, o0, o1);
+        if (what > 0 && o1 > o0)
+            k_oscillate_vec<VT><<<(o1 - o0 + kBlock - 1) / kBlock, kBlock, 0, s>>>(oscillators(), o0, o1, nb0, t, (VT*)vec,
+                                                                                  what == 2 ? 1 : 0);
+        HIPCHK(hipGetLastError());
+    }
+
+    // vDot into out (device), order 1 the reference's sequential sum, 0 the blocked order with
+    // its per-block partials in `scratch` (scratch_len doubles)
+    void vdot(int scene, const void* a, const void* b, int order, double* out, double* scratch, int64_t scratch_len,
+              hipStream_t s) override {
+        const int n = scenes[scene].n_nodes;
+        if (order == 1) {
+            k_vdot_seq<VT><<<1, 64, 0, s>>>((const VT*)a, (const VT*)b, n, out);
+        } else {
+            const int nb = (n + kBlock - 1) / kBlock;
+            if (nb > scratch_len) throw std::length_error("vDot: scene larger than the scratch buffer");
+            if (nb > 0) k_vdot_blocks<VT><<<nb, kBlock, 0, s>>>((const VT*)a, (const VT*)b, n, scratch);
+            k_vdot_final<<<1, kBlock, 0, s>>>(scratch, nb, out);
+        }
+        HIPCHK(hipGetLastError());
+    }
+
+    // CGLinearSolver::solve (CGLinearSolver.inl:71-315) on the caller's right-hand side with the
+    // scene's CG Data: x = (m M + b B + k K)^-1 b through GraphScatteredMatrix::apply's operator
+    // (mass addMDx, FEM addDForce with the last addForce's rotations, plane, projectResponse).
+    // The components carry no Rayleigh damping of their own here (rayleighMass / rayleighStiffness
+    // Data 0), so B = 0 and bFactor adds nothing (Mass.inl:94-104, BaseForceField.cpp:38-47).
+    // Blocks until done; stats as sofa_amd_scene_stats reports them.
+    void cg_solve(int scene, double mf, double bf, double kf, const void* bvec, void* xvec, sofa_amd_cg_stats* st,
+                  hipStream_t s) override {
+        if (mixed) throw std::invalid_argument("the MIXED engine solves by refinement; CG solve runs on the others");
+        (void)bf;
+        const int sid = sid_of(scene), n = scenes[scene].n_nodes, nb0 = node_begin(scene);
+        InstDev saved;
+        HIPCHK(hipMemcpy(&saved, inst + sid, sizeof(InstDev), hipMemcpyDeviceToHost));
+        InstDev tmp = saved;
+        tmp.m_cg = mf;
+        tmp.k_cg = kf;
+        HIPCHK(hipMemcpy(inst + sid, &tmp, sizeof(InstDev), hipMemcpyHostToDevice));
+        const int32_t one = scene;
+        prepare_set(&one, 1, s);
+        if (!tiled) {
+            const ElemView<R, NR> ev = view();
+            const NodeArrays na = nodes();
+            const Blk* eb = (const Blk*)cur->a;
+            const Blk* nbk = (const Blk*)cur->b;
+            double* ndp = seq_dots ? nd : nullptr;
+            if (cur->nb > 0) k_cg_begin<VT><<<cur->nb, kBlock, 0, s>>>(nbk, inst, (const VT*)bvec, nb0, b, r, xcg, partial, ndp);
+            for (int it = 1; it <= cur->maxit; it++) {
+                VT* p_new = (it & 1) ? p : p2;
+                const VT* p_old = (it & 1) ? p2 : p;
+                if (cur->na > 0) k_cg_elem<R, NR, VT><<<cur->na, kBlock, 0, s>>>(ev, eb, inst, r, p_old, ef);
+                if (cur->nb > 0) {
+                    k_cg_node<VT><<<cur->nb, kBlock, 0, s>>>(nbk, inst, na, ef, r, p_old, p_new, q, partial, ndp);
+                    k_cg_update<VT><<<cur->nb, kBlock, 0, s>>>(nbk, inst, p_new, q, xcg, r, partial, ndp);
+                }
+            }
+            k_solve_end<<<1, 64, 0, s>>>(cur->scenes, 1, inst, 0);
+        } else {
+            const TileArgs<R, NR> ta = tile_args();
+            const FinBlk* fb = (const FinBlk*)cur->b;
+            const NodeBlk* nbk = (const NodeBlk*)cur->c;
+            const SharedNodes sn{sh_gid, sh_ptr, sh_of, sh_ord};
+            bool exact = false;
+            if constexpr (std::is_same<R, double>::value) exact = exact_tiled();
+            if (exact) {
+                if constexpr (std::is_same<R, double>::value) {
+                    if (cur->nc > 0)
+                        k_cg_begin_tiled<VT, true><<<cur->nc, kBlock, 0, s>>>(nbk, (const VT*)bvec, nb0, b, r, xcg, prho, nd);
+                    k_seqdot<<<1, 64, 0, s>>>(cur->scenes, inst, nd, kDotBB);
+                    for (int it = 1; it <= cur->maxit; it++) {
+                        if (cur->na > 0) launch_apply(cur->na, s, ta, it, r, p, q, sp0, nullptr, pden, 0);
+                        if (cur->nb > 0)
+                            k_fin_exact<VT><<<cur->nb, kBlock, 0, s>>>(fb, inst, sn, mass, fixed, it, r, p, q, sp0, nd, contact);
+                        k_seqdot<<<1, 64, 0, s>>>(cur->scenes, inst, nd, kDotPQ);
+                        if (cur->nc > 0)
+                            k_update<VT, true><<<cur->nc, kBlock, 0, s>>>(nbk, inst, ranges_d, it, sn, sp0, mass, fixed, p, q,
+                                                                          xcg, r, pden, prho, contact, nd);
+                        k_seqdot<<<1, 64, 0, s>>>(cur->scenes, inst, nd, kDotRR);
+                    }
+                }
+            } else {
+                if (cur->nc > 0)
+                    k_cg_begin_tiled<VT, false><<<cur->nc, kBlock, 0, s>>>(nbk, (const VT*)bvec, nb0, b, r, xcg, prho, nd);
+                for (int it = 1; it <= cur->maxit; it++) {
+                    if (cur->na > 0) launch_apply(cur->na, s, ta, it, r, p, q, sp0, prho, pden, it == 1 ? 1 : 0);
+                    if (cur->nc > 0)
+                        k_update<VT><<<cur->nc, kBlock, 0, s>>>(nbk, inst, ranges_d, it, sn, sp0, mass, fixed, p, q, xcg, r,
+                                                                pden, prho, contact);
+                }
+            }
+            k_solve_end<<<1, 64, 0, s>>>(cur->scenes, 1, inst, 1);
+        }
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(xvec, xcg + 3 * (int64_t)nb0, 3 * (size_t)n * sizeof(VT), hipMemcpyDeviceToDevice, s));
+        HIPCHK(hipStreamSynchronize(s));
+        InstDev after;
+        HIPCHK(hipMemcpy(&after, inst + sid, sizeof(InstDev), hipMemcpyDeviceToHost));
+        if (st) {
+            st->iterations = after.last_iters;
+            st->end_reason = after.end_reason;
+            st->step_count = after.step_count;
+            st->ir_sweeps = 0;
+            st->err = after.err;
+            st->den = after.den;
+            st->time = after.time;
+        }
+        after.m_cg = saved.m_cg;
+        after.k_cg = saved.k_cg;
+        HIPCHK(hipMemcpy(inst + sid, &after, sizeof(InstDev), hipMemcpyHostToDevice));
+    }
+
     // computeVonMisesStress (TetrahedronFEMForceField.inl:2197-2347) from the scene's current
     // positions: per scene, built on first use, rows 1..3 of the inverse of each element's rest
     // matrix [1 x y z] (init, :1521-1542) and its Lame coefficients (computeMaterialStiffness:
@@ -1678,6 +1903,14 @@ EngineBase* make_engine_fp32() {
     return e;
 }
 
+// FP32 element record and element arithmetic on FP64 node vectors (state, CG vectors, per-node
+// sums) with FP64 CG scalars: the tiled kernels instantiated with R = float, VT = double
+EngineBase* make_engine_split() {
+    auto* e = new EngineT<float, 6, double>();
+    e->tiled = true;
+    return e;
+}
+
 void launch_inner_cg(const InnerCg& c, hipStream_t s) {
     for (int it = 1; it <= c.maxit; it++) {
         if (c.na > 0)
@@ -1702,6 +1935,7 @@ void launch_inner_cg(const InnerCg& c, hipStream_t s) {
 }
 #else
 EngineBase* make_engine_fp32();
+EngineBase* make_engine_split();
 #endif
 
 }  // namespace sofa_amd
@@ -1776,6 +2010,8 @@ int sofa_amd_engine_create(int device, int precision, sofa_amd_engine* out) {
         impl.reset(e);
     } else if (precision == SOFA_AMD_FP32) {
         impl.reset(make_engine_fp32());
+    } else if (precision == SOFA_AMD_SPLIT) {
+        impl.reset(make_engine_split());
     } else if (precision == SOFA_AMD_MIXED) {
         auto* e = new EngineT<double, 9, double>();
         e->tiled = true;
@@ -2129,6 +2365,97 @@ int sofa_amd_fem_add_dforce(sofa_amd_engine eng, int scene, const void* dx_dev, 
     GUARD_END
 }
 
+// ---- component-level calls of the masses, the projective constraints, the state and the CG ----
+static int mass_call(sofa_amd_engine eng, int scene, int what, const void* in, void* out, double factor, void* stream) {
+    GUARD_BEGIN
+    if (!scene_of(eng, scene) || !out || (what < 2 && !in)) return fail(SOFA_AMD_EINVAL, "invalid argument");
+    if (!usable(eng, scene)) return fail(SOFA_AMD_ESTATE, "scene not committed");
+    HIPCHK(hipSetDevice(eng->impl->device));
+    eng->impl->mass_call(scene, what, in, out, factor, stream ? (hipStream_t)stream : eng->impl->stream);
+    return SOFA_AMD_OK;
+    GUARD_END
+}
+
+int sofa_amd_mass_add_mdx(sofa_amd_engine eng, int scene, const void* dx_dev, void* f_dev, double factor, void* stream) {
+    return mass_call(eng, scene, 0, dx_dev, f_dev, factor, stream);
+}
+
+int sofa_amd_mass_acc_from_f(sofa_amd_engine eng, int scene, const void* f_dev, void* a_dev, void* stream) {
+    return mass_call(eng, scene, 1, f_dev, a_dev, 1.0, stream);
+}
+
+int sofa_amd_mass_add_force(sofa_amd_engine eng, int scene, void* f_dev, void* stream) {
+    return mass_call(eng, scene, 2, nullptr, f_dev, 1.0, stream);
+}
+
+static int project_call(sofa_amd_engine eng, int scene, int what, void* vec, double t, void* stream) {
+    GUARD_BEGIN
+    if (!scene_of(eng, scene) || !vec) return fail(SOFA_AMD_EINVAL, "invalid argument");
+    if (!usable(eng, scene)) return fail(SOFA_AMD_ESTATE, "scene not committed");
+    HIPCHK(hipSetDevice(eng->impl->device));
+    eng->impl->project(scene, what, vec, t, stream ? (hipStream_t)stream : eng->impl->stream);
+    return SOFA_AMD_OK;
+    GUARD_END
+}
+
+int sofa_amd_project_response(sofa_amd_engine eng, int scene, void* dx_dev, void* stream) {
+    return project_call(eng, scene, 0, dx_dev, 0.0, stream);
+}
+
+int sofa_amd_project_velocity(sofa_amd_engine eng, int scene, void* v_dev, double time, void* stream) {
+    return project_call(eng, scene, 1, v_dev, time, stream);
+}
+
+int sofa_amd_project_position(sofa_amd_engine eng, int scene, void* x_dev, double time, void* stream) {
+    return project_call(eng, scene, 2, x_dev, time, stream);
+}
+
+int sofa_amd_vdot(sofa_amd_engine eng, int scene, const void* a_dev, const void* b_dev, int order, double* result_dev,
+                  void* stream) {
+    GUARD_BEGIN
+    if (!scene_of(eng, scene) || !a_dev || !b_dev || !result_dev || order < 0 || order > 1)
+        return fail(SOFA_AMD_EINVAL, "invalid argument");
+    if (!usable(eng, scene)) return fail(SOFA_AMD_ESTATE, "scene not committed");
+    EngineBase* e = eng->impl.get();
+    HIPCHK(hipSetDevice(e->device));
+    const hipStream_t s = stream ? (hipStream_t)stream : e->stream;
+    const int64_t need = (e->scene_nodes(scene) + kBlock - 1) / kBlock;
+    if (order == 0 && need > e->vdot_len) {
+        // the partials buffer grows to the largest scene asked for (no call in flight uses it
+        // after the device-wide wait)
+        HIPCHK(hipDeviceSynchronize());
+        if (e->vdot_part) HIPCHK(hipFree(e->vdot_part));
+        e->vdot_part = nullptr;
+        e->vdot_len = 0;
+        e->vdot_part = dalloc<double>((size_t)need);
+        e->vdot_len = need;
+    }
+    e->vdot(scene, a_dev, b_dev, order, result_dev, e->vdot_part, e->vdot_len, s);
+    return SOFA_AMD_OK;
+    GUARD_END
+}
+
+int sofa_amd_cg_solve(sofa_amd_engine eng, int scene, double m_factor, double b_factor, double k_factor,
+                      const void* b_dev, void* x_dev, sofa_amd_cg_stats* stats, void* stream) {
+    GUARD_BEGIN
+    if (!scene_of(eng, scene) || !b_dev || !x_dev) return fail(SOFA_AMD_EINVAL, "invalid argument");
+    if (!usable(eng, scene)) return fail(SOFA_AMD_ESTATE, "scene not committed");
+    HIPCHK(hipSetDevice(eng->impl->device));
+    eng->impl->cg_solve(scene, m_factor, b_factor, k_factor, b_dev, x_dev, stats,
+                        stream ? (hipStream_t)stream : eng->impl->stream);
+    return SOFA_AMD_OK;
+    GUARD_END
+}
+
+int sofa_amd_fem_potential_energy(sofa_amd_engine eng, int scene, const void* x_dev, double* energy) {
+    GUARD_BEGIN
+    if (!scene_of(eng, scene) || !x_dev || !energy) return fail(SOFA_AMD_EINVAL, "invalid argument");
+    if (!usable(eng, scene)) return fail(SOFA_AMD_ESTATE, "scene not committed");
+    *energy = 0.0;   // method "large": TetrahedronFEMForceField.inl:625-717 sums the small method only
+    return SOFA_AMD_OK;
+    GUARD_END
+}
+
 int sofa_amd_engine_bench_apply(sofa_amd_engine eng, int scene, int reps, void* stream) {
     GUARD_BEGIN
     if (!eng || !eng->impl || (scene >= 0 && !scene_of(eng, scene)) || reps < 0) return fail(SOFA_AMD_EINVAL, "invalid argument");
@@ -2232,7 +2559,12 @@ int sofa_amd_engine_usage(sofa_amd_engine eng, int64_t* out22) {
 int sofa_amd_engine_set_dot_order(sofa_amd_engine eng, int order) {
     GUARD_BEGIN
     if (!eng || !eng->impl || order < 0 || order > 1) return fail(SOFA_AMD_EINVAL, "invalid argument");
-    if (eng->impl->precision != SOFA_AMD_FP64) return fail(SOFA_AMD_EINVAL, "dot order applies to the FP64 engine");
+    const int prec = eng->impl->precision;
+    if (prec != SOFA_AMD_FP64 && prec != SOFA_AMD_FP64_TILED)
+        return fail(SOFA_AMD_EINVAL, "the summation order applies to the FP64 and FP64_TILED engines");
+    // the tiled engine's tables depend on it (per-incidence slots of the shared nodes)
+    if (prec == SOFA_AMD_FP64_TILED && (eng->impl->committed || eng->impl->live_mode))
+        return fail(SOFA_AMD_ESTATE, "set the summation order of a tiled engine before its scenes are placed");
     HIPCHK(hipSetDevice(eng->impl->device));
     HIPCHK(hipDeviceSynchronize());
     eng->impl->seq_dots = order == 1;

@@ -74,7 +74,55 @@ struct InstDev {
     // PlaneForceField (PlaneForceField.inl:86-98), after the FEM in scene order
     double plane_n[3], plane_d, plane_k, plane_damp, plane_max;
     int plane_on, plane_bilateral;
+    // tiled engines in the reference's summation order: |b|^2, p.q and r.r of the current
+    // iteration, each summed node after node by k_seqdot (MechanicalObject::vDot's order)
+    double dots[3];
 };
+
+// which entry of InstDev::dots a k_seqdot launch writes
+constexpr int kDotBB = 0, kDotPQ = 1, kDotRR = 2;
+
+// MechanicalObject::vDot in the reference's order (MechanicalObject.inl:2334-2356): the scene's
+// per-node terms nd[node_begin + i], i = 0 .. n - 1 (each already the node's three-term dot
+// product), added one after the other from 0.0.  One wave per scene of the launch set: the wave
+// loads 512 terms per round (coalesced, one round ahead), parks them in LDS, and every lane runs
+// the same chain of dependent adds over them (broadcast LDS reads), so a scene costs about one
+// FP64 add latency per node.  The last round is padded with +0.0, which leaves the sum unchanged:
+// s + 0.0 == s for every s but -0.0, and a sum that starts at +0.0 is never -0.0 under
+// round-to-nearest.  The scene's scalar lands in InstDev::dots[which]; a stopped scene is skipped.
+constexpr int kSeqChunk = 512;
+static __global__ __launch_bounds__(64) void k_seqdot(const int* __restrict__ scenes, InstDev* __restrict__ inst,
+                                                       const double* __restrict__ nd, int which) {
+    __shared__ double buf[2][kSeqChunk];
+    InstDev* ins = &inst[scenes[blockIdx.x]];
+    if (ins->stopped) return;
+    const int b0 = ins->node_begin, n = ins->node_count;
+    const int lane = threadIdx.x;
+    constexpr int PER = kSeqChunk / 64;
+    double reg[PER];
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+        const int i = k * 64 + lane;
+        reg[k] = i < n ? nd[b0 + i] : 0.0;
+    }
+    double s = 0.0;
+    int round = 0;
+    for (int base = 0; base < n; base += kSeqChunk, round ^= 1) {
+#pragma unroll
+        for (int k = 0; k < PER; k++) buf[round][k * 64 + lane] = reg[k];
+        const int nb = base + kSeqChunk;
+#pragma unroll
+        for (int k = 0; k < PER; k++) {
+            const int i = nb + k * 64 + lane;
+            reg[k] = i < n ? nd[b0 + i] : 0.0;
+        }
+        __syncthreads();
+        const double* c = buf[round];
+#pragma unroll 64
+        for (int k = 0; k < kSeqChunk; k++) s = s + c[k];
+    }
+    if (lane == 0) ins->dots[which] = s;
+}
 
 struct Blk { int inst, begin, end, pad; };   // a block's scene and [begin, end) range
 

@@ -105,6 +105,38 @@ __device__ __forceinline__ double dot3d(const VT a[3], const VT b[3]) {
     return r;
 }
 
+// CGLinearSolver.inl:86-160 for nb_iter = 1 (rho = r.r = b.b), once |b|^2 is known: the exact
+// path's CG scalars of a scene at the start of a solve
+__device__ __forceinline__ void cg_start(InstDev* ins, double total) {
+    ins->normb = sqrt(total);
+    ins->rho = total;
+    ins->rho_1 = 0.0;
+    ins->iter = 1;
+    ins->stopped = 0;
+    ins->end_reason = 0;
+    ins->err = 1.0;
+    ins->den = 0.0;
+    if (ins->normb == 0.0) {
+        ins->stopped = 1; ins->end_reason = 4; ins->last_iters = 0;
+    } else {
+        const double err = sqrt(total) / ins->normb;
+        ins->err = err;
+        if (err <= ins->tol && !ins->first_solve) { ins->stopped = 1; ins->end_reason = 1; ins->last_iters = 1; }
+    }
+}
+
+// end of a component-level CG solve (no time step): the loop's exit record, timeStepCount++
+// (CGLinearSolver.inl:300); reopen = 1 for the tiled kernels, which run with stopped == 0
+static __global__ void k_solve_end(const int* __restrict__ scenes, int n, InstDev* __restrict__ inst, int reopen) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    InstDev* in = &inst[scenes[i]];
+    if (!in->stopped) { in->stopped = 1; in->end_reason = 0; in->last_iters = in->max_iter; }
+    in->first_solve = 0;
+    in->step_count += 1;
+    if (reopen) in->stopped = 0;
+}
+
 // ---------------------------------------------------------------------------------------
 // RHS
 // ---------------------------------------------------------------------------------------
@@ -202,24 +234,35 @@ __global__ __launch_bounds__(kBlock) void k_rhs_node(const Blk* __restrict__ nbl
     double total;
     const bool last = nd ? last_block_sequential(nd, ins, total)
                          : last_block_reduce(block_sum(bb, sh), partial, ins, bk.pad, sh, total);
-    if (last && threadIdx.x == 0) {
-        // CGLinearSolver.inl:86-160 for nb_iter = 1 (rho = r.r = b.b)
-        ins->normb = sqrt(total);
-        ins->rho = total;
-        ins->rho_1 = 0.0;
-        ins->iter = 1;
-        ins->stopped = 0;
-        ins->end_reason = 0;
-        ins->err = 1.0;
-        ins->den = 0.0;
-        if (ins->normb == 0.0) {
-            ins->stopped = 1; ins->end_reason = 4; ins->last_iters = 0;
-        } else {
-            const double err = sqrt(total) / ins->normb;
-            ins->err = err;
-            if (err <= ins->tol && !ins->first_solve) { ins->stopped = 1; ins->end_reason = 1; ins->last_iters = 1; }
-        }
+    if (last && threadIdx.x == 0) cg_start(ins, total);
+}
+
+// the exact path's start of CGLinearSolver::solve on a caller's right-hand side (component call):
+// r = b, x = 0, |b|^2 and the first iteration's tests as after the RHS
+template <typename VT>
+__global__ __launch_bounds__(kBlock) void k_cg_begin(const Blk* __restrict__ nblk, InstDev* __restrict__ inst,
+                                                      const VT* __restrict__ b_in, int nb0, VT* __restrict__ b,
+                                                      VT* __restrict__ r, VT* __restrict__ xcg,
+                                                      double* __restrict__ partial, double* __restrict__ nd) {
+    __shared__ double sh[kBlock];
+    const Blk bk = nblk[blockIdx.x];
+    InstDev* ins = &inst[bk.inst];
+    const int n = bk.begin + threadIdx.x;
+    double bb = 0.0;
+    if (n < bk.end) {
+        VT bn[3];
+        load3(b_in, n - nb0, bn[0], bn[1], bn[2]);
+        const int64_t o = 3 * (int64_t)n;
+        b[o] = bn[0]; b[o + 1] = bn[1]; b[o + 2] = bn[2];
+        r[o] = bn[0]; r[o + 1] = bn[1]; r[o + 2] = bn[2];
+        xcg[o] = 0; xcg[o + 1] = 0; xcg[o + 2] = 0;
+        bb = dot3d(bn, bn);
+        if (nd) nd[n] = bb;
     }
+    double total;
+    const bool last = nd ? last_block_sequential(nd, ins, total)
+                         : last_block_reduce(block_sum(bb, sh), partial, ins, bk.pad, sh, total);
+    if (last && threadIdx.x == 0) cg_start(ins, total);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -507,6 +550,166 @@ __global__ __launch_bounds__(kBlock) void k_gather(const int* __restrict__ inc_p
         }
     }
     out[o] = a[0]; out[o + 1] = a[1]; out[o + 2] = a[2];
+}
+
+// ---------------------------------------------------------------------------------------
+// component-level kernels of the masses, the projective constraints and the state (one call of
+// Mass / ProjectiveConstraintSet / MechanicalState on caller vectors of one scene; the caller's
+// vectors are indexed from the scene's first node nb0, engine tables by global node id)
+// ---------------------------------------------------------------------------------------
+// Mass::addMDx: res += factor M dx -- UniformMass.inl:403-419 (m *= factor once, then dx m),
+// DiagonalMass.inl:536-559, MeshMatrixMass.inl:1987-2047 (vertex term, then the consistent mass's
+// edges in ascending edge id: per node the same order as the reference's two loops)
+template <typename VT>
+__global__ __launch_bounds__(kBlock) void k_mass_mdx(const InstDev* __restrict__ inst, int sid, NodeArrays na, int nb0,
+                                                      int n, const VT* __restrict__ dx, VT* __restrict__ res,
+                                                      double factor) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const InstDev& in = inst[sid];
+    VT d[3], o[3];
+    load3(dx, i, d[0], d[1], d[2]);
+    load3(res, i, o[0], o[1], o[2]);
+    mass_mdx(in, na, nb0 + i, factor, d, o);
+    if (in.mass_kind == kMassMatrix)
+        mass_edges(na, nb0 + i, factor, [&](int j, VT e[3]) { load3(dx, j - nb0, e[0], e[1], e[2]); }, o);
+    const int64_t q = 3 * (int64_t)i;
+    res[q] = o[0]; res[q + 1] = o[1]; res[q + 2] = o[2];
+}
+
+// Mass::accFromF: a = M^-1 f -- UniformMass.inl:423-436 (f / m), DiagonalMass.inl:564-576
+// (f / m_i), MeshMatrixMass.inl:2051-2069 (lumped only: f / (m_i 2.5))
+template <typename VT>
+__global__ __launch_bounds__(kBlock) void k_mass_acc(const InstDev* __restrict__ inst, int sid,
+                                                      const double* __restrict__ mass, int nb0, int n,
+                                                      const VT* __restrict__ f, VT* __restrict__ a) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const InstDev& in = inst[sid];
+    VT m;
+    if (in.mass_kind == kMassUniform) m = (VT)in.vertex_mass;
+    else if (in.mass_kind == kMassDiagonal) m = (VT)mass[nb0 + i];
+    else m = (VT)mass[nb0 + i] * (VT)kLumpTet;
+    const int64_t q = 3 * (int64_t)i;
+#pragma unroll
+    for (int k = 0; k < 3; k++) a[q + k] = f[q + k] / m;
+}
+
+// Mass::addForce (gravity): UniformMass.inl:471-496 (f += g m), DiagonalMass.inl:1393-1413
+// (f += g m_i), MeshMatrixMass.inl:2073-2092 (f += g m_i 2.5, lumped coefficient in both modes)
+template <typename VT>
+__global__ __launch_bounds__(kBlock) void k_mass_add_force(const InstDev* __restrict__ inst, int sid,
+                                                            const double* __restrict__ mass, int nb0, int n,
+                                                            VT* __restrict__ f) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const InstDev& in = inst[sid];
+    const int64_t q = 3 * (int64_t)i;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        VT w;
+        if (in.mass_kind == kMassUniform) w = (VT)in.mg[k];
+        else if (in.mass_kind == kMassDiagonal) w = (VT)in.gravity[k] * (VT)mass[nb0 + i];
+        else w = ((VT)in.gravity[k] * (VT)mass[nb0 + i]) * (VT)kLumpTet;
+        f[q + k] += w;
+    }
+}
+
+// ProjectiveConstraintSet::projectResponse of the scene's constraints: FixedProjectiveConstraint
+// (FixedProjectiveConstraint.inl:184-207; fixAll clears every node) and
+// OscillatorProjectiveConstraint (OscillatorProjectiveConstraint.inl:65-70) clear their nodes.
+// velocity = 1: FixedProjectiveConstraint::projectVelocity (.inl:237-258): the fixed nodes only,
+// and only when activate_projectVelocity is set
+template <typename VT>
+__global__ __launch_bounds__(kBlock) void k_project(const InstDev* __restrict__ inst, int sid,
+                                                     const int8_t* __restrict__ fixed, int nb0, int n,
+                                                     VT* __restrict__ vec, int velocity) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const int8_t m = fixed[nb0 + i];
+    const bool clear = velocity ? (inst[sid].project_velocity && m == 1) : m != 0;
+    if (!clear) return;
+    const int64_t q = 3 * (int64_t)i;
+    vec[q] = 0; vec[q + 1] = 0; vec[q + 2] = 0;
+}
+
+// OscillatorProjectiveConstraint::projectPosition / projectVelocity (.inl:72-106) at time t on a
+// caller vector: x = mean + a sin(w t + phi) (position = 1) or v = a w cos(w t + phi)
+template <typename VT>
+__global__ __launch_bounds__(kBlock) void k_oscillate_vec(Oscillators os, int o0, int o1, int nb0, double t,
+                                                           VT* __restrict__ vec, int position) {
+    const int i = o0 + blockIdx.x * kBlock + threadIdx.x;
+    if (i >= o1) return;
+    const double w = os.w[i], ph = os.phase[i];
+    const int64_t q = 3 * (int64_t)(os.node[i] - nb0);
+    if (position) {
+        const double sn = sin(w * t + ph);
+#pragma unroll
+        for (int k = 0; k < 3; k++) vec[q + k] = (VT)(os.mean[3 * i + k] + os.amp[3 * i + k] * sn);
+    } else {
+        const double cs = cos(w * t + ph);
+#pragma unroll
+        for (int k = 0; k < 3; k++) vec[q + k] = (VT)((os.amp[3 * i + k] * w) * cs);
+    }
+}
+
+// BaseMechanicalState::vDot on caller vectors (MechanicalObject::vDot, MechanicalObject.inl
+// :2334-2356), in the reference's order: one wave computes each node's three-term dot product
+// and adds them node after node from 0.0 (the k_seqdot chain, terms formed on the fly; padding
+// with +0.0 leaves the sum unchanged)
+template <typename VT>
+__global__ __launch_bounds__(64) void k_vdot_seq(const VT* __restrict__ a, const VT* __restrict__ b, int n,
+                                                  double* __restrict__ out) {
+    __shared__ double buf[kSeqChunk];
+    const int lane = threadIdx.x;
+    constexpr int PER = kSeqChunk / 64;
+    double s = 0.0;
+    for (int base = 0; base < n; base += kSeqChunk) {
+#pragma unroll
+        for (int k = 0; k < PER; k++) {
+            const int i = base + k * 64 + lane;
+            double t = 0.0;
+            if (i < n) {
+                VT x[3], y[3];
+                load3(a, i, x[0], x[1], x[2]);
+                load3(b, i, y[0], y[1], y[2]);
+                t = dot3d(x, y);
+            }
+            buf[k * 64 + lane] = t;
+        }
+        __syncthreads();
+#pragma unroll 64
+        for (int k = 0; k < kSeqChunk; k++) s = s + buf[k];
+        __syncthreads();
+    }
+    if (lane == 0) *out = s;
+}
+
+// the same dot product in the FP64 engine's parallel blocked order: per 256-node block the
+// fixed tree of block_sum, then the block partials added by one block as last_block_reduce
+// does (each thread a strided sum, then block_sum) -- the oracle's dot_blocked
+template <typename VT>
+__global__ __launch_bounds__(kBlock) void k_vdot_blocks(const VT* __restrict__ a, const VT* __restrict__ b, int n,
+                                                         double* __restrict__ partial) {
+    __shared__ double sh[kBlock];
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    double t = 0.0;
+    if (i < n) {
+        VT x[3], y[3];
+        load3(a, i, x[0], x[1], x[2]);
+        load3(b, i, y[0], y[1], y[2]);
+        t = dot3d(x, y);
+    }
+    const double bs = block_sum(t, sh);
+    if (threadIdx.x == 0) partial[blockIdx.x] = bs;
+}
+static __global__ __launch_bounds__(kBlock) void k_vdot_final(const double* __restrict__ partial, int nb,
+                                                               double* __restrict__ out) {
+    __shared__ double sh[kBlock];
+    double acc = 0.0;
+    for (int k = threadIdx.x; k < nb; k += kBlock) acc = acc + partial[k];
+    const double total = block_sum(acc, sh);
+    if (threadIdx.x == 0) *out = total;
 }
 
 // ---------------------------------------------------------------------------------------

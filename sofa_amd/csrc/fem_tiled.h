@@ -67,6 +67,8 @@ struct SharedNodes {
     const int* __restrict__ gid;        // shared (and element-free) nodes
     const int* __restrict__ ptr;        // their partial slots [ptr[i], ptr[i+1]) in tile order
     const int* __restrict__ of;         // per node: its index in gid / ptr, or -1 (tile-owned)
+    const int* __restrict__ ord;        // reference order: [ptr[i], ptr[i+1]) indexes this list of
+                                        // per-incidence slots in ascending element id (else null)
 };
 
 // per-scene slot ranges of the two partial buffers
@@ -188,6 +190,7 @@ struct TileArgs {
     int8_t* __restrict__ contact;        // PlaneForceField contacts of the step (per node)
     const R* __restrict__ emass;        // per element rho V / 20 (consistent MeshMatrixMass), else unused
     float* __restrict__ x_lo;           // MIXED engines: float copy of the positions (written by the RHS)
+    double* __restrict__ nd;            // reference order: per-node dot terms for k_seqdot
 };
 
 // consistent-mass edge terms of one element, folded into its nodal contributions (which the
@@ -236,6 +239,19 @@ __device__ __forceinline__ void sum_incidences(const uint16_t* linc, int s0, int
     }
 }
 
+// reference order: each incidence's contribution lc[slot + c kTileElems] to its own global slot
+// base, base + 1, ... (the node's finalize adds them in ascending element id)
+template <typename R, typename VT>
+__device__ __forceinline__ void store_incidences(const uint16_t* linc, int s0, int s1, const R* lc, VT* __restrict__ sp,
+                                                 int base) {
+    for (int s = s0; s < s1; s++) {
+        const int sl = linc[s];
+        VT* o = sp + 3 * (int64_t)(base + (s - s0));
+#pragma unroll
+        for (int i = 0; i < 3; i++) o[i] = (VT)lc[sl + i * kTileElems];
+    }
+}
+
 __device__ __forceinline__ void unpack_ltet(uint2 w, int l[4]) {
     l[0] = (int)(w.x & 0xffffu); l[1] = (int)(w.x >> 16); l[2] = (int)(w.y & 0xffffu); l[3] = (int)(w.y >> 16);
 }
@@ -246,7 +262,9 @@ __device__ __forceinline__ void unpack_ltet(uint2 w, int l[4]) {
 // contribution buffer serves both element passes: the force pass, the node sums of f, then the
 // K v pass (element record and rotation kept in registers) and the node sums of b.
 // ---------------------------------------------------------------------------------------
-template <typename R, int NR, typename VT, int TB = 512>
+// EXACT: the reference's summation order -- a shared node's contributions go to per-incidence
+// slots (f to spf, K v to spv) instead of per-tile partials, and every node's |b|^2 term to nd
+template <typename R, int NR, typename VT, int TB = 512, bool EXACT = false>
 __global__ __launch_bounds__(TB) void k_tile_rhs(TileArgs<R, NR> a, const VT* __restrict__ x,
                                                   const VT* __restrict__ v, VT* __restrict__ f,
                                                   VT* __restrict__ b, VT* __restrict__ r, VT* __restrict__ xcg,
@@ -331,7 +349,8 @@ __global__ __launch_bounds__(TB) void k_tile_rhs(TileArgs<R, NR> a, const VT* __
             if (psl[m] < 0) mass_gravity(in, a.mass, gid[m], fn[m]);
             else { fn[m][0] = 0; fn[m][1] = 0; fn[m][2] = 0; }
             const int s0 = lptr[j], s1 = lptr[j + 1];
-            sum_incidences<false>(linc, s0, s1, lc, fn[m]);
+            if (EXACT && psl[m] >= 0) store_incidences(linc, s0, s1, lc, spf, psl[m]);
+            else sum_incidences<false>(linc, s0, s1, lc, fn[m]);
         }
     }
     __syncthreads();
@@ -390,7 +409,10 @@ __global__ __launch_bounds__(TB) void k_tile_rhs(TileArgs<R, NR> a, const VT* __
                 st3(r, g, bn);
                 const VT z[3] = {0, 0, 0};
                 st3(xcg, g, z);
-                bb += dot3d(bn, bn);
+                if (EXACT) a.nd[g] = dot3d(bn, bn);
+                else bb += dot3d(bn, bn);
+            } else if (EXACT) {
+                store_incidences(linc, s0, s1, lc, spv, psl[m]);
             } else {
                 const int ps = psl[m] & (kSlotOwner - 1);
                 VT pv[3] = {0, 0, 0};
@@ -400,18 +422,21 @@ __global__ __launch_bounds__(TB) void k_tile_rhs(TileArgs<R, NR> a, const VT* __
             }
         }
     }
-    const double tot = wave_block_sum<TB>(bb, sh);
-    if (threadIdx.x == 0) pbb[td.pslot] = tot;
+    if (!EXACT) {
+        const double tot = wave_block_sum<TB>(bb, sh);
+        if (threadIdx.x == 0) pbb[td.pslot] = tot;
+    }
 }
 
-template <typename VT>
+template <typename VT, bool EXACT = false>
 __global__ __launch_bounds__(kBlock) void k_fin_rhs(const FinBlk* __restrict__ fb_, const InstDev* __restrict__ inst,
                                                      const Ranges* __restrict__ ranges, SharedNodes sn, const double* __restrict__ mass,
                                                      const int8_t* __restrict__ fixed, const VT* __restrict__ v,
                                                      const VT* __restrict__ spf, const VT* __restrict__ spv,
                                                      VT* __restrict__ f, VT* __restrict__ b, VT* __restrict__ r,
                                                      VT* __restrict__ xcg, double* __restrict__ pbb,
-                                                     const VT* __restrict__ x, int8_t* __restrict__ contact) {
+                                                     const VT* __restrict__ x, int8_t* __restrict__ contact,
+                                                     double* __restrict__ nd = nullptr) {
     __shared__ double sh[kBlock / 64];
     const FinBlk fb = fb_[blockIdx.x];
     const InstDev& in = inst[fb.scene];
@@ -424,8 +449,9 @@ __global__ __launch_bounds__(kBlock) void k_fin_rhs(const FinBlk* __restrict__ f
         mass_gravity(in, mass, g, fn);
         const int s0 = sn.ptr[j], s1 = sn.ptr[j + 1];
         for (int s = s0; s < s1; s++) {
+            const int64_t o = 3 * (int64_t)(EXACT ? sn.ord[s] : s);
 #pragma unroll
-            for (int i = 0; i < 3; i++) fn[i] += spf[3 * (int64_t)s + i];
+            for (int i = 0; i < 3; i++) fn[i] += spf[o + i];
         }
         // PlaneForceField after the FEM (scene order); the contact is kept for addDForce
         bool touch = false;
@@ -439,8 +465,9 @@ __global__ __launch_bounds__(kBlock) void k_fin_rhs(const FinBlk* __restrict__ f
         VT bn[3] = {fn[0], fn[1], fn[2]};
         if (in.m_rhs != 0.0) mass_mdx(in, NodeArrays{nullptr, nullptr, mass, fixed}, g, in.m_rhs, vn, bn);
         for (int s = s0; s < s1; s++) {
+            const int64_t o = 3 * (int64_t)(EXACT ? sn.ord[s] : s);
 #pragma unroll
-            for (int i = 0; i < 3; i++) bn[i] -= spv[3 * (int64_t)s + i];
+            for (int i = 0; i < 3; i++) bn[i] -= spv[o + i];
         }
         if (touch && in.k_rhs != 0.0) plane_dforce(in, in.k_rhs, vn, bn);
         const VT h = (VT)in.b_scale;   // h (firstOrder: 1, b = f)
@@ -451,7 +478,9 @@ __global__ __launch_bounds__(kBlock) void k_fin_rhs(const FinBlk* __restrict__ f
         const VT z[3] = {0, 0, 0};
         st3(xcg, g, z);
         bb = dot3d(bn, bn);
+        if (EXACT) nd[g] = bb;
     }
+    if (EXACT) return;
     const double own = wave_block_sum<kBlock>(bb, sh);
     if (threadIdx.x == 0) pbb[fb.pslot] = fold_tiles(pbb, ranges[fb.scene], fb.bi) + own;
 }
@@ -469,7 +498,11 @@ __global__ __launch_bounds__(kBlock) void k_fin_rhs(const FinBlk* __restrict__ f
 // less per element, 3 more per tile-local node.  2: also the rotated rest coordinates from the
 // rest positions x0 (initLarge, TetrahedronFEMForceField.inl:835-868) and, for a scene of one
 // material, K from the rest volume -- NR (+ 3) reals less per element, 3 more per local node.
-template <typename R, int NR, typename VT, int TB = kBlock, int GEO = 0, bool WIDE = true, bool PLANE = true>
+// EXACT: the reference's summation order -- |b|^2 and r.r come from k_seqdot (InstDev::dots),
+// interior nodes write their p.q term to nd, shared nodes their per-incidence contributions
+// (finished by k_fin_exact), and no tile partial of p.q is formed.
+template <typename R, int NR, typename VT, int TB = kBlock, int GEO = 0, bool WIDE = true, bool PLANE = true,
+          bool EXACT = false>
 __global__ __launch_bounds__(TB) void k_tile_apply(TileArgs<R, NR> a, int iter, const VT* __restrict__ r,
                                                     VT* __restrict__ p, VT* __restrict__ q, VT* __restrict__ sp,
                                                     const double* __restrict__ pin,   // |b|^2 (iter 1) or r.r partials
@@ -560,8 +593,9 @@ __global__ __launch_bounds__(TB) void k_tile_apply(TileArgs<R, NR> a, int iter, 
             ins->rho_slot[1] = rho; ins->beta = beta; ins->iter = 1; ins->err = err;
         }
     } else if (iter == 1) {
-        const double bb = pin_nodes ? scene_sum<TB, WIDE>(pin, rg.b0, rg.nb, sh)
-                                    : rg.nf > 0 ? scene_sum<TB, WIDE>(pin, rg.f0, rg.nf, sh) : scene_sum<TB, WIDE>(pin, rg.t0, rg.nt, sh);
+        const double bb = EXACT ? ins->dots[kDotBB]
+                          : pin_nodes ? scene_sum<TB, WIDE>(pin, rg.b0, rg.nb, sh)
+                          : rg.nf > 0 ? scene_sum<TB, WIDE>(pin, rg.f0, rg.nf, sh) : scene_sum<TB, WIDE>(pin, rg.t0, rg.nt, sh);
         const double normb = sqrt(bb);
         if (normb == 0.0) { record_stop(ins, first, 4, 0, 1.0); return; }
         const double err = sqrt(bb) / normb;
@@ -571,7 +605,7 @@ __global__ __launch_bounds__(TB) void k_tile_apply(TileArgs<R, NR> a, int iter, 
         }
     } else if (iter > 1) {
         if (iter > ins->max_iter) { record_stop(ins, first, 0, ins->max_iter, ins->err); return; }
-        const double rho = scene_sum<TB, WIDE>(pin, rg.b0, rg.nb, sh);
+        const double rho = EXACT ? ins->dots[kDotRR] : scene_sum<TB, WIDE>(pin, rg.b0, rg.nb, sh);
         const double err = sqrt(rho) / ins->normb;
         if (err <= ins->tol) { record_stop(ins, first, 1, iter, err); return; }
         const double rho_1 = ins->rho_slot[(iter - 1) & 1];
@@ -669,7 +703,10 @@ __global__ __launch_bounds__(TB) void k_tile_apply(TileArgs<R, NR> a, int iter, 
                 if (PLANE && ct[m]) plane_dforce(in, in.k_cg, pp, qn);
                 if (fx[m]) { qn[0] = 0; qn[1] = 0; qn[2] = 0; }
                 st3(q, gid[m], qn);
-                pq += dot3d(pp, qn);
+                if (EXACT) a.nd[gid[m]] = dot3d(pp, qn);
+                else pq += dot3d(pp, qn);
+            } else if (EXACT) {
+                store_incidences(linc, s0, s1, lc, sp, psl[m]);
             } else {
                 VT acc[3] = {0, 0, 0};
                 sum_incidences<true>(linc, s0, s1, lc, acc);
@@ -691,8 +728,50 @@ __global__ __launch_bounds__(TB) void k_tile_apply(TileArgs<R, NR> a, int iter, 
             }
         }
     }
-    const double tot = wave_block_sum<TB>(pq, sh);
-    if (threadIdx.x == 0) pden[td.pslot] = tot;
+    if (!EXACT) {
+        const double tot = wave_block_sum<TB>(pq, sh);
+        if (threadIdx.x == 0) pden[td.pslot] = tot;
+    }
+}
+
+// reference order, after the tile apply: the shared nodes' p = r + beta p, q = (1 + h rm) M p
+// minus their element contributions one by one in ascending element id, plane and projection,
+// and their p.q terms for k_seqdot (the work k_update does for them in tile order otherwise)
+template <typename VT>
+__global__ __launch_bounds__(kBlock) void k_fin_exact(const FinBlk* __restrict__ fb_, InstDev* __restrict__ inst,
+                                                       SharedNodes sn, const double* __restrict__ mass,
+                                                       const int8_t* __restrict__ fixed, int iter,
+                                                       const VT* __restrict__ r, VT* __restrict__ p,
+                                                       VT* __restrict__ q, const VT* __restrict__ sp,
+                                                       double* __restrict__ nd, const int8_t* __restrict__ contact) {
+    const FinBlk fb = fb_[blockIdx.x];
+    const InstDev& in = inst[fb.scene];
+    if (in.stopped) return;
+    const int j = fb.begin + threadIdx.x;
+    if (j >= fb.end) return;
+    const int g = sn.gid[j];
+    VT pp[3];
+    ld3(r, g, pp);
+    if (iter > 1) {
+        const VT vb = (VT)in.beta;
+        VT po[3];
+        ld3(p, g, po);
+#pragma unroll
+        for (int i = 0; i < 3; i++) { VT t = po[i] * vb; t += pp[i]; pp[i] = t; }
+    }
+    st3(p, g, pp);
+    VT qn[3] = {0, 0, 0};
+    if (in.m_cg != 0.0) mass_mdx(in, NodeArrays{nullptr, nullptr, mass, fixed}, g, in.m_cg, pp, qn);
+    const int s0 = sn.ptr[j], s1 = sn.ptr[j + 1];
+    for (int s = s0; s < s1; s++) {
+        const int64_t o = 3 * (int64_t)sn.ord[s];
+#pragma unroll
+        for (int i = 0; i < 3; i++) qn[i] -= sp[o + i];
+    }
+    if (in.plane_on && contact[g]) plane_dforce(in, in.k_cg, pp, qn);
+    if (fixed[g]) { qn[0] = 0; qn[1] = 0; qn[2] = 0; }
+    st3(q, g, qn);
+    nd[g] = dot3d(pp, qn);
 }
 
 template <typename VT>
@@ -743,14 +822,16 @@ __global__ __launch_bounds__(kBlock) void k_fin_apply(const FinBlk* __restrict__
 // CGLinearSolver's p update and GraphScatteredMatrix::apply do for them).  Then den = p.q from
 // the tile partials (which already hold every node's term), the threshold test and alpha
 // (CGLinearSolver.inl:207-265), x += alpha p, r -= alpha q and the r.r partials.
-template <typename VT>
+// EXACT: every node's p and q are final (k_fin_exact finished the shared ones), den is the
+// reference-order p.q of k_seqdot, and each node's r.r term goes to nd
+template <typename VT, bool EXACT = false>
 __global__ __launch_bounds__(kBlock) void k_update(const NodeBlk* __restrict__ nb_, InstDev* __restrict__ inst,
                                                     const Ranges* __restrict__ ranges, int iter, SharedNodes sn,
                                                     const VT* __restrict__ sp, const double* __restrict__ mass,
                                                     const int8_t* __restrict__ fixed, VT* __restrict__ p,
                                                     const VT* __restrict__ q, VT* __restrict__ xcg, VT* __restrict__ r,
                                                     const double* __restrict__ pden, double* __restrict__ prho,
-                                                    const int8_t* __restrict__ contact) {
+                                                    const int8_t* __restrict__ contact, double* __restrict__ nd = nullptr) {
     __shared__ double sh[kBlock / 64];
     const NodeBlk nb = nb_[blockIdx.x];
     InstDev* ins = &inst[nb.scene];
@@ -761,7 +842,7 @@ __global__ __launch_bounds__(kBlock) void k_update(const NodeBlk* __restrict__ n
     for (int k = 0; k < kNodesPerThread; k++) {
         const int n = nb.begin + k * kBlock + threadIdx.x;
         if (n < nb.end) {
-            shk[k] = sn.of[n];
+            shk[k] = EXACT ? -1 : sn.of[n];
             ld3(xcg, n, xv[k]);
             ld3(r, n, rv[k]);
             if (shk[k] < 0) {
@@ -776,7 +857,7 @@ __global__ __launch_bounds__(kBlock) void k_update(const NodeBlk* __restrict__ n
     const bool cont = iter == 1 && ins->keep_dir && ins->sweep > 1;
     const Ranges rg = ranges[nb.scene];
     const bool first = nb.first != 0;
-    const double den = scene_sum<kBlock>(pden, rg.t0, rg.nt, sh);
+    const double den = EXACT ? ins->dots[kDotPQ] : scene_sum<kBlock>(pden, rg.t0, rg.nt, sh);
     const bool exempt = (iter == 1 && ins->first_solve);
     if (den == 0.0) { record_stop(ins, first, 3, iter, ins->err); if (first && threadIdx.x == 0) ins->den = den; return; }
     if (fabs(den) <= ins->thr && !exempt) {
@@ -821,10 +902,42 @@ __global__ __launch_bounds__(kBlock) void k_update(const NodeBlk* __restrict__ n
             }
             st3(xcg, n, xn);
             st3(r, n, rn);
-            rr += dot3d(rn, rn);
+            if (EXACT) nd[n] = dot3d(rn, rn);
+            else rr += dot3d(rn, rn);
         }
     }
+    if (EXACT) return;
     const double tot = wave_block_sum<kBlock>(rr, sh);
+    if (threadIdx.x == 0) prho[nb.pslot] = tot;
+}
+
+// start of a component-level CG solve on the tiled kernels: r = b = the caller's right-hand side,
+// x = 0, and |b|^2 as per-node-block partials (read by the tile apply of iteration 1 with
+// pin_nodes = 1) or, in the reference order, as per-node terms for k_seqdot
+template <typename VT, bool EXACT = false>
+__global__ __launch_bounds__(kBlock) void k_cg_begin_tiled(const NodeBlk* __restrict__ nb_, const VT* __restrict__ b_in,
+                                                            int nb0, VT* __restrict__ b, VT* __restrict__ r,
+                                                            VT* __restrict__ xcg, double* __restrict__ prho,
+                                                            double* __restrict__ nd) {
+    __shared__ double sh[kBlock / 64];
+    const NodeBlk nb = nb_[blockIdx.x];
+    double bb = 0.0;
+#pragma unroll
+    for (int k = 0; k < kNodesPerThread; k++) {
+        const int n = nb.begin + k * kBlock + threadIdx.x;
+        if (n < nb.end) {
+            VT bn[3];
+            ld3(b_in, n - nb0, bn);
+            st3(b, n, bn);
+            st3(r, n, bn);
+            const VT z[3] = {0, 0, 0};
+            st3(xcg, n, z);
+            if (EXACT) nd[n] = dot3d(bn, bn);
+            else bb += dot3d(bn, bn);
+        }
+    }
+    if (EXACT) return;
+    const double tot = wave_block_sum<kBlock>(bb, sh);
     if (threadIdx.x == 0) prho[nb.pslot] = tot;
 }
 

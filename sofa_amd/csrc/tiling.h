@@ -32,6 +32,7 @@ struct TilePlan {
     std::vector<int> ln_pslot;      // -1 interior, else scene-local shared-partial slot
     std::vector<int> sh_gid;        // nodes finished by the finalize kernel (ascending)
     std::vector<int> sh_ptr;        // size(sh_gid) + 1
+    std::vector<int> sh_ord;        // reference order only (plan_exact): slots in element order
     int n_slots = 0;
 };
 
@@ -230,6 +231,46 @@ static inline TilePlan plan_tiles(int n_nodes, const double* rest, const int32_t
         }
     }
     return P;
+}
+
+// The reference's summation order for the nodes shared by several tiles.  A per-tile partial
+// sum changes the association of a node's element contributions ((m + c1 + c2) + (c3 + c4)
+// instead of m + c1 + c2 + c3 + c4), so here every incidence of a shared node gets a slot of its
+// own: a tile writes its element's contribution to the node there (its incidences of the node
+// at ln_pslot, ln_pslot + 1, ... in its local list order), and the node's slots are listed in
+// ascending ORIGINAL element id in sh_ord[sh_ptr[k] .. sh_ptr[k + 1]), the order in which the
+// finalize kernel adds them after the mass term -- TetrahedronFEMForceField's element loop
+// (TetrahedronFEMForceField.inl:1630-1634).  Interior nodes keep their in-tile sums, which are
+// already in that order.
+static inline void plan_exact(TilePlan& P, int n_nodes, int max_elems) {
+    const int TT = (int)P.t_ebegin.size();
+    std::vector<int> shidx(n_nodes, -1);
+    for (size_t j = 0; j < P.sh_gid.size(); j++) shidx[P.sh_gid[j]] = (int)j;
+    std::vector<std::vector<std::pair<int, int>>> lists(P.sh_gid.size());   // (original element, slot)
+    int next = 0;
+    for (int ti = 0; ti < TT; ti++) {
+        for (int j = 0; j < P.t_lncount[ti]; j++) {
+            const int li = P.t_lnbegin[ti] + j;
+            const int k = shidx[P.ln_gid[li]];
+            if (k < 0) continue;
+            const int c0 = P.ln_inc_ptr[li], c1 = P.ln_inc_ptr[li + 1];
+            P.ln_pslot[li] = next;
+            for (int c = c0; c < c1; c++) {
+                const int local_elem = (int)P.ln_inc[c] % max_elems;   // slot = 3 max_elems v + element
+                lists[k].push_back({P.perm[P.t_ebegin[ti] + local_elem], next + (c - c0)});
+            }
+            next += c1 - c0;
+        }
+    }
+    P.sh_ord.clear();
+    P.sh_ord.reserve(next);
+    P.sh_ptr.assign(1, 0);
+    for (auto& lst : lists) {
+        std::sort(lst.begin(), lst.end());
+        for (auto& pr : lst) P.sh_ord.push_back(pr.second);
+        P.sh_ptr.push_back((int)P.sh_ord.size());
+    }
+    P.n_slots = next;
 }
 
 }  // namespace sofa_amd

@@ -28,9 +28,20 @@ unchanged. Read, with their Data names and defaults:
   ``threshold``);
 * ``PlaneForceField`` (``normal``, ``d``, ``stiffness``, ``damping``, ``maxForce``, ``bilateral``).
 
+``TetrahedronFEMForceField`` ``localStiffnessFactor`` is applied as the reference's
+computeMaterialStiffness does (element i's Young modulus times factor[i M / N], M factors for N
+elements, TetrahedronFEMForceField.inl:256-261), by handing the engine the products per element.
+
 Every other component (visual models, collision pipeline, plugins, surface or visual mappings)
-is listed in ``ignored`` and plays no part in the bodies; a component on the path with an option this build does not run
-(``method`` other than large, CG ``warmStart``, ``solveConstraint``) raises ``ValueError``.
+is listed in ``ignored`` and plays no part in the bodies.  A component on the path with a Data
+value this build does not run raises ``ValueError`` rather than running other physics:
+``method`` other than large, ``plasticMaxThreshold`` > 0, ``updateStiffnessMatrix``,
+``updateStiffness`` or ``computeGlobalMatrix`` set, a force field's or mass's own Rayleigh
+coefficient (``rayleighStiffness``, ``rayleighMass``) other than 0, PlaneForceField
+``localRange``, CG ``warmStart``, ``solveConstraint``, a DiagonalMass given per-vertex masses
+only, a UniformMass given no mass at all (the reference marks it invalid, UniformMass.inl
+:253-266), and a solver that governs more than one body (SOFA solves such bodies as one system,
+with one CG over all of them).
 """
 import xml.etree.ElementTree as ET
 
@@ -249,18 +260,37 @@ def load_scene(path_or_text, search_paths=()):
 
     # solvers govern their node's subtree (a context lookup searches up the graph): the nearest
     # EulerImplicitSolver / CGLinearSolver at or above a body's node is its solver
+    governed = {}
+
     def visit(node, solver_up, cg_up):
         solver = node.first("EulerImplicitSolver") or solver_up
         cg = node.first("CGLinearSolver") or cg_up
         fem = node.first("TetrahedronFEMForceField")
         mo = node.first("MechanicalObject")
         if solver is not None and fem is not None and mo is not None:
+            # one solver over several mechanical objects is one linear system in SOFA (one CG,
+            # its dot products summed over the objects): not what separate scenes compute
+            if id(solver) in governed:
+                raise ValueError(f"EulerImplicitSolver of node {solver.node.name!r} governs bodies "
+                                 f"{governed[id(solver)]!r} and {node.name!r}: one system over several "
+                                 "mechanical objects is not on this path (give each body its own solver)")
+            governed[id(solver)] = node.name
             bodies.append(_body(node, solver, cg, fem, mo, g, dt, node_positions))
             names.append(node.name)
         for c in node.children:
             visit(c, solver, cg)
     visit(root, None, None)
     return bodies, {"bodies": names, "ignored": sorted(set(ignored))}
+
+
+def _refuse(obj, key, allowed, what):
+    """ValueError when Data `key` of `obj` is set to a value other than the one(s) this path runs."""
+    v = obj.attr.get(key)
+    if v is None:
+        return
+    ok = allowed(v)
+    if not ok:
+        raise ValueError(f"{obj.tag} {key}={v!r}: {what} is not on this path")
 
 
 def _body(node, solver, cg, fem, mo, g, dt, node_positions):
@@ -282,8 +312,25 @@ def _body(node, solver, cg, fem, mo, g, dt, node_positions):
         tets = meshes.split_hexa_fem(grid.data["hexahedra"], nx, ny)
     if fem.attr.get("method", "large") != "large":
         raise ValueError("TetrahedronFEMForceField: only method=large is on this path")
+    zero = lambda v: all(x == 0.0 for x in _floats(v))
+    false = lambda v: not _bool(v)
+    _refuse(fem, "plasticMaxThreshold", zero, "plasticity (TetrahedronFEMForceField.inl:357-367)")
+    _refuse(fem, "updateStiffnessMatrix", false, "J recomputed from the deformed shape")
+    _refuse(fem, "updateStiffness", false, "the stiffness rebuilt from its Data every step")
+    for key in ("computeGlobalMatrix", "assembling"):
+        _refuse(fem, key, false, "the assembled global matrix")
+    _refuse(fem, "rayleighStiffness", zero, "the force field's own Rayleigh damping")
     E = _floats(fem.attr.get("youngModulus", "5000"))
     nu = _floats(fem.attr.get("poissonRatio", "0.45"))
+    lsf = _floats(fem.attr.get("localStiffnessFactor", ""))
+    if lsf:
+        # computeMaterialStiffness (:256-261): youngModulus = localStiffnessFactor[i M / N] *
+        # getYoungModulusInElement(i), the latter E[i] or E[0] beyond the list
+        # (BaseLinearElasticityFEMForceField.inl:113-129); the products are the same doubles
+        ne, m = len(tets), len(lsf)
+        Ei = np.array([E[i] if i < len(E) else E[0] for i in range(ne)], dtype=np.float64)
+        fac = np.asarray(lsf, dtype=np.float64)[(np.arange(ne, dtype=np.int64) * m) // ne]
+        E = list(fac * Ei)
     n = len(pos)
     spec = dict(pos=pos, tets=np.ascontiguousarray(tets, dtype=np.int32), E=np.array(E), nu=np.array(nu),
                 g=tuple(g), dt=dt)
@@ -292,16 +339,30 @@ def _body(node, solver, cg, fem, mo, g, dt, node_positions):
         spec["von_mises"] = vm
     # mass
     um, dm, mm = node.first("UniformMass"), node.first("DiagonalMass"), node.first("MeshMatrixMass")
+    for m in (um, dm, mm):
+        if m is not None:
+            _refuse(m, "rayleighMass", zero, "the mass's own Rayleigh damping")
     if um is not None:
+        # UniformMass.inl:233-266: totalMass has priority over vertexMass; neither is invalid
         if "totalMass" in um.attr:
             spec["mass"] = ("uniform", float(um.attr["totalMass"]) / float(n))   # UniformMass.inl:329-345
+        elif "vertexMass" in um.attr:
+            spec["mass"] = ("uniform", float(um.attr["vertexMass"]))
         else:
-            spec["mass"] = ("uniform", float(um.attr.get("vertexMass", "1")))
+            raise ValueError(f"node {node.name}: UniformMass without vertexMass or totalMass "
+                             "(the reference marks the component invalid)")
     elif dm is not None:
-        if "massDensity" in dm.attr and "totalMass" not in dm.attr:
+        # DiagonalMass::massInitialization (DiagonalMass.inl:880-935): totalMass, then
+        # massDensity, then vertexMass; none: totalMass = 1
+        if "totalMass" in dm.attr:
+            spec["mass"] = ("diagonal_total", float(dm.attr["totalMass"]))
+        elif "massDensity" in dm.attr:
             spec["mass"] = ("diagonal", float(dm.attr["massDensity"]))
+        elif "vertexMass" in dm.attr:
+            raise ValueError(f"node {node.name}: DiagonalMass given per-vertex masses (vertexMass) is not on "
+                             "this path; give massDensity or totalMass")
         else:
-            spec["mass"] = ("diagonal_total", float(dm.attr.get("totalMass", "1")))
+            spec["mass"] = ("diagonal_total", 1.0)
     elif mm is not None:
         rho = _floats(mm.attr.get("massDensity", "1"))
         kind = "meshmatrix_lumped" if _bool(mm.attr.get("lumping", "false")) else "meshmatrix"
@@ -345,6 +406,9 @@ def _body(node, solver, cg, fem, mo, g, dt, node_positions):
     spec["thr"] = float(c.get("threshold", "1e-5"))
     pf = node.first("PlaneForceField")
     if pf is not None:
+        _refuse(pf, "rayleighStiffness", zero, "the force field's own Rayleigh damping")
+        if "localRange" in pf.attr and _floats(pf.attr["localRange"]) != [-1.0, -1.0]:
+            raise ValueError(f"PlaneForceField localRange={pf.attr['localRange']!r}: a node range is not on this path")
         p = pf.attr
         spec["plane"] = dict(normal=tuple(_floats(p.get("normal", "0 1 0"))), d=float(p.get("d", "0")),
                              stiffness=float(p.get("stiffness", "500")), damping=float(p.get("damping", "5")),

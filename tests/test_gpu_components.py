@@ -1,0 +1,201 @@
+"""GPU parity of the component-level C ABI beyond the force field: Mass::addMDx / accFromF /
+addForce, the projective constraints' projectResponse / projectVelocity / projectPosition,
+vDot in both summation orders and CGLinearSolver::solve on caller device vectors -- each call
+against the oracle's restatement of the same component on the same inputs.
+
+Bars: bit-exact on the FP64 engine (same operations in the same order, contraction off); the
+oscillator's sin / cos come from the device's libm (the oracle's from the host's), so the
+oscillator entries are held to 1 ulp-level (4e-16 relative); the CG solve on the FP64 tiled
+engine in the reference summation order is bit-exact as well.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _capi():
+    from sofa_amd import capi
+    return capi
+
+
+def _scene(kind):
+    """cfg0 cantilever (5x5x21, z = 0 fixed) with the requested mass and two oscillating nodes."""
+    from sofa_amd import workloads
+    s = workloads.cfg0()
+    if kind == "diagonal":
+        s["mass"] = ("diagonal", 1000.0)
+    elif kind == "meshmatrix":
+        s["mass"] = ("meshmatrix", 1000.0)
+    elif kind == "lumped":
+        s["mass"] = ("meshmatrix_lumped", 1000.0)
+    top = np.nonzero(s["pos"][:, 2] == s["pos"][:, 2].max())[0][:2]
+    amp = np.zeros((len(top), 3))
+    amp[:, 0] = 0.003
+    amp[:, 1] = -0.002
+    s["oscillators"] = dict(index=top.astype(np.int32), mean=s["pos"][top].copy(), amplitude=amp,
+                            pulsation=np.full(len(top), 2 * np.pi * 1.7), phase=np.array([0.0, 0.4]))
+    return s
+
+
+def _oracle(spec, dot_mode=0):
+    from specs import oracle_from_spec
+    return oracle_from_spec(spec, dot_mode)
+
+
+def _dev(a):
+    import torch
+    return torch.tensor(np.ascontiguousarray(a), dtype=torch.float64, device="cuda")
+
+
+@pytest.mark.parametrize("kind", ["uniform", "diagonal", "meshmatrix", "lumped"])
+def test_mass_and_projection_calls(kind):
+    """addMDx (factor 1 and not), accFromF (refused for the consistent mass, as the reference
+    does), addForce, projectResponse, projectVelocity (with activate_projectVelocity) and
+    projectPosition at a given time, on the FP64 engine."""
+    import torch
+    from sofa_amd import workloads
+    capi = _capi()
+    print(f"case: mass and projection calls, {kind}", flush=True)
+    spec = _scene(kind)
+    spec["project_velocity"] = True
+    n = len(spec["pos"])
+    rng = np.random.default_rng(7)
+    dx, f0 = rng.standard_normal((n, 3)), rng.standard_normal((n, 3))
+    o = _oracle(spec)
+    o.set_project_velocity(True)
+    eng = capi.Engine(0, capi.FP64)
+    sid = workloads.add_to_engine(eng, spec)
+    eng.commit()
+    for factor in (1.0, -0.37):
+        dxd, fd = _dev(dx), _dev(f0)
+        eng.mass_add_mdx(sid, dxd.data_ptr(), fd.data_ptr(), factor)
+        eng.sync()
+        assert np.array_equal(fd.cpu().numpy(), o.mass_add_mdx(dx, f0, factor))
+    fd, ad = _dev(f0), _dev(np.zeros((n, 3)))
+    if kind == "meshmatrix":
+        with pytest.raises(capi.SofaAmdError) as ei:
+            eng.mass_acc_from_f(sid, fd.data_ptr(), ad.data_ptr())
+        assert ei.value.code == capi.EINVAL
+        with pytest.raises(ValueError):
+            o.mass_acc_from_f(f0)
+    else:
+        eng.mass_acc_from_f(sid, fd.data_ptr(), ad.data_ptr())
+        eng.sync()
+        a_ref = o.mass_acc_from_f(f0)
+        assert np.array_equal(ad.cpu().numpy(), a_ref)
+        # closed form: M (M^-1 f) = f up to the division's rounding
+        back = o.mass_add_mdx(a_ref, np.zeros((n, 3)), 1.0)
+        assert np.allclose(back, f0, rtol=1e-14, atol=1e-14)
+    fd = _dev(f0)
+    eng.mass_add_force(sid, fd.data_ptr())
+    eng.sync()
+    assert np.array_equal(fd.cpu().numpy(), o.mass_add_force(f0))
+    # projective constraints
+    vd = _dev(dx)
+    eng.project_response(sid, vd.data_ptr())
+    eng.sync()
+    pr = o.project(0, dx)
+    assert np.array_equal(vd.cpu().numpy(), pr)
+    assert np.all(pr[spec["fixed"]] == 0) and np.all(pr[spec["oscillators"]["index"]] == 0)
+    t = 0.37
+    for what, call in ((1, eng.project_velocity), (2, eng.project_position)):
+        vd = _dev(dx)
+        call(sid, vd.data_ptr(), t)
+        eng.sync()
+        got, ref = vd.cpu().numpy(), o.project(what, dx, t)
+        osc = spec["oscillators"]["index"]
+        rest = np.setdiff1d(np.arange(n), osc)
+        assert np.array_equal(got[rest], ref[rest])
+        assert np.allclose(got[osc], ref[osc], rtol=4e-16, atol=1e-18)
+    eng.close()
+
+
+@pytest.mark.parametrize("prec", ["FP64", "FP32"])
+def test_vdot_orders(prec):
+    """vDot of two caller vectors: order 1 equals the oracle's sequential sum bit for bit (FP64;
+    FP32 vectors: the same sum of the float values' exact products), order 0 the oracle's blocked
+    order; the scene (21,525 nodes) spans 84 blocks."""
+    import torch
+    from sofa_amd import meshes
+    capi = _capi()
+    print(f"case: vdot {prec}", flush=True)
+    pos, tets, fixed = meshes.beam((15, 15, 97), (1.0, 1.0, 4.0))
+    n = len(pos)
+    eng = capi.Engine(0, getattr(capi, prec))
+    sid = eng.add_scene(pos, tets, [1e5], [0.45], fixed, mass_value=1.0)
+    eng.commit()
+    rng = np.random.default_rng(11)
+    a, b = rng.standard_normal((n, 3)), rng.standard_normal((n, 3))
+    dt = torch.float64 if prec == "FP64" else torch.float32
+    if prec == "FP32":
+        a, b = a.astype(np.float32).astype(np.float64), b.astype(np.float32).astype(np.float64)
+    ad = torch.tensor(a, dtype=dt, device="cuda")
+    bd = torch.tensor(b, dtype=dt, device="cuda")
+    out = torch.zeros(2, dtype=torch.float64, device="cuda")
+    eng.vdot(sid, ad.data_ptr(), bd.data_ptr(), 1, out.data_ptr())
+    eng.vdot(sid, ad.data_ptr(), bd.data_ptr(), 0, out.data_ptr() + 8)
+    eng.sync()
+    from oracle import OracleScene
+    o = OracleScene(pos)
+    seq, blk = o.vdot(a, b, 0), o.vdot(a, b, 1)
+    got = out.cpu().numpy()
+    assert got[0] == seq, (got[0], seq)
+    assert got[1] == blk, (got[1], blk)
+    assert abs(seq - float(np.sum(a * b))) <= 1e-12 * np.sum(np.abs(a * b))
+    eng.close()
+
+
+@pytest.mark.parametrize("prec,order", [("FP64", 0), ("FP64", 1), ("FP64_TILED", 1)])
+def test_cg_solve_component(prec, order):
+    """CGLinearSolver::solve on a caller right-hand side, after addForce at a deformed state:
+    x, the iteration count, the end reason and timeStepCount equal the oracle's solve of the same
+    system (the Euler step's factors m = 1 + h rm, b = -h, k = -h (h + rs)) bit for bit; a second
+    solve (timeStepCount 1: no first-solve exemption) as well."""
+    import torch
+    from sofa_amd import workloads
+    capi = _capi()
+    print(f"case: cg solve {prec} order {order}", flush=True)
+    spec = workloads.cfg0()
+    spec["iters"], spec["tol"] = 40, 1e-7
+    n = len(spec["pos"])
+    rng = np.random.default_rng(5)
+    x = spec["pos"] + 0.03 * rng.standard_normal((n, 3))
+    o = _oracle(spec, dot_mode=1 if order == 0 else 0)
+    o.fem_add_force(x, np.zeros((n, 3)))
+    eng = capi.Engine(0, getattr(capi, prec))
+    if order == 1:
+        eng.set_dot_order(True)
+    sid = workloads.add_to_engine(eng, spec)
+    eng.commit()
+    eng.write(sid, x=x)                      # the tiled operator recomputes rotations from these
+    xd, fd = _dev(x), _dev(np.zeros((n, 3)))
+    eng.fem_add_force(sid, xd.data_ptr(), fd.data_ptr())
+    h, rs, rm = spec["dt"], spec["rs"], spec["rm"]
+    mf, bf, kf = 1 + h * rm, -h, -h * (h + rs)
+    for k in range(2):
+        b = o.project(0, rng.standard_normal((n, 3)))
+        bd, sol = _dev(b), _dev(np.zeros((n, 3)))
+        st = eng.cg_solve(sid, mf, bf, kf, bd.data_ptr(), sol.data_ptr())
+        ref = o.cg_solve(b, mf, bf, kf)
+        so = o.stats()
+        got = sol.cpu().numpy()
+        print(f"  solve {k}: {st['iterations']} iterations, end {st['end_reason']}, err {st['err']:.3e}", flush=True)
+        assert np.array_equal(got, ref), float(np.max(np.abs(got - ref)))
+        assert st["iterations"] == so["iterations"] and st["end_reason"] == so["endcond"]
+        assert st["step_count"] == so["step_count"] == k + 1
+    eng.close()
+
+
+def test_potential_energy_large_is_zero():
+    """TetrahedronFEMForceField::getPotentialEnergy sums the small method only: 0 for large."""
+    import torch
+    from sofa_amd import workloads
+    capi = _capi()
+    spec = workloads.cfg0()
+    eng = capi.Engine(0, capi.FP64)
+    sid = workloads.add_to_engine(eng, spec)
+    eng.commit()
+    xd = _dev(spec["pos"] * 1.01)
+    assert eng.fem_potential_energy(sid, xd.data_ptr()) == 0.0
+    eng.close()

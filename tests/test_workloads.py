@@ -1,5 +1,6 @@
 """Host-side workload and serving logic (CPU only): job streams, meshes, footprints."""
 import numpy as np
+import pytest
 
 from sofa_amd import workloads
 
@@ -369,3 +370,103 @@ def test_scene_reader_von_mises_attribute():
     for mode, want in (("0", None), ("1", 1), ("2", 2)):
         specs, info = scene_xml.load_scene(text % mode)
         assert len(specs) == 1 and specs[0].get("von_mises") == want, (mode, specs[0].get("von_mises"))
+
+
+_STRICT_SCENE = """<Node name="root" dt="0.01" gravity="0 -9.81 0">
+  <EulerImplicitSolver %(solver)s/><CGLinearSolver iterations="20" tolerance="1e-9" threshold="1e-9"/>
+  <Node name="beam">
+    <RegularGridTopology name="grid" n="3 3 5" min="0 0 0" max="1 1 2"/>
+    <MechanicalObject/>
+    <TetrahedronFEMForceField youngModulus="1e5 2e5" poissonRatio="0.3" method="large" %(fem)s/>
+    <%(mass)s/>
+    <BoxROI name="roi" box="-0.1 -0.1 -0.1 1.1 1.1 0.1"/>
+    <FixedProjectiveConstraint indices="@roi.indices"/>
+    %(extra)s
+  </Node>%(second)s</Node>"""
+
+
+def _strict(fem="", mass='UniformMass totalMass="10"', solver="", extra="", second=""):
+    return _STRICT_SCENE % dict(fem=fem, mass=mass, solver=solver, extra=extra, second=second)
+
+
+@pytest.mark.parametrize("fem,mass,extra", [
+    ('plasticMaxThreshold="0.1"', None, ""),
+    ('updateStiffnessMatrix="true"', None, ""),
+    ('updateStiffness="1"', None, ""),
+    ('computeGlobalMatrix="true"', None, ""),
+    ('rayleighStiffness="0.2"', None, ""),
+    ("", 'UniformMass totalMass="10" rayleighMass="0.1"', ""),
+    ("", "UniformMass", ""),
+    ("", 'DiagonalMass vertexMass="1 1 1"', ""),
+    ("", None, '<PlaneForceField normal="0 1 0" d="-1" rayleighStiffness="0.5"/>'),
+    ("", None, '<PlaneForceField normal="0 1 0" d="-1" localRange="0 10"/>'),
+])
+def test_scene_reader_refuses_unrun_data(fem, mass, extra):
+    """Data values this path does not run are refused, not silently dropped: plasticity,
+    updateStiffnessMatrix, updateStiffness, the assembled global matrix, a component's own
+    Rayleigh coefficient, a UniformMass with no mass Data (invalid in the reference), a
+    DiagonalMass of per-vertex masses only, PlaneForceField localRange."""
+    from sofa_amd import scene_xml
+    text = _strict(fem=fem, mass=mass or 'UniformMass totalMass="10"', extra=extra)
+    with pytest.raises(ValueError):
+        scene_xml.load_scene(text)
+    # the same scene without the offending value reads
+    assert len(scene_xml.load_scene(_strict())[0]) == 1
+
+
+def test_scene_reader_mass_precedence():
+    """UniformMass: totalMass over vertexMass (UniformMass.inl:233-266); DiagonalMass: totalMass,
+    then massDensity, then the default totalMass 1 (DiagonalMass.inl:880-935)."""
+    from sofa_amd import scene_xml
+    cases = [('UniformMass vertexMass="2" totalMass="45"', ("uniform", 1.0)),   # 45 kg over 45 nodes
+             ('UniformMass vertexMass="2"', ("uniform", 2.0)),
+             ('DiagonalMass massDensity="3" totalMass="7"', ("diagonal_total", 7.0)),
+             ('DiagonalMass massDensity="3"', ("diagonal", 3.0)),
+             ("DiagonalMass", ("diagonal_total", 1.0))]
+    for mass, want in cases:
+        (spec,), _ = scene_xml.load_scene(_strict(mass=mass))
+        assert spec["mass"] == want, (mass, spec["mass"])
+
+
+def test_scene_reader_local_stiffness_factor():
+    """localStiffnessFactor: element i's Young modulus is factor[i M / N] times its own
+    (youngModulus entry i, or entry 0 past the list), as computeMaterialStiffness computes it
+    (TetrahedronFEMForceField.inl:256-261); checked element by element."""
+    from sofa_amd import scene_xml
+    (spec,), _ = scene_xml.load_scene(_strict(fem='localStiffnessFactor="1 0.5 4"'))
+    n = len(spec["tets"])
+    E = np.asarray(spec["E"])
+    assert E.shape == (n,)
+    base = [1e5, 2e5]
+    for i in range(n):
+        f = [1.0, 0.5, 4.0][i * 3 // n]
+        assert E[i] == f * (base[i] if i < 2 else base[0])
+    (plain,), _ = scene_xml.load_scene(_strict())
+    assert list(plain["E"]) == [1e5, 2e5]
+
+
+def test_scene_reader_shared_solver_refused():
+    """One EulerImplicitSolver over two bodies is one linear system in SOFA (one CG over both
+    mechanical objects); the reader refuses it rather than solving the bodies apart."""
+    from sofa_amd import scene_xml
+    second = """<Node name="beam2">
+    <RegularGridTopology name="grid2" n="3 3 5" min="2 0 0" max="3 1 2"/>
+    <MechanicalObject/>
+    <TetrahedronFEMForceField youngModulus="1e5" poissonRatio="0.3"/>
+    <UniformMass totalMass="10"/>
+    <FixedProjectiveConstraint indices="0 1 2"/>
+  </Node>"""
+    with pytest.raises(ValueError, match="governs bodies"):
+        scene_xml.load_scene(_strict(second=second))
+    # each body with its own solver reads as two scenes
+    text = """<Node name="root" dt="0.01" gravity="0 -9.81 0">
+  <Node name="beam"><EulerImplicitSolver/><CGLinearSolver/>
+    <RegularGridTopology name="grid" n="3 3 5" min="0 0 0" max="1 1 2"/>
+    <MechanicalObject/><TetrahedronFEMForceField youngModulus="1e5" poissonRatio="0.3"/>
+    <UniformMass totalMass="10"/><FixedProjectiveConstraint indices="0 1 2"/></Node>
+  <Node name="beam2"><EulerImplicitSolver/><CGLinearSolver/>
+    <RegularGridTopology name="grid2" n="3 3 5" min="2 0 0" max="3 1 2"/>
+    <MechanicalObject/><TetrahedronFEMForceField youngModulus="1e5" poissonRatio="0.3"/>
+    <UniformMass totalMass="10"/><FixedProjectiveConstraint indices="0 1 2"/></Node></Node>"""
+    specs, info = scene_xml.load_scene(text)
+    assert info["bodies"] == ["beam", "beam2"] and len(specs) == 2
