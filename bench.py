@@ -42,13 +42,18 @@ def log(msg):
 
 
 def bench_stream():
-    """A dedicated stream (default priority): library calls and the HIP events that time them
-    share this non-null stream handle (NULL would select the engine's own stream)."""
-    import torch
+    """A dedicated stream of the library's runtime (default priority): library calls and the HIP
+    events that time them share this stream."""
+    from sofa_amd import capi
     global _STREAM
     if _STREAM is None:
-        _STREAM = torch.cuda.Stream()
+        _STREAM = capi.Stream()
     return _STREAM
+
+
+def _events(n):
+    from sofa_amd import capi
+    return [capi.Event() for _ in range(n)]
 
 
 def make_engine(specs, precision, graphs=True):
@@ -62,17 +67,16 @@ def make_engine(specs, precision, graphs=True):
 
 def time_merged(eng, ids, steps, warmup):
     """Steps of all scenes together; per-step latency from events around each merged step."""
-    import torch
     stream = bench_stream()
-    sp = stream.cuda_stream
+    sp = stream
     eng.step(ids, warmup, stream=sp)
-    torch.cuda.synchronize()
-    ev = [torch.cuda.Event(enable_timing=True) for _ in range(steps + 1)]
+    stream.synchronize()
+    ev = _events(steps + 1)
     ev[0].record(stream)
     for i in range(steps):
         eng.step(ids, 1, stream=sp)
         ev[i + 1].record(stream)
-    torch.cuda.synchronize()
+    stream.synchronize()
     total_ms = ev[0].elapsed_time(ev[steps])
     lat = np.array([ev[i].elapsed_time(ev[i + 1]) for i in range(steps)])
     return total_ms, lat
@@ -81,15 +85,14 @@ def time_merged(eng, ids, steps, warmup):
 def time_plain(eng, ids, steps, warmup):
     """One instance at a time: each scene's step is its own launch sequence; per-step latency of
     a scene = events around that scene's step."""
-    import torch
     stream = bench_stream()
-    sp = stream.cuda_stream
+    sp = stream
     for _ in range(warmup):
         for s in ids:
             eng.step([s], 1, stream=sp)
-    torch.cuda.synchronize()
+    stream.synchronize()
     n = len(ids)
-    ev = [torch.cuda.Event(enable_timing=True) for _ in range(steps * n + 1)]
+    ev = _events(steps * n + 1)
     ev[0].record(stream)
     k = 0
     for _ in range(steps):
@@ -97,7 +100,7 @@ def time_plain(eng, ids, steps, warmup):
             eng.step([s], 1, stream=sp)
             k += 1
             ev[k].record(stream)
-    torch.cuda.synchronize()
+    stream.synchronize()
     total_ms = ev[0].elapsed_time(ev[k])
     lat = np.array([ev[i].elapsed_time(ev[i + 1]) for i in range(k)])
     return total_ms, lat
@@ -115,17 +118,15 @@ def apply_roofline(eng, reps, pe, pv, nrec, E, N):
                    materials; none for a scene of one material (each beam here), whose K the
                    kernel derives from the rest volume;
       per node: x, x0, r and p read, p and q written."""
-    import torch
     stream = bench_stream()
-    sp = stream.cuda_stream
+    sp = stream
     eng.bench_apply(-1, 3, stream=sp)
-    torch.cuda.synchronize()
-    e0 = torch.cuda.Event(enable_timing=True)
-    e1 = torch.cuda.Event(enable_timing=True)
+    stream.synchronize()
+    e0, e1 = _events(2)
     e0.record(stream)
     eng.bench_apply(-1, reps, stream=sp)
     e1.record(stream)
-    torch.cuda.synchronize()
+    stream.synchronize()
     us = e0.elapsed_time(e1) * 1e3 / reps
     bytes_alg = E * (16 + nrec * pe) + N * 18 * pv
     ach = bytes_alg / (us * 1e-6) / 1e9
@@ -201,22 +202,11 @@ def sofacuda_scenes(steps):
 
 
 def copy_probe(mb=512, reps=5):
-    """A plain device copy of known size (calibrates the HBM byte counters; STREAM-copy rate)."""
-    import torch
+    """A plain device copy kernel of known size (the library's k_copy; calibrates the HBM byte
+    counters; STREAM-copy rate)."""
+    from sofa_amd import capi
     nbytes = mb * (1 << 20)
-    a = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda").fill_(1.0)
-    b = torch.empty_like(a)
-    torch.cuda.synchronize()
-    stream = bench_stream()
-    e0 = torch.cuda.Event(enable_timing=True)
-    e1 = torch.cuda.Event(enable_timing=True)
-    with torch.cuda.stream(stream):
-        e0.record(stream)
-        for _ in range(reps):
-            b.copy_(a)
-        e1.record(stream)
-    torch.cuda.synchronize()
-    us = e0.elapsed_time(e1) * 1e3 / reps
+    us = capi.copy_bench(nbytes, reps, stream=bench_stream())
     return {"bytes": 2 * nbytes, "us": us, "GBs": 2 * nbytes / us / 1e3}
 
 
@@ -389,9 +379,7 @@ def main():
               "(independent jobs shard across GPUs with no communication; see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
 
-    import torch
     from sofa_amd import capi, workloads
-    torch.cuda.set_device(0)
 
     # ---- headline: cfg3 job mix, batched vs plain, FP32
     specs = workloads.cfg3()

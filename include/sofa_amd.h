@@ -338,6 +338,27 @@ int sofa_amd_cg_solve(sofa_amd_engine eng, int scene, double m_factor, double b_
  * which is what this gives (synchronous). */
 int sofa_amd_fem_potential_energy(sofa_amd_engine eng, int scene, const void* x_dev, double* energy);
 
+/* ---- runtime plumbing (no reference counterpart) -------------------------------------------
+ * Streams (default priority), timing events and device memory of the library's own HIP runtime,
+ * for callers without a device-array library: the Python driver, the tests and the benchmark use
+ * these, so that a process maps one HIP / HSA runtime only.  All synchronous but event_record. */
+int sofa_amd_stream_create(int device, void** stream_out);
+int sofa_amd_stream_destroy(void* stream);
+int sofa_amd_stream_sync(void* stream);
+int sofa_amd_device_sync(int device);
+int sofa_amd_event_create(void** event_out);
+int sofa_amd_event_destroy(void* event);
+int sofa_amd_event_record(void* event, void* stream);
+int sofa_amd_event_sync(void* event);
+int sofa_amd_event_elapsed(void* start, void* end, float* ms);
+int sofa_amd_malloc(int device, int64_t bytes, void** ptr_out);
+int sofa_amd_free(void* ptr);
+/* kind 0 host to device, 1 device to host, 2 device to device */
+int sofa_amd_memcpy(void* dst, const void* src, int64_t bytes, int kind);
+/* A plain copy kernel of `bytes` (a multiple of 16) timed over `reps` launches on `stream`: the
+ * device-copy rate and the calibration of the HBM byte counters. */
+int sofa_amd_copy_bench(int device, int64_t bytes, int reps, void* stream, float* us_per_copy);
+
 /* ---- measurement and introspection (no reference counterpart) ---------------------------- */
 
 /* `reps` launches of the CG operator kernel of one scene (scene >= 0) or of every scene of the

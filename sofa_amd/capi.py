@@ -33,7 +33,10 @@ EXPORTS = [
     "sofa_amd_scene_prepare", "sofa_amd_engine_autotune", "sofa_amd_engine_set_apply_variant",
     "sofa_amd_mass_add_mdx", "sofa_amd_mass_acc_from_f", "sofa_amd_mass_add_force", "sofa_amd_project_response",
     "sofa_amd_project_velocity", "sofa_amd_project_position", "sofa_amd_vdot", "sofa_amd_cg_solve",
-    "sofa_amd_fem_potential_energy",
+    "sofa_amd_fem_potential_energy", "sofa_amd_stream_create", "sofa_amd_stream_destroy", "sofa_amd_stream_sync",
+    "sofa_amd_device_sync", "sofa_amd_event_create", "sofa_amd_event_destroy", "sofa_amd_event_record",
+    "sofa_amd_event_sync", "sofa_amd_event_elapsed", "sofa_amd_malloc", "sofa_amd_free", "sofa_amd_memcpy",
+    "sofa_amd_copy_bench",
 ]
 
 
@@ -63,9 +66,12 @@ def lib():
     if not os.path.exists(LIB_PATH):
         raise ImportError(f"{LIB_PATH} is missing: build it with `python -m sofa_amd.build` "
                           "(or __graft_entry__.build()); there is no CPU fallback")
-    # torch first: the process then binds the HIP runtime torch was built for, which the library
-    # shares; loaded the other way round, torch finds no GPU in the same process
-    import torch  # noqa: F401
+    # The library binds the HIP runtime it was built with (RUNPATH /opt/rocm), and streams, events
+    # and device buffers come from it (Stream, Event, DeviceArray below): a process that uses this
+    # package maps one HIP / HSA runtime.  torch bundles another (its libamdhip64 needs an
+    # unversioned libhsa-runtime64.so, so a runtime already mapped under the versioned name -- by
+    # the library, or by rocprofv3's preloaded tool -- is not reused and a second HSA runtime
+    # comes up beside it), so nothing here imports torch.
     L = ctypes.CDLL(LIB_PATH)
     c_int, c_double = ctypes.c_int, ctypes.c_double
     sig = {
@@ -121,6 +127,19 @@ def lib():
         "sofa_amd_cg_solve": (c_int, [_vp, c_int, c_double, c_double, c_double, _vp, _vp,
                                       ctypes.POINTER(CGStats), _vp]),
         "sofa_amd_fem_potential_energy": (c_int, [_vp, c_int, _vp, _dp]),
+        "sofa_amd_stream_create": (c_int, [c_int, ctypes.POINTER(_vp)]),
+        "sofa_amd_stream_destroy": (c_int, [_vp]),
+        "sofa_amd_stream_sync": (c_int, [_vp]),
+        "sofa_amd_device_sync": (c_int, [c_int]),
+        "sofa_amd_event_create": (c_int, [ctypes.POINTER(_vp)]),
+        "sofa_amd_event_destroy": (c_int, [_vp]),
+        "sofa_amd_event_record": (c_int, [_vp, _vp]),
+        "sofa_amd_event_sync": (c_int, [_vp]),
+        "sofa_amd_event_elapsed": (c_int, [_vp, _vp, ctypes.POINTER(ctypes.c_float)]),
+        "sofa_amd_malloc": (c_int, [c_int, ctypes.c_int64, ctypes.POINTER(_vp)]),
+        "sofa_amd_free": (c_int, [_vp]),
+        "sofa_amd_memcpy": (c_int, [_vp, _vp, ctypes.c_int64, c_int]),
+        "sofa_amd_copy_bench": (c_int, [c_int, ctypes.c_int64, c_int, _vp, ctypes.POINTER(ctypes.c_float)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -145,6 +164,113 @@ def _d(a):
 def _i(a):
     a = np.ascontiguousarray(a, dtype=np.int32)
     return a, a.ctypes.data_as(_ip)
+
+
+def _sh(stream):
+    """A stream argument: a Stream, a raw handle, or None (the engine's own stream)."""
+    return stream.handle if isinstance(stream, Stream) else stream
+
+
+def device_available(device=0):
+    """True when `device` is a usable gfx950 GPU (the library's own runtime; no exception)."""
+    return lib().sofa_amd_device_check(int(device)) == OK
+
+
+def device_sync(device=0):
+    check(lib().sofa_amd_device_sync(int(device)))
+
+
+class Stream:
+    """A HIP stream of the library's runtime (default priority).  `handle` is what the engine
+    calls take as `stream`."""
+
+    def __init__(self, device=0):
+        h = _vp()
+        check(lib().sofa_amd_stream_create(int(device), ctypes.byref(h)))
+        self.handle = h.value
+
+    def synchronize(self):
+        check(lib().sofa_amd_stream_sync(self.handle))
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib().sofa_amd_stream_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Event:
+    """A timing event: record(stream), synchronize(), elapsed_time(end) in milliseconds."""
+
+    def __init__(self):
+        h = _vp()
+        check(lib().sofa_amd_event_create(ctypes.byref(h)))
+        self.handle = h.value
+
+    def record(self, stream=None):
+        check(lib().sofa_amd_event_record(self.handle, stream.handle if isinstance(stream, Stream) else stream))
+
+    def synchronize(self):
+        check(lib().sofa_amd_event_sync(self.handle))
+
+    def elapsed_time(self, end):
+        ms = ctypes.c_float()
+        check(lib().sofa_amd_event_elapsed(self.handle, end.handle, ctypes.byref(ms)))
+        return float(ms.value)
+
+    def __del__(self):
+        try:
+            if getattr(self, "handle", None):
+                lib().sofa_amd_event_destroy(self.handle)
+                self.handle = None
+        except Exception:
+            pass
+
+
+class DeviceArray:
+    """A device buffer of the library's runtime holding a numpy array's dtype and shape:
+    DeviceArray(np_array) uploads, .numpy() downloads, .ptr is the device address."""
+
+    def __init__(self, array=None, shape=None, dtype=np.float64, device=0):
+        if array is not None:
+            host = np.ascontiguousarray(array)
+        else:
+            host = np.zeros(shape, dtype=dtype)
+        self.shape, self.dtype = host.shape, host.dtype
+        self.nbytes = int(host.nbytes)
+        h = _vp()
+        check(lib().sofa_amd_malloc(int(device), self.nbytes, ctypes.byref(h)))
+        self.ptr = h.value
+        check(lib().sofa_amd_memcpy(self.ptr, host.ctypes.data, self.nbytes, 0))
+
+    def numpy(self):
+        out = np.empty(self.shape, dtype=self.dtype)
+        check(lib().sofa_amd_memcpy(out.ctypes.data, self.ptr, self.nbytes, 1))
+        return out
+
+    def close(self):
+        if getattr(self, "ptr", None):
+            lib().sofa_amd_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def copy_bench(nbytes, reps=5, stream=None, device=0):
+    """A plain device copy kernel of `nbytes` timed over `reps` launches: microseconds per copy."""
+    us = ctypes.c_float()
+    check(lib().sofa_amd_copy_bench(int(device), int(nbytes), int(reps),
+                                    stream.handle if isinstance(stream, Stream) else stream, ctypes.byref(us)))
+    return float(us.value)
 
 
 class Engine:
@@ -252,7 +378,7 @@ class Engine:
     def step(self, scenes=None, n_steps=1, stream=None):
         ids = np.arange(len(self.n_nodes), dtype=np.int32) if scenes is None else np.asarray(scenes, dtype=np.int32)
         a, ap = _i(ids)
-        check(lib().sofa_amd_engine_step(self._h, ap, len(a), int(n_steps), stream))
+        check(lib().sofa_amd_engine_step(self._h, ap, len(a), int(n_steps), _sh(stream)))
 
     def sync(self):
         check(lib().sofa_amd_engine_sync(self._h))
@@ -351,39 +477,39 @@ class Engine:
 
     # ---- component-level calls on device pointers ----
     def fem_add_force(self, scene, x_dev_ptr, f_dev_ptr, stream=None):
-        check(lib().sofa_amd_fem_add_force(self._h, int(scene), x_dev_ptr, f_dev_ptr, stream))
+        check(lib().sofa_amd_fem_add_force(self._h, int(scene), x_dev_ptr, f_dev_ptr, _sh(stream)))
 
     def fem_add_dforce(self, scene, dx_dev_ptr, df_dev_ptr, k_factor, stream=None):
-        check(lib().sofa_amd_fem_add_dforce(self._h, int(scene), dx_dev_ptr, df_dev_ptr, float(k_factor), stream))
+        check(lib().sofa_amd_fem_add_dforce(self._h, int(scene), dx_dev_ptr, df_dev_ptr, float(k_factor), _sh(stream)))
 
     # ---- component-level calls of the masses, constraints, state and CG on device pointers ----
     def mass_add_mdx(self, scene, dx_dev_ptr, f_dev_ptr, factor=1.0, stream=None):
-        check(lib().sofa_amd_mass_add_mdx(self._h, int(scene), dx_dev_ptr, f_dev_ptr, float(factor), stream))
+        check(lib().sofa_amd_mass_add_mdx(self._h, int(scene), dx_dev_ptr, f_dev_ptr, float(factor), _sh(stream)))
 
     def mass_acc_from_f(self, scene, f_dev_ptr, a_dev_ptr, stream=None):
-        check(lib().sofa_amd_mass_acc_from_f(self._h, int(scene), f_dev_ptr, a_dev_ptr, stream))
+        check(lib().sofa_amd_mass_acc_from_f(self._h, int(scene), f_dev_ptr, a_dev_ptr, _sh(stream)))
 
     def mass_add_force(self, scene, f_dev_ptr, stream=None):
-        check(lib().sofa_amd_mass_add_force(self._h, int(scene), f_dev_ptr, stream))
+        check(lib().sofa_amd_mass_add_force(self._h, int(scene), f_dev_ptr, _sh(stream)))
 
     def project_response(self, scene, dx_dev_ptr, stream=None):
-        check(lib().sofa_amd_project_response(self._h, int(scene), dx_dev_ptr, stream))
+        check(lib().sofa_amd_project_response(self._h, int(scene), dx_dev_ptr, _sh(stream)))
 
     def project_velocity(self, scene, v_dev_ptr, time, stream=None):
-        check(lib().sofa_amd_project_velocity(self._h, int(scene), v_dev_ptr, float(time), stream))
+        check(lib().sofa_amd_project_velocity(self._h, int(scene), v_dev_ptr, float(time), _sh(stream)))
 
     def project_position(self, scene, x_dev_ptr, time, stream=None):
-        check(lib().sofa_amd_project_position(self._h, int(scene), x_dev_ptr, float(time), stream))
+        check(lib().sofa_amd_project_position(self._h, int(scene), x_dev_ptr, float(time), _sh(stream)))
 
     def vdot(self, scene, a_dev_ptr, b_dev_ptr, order, result_dev_ptr, stream=None):
         """vDot into one double at result_dev_ptr (order 1: the reference's sequential sum)."""
-        check(lib().sofa_amd_vdot(self._h, int(scene), a_dev_ptr, b_dev_ptr, int(order), result_dev_ptr, stream))
+        check(lib().sofa_amd_vdot(self._h, int(scene), a_dev_ptr, b_dev_ptr, int(order), result_dev_ptr, _sh(stream)))
 
     def cg_solve(self, scene, m_factor, b_factor, k_factor, b_dev_ptr, x_dev_ptr, stream=None):
         """CGLinearSolver::solve of (m M + b B + k K) x = b; returns the statistics."""
         st = CGStats()
         check(lib().sofa_amd_cg_solve(self._h, int(scene), float(m_factor), float(b_factor), float(k_factor),
-                                      b_dev_ptr, x_dev_ptr, ctypes.byref(st), stream))
+                                      b_dev_ptr, x_dev_ptr, ctypes.byref(st), _sh(stream)))
         return {"iterations": st.iterations, "end_reason": st.end_reason, "step_count": st.step_count,
                 "err": st.err, "den": st.den}
 
@@ -394,7 +520,7 @@ class Engine:
 
     # ---- measurement ----
     def bench_apply(self, scene, reps, stream=None):
-        check(lib().sofa_amd_engine_bench_apply(self._h, int(scene), int(reps), stream))
+        check(lib().sofa_amd_engine_bench_apply(self._h, int(scene), int(reps), _sh(stream)))
 
     def tiling(self, scene):
         out = np.zeros(7, dtype=np.int32)

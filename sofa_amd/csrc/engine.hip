@@ -2456,6 +2456,151 @@ int sofa_amd_fem_potential_energy(sofa_amd_engine eng, int scene, const void* x_
     GUARD_END
 }
 
+// ---- runtime plumbing: streams, events and device memory of the library's HIP runtime, for
+// callers that have no device-array library of their own (the Python driver, tests, benchmark) ----
+int sofa_amd_stream_create(int device, void** out) {
+    GUARD_BEGIN
+    if (!out) return fail(SOFA_AMD_EINVAL, "null output");
+    HIPCHK(hipSetDevice(device));
+    hipStream_t s = nullptr;
+    HIPCHK(hipStreamCreate(&s));          // default priority
+    *out = (void*)s;
+    return SOFA_AMD_OK;
+    GUARD_END
+}
+
+int sofa_amd_stream_destroy(void* stream) {
+    GUARD_BEGIN
+    if (!stream) return fail(SOFA_AMD_EINVAL, "null stream");
+    HIPCHK(hipStreamSynchronize((hipStream_t)stream));
+    HIPCHK(hipStreamDestroy((hipStream_t)stream));
+    return SOFA_AMD_OK;
+    GUARD_END
+}
+
+int sofa_amd_stream_sync(void* stream) {
+    GUARD_BEGIN
+    HIPCHK(hipStreamSynchronize((hipStream_t)stream));
+    return SOFA_AMD_OK;
+    GUARD_END
+}
+
+int sofa_amd_device_sync(int device) {
+    GUARD_BEGIN
+    HIPCHK(hipSetDevice(device));
+    HIPCHK(hipDeviceSynchronize());
+    return SOFA_AMD_OK;
+    GUARD_END
+}
+
+int sofa_amd_event_create(void** out) {
+    GUARD_BEGIN
+    if (!out) return fail(SOFA_AMD_EINVAL, "null output");
+    hipEvent_t e = nullptr;
+    HIPCHK(hipEventCreate(&e));
+    *out = (void*)e;
+    return SOFA_AMD_OK;
+    GUARD_END
+}
+
+int sofa_amd_event_destroy(void* event) {
+    GUARD_BEGIN
+    if (!event) return fail(SOFA_AMD_EINVAL, "null event");
+    HIPCHK(hipEventDestroy((hipEvent_t)event));
+    return SOFA_AMD_OK;
+    GUARD_END
+}
+
+int sofa_amd_event_record(void* event, void* stream) {
+    GUARD_BEGIN
+    if (!event) return fail(SOFA_AMD_EINVAL, "null event");
+    HIPCHK(hipEventRecord((hipEvent_t)event, (hipStream_t)stream));
+    return SOFA_AMD_OK;
+    GUARD_END
+}
+
+int sofa_amd_event_sync(void* event) {
+    GUARD_BEGIN
+    if (!event) return fail(SOFA_AMD_EINVAL, "null event");
+    HIPCHK(hipEventSynchronize((hipEvent_t)event));
+    return SOFA_AMD_OK;
+    GUARD_END
+}
+
+int sofa_amd_event_elapsed(void* start, void* end, float* ms) {
+    GUARD_BEGIN
+    if (!start || !end || !ms) return fail(SOFA_AMD_EINVAL, "invalid argument");
+    HIPCHK(hipEventElapsedTime(ms, (hipEvent_t)start, (hipEvent_t)end));
+    return SOFA_AMD_OK;
+    GUARD_END
+}
+
+int sofa_amd_malloc(int device, int64_t bytes, void** out) {
+    GUARD_BEGIN
+    if (!out || bytes < 0) return fail(SOFA_AMD_EINVAL, "invalid argument");
+    HIPCHK(hipSetDevice(device));
+    void* p = nullptr;
+    if (hipMalloc(&p, (size_t)std::max<int64_t>(bytes, 1)) != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(SOFA_AMD_ENOMEM, "device allocation of " + std::to_string(bytes) + " bytes failed");
+    }
+    *out = p;
+    return SOFA_AMD_OK;
+    GUARD_END
+}
+
+int sofa_amd_free(void* ptr) {
+    GUARD_BEGIN
+    if (ptr) HIPCHK(hipFree(ptr));
+    return SOFA_AMD_OK;
+    GUARD_END
+}
+
+int sofa_amd_memcpy(void* dst, const void* src, int64_t bytes, int kind) {
+    GUARD_BEGIN
+    if ((!dst || !src) && bytes > 0) return fail(SOFA_AMD_EINVAL, "null pointer");
+    if (bytes < 0 || kind < 0 || kind > 2) return fail(SOFA_AMD_EINVAL, "invalid argument");
+    const hipMemcpyKind k = kind == 0 ? hipMemcpyHostToDevice : kind == 1 ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
+    if (bytes > 0) HIPCHK(hipMemcpy(dst, src, (size_t)bytes, k));
+    return SOFA_AMD_OK;
+    GUARD_END
+}
+
+// a plain device copy kernel: each thread moves 16-byte words, grid-strided
+static __global__ __launch_bounds__(256) void k_copy(const uint4* __restrict__ a, uint4* __restrict__ b, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) b[i] = a[i];
+}
+
+int sofa_amd_copy_bench(int device, int64_t bytes, int reps, void* stream, float* us_per_copy) {
+    GUARD_BEGIN
+    if (bytes <= 0 || bytes % 16 != 0 || reps < 1 || !us_per_copy) return fail(SOFA_AMD_EINVAL, "invalid argument");
+    HIPCHK(hipSetDevice(device));
+    const hipStream_t s = (hipStream_t)stream;
+    void *a = nullptr, *b = nullptr;
+    HIPCHK(hipMalloc(&a, (size_t)bytes));
+    HIPCHK(hipMalloc(&b, (size_t)bytes));
+    HIPCHK(hipMemset(a, 1, (size_t)bytes));
+    const int64_t n = bytes / 16;
+    const int grid = (int)std::min<int64_t>((n + 255) / 256, 256 * 64);
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    k_copy<<<grid, 256, 0, s>>>((const uint4*)a, (uint4*)b, n);
+    HIPCHK(hipEventRecord(e0, s));
+    for (int i = 0; i < reps; i++) k_copy<<<grid, 256, 0, s>>>((const uint4*)a, (uint4*)b, n);
+    HIPCHK(hipEventRecord(e1, s));
+    HIPCHK(hipEventSynchronize(e1));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    *us_per_copy = 1e3f * ms / (float)reps;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    HIPCHK(hipFree(a));
+    HIPCHK(hipFree(b));
+    return SOFA_AMD_OK;
+    GUARD_END
+}
+
 int sofa_amd_engine_bench_apply(sofa_amd_engine eng, int scene, int reps, void* stream) {
     GUARD_BEGIN
     if (!eng || !eng->impl || (scene >= 0 && !scene_of(eng, scene)) || reps < 0) return fail(SOFA_AMD_EINVAL, "invalid argument");

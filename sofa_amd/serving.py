@@ -72,16 +72,16 @@ class Server:
         called for each finished job before its scene is released.  `event()` makes the timing
         events (HIP events by default).  Returns the measurements."""
         if event is None:
-            import torch
-            if stream is None:
-                stream = torch.cuda.Stream()
-            sp = stream.cuda_stream
+            own = stream is None
+            if own:
+                stream = capi.Stream()
+            sp = stream
 
             def event():
-                return torch.cuda.Event(enable_timing=True)
+                return capi.Event()
 
             def sync_all():
-                torch.cuda.synchronize()
+                stream.synchronize()
         else:
             sp = None
 

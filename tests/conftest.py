@@ -14,5 +14,5 @@ def pytest_configure(config):
 
 @pytest.fixture(scope="session")
 def gpu_available():
-    import torch
-    return torch.cuda.is_available()
+    from sofa_amd import capi
+    return capi.device_available(0)

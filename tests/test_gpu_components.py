@@ -43,9 +43,9 @@ def _oracle(spec, dot_mode=0):
     return oracle_from_spec(spec, dot_mode)
 
 
-def _dev(a):
-    import torch
-    return torch.tensor(np.ascontiguousarray(a), dtype=torch.float64, device="cuda")
+def _dev(a, dtype=np.float64):
+    from sofa_amd import capi
+    return capi.DeviceArray(np.asarray(a, dtype=dtype))
 
 
 @pytest.mark.parametrize("kind", ["uniform", "diagonal", "meshmatrix", "lumped"])
@@ -53,7 +53,6 @@ def test_mass_and_projection_calls(kind):
     """addMDx (factor 1 and not), accFromF (refused for the consistent mass, as the reference
     does), addForce, projectResponse, projectVelocity (with activate_projectVelocity) and
     projectPosition at a given time, on the FP64 engine."""
-    import torch
     from sofa_amd import workloads
     capi = _capi()
     print(f"case: mass and projection calls, {kind}", flush=True)
@@ -69,41 +68,41 @@ def test_mass_and_projection_calls(kind):
     eng.commit()
     for factor in (1.0, -0.37):
         dxd, fd = _dev(dx), _dev(f0)
-        eng.mass_add_mdx(sid, dxd.data_ptr(), fd.data_ptr(), factor)
+        eng.mass_add_mdx(sid, dxd.ptr, fd.ptr, factor)
         eng.sync()
-        assert np.array_equal(fd.cpu().numpy(), o.mass_add_mdx(dx, f0, factor))
+        assert np.array_equal(fd.numpy(), o.mass_add_mdx(dx, f0, factor))
     fd, ad = _dev(f0), _dev(np.zeros((n, 3)))
     if kind == "meshmatrix":
         with pytest.raises(capi.SofaAmdError) as ei:
-            eng.mass_acc_from_f(sid, fd.data_ptr(), ad.data_ptr())
+            eng.mass_acc_from_f(sid, fd.ptr, ad.ptr)
         assert ei.value.code == capi.EINVAL
         with pytest.raises(ValueError):
             o.mass_acc_from_f(f0)
     else:
-        eng.mass_acc_from_f(sid, fd.data_ptr(), ad.data_ptr())
+        eng.mass_acc_from_f(sid, fd.ptr, ad.ptr)
         eng.sync()
         a_ref = o.mass_acc_from_f(f0)
-        assert np.array_equal(ad.cpu().numpy(), a_ref)
+        assert np.array_equal(ad.numpy(), a_ref)
         # closed form: M (M^-1 f) = f up to the division's rounding
         back = o.mass_add_mdx(a_ref, np.zeros((n, 3)), 1.0)
         assert np.allclose(back, f0, rtol=1e-14, atol=1e-14)
     fd = _dev(f0)
-    eng.mass_add_force(sid, fd.data_ptr())
+    eng.mass_add_force(sid, fd.ptr)
     eng.sync()
-    assert np.array_equal(fd.cpu().numpy(), o.mass_add_force(f0))
+    assert np.array_equal(fd.numpy(), o.mass_add_force(f0))
     # projective constraints
     vd = _dev(dx)
-    eng.project_response(sid, vd.data_ptr())
+    eng.project_response(sid, vd.ptr)
     eng.sync()
     pr = o.project(0, dx)
-    assert np.array_equal(vd.cpu().numpy(), pr)
+    assert np.array_equal(vd.numpy(), pr)
     assert np.all(pr[spec["fixed"]] == 0) and np.all(pr[spec["oscillators"]["index"]] == 0)
     t = 0.37
     for what, call in ((1, eng.project_velocity), (2, eng.project_position)):
         vd = _dev(dx)
-        call(sid, vd.data_ptr(), t)
+        call(sid, vd.ptr, t)
         eng.sync()
-        got, ref = vd.cpu().numpy(), o.project(what, dx, t)
+        got, ref = vd.numpy(), o.project(what, dx, t)
         osc = spec["oscillators"]["index"]
         rest = np.setdiff1d(np.arange(n), osc)
         assert np.array_equal(got[rest], ref[rest])
@@ -116,7 +115,6 @@ def test_vdot_orders(prec):
     """vDot of two caller vectors: order 1 equals the oracle's sequential sum bit for bit (FP64;
     FP32 vectors: the same sum of the float values' exact products), order 0 the oracle's blocked
     order; the scene (21,525 nodes) spans 84 blocks."""
-    import torch
     from sofa_amd import meshes
     capi = _capi()
     print(f"case: vdot {prec}", flush=True)
@@ -127,19 +125,18 @@ def test_vdot_orders(prec):
     eng.commit()
     rng = np.random.default_rng(11)
     a, b = rng.standard_normal((n, 3)), rng.standard_normal((n, 3))
-    dt = torch.float64 if prec == "FP64" else torch.float32
+    dt = np.float64 if prec == "FP64" else np.float32
     if prec == "FP32":
         a, b = a.astype(np.float32).astype(np.float64), b.astype(np.float32).astype(np.float64)
-    ad = torch.tensor(a, dtype=dt, device="cuda")
-    bd = torch.tensor(b, dtype=dt, device="cuda")
-    out = torch.zeros(2, dtype=torch.float64, device="cuda")
-    eng.vdot(sid, ad.data_ptr(), bd.data_ptr(), 1, out.data_ptr())
-    eng.vdot(sid, ad.data_ptr(), bd.data_ptr(), 0, out.data_ptr() + 8)
+    ad, bd = _dev(a, dt), _dev(b, dt)
+    out = _dev(np.zeros(2))
+    eng.vdot(sid, ad.ptr, bd.ptr, 1, out.ptr)
+    eng.vdot(sid, ad.ptr, bd.ptr, 0, out.ptr + 8)
     eng.sync()
     from oracle import OracleScene
     o = OracleScene(pos)
     seq, blk = o.vdot(a, b, 0), o.vdot(a, b, 1)
-    got = out.cpu().numpy()
+    got = out.numpy()
     assert got[0] == seq, (got[0], seq)
     assert got[1] == blk, (got[1], blk)
     assert abs(seq - float(np.sum(a * b))) <= 1e-12 * np.sum(np.abs(a * b))
@@ -152,7 +149,6 @@ def test_cg_solve_component(prec, order):
     x, the iteration count, the end reason and timeStepCount equal the oracle's solve of the same
     system (the Euler step's factors m = 1 + h rm, b = -h, k = -h (h + rs)) bit for bit; a second
     solve (timeStepCount 1: no first-solve exemption) as well."""
-    import torch
     from sofa_amd import workloads
     capi = _capi()
     print(f"case: cg solve {prec} order {order}", flush=True)
@@ -170,16 +166,16 @@ def test_cg_solve_component(prec, order):
     eng.commit()
     eng.write(sid, x=x)                      # the tiled operator recomputes rotations from these
     xd, fd = _dev(x), _dev(np.zeros((n, 3)))
-    eng.fem_add_force(sid, xd.data_ptr(), fd.data_ptr())
+    eng.fem_add_force(sid, xd.ptr, fd.ptr)
     h, rs, rm = spec["dt"], spec["rs"], spec["rm"]
     mf, bf, kf = 1 + h * rm, -h, -h * (h + rs)
     for k in range(2):
         b = o.project(0, rng.standard_normal((n, 3)))
         bd, sol = _dev(b), _dev(np.zeros((n, 3)))
-        st = eng.cg_solve(sid, mf, bf, kf, bd.data_ptr(), sol.data_ptr())
+        st = eng.cg_solve(sid, mf, bf, kf, bd.ptr, sol.ptr)
         ref = o.cg_solve(b, mf, bf, kf)
         so = o.stats()
-        got = sol.cpu().numpy()
+        got = sol.numpy()
         print(f"  solve {k}: {st['iterations']} iterations, end {st['end_reason']}, err {st['err']:.3e}", flush=True)
         assert np.array_equal(got, ref), float(np.max(np.abs(got - ref)))
         assert st["iterations"] == so["iterations"] and st["end_reason"] == so["endcond"]
@@ -189,7 +185,6 @@ def test_cg_solve_component(prec, order):
 
 def test_potential_energy_large_is_zero():
     """TetrahedronFEMForceField::getPotentialEnergy sums the small method only: 0 for large."""
-    import torch
     from sofa_amd import workloads
     capi = _capi()
     spec = workloads.cfg0()
@@ -197,5 +192,5 @@ def test_potential_energy_large_is_zero():
     sid = workloads.add_to_engine(eng, spec)
     eng.commit()
     xd = _dev(spec["pos"] * 1.01)
-    assert eng.fem_potential_energy(sid, xd.data_ptr()) == 0.0
+    assert eng.fem_potential_energy(sid, xd.ptr) == 0.0
     eng.close()

@@ -62,7 +62,6 @@ def rel_err(a, ref):
 
 def test_fp64_component_calls_bitexact():
     """addForce then addDForce on a deformed state: bit-exact vs the oracle."""
-    import torch
     capi = _capi()
     print("case: fp64 component calls, cfg0 mesh", flush=True)
     c = beam_cfg0()
@@ -79,16 +78,15 @@ def test_fp64_component_calls_bitexact():
     eng = capi.Engine(0, capi.FP64)
     sid = add_gpu_scene(eng, c)
     eng.commit()
-    xd = torch.tensor(x, dtype=torch.float64, device="cuda")
-    fd = torch.tensor(f0, dtype=torch.float64, device="cuda")
-    dxd = torch.tensor(dx, dtype=torch.float64, device="cuda")
-    dfd = torch.tensor(f0, dtype=torch.float64, device="cuda")
-    torch.cuda.synchronize()
-    eng.fem_add_force(sid, xd.data_ptr(), fd.data_ptr())
-    eng.fem_add_dforce(sid, dxd.data_ptr(), dfd.data_ptr(), k)
+    xd = capi.DeviceArray(np.asarray(x, dtype=np.float64))
+    fd = capi.DeviceArray(np.asarray(f0, dtype=np.float64))
+    dxd = capi.DeviceArray(np.asarray(dx, dtype=np.float64))
+    dfd = capi.DeviceArray(np.asarray(f0, dtype=np.float64))
+    eng.fem_add_force(sid, xd.ptr, fd.ptr)
+    eng.fem_add_dforce(sid, dxd.ptr, dfd.ptr, k)
     eng.sync()
-    f_gpu = fd.cpu().numpy()
-    df_gpu = dfd.cpu().numpy()
+    f_gpu = fd.numpy()
+    df_gpu = dfd.numpy()
     assert np.array_equal(f_gpu, f_ref), np.max(np.abs(f_gpu - f_ref))
     assert np.array_equal(df_gpu, df_ref), np.max(np.abs(df_gpu - df_ref))
     # the element rotations updated by addForce match as well
@@ -343,7 +341,6 @@ def test_tiled_batched_equals_alone(prec):
 def test_tiled_component_calls_and_element_map():
     """On a tiled engine the component calls still sum in the reference's element order
     (bit-exact in FP64) and element ids map back to the caller's order."""
-    import torch
     capi = _capi()
     print("case: fp64 tiled component calls", flush=True)
     c = beam_medium()
@@ -358,16 +355,15 @@ def test_tiled_component_calls_and_element_map():
     eng = capi.Engine(0, capi.FP64_TILED)
     sid = add_gpu_scene(eng, c)
     eng.commit()
-    xd = torch.tensor(x, dtype=torch.float64, device="cuda")
-    fd = torch.zeros((n, 3), dtype=torch.float64, device="cuda")
-    dxd = torch.tensor(dx, dtype=torch.float64, device="cuda")
-    dfd = torch.zeros((n, 3), dtype=torch.float64, device="cuda")
-    torch.cuda.synchronize()
-    eng.fem_add_force(sid, xd.data_ptr(), fd.data_ptr())
-    eng.fem_add_dforce(sid, dxd.data_ptr(), dfd.data_ptr(), k)
+    xd = capi.DeviceArray(np.asarray(x, dtype=np.float64))
+    fd = capi.DeviceArray(shape=(n, 3))
+    dxd = capi.DeviceArray(np.asarray(dx, dtype=np.float64))
+    dfd = capi.DeviceArray(shape=(n, 3))
+    eng.fem_add_force(sid, xd.ptr, fd.ptr)
+    eng.fem_add_dforce(sid, dxd.ptr, dfd.ptr, k)
     eng.sync()
-    assert np.array_equal(fd.cpu().numpy(), f_ref)
-    assert np.array_equal(dfd.cpu().numpy(), df_ref)
+    assert np.array_equal(fd.numpy(), f_ref)
+    assert np.array_equal(dfd.numpy(), df_ref)
     for e in (0, 1234, len(c["tets"]) - 1):
         rot, kk, rest = eng.element(sid, e)
         assert np.array_equal(rot, orc.fem_element(e)["R"])
@@ -1088,7 +1084,6 @@ def test_assembled_stiffness(prec):
     oracle's dense assembly -- bit for bit in FP64, FP64 tiled and MIXED; FP32 (float element
     record and positions) within 1e-5 of the largest entry.  Applied to dx the matrix gives
     addDForce's df."""
-    import torch
     capi = _capi()
     from oracle import OracleScene
     print(f"case: assembled stiffness, {prec}", flush=True)
@@ -1100,7 +1095,7 @@ def test_assembled_stiffness(prec):
     nu = rng.uniform(0.2, 0.45, ne)
     Q, _ = np.linalg.qr(rng.normal(size=(3, 3)))
     x = (pos + 0.05 * np.sin(3 * pos[:, [2, 0, 1]])) @ Q.T
-    vt = torch.float32 if prec == "FP32" else torch.float64
+    vt = np.float32 if prec == "FP32" else np.float64
     if prec == "FP32":
         x = x.astype(np.float32).astype(np.float64)
     orc = OracleScene(pos)
@@ -1114,10 +1109,9 @@ def test_assembled_stiffness(prec):
     eng.commit()
     try:
         eng.step([other], 1)
-        xd = torch.tensor(x, dtype=vt, device="cuda")
-        fd = torch.zeros((n, 3), dtype=vt, device="cuda")
-        torch.cuda.synchronize()
-        eng.fem_add_force(sid, xd.data_ptr(), fd.data_ptr())
+        xd = capi.DeviceArray(np.asarray(x, dtype=vt))
+        fd = capi.DeviceArray(shape=(n, 3), dtype=vt)
+        eng.fem_add_force(sid, xd.ptr, fd.ptr)
         eng.sync()
         rp, cols, blocks = eng.stiffness(sid, k)
         share = [set() for _ in range(n)]
