@@ -321,6 +321,7 @@ struct EngineBase {
     bool live_mode = false;           // reserved: scenes are committed and released one by one
     bool seq_dots = false;            // exact path: dot products in the reference's sequential order
     bool stepped = false;             // a step has been enqueued
+    int branches = 0;                 // concurrent branches of a merged step (0: the engine's default)
     double* vdot_part = nullptr;      // vDot's per-block partials (grown on demand)
     int64_t vdot_len = 0;
     virtual int autotune(int reps, hipStream_t s, float* us_out) = 0;
@@ -398,7 +399,7 @@ struct EngineT : EngineBase {
     uint2* ltet = nullptr;
     int *ln_gid = nullptr, *ln_inc_ptr = nullptr, *ln_pslot = nullptr, *sh_gid = nullptr, *sh_ptr = nullptr;
     int* sh_of = nullptr;                          // per node: index into sh_gid / sh_ptr, or -1
-    int* sh_ord = nullptr;                         // reference order: shared-node slots in element order
+    uint8_t* inc_rank = nullptr;                   // reference order: incidence ranks of shared nodes
     uint16_t* ln_inc = nullptr;
     VT *sp0 = nullptr, *sp1 = nullptr;
     double *pbb = nullptr, *pden = nullptr, *prho = nullptr;
@@ -438,8 +439,13 @@ struct EngineT : EngineBase {
     std::map<std::vector<int32_t>, std::unique_ptr<LaunchSet>> sets;
     LaunchSet* cur = nullptr;
 
-    ~EngineT() override {
+    ~EngineT() override {   // the device is synchronized before an engine is destroyed
         sets.clear();
+        for (hipStream_t a : aux)
+            if (a) (void)hipStreamDestroy(a);
+        for (hipEvent_t e : join_ev)
+            if (e) (void)hipEventDestroy(e);
+        if (fork_ev) (void)hipEventDestroy(fork_ev);
         while (!vmd.empty()) free_vm(vmd.begin()->first);
         while (!kd.empty()) free_k(kd.begin()->first);
         free_device();
@@ -449,7 +455,7 @@ struct EngineT : EngineBase {
                         inst, partial, tiles_d, ranges_d, ltet, ln_gid, ln_inc_ptr, ln_pslot, sh_gid, sh_ptr,
                         ln_inc, sp0, sp1, pbb, pden, prho, p2, em_ptr, em_nbr, em_val, emass, osc_node,
                         osc_mean, osc_amp, osc_w, osc_phase, kmat32, emass32, r32, p32, q32,
-                        d32, sp32, pden32, prho32, prr, inst_in, sh_of, nd, x0, x32, x0_32, contact, sh_ord};
+                        d32, sp32, pden32, prho32, prr, inst_in, sh_of, nd, x0, x32, x0_32, contact, inc_rank};
         for (void* ptr : ptrs)
             if (ptr) (void)hipFree(ptr);
     }
@@ -856,7 +862,7 @@ struct EngineT : EngineBase {
         inst = nullptr; partial = nullptr; tiles_d = nullptr; ranges_d = nullptr; ltet = nullptr;
         ln_gid = ln_inc_ptr = ln_pslot = sh_gid = sh_ptr = nullptr; ln_inc = nullptr; sp0 = sp1 = nullptr;
         pbb = pden = prho = nullptr; kmat32 = emass32 = nullptr; x32 = x0_32 = nullptr;
-        r32 = p32 = q32 = d32 = sp32 = nullptr; pden32 = prho32 = prr = nullptr; inst_in = nullptr; sh_of = nullptr;
+        r32 = p32 = q32 = d32 = sp32 = nullptr; pden32 = prho32 = prr = nullptr; inst_in = nullptr; sh_of = nullptr; inc_rank = nullptr;
         cap = c;
         if (cap.node >= (int64_t)1 << 30 || cap.inc >= (int64_t)1 << 31 || cap.lnode >= (int64_t)1 << 31 ||
             cap.slot >= (int64_t)kSlotOwner)
@@ -871,7 +877,7 @@ struct EngineT : EngineBase {
         mass = dalloc<double>(N); fixed = dalloc<int8_t>(N); inc_ptr = dalloc<int>(N); sh_of = dalloc<int>(N);
         nd = dalloc<double>(N);
         HIPCHK(hipMemset(nd, 0, N * sizeof(double)));
-        sh_ord = dalloc<int>(std::max<int64_t>(1, cap.slot));
+        inc_rank = dalloc<uint8_t>(std::max<int64_t>(1, cap.inc));
         contact = dalloc<int8_t>(N);
         HIPCHK(hipMemset(contact, 0, std::max<int64_t>(1, N)));
         tet = dalloc<int4>(E); rot = dalloc<R>(9 * E); rest = dalloc<R>(NR * E); kmat = dalloc<R>(3 * E);
@@ -1015,12 +1021,7 @@ struct EngineT : EngineBase {
         sg[Sn] = nb0;
         for (int j = 0; j <= Sn; j++) spp[j] = (int)pl.at.slot + th.sh_ptr[j];
         put_at(sh_gid, pl.at.shn, sg); put_at(sh_ptr, pl.at.shn, spp);
-        if (exact_tiled()) {
-            // the node's slots in element order; the list shares the slot arena's range
-            std::vector<int> ordv(P.sh_ord);
-            for (int& o : ordv) o += (int)pl.at.slot;
-            put_at(sh_ord, pl.at.slot, ordv);
-        }
+        if (exact_tiled()) put_at(inc_rank, ib0, P.inc_rank);   // slot offsets of shared nodes' incidences
         std::vector<int> of(n + 1, -1);
         for (int j = 0; j < Sn; j++) of[th.sh_gid[j]] = (int)pl.at.shn + j;
         put_at(sh_of, nb0, of);
@@ -1115,6 +1116,7 @@ struct EngineT : EngineBase {
     void release_scene(int si) override {
         HIPCHK(hipDeviceSynchronize());   // nothing in flight reads the ranges any more
         sets.clear();                     // launch sets holding the scene's blocks
+        branch_plans.clear();
         cur = nullptr;
         release_ranges(place[si]);
         free_vm(si);
@@ -1148,6 +1150,7 @@ struct EngineT : EngineBase {
         if (sets.size() >= 4096) {       // bounded cache: drop every set (nothing is in flight after the sync)
             HIPCHK(hipDeviceSynchronize());
             sets.clear();
+            branch_plans.clear();
         }
         std::unique_ptr<LaunchSet> ls(new LaunchSet());
         int maxit = 0;
@@ -1240,7 +1243,7 @@ struct EngineT : EngineBase {
         const TileArgs<R, NR> ta = tile_args();
         const FinBlk* fb = (const FinBlk*)cur->b;
         const NodeBlk* nb = (const NodeBlk*)cur->c;
-        const SharedNodes sn{sh_gid, sh_ptr, sh_of, sh_ord};
+        const SharedNodes sn{sh_gid, sh_ptr, sh_of};
         if (cur->na > 0) k_tile_rhs<R, NR, VT, 512, true><<<cur->na, 512, 0, s>>>(ta, x, v, f, b, r, xcg, sp0, sp1, pbb);
         if (cur->nb > 0)
             k_fin_rhs<VT, true><<<cur->nb, kBlock, 0, s>>>(fb, inst, ranges_d, sn, mass, fixed, v, sp0, sp1, f, b, r, xcg,
@@ -1345,7 +1348,7 @@ struct EngineT : EngineBase {
         ta.ranges = ranges_d;
         ta.ev = view();
         ta.ltet = ltet;
-        ta.tn = TileNodes{ln_gid, ln_inc_ptr, ln_inc, ln_pslot};
+        ta.tn = TileNodes{ln_gid, ln_inc_ptr, ln_inc, ln_pslot, inc_rank};
         ta.mass = mass;
         ta.fixed = fixed;
         ta.contact = contact;
@@ -1359,7 +1362,7 @@ struct EngineT : EngineBase {
         const TileArgs<R, NR> ta = tile_args();
         const FinBlk* fb = (const FinBlk*)cur->b;
         const NodeBlk* nb = (const NodeBlk*)cur->c;
-        const SharedNodes sn{sh_gid, sh_ptr, sh_of, sh_ord};
+        const SharedNodes sn{sh_gid, sh_ptr, sh_of};
         if constexpr (std::is_same<R, double>::value) {
             if (exact_tiled()) { launch_step_exact(n, s); return; }
         }
@@ -1383,7 +1386,7 @@ struct EngineT : EngineBase {
         ta.x_lo = x32;                 // the inner operator recomputes the rotations from these
         const FinBlk* fb = (const FinBlk*)cur->b;
         const NodeBlk* nb = (const NodeBlk*)cur->c;
-        const SharedNodes sn{sh_gid, sh_ptr, sh_of, nullptr};
+        const SharedNodes sn{sh_gid, sh_ptr, sh_of};
         InnerCg ic;
         ic.ta.tiles = ta.tiles;
         ic.ta.inst = inst_in;
@@ -1434,9 +1437,64 @@ struct EngineT : EngineBase {
             }
     }
 
+    // a merged step in concurrent branches: the scenes are split into G groups, each stepped by its
+    // own launch sequence on its own stream (forked from and joined back into `s` by events), so
+    // that one group's latency-bound phases (the reference order's node-after-node dot products:
+    // one wave per scene, one dependent add per node) overlap the other groups' element and node
+    // kernels.  Scenes are dealt to the groups by node count, largest first, round-robin: each
+    // group holds one of the G longest dot-product chains and about 1/G of the work.
+    static constexpr int kMaxBranches = 8;
+    std::map<std::vector<int32_t>, std::vector<std::vector<int32_t>>> branch_plans;
+    hipStream_t aux[kMaxBranches - 1] = {};
+    hipEvent_t fork_ev = nullptr, join_ev[kMaxBranches - 1] = {};
+    int branch_count(int n) const {
+        // two by default: HIP gives a process 4 hardware queues (GPU_MAX_HW_QUEUES), two of which
+        // the runtime's own streams hold, and branches sharing a queue run one after the other
+        const int g = branches > 0 ? branches : (exact_tiled() ? 2 : 1);
+        return std::max(1, std::min({g, kMaxBranches, n / 2}));
+    }
+    void step_branches(const int32_t* ids, int n, int nsteps, hipStream_t s, int G) {
+        std::vector<int32_t> key(ids, ids + n);
+        key.push_back(-G);
+        auto it = branch_plans.find(key);
+        if (it == branch_plans.end()) {
+            std::vector<int32_t> order(ids, ids + n);
+            std::stable_sort(order.begin(), order.end(),
+                             [&](int32_t a, int32_t b) { return scenes[a].n_nodes > scenes[b].n_nodes; });
+            std::vector<std::vector<int32_t>> gr(G);
+            for (int k = 0; k < n; k++) gr[k % G].push_back(order[k]);
+            for (auto& g : gr) std::sort(g.begin(), g.end());
+            it = branch_plans.emplace(key, std::move(gr)).first;
+        }
+        if (!fork_ev) HIPCHK(hipEventCreateWithFlags(&fork_ev, hipEventDisableTiming));
+        for (int g = 1; g < G; g++) {
+            if (!aux[g - 1]) HIPCHK(hipStreamCreateWithFlags(&aux[g - 1], hipStreamNonBlocking));
+            if (!join_ev[g - 1]) HIPCHK(hipEventCreateWithFlags(&join_ev[g - 1], hipEventDisableTiming));
+        }
+        for (int st = 0; st < nsteps; st++) {
+            HIPCHK(hipEventRecord(fork_ev, s));
+            for (int g = 1; g < G; g++) HIPCHK(hipStreamWaitEvent(aux[g - 1], fork_ev, 0));
+            for (int g = 0; g < G; g++) {
+                const std::vector<int32_t>& grp = it->second[g];
+                step_set(grp.data(), (int)grp.size(), 1, g == 0 ? s : aux[g - 1]);
+            }
+            for (int g = 1; g < G; g++) {
+                HIPCHK(hipEventRecord(join_ev[g - 1], aux[g - 1]));
+                HIPCHK(hipStreamWaitEvent(s, join_ev[g - 1], 0));
+            }
+        }
+    }
+
     void step(const int32_t* ids, int n, int nsteps, hipStream_t s) override {
-        prepare_set(ids, n, s);
         if (nsteps > 0) stepped = true;
+        const int G = branch_count(n);
+        if (G > 1) step_branches(ids, n, nsteps, s, G);
+        else step_set(ids, n, nsteps, s);
+    }
+
+    // `nsteps` steps of one launch set (the listed scenes in one launch per phase) on `s`
+    void step_set(const int32_t* ids, int n, int nsteps, hipStream_t s) {
+        prepare_set(ids, n, s);
         if (!use_graphs) {
             for (int st = 0; st < nsteps; st++) launch_step(n, s);
             HIPCHK(hipGetLastError());
@@ -1683,7 +1741,7 @@ struct EngineT : EngineBase {
             const TileArgs<R, NR> ta = tile_args();
             const FinBlk* fb = (const FinBlk*)cur->b;
             const NodeBlk* nbk = (const NodeBlk*)cur->c;
-            const SharedNodes sn{sh_gid, sh_ptr, sh_of, sh_ord};
+            const SharedNodes sn{sh_gid, sh_ptr, sh_of};
             bool exact = false;
             if constexpr (std::is_same<R, double>::value) exact = exact_tiled();
             if (exact) {
@@ -2735,6 +2793,17 @@ int sofa_amd_engine_set_apply_variant(sofa_amd_engine eng, int variant) {
     if (!eng || !eng->impl) return fail(SOFA_AMD_EINVAL, "null engine");
     HIPCHK(hipSetDevice(eng->impl->device));
     eng->impl->set_apply_variant(variant);
+    return SOFA_AMD_OK;
+    GUARD_END
+}
+
+int sofa_amd_engine_set_branches(sofa_amd_engine eng, int branches) {
+    GUARD_BEGIN
+    if (!eng || !eng->impl) return fail(SOFA_AMD_EINVAL, "null engine");
+    if (branches < 0 || branches > 8) return fail(SOFA_AMD_EINVAL, "branches must be 0 (default) to 8");
+    HIPCHK(hipSetDevice(eng->impl->device));
+    HIPCHK(hipDeviceSynchronize());
+    eng->impl->branches = branches;
     return SOFA_AMD_OK;
     GUARD_END
 }

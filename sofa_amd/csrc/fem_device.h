@@ -84,16 +84,35 @@ constexpr int kDotBB = 0, kDotPQ = 1, kDotRR = 2;
 
 // MechanicalObject::vDot in the reference's order (MechanicalObject.inl:2334-2356): the scene's
 // per-node terms nd[node_begin + i], i = 0 .. n - 1 (each already the node's three-term dot
-// product), added one after the other from 0.0.  One wave per scene of the launch set: the wave
-// loads 512 terms per round (coalesced, one round ahead), parks them in LDS, and every lane runs
-// the same chain of dependent adds over them (broadcast LDS reads), so a scene costs about one
-// FP64 add latency per node.  The last round is padded with +0.0, which leaves the sum unchanged:
-// s + 0.0 == s for every s but -0.0, and a sum that starts at +0.0 is never -0.0 under
+// product), added one after the other from 0.0.  One wave per scene of the launch set.  The terms
+// pass through LDS in chunks of kSeqChunk: the wave loads the next chunk (coalesced) while it sums
+// the current one, and every lane runs the same chain of dependent adds over the chunk, reading
+// it from LDS (broadcast) in register groups of kSeqGroup terms, the next group's reads issued
+// before the current group's adds, so that the chain waits on the FP64 add latency only (about
+// one add per 6 cycles) and not on LDS.  The last chunk is padded with +0.0, which leaves the sum
+// unchanged: s + 0.0 == s for every s but -0.0, and a sum that starts at +0.0 is never -0.0 under
 // round-to-nearest.  The scene's scalar lands in InstDev::dots[which]; a stopped scene is skipped.
-constexpr int kSeqChunk = 512;
+constexpr int kSeqChunk = 1024, kSeqGroup = 16;
+__device__ __forceinline__ void seq_group_load(const double* c, double g[kSeqGroup]) {
+    const double2* c2 = reinterpret_cast<const double2*>(c);
+#pragma unroll
+    for (int k = 0; k < kSeqGroup / 2; k++) {
+        const double2 t = c2[k];
+        g[2 * k] = t.x;
+        g[2 * k + 1] = t.y;
+    }
+}
+// the compiler may not move instructions across this point: the next group's LDS reads, issued
+// before it, stay ahead of the current group's adds after it
+__device__ __forceinline__ void seq_fence() { __builtin_amdgcn_sched_barrier(0); }
+__device__ __forceinline__ double seq_group_add(double s, const double g[kSeqGroup]) {
+#pragma unroll
+    for (int k = 0; k < kSeqGroup; k++) s = s + g[k];
+    return s;
+}
 static __global__ __launch_bounds__(64) void k_seqdot(const int* __restrict__ scenes, InstDev* __restrict__ inst,
                                                        const double* __restrict__ nd, int which) {
-    __shared__ double buf[2][kSeqChunk];
+    __shared__ __align__(16) double buf[2][kSeqChunk];
     InstDev* ins = &inst[scenes[blockIdx.x]];
     if (ins->stopped) return;
     const int b0 = ins->node_begin, n = ins->node_count;
@@ -118,8 +137,21 @@ static __global__ __launch_bounds__(64) void k_seqdot(const int* __restrict__ sc
         }
         __syncthreads();
         const double* c = buf[round];
-#pragma unroll 64
-        for (int k = 0; k < kSeqChunk; k++) s = s + c[k];
+        double ga[kSeqGroup], gb[kSeqGroup];
+        seq_group_load(c, ga);
+#pragma unroll
+        for (int g = 0; g < kSeqChunk; g += 2 * kSeqGroup) {
+            seq_group_load(c + g + kSeqGroup, gb);
+            seq_fence();
+            s = seq_group_add(s, ga);
+            seq_fence();
+            // the next pair's first group (after the last pair: the chunk's first, unused)
+            seq_group_load(c + ((g + 2 * kSeqGroup) & (kSeqChunk - 1)), ga);
+            seq_fence();
+            s = seq_group_add(s, gb);
+            seq_fence();
+        }
+        __syncthreads();   // the buffer of this round is rewritten two rounds later
     }
     if (lane == 0) ins->dots[which] = s;
 }

@@ -59,7 +59,10 @@ struct TileNodes {
     const int* __restrict__ inc_ptr;    // incidence range (absolute, into inc)
     const uint16_t* __restrict__ inc;   // local element slot * 4 + vertex, ascending element id
     const int* __restrict__ pslot;      // -1: interior; else shared-partial slot of (node, tile),
-                                        // | kSlotOwner in the node's first tile
+                                        // | kSlotOwner in the node's first tile (reference order:
+                                        // the node's first per-incidence slot)
+    const uint8_t* __restrict__ rank;   // reference order: per incidence of a shared node, its
+                                        // slot's offset from the node's first slot (else null)
 };
 constexpr int kSlotOwner = 1 << 30;
 
@@ -67,8 +70,6 @@ struct SharedNodes {
     const int* __restrict__ gid;        // shared (and element-free) nodes
     const int* __restrict__ ptr;        // their partial slots [ptr[i], ptr[i+1]) in tile order
     const int* __restrict__ of;         // per node: its index in gid / ptr, or -1 (tile-owned)
-    const int* __restrict__ ord;        // reference order: [ptr[i], ptr[i+1]) indexes this list of
-                                        // per-incidence slots in ascending element id (else null)
 };
 
 // per-scene slot ranges of the two partial buffers
@@ -240,13 +241,14 @@ __device__ __forceinline__ void sum_incidences(const uint16_t* linc, int s0, int
 }
 
 // reference order: each incidence's contribution lc[slot + c kTileElems] to its own global slot
-// base, base + 1, ... (the node's finalize adds them in ascending element id)
+// base + rank (the node's slots run in ascending element id, the order its finalize adds them);
+// rank: the tile's incidence ranks, indexed like linc
 template <typename R, typename VT>
-__device__ __forceinline__ void store_incidences(const uint16_t* linc, int s0, int s1, const R* lc, VT* __restrict__ sp,
-                                                 int base) {
+__device__ __forceinline__ void store_incidences(const uint16_t* linc, const uint8_t* __restrict__ rank, int s0,
+                                                 int s1, const R* lc, VT* __restrict__ sp, int base) {
     for (int s = s0; s < s1; s++) {
         const int sl = linc[s];
-        VT* o = sp + 3 * (int64_t)(base + (s - s0));
+        VT* o = sp + 3 * (int64_t)(base + rank[s]);
 #pragma unroll
         for (int i = 0; i < 3; i++) o[i] = (VT)lc[sl + i * kTileElems];
     }
@@ -349,7 +351,7 @@ __global__ __launch_bounds__(TB) void k_tile_rhs(TileArgs<R, NR> a, const VT* __
             if (psl[m] < 0) mass_gravity(in, a.mass, gid[m], fn[m]);
             else { fn[m][0] = 0; fn[m][1] = 0; fn[m][2] = 0; }
             const int s0 = lptr[j], s1 = lptr[j + 1];
-            if (EXACT && psl[m] >= 0) store_incidences(linc, s0, s1, lc, spf, psl[m]);
+            if (EXACT && psl[m] >= 0) store_incidences(linc, a.tn.rank + td.inc_begin, s0, s1, lc, spf, psl[m]);
             else sum_incidences<false>(linc, s0, s1, lc, fn[m]);
         }
     }
@@ -412,7 +414,7 @@ __global__ __launch_bounds__(TB) void k_tile_rhs(TileArgs<R, NR> a, const VT* __
                 if (EXACT) a.nd[g] = dot3d(bn, bn);
                 else bb += dot3d(bn, bn);
             } else if (EXACT) {
-                store_incidences(linc, s0, s1, lc, spv, psl[m]);
+                store_incidences(linc, a.tn.rank + td.inc_begin, s0, s1, lc, spv, psl[m]);
             } else {
                 const int ps = psl[m] & (kSlotOwner - 1);
                 VT pv[3] = {0, 0, 0};
@@ -449,7 +451,7 @@ __global__ __launch_bounds__(kBlock) void k_fin_rhs(const FinBlk* __restrict__ f
         mass_gravity(in, mass, g, fn);
         const int s0 = sn.ptr[j], s1 = sn.ptr[j + 1];
         for (int s = s0; s < s1; s++) {
-            const int64_t o = 3 * (int64_t)(EXACT ? sn.ord[s] : s);
+            const int64_t o = 3 * (int64_t)s;
 #pragma unroll
             for (int i = 0; i < 3; i++) fn[i] += spf[o + i];
         }
@@ -465,7 +467,7 @@ __global__ __launch_bounds__(kBlock) void k_fin_rhs(const FinBlk* __restrict__ f
         VT bn[3] = {fn[0], fn[1], fn[2]};
         if (in.m_rhs != 0.0) mass_mdx(in, NodeArrays{nullptr, nullptr, mass, fixed}, g, in.m_rhs, vn, bn);
         for (int s = s0; s < s1; s++) {
-            const int64_t o = 3 * (int64_t)(EXACT ? sn.ord[s] : s);
+            const int64_t o = 3 * (int64_t)s;
 #pragma unroll
             for (int i = 0; i < 3; i++) bn[i] -= spv[o + i];
         }
@@ -706,7 +708,7 @@ __global__ __launch_bounds__(TB) void k_tile_apply(TileArgs<R, NR> a, int iter, 
                 if (EXACT) a.nd[gid[m]] = dot3d(pp, qn);
                 else pq += dot3d(pp, qn);
             } else if (EXACT) {
-                store_incidences(linc, s0, s1, lc, sp, psl[m]);
+                store_incidences(linc, a.tn.rank + td.inc_begin, s0, s1, lc, sp, psl[m]);
             } else {
                 VT acc[3] = {0, 0, 0};
                 sum_incidences<true>(linc, s0, s1, lc, acc);
@@ -764,7 +766,7 @@ __global__ __launch_bounds__(kBlock) void k_fin_exact(const FinBlk* __restrict__
     if (in.m_cg != 0.0) mass_mdx(in, NodeArrays{nullptr, nullptr, mass, fixed}, g, in.m_cg, pp, qn);
     const int s0 = sn.ptr[j], s1 = sn.ptr[j + 1];
     for (int s = s0; s < s1; s++) {
-        const int64_t o = 3 * (int64_t)sn.ord[s];
+        const int64_t o = 3 * (int64_t)s;
 #pragma unroll
         for (int i = 0; i < 3; i++) qn[i] -= sp[o + i];
     }

@@ -32,7 +32,8 @@ struct TilePlan {
     std::vector<int> ln_pslot;      // -1 interior, else scene-local shared-partial slot
     std::vector<int> sh_gid;        // nodes finished by the finalize kernel (ascending)
     std::vector<int> sh_ptr;        // size(sh_gid) + 1
-    std::vector<int> sh_ord;        // reference order only (plan_exact): slots in element order
+    std::vector<uint8_t> inc_rank;  // reference order only (plan_exact): per incidence of a shared
+                                    // node, its rank in the node's list in ascending element id
     int n_slots = 0;
 };
 
@@ -236,41 +237,43 @@ static inline TilePlan plan_tiles(int n_nodes, const double* rest, const int32_t
 // The reference's summation order for the nodes shared by several tiles.  A per-tile partial
 // sum changes the association of a node's element contributions ((m + c1 + c2) + (c3 + c4)
 // instead of m + c1 + c2 + c3 + c4), so here every incidence of a shared node gets a slot of its
-// own: a tile writes its element's contribution to the node there (its incidences of the node
-// at ln_pslot, ln_pslot + 1, ... in its local list order), and the node's slots are listed in
-// ascending ORIGINAL element id in sh_ord[sh_ptr[k] .. sh_ptr[k + 1]), the order in which the
-// finalize kernel adds them after the mass term -- TetrahedronFEMForceField's element loop
-// (TetrahedronFEMForceField.inl:1630-1634).  Interior nodes keep their in-tile sums, which are
-// already in that order.
+// own.  A shared node's slots are consecutive, in ascending ORIGINAL element id, at
+// sh_ptr[k] .. sh_ptr[k + 1]: the order in which the finalize kernel adds them after the mass
+// term -- TetrahedronFEMForceField's element loop (TetrahedronFEMForceField.inl:1630-1634) -- so
+// that it reads each node's contributions as one contiguous run.  A tile writes its element's
+// contribution to the node at ln_pslot (the node's first slot) + inc_rank of the incidence.
+// Interior nodes keep their in-tile sums, which are already in that order.
 static inline void plan_exact(TilePlan& P, int n_nodes, int max_elems) {
     const int TT = (int)P.t_ebegin.size();
     std::vector<int> shidx(n_nodes, -1);
     for (size_t j = 0; j < P.sh_gid.size(); j++) shidx[P.sh_gid[j]] = (int)j;
-    std::vector<std::vector<std::pair<int, int>>> lists(P.sh_gid.size());   // (original element, slot)
-    int next = 0;
+    std::vector<std::vector<std::pair<int, int>>> lists(P.sh_gid.size());   // (original element, incidence)
     for (int ti = 0; ti < TT; ti++) {
         for (int j = 0; j < P.t_lncount[ti]; j++) {
             const int li = P.t_lnbegin[ti] + j;
             const int k = shidx[P.ln_gid[li]];
             if (k < 0) continue;
-            const int c0 = P.ln_inc_ptr[li], c1 = P.ln_inc_ptr[li + 1];
-            P.ln_pslot[li] = next;
-            for (int c = c0; c < c1; c++) {
+            for (int c = P.ln_inc_ptr[li]; c < P.ln_inc_ptr[li + 1]; c++) {
                 const int local_elem = (int)P.ln_inc[c] % max_elems;   // slot = 3 max_elems v + element
-                lists[k].push_back({P.perm[P.t_ebegin[ti] + local_elem], next + (c - c0)});
+                lists[k].push_back({P.perm[P.t_ebegin[ti] + local_elem], c});
             }
-            next += c1 - c0;
         }
     }
-    P.sh_ord.clear();
-    P.sh_ord.reserve(next);
+    P.inc_rank.assign(P.ln_inc.size(), 0);
     P.sh_ptr.assign(1, 0);
     for (auto& lst : lists) {
+        if (lst.size() > 256) throw std::invalid_argument("a node with more than 256 tetrahedra");
         std::sort(lst.begin(), lst.end());
-        for (auto& pr : lst) P.sh_ord.push_back(pr.second);
-        P.sh_ptr.push_back((int)P.sh_ord.size());
+        for (size_t r = 0; r < lst.size(); r++) P.inc_rank[lst[r].second] = (uint8_t)r;
+        P.sh_ptr.push_back(P.sh_ptr.back() + (int)lst.size());
     }
-    P.n_slots = next;
+    for (int ti = 0; ti < TT; ti++)
+        for (int j = 0; j < P.t_lncount[ti]; j++) {
+            const int li = P.t_lnbegin[ti] + j;
+            const int k = shidx[P.ln_gid[li]];
+            if (k >= 0) P.ln_pslot[li] = P.sh_ptr[k];
+        }
+    P.n_slots = P.sh_ptr.back();
 }
 
 }  // namespace sofa_amd

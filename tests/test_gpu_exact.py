@@ -48,7 +48,7 @@ def _specs(which):
     return [mix[i] for i in MIX_BEAMS], 100
 
 
-def _run(prec, specs, nsteps, reference_order=False, graphs=True):
+def _run(prec, specs, nsteps, reference_order=False, graphs=True, branches=0):
     from sofa_amd import workloads
     capi = _capi()
     eng = capi.Engine(0, getattr(capi, prec))
@@ -57,6 +57,7 @@ def _run(prec, specs, nsteps, reference_order=False, graphs=True):
     ids = [workloads.add_to_engine(eng, s) for s in specs]
     eng.commit()
     eng.set_graphs(graphs)
+    eng.set_branches(branches)
     eng.step(ids, nsteps)
     out = [eng.read(i)[:2] + (eng.stats(i),) for i in ids]
     eng.close()
@@ -101,6 +102,20 @@ def test_exact_tiled_eager_equals_graph():
     b = _run("FP64_TILED", specs[:3], 20, reference_order=True, graphs=True)
     for (xa, va, _), (xb, vb, _) in zip(a, b):
         assert np.array_equal(xa, xb) and np.array_equal(va, vb)
+
+
+@pytest.mark.parametrize("graphs", [False, True])
+def test_exact_tiled_branches(graphs):
+    """A merged step in concurrent branches (scenes split over streams) equals the one-branch step
+    bit for bit, eager and replayed from graphs: scenes are independent."""
+    specs, _ = _specs("mix8")
+    print(f"case: exact tiled, 8 beams, branches 1 / 3 / 4, graphs {graphs}", flush=True)
+    one = _run("FP64_TILED", specs, 15, reference_order=True, graphs=graphs, branches=1)
+    for br in (3, 4):
+        got = _run("FP64_TILED", specs, 15, reference_order=True, graphs=graphs, branches=br)
+        for (xa, va, sa), (xb, vb, sb) in zip(one, got):
+            assert np.array_equal(xa, xb) and np.array_equal(va, vb)
+            assert sa["iterations"] == sb["iterations"]
 
 
 def test_exact_tiled_refuses_consistent_mass():
