@@ -332,6 +332,16 @@ int sofa_amd_project_position(sofa_amd_engine eng, int scene, void* x_dev, doubl
  * allocates on the engine and grows to the largest scene asked for. */
 int sofa_amd_vdot(sofa_amd_engine eng, int scene, const void* a_dev, const void* b_dev, int order,
                   double* result_dev, void* stream);
+/* sofa_amd_vdot without any device allocation: no reference counterpart.  For callers that
+ * cannot allocate device memory (code under graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  The same kernels and result as sofa_amd_vdot, bit for bit, but the order-0
+ * block partials go to a file-scope device array of the library (512 KiB per GPU, held until the
+ * process ends) instead of a buffer the library allocates on the engine: scenes of up to
+ * 16,777,216 nodes (SOFA_AMD_EINVAL beyond).  Every call on one GPU shares that GPU's array, so
+ * calls that overlap on one GPU (other streams, other host threads, other engines) race; calls
+ * ordered on one stream do not.  Enqueued on `stream` (NULL = the engine's stream). */
+int sofa_amd_vdot_noalloc(sofa_amd_engine eng, int scene, const void* a_dev, const void* b_dev, int order,
+                          double* result_dev, void* stream);
 /* CGLinearSolver::solve (CGLinearSolver.h:68, CGLinearSolver.inl:71-315) of
  * (m_factor M + b_factor B + k_factor K) x = b on the scene, through MatrixLinearSolver::solveSystem
  * (MatrixLinearSolver.inl:218-233) and GraphScatteredMatrix::apply (GraphScatteredTypes.cpp:33-45):

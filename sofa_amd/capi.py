@@ -32,7 +32,7 @@ EXPORTS = [
     "sofa_amd_engine_usage", "sofa_amd_scene_set_fix_all", "sofa_amd_scene_set_integrator", "sofa_amd_scene_set_fem_hexa", "sofa_amd_scene_add_plane", "sofa_amd_scene_set_project_velocity", "sofa_amd_engine_set_dot_order",
     "sofa_amd_scene_prepare", "sofa_amd_engine_autotune", "sofa_amd_engine_set_apply_variant",
     "sofa_amd_mass_add_mdx", "sofa_amd_mass_acc_from_f", "sofa_amd_mass_add_force", "sofa_amd_project_response",
-    "sofa_amd_project_velocity", "sofa_amd_project_position", "sofa_amd_vdot", "sofa_amd_cg_solve",
+    "sofa_amd_project_velocity", "sofa_amd_project_position", "sofa_amd_vdot", "sofa_amd_vdot_noalloc", "sofa_amd_cg_solve",
     "sofa_amd_fem_potential_energy", "sofa_amd_stream_create", "sofa_amd_stream_destroy", "sofa_amd_stream_sync",
     "sofa_amd_device_sync", "sofa_amd_event_create", "sofa_amd_event_destroy", "sofa_amd_event_record",
     "sofa_amd_event_sync", "sofa_amd_event_elapsed", "sofa_amd_malloc", "sofa_amd_free", "sofa_amd_memcpy",
@@ -125,6 +125,7 @@ def lib():
         "sofa_amd_project_velocity": (c_int, [_vp, c_int, _vp, c_double, _vp]),
         "sofa_amd_project_position": (c_int, [_vp, c_int, _vp, c_double, _vp]),
         "sofa_amd_vdot": (c_int, [_vp, c_int, _vp, _vp, c_int, _vp, _vp]),
+        "sofa_amd_vdot_noalloc": (c_int, [_vp, c_int, _vp, _vp, c_int, _vp, _vp]),
         "sofa_amd_cg_solve": (c_int, [_vp, c_int, c_double, c_double, c_double, _vp, _vp,
                                       ctypes.POINTER(CGStats), _vp]),
         "sofa_amd_fem_potential_energy": (c_int, [_vp, c_int, _vp, _dp]),
@@ -509,6 +510,11 @@ class Engine:
     def vdot(self, scene, a_dev_ptr, b_dev_ptr, order, result_dev_ptr, stream=None):
         """vDot into one double at result_dev_ptr (order 1: the reference's sequential sum)."""
         check(lib().sofa_amd_vdot(self._h, int(scene), a_dev_ptr, b_dev_ptr, int(order), result_dev_ptr, _sh(stream)))
+
+    def vdot_noalloc(self, scene, a_dev_ptr, b_dev_ptr, order, result_dev_ptr, stream=None):
+        """vDot with the block partials in the library's static device array (no allocation)."""
+        check(lib().sofa_amd_vdot_noalloc(self._h, int(scene), a_dev_ptr, b_dev_ptr, int(order), result_dev_ptr,
+                                          _sh(stream)))
 
     def cg_solve(self, scene, m_factor, b_factor, k_factor, b_dev_ptr, x_dev_ptr, stream=None):
         """CGLinearSolver::solve of (m M + b B + k K) x = b; returns the statistics."""

@@ -2493,6 +2493,56 @@ int sofa_amd_vdot(sofa_amd_engine eng, int scene, const void* a_dev, const void*
     GUARD_END
 }
 
+}  // extern "C"
+
+// ---- the allocation-free vDot: partial sums in a file-scope device array ---------------------
+// kVdotScratchMax doubles (512 KiB per GPU, held until the process ends): the block partials of
+// a scene of up to kVdotScratchMax * kBlock = 16,777,216 nodes.  In host code the array's name is
+// not a device address: each device's copy is reached through the address hipGetSymbolAddress
+// returns with that device current, taken once per device on its first call and kept.
+constexpr int64_t kVdotScratchMax = 65536;
+static __device__ __align__(256) double scratch[kVdotScratchMax];
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static double* g_scratch_ptr[kMaxDevices] = {};
+
+// the device's copy of `scratch` (the device must be current)
+static double* scratch_on(int device) {
+    if (device < 0 || device >= kMaxDevices) throw std::invalid_argument("device index out of range");
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            throw HipError(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_ptr[device] = (double*)ptr;
+    }
+    return g_scratch_ptr[device];
+}
+
+extern "C" {
+
+int sofa_amd_vdot_noalloc(sofa_amd_engine eng, int scene, const void* a_dev, const void* b_dev, int order,
+                          double* result_dev, void* stream) {
+    GUARD_BEGIN
+    if (!scene_of(eng, scene) || !a_dev || !b_dev || !result_dev || order < 0 || order > 1)
+        return fail(SOFA_AMD_EINVAL, "invalid argument");
+    if (!usable(eng, scene)) return fail(SOFA_AMD_ESTATE, "scene not committed");
+    EngineBase* e = eng->impl.get();
+    HIPCHK(hipSetDevice(e->device));
+    const hipStream_t s = stream ? (hipStream_t)stream : e->stream;
+    const int64_t n = e->scene_nodes(scene);
+    const int64_t need = (n + kBlock - 1) / kBlock;
+    if (need > kVdotScratchMax)
+        return fail(SOFA_AMD_EINVAL, "vDot without allocation: the scene has " + std::to_string(n) +
+                                         " nodes, more than the " + std::to_string(kVdotScratchMax * kBlock) +
+                                         " its static partials buffer covers");
+    double* part = scratch_on(e->device);
+    e->vdot(scene, a_dev, b_dev, order, result_dev, part, kVdotScratchMax, s);
+    return SOFA_AMD_OK;
+    GUARD_END
+}
+
 int sofa_amd_cg_solve(sofa_amd_engine eng, int scene, double m_factor, double b_factor, double k_factor,
                       const void* b_dev, void* x_dev, sofa_amd_cg_stats* stats, void* stream) {
     GUARD_BEGIN
