@@ -363,39 +363,74 @@ def serve_cfg4(seconds, arrivals, budget_gb):
     return out
 
 
-def cpu_baseline(specs, steps, stride):
-    """The FP64 oracle (C++ restatement of the reference path) on this host, over a bounded
-    sample: every `stride`-th beam of the mix, `steps` step(s) each, timed with the element loops
-    on all usable cores (at most 16, the host share of one GPU) and on one thread (the
-    reference's own loops are serial); both give the same bits."""
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _oracle_sample(specs, threads, warmup, steps, runs):
+    """The oracle on `specs`, `threads` threads: set-up untimed, `warmup` steps untimed, then
+    `runs` runs of `steps` steps of every scene; element-steps/s of the median run."""
     from oracle import OracleScene
-    threads = max(1, min(16, len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()))
-    res = {}
-    for nt in (threads, 1):
-        elem_steps, secs = 0, 0.0
-        for s in specs[::stride]:
-            o = OracleScene(s["pos"])
-            o.set_gravity(s["g"])
-            o.set_fem(s["tets"], np.atleast_1d(s["E"]), np.atleast_1d(s["nu"]), "large")
-            o.set_uniform_mass(vertex_mass=s["mass"][1])
-            o.set_fixed(s["fixed"])
-            o.set_solver(s["rs"], s["rm"], s["iters"], s["tol"], s["thr"])
-            o.set_threads(nt)
+    scenes = []
+    for s in specs:
+        o = OracleScene(s["pos"])
+        o.set_gravity(s["g"])
+        o.set_fem(s["tets"], np.atleast_1d(s["E"]), np.atleast_1d(s["nu"]), "large")
+        o.set_uniform_mass(vertex_mass=s["mass"][1])
+        o.set_fixed(s["fixed"])
+        o.set_solver(s["rs"], s["rm"], s["iters"], s["tol"], s["thr"])
+        o.set_threads(threads)
+        o.step(s["dt"], warmup)
+        scenes.append((o, s["dt"]))
+    secs = []
+    for _ in range(runs):
+        t = 0.0
+        for o, dt in scenes:
             t0 = time.perf_counter()
-            o.step(s["dt"], steps)
-            secs += time.perf_counter() - t0
-            elem_steps += len(s["tets"]) * steps
-        res[nt] = (elem_steps, secs)
-    n = len(specs[::stride])
-    es, secs = res[threads]
-    es1, secs1 = res[1]
-    return {"value": es / secs, "unit": "element-steps/s", "cores": threads, "kind": "port",
-            "sample": f"{n} of the 256 beams (every {stride}th), {steps} step(s) each, FP64, CG 25: {es} element-steps "
-                      f"in {secs:.2f} s on {threads} threads (element loops parallel, per-node sums in element order); "
-                      f"one thread: {es1 / secs1:.4g} element-steps/s ({secs1:.2f} s); oracle/fem_oracle.cpp "
-                      "(-O2 -ffp-contract=off)",
-            "single_thread_value": es1 / secs1}
+            o.step(dt, steps)
+            t += time.perf_counter() - t0
+        secs.append(t)
+    med = float(np.median(secs))
+    es = sum(len(s["tets"]) for s in specs) * steps
+    return es / med, med, secs
+
+
+def cpu_baseline(specs, steps, stride, stride1):
+    """The FP64 oracle (C++ restatement of the reference path, built here with -O3 -march=native
+    -ffp-contract=off for this host) on the host cores, per the committed protocol: scene set-up
+    untimed, 1 warm-up step, then 3 runs of `steps` steps, the median run reported.  Samples: every
+    `stride`-th beam of the mix on all usable cores (at most 16, the host share of one GPU; the
+    element loops parallel, per-node sums in element order, the same bits as serial), every
+    `stride1`-th beam on one thread (the reference's own loops are serial), and the single
+    1.31M-tet beam on all usable cores (5 steps per run)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    oracle.use_native_build()
+    from sofa_amd import workloads
+    threads = max(1, min(16, len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()))
+    mix = specs[::stride]
+    v, med, runs = _oracle_sample(mix, threads, 1, steps, 3)
+    mix1 = specs[::stride1]
+    v1, med1, runs1 = _oracle_sample(mix1, 1, 1, steps, 3)
+    c1 = workloads.cfg1()
+    vc1, medc1, runsc1 = _oracle_sample([c1], threads, 1, 5, 3)
+    cpu = _cpu_model()
+    return {"value": v, "unit": "element-steps/s", "cores": threads, "kind": "port", "cpu_model": cpu,
+            "sample": f"{len(mix)} of the 256 beams (every {stride}th, {sum(len(s['tets']) for s in mix)} tets), "
+                      f"1 warm-up step, median of 3 runs of {steps} steps ({', '.join(f'{x:.2f}' for x in runs)} s), "
+                      f"FP64, CG 25, {threads} threads on {cpu}; oracle/fem_oracle.cpp built -O3 -march=native "
+                      "-ffp-contract=off",
+            "single_thread": {"value": v1, "sample": f"{len(mix1)} beams (every {stride1}th), median of 3 runs of "
+                                                      f"{steps} steps ({', '.join(f'{x:.2f}' for x in runs1)} s)"},
+            "cfg1_single_beam": {"value": vc1, "threads": threads,
+                                 "sample": "1.31M-tet beam, CG 50, 1 warm-up step, median of 3 runs of 5 steps "
+                                           f"({', '.join(f'{x:.2f}' for x in runsc1)} s)"}}
 
 
 def main():
@@ -406,8 +441,9 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=3)
     ap.add_argument("--apply-reps", type=int, default=50)
     ap.add_argument("--error-steps", type=int, default=100)
-    ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--cpu-stride", type=int, default=4)
+    ap.add_argument("--cpu-stride-single", type=int, default=16)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--paths", default=",".join(p[0] for p in MIX_PATHS),
                     help="precision paths timed on the job mix (comma separated)")
@@ -532,7 +568,7 @@ def main():
     scn = None if args.no_sofacuda else sofacuda_scenes(args.sofacuda_steps)
     log(f"sofacuda scenes: {scn}")
 
-    cpu = None if args.no_cpu_baseline else cpu_baseline(specs, args.cpu_steps, args.cpu_stride)
+    cpu = None if args.no_cpu_baseline else cpu_baseline(specs, args.cpu_steps, args.cpu_stride, args.cpu_stride_single)
     log(f"cpu baseline: {cpu}")
 
     dtypes = {"fp64_exact": "f64", "fp64_tiled": "f64", "fp64_parity": "f64", "split": "f32/f64", "fp32": "f32"}

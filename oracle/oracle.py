@@ -23,6 +23,16 @@ def build():
     subprocess.check_call(["make", "-s", "-C", _HERE, "liboracle.so"])
 
 
+def use_native_build():
+    """Load the -O3 -march=native build (made here, for this host) instead of the parity build:
+    the CPU-baseline timing leg calls this before anything else touches the oracle."""
+    global _LIB_PATH
+    if _lib is not None:
+        raise RuntimeError("the oracle library is already loaded")
+    subprocess.check_call(["make", "-s", "-C", _HERE, "liboracle_native.so"])
+    _LIB_PATH = os.path.join(_HERE, "liboracle_native.so")
+
+
 def lib():
     global _lib
     if _lib is None:

@@ -171,6 +171,52 @@ def test_serving_scheduler_plain_and_refusal():
     assert all(len(t) == 1 for t in eng.log)
 
 
+def test_serving_close_waits_for_preparations():
+    """Server.close() while host preparations (tiling, element set-up) are still running on the
+    worker threads: the engine they read is closed only after every running preparation has
+    returned, and queued ones are dropped (they never start after the close)."""
+    import threading
+    import time
+    from sofa_amd import capi, serving
+
+    class SlowPrepareEngine(_FakeEngine):
+        def __init__(self):
+            super().__init__(10 ** 6, 10 ** 7)
+            self.running = 0
+            self.started = 0
+            self.closed = False
+            self.prepared_after_close = 0
+            self.lock = threading.Lock()
+            self.gate = threading.Event()
+
+        def prepare_scene(self, sid):
+            with self.lock:
+                if self.closed:
+                    self.prepared_after_close += 1
+                self.running += 1
+                self.started += 1
+            self.gate.wait(2.0)
+            time.sleep(0.05)
+            with self.lock:
+                self.running -= 1
+
+        def close(self):
+            with self.lock:
+                assert self.running == 0, "engine closed while a preparation reads it"
+                self.closed = True
+
+    eng = SlowPrepareEngine()
+    srv = serving.Server(capi.FP32, budget_bytes=1 << 40, engine=eng, build_workers=2)
+    futs = [srv.pool.submit(eng.prepare_scene, k) for k in range(8)]
+    while eng.started < 2:                       # two preparations in flight, six queued
+        time.sleep(0.001)
+    threading.Timer(0.05, eng.gate.set).start()
+    srv.close()
+    assert eng.closed and eng.running == 0
+    assert eng.prepared_after_close == 0
+    assert sum(f.cancelled() for f in futs) >= 1
+
+
 def test_fem_hexa_tessellation():
     """TetrahedronFEMForceField::init's 6-tetrahedra tessellation of a grid (cube corners mirrored
     by cube parity): conforming (every interior triangle in exactly two tetrahedra, every boundary
