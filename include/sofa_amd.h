@@ -122,6 +122,18 @@ int sofa_amd_scene_add_oscillators(sofa_amd_engine eng, int scene, int n, const 
                                    const double* mean, const double* amplitude, const double* pulsation,
                                    const double* phase);
 
+/* LinearMovementProjectiveConstraint<Vec3d> (LinearMovementProjectiveConstraint.inl:40-330): the
+ * `n` nodes `indices` follow the piecewise-linear movements given at `n_keys` key times (in
+ * chronological order; movements: 3 per key) while the scene time t (the step's start) lies in
+ * [keyTimes[k], keyTimes[k+1]) for some k: their responses are cleared in the solve, and after
+ * the position update x = x0 + m(t) (relative = 1; x0 = their positions at the first projection)
+ * or x = m(t), v = (m[k+1] - m[k]) / (keyTimes[k+1] - keyTimes[k]), with
+ * m(t) = m[k] + (m[k+1] - m[k]) (t - keyTimes[k]) / (keyTimes[k+1] - keyTimes[k]).  Outside the
+ * window (and at the last key time itself) the nodes are free.  Applied after the fixed and
+ * oscillator constraints; one per scene. */
+int sofa_amd_scene_add_linear_movement(sofa_amd_engine eng, int scene, int n, const int32_t* indices, int n_keys,
+                                       const double* key_times, const double* movements, int relative);
+
 /* Context gravity (Node::setGravity), used by the masses' addForce. */
 int sofa_amd_scene_set_gravity(sofa_amd_engine eng, int scene, double gx, double gy, double gz);
 

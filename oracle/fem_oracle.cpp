@@ -928,6 +928,70 @@ struct PlaneForceField : public ForceField {
 // ------------------------------------------------------------------------------------------
 struct Oscillator { int index; Vec3 mean, amplitude; Real pulsation, phase; };
 
+// LinearMovementProjectiveConstraint<Vec3d> (LinearMovementProjectiveConstraint.inl:40-330):
+// indices, keyTimes, movements, relativeMovements; x0 captured at the first projectPosition
+struct LinearMovement {
+    std::vector<int> indices;
+    std::vector<Real> keyTimes;
+    std::vector<Vec3> movements;
+    bool relative = true;
+    std::vector<Vec3> x0;
+    bool x0set = false;
+    // findKeyTimes (:296-330) at context time cT: true when cT lies in [keyTimes[k],
+    // keyTimes[k+1]) for some k ("finished" with nextT != prevT); prev / next the bracketing keys
+    bool window(Real cT, Real& prevT, Real& nextT, Vec3& prevM, Vec3& nextM) const {
+        bool finished = false;
+        if (!keyTimes.empty() && cT >= keyTimes.front() && cT <= keyTimes.back()) {
+            nextT = keyTimes.front();
+            prevT = nextT;
+            for (size_t k = 0; k < keyTimes.size() && !finished; k++) {
+                if (keyTimes[k] <= cT) {
+                    prevT = keyTimes[k];
+                    prevM = movements[k];
+                } else {
+                    nextT = keyTimes[k];
+                    nextM = movements[k];
+                    finished = true;
+                }
+            }
+        }
+        return finished && nextT != prevT;
+    }
+    // projectResponseT (:163-183): clear the constrained responses inside a movement window
+    void projectResponse(std::vector<Vec3>& r, Real cT) const {
+        Real pT = 0, nT = 0;
+        Vec3 pM, nM;
+        if (!window(cT, pT, nT, pM, nM)) return;
+        for (int i : indices) r[i] = Vec3();
+    }
+    // projectVelocity (:193-214): v = (nextM - prevM) * (1 / (nextT - prevT))
+    void projectVelocity(std::vector<Vec3>& vv, Real cT) const {
+        Real pT = 0, nT = 0;
+        Vec3 pM, nM;
+        if (!window(cT, pT, nT, pM, nM)) return;
+        const Real inv = 1.0 / (nT - pT);
+        for (int i : indices)
+            for (int k = 0; k < 3; k++) vv[i][k] = (nM[k] - pM[k]) * inv;
+    }
+    // projectPosition (:218-244) + interpolatePosition (:248-270): x0 captured on the first call;
+    // m = prevM + (nextM - prevM) * ((cT - prevT) / (nextT - prevT)); x = x0 + m (relative) or m
+    void projectPosition(std::vector<Vec3>& xx, Real cT) {
+        if (!x0set) {
+            x0.assign(xx.size(), Vec3());
+            for (int i : indices) x0[i] = xx[i];
+            x0set = true;
+        }
+        Real pT = 0, nT = 0;
+        Vec3 pM, nM;
+        if (!window(cT, pT, nT, pM, nM)) return;
+        const Real dt = (cT - pT) / (nT - pT);
+        Vec3 m;
+        for (int k = 0; k < 3; k++) m[k] = pM[k] + (nM[k] - pM[k]) * dt;
+        for (int i : indices)
+            for (int k = 0; k < 3; k++) xx[i][k] = relative ? x0[i][k] + m[k] : m[k];
+    }
+};
+
 struct Scene {
     std::vector<Vec3> x, v, f, x0;
     std::unique_ptr<MassBase> mass;
@@ -937,6 +1001,7 @@ struct Scene {
     bool fixAll = false;
     bool projectVelocity = false;   // FixedProjectiveConstraint activate_projectVelocity (default false)
     std::vector<Oscillator> oscillators;
+    std::vector<LinearMovement> movements;     // after the oscillators in scene order
     Vec3 gravity;
     Real rayleighStiffness = 0, rayleighMass = 0;
     // EulerImplicitSolver Data (EulerImplicitSolver.cpp:40-51): firstOrder, trapezoidalScheme,
@@ -1006,6 +1071,7 @@ struct Scene {
         if (fixAll) { for (auto& e : r) e = Vec3(); }
         else for (int i : fixed) r[i] = Vec3();
         for (const auto& o : oscillators) r[o.index] = Vec3();
+        for (const auto& lm : movements) lm.projectResponse(r, time);
     }
 
     // projectVelocity at context time t: FixedProjectiveConstraint (.inl:237-258, when
@@ -1017,11 +1083,13 @@ struct Scene {
         }
         for (const auto& o : oscillators)
             for (int k = 0; k < 3; k++) vv[o.index][k] = o.amplitude[k] * o.pulsation * std::cos(o.pulsation * t + o.phase);
+        for (const auto& lm : movements) lm.projectVelocity(vv, t);
     }
     // projectPosition at context time t: Fixed is a no-op (.inl:262-265); Oscillator (:89-106)
     void projectPositionAt(std::vector<Vec3>& xx, Real t) {
         for (const auto& o : oscillators)
             for (int k = 0; k < 3; k++) xx[o.index][k] = o.mean[k] + o.amplitude[k] * std::sin(o.pulsation * t + o.phase);
+        for (auto& lm : movements) lm.projectPosition(xx, t);
     }
 
     // GraphScatteredMatrix::apply (GraphScatteredTypes.cpp:33-45)
@@ -1158,6 +1226,12 @@ struct Scene {
                 v[o.index][k] = o.amplitude[k] * o.pulsation * std::cos(o.pulsation * t + o.phase);
             }
         }
+        // LinearMovementProjectiveConstraint: projectPosition, then projectVelocity
+        // (MechanicalProjectPositionAndVelocityVisitor.cpp:35-40)
+        for (auto& lm : movements) {
+            lm.projectPosition(x, time);
+            lm.projectVelocity(v, time);
+        }
         time += dt;
     }
 };
@@ -1256,6 +1330,16 @@ void orc_set_fixed(void* h, const int* idx, int n, int fix_all) {
     s->fixAll = fix_all != 0;
 }
 
+void orc_add_linear_movement(void* h, const int* idx, int n, const double* key_times, const double* moves, int nk,
+                             int relative) {
+    Scene* s = static_cast<Scene*>(h);
+    LinearMovement lm;
+    lm.indices.assign(idx, idx + n);
+    lm.keyTimes.assign(key_times, key_times + nk);
+    for (int k = 0; k < nk; k++) lm.movements.push_back(Vec3(moves[3 * k], moves[3 * k + 1], moves[3 * k + 2]));
+    lm.relative = relative != 0;
+    s->movements.push_back(lm);
+}
 void orc_set_project_velocity(void* h, int on) { static_cast<Scene*>(h)->projectVelocity = on != 0; }
 
 void orc_add_oscillator(void* h, int index, const double* mean, const double* amplitude, double pulsation, double phase) {

@@ -57,6 +57,7 @@ def lib():
         L.orc_set_dot_mode.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.orc_set_integrator.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_double]
         L.orc_set_project_velocity.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.orc_add_linear_movement.argtypes = [ctypes.c_void_p, _ip, ctypes.c_int, _dp, _dp, ctypes.c_int, ctypes.c_int]
         L.orc_set_threads.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.orc_step.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_int]
         L.orc_get_vec.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp]
@@ -178,6 +179,15 @@ class OracleScene:
     def set_project_velocity(self, on=True):
         """FixedProjectiveConstraint activate_projectVelocity."""
         lib().orc_set_project_velocity(self._h, 1 if on else 0)
+
+    def add_linear_movement(self, indices, key_times, movements, relative=True):
+        """LinearMovementProjectiveConstraint: indices, keyTimes, movements (one 3-vector per key),
+        relativeMovements."""
+        ia, ip = _i(indices)
+        ta, tp = _d(key_times)
+        ma, mp = _d(np.asarray(movements, dtype=np.float64).reshape(-1, 3))
+        assert len(ma) == len(ta)
+        lib().orc_add_linear_movement(self._h, ip, len(ia), tp, mp, len(ta), 1 if relative else 0)
 
     def set_dot_mode(self, mode):
         lib().orc_set_dot_mode(self._h, int(mode))

@@ -215,6 +215,10 @@ struct SceneHost {
     // OscillatorProjectiveConstraint entries
     std::vector<int32_t> osc_idx;
     std::vector<double> osc_mean, osc_amp, osc_w, osc_phase;
+    // LinearMovementProjectiveConstraint: indices, keyTimes, movements (3 per key), relativeMovements
+    bool lm_on = false, lm_relative = true;
+    std::vector<int32_t> lm_idx;
+    std::vector<double> lm_times, lm_moves;
     double dt = 0.01, rs = 0, rm = 0;
     bool first_order = false, trapezoidal = false;   // EulerImplicitSolver firstOrder, trapezoidalScheme
     double vdamping = 0;                    // EulerImplicitSolver vdamping
@@ -417,22 +421,27 @@ struct EngineT : EngineBase {
         std::vector<FinBlk> fins;
         std::vector<NodeBlk> nblks;
         std::vector<Blk> oblks;
+        std::vector<Blk> lblks;          // LinearMovementProjectiveConstraint entries
     };
     std::vector<Place> place;
+    // per scene (by scene index): the LinearMovementProjectiveConstraint's device data, one
+    // allocation: node ids (int), then x0 (3 doubles each) and the keys (4 doubles each), then the
+    // base masks (int8)
+    std::map<int, void*> lmd;
     Extent cap;
     RangeAlloc a_node, a_elem, a_inc, a_tile, a_lnode, a_shn, a_slot, a_fin, a_nblk, a_osc, a_sid;
     // launch sets: the block lists (a: element blocks / tiles, b: node blocks / finalize blocks,
     // c: tiled node blocks) and the captured step graph of each set of scenes stepped together
     struct LaunchSet {
-        void *a = nullptr, *b = nullptr, *c = nullptr, *o = nullptr;
+        void *a = nullptr, *b = nullptr, *c = nullptr, *o = nullptr, *l = nullptr;
         int* scenes = nullptr;          // scene record indices
-        int na = 0, nb = 0, nc = 0, no = 0, maxit = 0, maxsw = 0, n = 0;
+        int na = 0, nb = 0, nc = 0, no = 0, nl = 0, maxit = 0, maxsw = 0, n = 0;
         int maxpart = 0;                // most partial sums the operator reduces for one scene
         bool plane = false;             // some scene has a PlaneForceField
         hipGraphExec_t gexec = nullptr;
         ~LaunchSet() {
             if (gexec) (void)hipGraphExecDestroy(gexec);
-            for (void* ptr : {a, b, c, o, (void*)scenes})
+            for (void* ptr : {a, b, c, o, l, (void*)scenes})
                 if (ptr) (void)hipFree(ptr);
         }
     };
@@ -448,6 +457,7 @@ struct EngineT : EngineBase {
         if (fork_ev) (void)hipEventDestroy(fork_ev);
         while (!vmd.empty()) free_vm(vmd.begin()->first);
         while (!kd.empty()) free_k(kd.begin()->first);
+        while (!lmd.empty()) free_lm(lmd.begin()->first);
         free_device();
     }
     void free_device() {
@@ -688,6 +698,49 @@ struct EngineT : EngineBase {
         return in;
     }
 
+    // the scene's LinearMovementProjectiveConstraint on the device: its entries, x0 and keys in
+    // one allocation, the record's pointers and counts set in `in`, the entry blocks returned;
+    // `mask` is the scene's response mask without the movement (0 free, 1 fixed, 2 oscillator)
+    std::vector<Blk> place_lm(int si, int rec, int nb0, const std::vector<int8_t>& mask, InstDev& in) {
+        std::vector<Blk> blocks;
+        const SceneHost& s = scenes[si];
+        if (!s.lm_on) return blocks;
+        free_lm(si);
+        const int n = (int)s.lm_idx.size(), nk = (int)s.lm_times.size();
+        const size_t b_node = 0, b_x0 = ((size_t)n * 4 + 15) / 16 * 16, b_keys = b_x0 + (size_t)n * 24;
+        const size_t b_base = b_keys + (size_t)nk * 32, bytes = b_base + (size_t)n + 16;
+        char* buf = dalloc<char>(bytes);
+        lmd[si] = buf;
+        std::vector<int32_t> node(n);
+        std::vector<int8_t> base(n);
+        for (int j = 0; j < n; j++) {
+            node[j] = nb0 + s.lm_idx[j];
+            base[j] = mask[s.lm_idx[j]];
+        }
+        std::vector<double> keys(s.lm_times);
+        keys.insert(keys.end(), s.lm_moves.begin(), s.lm_moves.end());
+        if (n) HIPCHK(hipMemcpy(buf + b_node, node.data(), n * sizeof(int32_t), hipMemcpyHostToDevice));
+        HIPCHK(hipMemset(buf + b_x0, 0, (size_t)n * 24));
+        HIPCHK(hipMemcpy(buf + b_keys, keys.data(), keys.size() * sizeof(double), hipMemcpyHostToDevice));
+        if (n) HIPCHK(hipMemcpy(buf + b_base, base.data(), n, hipMemcpyHostToDevice));
+        in.lm_node = (const int*)(buf + b_node);
+        in.lm_x0 = (double*)(buf + b_x0);
+        in.lm_keys = (const double*)(buf + b_keys);
+        in.lm_base = (const int8_t*)(buf + b_base);
+        in.lm_n = n;
+        in.lm_nk = nk;
+        in.lm_relative = s.lm_relative ? 1 : 0;
+        in.lm_x0_set = 0;
+        for (int j0 = 0; j0 < n; j0 += kBlock) blocks.push_back(Blk{rec, j0, std::min(n, j0 + kBlock), 0});
+        return blocks;
+    }
+    void free_lm(int si) {
+        auto it = lmd.find(si);
+        if (it == lmd.end()) return;
+        (void)hipFree(it->second);
+        lmd.erase(it);
+    }
+
     void commit() override {
         HIPCHK(hipSetDevice(device));
         if (tiled) commit_tiled();
@@ -772,6 +825,7 @@ struct EngineT : EngineBase {
             in = make_inst(si);
             in.node_begin = nb0; in.elem_begin = (int)eb0;
             in.nblk_begin = s.nblk_begin; in.nblk_count = s.nblk_count;
+            place[si].lblks = place_lm(si, si, nb0, h.fixed, in);
         }
         x = dalloc<VT>(3 * Ntot); v = dalloc<VT>(3 * Ntot); f = dalloc<VT>(3 * Ntot); b = dalloc<VT>(3 * Ntot);
         r = dalloc<VT>(3 * Ntot); p = dalloc<VT>(3 * Ntot); q = dalloc<VT>(3 * Ntot); xcg = dalloc<VT>(3 * Ntot);
@@ -1045,6 +1099,7 @@ struct EngineT : EngineBase {
         // scene records
         InstDev in = make_inst(si);
         in.node_begin = nb0; in.elem_begin = (int)eb0;
+        pl.lblks = place_lm(si, sid, nb0, th.eh.fixed, in);
         put_at(inst, sid, &in, 1);
         const Ranges rg{(int)pl.at.tile, T, (int)(cap.tile + pl.at.fin), (int)pl.size.fin, (int)pl.at.nblk,
                         (int)pl.size.nblk};
@@ -1121,6 +1176,7 @@ struct EngineT : EngineBase {
         release_ranges(place[si]);
         free_vm(si);
         free_k(si);
+        free_lm(si);
         iperm[si].clear();
         iperm[si].shrink_to_fit();
         std::lock_guard<std::mutex> g(prep_mu);
@@ -1185,13 +1241,16 @@ struct EngineT : EngineBase {
             put(ls->a, ta); put(ls->b, fa); put(ls->c, na);
             ls->na = (int)ta.size(); ls->nb = (int)fa.size(); ls->nc = (int)na.size();
         }
-        std::vector<Blk> ob;
+        std::vector<Blk> ob, lb;
         for (int32_t id : want) {
             ob.insert(ob.end(), place[id].oblks.begin(), place[id].oblks.end());
+            lb.insert(lb.end(), place[id].lblks.begin(), place[id].lblks.end());
             sids.push_back(sid_of(id));
         }
         put(ls->o, ob);
         ls->no = (int)ob.size();
+        put(ls->l, lb);
+        ls->nl = (int)lb.size();
         void* sc = nullptr;
         put(sc, sids);
         ls->scenes = (int*)sc;
@@ -1212,6 +1271,7 @@ struct EngineT : EngineBase {
         const NodeArrays na = nodes();
         const Blk* eb = (const Blk*)cur->a;
         const Blk* nb = (const Blk*)cur->b;
+        if (cur->nl > 0) k_lm_mask<false><<<cur->nl, kBlock, 0, s>>>((const Blk*)cur->l, inst, 0, fixed);
         if (cur->na > 0) k_rhs_elem<R, NR, VT><<<cur->na, kBlock, 0, s>>>(ev, eb, inst, x, v, ef, ef2);
         double* ndp = seq_dots ? nd : nullptr;
         if (cur->nb > 0) k_rhs_node<VT><<<cur->nb, kBlock, 0, s>>>(nb, inst, na, ef, ef2, v, f, b, r, xcg, partial, ndp, x);
@@ -1227,6 +1287,7 @@ struct EngineT : EngineBase {
         }
         if (cur->nb > 0) k_euler_update<VT><<<cur->nb, kBlock, 0, s>>>(nb, inst, fixed, xcg, v, x);
         if (cur->no > 0) k_oscillate<VT><<<cur->no, kBlock, 0, s>>>((const Blk*)cur->o, inst, oscillators(), x, v);
+        if (cur->nl > 0) k_lm_project<VT><<<cur->nl, kBlock, 0, s>>>((const Blk*)cur->l, inst, x, v);
         k_finalize<<<(n + 63) / 64, 64, 0, s>>>(cur->scenes, n, inst);
     }
 
@@ -1244,6 +1305,7 @@ struct EngineT : EngineBase {
         const FinBlk* fb = (const FinBlk*)cur->b;
         const NodeBlk* nb = (const NodeBlk*)cur->c;
         const SharedNodes sn{sh_gid, sh_ptr, sh_of};
+        if (cur->nl > 0) k_lm_mask<false><<<cur->nl, kBlock, 0, s>>>((const Blk*)cur->l, inst, 0, fixed);
         if (cur->na > 0) k_tile_rhs<R, NR, VT, 512, true><<<cur->na, 512, 0, s>>>(ta, x, v, f, b, r, xcg, sp0, sp1, pbb);
         if (cur->nb > 0)
             k_fin_rhs<VT, true><<<cur->nb, kBlock, 0, s>>>(fb, inst, ranges_d, sn, mass, fixed, v, sp0, sp1, f, b, r, xcg,
@@ -1261,6 +1323,7 @@ struct EngineT : EngineBase {
         }
         if (cur->nc > 0) k_integrate<VT><<<cur->nc, kBlock, 0, s>>>(nb, inst, fixed, xcg, v, x);
         if (cur->no > 0) k_oscillate<VT><<<cur->no, kBlock, 0, s>>>((const Blk*)cur->o, inst, oscillators(), x, v);
+        if (cur->nl > 0) k_lm_project<VT><<<cur->nl, kBlock, 0, s>>>((const Blk*)cur->l, inst, x, v);
         k_step_end_tiled<<<(n + 63) / 64, 64, 0, s>>>(cur->scenes, n, inst);
     }
 
@@ -1366,6 +1429,7 @@ struct EngineT : EngineBase {
         if constexpr (std::is_same<R, double>::value) {
             if (exact_tiled()) { launch_step_exact(n, s); return; }
         }
+        if (cur->nl > 0) k_lm_mask<false><<<cur->nl, kBlock, 0, s>>>((const Blk*)cur->l, inst, 0, fixed);
         if (cur->na > 0) k_tile_rhs<R, NR, VT, 512><<<cur->na, 512, 0, s>>>(ta, x, v, f, b, r, xcg, sp0, sp1, pbb);
         if (cur->nb > 0) k_fin_rhs<VT><<<cur->nb, kBlock, 0, s>>>(fb, inst, ranges_d, sn, mass, fixed, v, sp0, sp1, f, b, r, xcg, pbb, x, contact);
         for (int it = 1; it <= cur->maxit; it++) {
@@ -1376,6 +1440,7 @@ struct EngineT : EngineBase {
         }
         if (cur->nc > 0) k_integrate<VT><<<cur->nc, kBlock, 0, s>>>(nb, inst, fixed, xcg, v, x);
         if (cur->no > 0) k_oscillate<VT><<<cur->no, kBlock, 0, s>>>((const Blk*)cur->o, inst, oscillators(), x, v);
+        if (cur->nl > 0) k_lm_project<VT><<<cur->nl, kBlock, 0, s>>>((const Blk*)cur->l, inst, x, v);
         k_step_end_tiled<<<(n + 63) / 64, 64, 0, s>>>(cur->scenes, n, inst);
     }
 
@@ -1408,6 +1473,7 @@ struct EngineT : EngineBase {
         ic.sn = sn;
         ic.r = r32; ic.p = p32; ic.q = q32; ic.d = d32; ic.sp = sp32;
         ic.pden = pden32; ic.prho = prho32;
+        if (cur->nl > 0) k_lm_mask<false><<<cur->nl, kBlock, 0, s>>>((const Blk*)cur->l, inst, 0, fixed);
         if (cur->na > 0) k_tile_rhs<R, NR, VT, 512><<<cur->na, 512, 0, s>>>(ta, x, v, f, b, r, xcg, sp0, sp1, pbb);
         if (cur->nb > 0) k_fin_rhs<VT><<<cur->nb, kBlock, 0, s>>>(fb, inst, ranges_d, sn, mass, fixed, v, sp0, sp1, f, b, r, xcg, pbb, x, contact);
         for (int sw = 1; sw <= cur->maxsw + 1; sw++) {
@@ -1426,6 +1492,7 @@ struct EngineT : EngineBase {
         }
         if (cur->nc > 0) k_integrate<VT><<<cur->nc, kBlock, 0, s>>>(nb, inst, fixed, xcg, v, x);
         if (cur->no > 0) k_oscillate<VT><<<cur->no, kBlock, 0, s>>>((const Blk*)cur->o, inst, oscillators(), x, v);
+        if (cur->nl > 0) k_lm_project<VT><<<cur->nl, kBlock, 0, s>>>((const Blk*)cur->l, inst, x, v);
         k_step_end_mixed<<<(n + 63) / 64, 64, 0, s>>>(cur->scenes, n, inst, inst_in);
     }
 
@@ -1676,12 +1743,20 @@ struct EngineT : EngineBase {
     void project(int scene, int what, void* vec, double t, hipStream_t s) override {
         const int n = scenes[scene].n_nodes, nb0 = node_begin(scene), sid = sid_of(scene);
         const int grid = (n + kBlock - 1) / kBlock;
+        const int nlm = scenes[scene].lm_on ? (int)scenes[scene].lm_idx.size() : 0;
+        // the movement's response mask at the scene's current time
+        if (what == 0 && nlm > 0)
+            k_lm_mask<true><<<(nlm + kBlock - 1) / kBlock, kBlock, 0, s>>>(nullptr, inst, sid, fixed);
         if (what < 2 && grid > 0) k_project<VT><<<grid, kBlock, 0, s>>>(inst, sid, fixed, nb0, n, (VT*)vec, what);
         int o0, o1;
         osc_range(scene, o0, o1);
         if (what > 0 && o1 > o0)
             k_oscillate_vec<VT><<<(o1 - o0 + kBlock - 1) / kBlock, kBlock, 0, s>>>(oscillators(), o0, o1, nb0, t, (VT*)vec,
                                                                                   what == 2 ? 1 : 0);
+        if (what > 0 && nlm > 0) {
+            k_lm_vec<VT><<<(nlm + kBlock - 1) / kBlock, kBlock, 0, s>>>(inst, sid, t, nb0, (VT*)vec, what == 2 ? 1 : 0);
+            if (what == 2) k_lm_mark_x0<<<1, 64, 0, s>>>(inst, sid);
+        }
         HIPCHK(hipGetLastError());
     }
 
@@ -2208,6 +2283,28 @@ int sofa_amd_scene_add_oscillators(sofa_amd_engine eng, int scene, int n, const 
         s->osc_w.push_back(pulsation[i]);
         s->osc_phase.push_back(phase[i]);
     }
+    return SOFA_AMD_OK;
+}
+
+int sofa_amd_scene_add_linear_movement(sofa_amd_engine eng, int scene, int n, const int32_t* indices, int n_keys,
+                                       const double* key_times, const double* movements, int relative) {
+    SceneHost* s = scene_of(eng, scene);
+    if (!s || n < 0 || n_keys < 1 || (n > 0 && !indices) || !key_times || !movements)
+        return fail(SOFA_AMD_EINVAL, "invalid argument");
+    if (!editable(eng, scene)) return fail(SOFA_AMD_ESTATE, "scene already committed");
+    if (s->lm_on) return fail(SOFA_AMD_EINVAL, "one LinearMovementProjectiveConstraint per scene");
+    for (int i = 0; i < n; i++)
+        if (indices[i] < 0 || indices[i] >= s->n_nodes) return fail(SOFA_AMD_EINVAL, "movement index out of range");
+    for (int k = 0; k < n_keys; k++) {
+        if (!std::isfinite(key_times[k])) return fail(SOFA_AMD_EINVAL, "non-finite key time");
+        if (k > 0 && key_times[k] < key_times[k - 1])
+            return fail(SOFA_AMD_EINVAL, "key times must be in chronological order");
+    }
+    s->lm_on = true;
+    s->lm_relative = relative != 0;
+    s->lm_idx.assign(indices, indices + n);
+    s->lm_times.assign(key_times, key_times + n_keys);
+    s->lm_moves.assign(movements, movements + 3 * (size_t)n_keys);
     return SOFA_AMD_OK;
 }
 
@@ -2794,6 +2891,7 @@ int sofa_amd_scene_release(sofa_amd_engine eng, int scene) {
     s.tets.clear(); s.tets.shrink_to_fit();
     s.young.assign(1, 5000.0); s.poisson.assign(1, 0.45); s.density.assign(1, 1.0);
     s.osc_idx.clear(); s.osc_mean.clear(); s.osc_amp.clear(); s.osc_w.clear(); s.osc_phase.clear();
+    s.lm_idx.clear(); s.lm_times.clear(); s.lm_moves.clear();
     s.fixed.clear();
     s.released = true;
     return SOFA_AMD_OK;

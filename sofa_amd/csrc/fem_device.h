@@ -77,6 +77,15 @@ struct InstDev {
     // tiled engines in the reference's summation order: |b|^2, p.q and r.r of the current
     // iteration, each summed node after node by k_seqdot (MechanicalObject::vDot's order)
     double dots[3];
+    // LinearMovementProjectiveConstraint (LinearMovementProjectiveConstraint.inl:160-330), one
+    // per scene: lm_n constrained nodes (global ids), the mask each has without the movement's bit,
+    // their positions at the first projection (x0, captured then), lm_nk key times followed by
+    // 3 lm_nk movements
+    const int* lm_node;
+    const int8_t* lm_base;
+    double* lm_x0;
+    const double* lm_keys;
+    int lm_n, lm_nk, lm_relative, lm_x0_set;
 };
 
 // which entry of InstDev::dots a k_seqdot launch writes

@@ -441,7 +441,7 @@ __global__ __launch_bounds__(kBlock) void k_euler_update(const Blk* __restrict__
         v[o + k] = in.vdamp != 1.0 ? vv * (VT)in.vdamp : vv;
     }
     // FixedProjectiveConstraint::projectVelocity (.inl:237-258), when activated
-    if (in.project_velocity && fixed[n] == 1) {
+    if (in.project_velocity && (fixed[n] & 1)) {
 #pragma unroll
         for (int k = 0; k < 3; k++) v[o + k] = 0;
     }
@@ -455,6 +455,7 @@ static __global__ void k_finalize(const int* __restrict__ scenes, int n, InstDev
     in->first_solve = 0;
     in->step_count += 1;
     in->time += in->dt;
+    in->lm_x0_set = 1;         // LinearMovementProjectiveConstraint's x0 captured by this step
 }
 
 // ---------------------------------------------------------------------------------------
@@ -485,6 +486,107 @@ __global__ __launch_bounds__(kBlock) void k_oscillate(const Blk* __restrict__ ob
         x[o + k] = (VT)(os.mean[3 * i + k] + os.amp[3 * i + k] * sn);
         v[o + k] = (VT)((os.amp[3 * i + k] * w) * cs);
     }
+}
+
+// ---------------------------------------------------------------------------------------
+// LinearMovementProjectiveConstraint at the context time of the step start
+// ---------------------------------------------------------------------------------------
+// findKeyTimes (LinearMovementProjectiveConstraint.inl:296-330): true when cT lies in
+// [keyTimes[k], keyTimes[k+1]) for some k (the reference's "finished && nextT != prevT"), with the
+// bracketing key times and movements
+__device__ inline bool lm_window(const InstDev& in, double cT, double& pT, double& nT, double pM[3], double nM[3]) {
+    const int nk = in.lm_nk;
+    const double* kt = in.lm_keys;
+    const double* km = in.lm_keys + nk;
+    if (nk == 0 || !(cT >= kt[0] && cT <= kt[nk - 1])) return false;
+    nT = kt[0];
+    pT = nT;
+    for (int k = 0; k < nk; k++) {
+        if (kt[k] <= cT) {
+            pT = kt[k];
+            for (int c = 0; c < 3; c++) pM[c] = km[3 * k + c];
+        } else {
+            nT = kt[k];
+            for (int c = 0; c < 3; c++) nM[c] = km[3 * k + c];
+            return nT != pT;
+        }
+    }
+    return false;
+}
+
+// the response mask of the constrained nodes for the step starting at the scene's time: the
+// movement's bit (4) while a movement is on (projectResponseT, :160-183, clears their responses)
+template <bool SCENE>
+__global__ __launch_bounds__(kBlock) void k_lm_mask(const Blk* __restrict__ lblk, const InstDev* __restrict__ inst,
+                                                     int sid, int8_t* __restrict__ fixed) {
+    int si, i;
+    if (SCENE) {
+        si = sid;
+        i = blockIdx.x * kBlock + threadIdx.x;
+        if (i >= inst[si].lm_n) return;
+    } else {
+        const Blk bk = lblk[blockIdx.x];
+        si = bk.inst;
+        i = bk.begin + threadIdx.x;
+        if (i >= bk.end) return;
+    }
+    const InstDev& in = inst[si];
+    double pT = 0, nT = 0, pM[3] = {0, 0, 0}, nM[3] = {0, 0, 0};
+    const bool on = lm_window(in, in.time, pT, nT, pM, nM);
+    fixed[in.lm_node[i]] = (int8_t)(in.lm_base[i] | (on ? 4 : 0));
+}
+
+// projectPosition then projectVelocity (:193-244, in the order of MechanicalProjectPosition-
+// AndVelocityVisitor.cpp:35-40): x0 captured on the first projection; inside a movement window
+// m = prevM + (nextM - prevM) (cT - prevT) / (nextT - prevT), x = x0 + m (relative) or m,
+// v = (nextM - prevM) (1 / (nextT - prevT)).  The scene's time is the step's start; the capture
+// flag is set by the step-end kernel.
+template <typename VT>
+__global__ __launch_bounds__(kBlock) void k_lm_project(const Blk* __restrict__ lblk, InstDev* __restrict__ inst,
+                                                        VT* __restrict__ x, VT* __restrict__ v) {
+    const Blk bk = lblk[blockIdx.x];
+    const int i = bk.begin + threadIdx.x;
+    if (i >= bk.end) return;
+    const InstDev& in = inst[bk.inst];
+    const int64_t o = 3 * (int64_t)in.lm_node[i];
+    double* x0 = in.lm_x0 + 3 * (int64_t)i;
+    if (!in.lm_x0_set)
+        for (int c = 0; c < 3; c++) x0[c] = (double)x[o + c];
+    double pT = 0, nT = 0, pM[3] = {0, 0, 0}, nM[3] = {0, 0, 0};
+    if (!lm_window(in, in.time, pT, nT, pM, nM)) return;
+    const double dt = (in.time - pT) / (nT - pT);
+    const double inv = 1.0 / (nT - pT);
+    for (int c = 0; c < 3; c++) {
+        const double m = pM[c] + (nM[c] - pM[c]) * dt;
+        x[o + c] = (VT)(in.lm_relative ? x0[c] + m : m);
+        v[o + c] = (VT)((nM[c] - pM[c]) * inv);
+    }
+}
+
+// the same on one caller vector of one scene at time t (component-level projectPosition /
+// projectVelocity); the caller's vector starts at the scene's first node nb0
+template <typename VT>
+__global__ __launch_bounds__(kBlock) void k_lm_vec(const InstDev* __restrict__ inst, int sid, double t, int nb0,
+                                                    VT* __restrict__ vec, int position) {
+    const InstDev& in = inst[sid];
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= in.lm_n) return;
+    const int64_t o = 3 * (int64_t)(in.lm_node[i] - nb0);
+    double* x0 = in.lm_x0 + 3 * (int64_t)i;
+    if (position && !in.lm_x0_set)
+        for (int c = 0; c < 3; c++) x0[c] = (double)vec[o + c];
+    double pT = 0, nT = 0, pM[3] = {0, 0, 0}, nM[3] = {0, 0, 0};
+    if (!lm_window(in, t, pT, nT, pM, nM)) return;
+    const double dt = (t - pT) / (nT - pT);
+    const double inv = 1.0 / (nT - pT);
+    for (int c = 0; c < 3; c++) {
+        const double m = pM[c] + (nM[c] - pM[c]) * dt;
+        vec[o + c] = position ? (VT)(in.lm_relative ? x0[c] + m : m) : (VT)((nM[c] - pM[c]) * inv);
+    }
+}
+
+static __global__ void k_lm_mark_x0(InstDev* __restrict__ inst, int sid) {
+    if (threadIdx.x == 0) inst[sid].lm_x0_set = 1;
 }
 
 // ---------------------------------------------------------------------------------------
@@ -627,7 +729,7 @@ __global__ __launch_bounds__(kBlock) void k_project(const InstDev* __restrict__ 
     const int i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
     const int8_t m = fixed[nb0 + i];
-    const bool clear = velocity ? (inst[sid].project_velocity && m == 1) : m != 0;
+    const bool clear = velocity ? (inst[sid].project_velocity && (m & 1)) : m != 0;
     if (!clear) return;
     const int64_t q = 3 * (int64_t)i;
     vec[q] = 0; vec[q + 1] = 0; vec[q + 2] = 0;

@@ -134,6 +134,7 @@ static __global__ void k_step_end_mixed(const int* __restrict__ scenes, int n, I
     o->first_solve = 0;
     o->step_count += 1;
     o->time += o->dt;
+    o->lm_x0_set = 1;
     o->stopped = 0;
     inner[scenes[i]].stopped = 1;
 }

@@ -967,7 +967,7 @@ __global__ __launch_bounds__(kBlock) void k_integrate(const NodeBlk* __restrict_
                 v[o + i] = damp ? vv * vd : vv;
             }
             // FixedProjectiveConstraint::projectVelocity (.inl:237-258), when activated
-            if (in.project_velocity && fixed[n] == 1) {
+            if (in.project_velocity && (fixed[n] & 1)) {
 #pragma unroll
                 for (int i = 0; i < 3; i++) v[o + i] = 0;
             }
@@ -984,6 +984,7 @@ static __global__ void k_step_end_tiled(const int* __restrict__ scenes, int n, I
     in->first_solve = 0;
     in->step_count += 1;
     in->time += in->dt;
+    in->lm_x0_set = 1;
     in->stopped = 0;   // ready for the next solve
 }
 

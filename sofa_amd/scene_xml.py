@@ -22,7 +22,10 @@ unchanged. Read, with their Data names and defaults:
   ``totalMass``), ``MeshMatrixMass`` (``massDensity``, ``lumping``);
 * ``BoxROI`` (``box``: one or more 6-tuples, bounds inclusive), ``FixedProjectiveConstraint``
   (``indices`` inline or ``@ROI.indices``, ``fixAll``, ``activate_projectVelocity``),
-  ``OscillatorProjectiveConstraint`` (``oscillators``: index, mean, amplitude, pulsation, phase);
+  ``OscillatorProjectiveConstraint`` (``oscillators``: index, mean, amplitude, pulsation, phase),
+  ``LinearMovementProjectiveConstraint`` (``indices`` inline or ``@ROI.indices``, ``keyTimes``,
+  ``movements``, ``relativeMovements``; defaults indices {0}, keyTimes {0}, movements {0 0 0},
+  relative true);
 * ``EulerImplicitSolver`` (``rayleighStiffness``, ``rayleighMass``, ``firstOrder``,
   ``trapezoidalScheme``, ``vdamping``), ``CGLinearSolver`` (``iterations``, ``tolerance``,
   ``threshold``);
@@ -90,6 +93,7 @@ HANDLED = {
     "RegularGridTopology", "GridTopology", "TetrahedronSetTopologyContainer", "Hexa2TetraTopologicalMapping",
     "MechanicalObject", "TetrahedronFEMForceField", "UniformMass", "DiagonalMass", "MeshMatrixMass", "BoxROI",
     "FixedProjectiveConstraint", "FixedConstraint", "OscillatorProjectiveConstraint", "EulerImplicitSolver",
+    "LinearMovementProjectiveConstraint", "LinearMovementConstraint",
     "CGLinearSolver", "PlaneForceField", "MeshGmshLoader", "MeshVTKLoader",
 }
 
@@ -389,6 +393,20 @@ def _body(node, solver, cg, fem, mo, g, dt, node_positions):
         rec = np.asarray(vals).reshape(-1, 9)
         spec["oscillators"] = dict(index=rec[:, 0].astype(np.int32), mean=rec[:, 1:4], amplitude=rec[:, 4:7],
                                    pulsation=rec[:, 7], phase=rec[:, 8])
+    lm = node.first("LinearMovementProjectiveConstraint", "LinearMovementConstraint")
+    if lm is not None:
+        idx = _vec(lm, "indices")
+        if idx is None:
+            idx = np.array([0], np.int32)                    # Data default {0}
+        elif isinstance(idx, str):
+            idx = np.asarray(_floats(idx), dtype=np.int32)
+        keys = _floats(lm.attr.get("keyTimes", "0"))
+        moves = np.asarray(_floats(lm.attr.get("movements", "0 0 0")), dtype=np.float64).reshape(-1, 3)
+        if len(moves) != len(keys):
+            raise ValueError("LinearMovementProjectiveConstraint: one movement per key time")
+        spec["linear_movement"] = dict(indices=np.asarray(idx, dtype=np.int32), key_times=np.asarray(keys),
+                                       movements=moves,
+                                       relative=_bool(lm.attr.get("relativeMovements", "true")))
     # solvers
     a = solver.attr
     if _bool(a.get("solveConstraint", "false")):

@@ -191,4 +191,6 @@ def add_to_engine(eng, spec):
         eng.set_integrator(s, **spec["integrator"])
     if "plane" in spec:
         eng.add_plane(s, **spec["plane"])
+    if "linear_movement" in spec:
+        eng.add_linear_movement(s, **spec["linear_movement"])
     return s

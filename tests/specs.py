@@ -21,6 +21,8 @@ def oracle_from_spec(spec, dot_mode):
         osc = spec["oscillators"]
         o.add_oscillator(int(osc["index"][k]), osc["mean"][k], osc["amplitude"][k], float(osc["pulsation"][k]),
                          float(osc["phase"][k]))
+    if "linear_movement" in spec:
+        o.add_linear_movement(**spec["linear_movement"])
     o.set_solver(spec["rs"], spec["rm"], spec["iters"], spec["tol"], spec["thr"])
     if "plane" in spec:
         o.add_plane(**spec["plane"])

@@ -194,3 +194,78 @@ def test_potential_energy_large_is_zero():
     xd = _dev(spec["pos"] * 1.01)
     assert eng.fem_potential_energy(sid, xd.ptr) == 0.0
     eng.close()
+
+
+def _lm_spec(relative):
+    """cfg0 cantilever with a LinearMovementProjectiveConstraint on its tip face: free for the
+    first 10 steps, driven through two segments, free again after the last key time."""
+    from sofa_amd import workloads
+    s = workloads.cfg0()
+    top = np.nonzero(s["pos"][:, 2] == s["pos"][:, 2].max())[0].astype(np.int32)
+    moves = np.array([[0.0, 0.0, 0.0], [0.03, -0.02, 0.01], [0.0, 0.04, -0.02]])
+    if not relative:
+        moves = moves + np.array([0.5, 0.5, 4.0])
+    s["linear_movement"] = dict(indices=top, key_times=[0.1, 0.25, 0.45], movements=moves, relative=relative)
+    return s
+
+
+@pytest.mark.parametrize("relative", [True, False])
+@pytest.mark.parametrize("prec", ["FP64", "FP64_TILED", "FP32"])
+def test_linear_movement(prec, relative):
+    """LinearMovementProjectiveConstraint through the whole step, 60 steps (before, inside and after
+    the movement windows): the FP64 engine and the FP64 tiled engine in the reference summation
+    order equal the oracle's reference-order run bit for bit; FP32 within 2e-5 of it."""
+    from sofa_amd import workloads
+    capi = _capi()
+    print(f"case: linear movement {prec} relative {relative}", flush=True)
+    spec = _lm_spec(relative)
+    eng = capi.Engine(0, getattr(capi, prec))
+    if prec != "FP32":
+        eng.set_dot_order(True)
+    sid = workloads.add_to_engine(eng, spec)
+    eng.commit()
+    eng.set_graphs(True)
+    eng.step([sid], 60)
+    x, v, _ = eng.read(sid)
+    eng.close()
+    o = _oracle(spec, dot_mode=0)
+    o.step(spec["dt"], 60)
+    xo, vo = o.get("x"), o.get("v")
+    top = spec["linear_movement"]["indices"]
+    assert np.max(np.abs(xo[top] - spec["pos"][top])) > 1e-3        # the tip was driven
+    if prec == "FP32":
+        e = float(np.max(np.linalg.norm(x - xo, axis=1)) / np.max(np.linalg.norm(xo, axis=1)))
+        assert e <= 2e-5, e
+    else:
+        assert np.array_equal(x, xo), float(np.max(np.abs(x - xo)))
+        assert np.array_equal(v, vo)
+
+
+def test_linear_movement_component_calls():
+    """projectResponse / projectVelocity / projectPosition of the movement on caller vectors at a
+    time inside a window and at one outside, against the oracle's (FP64, bit for bit)."""
+    from sofa_amd import workloads
+    capi = _capi()
+    print("case: linear movement component calls", flush=True)
+    spec = _lm_spec(True)
+    n = len(spec["pos"])
+    eng = capi.Engine(0, capi.FP64)
+    sid = workloads.add_to_engine(eng, spec)
+    eng.commit()
+    o = _oracle(spec)
+    rng = np.random.default_rng(3)
+    # projectPosition first at t = 0.3 (inside: x0 captured from this vector), then later times
+    for what, t in ((2, 0.3), (1, 0.3), (1, 0.05), (2, 0.33), (2, 0.5)):
+        a = rng.standard_normal((n, 3))
+        ad = _dev(a)
+        eng.project_position(sid, ad.ptr, t) if what == 2 else eng.project_velocity(sid, ad.ptr, t)
+        eng.sync()
+        ref = o.project(what, a, t)
+        assert np.array_equal(ad.numpy(), ref), (what, t)
+        ad.close()
+    a = rng.standard_normal((n, 3))
+    ad = _dev(a)
+    eng.project_response(sid, ad.ptr)           # at the scene's time 0: before the window
+    eng.sync()
+    assert np.array_equal(ad.numpy(), o.project(0, a, 0.0))
+    eng.close()

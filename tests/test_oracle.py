@@ -178,6 +178,80 @@ def test_project_velocity_closed_form(project):
     assert np.all(vv[fixed] == (0.0 if project else np.array([0.0, 0.0, 0.1])))
 
 
+def _lm_beam(with_fixed_top=False):
+    """3x3x6 cantilever (z = 0 fixed) under gravity; the top face's nodes returned for a
+    LinearMovementProjectiveConstraint."""
+    pos, tets, fixed = meshes.beam((3, 3, 6), (1.0, 1.0, 2.5))
+    top = np.nonzero(pos[:, 2] == pos[:, 2].max())[0].astype(np.int32)
+    s = OracleScene(pos)
+    s.set_gravity((0.0, -9.81, 0.0))
+    s.set_fem(tets, [3e4], [0.4], "large")
+    s.set_uniform_mass(total_mass=80.0)
+    s.set_fixed(np.concatenate([fixed, top]) if with_fixed_top else fixed)
+    s.set_solver(0.1, 0.1, 25, 1e-12, 1e-30)
+    return s, pos, top
+
+
+def test_linear_movement_zero_movement_is_fixed():
+    """LinearMovementProjectiveConstraint with zero movements over a window that starts at t = 0:
+    the response cleared, v = 0 and x = x0 + 0 -- the FixedProjectiveConstraint trajectory bit for
+    bit while the window lasts (keyTimes 0 .. 0.3, 30 steps of 0.01: the last step starts at 0.29)."""
+    a, pos, top = _lm_beam(with_fixed_top=True)
+    b, _, _ = _lm_beam()
+    b.add_linear_movement(top, [0.0, 0.3], np.zeros((2, 3)))
+    a.step(0.01, 30)
+    b.step(0.01, 30)
+    assert np.array_equal(a.get("x"), b.get("x")) and np.array_equal(a.get("v"), b.get("v"))
+    assert np.max(np.abs(a.get("x") - pos)) > 1e-4        # the rest of the beam moved
+
+
+def test_linear_movement_free_outside_the_window():
+    """Before the first key time (and at the last one exactly: findKeyTimes finds no next key) the
+    constraint does nothing: the trajectory equals the unconstrained beam's bit for bit."""
+    a, _, top = _lm_beam()
+    b, _, _ = _lm_beam()
+    b.add_linear_movement(top, [0.2, 0.5], [(0.0, 0.0, 0.0), (0.01, 0.0, 0.0)])
+    a.step(0.01, 20)                                        # steps starting at 0.00 .. 0.19
+    b.step(0.01, 20)
+    assert np.array_equal(a.get("x"), b.get("x")) and np.array_equal(a.get("v"), b.get("v"))
+    c, _, _ = _lm_beam()
+    c.add_linear_movement(top, [0.0], [(0.05, 0.0, 0.0)])   # a single key: never a window
+    a2, _, _ = _lm_beam()
+    a2.step(0.01, 5)
+    c.step(0.01, 5)
+    assert np.array_equal(a2.get("x"), c.get("x"))
+
+
+@pytest.mark.parametrize("relative", [True, False])
+def test_linear_movement_interpolation_closed_form(relative):
+    """Inside the window the driven nodes follow the piecewise-linear key movements at the step-start
+    time t: x = x0 + m(t) (relative; x0 = their positions at the first projection) or x = m(t)
+    (absolute), m(t) = m_k + (m_k+1 - m_k) (t - t_k) / (t_k+1 - t_k), and v = the segment's slope."""
+    s, pos, top = _lm_beam()
+    keys = [0.0, 0.1, 0.25]
+    moves = np.array([[0.0, 0.0, 0.0], [0.02, -0.01, 0.005], [0.0, 0.03, -0.01]])
+    if not relative:
+        moves = moves + np.array([0.5, 0.5, 2.5])
+    s.add_linear_movement(top, keys, moves, relative=relative)
+    h = 0.01
+    x0 = None
+    t = 0.0
+    for k in range(25):                                     # step-start times 0.00 .. 0.24
+        if k:
+            t += h                                          # the scene's time accumulates dt
+        s.step(h, 1)
+        seg = 0 if t < keys[1] else 1
+        dt = (t - keys[seg]) / (keys[seg + 1] - keys[seg])
+        m = moves[seg] + (moves[seg + 1] - moves[seg]) * dt
+        slope = (moves[seg + 1] - moves[seg]) * (1.0 / (keys[seg + 1] - keys[seg]))
+        x, v = s.get("x"), s.get("v")
+        if x0 is None:
+            x0 = pos[top]                                   # the response was cleared in step 1
+        want = x0 + m if relative else np.broadcast_to(m, x0.shape)
+        assert np.allclose(x[top], want, rtol=0, atol=1e-15), (k, np.max(np.abs(x[top] - want)))
+        assert np.all(v[top] == slope), k
+
+
 def test_diagonal_total_mass():
     """DiagonalMass given totalMass (DiagonalMass.inl:1235-1257): the vertex masses sum to the
     total and are proportional to the density-1 masses (each vertex V/4 of each incident

@@ -217,6 +217,40 @@ def test_serving_close_waits_for_preparations():
     assert sum(f.cancelled() for f in futs) >= 1
 
 
+def test_scene_reader_linear_movement():
+    """LinearMovementProjectiveConstraint Data (indices through a BoxROI link, keyTimes,
+    movements, relativeMovements) and its defaults."""
+    from sofa_amd import scene_xml
+    text = """<Node name="root" dt="0.01" gravity="0 -9.81 0">
+      <Node name="beam">
+        <EulerImplicitSolver rayleighStiffness="0.1" rayleighMass="0.1"/>
+        <CGLinearSolver iterations="25" tolerance="1e-9" threshold="1e-9"/>
+        <RegularGridTopology name="grid" n="3 3 5" min="0 0 0" max="1 1 2"/>
+        <MechanicalObject name="mo"/>
+        <TetrahedronFEMForceField youngModulus="1e5" poissonRatio="0.4" method="large"/>
+        <UniformMass totalMass="10"/>
+        <BoxROI name="top" box="-0.1 -0.1 1.9 1.1 1.1 2.1"/>
+        <LinearMovementProjectiveConstraint indices="@top.indices" keyTimes="0 0.5 1"
+            movements="0 0 0  0.1 0 0  0.1 0.2 0" relativeMovements="false"/>
+        <FixedProjectiveConstraint indices="0 1 2"/>
+      </Node>
+    </Node>"""
+    (spec,), info = scene_xml.load_scene(text)
+    lm = spec["linear_movement"]
+    top = np.nonzero(spec["pos"][:, 2] >= 1.9)[0]
+    assert np.array_equal(np.sort(lm["indices"]), top) and len(top) == 9
+    assert list(lm["key_times"]) == [0.0, 0.5, 1.0] and lm["relative"] is False
+    assert np.array_equal(lm["movements"], [[0, 0, 0], [0.1, 0, 0], [0.1, 0.2, 0]])
+    (spec2,), _ = scene_xml.load_scene(text.replace(
+        '<LinearMovementProjectiveConstraint indices="@top.indices" keyTimes="0 0.5 1"\n'
+        '            movements="0 0 0  0.1 0 0  0.1 0.2 0" relativeMovements="false"/>',
+        "<LinearMovementProjectiveConstraint/>"))
+    d = spec2["linear_movement"]
+    assert list(d["indices"]) == [0] and list(d["key_times"]) == [0.0] and d["relative"] is True
+    with pytest.raises(ValueError):
+        scene_xml.load_scene(text.replace('keyTimes="0 0.5 1"', 'keyTimes="0 0.5"'))
+
+
 def test_fem_hexa_tessellation():
     """TetrahedronFEMForceField::init's 6-tetrahedra tessellation of a grid (cube corners mirrored
     by cube parity): conforming (every interior triangle in exactly two tetrahedra, every boundary
