@@ -347,14 +347,18 @@ def cfg2_mixed(steps, err_steps):
 
 def serve_cfg4(seconds, arrivals, budget_gb):
     """cfg4 job stream (200 jobs/s Poisson, seed 4) over the first `arrivals` seconds of arrivals,
-    served for `seconds` of wall time, batched and plain, FP32."""
+    served for `seconds` of wall time, batched and plain, FP32, every launch set's step replayed
+    from a hipGraph; the admission budget is `budget_gb` or, when None, 80 % of the free HBM."""
     from sofa_amd import capi, serving, workloads
     jobs = workloads.cfg4_jobs(rate=200.0, duration=arrivals, seed=4)
-    out = {"workload": f"cfg4 job stream: Poisson 200 jobs/s for {arrivals:g} s ({len(jobs)} jobs), beams and voxel "
-                       "ellipsoids, nodes log-uniform [1e3, 2e5], steps U{50..500}, oscillating BC; "
-                       f"{seconds:g} s served per mode, HBM budget {budget_gb:g} GB, FP32, eager launches"}
+    budget = None if budget_gb is None else int(budget_gb * (1 << 30))
+    out = {"workload": f"cfg4 job stream: Poisson 200 jobs/s for {arrivals:g} s ({len(jobs)} jobs), beams and "
+                       "ellipsoids (voxel ellipsoids split 6 ways in place of the jittered-grid Delaunay recipe, "
+                       "whose per-job Delaunay takes seconds on the host), nodes log-uniform [1e3, 2e5], steps "
+                       f"U{{50..500}}, oscillating BC; {seconds:g} s served per mode, FP32, hipGraph step replay",
+           "budget": "80 % of the free HBM" if budget is None else f"{budget_gb:g} GB"}
     for batched in (True, False):
-        srv = serving.Server(capi.FP32, budget_bytes=int(budget_gb * (1 << 30)), batched=batched)
+        srv = serving.Server(capi.FP32, budget_bytes=budget, batched=batched, graphs=True, build_workers=8)
         res = srv.run(jobs, wall_limit=seconds, stream=bench_stream(),
                       log=lambda m: print(m, file=sys.stderr, flush=True))
         srv.close()
@@ -459,9 +463,10 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-sofacuda", action="store_true")
     ap.add_argument("--sofacuda-steps", type=int, default=1000)
-    ap.add_argument("--serve-seconds", type=float, default=15.0)
-    ap.add_argument("--serve-arrivals", type=float, default=5.0)
-    ap.add_argument("--serve-budget-gb", type=float, default=8.0)
+    ap.add_argument("--serve-seconds", type=float, default=60.0)
+    ap.add_argument("--serve-arrivals", type=float, default=60.0)
+    ap.add_argument("--serve-budget-gb", type=float, default=None,
+                    help="admission budget of the served stream (default: 80 %% of the free HBM)")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))

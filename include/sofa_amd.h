@@ -80,6 +80,10 @@ const char* sofa_amd_last_error(void);
  * (PluginManager.cpp:258-266 unloads a plugin that answers false). */
 int sofa_amd_device_check(int device);
 
+/* Free and total device memory of `device` (hipMemGetInfo): a serving loop sizes its admission
+ * budget from it.  No reference counterpart. */
+int sofa_amd_device_memory(int device, int64_t* free_bytes, int64_t* total_bytes);
+
 /* ---- engine: a set of device-resident scenes advanced in merged launches ----------------- */
 
 /* No reference counterpart (SOFA runs one scene per process); holds device buffers. */

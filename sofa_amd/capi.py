@@ -20,7 +20,7 @@ END_REASONS = {0: "iterations", 1: "tolerance", 2: "threshold", 3: "den==0", 4: 
 
 # every symbol include/sofa_amd.h declares
 EXPORTS = [
-    "sofa_amd_module_name", "sofa_amd_module_version", "sofa_amd_last_error", "sofa_amd_device_check",
+    "sofa_amd_module_name", "sofa_amd_module_version", "sofa_amd_last_error", "sofa_amd_device_check", "sofa_amd_device_memory",
     "sofa_amd_engine_create", "sofa_amd_engine_destroy", "sofa_amd_scene_create", "sofa_amd_scene_set_fem",
     "sofa_amd_scene_set_mass", "sofa_amd_scene_set_gravity", "sofa_amd_scene_set_fixed",
     "sofa_amd_scene_set_solver", "sofa_amd_engine_commit", "sofa_amd_engine_step", "sofa_amd_engine_sync",
@@ -79,6 +79,7 @@ def lib():
         "sofa_amd_module_version": (ctypes.c_char_p, []),
         "sofa_amd_last_error": (ctypes.c_char_p, []),
         "sofa_amd_device_check": (c_int, [c_int]),
+        "sofa_amd_device_memory": (c_int, [c_int, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)]),
         "sofa_amd_engine_create": (c_int, [c_int, c_int, ctypes.POINTER(_vp)]),
         "sofa_amd_engine_destroy": (c_int, [_vp]),
         "sofa_amd_scene_create": (c_int, [_vp, c_int, _dp, _ip]),
@@ -177,6 +178,13 @@ def _sh(stream):
 def device_available(device=0):
     """True when `device` is a usable gfx950 GPU (the library's own runtime; no exception)."""
     return lib().sofa_amd_device_check(int(device)) == OK
+
+
+def device_memory(device=0):
+    """(free, total) bytes of device memory."""
+    fr, tot = ctypes.c_int64(), ctypes.c_int64()
+    check(lib().sofa_amd_device_memory(int(device), ctypes.byref(fr), ctypes.byref(tot)))
+    return fr.value, tot.value
 
 
 def device_sync(device=0):

@@ -905,6 +905,11 @@ struct EngineT : EngineBase {
     // device buffers of capacity `c` in every index space (all scenes' data is dropped)
     void reserve(const Extent& c) override {
         if (!tiled) throw std::invalid_argument("live scenes need a tiled engine (FP32, FP64_TILED or MIXED)");
+        {   // every index space is addressed with 32-bit indices on the device
+            Extent t = c;
+            for (int k = 0; k < 11; k++)
+                if (field(t, k) >= INT32_MAX) throw std::invalid_argument("capacity beyond the 32-bit index spaces");
+        }
         HIPCHK(hipSetDevice(device));
         HIPCHK(hipDeviceSynchronize());
         sets.clear();
@@ -2129,6 +2134,20 @@ int sofa_amd_device_check(int device) {
     return SOFA_AMD_OK;
 }
 
+int sofa_amd_device_memory(int device, int64_t* free_bytes, int64_t* total_bytes) {
+    GUARD_BEGIN
+    if (!free_bytes || !total_bytes) return fail(SOFA_AMD_EINVAL, "null output");
+    const int rc = sofa_amd_device_check(device);
+    if (rc != SOFA_AMD_OK) return rc;
+    HIPCHK(hipSetDevice(device));
+    size_t fr = 0, tot = 0;
+    HIPCHK(hipMemGetInfo(&fr, &tot));
+    *free_bytes = (int64_t)fr;
+    *total_bytes = (int64_t)tot;
+    return SOFA_AMD_OK;
+    GUARD_END
+}
+
 int sofa_amd_engine_create(int device, int precision, sofa_amd_engine* out) {
     GUARD_BEGIN
     if (!out) return fail(SOFA_AMD_EINVAL, "null output");
@@ -2840,7 +2859,9 @@ int sofa_amd_engine_reserve(sofa_amd_engine eng, int64_t max_nodes, int64_t max_
     c.tile = max_tets / 128 + max_scenes;
     c.lnode = 2 * max_nodes + c.tile;
     c.shn = max_nodes + max_scenes;
-    c.slot = 2 * max_nodes;
+    // shared-partial slots: one per (shared node, tile), or one per incidence of a shared node in
+    // the reference summation order
+    c.slot = e->seq_dots ? std::min<int64_t>(4 * max_tets, INT32_MAX - 1) : 2 * max_nodes;
     c.fin = max_nodes / kBlock + max_scenes;
     c.nblk = max_nodes / (kBlock * kNodesPerThread) + max_scenes;
     c.osc = max_nodes;

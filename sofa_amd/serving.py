@@ -11,7 +11,12 @@ its own launches (the one-instance-at-a-time path the batched mode is compared w
 Reported: element-steps/s over the run; per-step latency = device time of the merged step (HIP
 events around it on its stream) attributed to every job it advances, p50 / p99 over all
 (job, step) pairs; admission wait = wall time from arrival to the start of the job's first
-step, p50 / p99.  The scheduler is host code around the C ABI; all per-step work is on the GPU.
+step, p50 / p99; in plain mode also each job step's own device time, summarised per job size
+(element-steps/s of one job alone on the GPU, by tetrahedron count).  The scheduler is host code
+around the C ABI; all per-step work is on the GPU.  With graphs on, every launch set's step is
+captured once and replayed (one graph per set of running jobs in batched mode, one per job in
+plain mode).  Without an explicit budget the admission budget is 80 % of the device's free memory
+when the server starts.
 """
 from __future__ import annotations
 
@@ -44,19 +49,24 @@ def footprint_bytes(precision, nodes, tets):
 class Server:
     """One GPU, one live engine, a FIFO admission queue bounded by an HBM budget."""
 
-    def __init__(self, precision=capi.FP32, budget_bytes=8 << 30, max_scenes=2048, batched=True,
-                 build_workers=4, engine=None):
+    def __init__(self, precision=capi.FP32, budget_bytes=None, max_scenes=2048, batched=True,
+                 build_workers=4, engine=None, graphs=False):
         self.precision = precision
+        if budget_bytes is None:
+            free, _ = capi.device_memory(0)
+            budget_bytes = int(0.8 * free)
         self.budget = int(budget_bytes)
         self.batched = batched
         if engine is None:
             # arena capacity: the budget expressed in nodes and tetrahedra (a tetrahedral mesh
             # has about 5-6 tetrahedra per node; the node arena is sized for the sparser side)
             per_tet = footprint_bytes(precision, 0, 1)
-            max_tets = self.budget // per_tet
+            # the engine's incidence space (4 per tetrahedron) is addressed with 32-bit indices
+            max_tets = min(self.budget // per_tet, 500_000_000)
             max_nodes = max_tets // 3
             engine = capi.Engine(0, precision)
             engine.reserve(max_nodes, max_tets, max_scenes)
+            engine.set_graphs(graphs)
         self.eng = engine
         self.pool = cf.ThreadPoolExecutor(build_workers)
 
@@ -96,6 +106,7 @@ class Server:
         used = 0
         admitted, done, rejected = {}, 0, 0
         waits, step_records = [], []                  # (start event, end event, [tets per job])
+        job_records = []                              # plain mode: (start event, end event, tets) per job step
         elem_steps = 0
         t0 = time.perf_counter()
         inflight = collections.deque()
@@ -167,7 +178,11 @@ class Server:
                 self.eng.step(ids, 1, stream=sp)
             else:
                 for s in ids:
+                    j0, j1 = event(), event()
+                    j0.record(stream)
                     self.eng.step([s], 1, stream=sp)
+                    j1.record(stream)
+                    job_records.append((j0, j1, active[s][2]))
             e1.record(stream)
             tets = [active[s][2] for s in ids]
             step_records.append((e0, e1, tets))
@@ -198,10 +213,23 @@ class Server:
             lat.extend([ms] * len(tets))
         lat = np.asarray(lat) if lat else np.zeros(1)
         waits_a = np.asarray(waits) if waits else np.zeros(1)
+        by_size = None
+        if job_records:
+            tets = np.array([t for _, _, t in job_records], dtype=np.float64)
+            ms = np.array([a.elapsed_time(b) for a, b, _ in job_records])
+            edges = [0, 1e4, 3e4, 1e5, 3e5, 1e6, np.inf]
+            by_size = []
+            for lo, hi in zip(edges[:-1], edges[1:]):
+                sel = (tets >= lo) & (tets < hi)
+                if sel.any():
+                    by_size.append({"tets": [lo, None if hi == np.inf else hi], "job_steps": int(sel.sum()),
+                                    "element_steps": int(tets[sel].sum()),
+                                    "median_ms_per_step": float(np.median(ms[sel])),
+                                    "element_steps_per_s_alone": float(tets[sel].sum() / (ms[sel].sum() / 1e3))})
         return {"mode": "batched" if self.batched else "plain", "wall_s": wall, "element_steps": int(elem_steps),
                 "value": elem_steps / wall, "ticks": len(step_records), "jobs_arrived": nxt,
                 "jobs_admitted": len(admitted), "jobs_done": done, "jobs_rejected": rejected, "jobs_running_at_end": len(active),
                 "queue_at_end": len(queue),
                 "step_latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
                 "admission_wait_s": {"p50": float(np.percentile(waits_a, 50)), "p99": float(np.percentile(waits_a, 99))},
-                "budget_bytes": self.budget}
+                "budget_bytes": self.budget, "plain_by_job_size": by_size}
