@@ -138,6 +138,18 @@ int sofa_amd_scene_add_oscillators(sofa_amd_engine eng, int scene, int n, const 
 int sofa_amd_scene_add_linear_movement(sofa_amd_engine eng, int scene, int n, const int32_t* indices, int n_keys,
                                        const double* key_times, const double* movements, int relative);
 
+/* PCGLinearSolver (PCGLinearSolver.inl:178-262) in place of the scene's CGLinearSolver, with a
+ * JacobiPreconditioner (kind 1) or a BlockJacobiPreconditioner (kind 2) linked; 0 restores the
+ * CGLinearSolver.  The CG Data of sofa_amd_scene_set_solver are the PCG's: `iterations` and
+ * `tolerance` (on r.z relative to b.b; no threshold).  The preconditioner is assembled every
+ * step (the PreconditionedMatrixFreeSystem's assemblingRate 1): the 3x3 diagonal blocks of
+ * (1 + h rm) M + h (h + rs) K with the step's rotations (Jacobi keeps their diagonals), the
+ * identity on fixed and LinearMovement nodes (Dirichlet rows), inverted.  The solve starts from
+ * the previous step's solution, as the reference's does (the solver's solution vector persists);
+ * the statistics report PCG's iteration count (maxIter + 1 when the budget runs out) and
+ * err = r.z / b.b.  FP64 engine. */
+int sofa_amd_scene_set_preconditioner(sofa_amd_engine eng, int scene, int kind);
+
 /* Context gravity (Node::setGravity), used by the masses' addForce. */
 int sofa_amd_scene_set_gravity(sofa_amd_engine eng, int scene, double gx, double gy, double gz);
 

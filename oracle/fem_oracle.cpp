@@ -720,6 +720,8 @@ struct MassBase : public ForceField {
     virtual void addMDx(std::vector<Vec3>& res, const std::vector<Vec3>& dx, Real factor) = 0;
     // Mass::accFromF: a = M^-1 f; false where the reference refuses (a consistent MeshMatrixMass)
     virtual bool accFromF(std::vector<Vec3>& a, const std::vector<Vec3>& f) = 0;
+    // node i's entry on the diagonal of the assembled mass matrix (buildMassMatrix)
+    virtual Real diagMass(size_t i) const = 0;
     void addMBKdx(std::vector<Vec3>& df, const std::vector<Vec3>& dx, Real mFact, Real bFact, Real) override {
         // Mass.inl:94-104: ForceField::addMBKdx calls Mass::addDForce, a no-op; then addMDx
         const Real m = mFact - bFact * rayleighMass;
@@ -742,6 +744,7 @@ struct UniformMass : public MassBase {
             for (int q = 0; q < 3; q++) a[i][q] = f[i][q] / m;
         return true;
     }
+    Real diagMass(size_t) const override { return vertexMass; }
     void addForce(std::vector<Vec3>& f, const std::vector<Vec3>&, const std::vector<Vec3>&) override {
         Vec3 mg(gravity[0] * vertexMass, gravity[1] * vertexMass, gravity[2] * vertexMass);   // :471-496
         for (size_t i = 0; i < f.size(); i++)
@@ -786,6 +789,7 @@ struct DiagonalMass : public MassBase {
             for (int q = 0; q < 3; q++) a[i][q] = f[i][q] / masses[i];
         return true;
     }
+    Real diagMass(size_t i) const override { return masses[i]; }
     void addForce(std::vector<Vec3>& f, const std::vector<Vec3>&, const std::vector<Vec3>&) override {
         for (size_t i = 0; i < masses.size(); i++)      // :1393-1413
             for (int q = 0; q < 3; q++) f[i][q] += gravity[q] * masses[i];
@@ -856,6 +860,8 @@ struct MeshMatrixMass : public MassBase {
             }
         }
     }
+    // buildMassMatrix: the lumped vertex mass, or the consistent matrix's vertex term
+    Real diagMass(size_t i) const override { return lumping ? vertexMass[i] * lumpCoeff : vertexMass[i]; }
     // :2051-2069: lumped only (the reference refuses the sparse mass)
     bool accFromF(std::vector<Vec3>& a, const std::vector<Vec3>& f) override {
         if (!lumping) return false;
@@ -1002,6 +1008,13 @@ struct Scene {
     bool projectVelocity = false;   // FixedProjectiveConstraint activate_projectVelocity (default false)
     std::vector<Oscillator> oscillators;
     std::vector<LinearMovement> movements;     // after the oscillators in scene order
+    // PCGLinearSolver (PCGLinearSolver.inl:178-262) in place of CGLinearSolver: 0 none, 1 with a
+    // JacobiPreconditioner, 2 with a BlockJacobiPreconditioner; its solution vector persists
+    // across steps (the EulerImplicitSolver's x, realloc'd once at init), so each solve starts
+    // from the previous step's solution
+    int precond = 0;
+    std::vector<Vec3> xsol;
+    int lastPcgIterations = 0;
     Vec3 gravity;
     Real rayleighStiffness = 0, rayleighMass = 0;
     // EulerImplicitSolver Data (EulerImplicitSolver.cpp:40-51): firstOrder, trapezoidalScheme,
@@ -1099,6 +1112,123 @@ struct Scene {
         projectResponse(q);
     }
 
+    // The preconditioner's matrix as MatrixLinearSystem assembles it: mF M (buildMassMatrix's
+    // diagonal) then kF (-R J K J^T R^T) of every element in order (buildStiffnessMatrix's node
+    // blocks), each value scaled by its factor as it is added; the 3x3 diagonal blocks are kept
+    // (block Jacobi: BlockDiagonalMatrix<3>; Jacobi: DiagonalMatrix keeps their diagonals).
+    // Dirichlet rows (FixedProjectiveConstraint, LinearMovementProjectiveConstraint indices:
+    // applyConstraint(ZeroDirichletCondition*), MatrixLinearSystem.inl:1177-1189) become the
+    // identity.  Then inverted: DiagonalMatrix::invert (1 / d) or Mat<3,3>::invert
+    // (invertMatrix, Mat.h:1172-1192, with determinant, Mat.h:989-997).
+    void precondInverse(std::vector<Mat3>& P, Real mF, Real kF) {
+        const size_t N = n();
+        P.assign(N, Mat3());
+        if (mass)
+            for (size_t i = 0; i < N; i++) {
+                const Real m = mass->diagMass(i) * mF;
+                for (int a = 0; a < 3; a++) P[i].m[a][a] += m;
+            }
+        if (fem) {
+            Mat3 ident;
+            for (int a = 0; a < 3; a++) ident.m[a][a] = 1;
+            Real tmp[144];
+            for (size_t it = 0; it < fem->elems.size(); it++) {
+                fem->computeStiffnessMatrix(it, fem->method == TetraFEM::SMALL ? ident : fem->rotations[it], tmp);
+                for (int v = 0; v < 4; v++) {
+                    Mat3& B = P[fem->elems[it][v]];
+                    for (int a = 0; a < 3; a++)
+                        for (int b = 0; b < 3; b++) B.m[a][b] += (-tmp[(3 * v + a) * 12 + 3 * v + b]) * kF;
+                }
+            }
+        }
+        auto dirichlet = [&](size_t i) {
+            P[i] = Mat3();
+            for (int a = 0; a < 3; a++) P[i].m[a][a] = 1;
+        };
+        if (fixAll) for (size_t i = 0; i < N; i++) dirichlet(i);
+        else for (int i : fixed) dirichlet(i);
+        for (const auto& lm : movements)
+            for (int i : lm.indices) dirichlet(i);
+        for (size_t i = 0; i < N; i++) {
+            Mat3& B = P[i];
+            if (precond == 1) {
+                for (int a = 0; a < 3; a++) {
+                    B.m[a][a] = 1.0 / B.m[a][a];
+                    for (int b = 0; b < 3; b++) if (a != b) B.m[a][b] = 0;
+                }
+            } else {
+                const Mat3 f = B;
+                auto F = [&](int r, int c) { return f.m[r][c]; };
+                const Real det = F(0, 0) * F(1, 1) * F(2, 2) + F(1, 0) * F(2, 1) * F(0, 2) + F(2, 0) * F(0, 1) * F(1, 2)
+                                 - F(0, 0) * F(2, 1) * F(1, 2) - F(1, 0) * F(0, 1) * F(2, 2) - F(2, 0) * F(1, 1) * F(0, 2);
+                B.m[0][0] = (F(1, 1) * F(2, 2) - F(2, 1) * F(1, 2)) / det;
+                B.m[1][0] = (F(1, 2) * F(2, 0) - F(2, 2) * F(1, 0)) / det;
+                B.m[2][0] = (F(1, 0) * F(2, 1) - F(2, 0) * F(1, 1)) / det;
+                B.m[0][1] = (F(2, 1) * F(0, 2) - F(0, 1) * F(2, 2)) / det;
+                B.m[1][1] = (F(2, 2) * F(0, 0) - F(0, 2) * F(2, 0)) / det;
+                B.m[2][1] = (F(2, 0) * F(0, 1) - F(0, 0) * F(2, 1)) / det;
+                B.m[0][2] = (F(0, 1) * F(1, 2) - F(1, 1) * F(0, 2)) / det;
+                B.m[1][2] = (F(0, 2) * F(1, 0) - F(1, 2) * F(0, 0)) / det;
+                B.m[2][2] = (F(0, 0) * F(1, 1) - F(1, 0) * F(0, 1)) / det;
+            }
+        }
+    }
+    // z = P^-1 r: DiagonalMatrix::mult (z_i = d_i r_i, DiagonalMatrix.h:187-190) or
+    // BlockDiagonalMatrix::mul (z_j = sum_i B(i, j) r_i from 0, BlockDiagonalMatrix.h:202-221)
+    void applyPrecond(const std::vector<Mat3>& P, const std::vector<Vec3>& r, std::vector<Vec3>& z) const {
+        for (size_t i = 0; i < r.size(); i++) {
+            if (precond == 1) {
+                for (int a = 0; a < 3; a++) z[i][a] = P[i].m[a][a] * r[i][a];
+            } else {
+                for (int j = 0; j < 3; j++) {
+                    Real acc = 0;
+                    for (int a = 0; a < 3; a++) acc += P[i].m[a][j] * r[i][a];
+                    z[i][j] = acc;
+                }
+            }
+        }
+    }
+    // PCGLinearSolver::solve (PCGLinearSolver.inl:178-262) with the Jacobi / block-Jacobi
+    // preconditioner, from the solution vector's current value
+    void pcgSolve(std::vector<Vec3>& x, const std::vector<Vec3>& b, Real mF, Real bF, Real kF) {
+        const size_t N = n();
+        std::vector<Mat3> P;
+        precondInverse(P, mF, kF);
+        std::vector<Vec3> r(N), w(N), s(N);
+        const Real b_norm = vdot(b, b);
+        const Real tol = cgTolerance * b_norm;
+        applyA(r, x, mF, bF, kF);                                   // r = M * x
+        for (size_t i = 0; i < N; i++)                              // r = -1 r + b (r *= -1; r += b)
+            for (int k = 0; k < 3; k++) { r[i][k] *= -1.0; r[i][k] += b[i][k]; }
+        applyPrecond(P, r, w);
+        Real r_norm = vdot(r, w);
+        unsigned iter = 1;
+        lastErr = b_norm != 0 ? r_norm / b_norm : 0;
+        lastDen = 0;
+        while (iter <= cgIterations && r_norm > tol) {
+            applyA(s, w, mF, bF, kF);                               // s = M * w
+            const Real dtq = vdot(w, s);
+            const Real alpha = r_norm / dtq;
+            lastDen = dtq;
+            for (size_t i = 0; i < N; i++)
+                for (int k = 0; k < 3; k++) {
+                    x[i][k] += w[i][k] * alpha;                     // x.peq(w, alpha)
+                    r[i][k] += s[i][k] * (-alpha);                  // r.peq(s, -alpha)
+                }
+            applyPrecond(P, r, s);
+            const Real deltaOld = r_norm;
+            r_norm = vdot(r, s);
+            lastErr = b_norm != 0 ? r_norm / b_norm : 0;
+            const Real beta = r_norm / deltaOld;
+            for (size_t i = 0; i < N; i++)                          // w = s + w beta (w *= beta; w += s)
+                for (int k = 0; k < 3; k++) { w[i][k] *= beta; w[i][k] += s[i][k]; }
+            iter++;
+        }
+        lastIterations = (int)iter;
+        lastEndCond = iter > cgIterations ? 0 : 1;
+        timeStepCount++;
+    }
+
     // CGLinearSolver<GraphScattered>::solve (CGLinearSolver.inl:71-315); warmStart = false
     void cgSolve(std::vector<Vec3>& xs, const std::vector<Vec3>& b, Real mF, Real bF, Real kF) {
         const size_t N = n();
@@ -1193,7 +1323,13 @@ struct Scene {
         const Real bFact = firstOrder ? 0 : -tr * h;
         const Real kFact = firstOrder ? -h * tr : -tr * h * (tr * h + rayleighStiffness);
         std::vector<Vec3> xs(N);
-        cgSolve(xs, b, mFact, bFact, kFact);
+        if (precond) {
+            if (xsol.size() != N) xsol.assign(N, Vec3());
+            pcgSolve(xsol, b, mFact, bFact, kFact);
+            xs = xsol;
+        } else {
+            cgSolve(xs, b, mFact, bFact, kFact);
+        }
         // the vMultiOp update (:246-268; MechanicalObject.inl:2209-2230): v = v + x (firstOrder:
         // v = x), pos = pos + v h
         const Real fxv = (Real)h;
@@ -1339,6 +1475,17 @@ void orc_add_linear_movement(void* h, const int* idx, int n, const double* key_t
     for (int k = 0; k < nk; k++) lm.movements.push_back(Vec3(moves[3 * k], moves[3 * k + 1], moves[3 * k + 2]));
     lm.relative = relative != 0;
     s->movements.push_back(lm);
+}
+void orc_set_preconditioner(void* h, int kind) { static_cast<Scene*>(h)->precond = kind; }
+// the preconditioner's inverted 3x3 blocks (Jacobi: the inverted diagonals) for factors mF, kF,
+// with the current rotations: out 9 per node, row-major
+void orc_precond_blocks(void* h, double mF, double kF, double* out) {
+    Scene* s = static_cast<Scene*>(h);
+    std::vector<Mat3> P;
+    s->precondInverse(P, mF, kF);
+    for (size_t i = 0; i < P.size(); i++)
+        for (int a = 0; a < 3; a++)
+            for (int b = 0; b < 3; b++) out[9 * i + 3 * a + b] = P[i].m[a][b];
 }
 void orc_set_project_velocity(void* h, int on) { static_cast<Scene*>(h)->projectVelocity = on != 0; }
 

@@ -57,6 +57,8 @@ def lib():
         L.orc_set_dot_mode.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.orc_set_integrator.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_double]
         L.orc_set_project_velocity.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.orc_set_preconditioner.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.orc_precond_blocks.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double, _dp]
         L.orc_add_linear_movement.argtypes = [ctypes.c_void_p, _ip, ctypes.c_int, _dp, _dp, ctypes.c_int, ctypes.c_int]
         L.orc_set_threads.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.orc_step.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_int]
@@ -179,6 +181,16 @@ class OracleScene:
     def set_project_velocity(self, on=True):
         """FixedProjectiveConstraint activate_projectVelocity."""
         lib().orc_set_project_velocity(self._h, 1 if on else 0)
+
+    def set_preconditioner(self, kind):
+        """PCGLinearSolver in place of CGLinearSolver: 0 none, 1 Jacobi, 2 block Jacobi."""
+        lib().orc_set_preconditioner(self._h, int(kind))
+
+    def precond_blocks(self, m_factor, k_factor):
+        """The preconditioner's inverted 3x3 blocks (n, 3, 3) with the current rotations."""
+        out = np.zeros((self.n, 3, 3))
+        lib().orc_precond_blocks(self._h, float(m_factor), float(k_factor), out.ctypes.data_as(_dp))
+        return out
 
     def add_linear_movement(self, indices, key_times, movements, relative=True):
         """LinearMovementProjectiveConstraint: indices, keyTimes, movements (one 3-vector per key),

@@ -217,6 +217,7 @@ struct SceneHost {
     std::vector<double> osc_mean, osc_amp, osc_w, osc_phase;
     // LinearMovementProjectiveConstraint: indices, keyTimes, movements (3 per key), relativeMovements
     bool lm_on = false, lm_relative = true;
+    int precond = 0;                        // PCGLinearSolver: 0 none (CGLinearSolver), 1 Jacobi, 2 block Jacobi
     std::vector<int32_t> lm_idx;
     std::vector<double> lm_times, lm_moves;
     double dt = 0.01, rs = 0, rm = 0;
@@ -387,6 +388,8 @@ struct EngineT : EngineBase {
     std::vector<std::vector<int>> iperm;           // per scene: original element -> position
     VT* p2 = nullptr;                              // previous CG direction (exact path)
     double* nd = nullptr;                          // per-node dot terms (exact path, reference order)
+    double* pinv = nullptr;                        // PCG: inverted preconditioner block per node (9)
+    VT* zv = nullptr;                              // PCG: preconditioned residual z = P^-1 r
     int *em_ptr = nullptr, *em_nbr = nullptr;      // MeshMatrixMass edges per node (exact path)
     double* em_val = nullptr;
     R* emass = nullptr;                            // per element rho V / 20 (tiled path)
@@ -438,6 +441,7 @@ struct EngineT : EngineBase {
         int na = 0, nb = 0, nc = 0, no = 0, nl = 0, maxit = 0, maxsw = 0, n = 0;
         int maxpart = 0;                // most partial sums the operator reduces for one scene
         bool plane = false;             // some scene has a PlaneForceField
+        bool pcg = false;               // some scene solves with PCGLinearSolver
         hipGraphExec_t gexec = nullptr;
         ~LaunchSet() {
             if (gexec) (void)hipGraphExecDestroy(gexec);
@@ -465,7 +469,7 @@ struct EngineT : EngineBase {
                         inst, partial, tiles_d, ranges_d, ltet, ln_gid, ln_inc_ptr, ln_pslot, sh_gid, sh_ptr,
                         ln_inc, sp0, sp1, pbb, pden, prho, p2, em_ptr, em_nbr, em_val, emass, osc_node,
                         osc_mean, osc_amp, osc_w, osc_phase, kmat32, emass32, r32, p32, q32,
-                        d32, sp32, pden32, prho32, prr, inst_in, sh_of, nd, x0, x32, x0_32, contact, inc_rank};
+                        d32, sp32, pden32, prho32, prr, inst_in, sh_of, nd, x0, x32, x0_32, contact, inc_rank, pinv, zv};
         for (void* ptr : ptrs)
             if (ptr) (void)hipFree(ptr);
     }
@@ -679,6 +683,7 @@ struct EngineT : EngineBase {
         in.first_solve = 1;
         in.stopped = tiled ? 0 : 1;
         in.max_sweeps = s.max_sweeps;
+        in.pcg = s.precond;
         if (one_material(s)) host::material_factors(s.young[0], s.poisson[0], in.ks);
         return in;
     }
@@ -733,6 +738,18 @@ struct EngineT : EngineBase {
         in.lm_x0_set = 0;
         for (int j0 = 0; j0 < n; j0 += kBlock) blocks.push_back(Blk{rec, j0, std::min(n, j0 + kBlock), 0});
         return blocks;
+    }
+    // PCG buffers (per node: the inverted preconditioner block and z), once some scene asks for
+    // PCGLinearSolver; the solution vector starts at 0 for a scene's first solve
+    void alloc_pcg() {
+        if (pinv) return;
+        bool any = false;
+        for (const SceneHost& sc : scenes) any = any || sc.precond != 0;
+        if (!any) return;
+        const int64_t n = tiled ? cap.node : Ntot;
+        pinv = dalloc<double>(9 * (size_t)n);
+        zv = dalloc<VT>(3 * (size_t)n);
+        HIPCHK(hipMemset(zv, 0, 3 * (size_t)n * sizeof(VT)));
     }
     void free_lm(int si) {
         auto it = lmd.find(si);
@@ -845,6 +862,7 @@ struct EngineT : EngineBase {
         p2 = dalloc<VT>(3 * Ntot);
         HIPCHK(hipMemset(p2, 0, 3 * Ntot * sizeof(VT)));
         nd = dalloc<double>(Ntot);
+        alloc_pcg();
         upload(inst, inst_h);
         partial = dalloc<double>(nblk_h.size());
         HIPCHK(hipMemset(partial, 0, std::max<size_t>(1, nblk_h.size()) * sizeof(double)));
@@ -922,6 +940,7 @@ struct EngineT : EngineBase {
         ln_gid = ln_inc_ptr = ln_pslot = sh_gid = sh_ptr = nullptr; ln_inc = nullptr; sp0 = sp1 = nullptr;
         pbb = pden = prho = nullptr; kmat32 = emass32 = nullptr; x32 = x0_32 = nullptr;
         r32 = p32 = q32 = d32 = sp32 = nullptr; pden32 = prho32 = prr = nullptr; inst_in = nullptr; sh_of = nullptr; inc_rank = nullptr;
+        pinv = nullptr; zv = nullptr;
         cap = c;
         if (cap.node >= (int64_t)1 << 30 || cap.inc >= (int64_t)1 << 31 || cap.lnode >= (int64_t)1 << 31 ||
             cap.slot >= (int64_t)kSlotOwner)
@@ -1222,6 +1241,7 @@ struct EngineT : EngineBase {
                 const SceneHost& sc = scenes[id];
                 push_range(eb, eblk_h, sc.eblk_begin, sc.eblk_count);
                 push_range(nb, nblk_h, sc.nblk_begin, sc.nblk_count);
+                ls->pcg = ls->pcg || sc.precond != 0;
                 maxit = std::max(maxit, sc.cg_iter);
             }
             put(ls->a, eb); put(ls->b, nb);
@@ -1242,6 +1262,7 @@ struct EngineT : EngineBase {
                 const int bparts = pl.fins.empty() ? (int)pl.tiles.size() : (int)pl.fins.size();
                 ls->maxpart = std::max({ls->maxpart, bparts, (int)pl.nblks.size()});
                 ls->plane = ls->plane || scenes[id].plane_on;
+                ls->pcg = ls->pcg || scenes[id].precond != 0;
             }
             put(ls->a, ta); put(ls->b, fa); put(ls->c, na);
             ls->na = (int)ta.size(); ls->nb = (int)fa.size(); ls->nc = (int)na.size();
@@ -1280,14 +1301,24 @@ struct EngineT : EngineBase {
         if (cur->na > 0) k_rhs_elem<R, NR, VT><<<cur->na, kBlock, 0, s>>>(ev, eb, inst, x, v, ef, ef2);
         double* ndp = seq_dots ? nd : nullptr;
         if (cur->nb > 0) k_rhs_node<VT><<<cur->nb, kBlock, 0, s>>>(nb, inst, na, ef, ef2, v, f, b, r, xcg, partial, ndp, x);
+        if (cur->pcg) {
+            // PCGLinearSolver: the preconditioner from this step's rotations, then r = b - A x from the
+            // previous solution, z = P^-1 r (CG scenes of the set pass through untouched)
+            if (cur->nb > 0)
+                k_pcg_diag<R, NR, Blk><<<cur->nb, kBlock, 0, s>>>(ev, nb, inst, na.inc_ptr, na.inc, mass, fixed, pinv);
+            if (cur->nl > 0) k_pcg_lm_identity<<<cur->nl, kBlock, 0, s>>>((const Blk*)cur->l, inst, pinv);
+            k_pcg_open<<<(n + 63) / 64, 64, 0, s>>>(cur->scenes, n, inst);
+            if (cur->na > 0) k_cg_elem<R, NR, VT><<<cur->na, kBlock, 0, s>>>(ev, eb, inst, xcg, p2, ef);
+            if (cur->nb > 0) k_pcg_begin<VT><<<cur->nb, kBlock, 0, s>>>(nb, inst, na, ef, xcg, b, r, zv, pinv, partial, ndp);
+        }
         for (int it = 1; it <= cur->maxit; it++) {
             // direction of iteration it in buffer it & 1, the previous one in the other
             VT* p_new = (it & 1) ? p : p2;
             const VT* p_old = (it & 1) ? p2 : p;
-            if (cur->na > 0) k_cg_elem<R, NR, VT><<<cur->na, kBlock, 0, s>>>(ev, eb, inst, r, p_old, ef);
+            if (cur->na > 0) k_cg_elem<R, NR, VT><<<cur->na, kBlock, 0, s>>>(ev, eb, inst, r, p_old, ef, zv);
             if (cur->nb > 0) {
-                k_cg_node<VT><<<cur->nb, kBlock, 0, s>>>(nb, inst, na, ef, r, p_old, p_new, q, partial, ndp);
-                k_cg_update<VT><<<cur->nb, kBlock, 0, s>>>(nb, inst, p_new, q, xcg, r, partial, ndp);
+                k_cg_node<VT><<<cur->nb, kBlock, 0, s>>>(nb, inst, na, ef, r, p_old, p_new, q, partial, ndp, zv);
+                k_cg_update<VT><<<cur->nb, kBlock, 0, s>>>(nb, inst, p_new, q, xcg, r, partial, ndp, pinv, zv);
             }
         }
         if (cur->nb > 0) k_euler_update<VT><<<cur->nb, kBlock, 0, s>>>(nb, inst, fixed, xcg, v, x);
@@ -2324,6 +2355,16 @@ int sofa_amd_scene_add_linear_movement(sofa_amd_engine eng, int scene, int n, co
     s->lm_idx.assign(indices, indices + n);
     s->lm_times.assign(key_times, key_times + n_keys);
     s->lm_moves.assign(movements, movements + 3 * (size_t)n_keys);
+    return SOFA_AMD_OK;
+}
+
+int sofa_amd_scene_set_preconditioner(sofa_amd_engine eng, int scene, int kind) {
+    SceneHost* s = scene_of(eng, scene);
+    if (!s || kind < 0 || kind > 2) return fail(SOFA_AMD_EINVAL, "invalid argument");
+    if (!editable(eng, scene)) return fail(SOFA_AMD_ESTATE, "scene already committed");
+    if (kind != 0 && eng->impl->precision != SOFA_AMD_FP64)
+        return fail(SOFA_AMD_EINVAL, "PCGLinearSolver runs on the FP64 engine");
+    s->precond = kind;
     return SOFA_AMD_OK;
 }
 

@@ -86,6 +86,11 @@ struct InstDev {
     double* lm_x0;
     const double* lm_keys;
     int lm_n, lm_nk, lm_relative, lm_x0_set;
+    // PCGLinearSolver (PCGLinearSolver.inl:178-262) in place of CGLinearSolver: 0 none, 1 with a
+    // JacobiPreconditioner, 2 with a BlockJacobiPreconditioner; b.b of the step and the tolerance
+    // on r.z (tolerance b.b)
+    int pcg, pad_pcg;
+    double bnorm2, pcg_tol;
 };
 
 // which entry of InstDev::dots a k_seqdot launch writes

@@ -105,6 +105,27 @@ __device__ __forceinline__ double dot3d(const VT a[3], const VT b[3]) {
     return r;
 }
 
+// z = P^-1 r with P^-1 stored as 9 doubles per node (row-major): DiagonalMatrix::mult
+// (z_i = d_i r_i, DiagonalMatrix.h:187-190) or BlockDiagonalMatrix::mul (z_j = sum_i B(i, j) r_i
+// from 0, BlockDiagonalMatrix.h:202-221) -- the preconditioner solve of PCGLinearSolver
+template <typename VT>
+__device__ __forceinline__ void precond_apply(int kind, const double* __restrict__ P, const VT r[3], VT z[3]) {
+    if (kind == 1) {
+#pragma unroll
+        for (int a = 0; a < 3; a++) z[a] = (VT)(P[4 * a] * (double)r[a]);
+    } else {
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            double acc = 0.0;
+#pragma unroll
+            for (int a = 0; a < 3; a++) acc += P[3 * a + j] * (double)r[a];
+            z[j] = (VT)acc;
+        }
+    }
+}
+
+__device__ __forceinline__ int blk_scene(const Blk& b) { return b.inst; }
+
 // CGLinearSolver.inl:86-160 for nb_iter = 1 (rho = r.r = b.b), once |b|^2 is known: the exact
 // path's CG scalars of a scene at the start of a solve
 __device__ __forceinline__ void cg_start(InstDev* ins, double total) {
@@ -227,14 +248,17 @@ __global__ __launch_bounds__(kBlock) void k_rhs_node(const Blk* __restrict__ nbl
         const int64_t o = 3 * (int64_t)n;
         b[o] = bn[0]; b[o + 1] = bn[1]; b[o + 2] = bn[2];
         r[o] = bn[0]; r[o + 1] = bn[1]; r[o + 2] = bn[2];
-        xcg[o] = 0; xcg[o + 1] = 0; xcg[o + 2] = 0;
+        if (!in.pcg) { xcg[o] = 0; xcg[o + 1] = 0; xcg[o + 2] = 0; }   // PCG starts from the last solution
         bb = dot3d(bn, bn);
         if (nd) nd[n] = bb;
     }
     double total;
     const bool last = nd ? last_block_sequential(nd, ins, total)
                          : last_block_reduce(block_sum(bb, sh), partial, ins, bk.pad, sh, total);
-    if (last && threadIdx.x == 0) cg_start(ins, total);
+    if (last && threadIdx.x == 0) {
+        if (ins->pcg) ins->bnorm2 = total;
+        else cg_start(ins, total);
+    }
 }
 
 // the exact path's start of CGLinearSolver::solve on a caller's right-hand side (component call):
@@ -270,11 +294,14 @@ __global__ __launch_bounds__(kBlock) void k_cg_begin(const Blk* __restrict__ nbl
 // ---------------------------------------------------------------------------------------
 template <typename R, int NR, typename VT>
 __global__ __launch_bounds__(kBlock) void k_cg_elem(ElemView<R, NR> ev, const Blk* __restrict__ eblk,
-                                                     const InstDev* __restrict__ inst, const VT* __restrict__ r,
-                                                     const VT* __restrict__ p_old, VT* __restrict__ ef) {
+                                                     const InstDev* __restrict__ inst, const VT* __restrict__ r_,
+                                                     const VT* __restrict__ p_old, VT* __restrict__ ef,
+                                                     const VT* __restrict__ z = nullptr) {
     const Blk bk = eblk[blockIdx.x];
     const InstDev& in = inst[bk.inst];
     if (in.stopped) return;
+    // PCG: the direction is built from the preconditioned residual z (w = z + beta w)
+    const VT* __restrict__ r = (in.pcg && z) ? z : r_;
     const int64_t e = (int64_t)bk.begin + threadIdx.x;
     if (e >= bk.end) return;
     const int4 t = ev.tet[e];
@@ -307,13 +334,15 @@ __global__ __launch_bounds__(kBlock) void k_cg_elem(ElemView<R, NR> ev, const Bl
 template <typename VT>
 __global__ __launch_bounds__(kBlock) void k_cg_node(const Blk* __restrict__ nblk, InstDev* __restrict__ inst,
                                                      NodeArrays na, const VT* __restrict__ ef,
-                                                     const VT* __restrict__ r, const VT* __restrict__ p_old,
+                                                     const VT* __restrict__ r_, const VT* __restrict__ p_old,
                                                      VT* __restrict__ p, VT* __restrict__ q,
-                                                     double* __restrict__ partial, double* __restrict__ nd) {
+                                                     double* __restrict__ partial, double* __restrict__ nd,
+                                                     const VT* __restrict__ z = nullptr) {
     __shared__ double sh[kBlock];
     const Blk bk = nblk[blockIdx.x];
     InstDev* ins = &inst[bk.inst];
     if (ins->stopped) return;
+    const VT* __restrict__ r = (ins->pcg && z) ? z : r_;
     const int n = bk.begin + threadIdx.x;
     double pq = 0.0;
     if (n < bk.end) {
@@ -355,8 +384,12 @@ __global__ __launch_bounds__(kBlock) void k_cg_node(const Blk* __restrict__ nblk
     const bool last = nd ? last_block_sequential(nd, ins, den)
                          : last_block_reduce(block_sum(pq, sh), partial, ins, bk.pad, sh, den);
     if (last && threadIdx.x == 0) {
-        // CGLinearSolver.inl:207-265
         ins->den = den;
+        if (ins->pcg) {                     // PCGLinearSolver.inl:222-224: alpha = r.z / w.s
+            ins->alpha = ins->rho / den;
+            return;
+        }
+        // CGLinearSolver.inl:207-265
         const bool exempt = (ins->iter == 1 && ins->first_solve);
         if (den != 0.0) {
             if (fabs(den) <= ins->thr && !exempt) {
@@ -374,7 +407,9 @@ template <typename VT>
 __global__ __launch_bounds__(kBlock) void k_cg_update(const Blk* __restrict__ nblk, InstDev* __restrict__ inst,
                                                        const VT* __restrict__ p, const VT* __restrict__ q,
                                                        VT* __restrict__ xcg, VT* __restrict__ r,
-                                                       double* __restrict__ partial, double* __restrict__ nd) {
+                                                       double* __restrict__ partial, double* __restrict__ nd,
+                                                       const double* __restrict__ pinv = nullptr,
+                                                       VT* __restrict__ z = nullptr) {
     __shared__ double sh[kBlock];
     const Blk bk = nblk[blockIdx.x];
     InstDev* ins = &inst[bk.inst];
@@ -392,7 +427,14 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(const Blk* __restrict__ nb
             rn[k] = r[o + k] + q[o + k] * ma;
             r[o + k] = rn[k];
         }
-        rr = dot3d(rn, rn);
+        if (ins->pcg && pinv) {             // s = P^-1 r; r_norm = r.s (PCGLinearSolver.inl:229-244)
+            VT zn[3];
+            precond_apply(ins->pcg, pinv + 9 * (int64_t)n, rn, zn);
+            z[o] = zn[0]; z[o + 1] = zn[1]; z[o + 2] = zn[2];
+            rr = dot3d(rn, zn);
+        } else {
+            rr = dot3d(rn, rn);
+        }
         if (nd) nd[n] = rr;
     }
     double rho;
@@ -404,6 +446,15 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(const Blk* __restrict__ nb
         ins->rho = rho;
         const int it = ins->iter + 1;
         ins->iter = it;
+        if (ins->pcg) {                     // while (iter <= maxIter && r_norm > tol)
+            ins->err = ins->bnorm2 != 0.0 ? rho / ins->bnorm2 : 0.0;
+            if (it > ins->max_iter || !(rho > ins->pcg_tol)) {
+                ins->stopped = 1; ins->end_reason = it > ins->max_iter ? 0 : 1; ins->last_iters = it;
+            } else {
+                ins->beta = rho / ins->rho_1;
+            }
+            return;
+        }
         if (it > ins->max_iter) {
             ins->stopped = 1; ins->end_reason = 0; ins->last_iters = ins->max_iter;
         } else {
@@ -957,6 +1008,82 @@ static __global__ __launch_bounds__(kBlock) void k_von_mises_node(const int* __r
 // assembled stiffness (TetrahedronFEMForceField::addKToMatrix / buildStiffnessMatrix,
 // .inl:1921-1999, with computeStiffnessMatrix, :212-232)
 // ---------------------------------------------------------------------------------------
+// block (n1, n2) of R J K J^T R^T of element e (computeStiffnessMatrix, TetrahedronFEMForceField
+// .inl:212-232, with the last addForce's rotation): J's rows of n1 and n2 only, with R =
+// rotations[e], JK = J(n1 rows) K, B = JK J(n2 rows)^T, S = R B, block = S R^T, every sum in the
+// reference's dense-product order (zero terms of its 12x12 products leave its sums unchanged)
+template <typename R, int NR>
+__device__ inline void elem_stiffness_block(const ElemView<R, NR>& ev, int64_t e, int n1, int n2, double blk[9]) {
+#pragma clang fp contract(off)
+    R gr[12], rcr[9];
+    jgroups<R, NR>(ev, e, gr, rcr);
+    double rot[9];
+#pragma unroll
+    for (int k = 0; k < 9; k++) rot[k] = (double)ev.rot[k * ev.E + e];
+    const double k00 = (double)ev.kmat[e], k01 = (double)ev.kmat[ev.E + e], k33 = (double)ev.kmat[2 * ev.E + e];
+    double Km[6][6];
+#pragma unroll
+    for (int l = 0; l < 6; l++)
+#pragma unroll
+        for (int m = 0; m < 6; m++)
+            Km[l][m] = (l < 3 && m < 3) ? (l == m ? k00 : k01) : ((l == m) ? k33 : 0.0);
+    // J rows of a vertex (computeStrainDisplacement's placement of its three groups)
+    double J1[3][6], J2[3][6];
+#pragma unroll
+    for (int q = 0; q < 3; q++)
+#pragma unroll
+        for (int l = 0; l < 6; l++) { J1[q][l] = 0.0; J2[q][l] = 0.0; }
+    {
+        const double a0 = (double)gr[3 * n1], a1 = (double)gr[3 * n1 + 1], a2 = (double)gr[3 * n1 + 2];
+        J1[0][0] = a0; J1[0][3] = a1; J1[0][5] = a2;
+        J1[1][3] = a0; J1[1][1] = a1; J1[1][4] = a2;
+        J1[2][5] = a0; J1[2][4] = a1; J1[2][2] = a2;
+        const double b0 = (double)gr[3 * n2], b1 = (double)gr[3 * n2 + 1], b2 = (double)gr[3 * n2 + 2];
+        J2[0][0] = b0; J2[0][3] = b1; J2[0][5] = b2;
+        J2[1][3] = b0; J2[1][1] = b1; J2[1][4] = b2;
+        J2[2][5] = b0; J2[2][4] = b1; J2[2][2] = b2;
+    }
+    double JK[3][6];
+#pragma unroll
+    for (int q = 0; q < 3; q++)
+#pragma unroll
+        for (int m = 0; m < 6; m++) {
+            double a = 0.0;
+#pragma unroll
+            for (int l = 0; l < 6; l++) a += J1[q][l] * Km[l][m];
+            JK[q][m] = a;
+        }
+    double B[3][3];
+#pragma unroll
+    for (int q1 = 0; q1 < 3; q1++)
+#pragma unroll
+        for (int q2 = 0; q2 < 3; q2++) {
+            double a = 0.0;
+#pragma unroll
+            for (int m = 0; m < 6; m++) a += JK[q1][m] * J2[q2][m];
+            B[q1][q2] = a;
+        }
+    double S[3][3];
+#pragma unroll
+    for (int q1 = 0; q1 < 3; q1++)
+#pragma unroll
+        for (int p = 0; p < 3; p++) {
+            double a = 0.0;
+#pragma unroll
+            for (int r = 0; r < 3; r++) a += rot[3 * q1 + r] * B[r][p];
+            S[q1][p] = a;
+        }
+#pragma unroll
+    for (int q1 = 0; q1 < 3; q1++)
+#pragma unroll
+        for (int q2 = 0; q2 < 3; q2++) {
+            double a = 0.0;
+#pragma unroll
+            for (int p = 0; p < 3; p++) a += S[q1][p] * rot[3 * q2 + p];
+            blk[3 * q1 + q2] = a;
+        }
+}
+
 // One thread per 3x3 block (a, b) of the scene's matrix: the block is the sum, in ascending
 // element id (the element loop's order), of -(R J K J^T R^T)(n1, n2) * k over the elements and
 // vertex pairs (n1, n2) with node a at n1 and b at n2 -- slots[] lists them per block as
@@ -979,76 +1106,159 @@ __global__ __launch_bounds__(kBlock) void k_stiffness_blocks(ElemView<R, NR> ev,
         const int slot = slots[s];
         const int64_t e = slot >> 4;
         const int n1 = (slot >> 2) & 3, n2 = slot & 3;
-        R gr[12], rcr[9];
-        jgroups<R, NR>(ev, e, gr, rcr);
-        double rot[9];
+        double blk[9];
+        elem_stiffness_block<R, NR>(ev, e, n1, n2, blk);
 #pragma unroll
-        for (int k = 0; k < 9; k++) rot[k] = (double)ev.rot[k * ev.E + e];
-        const double k00 = (double)ev.kmat[e], k01 = (double)ev.kmat[ev.E + e], k33 = (double)ev.kmat[2 * ev.E + e];
-        double Km[6][6];
-#pragma unroll
-        for (int l = 0; l < 6; l++)
-#pragma unroll
-            for (int m = 0; m < 6; m++)
-                Km[l][m] = (l < 3 && m < 3) ? (l == m ? k00 : k01) : ((l == m) ? k33 : 0.0);
-        // J rows of a vertex (computeStrainDisplacement's placement of its three groups)
-        double J1[3][6], J2[3][6];
-#pragma unroll
-        for (int q = 0; q < 3; q++)
-#pragma unroll
-            for (int l = 0; l < 6; l++) { J1[q][l] = 0.0; J2[q][l] = 0.0; }
-        {
-            const double a0 = (double)gr[3 * n1], a1 = (double)gr[3 * n1 + 1], a2 = (double)gr[3 * n1 + 2];
-            J1[0][0] = a0; J1[0][3] = a1; J1[0][5] = a2;
-            J1[1][3] = a0; J1[1][1] = a1; J1[1][4] = a2;
-            J1[2][5] = a0; J1[2][4] = a1; J1[2][2] = a2;
-            const double b0 = (double)gr[3 * n2], b1 = (double)gr[3 * n2 + 1], b2 = (double)gr[3 * n2 + 2];
-            J2[0][0] = b0; J2[0][3] = b1; J2[0][5] = b2;
-            J2[1][3] = b0; J2[1][1] = b1; J2[1][4] = b2;
-            J2[2][5] = b0; J2[2][4] = b1; J2[2][2] = b2;
-        }
-        double JK[3][6];
-#pragma unroll
-        for (int q = 0; q < 3; q++)
-#pragma unroll
-            for (int m = 0; m < 6; m++) {
-                double a = 0.0;
-#pragma unroll
-                for (int l = 0; l < 6; l++) a += J1[q][l] * Km[l][m];
-                JK[q][m] = a;
-            }
-        double B[3][3];
-#pragma unroll
-        for (int q1 = 0; q1 < 3; q1++)
-#pragma unroll
-            for (int q2 = 0; q2 < 3; q2++) {
-                double a = 0.0;
-#pragma unroll
-                for (int m = 0; m < 6; m++) a += JK[q1][m] * J2[q2][m];
-                B[q1][q2] = a;
-            }
-        double S[3][3];
-#pragma unroll
-        for (int q1 = 0; q1 < 3; q1++)
-#pragma unroll
-            for (int p = 0; p < 3; p++) {
-                double a = 0.0;
-#pragma unroll
-                for (int r = 0; r < 3; r++) a += rot[3 * q1 + r] * B[r][p];
-                S[q1][p] = a;
-            }
-#pragma unroll
-        for (int q1 = 0; q1 < 3; q1++)
-#pragma unroll
-            for (int q2 = 0; q2 < 3; q2++) {
-                double a = 0.0;
-#pragma unroll
-                for (int p = 0; p < 3; p++) a += S[q1][p] * rot[3 * q2 + p];
-                acc[3 * q1 + q2] += (-a) * kf;
-            }
+        for (int i = 0; i < 9; i++) acc[i] += (-blk[i]) * kf;
     }
 #pragma unroll
     for (int i = 0; i < 9; i++) out[9 * (int64_t)b + i] = acc[i];
+}
+
+// ---------------------------------------------------------------------------------------
+// PCGLinearSolver with a Jacobi / block-Jacobi preconditioner (PCGLinearSolver.inl:178-262,
+// JacobiPreconditioner.inl:43-52, BlockJacobiPreconditioner.inl:41-52)
+// ---------------------------------------------------------------------------------------
+// node n's preconditioner block, inverted: mF (vertex mass) I, then kF (-R J K J^T R^T)(v, v) of
+// its elements in ascending element id (MatrixLinearSystem's assembly: mass, then the FEM's
+// element loop, each value scaled by its factor); the identity for a fixed node (Dirichlet row,
+// MatrixLinearSystem.inl:1177-1189); then DiagonalMatrix::invert (1 / d, off-diagonal dropped) or
+// Mat<3,3>::invert (invertMatrix, Mat.h:1172-1192, with the determinant of Mat.h:989-997)
+template <typename R, int NR>
+__device__ inline void pcg_block(const InstDev& in, const ElemView<R, NR>& ev, const int* __restrict__ inc_ptr,
+                                 const int* __restrict__ inc, const double* __restrict__ mass,
+                                 const int8_t* __restrict__ fixed, int n, double* __restrict__ out) {
+#pragma clang fp contract(off)
+    double B[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    if (fixed[n] & 1) {
+        B[0] = B[4] = B[8] = 1.0;
+    } else {
+        double m = in.mass_kind == kMassUniform ? in.vertex_mass : mass[n];
+        if (in.mass_kind == kMassMatrixLumped) m = m * kLumpTet;
+        const double mm = m * in.m_cg;
+        B[0] += mm; B[4] += mm; B[8] += mm;
+        for (int s = inc_ptr[n]; s < inc_ptr[n + 1]; s++) {
+            const int64_t e = inc[s] >> 2;
+            const int v = inc[s] & 3;
+            double blk[9];
+            elem_stiffness_block<R, NR>(ev, e, v, v, blk);
+#pragma unroll
+            for (int i = 0; i < 9; i++) B[i] += (-blk[i]) * in.k_cg;
+        }
+    }
+    if (in.pcg == 1) {
+#pragma unroll
+        for (int i = 0; i < 9; i++) out[i] = (i % 4 == 0) ? 1.0 / B[i] : 0.0;
+        return;
+    }
+    auto F = [&](int r, int c) { return B[3 * r + c]; };
+    const double det = F(0, 0) * F(1, 1) * F(2, 2) + F(1, 0) * F(2, 1) * F(0, 2) + F(2, 0) * F(0, 1) * F(1, 2)
+                       - F(0, 0) * F(2, 1) * F(1, 2) - F(1, 0) * F(0, 1) * F(2, 2) - F(2, 0) * F(1, 1) * F(0, 2);
+    out[0] = (F(1, 1) * F(2, 2) - F(2, 1) * F(1, 2)) / det;
+    out[3] = (F(1, 2) * F(2, 0) - F(2, 2) * F(1, 0)) / det;
+    out[6] = (F(1, 0) * F(2, 1) - F(2, 0) * F(1, 1)) / det;
+    out[1] = (F(2, 1) * F(0, 2) - F(0, 1) * F(2, 2)) / det;
+    out[4] = (F(2, 2) * F(0, 0) - F(0, 2) * F(2, 0)) / det;
+    out[7] = (F(2, 0) * F(0, 1) - F(0, 0) * F(2, 1)) / det;
+    out[2] = (F(0, 1) * F(1, 2) - F(1, 1) * F(0, 2)) / det;
+    out[5] = (F(0, 2) * F(1, 0) - F(1, 2) * F(0, 0)) / det;
+    out[8] = (F(0, 0) * F(1, 1) - F(1, 0) * F(0, 1)) / det;
+}
+
+// every node of the PCG scenes of the launch set: its inverted preconditioner block (the
+// rotations of this step's addForce)
+template <typename R, int NR, typename BLK>
+__global__ __launch_bounds__(kBlock) void k_pcg_diag(ElemView<R, NR> ev, const BLK* __restrict__ nblk,
+                                                      const InstDev* __restrict__ inst, const int* __restrict__ inc_ptr,
+                                                      const int* __restrict__ inc, const double* __restrict__ mass,
+                                                      const int8_t* __restrict__ fixed, double* __restrict__ pinv) {
+    const BLK bk = nblk[blockIdx.x];
+    const InstDev& in = inst[blk_scene(bk)];
+    if (!in.pcg) return;
+    for (int n = bk.begin + threadIdx.x; n < bk.end; n += kBlock)
+        pcg_block<R, NR>(in, ev, inc_ptr, inc, mass, fixed, n, pinv + 9 * (int64_t)n);
+}
+
+// LinearMovementProjectiveConstraint's indices are Dirichlet rows of the assembled matrix
+// (applyConstraint(ZeroDirichletCondition*)) whatever the time: the identity block
+static __global__ __launch_bounds__(kBlock) void k_pcg_lm_identity(const Blk* __restrict__ lblk,
+                                                                    const InstDev* __restrict__ inst,
+                                                                    double* __restrict__ pinv) {
+    const Blk bk = lblk[blockIdx.x];
+    const int i = bk.begin + threadIdx.x;
+    const InstDev& in = inst[bk.inst];
+    if (i >= bk.end || !in.pcg) return;
+    double* P = pinv + 9 * (int64_t)in.lm_node[i];
+#pragma unroll
+    for (int k = 0; k < 9; k++) P[k] = (k % 4 == 0) ? 1.0 : 0.0;
+}
+
+// the PCG scenes of the launch set start a solve: iteration 1, from the solution vector
+static __global__ void k_pcg_open(const int* __restrict__ scenes, int n, InstDev* __restrict__ inst) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    InstDev* in = &inst[scenes[i]];
+    if (!in->pcg) return;
+    in->stopped = 0;
+    in->iter = 1;
+    in->end_reason = 0;
+    in->pcg_tol = in->tol * in->bnorm2;      // tol = tolerance b.b
+}
+
+// r = M x (the operator on the solution vector, element contributions from k_cg_elem at
+// iteration 1 reading x), r = -r + b (cgstep_beta(r, b, -1): r *= -1; r += b), w = P^-1 r,
+// r_norm = r.w, the loop's first test (PCGLinearSolver.inl:193-215); w goes to z, from which
+// iteration 1's direction is taken
+template <typename VT>
+__global__ __launch_bounds__(kBlock) void k_pcg_begin(const Blk* __restrict__ nblk, InstDev* __restrict__ inst,
+                                                       NodeArrays na, const VT* __restrict__ ef,
+                                                       const VT* __restrict__ xcg, const VT* __restrict__ b,
+                                                       VT* __restrict__ r, VT* __restrict__ z,
+                                                       const double* __restrict__ pinv, double* __restrict__ partial,
+                                                       double* __restrict__ nd) {
+    __shared__ double sh[kBlock];
+    const Blk bk = nblk[blockIdx.x];
+    InstDev* ins = &inst[bk.inst];
+    if (!ins->pcg || ins->stopped) return;
+    const int n = bk.begin + threadIdx.x;
+    double rz = 0.0;
+    if (n < bk.end) {
+        const InstDev& in = *ins;
+        const int64_t o = 3 * (int64_t)n;
+        VT xn[3];
+        load3(xcg, n, xn[0], xn[1], xn[2]);
+        VT qn[3] = {0, 0, 0};
+        if (in.m_cg != 0.0) {
+            mass_mdx(in, na, n, in.m_cg, xn, qn);
+            if (in.mass_kind == kMassMatrix)
+                mass_edges(na, n, in.m_cg, [&](int j, VT d[3]) { load3(xcg, j, d[0], d[1], d[2]); }, qn);
+        }
+        const int s0 = na.inc_ptr[n], s1 = na.inc_ptr[n + 1];
+        for (int s = s0; s < s1; s++) {
+            const int64_t slot = na.inc[s];
+            qn[0] -= ef[3 * slot]; qn[1] -= ef[3 * slot + 1]; qn[2] -= ef[3 * slot + 2];
+        }
+        if (in.plane_on && na.contact[n]) plane_dforce(in, in.k_cg, xn, qn);
+        if (na.fixed[n]) { qn[0] = 0; qn[1] = 0; qn[2] = 0; }
+        VT rn[3], zn[3];
+#pragma unroll
+        for (int k = 0; k < 3; k++) { VT t = qn[k]; t *= (VT)-1.0; t += b[o + k]; rn[k] = t; r[o + k] = t; }
+        precond_apply(in.pcg, pinv + 9 * (int64_t)n, rn, zn);
+        z[o] = zn[0]; z[o + 1] = zn[1]; z[o + 2] = zn[2];
+        rz = dot3d(rn, zn);
+        if (nd) nd[n] = rz;
+    }
+    double rho;
+    const bool last = nd ? last_block_sequential(nd, ins, rho)
+                         : last_block_reduce(block_sum(rz, sh), partial, ins, bk.pad, sh, rho);
+    if (last && threadIdx.x == 0) {
+        ins->rho = rho;
+        ins->rho_1 = 0.0;
+        ins->err = ins->bnorm2 != 0.0 ? rho / ins->bnorm2 : 0.0;
+        if (ins->max_iter < 1 || !(rho > ins->pcg_tol)) {
+            ins->stopped = 1; ins->end_reason = 1; ins->last_iters = 1;
+        }
+    }
 }
 
 }  // namespace sofa_amd

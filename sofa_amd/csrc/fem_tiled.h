@@ -53,6 +53,7 @@ constexpr int kTileOneMaterial = 4;   // the scene has one E and nu: K follows f
 
 struct FinBlk { int scene, begin, end, pslot, bi, pad0, pad1, pad2; };  // shared-node range; bi = index in scene
 struct NodeBlk { int scene, begin, end, pslot, first, pad0, pad1, pad2; };  // node range
+__device__ __forceinline__ int blk_scene(const NodeBlk& b) { return b.scene; }
 
 struct TileNodes {
     const int* __restrict__ gid;        // global node id of each tile-local node

@@ -193,4 +193,6 @@ def add_to_engine(eng, spec):
         eng.add_plane(s, **spec["plane"])
     if "linear_movement" in spec:
         eng.add_linear_movement(s, **spec["linear_movement"])
+    if spec.get("preconditioner"):
+        eng.set_preconditioner(s, spec["preconditioner"])
     return s

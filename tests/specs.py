@@ -23,6 +23,8 @@ def oracle_from_spec(spec, dot_mode):
                          float(osc["phase"][k]))
     if "linear_movement" in spec:
         o.add_linear_movement(**spec["linear_movement"])
+    if spec.get("preconditioner"):
+        o.set_preconditioner(spec["preconditioner"])
     o.set_solver(spec["rs"], spec["rm"], spec["iters"], spec["tol"], spec["thr"])
     if "plane" in spec:
         o.add_plane(**spec["plane"])

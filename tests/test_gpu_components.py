@@ -269,3 +269,53 @@ def test_linear_movement_component_calls():
     eng.sync()
     assert np.array_equal(ad.numpy(), o.project(0, a, 0.0))
     eng.close()
+
+
+def _pcg_spec(which, kind):
+    from sofa_amd import workloads
+    if which == "beam":
+        s = workloads.cfg0()
+        rng = np.random.default_rng(17)
+        s["E"] = 10 ** rng.uniform(4.0, 6.0, len(s["tets"]))     # stiffness spread over two decades
+        s["iters"], s["tol"] = 60, 1e-12
+        n = 30
+    else:
+        # the ellipsoid recipe at 12 mm (consistent MeshMatrixMass, per-element materials), the
+        # driven face at rest so that no sin / cos enters
+        s = workloads.cfg2(pitch=0.012, amplitude=0.0)
+        s["iters"], s["tol"] = 200, 1e-16
+        n = 12
+    s["preconditioner"] = kind
+    return s, n
+
+
+@pytest.mark.parametrize("order", [0, 1])
+@pytest.mark.parametrize("kind", [1, 2])
+@pytest.mark.parametrize("which", ["beam", "ellipsoid"])
+def test_pcg_bitexact(which, kind, order):
+    """PCGLinearSolver with a Jacobi (1) / block-Jacobi (2) preconditioner on the FP64 engine:
+    positions, velocities and the last solve's iteration count equal the oracle's bit for bit, in
+    the blocked (0) and the reference's sequential (1) dot-product order, over steps whose solves
+    start from the previous step's solution."""
+    from sofa_amd import workloads
+    capi = _capi()
+    print(f"case: pcg {which} kind {kind} order {order}", flush=True)
+    spec, nsteps = _pcg_spec(which, kind)
+    eng = capi.Engine(0, capi.FP64)
+    if order == 1:
+        eng.set_dot_order(True)
+    sid = workloads.add_to_engine(eng, spec)
+    eng.commit()
+    eng.set_graphs(True)
+    eng.step([sid], nsteps)
+    x, v, _ = eng.read(sid)
+    st = eng.stats(sid)
+    eng.close()
+    o = _oracle(spec, dot_mode=1 if order == 0 else 0)
+    o.step(spec["dt"], nsteps)
+    so = o.stats()
+    print(f"  last solve: {st['iterations']} iterations (oracle {so['iterations']}), r.z/b.b {st['err']:.3e}", flush=True)
+    assert st["iterations"] == so["iterations"]
+    assert np.array_equal(x, o.get("x")), float(np.max(np.abs(x - o.get("x"))))
+    assert np.array_equal(v, o.get("v"))
+    assert np.max(np.abs(x - spec["pos"])) > 1e-5
