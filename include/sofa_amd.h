@@ -241,11 +241,11 @@ int sofa_amd_engine_set_graphs(sofa_amd_engine eng, int enable);
 /* A merged step in concurrent branches: the stepped scenes are split into `branches` groups
  * (largest node counts dealt round-robin), each advanced by its own launch sequence on its own
  * stream, forked from and joined back into the step's stream, so that one group's latency-bound
- * phases overlap the others' kernels.  0 (default): 2 on FP64_TILED engines in the reference's
- * summation order, whose node-after-node dot products are single-wave chains, else 1.  At most
- * 8, and at most one branch per two scenes.  Branches only run concurrently on distinct hardware
- * queues (HIP's GPU_MAX_HW_QUEUES, 4 by default, two of them taken by the runtime's own
- * streams).  Results do not depend on it (scenes are
+ * phases overlap the others' kernels.  0 (default): on FP64_TILED engines in the reference's
+ * summation order, whose node-after-node dot products are single-wave chains, one branch per
+ * hardware queue the runtime leaves free (HIP's GPU_MAX_HW_QUEUES, 4 unless set, minus the two
+ * its own streams hold), at most 3; else 1.  At most 8, and at most one branch per two scenes.
+ * Branches only run concurrently on distinct hardware queues.  Results do not depend on it (scenes are
  * independent).  No reference counterpart. */
 int sofa_amd_engine_set_branches(sofa_amd_engine eng, int branches);
 
