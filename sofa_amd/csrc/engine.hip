@@ -882,15 +882,14 @@ struct EngineT : EngineBase {
     // on another host thread than the one stepping the engine
     void build_tiled(const SceneHost& s, int si, TiledHost& th) const {
         const int nE = (int)(s.tets.size() / 4), n = s.n_nodes;
-        th.plan = plan_tiles(n, s.rest.data(), s.tets.data(), nE, kTileElems, kTileNodes);
         if (exact_tiled()) {
             // the consistent mass's edge terms ride on the element contributions in the tiles,
             // which is not the reference's order (its edge loop runs before the FEM)
             if (s.mass_kind == SOFA_AMD_MASS_MESHMATRIX)
                 throw std::invalid_argument("the reference summation order on a tiled engine needs a lumped mass "
                                             "(UniformMass, DiagonalMass, lumped MeshMatrixMass)");
-            plan_exact(th.plan, n, kTileElems);
         }
+        th.plan = plan_best(n, s.rest.data(), s.tets.data(), nE, kTileElems, kTileNodes, exact_tiled(), (int)sizeof(VT));
         th.iperm = th.plan.iperm;
         setup_scene(s, si, th.iperm, th.eh);
         th.sh_gid = th.plan.sh_gid;

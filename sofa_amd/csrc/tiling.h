@@ -1,7 +1,9 @@
 // tiling.h -- host planning of element tiles for the LDS-gather kernels (fem_tiled.h).
 //
 // Elements are ordered by the Morton code of their rest centroid in element units (each axis
-// scaled by the median element extent along it; ties by element id) and cut
+// scaled by the median element extent along it; ties by element id), or kept in element-id order
+// (for meshes numbered slab by slab, such as generated grids: there a node's elements fall in one
+// or two consecutive tiles, each holding a contiguous part of the node's list), and cut
 // greedily into tiles of at most `max_elems` elements and `max_nodes` distinct nodes.  For
 // each tile-local node the incidence list is sorted by ORIGINAL element id: the order in which
 // the reference's element loop accumulates into that node (TetrahedronFEMForceField.inl:
@@ -47,59 +49,62 @@ static inline uint64_t spread3(uint64_t v) {   // 21 bits -> every third bit
     return v;
 }
 
+enum TileOrder { kOrderMorton = 0, kOrderElementId = 1 };
 static inline TilePlan plan_tiles(int n_nodes, const double* rest, const int32_t* tets, int E, int max_elems,
-                                  int max_nodes) {
+                                  int max_nodes, TileOrder order = kOrderMorton) {
     TilePlan P;
-    // Morton order of element centroids
-    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
-    for (int i = 0; i < n_nodes; i++)
-        for (int k = 0; k < 3; k++) {
-            lo[k] = std::min(lo[k], rest[3 * i + k]);
-            hi[k] = std::max(hi[k], rest[3 * i + k]);
-        }
-    // the curve runs in element units: each axis is scaled by the typical element extent along
-    // it (median over a sample of elements), so that meshes of flat or long elements (a thin
-    // beam cut into many thin layers) still get compact tiles
-    double scale[3] = {1, 1, 1};
-    {
-        const int stride = std::max(1, E / 65536);
-        std::vector<double> w[3];
-        for (int e = 0; e < E; e += stride)
-            for (int k = 0; k < 3; k++) {
-                double a = 1e300, b = -1e300;
-                for (int v = 0; v < 4; v++) {
-                    const double c = rest[3 * tets[4 * e + v] + k];
-                    a = std::min(a, c);
-                    b = std::max(b, c);
-                }
-                w[k].push_back(b - a);
-            }
-        for (int k = 0; k < 3; k++) {
-            if (w[k].empty()) continue;
-            std::nth_element(w[k].begin(), w[k].begin() + w[k].size() / 2, w[k].end());
-            const double m = w[k][w[k].size() / 2];
-            if (m > 0) scale[k] = m;
-        }
-        if (!(scale[0] > 0 && scale[1] > 0 && scale[2] > 0)) scale[0] = scale[1] = scale[2] = 1;
-    }
-    double ext = 0;
-    for (int k = 0; k < 3; k++) ext = std::max(ext, (hi[k] - lo[k]) / scale[k]);
-    if (!(ext > 0)) ext = 1;
-    std::vector<uint64_t> code(E);
-    for (int e = 0; e < E; e++) {
-        uint64_t c = 0;
-        for (int k = 0; k < 3; k++) {
-            double s = 0;
-            for (int v = 0; v < 4; v++) s += rest[3 * tets[4 * e + v] + k];
-            double u = (s * 0.25 - lo[k]) / scale[k] / ext;
-            u = std::min(std::max(u, 0.0), 1.0);
-            c |= spread3((uint64_t)(u * 2097151.0)) << k;
-        }
-        code[e] = c;
-    }
     P.perm.resize(E);
     std::iota(P.perm.begin(), P.perm.end(), 0);
-    std::stable_sort(P.perm.begin(), P.perm.end(), [&](int a, int b) { return code[a] < code[b]; });
+    if (order == kOrderMorton) {
+        // Morton order of element centroids
+        double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+        for (int i = 0; i < n_nodes; i++)
+            for (int k = 0; k < 3; k++) {
+                lo[k] = std::min(lo[k], rest[3 * i + k]);
+                hi[k] = std::max(hi[k], rest[3 * i + k]);
+            }
+        // the curve runs in element units: each axis is scaled by the typical element extent along
+        // it (median over a sample of elements), so that meshes of flat or long elements (a thin
+        // beam cut into many thin layers) still get compact tiles
+        double scale[3] = {1, 1, 1};
+        {
+            const int stride = std::max(1, E / 65536);
+            std::vector<double> w[3];
+            for (int e = 0; e < E; e += stride)
+                for (int k = 0; k < 3; k++) {
+                    double a = 1e300, b = -1e300;
+                    for (int v = 0; v < 4; v++) {
+                        const double c = rest[3 * tets[4 * e + v] + k];
+                        a = std::min(a, c);
+                        b = std::max(b, c);
+                    }
+                    w[k].push_back(b - a);
+                }
+            for (int k = 0; k < 3; k++) {
+                if (w[k].empty()) continue;
+                std::nth_element(w[k].begin(), w[k].begin() + w[k].size() / 2, w[k].end());
+                const double m = w[k][w[k].size() / 2];
+                if (m > 0) scale[k] = m;
+            }
+            if (!(scale[0] > 0 && scale[1] > 0 && scale[2] > 0)) scale[0] = scale[1] = scale[2] = 1;
+        }
+        double ext = 0;
+        for (int k = 0; k < 3; k++) ext = std::max(ext, (hi[k] - lo[k]) / scale[k]);
+        if (!(ext > 0)) ext = 1;
+        std::vector<uint64_t> code(E);
+        for (int e = 0; e < E; e++) {
+            uint64_t c = 0;
+            for (int k = 0; k < 3; k++) {
+                double s = 0;
+                for (int v = 0; v < 4; v++) s += rest[3 * tets[4 * e + v] + k];
+                double u = (s * 0.25 - lo[k]) / scale[k] / ext;
+                u = std::min(std::max(u, 0.0), 1.0);
+                c |= spread3((uint64_t)(u * 2097151.0)) << k;
+            }
+            code[e] = c;
+        }
+        std::stable_sort(P.perm.begin(), P.perm.end(), [&](int a, int b) { return code[a] < code[b]; });
+    }
     P.iperm.assign(E, 0);
     for (int i = 0; i < E; i++) P.iperm[P.perm[i]] = i;
 
@@ -274,6 +279,26 @@ static inline void plan_exact(TilePlan& P, int n_nodes, int max_elems) {
             if (k >= 0) P.ln_pslot[li] = P.sh_ptr[k];
         }
     P.n_slots = P.sh_ptr.back();
+}
+
+// device bytes one operator launch moves for the plan's replication (every tile-local node reads
+// its id, slot, incidence pointer and mask (13 B) and r and p (6 reals); every partial slot is
+// written by a tile and read by the finalize kernel (6 reals))
+static inline double plan_cost(const TilePlan& P, int real_bytes) {
+    return (double)P.ln_gid.size() * (13.0 + 6.0 * real_bytes) + (double)P.n_slots * 6.0 * real_bytes;
+}
+
+// the cheaper of the two element orders for this mesh (exact: the reference-order slot layout
+// of plan_exact, one slot per incidence of a shared node); ties keep the Morton order
+static inline TilePlan plan_best(int n_nodes, const double* rest, const int32_t* tets, int E, int max_elems,
+                                 int max_nodes, bool exact, int real_bytes) {
+    TilePlan a = plan_tiles(n_nodes, rest, tets, E, max_elems, max_nodes, kOrderMorton);
+    TilePlan b = plan_tiles(n_nodes, rest, tets, E, max_elems, max_nodes, kOrderElementId);
+    if (exact) {
+        plan_exact(a, n_nodes, max_elems);
+        plan_exact(b, n_nodes, max_elems);
+    }
+    return plan_cost(b, real_bytes) < plan_cost(a, real_bytes) ? std::move(b) : std::move(a);
 }
 
 }  // namespace sofa_amd
