@@ -408,6 +408,9 @@ struct EngineT : EngineBase {
     int *ln_gid = nullptr, *ln_inc_ptr = nullptr, *ln_pslot = nullptr, *sh_gid = nullptr, *sh_ptr = nullptr;
     int* sh_of = nullptr;                          // per node: index into sh_gid / sh_ptr, or -1
     uint8_t* inc_rank = nullptr;                   // reference order: incidence ranks of shared nodes
+    uint8_t* inc_arank = nullptr;                  // reference order, CG operator: slot offsets (0: folded)
+    uint8_t* sh_acnt = nullptr;                    // reference order, CG operator: slots after the first
+    uint8_t* ln_lead = nullptr;                    // reference order, CG operator: folded incidences per local node
     uint16_t* ln_inc = nullptr;
     VT *sp0 = nullptr, *sp1 = nullptr;
     double *pbb = nullptr, *pden = nullptr, *prho = nullptr;
@@ -470,7 +473,8 @@ struct EngineT : EngineBase {
                         inst, partial, tiles_d, ranges_d, ltet, ln_gid, ln_inc_ptr, ln_pslot, sh_gid, sh_ptr,
                         ln_inc, sp0, sp1, pbb, pden, prho, p2, em_ptr, em_nbr, em_val, emass, osc_node,
                         osc_mean, osc_amp, osc_w, osc_phase, kmat32, emass32, r32, p32, q32,
-                        d32, sp32, pden32, prho32, prr, inst_in, sh_of, nd, x0, x32, x0_32, contact, inc_rank, pinv, zv};
+                        d32, sp32, pden32, prho32, prr, inst_in, sh_of, nd, x0, x32, x0_32, contact, inc_rank, pinv, zv,
+                        inc_arank, sh_acnt, ln_lead};
         for (void* ptr : ptrs)
             if (ptr) (void)hipFree(ptr);
     }
@@ -940,7 +944,7 @@ struct EngineT : EngineBase {
         ln_gid = ln_inc_ptr = ln_pslot = sh_gid = sh_ptr = nullptr; ln_inc = nullptr; sp0 = sp1 = nullptr;
         pbb = pden = prho = nullptr; kmat32 = emass32 = nullptr; x32 = x0_32 = nullptr;
         r32 = p32 = q32 = d32 = sp32 = nullptr; pden32 = prho32 = prr = nullptr; inst_in = nullptr; sh_of = nullptr; inc_rank = nullptr;
-        pinv = nullptr; zv = nullptr;
+        pinv = nullptr; zv = nullptr; inc_arank = nullptr; sh_acnt = nullptr; ln_lead = nullptr;
         cap = c;
         if (cap.node >= (int64_t)1 << 30 || cap.inc >= (int64_t)1 << 31 || cap.lnode >= (int64_t)1 << 31 ||
             cap.slot >= (int64_t)kSlotOwner)
@@ -956,6 +960,9 @@ struct EngineT : EngineBase {
         nd = dalloc<double>(N);
         HIPCHK(hipMemset(nd, 0, N * sizeof(double)));
         inc_rank = dalloc<uint8_t>(std::max<int64_t>(1, cap.inc));
+        inc_arank = dalloc<uint8_t>(std::max<int64_t>(1, cap.inc));
+        sh_acnt = dalloc<uint8_t>(std::max<int64_t>(1, cap.shn));
+        ln_lead = dalloc<uint8_t>(std::max<int64_t>(1, cap.lnode));
         contact = dalloc<int8_t>(N);
         HIPCHK(hipMemset(contact, 0, std::max<int64_t>(1, N)));
         tet = dalloc<int4>(E); rot = dalloc<R>(9 * E); rest = dalloc<R>(NR * E); kmat = dalloc<R>(3 * E);
@@ -1064,6 +1071,7 @@ struct EngineT : EngineBase {
         for (size_t k = 0; k < P.sh_gid.size(); k++) first_slot[P.sh_gid[k]] = P.sh_ptr[k];
         std::vector<TileDesc> tds(T);
         std::vector<int> lg, lp, lps;
+        std::vector<uint8_t> lld;
         lg.reserve(pl.size.lnode); lp.reserve(pl.size.lnode); lps.reserve(pl.size.lnode);
         for (int t = 0; t < T; t++) {
             TileDesc& td = tds[t];
@@ -1081,6 +1089,7 @@ struct EngineT : EngineBase {
                 const bool end = j == P.t_lncount[t];
                 lg.push_back(end ? nb0 : P.ln_gid[li] + nb0);
                 lp.push_back((int)ib0 + P.ln_inc_ptr[li]);
+                if (exact_tiled()) lld.push_back(end ? 0 : P.ln_lead[li]);
                 if (end || P.ln_pslot[li] < 0) {
                     lps.push_back(-1);
                 } else {
@@ -1091,6 +1100,7 @@ struct EngineT : EngineBase {
         }
         put_at(tiles_d, pl.at.tile, tds);
         put_at(ln_gid, pl.at.lnode, lg); put_at(ln_inc_ptr, pl.at.lnode, lp); put_at(ln_pslot, pl.at.lnode, lps);
+        if (exact_tiled()) put_at(ln_lead, pl.at.lnode, lld);
         pl.tiles = tds;
         // shared nodes and their blocks
         const int Sn = (int)th.sh_gid.size();
@@ -1099,7 +1109,11 @@ struct EngineT : EngineBase {
         sg[Sn] = nb0;
         for (int j = 0; j <= Sn; j++) spp[j] = (int)pl.at.slot + th.sh_ptr[j];
         put_at(sh_gid, pl.at.shn, sg); put_at(sh_ptr, pl.at.shn, spp);
-        if (exact_tiled()) put_at(inc_rank, ib0, P.inc_rank);   // slot offsets of shared nodes' incidences
+        if (exact_tiled()) {   // slot offsets of shared nodes' incidences (force pass, CG operator)
+            put_at(inc_rank, ib0, P.inc_rank);
+            put_at(inc_arank, ib0, P.inc_arank);
+            put_at(sh_acnt, pl.at.shn, P.sh_acnt);
+        }
         std::vector<int> of(n + 1, -1);
         for (int j = 0; j < Sn; j++) of[th.sh_gid[j]] = (int)pl.at.shn + j;
         put_at(sh_of, nb0, of);
@@ -1340,7 +1354,7 @@ struct EngineT : EngineBase {
         const TileArgs<R, NR> ta = tile_args();
         const FinBlk* fb = (const FinBlk*)cur->b;
         const NodeBlk* nb = (const NodeBlk*)cur->c;
-        const SharedNodes sn{sh_gid, sh_ptr, sh_of};
+        const SharedNodes sn{sh_gid, sh_ptr, sh_of, sh_acnt};
         if (cur->nl > 0) k_lm_mask<false><<<cur->nl, kBlock, 0, s>>>((const Blk*)cur->l, inst, 0, fixed);
         if (cur->na > 0) k_tile_rhs<R, NR, VT, 512, true><<<cur->na, 512, 0, s>>>(ta, x, v, f, b, r, xcg, sp0, sp1, pbb);
         if (cur->nb > 0)
@@ -1447,7 +1461,7 @@ struct EngineT : EngineBase {
         ta.ranges = ranges_d;
         ta.ev = view();
         ta.ltet = ltet;
-        ta.tn = TileNodes{ln_gid, ln_inc_ptr, ln_inc, ln_pslot, inc_rank};
+        ta.tn = TileNodes{ln_gid, ln_inc_ptr, ln_inc, ln_pslot, inc_rank, inc_arank, ln_lead};
         ta.mass = mass;
         ta.fixed = fixed;
         ta.contact = contact;
@@ -1461,7 +1475,7 @@ struct EngineT : EngineBase {
         const TileArgs<R, NR> ta = tile_args();
         const FinBlk* fb = (const FinBlk*)cur->b;
         const NodeBlk* nb = (const NodeBlk*)cur->c;
-        const SharedNodes sn{sh_gid, sh_ptr, sh_of};
+        const SharedNodes sn{sh_gid, sh_ptr, sh_of, sh_acnt};
         if constexpr (std::is_same<R, double>::value) {
             if (exact_tiled()) { launch_step_exact(n, s); return; }
         }
@@ -1487,7 +1501,7 @@ struct EngineT : EngineBase {
         ta.x_lo = x32;                 // the inner operator recomputes the rotations from these
         const FinBlk* fb = (const FinBlk*)cur->b;
         const NodeBlk* nb = (const NodeBlk*)cur->c;
-        const SharedNodes sn{sh_gid, sh_ptr, sh_of};
+        const SharedNodes sn{sh_gid, sh_ptr, sh_of, sh_acnt};
         InnerCg ic;
         ic.ta.tiles = ta.tiles;
         ic.ta.inst = inst_in;
@@ -1859,7 +1873,7 @@ struct EngineT : EngineBase {
             const TileArgs<R, NR> ta = tile_args();
             const FinBlk* fb = (const FinBlk*)cur->b;
             const NodeBlk* nbk = (const NodeBlk*)cur->c;
-            const SharedNodes sn{sh_gid, sh_ptr, sh_of};
+            const SharedNodes sn{sh_gid, sh_ptr, sh_of, sh_acnt};
             bool exact = false;
             if constexpr (std::is_same<R, double>::value) exact = exact_tiled();
             if (exact) {

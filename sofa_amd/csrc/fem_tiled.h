@@ -64,6 +64,11 @@ struct TileNodes {
                                         // the node's first per-incidence slot)
     const uint8_t* __restrict__ rank;   // reference order: per incidence of a shared node, its
                                         // slot's offset from the node's first slot (else null)
+    const uint8_t* __restrict__ arank;  // reference order, CG operator: 0 for an incidence of the
+                                        // node's leading run (folded into its first slot by this
+                                        // tile), else its slot's offset from the node's first slot
+    const uint8_t* __restrict__ lead;   // reference order, CG operator: per tile-local node, the
+                                        // number of its first incidences this tile folds (else 0)
 };
 constexpr int kSlotOwner = 1 << 30;
 
@@ -71,6 +76,7 @@ struct SharedNodes {
     const int* __restrict__ gid;        // shared (and element-free) nodes
     const int* __restrict__ ptr;        // their partial slots [ptr[i], ptr[i+1]) in tile order
     const int* __restrict__ of;         // per node: its index in gid / ptr, or -1 (tile-owned)
+    const uint8_t* __restrict__ acnt;   // reference order, CG operator: slots after the first
 };
 
 // per-scene slot ranges of the two partial buffers
@@ -502,8 +508,10 @@ __global__ __launch_bounds__(kBlock) void k_fin_rhs(const FinBlk* __restrict__ f
 // rest positions x0 (initLarge, TetrahedronFEMForceField.inl:835-868) and, for a scene of one
 // material, K from the rest volume -- NR (+ 3) reals less per element, 3 more per local node.
 // EXACT: the reference's summation order -- |b|^2 and r.r come from k_seqdot (InstDev::dots),
-// interior nodes write their p.q term to nd, shared nodes their per-incidence contributions
-// (finished by k_fin_exact), and no tile partial of p.q is formed.
+// interior nodes write their p.q term to nd; a shared node's leading run (its first incidences in
+// ascending element id, when they lie in this tile) is summed here after the mass term into the
+// node's first slot, every later incidence goes to a slot of its own (finished by k_fin_exact);
+// no tile partial of p.q is formed.
 template <typename R, int NR, typename VT, int TB = kBlock, int GEO = 0, bool WIDE = true, bool PLANE = true,
           bool EXACT = false>
 __global__ __launch_bounds__(TB) void k_tile_apply(TileArgs<R, NR> a, int iter, const VT* __restrict__ r,
@@ -549,7 +557,7 @@ __global__ __launch_bounds__(TB) void k_tile_apply(TileArgs<R, NR> a, int iter, 
             em[m] = consistent ? a.emass[e] : (R)0;
         }
     }
-    int gid[NPT], psl[NPT];
+    int gid[NPT], psl[NPT], lead[NPT];
     int8_t fx[NPT], ct[NPT];
 #pragma unroll
     for (int m = 0; m < NPT; m++) {
@@ -558,6 +566,7 @@ __global__ __launch_bounds__(TB) void k_tile_apply(TileArgs<R, NR> a, int iter, 
             const int li = td.ln_begin + j;
             gid[m] = a.tn.gid[li];
             psl[m] = a.tn.pslot[li];
+            lead[m] = EXACT ? a.tn.lead[li] : 0;
         }
     }
     InstDev* ins = &a.inst[td.scene];
@@ -709,7 +718,18 @@ __global__ __launch_bounds__(TB) void k_tile_apply(TileArgs<R, NR> a, int iter, 
                 if (EXACT) a.nd[gid[m]] = dot3d(pp, qn);
                 else pq += dot3d(pp, qn);
             } else if (EXACT) {
-                store_incidences(linc, a.tn.rank + td.inc_begin, s0, s1, lc, sp, psl[m]);
+                int s = s0;
+                if (lead[m] > 0) {
+                    // this tile holds the node's leading run: the node's sum starts here (the mass
+                    // term, then the run's contributions one by one) and goes to its first slot
+                    const VT pp[3] = {(VT)lu[3 * j], (VT)lu[3 * j + 1], (VT)lu[3 * j + 2]};
+                    VT acc[3] = {0, 0, 0};
+                    if (in.m_cg != 0.0) mass_mdx(in, NodeArrays{nullptr, nullptr, a.mass, a.fixed}, gid[m], in.m_cg, pp, acc);
+                    s = s0 + lead[m];
+                    sum_incidences<true>(linc, s0, s, lc, acc);
+                    st3(sp, psl[m], acc);
+                }
+                store_incidences(linc, a.tn.arank + td.inc_begin, s, s1, lc, sp, psl[m]);
             } else {
                 VT acc[3] = {0, 0, 0};
                 sum_incidences<true>(linc, s0, s1, lc, acc);
@@ -738,8 +758,9 @@ __global__ __launch_bounds__(TB) void k_tile_apply(TileArgs<R, NR> a, int iter, 
 }
 
 // reference order, after the tile apply: the shared nodes' p = r + beta p, q = (1 + h rm) M p
-// minus their element contributions one by one in ascending element id, plane and projection,
-// and their p.q terms for k_seqdot (the work k_update does for them in tile order otherwise)
+// minus their element contributions one by one in ascending element id (continued from the sum
+// the leading run's tile left in the node's first slot), plane and projection, and their p.q
+// terms for k_seqdot (the work k_update does for them in tile order otherwise)
 template <typename VT>
 __global__ __launch_bounds__(kBlock) void k_fin_exact(const FinBlk* __restrict__ fb_, InstDev* __restrict__ inst,
                                                        SharedNodes sn, const double* __restrict__ mass,
@@ -763,10 +784,12 @@ __global__ __launch_bounds__(kBlock) void k_fin_exact(const FinBlk* __restrict__
         for (int i = 0; i < 3; i++) { VT t = po[i] * vb; t += pp[i]; pp[i] = t; }
     }
     st3(p, g, pp);
-    VT qn[3] = {0, 0, 0};
-    if (in.m_cg != 0.0) mass_mdx(in, NodeArrays{nullptr, nullptr, mass, fixed}, g, in.m_cg, pp, qn);
-    const int s0 = sn.ptr[j], s1 = sn.ptr[j + 1];
-    for (int s = s0; s < s1; s++) {
+    // the node's first slot holds its sum up to the end of its leading run (the mass term and
+    // that run, added by the run's tile); the rest of its contributions follow, one per slot
+    const int s0 = sn.ptr[j], s1 = s0 + 1 + (int)sn.acnt[j];
+    VT qn[3];
+    ld3(sp, s0, qn);
+    for (int s = s0 + 1; s < s1; s++) {
         const int64_t o = 3 * (int64_t)s;
 #pragma unroll
         for (int i = 0; i < 3; i++) qn[i] -= sp[o + i];

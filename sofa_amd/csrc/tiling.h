@@ -36,6 +36,14 @@ struct TilePlan {
     std::vector<int> sh_ptr;        // size(sh_gid) + 1
     std::vector<uint8_t> inc_rank;  // reference order only (plan_exact): per incidence of a shared
                                     // node, its rank in the node's list in ascending element id
+    std::vector<uint8_t> inc_arank; // reference order, CG operator: 0 for an incidence of the node's
+                                    // leading run (folded into the node's first slot by its tile),
+                                    // else its slot's offset from the node's first slot
+    std::vector<uint8_t> sh_acnt;   // reference order, CG operator: per shared node, its slots after
+                                    // the first (the incidences after the leading run)
+    std::vector<uint8_t> ln_lead;   // reference order, CG operator: per tile-local node, how many of
+                                    // its first incidences its tile folds (the node's leading run
+                                    // when the tile holds it, else 0)
     int n_slots = 0;
 };
 
@@ -248,11 +256,18 @@ static inline TilePlan plan_tiles(int n_nodes, const double* rest, const int32_t
 // that it reads each node's contributions as one contiguous run.  A tile writes its element's
 // contribution to the node at ln_pslot (the node's first slot) + inc_rank of the incidence.
 // Interior nodes keep their in-tile sums, which are already in that order.
+//
+// The CG operator folds each shared node's leading run -- the longest prefix of its list that
+// lies in one tile -- in that tile: the tile starts the node's sum (the mass term, then those
+// contributions one by one) and writes it to the node's first slot; the rest of the list takes
+// the following slots, one per incidence.  The finalize kernel continues the sum from the first
+// slot: the same operations in the same order.
 static inline void plan_exact(TilePlan& P, int n_nodes, int max_elems) {
     const int TT = (int)P.t_ebegin.size();
     std::vector<int> shidx(n_nodes, -1);
     for (size_t j = 0; j < P.sh_gid.size(); j++) shidx[P.sh_gid[j]] = (int)j;
-    std::vector<std::vector<std::pair<int, int>>> lists(P.sh_gid.size());   // (original element, incidence)
+    struct Inc { int elem, c, tile, li; bool operator<(const Inc& o) const { return elem < o.elem || (elem == o.elem && c < o.c); } };
+    std::vector<std::vector<Inc>> lists(P.sh_gid.size());   // (original element, incidence, tile, local node)
     for (int ti = 0; ti < TT; ti++) {
         for (int j = 0; j < P.t_lncount[ti]; j++) {
             const int li = P.t_lnbegin[ti] + j;
@@ -260,16 +275,27 @@ static inline void plan_exact(TilePlan& P, int n_nodes, int max_elems) {
             if (k < 0) continue;
             for (int c = P.ln_inc_ptr[li]; c < P.ln_inc_ptr[li + 1]; c++) {
                 const int local_elem = (int)P.ln_inc[c] % max_elems;   // slot = 3 max_elems v + element
-                lists[k].push_back({P.perm[P.t_ebegin[ti] + local_elem], c});
+                lists[k].push_back({P.perm[P.t_ebegin[ti] + local_elem], c, ti, li});
             }
         }
     }
     P.inc_rank.assign(P.ln_inc.size(), 0);
+    P.inc_arank.assign(P.ln_inc.size(), 0);
+    P.sh_acnt.assign(P.sh_gid.size(), 0);
+    P.ln_lead.assign(P.ln_gid.size(), 0);
     P.sh_ptr.assign(1, 0);
-    for (auto& lst : lists) {
+    for (size_t k = 0; k < lists.size(); k++) {
+        auto& lst = lists[k];
         if (lst.size() > 256) throw std::invalid_argument("a node with more than 256 tetrahedra");
         std::sort(lst.begin(), lst.end());
-        for (size_t r = 0; r < lst.size(); r++) P.inc_rank[lst[r].second] = (uint8_t)r;
+        size_t lead = 1;
+        while (lead < lst.size() && lst[lead].tile == lst[0].tile) lead++;
+        for (size_t r = 0; r < lst.size(); r++) {
+            P.inc_rank[lst[r].c] = (uint8_t)r;
+            P.inc_arank[lst[r].c] = (uint8_t)(r < lead ? 0 : r - lead + 1);
+        }
+        P.sh_acnt[k] = (uint8_t)(lst.size() - lead);
+        P.ln_lead[lst[0].li] = (uint8_t)lead;
         P.sh_ptr.push_back(P.sh_ptr.back() + (int)lst.size());
     }
     for (int ti = 0; ti < TT; ti++)
