@@ -244,7 +244,7 @@ int sofa_amd_engine_set_graphs(sofa_amd_engine eng, int enable);
  * phases overlap the others' kernels.  0 (default): on FP64_TILED engines in the reference's
  * summation order, whose node-after-node dot products are single-wave chains, one branch per
  * hardware queue the runtime leaves free (HIP's GPU_MAX_HW_QUEUES, 4 unless set, minus the two
- * its own streams hold), at most 3; else 1.  At most 8, and at most one branch per two scenes.
+ * its own streams hold), at most 4; else 1.  At most 8, and at most one branch per two scenes.
  * Branches only run concurrently on distinct hardware queues.  Results do not depend on it (scenes are
  * independent).  No reference counterpart. */
 int sofa_amd_engine_set_branches(sofa_amd_engine eng, int branches);

@@ -1565,13 +1565,13 @@ struct EngineT : EngineBase {
     hipStream_t aux[kMaxBranches - 1] = {};
     hipEvent_t fork_ev = nullptr, join_ev[kMaxBranches - 1] = {};
     // the default: one branch per hardware queue the runtime's own streams leave free (HIP gives a
-    // process GPU_MAX_HW_QUEUES of them, 4 unless set; two are taken), at most 3 (measured on the
-    // 256-beam mix: 2 branches 29.1 ms/step, 3 26.9, 4 27.2 with 8 queues); branches sharing a
-    // queue run one after the other
+    // process GPU_MAX_HW_QUEUES of them, 4 unless set; two are taken), at most 4 (measured on the
+    // 256-beam mix with 8 queues: 1 branch 30.6 ms/step, 2 26.0, 3 24.0, 4 23.7); branches sharing
+    // a queue run one after the other
     static int default_branches() {
         const char* e = std::getenv("GPU_MAX_HW_QUEUES");
         const int queues = e ? std::atoi(e) : 4;
-        return std::max(1, std::min(3, queues - 2));
+        return std::max(1, std::min(4, queues - 2));
     }
     int branch_count(int n) const {
         const int g = branches > 0 ? branches : (exact_tiled() ? default_branches() : 1);
