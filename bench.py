@@ -145,10 +145,10 @@ def apply_roofline(eng, reps, pe, pv, nrec, E, N, node_extra=0):
             "traffic": None, "us_per_launch": us, "algorithmic_bytes": bytes_alg}
 
 
-def measured_traffic(path):
-    """HBM bytes per launch of the apply kernel on the beam mix, from the rocprofv3 FETCH_SIZE /
-    WRITE_SIZE runs summarised in the profiles folder (counters scaled by a copy of known size);
-    None when that summary is absent."""
+def _profile_json(path, variant):
+    """The json block of a summary in the profiles folder; with per-variant entries (the
+    autotuner's launches of every operator variant are in the same counter runs), the chosen
+    variant's.  None when the summary or the variant's entry is absent."""
     import re
     try:
         text = open(os.path.join(ROOT, path)).read()
@@ -158,35 +158,31 @@ def measured_traffic(path):
     if not m:
         return None
     d = json.loads(m.group(1))
-    return {"bytes": d["traffic_bytes_per_launch"], "source": path, "kernel": d["kernel"]}
-
-
-def measured_issue(path):
-    """Instruction-issue figures of the apply kernel on the beam mix (rocprofv3 SQ counters:
-    VALU instructions per wave, the fraction of cycles the VALUs issue, occupancy), from the
-    summary in the profiles folder; None when it is absent."""
-    import re
-    try:
-        text = open(os.path.join(ROOT, path)).read()
-    except OSError:
-        return None
-    m = re.search(r"```json\s*(\{.*?\})\s*```", text, re.S)
-    if not m:
-        return None
-    d = json.loads(m.group(1))
-    d["source"] = path
+    if "variants" in d:
+        d = d["variants"].get(str(variant))
     return d
 
 
-# SofaCUDA's own benchmark of this component stack (applications/plugins/SofaCUDA/doc/
-# SofaCUDA_benchmarks.csv:15-22, hardware at :37-39): mean seconds per 1000 FP32 steps,
-# as element-steps/s (tetrahedra x 1000 / seconds)
-SOFACUDA_PUBLISHED = {
-    "beam10x10x40": {"tets": 18954, "gpu_rtx2070": 18954 * 1000 / 2.798994, "gpu_gtx1060": 18954 * 1000 / 3.824197,
-                     "cpu_i7_10750h": 18954 * 1000 / 33.56943, "cpu_ryzen7_3800x": 18954 * 1000 / 32.29484},
-    "beam16x16x76": {"tets": 101250, "gpu_rtx2070": 101250 * 1000 / 2.595726, "gpu_gtx1060": 101250 * 1000 / 3.432895,
-                     "cpu_i7_10750h": 101250 * 1000 / 109.0122, "cpu_ryzen7_3800x": 101250 * 1000 / 114.5244},
-}
+def measured_traffic(path, variant=None):
+    """HBM bytes per launch of the apply kernel on the beam mix, from the rocprofv3 FETCH_SIZE /
+    WRITE_SIZE runs summarised in the profiles folder (counters scaled by a copy of known size);
+    None when that summary is absent."""
+    d = _profile_json(path, variant)
+    if d is None:
+        return None
+    return {"bytes": d["traffic_bytes_per_launch"], "source": path, "kernel": d["kernel"]}
+
+
+def measured_issue(path, variant=None):
+    """Instruction-issue figures of the apply kernel on the beam mix (rocprofv3 SQ counters:
+    VALU instructions per wave, the fraction of cycles the VALUs issue, occupancy), from the
+    summary in the profiles folder; None when it is absent."""
+    d = _profile_json(path, variant)
+    if d is None:
+        return None
+    d = dict(d)
+    d["source"] = path
+    return d
 
 
 def sofacuda_scenes(steps):
@@ -504,12 +500,12 @@ def main():
         roof["kernel"] = (f"k_tile_apply<{'float' if head in ('fp32', 'split') else 'double'}> over the 256-beam mix "
                           "(CG operator: addDForce + mass + projection + p update), path " + head)
         roof["autotuned_variant"] = {"chosen": hrow.get("variant"), "us_per_launch": hrow.get("variant_us")}
-        tr = measured_traffic(f"profiles/k_tile_apply_beam_mix_{head}_traffic.md")
+        tr = measured_traffic(f"profiles/k_tile_apply_beam_mix_{head}_traffic.md", hrow.get("variant"))
         if tr is not None:
             roof["traffic"] = tr["bytes"]
             roof["traffic_source"] = tr["source"] + " (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)"
             roof["traffic_over_algorithmic"] = tr["bytes"] / roof["algorithmic_bytes"]
-        iss = measured_issue(f"profiles/k_tile_apply_beam_mix_{head}_issue.md")
+        iss = measured_issue(f"profiles/k_tile_apply_beam_mix_{head}_issue.md", hrow.get("variant"))
         if iss is not None:
             roof["issue"] = iss
     log(f"headline path {head}: {value:.4g} element-steps/s")
