@@ -185,6 +185,17 @@ def measured_issue(path, variant=None):
     return d
 
 
+# SofaCUDA's own benchmark of this component stack (applications/plugins/SofaCUDA/doc/
+# SofaCUDA_benchmarks.csv:15-22, hardware at :37-39): mean seconds per 1000 FP32 steps,
+# as element-steps/s (tetrahedra x 1000 / seconds)
+SOFACUDA_PUBLISHED = {
+    "beam10x10x40": {"tets": 18954, "gpu_rtx2070": 18954 * 1000 / 2.798994, "gpu_gtx1060": 18954 * 1000 / 3.824197,
+                     "cpu_i7_10750h": 18954 * 1000 / 33.56943, "cpu_ryzen7_3800x": 18954 * 1000 / 32.29484},
+    "beam16x16x76": {"tets": 101250, "gpu_rtx2070": 101250 * 1000 / 2.595726, "gpu_gtx1060": 101250 * 1000 / 3.432895,
+                     "cpu_i7_10750h": 101250 * 1000 / 109.0122, "cpu_ryzen7_3800x": 101250 * 1000 / 114.5244},
+}
+
+
 def sofacuda_scenes(steps):
     """SofaCUDA's TetrahedronFEMForceField benchmark scenes (beam10x10x40; beam16x16x76 with its
     PlaneForceField floor), read from SOFA scene files by sofa_amd.scene_xml, each stepped alone
@@ -336,6 +347,7 @@ def cfg2_mixed(steps, err_steps):
         eng.step(ids, err_steps)
         xs[name] = (eng.read(ids[0])[0], eng.stats(ids[0]))
         eng.close()
+    out["preconditioned"] = cfg2_pcg(c, steps=2)
     xr = xs["fp64"][0]
     d = np.linalg.norm(xs["mixed"][0] - xr, axis=1).max()
     out["accuracy"] = {"mesh": f"same recipe at 12 mm pitch: {len(cs['tets'])} tets, {len(cs['pos'])} nodes",
@@ -343,6 +355,36 @@ def cfg2_mixed(steps, err_steps):
                        "eps_disp": float(d / np.linalg.norm(xr - cs["pos"], axis=1).max()),
                        "mixed_last_solve": {k: xs["mixed"][1][k] for k in ("iterations", "ir_sweeps", "err")},
                        "fp64_last_solve": {k: xs["fp64"][1][k] for k in ("iterations", "err")}}
+    return out
+
+
+def cfg2_pcg(c, steps, max_iter=20000):
+    """Iterations to convergence on the full ellipsoid, FP64 engine, each solver to its own
+    stopping rule: CGLinearSolver until |r| <= 1e-8 |b|; PCGLinearSolver with a Jacobi or a
+    block-Jacobi preconditioner (rebuilt every step from the mass and the element stiffness
+    blocks) until r.z <= 1e-16 b.b (its tolerance is on the preconditioned squared norm:
+    PCGLinearSolver.inl:205-262), starting from the previous step's solution.  `steps` steps from
+    rest, eager launches (the solve exits early; a captured step would replay max_iter
+    iterations); the last step's iterations and time reported."""
+    from sofa_amd import capi
+    out = {"rule": "CG: |r|/|b| <= 1e-8; PCG: r.z / b.b <= 1e-16", "max_iterations": max_iter, "steps": steps}
+    for name, kind, tol in (("cg", 0, 1e-8), ("pcg_jacobi", 1, 1e-16), ("pcg_block_jacobi", 2, 1e-16)):
+        spec = dict(c, iters=max_iter, tol=tol)
+        if kind:
+            spec["preconditioner"] = kind
+        eng, ids = make_engine([spec], capi.FP64, graphs=False)
+        stream = bench_stream()
+        eng.step(ids, steps - 1, stream=stream)
+        e0, e1 = _events(2)
+        e0.record(stream)
+        eng.step(ids, 1, stream=stream)
+        e1.record(stream)
+        stream.synchronize()
+        st = eng.stats(ids[0])
+        out[name] = {"iterations": st["iterations"], "err": st["err"], "end": capi.END_REASONS.get(st["end_reason"]),
+                     "ms_per_step": e0.elapsed_time(e1)}
+        eng.close()
+        log(f"cfg2 {name}: {out[name]}")
     return out
 
 
