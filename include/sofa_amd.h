@@ -266,7 +266,8 @@ int sofa_amd_engine_set_dot_order(sofa_amd_engine eng, int order);
  * elements each), rotation and rest coordinates; 3: 256 threads, rotation only.  Variants differ
  * in rounding only (the order of block sums, geometry recomputed in the engine's precision).
  * sofa_amd_engine_autotune times all of them on the committed scenes (`reps` launches each, one
- * launch over every scene) and keeps the fastest; it must run before the first step.  `chosen`
+ * launch over every scene) and keeps the fastest, a later variant replacing an earlier one only
+ * when faster by 1 % or more; it must run before the first step.  `chosen`
  * and `us4` (the 4 timings in microseconds per launch) may be NULL.  No reference counterpart. */
 int sofa_amd_engine_autotune(sofa_amd_engine eng, int reps, int* chosen, float* us4);
 int sofa_amd_engine_set_apply_variant(sofa_amd_engine eng, int variant);

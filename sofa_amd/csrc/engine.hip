@@ -1413,8 +1413,8 @@ struct EngineT : EngineBase {
         }
     }
     // times every variant on the engine's scenes (all of them in one launch, `reps` launches
-    // each, events on `s`) and keeps the fastest; before the first step only (the timing runs
-    // the operator on the CG buffers)
+    // each, events on `s`) and keeps the fastest (by at least 1 %); before the first step only
+    // (the timing runs the operator on the CG buffers)
     int autotune(int reps, hipStream_t s, float* us_out) override {
         if (!tiled) throw std::invalid_argument("autotune needs a tiled engine");
         if (stepped) throw std::logic_error("autotune before the first step");
@@ -1439,7 +1439,9 @@ struct EngineT : EngineBase {
             HIPCHK(hipEventElapsedTime(&ms, e0, e1));
             const float us = 1e3f * ms / (float)std::max(1, reps);
             if (us_out) us_out[v] = us;
-            if (v == 0 || us < best_us) { best = v; best_us = us; }
+            // a variant displaces the kept one only when measurably faster (1 %): within timing
+            // noise the earlier one stays, variant 0 (everything recomputed) reading the fewest bytes
+            if (v == 0 || us < 0.99f * best_us) { best = v; best_us = us; }
         }
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
