@@ -323,8 +323,9 @@ def cfg2_mixed(steps, err_steps):
                        "E 10^U(3,5), nu U(0.40,0.49) per element, MeshMatrixMass 1050 consistent, x<-0.13 fixed, "
                        "x>0.13 oscillating 5 mm at 1 Hz, dt 0.01, CG tolerance 1e-8"}
     keep = dict(c, refinement=dict(inner_iterations=30, inner_tolerance=c["tol"], max_sweeps=3, keep_direction=True))
+    jac = dict(c, preconditioner=1)
     for name, prec, spec in (("mixed", capi.MIXED, c), ("mixed_keep_direction", capi.MIXED, keep),
-                             ("fp64_tiled", capi.FP64_TILED, c)):
+                             ("mixed_jacobi", capi.MIXED, jac), ("fp64_tiled", capi.FP64_TILED, c)):
         eng, ids = make_engine([spec], prec, graphs=True)
         t, lat = time_merged(eng, ids, steps, 2)
         st = eng.stats(ids[0])
@@ -334,6 +335,7 @@ def cfg2_mixed(steps, err_steps):
                                     "err": st["err"], "end": capi.END_REASONS.get(st["end_reason"])}}
         eng.close()
     out["mixed_over_fp64"] = out["mixed"]["value"] / out["fp64_tiled"]["value"]
+    out["mixed_jacobi_over_fp64"] = out["mixed_jacobi"]["value"] / out["fp64_tiled"]["value"]
     out["mixed_keep_direction_over_fp64"] = out["mixed_keep_direction"]["value"] / out["fp64_tiled"]["value"]
     # accuracy at convergence
     cs = workloads.cfg2(pitch=0.012)
@@ -359,20 +361,29 @@ def cfg2_mixed(steps, err_steps):
 
 
 def cfg2_pcg(c, steps, max_iter=20000):
-    """Iterations to convergence on the full ellipsoid, FP64 engine, each solver to its own
-    stopping rule: CGLinearSolver until |r| <= 1e-8 |b|; PCGLinearSolver with a Jacobi or a
+    """Iterations to convergence on the full ellipsoid, each solver to its own stopping rule:
+    on the FP64 engine CGLinearSolver until |r| <= 1e-8 |b|, PCGLinearSolver with a Jacobi or a
     block-Jacobi preconditioner (rebuilt every step from the mass and the element stiffness
     blocks) until r.z <= 1e-16 b.b (its tolerance is on the preconditioned squared norm:
-    PCGLinearSolver.inl:205-262), starting from the previous step's solution.  `steps` steps from
-    rest, eager launches (the solve exits early; a captured step would replay max_iter
-    iterations); the last step's iterations and time reported."""
+    PCGLinearSolver.inl:205-262), starting from the previous step's solution; MIXED with the
+    Jacobi preconditioner in its inner FP32 CG (30 iterations per sweep continuing one search
+    direction) until the FP64 |r| <= 1e-8 |b|.  `steps` steps from rest, eager launches (the
+    solve exits early; a captured step would replay every iteration of the budget); the last
+    step's iterations (MIXED: inner FP32 iterations of the step) and time reported."""
     from sofa_amd import capi
-    out = {"rule": "CG: |r|/|b| <= 1e-8; PCG: r.z / b.b <= 1e-16", "max_iterations": max_iter, "steps": steps}
-    for name, kind, tol in (("cg", 0, 1e-8), ("pcg_jacobi", 1, 1e-16), ("pcg_block_jacobi", 2, 1e-16)):
+    out = {"rule": "CG and MIXED: |r|/|b| <= 1e-8; PCG: r.z / b.b <= 1e-16", "max_iterations": max_iter,
+           "steps": steps}
+    sweeps = max_iter // 30
+    for name, kind, tol, prec in (("cg", 0, 1e-8, capi.FP64), ("pcg_jacobi", 1, 1e-16, capi.FP64),
+                                  ("pcg_block_jacobi", 2, 1e-16, capi.FP64), ("mixed_jacobi", 1, 1e-8, capi.MIXED)):
         spec = dict(c, iters=max_iter, tol=tol)
         if kind:
             spec["preconditioner"] = kind
-        eng, ids = make_engine([spec], capi.FP64, graphs=False)
+        if prec == capi.MIXED:
+            # inner tolerance 0: the preconditioned inner CG's own residual test (in the
+            # preconditioned norm) would end sweeps while the FP64 |r| is still above the bound
+            spec["refinement"] = dict(inner_iterations=30, inner_tolerance=0.0, max_sweeps=sweeps, keep_direction=True)
+        eng, ids = make_engine([spec], prec, graphs=False)
         stream = bench_stream()
         eng.step(ids, steps - 1, stream=stream)
         e0, e1 = _events(2)
@@ -383,6 +394,8 @@ def cfg2_pcg(c, steps, max_iter=20000):
         st = eng.stats(ids[0])
         out[name] = {"iterations": st["iterations"], "err": st["err"], "end": capi.END_REASONS.get(st["end_reason"]),
                      "ms_per_step": e0.elapsed_time(e1)}
+        if prec == capi.MIXED:
+            out[name]["ir_sweeps"] = st["ir_sweeps"]
         eng.close()
         log(f"cfg2 {name}: {out[name]}")
     return out

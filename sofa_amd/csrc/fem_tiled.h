@@ -199,6 +199,8 @@ struct TileArgs {
     const R* __restrict__ emass;        // per element rho V / 20 (consistent MeshMatrixMass), else unused
     float* __restrict__ x_lo;           // MIXED engines: float copy of the positions (written by the RHS)
     double* __restrict__ nd;            // reference order: per-node dot terms for k_seqdot
+    const float* __restrict__ dinv;     // MIXED inner CG with a Jacobi preconditioner: the inverted
+                                        // diagonal per node (z = dinv r); null: plain CG
 };
 
 // consistent-mass edge terms of one element, folded into its nodal contributions (which the
@@ -578,11 +580,13 @@ __global__ __launch_bounds__(TB) void k_tile_apply(TileArgs<R, NR> a, int iter, 
     if (ins->stopped) return;
     // ---- 2. the tile's nodes: r and p (dependent on the node ids), masks
     VT rr_[NPT][3], po_[NPT][3], xx_[NPT][3], x0_[NPT][3];
+    float dv_[NPT][3];
 #pragma unroll
     for (int m = 0; m < NPT; m++) {
         const int j = threadIdx.x + m * TB;
         if (j < td.ln_count) {
             ld3(r, gid[m], rr_[m]);
+            if (a.dinv) ld3(a.dinv, gid[m], dv_[m]);
             if (iter > 1 || cont) ld3(p, gid[m], po_[m]);
             if (GEO >= 1) ld3(x, gid[m], xx_[m]);
             if (GEO == 2) ld3(x0, gid[m], x0_[m]);
@@ -626,13 +630,18 @@ __global__ __launch_bounds__(TB) void k_tile_apply(TileArgs<R, NR> a, int iter, 
             ins->rho_slot[iter & 1] = rho; ins->beta = beta; ins->iter = iter; ins->err = err;
         }
     }
-    // ---- 4. stage p = r + beta p (interior nodes: this tile owns p and writes it back)
+    // ---- 4. stage p = r + beta p, or z + beta p with z = dinv r (interior nodes: this tile owns
+    //         p and writes it back)
     const VT vb = (VT)beta;
 #pragma unroll
     for (int m = 0; m < NPT; m++) {
         const int j = threadIdx.x + m * TB;
         if (j < td.ln_count) {
             VT pp[3] = {rr_[m][0], rr_[m][1], rr_[m][2]};
+            if (a.dinv) {
+#pragma unroll
+                for (int i = 0; i < 3; i++) pp[i] = pp[i] * (VT)dv_[m][i];
+            }
             if (iter > 1 || cont) {
 #pragma unroll
                 for (int i = 0; i < 3; i++) { VT t = po_[m][i] * vb; t += pp[i]; pp[i] = t; }
@@ -849,7 +858,9 @@ __global__ __launch_bounds__(kBlock) void k_fin_apply(const FinBlk* __restrict__
 // the tile partials (which already hold every node's term), the threshold test and alpha
 // (CGLinearSolver.inl:207-265), x += alpha p, r -= alpha q and the r.r partials.
 // EXACT: every node's p and q are final (k_fin_exact finished the shared ones), den is the
-// reference-order p.q of k_seqdot, and each node's r.r term goes to nd
+// reference-order p.q of k_seqdot, and each node's r.r term goes to nd.  dinv (MIXED inner CG
+// with a Jacobi preconditioner): shared nodes' p = z + beta p with z = dinv r, and r.z partials
+// in place of r.r
 template <typename VT, bool EXACT = false>
 __global__ __launch_bounds__(kBlock) void k_update(const NodeBlk* __restrict__ nb_, InstDev* __restrict__ inst,
                                                     const Ranges* __restrict__ ranges, int iter, SharedNodes sn,
@@ -857,12 +868,14 @@ __global__ __launch_bounds__(kBlock) void k_update(const NodeBlk* __restrict__ n
                                                     const int8_t* __restrict__ fixed, VT* __restrict__ p,
                                                     const VT* __restrict__ q, VT* __restrict__ xcg, VT* __restrict__ r,
                                                     const double* __restrict__ pden, double* __restrict__ prho,
-                                                    const int8_t* __restrict__ contact, double* __restrict__ nd = nullptr) {
+                                                    const int8_t* __restrict__ contact, double* __restrict__ nd = nullptr,
+                                                    const float* __restrict__ dinv = nullptr) {
     __shared__ double sh[kBlock / 64];
     const NodeBlk nb = nb_[blockIdx.x];
     InstDev* ins = &inst[nb.scene];
     // node data first (registers), then the reduction
     VT pv[kNodesPerThread][3], qv[kNodesPerThread][3], xv[kNodesPerThread][3], rv[kNodesPerThread][3];
+    float dv[kNodesPerThread][3];
     int shk[kNodesPerThread];
 #pragma unroll
     for (int k = 0; k < kNodesPerThread; k++) {
@@ -871,6 +884,7 @@ __global__ __launch_bounds__(kBlock) void k_update(const NodeBlk* __restrict__ n
             shk[k] = EXACT ? -1 : sn.of[n];
             ld3(xcg, n, xv[k]);
             ld3(r, n, rv[k]);
+            if (dinv) ld3(dinv, n, dv[k]);
             if (shk[k] < 0) {
                 ld3(p, n, pv[k]);
                 ld3(q, n, qv[k]);
@@ -903,6 +917,10 @@ __global__ __launch_bounds__(kBlock) void k_update(const NodeBlk* __restrict__ n
         if (n < nb.end) {
             if (shk[k] >= 0) {
                 VT pp[3] = {rv[k][0], rv[k][1], rv[k][2]};
+                if (dinv) {
+#pragma unroll
+                    for (int i = 0; i < 3; i++) pp[i] = pp[i] * (VT)dv[k][i];
+                }
                 if (iter > 1 || cont) {
 #pragma unroll
                     for (int i = 0; i < 3; i++) { VT t = pv[k][i] * vb; t += pp[i]; pp[i] = t; }
@@ -928,8 +946,14 @@ __global__ __launch_bounds__(kBlock) void k_update(const NodeBlk* __restrict__ n
             }
             st3(xcg, n, xn);
             st3(r, n, rn);
-            if (EXACT) nd[n] = dot3d(rn, rn);
-            else rr += dot3d(rn, rn);
+            if (EXACT) {
+                nd[n] = dot3d(rn, rn);
+            } else if (dinv) {   // r.z
+                const VT zn[3] = {rn[0] * (VT)dv[k][0], rn[1] * (VT)dv[k][1], rn[2] * (VT)dv[k][2]};
+                rr += dot3d(rn, zn);
+            } else {
+                rr += dot3d(rn, rn);
+            }
         }
     }
     if (EXACT) return;

@@ -319,3 +319,45 @@ def test_pcg_bitexact(which, kind, order):
     assert np.array_equal(x, o.get("x")), float(np.max(np.abs(x - o.get("x"))))
     assert np.array_equal(v, o.get("v"))
     assert np.max(np.abs(x - spec["pos"])) > 1e-5
+
+
+@pytest.mark.parametrize("which", ["beam", "ellipsoid"])
+def test_mixed_jacobi_inner_cg(which):
+    """MIXED with PCGLinearSolver's Jacobi preconditioner in its inner FP32 CG (sweeps of 10
+    iterations continuing one search direction, until the FP64 |r|/|b| <= 1e-8) stays within 1e-6
+    relative of the oracle's FP64 Jacobi PCG run to convergence, and needs no more inner iterations
+    than the plain inner CG on the same scene (element stiffness spread over two decades; on the
+    ellipsoid, whose systems take the plain CG thousands of iterations, strictly fewer).  The
+    inner tolerance is 0: the inner CG measures its residual in the preconditioned norm, which
+    can be below the outer test's bound while |r| is not, and would then end every sweep at once.
+    MIXED has no reference counterpart: the comparison point is the converged FP64 solution."""
+    from sofa_amd import workloads
+    capi = _capi()
+    print(f"case: mixed jacobi {which}", flush=True)
+    spec, nsteps = _pcg_spec(which, 1)
+    o = _oracle(dict(spec, iters=5000, tol=1e-26))
+    o.step(spec["dt"], nsteps)
+    xo = o.get("x")
+    totals = {}
+    for kind in (1, 0):
+        s = dict(spec, tol=1e-8, preconditioner=kind,
+                 refinement=dict(inner_iterations=10, inner_tolerance=0.0, max_sweeps=1000, keep_direction=True))
+        eng = capi.Engine(0, capi.MIXED)
+        sid = workloads.add_to_engine(eng, s)
+        eng.commit()
+        its = 0
+        for _ in range(nsteps):
+            eng.step([sid], 1)
+            st = eng.stats(sid)
+            assert st["end_reason"] == 1, (kind, st)
+            its += st["iterations"]
+        x, _, _ = eng.read(sid)
+        eng.close()
+        e = float(np.max(np.linalg.norm(x - xo, axis=1)) / np.max(np.linalg.norm(xo, axis=1)))
+        print(f"  preconditioner {kind}: {its} inner FP32 iterations over {nsteps} steps, eps_rel {e:.3e} "
+              "against the converged FP64 PCG", flush=True)
+        assert e <= 1e-6
+        totals[kind] = its
+    assert totals[1] <= totals[0]
+    if which == "ellipsoid":
+        assert totals[1] < totals[0]
