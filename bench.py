@@ -401,6 +401,45 @@ def cfg2_pcg(c, steps, max_iter=20000):
     return out
 
 
+def cfg2_long(steps, max_iter=20000):
+    """The full ellipsoid stepped `steps` times from rest (the configuration asks for 200) by
+    MIXED with the Jacobi inner CG until the FP64 |r| <= 1e-8 |b| and by the FP64 engine's
+    block-Jacobi PCGLinearSolver until r.z <= 1e-16 b.b, and the two trajectories compared
+    (eps_rel, eps_disp of the positions after the last step).  Off by default: minutes of GPU."""
+    from sofa_amd import capi, workloads
+    c = workloads.cfg2()
+    out = {"steps": steps, "rule": "MIXED + Jacobi: |r|/|b| <= 1e-8; FP64 block-Jacobi PCG: r.z / b.b <= 1e-16"}
+    xs = {}
+    for name, prec, spec in (
+            ("mixed_jacobi", capi.MIXED, dict(c, iters=max_iter, tol=1e-8, preconditioner=1,
+                                              refinement=dict(inner_iterations=30, inner_tolerance=0.0,
+                                                              max_sweeps=max_iter // 30, keep_direction=True))),
+            ("fp64_pcg_block_jacobi", capi.FP64, dict(c, iters=max_iter, tol=1e-16, preconditioner=2))):
+        eng, ids = make_engine([spec], prec, graphs=False)
+        stream = bench_stream()
+        e0, e1 = _events(2)
+        its = []
+        e0.record(stream)
+        for k in range(steps):
+            eng.step(ids, 1, stream=stream)
+            its.append(eng.stats(ids[0])["iterations"])
+            if k % 20 == 0:
+                log(f"cfg2 long {name}: step {k}, {its[-1]} iterations")
+        e1.record(stream)
+        stream.synchronize()
+        st = eng.stats(ids[0])
+        xs[name] = eng.read(ids[0])[0]
+        out[name] = {"ms_per_step": e0.elapsed_time(e1) / steps, "iterations_mean": float(np.mean(its)),
+                     "iterations_max": int(np.max(its)), "last_err": st["err"],
+                     "last_end": capi.END_REASONS.get(st["end_reason"])}
+        eng.close()
+    xr = xs["fp64_pcg_block_jacobi"]
+    d = np.linalg.norm(xs["mixed_jacobi"] - xr, axis=1).max()
+    out["eps_rel"] = float(d / np.linalg.norm(xr, axis=1).max())
+    out["eps_disp"] = float(d / np.linalg.norm(xr - c["pos"], axis=1).max())
+    return out
+
+
 def serve_cfg4(seconds, arrivals, budget_gb):
     """cfg4 job stream (200 jobs/s Poisson, seed 4) over the first `arrivals` seconds of arrivals,
     served for `seconds` of wall time, batched and plain, FP32, every launch set's step replayed
@@ -513,6 +552,9 @@ def main():
                     help="concurrent branches of the merged step on the mix (-1: each engine's default)")
     ap.add_argument("--no-cfg1", action="store_true")
     ap.add_argument("--no-cfg2", action="store_true")
+    ap.add_argument("--cfg2-long", type=int, default=0,
+                    help="also step the full ellipsoid this many steps with MIXED + Jacobi and FP64 PCG "
+                         "to convergence and compare the trajectories (minutes of GPU; 0: off)")
     ap.add_argument("--cfg2-steps", type=int, default=20)
     ap.add_argument("--cfg2-error-steps", type=int, default=30)
     ap.add_argument("--no-serve", action="store_true")
@@ -624,6 +666,8 @@ def main():
         log(f"cfg1: {cfg1}")
 
     cfg2 = None if args.no_cfg2 else cfg2_mixed(args.cfg2_steps, args.cfg2_error_steps)
+    if cfg2 is not None and args.cfg2_long > 0:
+        cfg2["long_run"] = cfg2_long(args.cfg2_long)
     log(f"cfg2: {cfg2}")
     served = None if args.no_serve else serve_cfg4(args.serve_seconds, args.serve_arrivals, args.serve_budget_gb)
     log(f"cfg4: {served}")
