@@ -1013,6 +1013,9 @@ struct Scene {
     // across steps (the EulerImplicitSolver's x, realloc'd once at init), so each solve starts
     // from the previous step's solution
     int precond = 0;
+    // CGLinearSolver warmStart (CGLinearSolver.inl:39,94-105): the solve starts from the solution
+    // vector's current value, the previous step's solution (the same persistent vector as PCG's)
+    bool warmStart = false;
     std::vector<Vec3> xsol;
     int lastPcgIterations = 0;
     Vec3 gravity;
@@ -1229,13 +1232,19 @@ struct Scene {
         timeStepCount++;
     }
 
-    // CGLinearSolver<GraphScattered>::solve (CGLinearSolver.inl:71-315); warmStart = false
-    void cgSolve(std::vector<Vec3>& xs, const std::vector<Vec3>& b, Real mF, Real bF, Real kF) {
+    // CGLinearSolver<GraphScattered>::solve (CGLinearSolver.inl:71-315)
+    void cgSolve(std::vector<Vec3>& xs, const std::vector<Vec3>& b, Real mF, Real bF, Real kF, bool warm = false) {
         const size_t N = n();
         std::vector<Vec3> p(N), q(N), r(N);
         Real rho, rho_1 = 0, alpha, beta;
-        for (auto& e : xs) e = Vec3();                        // x.clear()
-        r = b;                                                // r = b
+        if (warm) {
+            applyA(r, xs, mF, bF, kF);                        // r = A * x
+            for (size_t i = 0; i < N; i++)                    // r.eq(b, r, -1.0): r = b + r (-1)
+                for (int k = 0; k < 3; k++) r[i][k] = b[i][k] + r[i][k] * (-1.0);
+        } else {
+            for (auto& e : xs) e = Vec3();                    // x.clear()
+            r = b;                                            // r = b
+        }
         const Real normb = std::sqrt(vdot(b, b));             // b.norm()
         unsigned nb_iter = 0;
         int endcond = 0;
@@ -1326,6 +1335,10 @@ struct Scene {
         if (precond) {
             if (xsol.size() != N) xsol.assign(N, Vec3());
             pcgSolve(xsol, b, mFact, bFact, kFact);
+            xs = xsol;
+        } else if (warmStart) {
+            if (xsol.size() != N) xsol.assign(N, Vec3());
+            cgSolve(xsol, b, mFact, bFact, kFact, true);
             xs = xsol;
         } else {
             cgSolve(xs, b, mFact, bFact, kFact);
@@ -1477,6 +1490,7 @@ void orc_add_linear_movement(void* h, const int* idx, int n, const double* key_t
     s->movements.push_back(lm);
 }
 void orc_set_preconditioner(void* h, int kind) { static_cast<Scene*>(h)->precond = kind; }
+void orc_set_warm_start(void* h, int on) { static_cast<Scene*>(h)->warmStart = on != 0; }
 // the preconditioner's inverted 3x3 blocks (Jacobi: the inverted diagonals) for factors mF, kF,
 // with the current rotations: out 9 per node, row-major
 void orc_precond_blocks(void* h, double mF, double kF, double* out) {

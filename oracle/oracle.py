@@ -58,6 +58,7 @@ def lib():
         L.orc_set_integrator.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_double]
         L.orc_set_project_velocity.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.orc_set_preconditioner.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.orc_set_warm_start.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.orc_precond_blocks.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double, _dp]
         L.orc_add_linear_movement.argtypes = [ctypes.c_void_p, _ip, ctypes.c_int, _dp, _dp, ctypes.c_int, ctypes.c_int]
         L.orc_set_threads.argtypes = [ctypes.c_void_p, ctypes.c_int]
@@ -185,6 +186,10 @@ class OracleScene:
     def set_preconditioner(self, kind):
         """PCGLinearSolver in place of CGLinearSolver: 0 none, 1 Jacobi, 2 block Jacobi."""
         lib().orc_set_preconditioner(self._h, int(kind))
+
+    def set_warm_start(self, on=True):
+        """CGLinearSolver warmStart: each solve starts from the previous step's solution."""
+        lib().orc_set_warm_start(self._h, 1 if on else 0)
 
     def precond_blocks(self, m_factor, k_factor):
         """The preconditioner's inverted 3x3 blocks (n, 3, 3) with the current rotations."""

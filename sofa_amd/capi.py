@@ -27,7 +27,7 @@ EXPORTS = [
     "sofa_amd_scene_read", "sofa_amd_scene_write", "sofa_amd_scene_stats", "sofa_amd_scene_element",
     "sofa_amd_scene_von_mises", "sofa_amd_scene_stiffness",
     "sofa_amd_fem_add_force", "sofa_amd_fem_add_dforce", "sofa_amd_engine_bench_apply", "sofa_amd_scene_tiling",
-    "sofa_amd_engine_set_graphs", "sofa_amd_engine_set_branches", "sofa_amd_scene_set_mass_density", "sofa_amd_scene_add_oscillators", "sofa_amd_scene_add_linear_movement", "sofa_amd_scene_set_preconditioner",
+    "sofa_amd_engine_set_graphs", "sofa_amd_engine_set_branches", "sofa_amd_scene_set_mass_density", "sofa_amd_scene_add_oscillators", "sofa_amd_scene_add_linear_movement", "sofa_amd_scene_set_preconditioner", "sofa_amd_scene_set_warm_start",
     "sofa_amd_scene_set_refinement", "sofa_amd_engine_reserve", "sofa_amd_scene_commit", "sofa_amd_scene_release",
     "sofa_amd_engine_usage", "sofa_amd_scene_set_fix_all", "sofa_amd_scene_set_integrator", "sofa_amd_scene_set_fem_hexa", "sofa_amd_scene_add_plane", "sofa_amd_scene_set_project_velocity", "sofa_amd_engine_set_dot_order",
     "sofa_amd_scene_prepare", "sofa_amd_engine_autotune", "sofa_amd_engine_set_apply_variant",
@@ -107,6 +107,7 @@ def lib():
         "sofa_amd_scene_add_oscillators": (c_int, [_vp, c_int, c_int, _ip, _dp, _dp, _dp, _dp]),
         "sofa_amd_scene_add_linear_movement": (c_int, [_vp, c_int, c_int, _ip, c_int, _dp, _dp, c_int]),
         "sofa_amd_scene_set_preconditioner": (c_int, [_vp, c_int, c_int]),
+        "sofa_amd_scene_set_warm_start": (c_int, [_vp, c_int, c_int]),
         "sofa_amd_scene_set_refinement": (c_int, [_vp, c_int, c_int, c_double, c_int, c_int]),
         "sofa_amd_engine_reserve": (c_int, [_vp, ctypes.c_int64, ctypes.c_int64, c_int]),
         "sofa_amd_scene_commit": (c_int, [_vp, c_int]),
@@ -416,6 +417,10 @@ class Engine:
     def set_preconditioner(self, scene, kind):
         """PCGLinearSolver in place of CGLinearSolver: 0 none, 1 Jacobi, 2 block Jacobi."""
         check(lib().sofa_amd_scene_set_preconditioner(self._h, int(scene), int(kind)))
+
+    def set_warm_start(self, scene, on=True):
+        """CGLinearSolver warmStart: each solve starts from the previous step's solution (FP64)."""
+        check(lib().sofa_amd_scene_set_warm_start(self._h, int(scene), 1 if on else 0))
 
     def set_project_velocity(self, scene, on=True):
         """FixedProjectiveConstraint activate_projectVelocity."""

@@ -219,6 +219,7 @@ struct SceneHost {
     // LinearMovementProjectiveConstraint: indices, keyTimes, movements (3 per key), relativeMovements
     bool lm_on = false, lm_relative = true;
     int precond = 0;                        // PCGLinearSolver: 0 none (CGLinearSolver), 1 Jacobi, 2 block Jacobi
+    bool warm_start = false;                // CGLinearSolver warmStart (FP64 engine)
     std::vector<int32_t> lm_idx;
     std::vector<double> lm_times, lm_moves;
     double dt = 0.01, rs = 0, rm = 0;
@@ -447,6 +448,7 @@ struct EngineT : EngineBase {
         int maxpart = 0;                // most partial sums the operator reduces for one scene
         bool plane = false;             // some scene has a PlaneForceField
         bool pcg = false;               // some scene solves with PCGLinearSolver
+        bool warm = false;              // some scene's CGLinearSolver has warmStart
         hipGraphExec_t gexec = nullptr;
         ~LaunchSet() {
             if (gexec) (void)hipGraphExecDestroy(gexec);
@@ -690,6 +692,7 @@ struct EngineT : EngineBase {
         in.stopped = tiled ? 0 : 1;
         in.max_sweeps = s.max_sweeps;
         in.pcg = s.precond;
+        in.warm_start = s.warm_start ? 1 : 0;
         if (one_material(s)) host::material_factors(s.young[0], s.poisson[0], in.ks);
         return in;
     }
@@ -1258,6 +1261,7 @@ struct EngineT : EngineBase {
                 push_range(eb, eblk_h, sc.eblk_begin, sc.eblk_count);
                 push_range(nb, nblk_h, sc.nblk_begin, sc.nblk_count);
                 ls->pcg = ls->pcg || sc.precond != 0;
+                ls->warm = ls->warm || (sc.warm_start && sc.precond == 0);
                 maxit = std::max(maxit, sc.cg_iter);
             }
             put(ls->a, eb); put(ls->b, nb);
@@ -1326,6 +1330,13 @@ struct EngineT : EngineBase {
             k_pcg_open<<<(n + 63) / 64, 64, 0, s>>>(cur->scenes, n, inst);
             if (cur->na > 0) k_cg_elem<R, NR, VT><<<cur->na, kBlock, 0, s>>>(ev, eb, inst, xcg, p2, ef);
             if (cur->nb > 0) k_pcg_begin<VT><<<cur->nb, kBlock, 0, s>>>(nb, inst, na, ef, xcg, b, r, zv, pinv, partial, ndp);
+        }
+        if (cur->warm) {
+            // CGLinearSolver warmStart: r = b - A x from the previous solution (other scenes of the
+            // set pass through untouched)
+            k_warm_open<<<(n + 63) / 64, 64, 0, s>>>(cur->scenes, n, inst);
+            if (cur->na > 0) k_cg_elem<R, NR, VT><<<cur->na, kBlock, 0, s>>>(ev, eb, inst, xcg, p2, ef);
+            if (cur->nb > 0) k_cg_warm_begin<VT><<<cur->nb, kBlock, 0, s>>>(nb, inst, na, ef, xcg, b, r, partial, ndp);
         }
         for (int it = 1; it <= cur->maxit; it++) {
             // direction of iteration it in buffer it & 1, the previous one in the other
@@ -2400,6 +2411,16 @@ int sofa_amd_scene_set_preconditioner(sofa_amd_engine eng, int scene, int kind) 
     if (kind != 0 && eng->impl->precision != SOFA_AMD_FP64 && !(kind == 1 && eng->impl->precision == SOFA_AMD_MIXED))
         return fail(SOFA_AMD_EINVAL, "PCGLinearSolver runs on the FP64 engine (Jacobi also in the MIXED inner CG)");
     s->precond = kind;
+    return SOFA_AMD_OK;
+}
+
+int sofa_amd_scene_set_warm_start(sofa_amd_engine eng, int scene, int on) {
+    SceneHost* s = scene_of(eng, scene);
+    if (!s) return fail(SOFA_AMD_EINVAL, "invalid scene");
+    if (!editable(eng, scene)) return fail(SOFA_AMD_ESTATE, "scene already committed");
+    if (on && eng->impl->precision != SOFA_AMD_FP64)
+        return fail(SOFA_AMD_EINVAL, "CGLinearSolver warmStart runs on the FP64 engine");
+    s->warm_start = on != 0;
     return SOFA_AMD_OK;
 }
 

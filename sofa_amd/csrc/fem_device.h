@@ -88,8 +88,9 @@ struct InstDev {
     int lm_n, lm_nk, lm_relative, lm_x0_set;
     // PCGLinearSolver (PCGLinearSolver.inl:178-262) in place of CGLinearSolver: 0 none, 1 with a
     // JacobiPreconditioner, 2 with a BlockJacobiPreconditioner; b.b of the step and the tolerance
-    // on r.z (tolerance b.b)
-    int pcg, pad_pcg;
+    // on r.z (tolerance b.b).  warm_start: CGLinearSolver warmStart (CGLinearSolver.inl:94-105),
+    // the solve starting from the previous step's solution
+    int pcg, warm_start;
     double bnorm2, pcg_tol;
 };
 

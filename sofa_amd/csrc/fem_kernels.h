@@ -146,6 +146,26 @@ __device__ __forceinline__ void cg_start(InstDev* ins, double total) {
     }
 }
 
+// CGLinearSolver's first tests after a warm start: |b| from b.b, rho = r.r of r = b - A x, the
+// iteration-1 tolerance test with the first step's exemption (CGLinearSolver.inl:94-105,139-170)
+__device__ __forceinline__ void cg_start_warm(InstDev* ins, double bb, double rr) {
+    ins->normb = sqrt(bb);
+    ins->rho = rr;
+    ins->rho_1 = 0.0;
+    ins->iter = 1;
+    ins->stopped = 0;
+    ins->end_reason = 0;
+    ins->err = 1.0;
+    ins->den = 0.0;
+    if (ins->normb == 0.0) {
+        ins->stopped = 1; ins->end_reason = 4; ins->last_iters = 0;
+    } else {
+        const double err = sqrt(rr) / ins->normb;
+        ins->err = err;
+        if (err <= ins->tol && !ins->first_solve) { ins->stopped = 1; ins->end_reason = 1; ins->last_iters = 1; }
+    }
+}
+
 // end of a component-level CG solve (no time step): the loop's exit record, timeStepCount++
 // (CGLinearSolver.inl:300); reopen = 1 for the tiled kernels, which run with stopped == 0
 static __global__ void k_solve_end(const int* __restrict__ scenes, int n, InstDev* __restrict__ inst, int reopen) {
@@ -248,7 +268,8 @@ __global__ __launch_bounds__(kBlock) void k_rhs_node(const Blk* __restrict__ nbl
         const int64_t o = 3 * (int64_t)n;
         b[o] = bn[0]; b[o + 1] = bn[1]; b[o + 2] = bn[2];
         r[o] = bn[0]; r[o + 1] = bn[1]; r[o + 2] = bn[2];
-        if (!in.pcg) { xcg[o] = 0; xcg[o + 1] = 0; xcg[o + 2] = 0; }   // PCG starts from the last solution
+        // PCG and a CG with warmStart start from the last solution
+        if (!in.pcg && !in.warm_start) { xcg[o] = 0; xcg[o + 1] = 0; xcg[o + 2] = 0; }
         bb = dot3d(bn, bn);
         if (nd) nd[n] = bb;
     }
@@ -256,7 +277,7 @@ __global__ __launch_bounds__(kBlock) void k_rhs_node(const Blk* __restrict__ nbl
     const bool last = nd ? last_block_sequential(nd, ins, total)
                          : last_block_reduce(block_sum(bb, sh), partial, ins, bk.pad, sh, total);
     if (last && threadIdx.x == 0) {
-        if (ins->pcg) ins->bnorm2 = total;
+        if (ins->pcg || ins->warm_start) ins->bnorm2 = total;
         else cg_start(ins, total);
     }
 }
@@ -1259,6 +1280,62 @@ __global__ __launch_bounds__(kBlock) void k_pcg_begin(const Blk* __restrict__ nb
             ins->stopped = 1; ins->end_reason = 1; ins->last_iters = 1;
         }
     }
+}
+
+// the warm-start scenes of the launch set open their solve: iteration 1, so that k_cg_elem
+// evaluates the element part of A x from the solution vector
+static __global__ void k_warm_open(const int* __restrict__ scenes, int n, InstDev* __restrict__ inst) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    InstDev* in = &inst[scenes[i]];
+    if (!in->warm_start || in->pcg) return;
+    in->stopped = 0;
+    in->iter = 1;
+    in->end_reason = 0;
+}
+
+// CGLinearSolver warmStart: r = A x (mass, the element contributions of k_cg_elem on x, plane,
+// projection) then r.eq(b, r, -1.0): r = b + r (-1); rho = r.r and the first tests
+template <typename VT>
+__global__ __launch_bounds__(kBlock) void k_cg_warm_begin(const Blk* __restrict__ nblk, InstDev* __restrict__ inst,
+                                                           NodeArrays na, const VT* __restrict__ ef,
+                                                           const VT* __restrict__ xcg, const VT* __restrict__ b,
+                                                           VT* __restrict__ r, double* __restrict__ partial,
+                                                           double* __restrict__ nd) {
+    __shared__ double sh[kBlock];
+    const Blk bk = nblk[blockIdx.x];
+    InstDev* ins = &inst[bk.inst];
+    if (!ins->warm_start || ins->pcg || ins->stopped) return;
+    const int n = bk.begin + threadIdx.x;
+    double rr = 0.0;
+    if (n < bk.end) {
+        const InstDev& in = *ins;
+        const int64_t o = 3 * (int64_t)n;
+        VT xn[3];
+        load3(xcg, n, xn[0], xn[1], xn[2]);
+        VT qn[3] = {0, 0, 0};
+        if (in.m_cg != 0.0) {
+            mass_mdx(in, na, n, in.m_cg, xn, qn);
+            if (in.mass_kind == kMassMatrix)
+                mass_edges(na, n, in.m_cg, [&](int j, VT d[3]) { load3(xcg, j, d[0], d[1], d[2]); }, qn);
+        }
+        const int s0 = na.inc_ptr[n], s1 = na.inc_ptr[n + 1];
+        for (int s = s0; s < s1; s++) {
+            const int64_t slot = na.inc[s];
+            qn[0] -= ef[3 * slot]; qn[1] -= ef[3 * slot + 1]; qn[2] -= ef[3 * slot + 2];
+        }
+        if (in.plane_on && na.contact[n]) plane_dforce(in, in.k_cg, xn, qn);
+        if (na.fixed[n]) { qn[0] = 0; qn[1] = 0; qn[2] = 0; }
+        VT rn[3];
+#pragma unroll
+        for (int k = 0; k < 3; k++) { VT t = qn[k] * (VT)-1.0; rn[k] = b[o + k] + t; r[o + k] = rn[k]; }
+        rr = dot3d(rn, rn);
+        if (nd) nd[n] = rr;
+    }
+    double total;
+    const bool last = nd ? last_block_sequential(nd, ins, total)
+                         : last_block_reduce(block_sum(rr, sh), partial, ins, bk.pad, sh, total);
+    if (last && threadIdx.x == 0) cg_start_warm(ins, ins->bnorm2, total);
 }
 
 }  // namespace sofa_amd

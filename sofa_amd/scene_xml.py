@@ -41,7 +41,7 @@ value this build does not run raises ``ValueError`` rather than running other ph
 ``method`` other than large, ``plasticMaxThreshold`` > 0, ``updateStiffnessMatrix``,
 ``updateStiffness`` or ``computeGlobalMatrix`` set, a force field's or mass's own Rayleigh
 coefficient (``rayleighStiffness``, ``rayleighMass``) other than 0, PlaneForceField
-``localRange``, CG ``warmStart``, ``solveConstraint``, a DiagonalMass given per-vertex masses
+``localRange``, ``solveConstraint``, a DiagonalMass given per-vertex masses
 only, a UniformMass given no mass at all (the reference marks it invalid, UniformMass.inl
 :253-266), and a solver that governs more than one body (SOFA solves such bodies as one system,
 with one CG over all of them).
@@ -418,7 +418,7 @@ def _body(node, solver, cg, fem, mo, g, dt, node_positions):
                               vdamping=float(a.get("vdamping", "0")))
     c = cg.attr if cg is not None else {}
     if _bool(c.get("warmStart", "false")):
-        raise ValueError("CGLinearSolver warmStart is not on this path")
+        spec["warm_start"] = True          # CGLinearSolver.inl:94-105 (FP64 engine)
     spec["iters"] = int(float(c.get("iterations", "25")))
     spec["tol"] = float(c.get("tolerance", "1e-5"))
     spec["thr"] = float(c.get("threshold", "1e-5"))

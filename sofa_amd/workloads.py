@@ -195,4 +195,6 @@ def add_to_engine(eng, spec):
         eng.add_linear_movement(s, **spec["linear_movement"])
     if spec.get("preconditioner"):
         eng.set_preconditioner(s, spec["preconditioner"])
+    if spec.get("warm_start"):
+        eng.set_warm_start(s, True)
     return s

@@ -25,6 +25,8 @@ def oracle_from_spec(spec, dot_mode):
         o.add_linear_movement(**spec["linear_movement"])
     if spec.get("preconditioner"):
         o.set_preconditioner(spec["preconditioner"])
+    if spec.get("warm_start"):
+        o.set_warm_start(True)
     o.set_solver(spec["rs"], spec["rm"], spec["iters"], spec["tol"], spec["thr"])
     if "plane" in spec:
         o.add_plane(**spec["plane"])

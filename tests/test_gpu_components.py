@@ -361,3 +361,35 @@ def test_mixed_jacobi_inner_cg(which):
     assert totals[1] <= totals[0]
     if which == "ellipsoid":
         assert totals[1] < totals[0]
+
+
+@pytest.mark.parametrize("order", [0, 1])
+@pytest.mark.parametrize("which", ["beam", "ellipsoid"])
+def test_cg_warm_start_bitexact(which, order):
+    """CGLinearSolver warmStart on the FP64 engine: each solve starts from the previous step's
+    solution (r = b - A x); positions, velocities and the last solve's iteration count equal the
+    oracle's bit for bit in both dot-product orders."""
+    from sofa_amd import workloads
+    capi = _capi()
+    print(f"case: cg warm start {which} order {order}", flush=True)
+    spec, nsteps = _pcg_spec(which, 0)
+    spec["preconditioner"] = 0
+    spec["warm_start"] = True
+    spec["tol"] = 1e-4 if which == "beam" else 1e-6      # the beam's solves end on the tolerance
+    eng = capi.Engine(0, capi.FP64)
+    if order == 1:
+        eng.set_dot_order(True)
+    sid = workloads.add_to_engine(eng, spec)
+    eng.commit()
+    eng.set_graphs(True)
+    eng.step([sid], nsteps)
+    x, v, _ = eng.read(sid)
+    st = eng.stats(sid)
+    eng.close()
+    o = _oracle(spec, dot_mode=1 if order == 0 else 0)
+    o.step(spec["dt"], nsteps)
+    so = o.stats()
+    print(f"  last solve: {st['iterations']} iterations (oracle {so['iterations']}), |r|/|b| {st['err']:.3e}", flush=True)
+    assert st["iterations"] == so["iterations"]
+    assert np.array_equal(x, o.get("x")), float(np.max(np.abs(x - o.get("x"))))
+    assert np.array_equal(v, o.get("v"))

@@ -549,6 +549,45 @@ def test_pcg_warm_start():
     assert first > 5 and s.stats()["iterations"] == 1, (first, s.stats())
 
 
+def test_cg_warm_start():
+    """CGLinearSolver warmStart (CGLinearSolver.inl:94-105): r = b - A x from the solution vector,
+    which keeps the previous step's solution.  From rest the vector is 0 and r = b + (A 0)(-1) = b
+    exactly, so the first step equals the cold solve bit for bit; a second step whose system and
+    right-hand side equal the first's converges at its first test; over steps of a moving body
+    the warm solves reach the cold solves' converged states (tolerance 1e-14) in fewer iterations."""
+    pos, tets, fixed = small_mesh()
+
+    def scene(warm, tol):
+        s = OracleScene(pos)
+        s.set_gravity((0.0, -9.81, 0.0))
+        s.set_fem(tets, [2e4], [0.4], "large")
+        s.set_uniform_mass(vertex_mass=1.0)
+        s.set_fixed(fixed)
+        s.set_solver(0.1, 0.1, 500, tol, 1e-30)
+        if warm:
+            s.set_warm_start(True)
+        return s
+    cold, warm = scene(False, 1e-12), scene(True, 1e-12)
+    cold.step(0.01, 1)
+    warm.step(0.01, 1)
+    assert np.array_equal(cold.get("x"), warm.get("x")) and np.array_equal(cold.get("v"), warm.get("v"))
+    assert cold.stats()["iterations"] == warm.stats()["iterations"] > 5
+    warm.set("x", pos)
+    warm.set("v", np.zeros_like(pos))
+    warm.step(0.01, 1)
+    assert warm.stats()["iterations"] == 1 and warm.stats()["endcond"] == 1, warm.stats()
+    cold, warm = scene(False, 1e-14), scene(True, 1e-14)
+    its = {"cold": 0, "warm": 0}
+    for _ in range(10):
+        cold.step(0.01, 1)
+        warm.step(0.01, 1)
+        its["cold"] += cold.stats()["iterations"]
+        its["warm"] += warm.stats()["iterations"]
+    scale = np.max(np.abs(cold.get("x") - pos))
+    assert np.max(np.abs(warm.get("x") - cold.get("x"))) <= 1e-10 * scale
+    assert its["warm"] < its["cold"], its
+
+
 @pytest.mark.parametrize("case", GOLD["meshMatrixMassTetra"]["cases"])
 def test_meshmatrixmass_tetra_golden(case):
     """MeshMatrixMass vertex masses and total mass on the 6-tet cube (MeshMatrixMass_test)."""

@@ -494,6 +494,16 @@ def test_scene_reader_refuses_unrun_data(fem, mass, extra):
     assert len(scene_xml.load_scene(_strict())[0]) == 1
 
 
+def test_scene_reader_warm_start():
+    """CGLinearSolver warmStart is read into the spec (the FP64 engine runs it)."""
+    from sofa_amd import scene_xml
+    text = _strict().replace('<CGLinearSolver iterations="20"', '<CGLinearSolver warmStart="true" iterations="20"')
+    (spec,), _ = scene_xml.load_scene(text)
+    assert spec.get("warm_start") is True
+    (spec,), _ = scene_xml.load_scene(_strict())
+    assert not spec.get("warm_start")
+
+
 def test_scene_reader_mass_precedence():
     """UniformMass: totalMass over vertexMass (UniformMass.inl:233-266); DiagonalMass: totalMass,
     then massDensity, then the default totalMass 1 (DiagonalMass.inl:880-935)."""
