@@ -440,7 +440,7 @@ def cfg2_long(steps, max_iter=20000):
     return out
 
 
-def serve_cfg4(seconds, arrivals, budget_gb):
+def serve_cfg4(seconds, arrivals, budget_gb, exact_seconds=0.0, exact_budget_gb=64.0):
     """cfg4 job stream (200 jobs/s Poisson, seed 4) over the first `arrivals` seconds of arrivals,
     served for `seconds` of wall time, batched and plain, FP32, every launch set's step replayed
     from a hipGraph; the admission budget is `budget_gb` or, when None, 80 % of the free HBM."""
@@ -459,6 +459,21 @@ def serve_cfg4(seconds, arrivals, budget_gb):
         srv.close()
         out["batched" if batched else "plain"] = res
     out["served_over_plain"] = out["batched"]["value"] / out["plain"]["value"]
+    if exact_seconds > 0:
+        # the same stream in the reference summation order (FP64 on the tiled kernels, every job's
+        # trajectory the reference's bit for bit: the bar the headline path meets), a fixed
+        # admission budget of its own
+        ex = {"precision": "FP64_TILED, reference summation order", "budget": f"{exact_budget_gb:g} GB",
+              "seconds": exact_seconds}
+        for batched in (True, False):
+            srv = serving.Server(capi.FP64_TILED, budget_bytes=int(exact_budget_gb * (1 << 30)), batched=batched,
+                                 graphs=True, build_workers=8, reference_order=True)
+            res = srv.run(jobs, wall_limit=exact_seconds, stream=bench_stream(),
+                          log=lambda m: print(m, file=sys.stderr, flush=True))
+            srv.close()
+            ex["batched" if batched else "plain"] = res
+        ex["served_over_plain"] = ex["batched"]["value"] / ex["plain"]["value"]
+        out["reference_order"] = ex
     return out
 
 
@@ -562,6 +577,9 @@ def main():
     ap.add_argument("--no-sofacuda", action="store_true")
     ap.add_argument("--sofacuda-steps", type=int, default=1000)
     ap.add_argument("--serve-seconds", type=float, default=60.0)
+    ap.add_argument("--serve-exact-seconds", type=float, default=30.0,
+                    help="wall seconds per mode of the stream served in the reference summation order (0: off)")
+    ap.add_argument("--serve-exact-budget-gb", type=float, default=64.0)
     ap.add_argument("--serve-arrivals", type=float, default=60.0)
     ap.add_argument("--serve-budget-gb", type=float, default=None,
                     help="admission budget of the served stream (default: 80 %% of the free HBM)")
@@ -669,7 +687,8 @@ def main():
     if cfg2 is not None and args.cfg2_long > 0:
         cfg2["long_run"] = cfg2_long(args.cfg2_long)
     log(f"cfg2: {cfg2}")
-    served = None if args.no_serve else serve_cfg4(args.serve_seconds, args.serve_arrivals, args.serve_budget_gb)
+    served = None if args.no_serve else serve_cfg4(args.serve_seconds, args.serve_arrivals, args.serve_budget_gb,
+                                                   args.serve_exact_seconds, args.serve_exact_budget_gb)
     log(f"cfg4: {served}")
 
     scn = None if args.no_sofacuda else sofacuda_scenes(args.sofacuda_steps)

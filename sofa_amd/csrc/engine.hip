@@ -1619,11 +1619,13 @@ struct EngineT : EngineBase {
             if (!aux[g - 1]) HIPCHK(hipStreamCreateWithFlags(&aux[g - 1], hipStreamNonBlocking));
             if (!join_ev[g - 1]) HIPCHK(hipEventCreateWithFlags(&join_ev[g - 1], hipEventDisableTiming));
         }
+        // a copy: step_set may empty the plan cache (prepare_set's bounded cache of launch sets)
+        const std::vector<std::vector<int32_t>> groups = it->second;
         for (int st = 0; st < nsteps; st++) {
             HIPCHK(hipEventRecord(fork_ev, s));
             for (int g = 1; g < G; g++) HIPCHK(hipStreamWaitEvent(aux[g - 1], fork_ev, 0));
             for (int g = 0; g < G; g++) {
-                const std::vector<int32_t>& grp = it->second[g];
+                const std::vector<int32_t>& grp = groups[g];
                 step_set(grp.data(), (int)grp.size(), 1, g == 0 ? s : aux[g - 1]);
             }
             for (int g = 1; g < G; g++) {

@@ -32,8 +32,9 @@ from . import capi, workloads
 # sofa_amd_engine_reserve): node vectors x v f b r p q x_cg (3 reals each), mass (8), mask (1),
 # incidence pointer (4), tile-local node tables (~1.3 x 12 B), shared tables; per element: index
 # (16 + 8), rotation 9, rest 6 or 9, stiffness 3, edge mass 1 reals, element force 12 reals,
-# incidences 4 x (4 + 2)
-def footprint_bytes(precision, nodes, tets):
+# incidences 4 x (4 + 2); in the reference summation order also two per-incidence slot arrays
+# (4 x 2 x 3 reals) and two per-incidence rank tables (4 x 2 bytes)
+def footprint_bytes(precision, nodes, tets, reference_order=False):
     pv = 4 if precision == capi.FP32 else 8
     pe = 4 if precision == capi.FP32 else 8
     nr = 6 if precision == capi.FP32 else 9
@@ -43,6 +44,9 @@ def footprint_bytes(precision, nodes, tets):
         # float r, p, q, d, x, x0 and shared partials; float K entries and edge masses
         per_node += 6 * 3 * 4 + 3 * 4
         per_tet += (3 + 1) * 4
+    if reference_order:
+        per_tet += 4 * 2 * 3 * pv + 4 * 2
+        per_node += 8
     return per_node * nodes + per_tet * tets
 
 
@@ -50,8 +54,11 @@ class Server:
     """One GPU, one live engine, a FIFO admission queue bounded by an HBM budget."""
 
     def __init__(self, precision=capi.FP32, budget_bytes=None, max_scenes=2048, batched=True,
-                 build_workers=4, engine=None, graphs=False):
+                 build_workers=4, engine=None, graphs=False, reference_order=False):
+        """reference_order (FP64_TILED): every sum in the reference's order, the served jobs'
+        trajectories those of the reference bit for bit."""
         self.precision = precision
+        self.reference_order = reference_order
         if budget_bytes is None:
             free, _ = capi.device_memory(0)
             budget_bytes = int(0.8 * free)
@@ -60,11 +67,13 @@ class Server:
         if engine is None:
             # arena capacity: the budget expressed in nodes and tetrahedra (a tetrahedral mesh
             # has about 5-6 tetrahedra per node; the node arena is sized for the sparser side)
-            per_tet = footprint_bytes(precision, 0, 1)
+            per_tet = footprint_bytes(precision, 0, 1, reference_order)
             # the engine's incidence space (4 per tetrahedron) is addressed with 32-bit indices
             max_tets = min(self.budget // per_tet, 500_000_000)
             max_nodes = max_tets // 3
             engine = capi.Engine(0, precision)
+            if reference_order:
+                engine.set_dot_order(True)         # before the arenas: the slot space depends on it
             engine.reserve(max_nodes, max_tets, max_scenes)
             engine.set_graphs(graphs)
         self.eng = engine
@@ -145,7 +154,7 @@ class Server:
                 if not stage(j, wait=not active):
                     break                              # its mesh is still being built
                 sid, fut, spec = staged[j]
-                need = footprint_bytes(self.precision, len(spec["pos"]), len(spec["tets"]))
+                need = footprint_bytes(self.precision, len(spec["pos"]), len(spec["tets"]), self.reference_order)
                 if used + need > self.budget and active:
                     break
                 if fut is not None:
