@@ -2040,10 +2040,17 @@ struct EngineT : EngineBase {
         if (cols) std::copy(d.cols.begin(), d.cols.end(), cols);
         if (values && nb > 0) {
             HIPCHK(hipDeviceSynchronize());     // the rotations of the last step / addForce
+            // the element blocks (16 per element, 72 B each), for this call only
+            const int64_t eb0 = elem_begin(scene), n16 = 16 * nE;
+            double* eblk = dalloc<double>(9 * (size_t)n16);
+            k_stiffness_elem<R, NR><<<(unsigned)((n16 + kBlock - 1) / kBlock), kBlock, 0, stream>>>(view(), eb0, n16, eblk);
             const int grid = (nb + kBlock - 1) / kBlock;
-            k_stiffness_blocks<R, NR><<<grid, kBlock, 0, stream>>>(view(), d.slot_ptr, d.slots, nb, kf, d.vals);
-            HIPCHK(hipGetLastError());
-            HIPCHK(hipStreamSynchronize(stream));
+            k_stiffness_blocks<R, NR><<<grid, kBlock, 0, stream>>>(d.slot_ptr, d.slots, nb, 16 * eb0, kf, eblk, d.vals);
+            const hipError_t le = hipGetLastError();
+            const hipError_t se = hipStreamSynchronize(stream);
+            (void)hipFree(eblk);
+            HIPCHK(le);
+            HIPCHK(se);
             HIPCHK(hipMemcpy(values, d.vals, 9 * (size_t)nb * sizeof(double), hipMemcpyDeviceToHost));
         }
         return nb;

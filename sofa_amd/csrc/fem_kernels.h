@@ -1105,35 +1105,64 @@ __device__ inline void elem_stiffness_block(const ElemView<R, NR>& ev, int64_t e
         }
 }
 
-// One thread per 3x3 block (a, b) of the scene's matrix: the block is the sum, in ascending
-// element id (the element loop's order), of -(R J K J^T R^T)(n1, n2) * k over the elements and
-// vertex pairs (n1, n2) with node a at n1 and b at n2 -- slots[] lists them per block as
-// device element * 16 + n1 * 4 + n2.  The element block needs only J's rows of n1 and n2: with
-// R = rotations[e] (the last addForce's), JK = J(n1 rows) K, B = JK J(n2 rows)^T, S = R B,
-// block = S R^T, every sum in the reference's dense-product order (zero terms of the reference's
-// 12x12 products leave its sums unchanged).  Computed in double, no contraction where the
-// translation unit honours the pragma.
+// The assembled matrix in two passes.  (1) One thread per element and vertex pair (n1, n2): the
+// element block -(R J K J^T R^T)(n1, n2) -- J's rows of n1 and n2 only: with R = rotations[e]
+// (the last addForce's), JK = J(n1 rows) K, B = JK J(n2 rows)^T, S = R B, block = S R^T, every
+// sum in the reference's dense-product order (zero terms of the reference's 12x12 products leave
+// its sums unchanged) -- into the scene's element-block buffer at (element - e0) * 16 + n1 * 4 +
+// n2.  (2) One thread per 3x3 block (a, b): the sum, in ascending element id (the element loop's
+// order), of block * k over the contributions slots[] lists for it (device element * 16 + n1 * 4
+// + n2).  Each element record is read once (not once per block it touches), and both passes
+// store through LDS so that a wave's stores are contiguous.  Computed in double, no contraction
+// where the translation unit honours the pragma.
 template <typename R, int NR>
-__global__ __launch_bounds__(kBlock) void k_stiffness_blocks(ElemView<R, NR> ev, const int* __restrict__ slot_ptr,
-                                                             const int* __restrict__ slots, int nblk, double kf,
-                                                             double* __restrict__ out) {
+__global__ __launch_bounds__(kBlock) void k_stiffness_elem(ElemView<R, NR> ev, int64_t e0, int64_t n16,
+                                                            double* __restrict__ eblk) {
 #pragma clang fp contract(off)
-    const int b = blockIdx.x * kBlock + threadIdx.x;
-    if (b >= nblk) return;
-    double acc[9];
-#pragma unroll
-    for (int i = 0; i < 9; i++) acc[i] = 0.0;
-    for (int s = slot_ptr[b]; s < slot_ptr[b + 1]; s++) {
-        const int slot = slots[s];
-        const int64_t e = slot >> 4;
-        const int n1 = (slot >> 2) & 3, n2 = slot & 3;
+    __shared__ double st[9 * kBlock];
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < n16) {
         double blk[9];
-        elem_stiffness_block<R, NR>(ev, e, n1, n2, blk);
+        const int j = (int)(i & 15);
+        elem_stiffness_block<R, NR>(ev, e0 + (i >> 4), j >> 2, j & 3, blk);
 #pragma unroll
-        for (int i = 0; i < 9; i++) acc[i] += (-blk[i]) * kf;
+        for (int k = 0; k < 9; k++) st[9 * threadIdx.x + k] = blk[k];
     }
+    __syncthreads();
+    const int64_t o = 9 * (int64_t)blockIdx.x * kBlock, end = 9 * n16;
 #pragma unroll
-    for (int i = 0; i < 9; i++) out[9 * (int64_t)b + i] = acc[i];
+    for (int k = 0; k < 9; k++) {
+        const int64_t w = o + k * kBlock + threadIdx.x;
+        if (w < end) eblk[w] = st[k * kBlock + threadIdx.x];
+    }
+}
+
+template <typename R, int NR>
+__global__ __launch_bounds__(kBlock) void k_stiffness_blocks(const int* __restrict__ slot_ptr, const int* __restrict__ slots,
+                                                             int nblk, int64_t slot0, double kf,
+                                                             const double* __restrict__ eblk, double* __restrict__ out) {
+#pragma clang fp contract(off)
+    __shared__ double st[9 * kBlock];
+    const int b = blockIdx.x * kBlock + threadIdx.x;
+    if (b < nblk) {
+        double acc[9];
+#pragma unroll
+        for (int i = 0; i < 9; i++) acc[i] = 0.0;
+        for (int s = slot_ptr[b]; s < slot_ptr[b + 1]; s++) {
+            const double* blk = eblk + 9 * ((int64_t)slots[s] - slot0);
+#pragma unroll
+            for (int i = 0; i < 9; i++) acc[i] += (-blk[i]) * kf;
+        }
+#pragma unroll
+        for (int i = 0; i < 9; i++) st[9 * threadIdx.x + i] = acc[i];
+    }
+    __syncthreads();
+    const int64_t o = 9 * (int64_t)blockIdx.x * kBlock, end = 9 * (int64_t)nblk;
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+        const int64_t w = o + k * kBlock + threadIdx.x;
+        if (w < end) out[w] = st[k * kBlock + threadIdx.x];
+    }
 }
 
 // ---------------------------------------------------------------------------------------
