@@ -574,11 +574,24 @@ __global__ __launch_bounds__(TB) void k_tile_apply(TileArgs<R, NR> a, int iter, 
     InstDev* ins = &a.inst[td.scene];
     // a refinement sweep that keeps the previous search direction continues CG at its iteration 1
     const bool cont = iter == 1 && ins->keep_dir && ins->sweep > 1;
+    // the incidence tables into registers first: their loads are in flight while the node ids
+    // arrive and the node gathers below are issued; they go to LDS after those
     const int ninc = 4 * td.e_count;
-    for (int j = threadIdx.x; j < ninc; j += TB) linc[j] = a.tn.inc[td.inc_begin + j];
-    for (int j = threadIdx.x; j <= td.ln_count; j += TB) lptr[j] = a.tn.inc_ptr[td.ln_begin + j] - td.inc_begin;
-    if (ins->stopped) return;
-    // ---- 2. the tile's nodes: r and p (dependent on the node ids), masks
+    constexpr int IPT = (4 * kTileElems + TB - 1) / TB;     // incidences per thread
+    constexpr int PPT = (kTileNodes + 1 + TB - 1) / TB;     // incidence pointers per thread
+    int inc_r[IPT], ptr_r[PPT];
+#pragma unroll
+    for (int k = 0; k < IPT; k++) {
+        const int j = threadIdx.x + k * TB;
+        if (j < ninc) inc_r[k] = a.tn.inc[td.inc_begin + j];
+    }
+#pragma unroll
+    for (int k = 0; k < PPT; k++) {
+        const int j = threadIdx.x + k * TB;
+        if (j <= td.ln_count) ptr_r[k] = a.tn.inc_ptr[td.ln_begin + j] - td.inc_begin;
+    }
+    // ---- 2. the tile's nodes: r and p (dependent on the node ids), masks; issued before the
+    //         scene's stop flag is tested (in-bounds loads, unused for a stopped scene)
     VT rr_[NPT][3], po_[NPT][3], xx_[NPT][3], x0_[NPT][3];
     float dv_[NPT][3];
 #pragma unroll
@@ -594,6 +607,17 @@ __global__ __launch_bounds__(TB) void k_tile_apply(TileArgs<R, NR> a, int iter, 
             ct[m] = PLANE && ins->plane_on ? a.contact[gid[m]] : 0;
         }
     }
+#pragma unroll
+    for (int k = 0; k < IPT; k++) {
+        const int j = threadIdx.x + k * TB;
+        if (j < ninc) linc[j] = (uint16_t)inc_r[k];
+    }
+#pragma unroll
+    for (int k = 0; k < PPT; k++) {
+        const int j = threadIdx.x + k * TB;
+        if (j <= td.ln_count) lptr[j] = ptr_r[k];
+    }
+    if (ins->stopped) return;
     const Ranges rg = a.ranges[td.scene];
     const bool first = (td.flags & kTileFirst) != 0;
     // ---- 3. scalars: rho, the tolerance test, beta (CGLinearSolver.inl:130-196)
