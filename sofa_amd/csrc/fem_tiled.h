@@ -559,17 +559,18 @@ __global__ __launch_bounds__(TB) void k_tile_apply(TileArgs<R, NR> a, int iter, 
             em[m] = consistent ? a.emass[e] : (R)0;
         }
     }
+    // tile-local node tables: every thread loads, at its node or (past the tile's count) at the
+    // tile's last node, which every tile has -- no branch, so the compiler keeps all these loads
+    // and the gathers that depend on them in flight together; values past the count are unused
     int gid[NPT], psl[NPT], lead[NPT];
     int8_t fx[NPT], ct[NPT];
 #pragma unroll
     for (int m = 0; m < NPT; m++) {
-        const int j = threadIdx.x + m * TB;
-        if (j < td.ln_count) {
-            const int li = td.ln_begin + j;
-            gid[m] = a.tn.gid[li];
-            psl[m] = a.tn.pslot[li];
-            lead[m] = EXACT ? a.tn.lead[li] : 0;
-        }
+        const int j = min((int)threadIdx.x + m * TB, td.ln_count - 1);
+        const int li = td.ln_begin + j;
+        gid[m] = a.tn.gid[li];
+        psl[m] = a.tn.pslot[li];
+        lead[m] = EXACT ? a.tn.lead[li] : 0;
     }
     InstDev* ins = &a.inst[td.scene];
     // a refinement sweep that keeps the previous search direction continues CG at its iteration 1
@@ -579,32 +580,39 @@ __global__ __launch_bounds__(TB) void k_tile_apply(TileArgs<R, NR> a, int iter, 
     const int ninc = 4 * td.e_count;
     constexpr int IPT = (4 * kTileElems + TB - 1) / TB;     // incidences per thread
     constexpr int PPT = (kTileNodes + 1 + TB - 1) / TB;     // incidence pointers per thread
+    // (clamped like the node tables; a tile without elements reads entry 0, which every buffer has)
     int inc_r[IPT], ptr_r[PPT];
 #pragma unroll
     for (int k = 0; k < IPT; k++) {
         const int j = threadIdx.x + k * TB;
-        if (j < ninc) inc_r[k] = a.tn.inc[td.inc_begin + j];
+        inc_r[k] = a.tn.inc[ninc > 0 ? td.inc_begin + min(j, ninc - 1) : 0];
     }
 #pragma unroll
     for (int k = 0; k < PPT; k++) {
-        const int j = threadIdx.x + k * TB;
-        if (j <= td.ln_count) ptr_r[k] = a.tn.inc_ptr[td.ln_begin + j] - td.inc_begin;
+        const int j = min((int)threadIdx.x + k * TB, td.ln_count);
+        ptr_r[k] = a.tn.inc_ptr[td.ln_begin + j] - td.inc_begin;
     }
     // ---- 2. the tile's nodes: r and p (dependent on the node ids), masks; issued before the
     //         scene's stop flag is tested (in-bounds loads, unused for a stopped scene)
     VT rr_[NPT][3], po_[NPT][3], xx_[NPT][3], x0_[NPT][3];
     float dv_[NPT][3];
+    // p is read from iteration 2 on (or continuing a direction); without a p (a plain operator
+    // application) the loads read r instead, unused
+    const VT* __restrict__ pl = p ? p : r;
+    const bool plane = PLANE && ins->plane_on;
 #pragma unroll
     for (int m = 0; m < NPT; m++) {
-        const int j = threadIdx.x + m * TB;
-        if (j < td.ln_count) {
-            ld3(r, gid[m], rr_[m]);
-            if (a.dinv) ld3(a.dinv, gid[m], dv_[m]);
-            if (iter > 1 || cont) ld3(p, gid[m], po_[m]);
-            if (GEO >= 1) ld3(x, gid[m], xx_[m]);
-            if (GEO == 2) ld3(x0, gid[m], x0_[m]);
-            fx[m] = a.fixed[gid[m]];
-            ct[m] = PLANE && ins->plane_on ? a.contact[gid[m]] : 0;
+        ld3(r, gid[m], rr_[m]);
+        if (a.dinv) ld3(a.dinv, gid[m], dv_[m]);
+        ld3(pl, gid[m], po_[m]);
+        if (GEO >= 1) ld3(x, gid[m], xx_[m]);
+        if (GEO == 2) ld3(x0, gid[m], x0_[m]);
+        fx[m] = a.fixed[gid[m]];
+        if (PLANE) {
+            const int8_t c = a.contact[gid[m]];
+            ct[m] = plane ? c : 0;
+        } else {
+            ct[m] = 0;
         }
     }
 #pragma unroll
