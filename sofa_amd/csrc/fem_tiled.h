@@ -307,28 +307,53 @@ __global__ __launch_bounds__(TB) void k_tile_rhs(TileArgs<R, NR> a, const VT* __
             em[m] = in.mass_kind == kMassMatrix ? a.emass[e] : (R)0;
         }
     }
+    // node tables, incidence tables and node gathers at clamped indices (no branch: all in flight
+    // together; values past the tile's counts unused), then staged in LDS
     int gid[NPT], psl[NPT];
+#pragma unroll
+    for (int m = 0; m < NPT; m++) {
+        const int li = td.ln_begin + min((int)threadIdx.x + m * TB, td.ln_count - 1);
+        gid[m] = a.tn.gid[li];
+        psl[m] = a.tn.pslot[li];
+    }
+    const int ninc = 4 * td.e_count;
+    constexpr int IPT = (4 * kTileElems + TB - 1) / TB;     // incidences per thread
+    constexpr int PPT = (kTileNodes + 1 + TB - 1) / TB;     // incidence pointers per thread
+    int inc_r[IPT], ptr_r[PPT];
+#pragma unroll
+    for (int k = 0; k < IPT; k++)
+        inc_r[k] = a.tn.inc[ninc > 0 ? td.inc_begin + min((int)threadIdx.x + k * TB, ninc - 1) : 0];
+#pragma unroll
+    for (int k = 0; k < PPT; k++)
+        ptr_r[k] = a.tn.inc_ptr[td.ln_begin + min((int)threadIdx.x + k * TB, td.ln_count)] - td.inc_begin;
+    VT xt[NPT][3], vt[NPT][3];
+#pragma unroll
+    for (int m = 0; m < NPT; m++) {
+        ld3(x, gid[m], xt[m]);
+        ld3(v, gid[m], vt[m]);
+    }
 #pragma unroll
     for (int m = 0; m < NPT; m++) {
         const int j = threadIdx.x + m * TB;
         if (j < td.ln_count) {
-            const int li = td.ln_begin + j;
-            gid[m] = a.tn.gid[li];
-            psl[m] = a.tn.pslot[li];
-            VT t[3];
-            ld3(x, gid[m], t);
-            lx[3 * j] = (R)t[0]; lx[3 * j + 1] = (R)t[1]; lx[3 * j + 2] = (R)t[2];
+            lx[3 * j] = (R)xt[m][0]; lx[3 * j + 1] = (R)xt[m][1]; lx[3 * j + 2] = (R)xt[m][2];
             if (a.x_lo) {   // every tile of a shared node writes the same value
-                const float xf[3] = {(float)t[0], (float)t[1], (float)t[2]};
+                const float xf[3] = {(float)xt[m][0], (float)xt[m][1], (float)xt[m][2]};
                 st3(a.x_lo, gid[m], xf);
             }
-            ld3(v, gid[m], t);
-            lv[3 * j] = (R)t[0]; lv[3 * j + 1] = (R)t[1]; lv[3 * j + 2] = (R)t[2];
+            lv[3 * j] = (R)vt[m][0]; lv[3 * j + 1] = (R)vt[m][1]; lv[3 * j + 2] = (R)vt[m][2];
         }
     }
-    const int ninc = 4 * td.e_count;
-    for (int j = threadIdx.x; j < ninc; j += TB) linc[j] = a.tn.inc[td.inc_begin + j];
-    for (int j = threadIdx.x; j <= td.ln_count; j += TB) lptr[j] = a.tn.inc_ptr[td.ln_begin + j] - td.inc_begin;
+#pragma unroll
+    for (int k = 0; k < IPT; k++) {
+        const int j = threadIdx.x + k * TB;
+        if (j < ninc) linc[j] = (uint16_t)inc_r[k];
+    }
+#pragma unroll
+    for (int k = 0; k < PPT; k++) {
+        const int j = threadIdx.x + k * TB;
+        if (j <= td.ln_count) lptr[j] = ptr_r[k];
+    }
     __syncthreads();
     // ---- element force (accumulateForceLarge); the rotations are stored for the operator
 #pragma unroll
