@@ -263,6 +263,34 @@ __device__ __forceinline__ void store_incidences(const uint16_t* linc, const uin
     }
 }
 
+// acc (+|-)= the partials or slots sp[s0 .. s1) in that order: the loads four at a time, then
+// their adds in slot order (the same operations as one slot after the other)
+template <bool SUB, typename VT>
+__device__ __forceinline__ void sum_slots(const VT* __restrict__ sp, int s0, int s1, VT acc[3]) {
+    int s = s0;
+    for (; s + 4 <= s1; s += 4) {
+        VT t[4][3];
+#pragma unroll
+        for (int u = 0; u < 4; u++) ld3(sp, s + u, t[u]);
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+#pragma unroll
+            for (int i = 0; i < 3; i++) {
+                if (SUB) acc[i] -= t[u][i];
+                else acc[i] += t[u][i];
+            }
+    }
+    for (; s < s1; s++) {
+        VT t[3];
+        ld3(sp, s, t);
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            if (SUB) acc[i] -= t[i];
+            else acc[i] += t[i];
+        }
+    }
+}
+
 __device__ __forceinline__ void unpack_ltet(uint2 w, int l[4]) {
     l[0] = (int)(w.x & 0xffffu); l[1] = (int)(w.x >> 16); l[2] = (int)(w.y & 0xffffu); l[3] = (int)(w.y >> 16);
 }
@@ -484,11 +512,7 @@ __global__ __launch_bounds__(kBlock) void k_fin_rhs(const FinBlk* __restrict__ f
         ld3(v, g, vn);
         mass_gravity(in, mass, g, fn);
         const int s0 = sn.ptr[j], s1 = sn.ptr[j + 1];
-        for (int s = s0; s < s1; s++) {
-            const int64_t o = 3 * (int64_t)s;
-#pragma unroll
-            for (int i = 0; i < 3; i++) fn[i] += spf[o + i];
-        }
+        sum_slots<false>(spf, s0, s1, fn);
         // PlaneForceField after the FEM (scene order); the contact is kept for addDForce
         bool touch = false;
         if (in.plane_on) {
@@ -500,11 +524,7 @@ __global__ __launch_bounds__(kBlock) void k_fin_rhs(const FinBlk* __restrict__ f
         st3(f, g, fn);
         VT bn[3] = {fn[0], fn[1], fn[2]};
         if (in.m_rhs != 0.0) mass_mdx(in, NodeArrays{nullptr, nullptr, mass, fixed}, g, in.m_rhs, vn, bn);
-        for (int s = s0; s < s1; s++) {
-            const int64_t o = 3 * (int64_t)s;
-#pragma unroll
-            for (int i = 0; i < 3; i++) bn[i] -= spv[o + i];
-        }
+        sum_slots<true>(spv, s0, s1, bn);
         if (touch && in.k_rhs != 0.0) plane_dforce(in, in.k_rhs, vn, bn);
         const VT h = (VT)in.b_scale;   // h (firstOrder: 1, b = f)
         bn[0] *= h; bn[1] *= h; bn[2] *= h;
@@ -855,11 +875,7 @@ __global__ __launch_bounds__(kBlock) void k_fin_exact(const FinBlk* __restrict__
     const int s0 = sn.ptr[j], s1 = s0 + 1 + (int)sn.acnt[j];
     VT qn[3];
     ld3(sp, s0, qn);
-    for (int s = s0 + 1; s < s1; s++) {
-        const int64_t o = 3 * (int64_t)s;
-#pragma unroll
-        for (int i = 0; i < 3; i++) qn[i] -= sp[o + i];
-    }
+    sum_slots<true>(sp, s0 + 1, s1, qn);
     if (in.plane_on && contact[g]) plane_dforce(in, in.k_cg, pp, qn);
     if (fixed[g]) { qn[0] = 0; qn[1] = 0; qn[2] = 0; }
     st3(q, g, qn);
@@ -896,10 +912,7 @@ __global__ __launch_bounds__(kBlock) void k_fin_apply(const FinBlk* __restrict__
         VT qn[3] = {0, 0, 0};
         if (in.m_cg != 0.0) mass_mdx(in, NodeArrays{nullptr, nullptr, mass, fixed}, g, in.m_cg, pp, qn);
         const int s0 = sn.ptr[j], s1 = sn.ptr[j + 1];
-        for (int s = s0; s < s1; s++) {
-#pragma unroll
-            for (int i = 0; i < 3; i++) qn[i] += sp[3 * (int64_t)s + i];
-        }
+        sum_slots<false>(sp, s0, s1, qn);
         if (in.plane_on && contact[g]) plane_dforce(in, in.k_cg, pp, qn);
         if (fixed[g]) { qn[0] = 0; qn[1] = 0; qn[2] = 0; }
         st3(q, g, qn);
@@ -986,10 +999,7 @@ __global__ __launch_bounds__(kBlock) void k_update(const NodeBlk* __restrict__ n
                 VT qn[3] = {0, 0, 0};
                 if (in.m_cg != 0.0) mass_mdx(in, NodeArrays{nullptr, nullptr, mass, fixed}, n, in.m_cg, pp, qn);
                 const int s0 = sn.ptr[shk[k]], s1 = sn.ptr[shk[k] + 1];
-                for (int s = s0; s < s1; s++) {
-#pragma unroll
-                    for (int i = 0; i < 3; i++) qn[i] += sp[3 * (int64_t)s + i];
-                }
+                sum_slots<false>(sp, s0, s1, qn);
                 if (in.plane_on && contact[n]) plane_dforce(in, in.k_cg, pp, qn);
                 if (fixed[n]) { qn[0] = 0; qn[1] = 0; qn[2] = 0; }
 #pragma unroll
