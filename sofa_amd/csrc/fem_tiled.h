@@ -947,21 +947,18 @@ __global__ __launch_bounds__(kBlock) void k_update(const NodeBlk* __restrict__ n
     VT pv[kNodesPerThread][3], qv[kNodesPerThread][3], xv[kNodesPerThread][3], rv[kNodesPerThread][3];
     float dv[kNodesPerThread][3];
     int shk[kNodesPerThread];
+    // every load at a clamped node index (no branch: all in flight together; a node past the
+    // block's end repeats its last node, unused), p and q also where the shared-node path below
+    // replaces them
 #pragma unroll
     for (int k = 0; k < kNodesPerThread; k++) {
-        const int n = nb.begin + k * kBlock + threadIdx.x;
-        if (n < nb.end) {
-            shk[k] = EXACT ? -1 : sn.of[n];
-            ld3(xcg, n, xv[k]);
-            ld3(r, n, rv[k]);
-            if (dinv) ld3(dinv, n, dv[k]);
-            if (shk[k] < 0) {
-                ld3(p, n, pv[k]);
-                ld3(q, n, qv[k]);
-            } else if (iter > 1 || (ins->keep_dir && ins->sweep > 1)) {
-                ld3(p, n, pv[k]);
-            }
-        }
+        const int n = min(nb.begin + k * kBlock + (int)threadIdx.x, nb.end - 1);
+        shk[k] = EXACT ? -1 : sn.of[n];
+        ld3(xcg, n, xv[k]);
+        ld3(r, n, rv[k]);
+        if (dinv) ld3(dinv, n, dv[k]);
+        ld3(p, n, pv[k]);
+        ld3(q, n, qv[k]);
     }
     if (ins->stopped) return;
     const bool cont = iter == 1 && ins->keep_dir && ins->sweep > 1;
