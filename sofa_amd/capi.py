@@ -27,7 +27,7 @@ EXPORTS = [
     "sofa_amd_scene_read", "sofa_amd_scene_write", "sofa_amd_scene_stats", "sofa_amd_scene_element",
     "sofa_amd_scene_von_mises", "sofa_amd_scene_stiffness",
     "sofa_amd_fem_add_force", "sofa_amd_fem_add_dforce", "sofa_amd_engine_bench_apply", "sofa_amd_scene_tiling",
-    "sofa_amd_engine_set_graphs", "sofa_amd_engine_set_branches", "sofa_amd_scene_set_mass_density", "sofa_amd_scene_add_oscillators", "sofa_amd_scene_add_linear_movement", "sofa_amd_scene_set_preconditioner", "sofa_amd_scene_set_warm_start",
+    "sofa_amd_engine_set_graphs", "sofa_amd_engine_set_branches", "sofa_amd_engine_set_branch_turns", "sofa_amd_scene_set_mass_density", "sofa_amd_scene_add_oscillators", "sofa_amd_scene_add_linear_movement", "sofa_amd_scene_set_preconditioner", "sofa_amd_scene_set_warm_start",
     "sofa_amd_scene_set_refinement", "sofa_amd_engine_reserve", "sofa_amd_scene_commit", "sofa_amd_scene_release",
     "sofa_amd_engine_usage", "sofa_amd_scene_set_fix_all", "sofa_amd_scene_set_integrator", "sofa_amd_scene_set_fem_hexa", "sofa_amd_scene_add_plane", "sofa_amd_scene_set_project_velocity", "sofa_amd_engine_set_dot_order",
     "sofa_amd_scene_prepare", "sofa_amd_engine_autotune", "sofa_amd_engine_set_apply_variant",
@@ -103,6 +103,7 @@ def lib():
         "sofa_amd_scene_tiling": (c_int, [_vp, c_int, _ip]),
         "sofa_amd_engine_set_graphs": (c_int, [_vp, c_int]),
         "sofa_amd_engine_set_branches": (c_int, [_vp, c_int]),
+        "sofa_amd_engine_set_branch_turns": (c_int, [_vp, c_int]),
         "sofa_amd_scene_set_mass_density": (c_int, [_vp, c_int, c_int, _dp, c_int]),
         "sofa_amd_scene_add_oscillators": (c_int, [_vp, c_int, c_int, _ip, _dp, _dp, _dp, _dp]),
         "sofa_amd_scene_add_linear_movement": (c_int, [_vp, c_int, c_int, _ip, c_int, _dp, _dp, c_int]),
@@ -461,6 +462,11 @@ class Engine:
     def set_branches(self, branches):
         """Concurrent branches of a merged step (0: the engine's default)."""
         check(lib().sofa_amd_engine_set_branches(self._h, int(branches)))
+
+    def set_branch_turns(self, enable):
+        """Reference-order steps in branches: operator phases take turns (True) or the branches
+        run freely (False, default)."""
+        check(lib().sofa_amd_engine_set_branch_turns(self._h, 1 if enable else 0))
 
     def set_graphs(self, enable=True):
         check(lib().sofa_amd_engine_set_graphs(self._h, 1 if enable else 0))

@@ -329,6 +329,7 @@ struct EngineBase {
     bool seq_dots = false;            // exact path: dot products in the reference's sequential order
     bool stepped = false;             // a step has been enqueued
     int branches = 0;                 // concurrent branches of a merged step (0: the engine's default)
+    bool branch_turns = false;        // the branches' operator phases take turns (reference order)
     double* vdot_part = nullptr;      // vDot's per-block partials (grown on demand)
     int64_t vdot_len = 0;
     virtual int autotune(int reps, hipStream_t s, float* us_out) = 0;
@@ -458,12 +459,15 @@ struct EngineT : EngineBase {
     };
     std::map<std::vector<int32_t>, std::unique_ptr<LaunchSet>> sets;
     LaunchSet* cur = nullptr;
+    uint64_t sets_dropped = 0;          // times prepare_set emptied the cache
 
     ~EngineT() override {   // the device is synchronized before an engine is destroyed
         sets.clear();
         for (hipStream_t a : aux)
             if (a) (void)hipStreamDestroy(a);
         for (hipEvent_t e : join_ev)
+            if (e) (void)hipEventDestroy(e);
+        for (hipEvent_t e : turn_ev)
             if (e) (void)hipEventDestroy(e);
         if (fork_ev) (void)hipEventDestroy(fork_ev);
         while (!vmd.empty()) free_vm(vmd.begin()->first);
@@ -1250,6 +1254,7 @@ struct EngineT : EngineBase {
             HIPCHK(hipDeviceSynchronize());
             sets.clear();
             branch_plans.clear();
+            sets_dropped++;
         }
         std::unique_ptr<LaunchSet> ls(new LaunchSet());
         int maxit = 0;
@@ -1364,9 +1369,17 @@ struct EngineT : EngineBase {
     // apply (interior nodes finished, shared nodes' contributions per incidence), the shared
     // nodes' finalize, p.q node after node, the update, r.r node after node
     void launch_step_exact(int n, hipStream_t s) {
+        exact_open(n, s);
+        for (int it = 1; it <= cur->maxit; it++) {
+            exact_operator(it, s);
+            exact_dots(it, n, s);
+        }
+        exact_close(n, s);
+    }
+    // its phases, on the current launch set: the right-hand side and |b|^2
+    void exact_open(int n, hipStream_t s) {
         const TileArgs<R, NR> ta = tile_args();
         const FinBlk* fb = (const FinBlk*)cur->b;
-        const NodeBlk* nb = (const NodeBlk*)cur->c;
         const SharedNodes sn{sh_gid, sh_ptr, sh_of, sh_acnt};
         if (cur->nl > 0) k_lm_mask<false><<<cur->nl, kBlock, 0, s>>>((const Blk*)cur->l, inst, 0, fixed);
         if (cur->na > 0) k_tile_rhs<R, NR, VT, 512, true><<<cur->na, 512, 0, s>>>(ta, x, v, f, b, r, xcg, sp0, sp1, pbb);
@@ -1374,16 +1387,29 @@ struct EngineT : EngineBase {
             k_fin_rhs<VT, true><<<cur->nb, kBlock, 0, s>>>(fb, inst, ranges_d, sn, mass, fixed, v, sp0, sp1, f, b, r, xcg,
                                                              pbb, x, contact, nd);
         k_seqdot<<<n, 64, 0, s>>>(cur->scenes, inst, nd, kDotBB);
-        for (int it = 1; it <= cur->maxit; it++) {
-            if (cur->na > 0) launch_apply(cur->na, s, ta, it, r, p, q, sp0, nullptr, pden, 0);
-            if (cur->nb > 0)
-                k_fin_exact<VT><<<cur->nb, kBlock, 0, s>>>(fb, inst, sn, mass, fixed, it, r, p, q, sp0, nd, contact);
-            k_seqdot<<<n, 64, 0, s>>>(cur->scenes, inst, nd, kDotPQ);
-            if (cur->nc > 0)
-                k_update<VT, true><<<cur->nc, kBlock, 0, s>>>(nb, inst, ranges_d, it, sn, sp0, mass, fixed, p, q, xcg, r,
-                                                              pden, prho, contact, nd);
-            k_seqdot<<<n, 64, 0, s>>>(cur->scenes, inst, nd, kDotRR);
-        }
+    }
+    // CG iteration `it`'s operator (q = A p, the p.q terms): the throughput-bound phase
+    void exact_operator(int it, hipStream_t s) {
+        const FinBlk* fb = (const FinBlk*)cur->b;
+        const SharedNodes sn{sh_gid, sh_ptr, sh_of, sh_acnt};
+        if (cur->na > 0) launch_apply(cur->na, s, tile_args(), it, r, p, q, sp0, nullptr, pden, 0);
+        if (cur->nb > 0)
+            k_fin_exact<VT><<<cur->nb, kBlock, 0, s>>>(fb, inst, sn, mass, fixed, it, r, p, q, sp0, nd, contact);
+    }
+    // its dot products and update: p.q node after node, x and r updated, r.r node after node (one
+    // wave per scene: the latency-bound phase)
+    void exact_dots(int it, int n, hipStream_t s) {
+        const NodeBlk* nb = (const NodeBlk*)cur->c;
+        const SharedNodes sn{sh_gid, sh_ptr, sh_of, sh_acnt};
+        k_seqdot<<<n, 64, 0, s>>>(cur->scenes, inst, nd, kDotPQ);
+        if (cur->nc > 0)
+            k_update<VT, true><<<cur->nc, kBlock, 0, s>>>(nb, inst, ranges_d, it, sn, sp0, mass, fixed, p, q, xcg, r,
+                                                          pden, prho, contact, nd);
+        k_seqdot<<<n, 64, 0, s>>>(cur->scenes, inst, nd, kDotRR);
+    }
+    // the Euler update and the step's end
+    void exact_close(int n, hipStream_t s) {
+        const NodeBlk* nb = (const NodeBlk*)cur->c;
         if (cur->nc > 0) k_integrate<VT><<<cur->nc, kBlock, 0, s>>>(nb, inst, fixed, xcg, v, x);
         if (cur->no > 0) k_oscillate<VT><<<cur->no, kBlock, 0, s>>>((const Blk*)cur->o, inst, oscillators(), x, v);
         if (cur->nl > 0) k_lm_project<VT><<<cur->nl, kBlock, 0, s>>>((const Blk*)cur->l, inst, x, v);
@@ -1621,6 +1647,7 @@ struct EngineT : EngineBase {
         }
         // a copy: step_set may empty the plan cache (prepare_set's bounded cache of launch sets)
         const std::vector<std::vector<int32_t>> groups = it->second;
+        if (branch_turns && exact_tiled()) { step_turns(groups, nsteps, s, G); return; }
         for (int st = 0; st < nsteps; st++) {
             HIPCHK(hipEventRecord(fork_ev, s));
             for (int g = 1; g < G; g++) HIPCHK(hipStreamWaitEvent(aux[g - 1], fork_ev, 0));
@@ -1633,6 +1660,64 @@ struct EngineT : EngineBase {
                 HIPCHK(hipStreamWaitEvent(s, join_ev[g - 1], 0));
             }
         }
+    }
+
+    // the branches' operator phases taking turns (the reference order's step): per CG iteration
+    // each group's operator and finalize wait for the previous group's (round-robin over the
+    // groups), so that while one group runs its two node-after-node dot products (one wave per
+    // scene) the GPU runs the other groups' operators one at a time, instead of every group's
+    // operator at once and then every group's chains on an idle GPU.  Launched directly: a graph
+    // per group cannot wait for another group's kernels.  Off by default: measured on the 256-beam
+    // mix (4 branches, 8 queues) 22.3 ms/step against 21.6 running freely -- a group's operator
+    // alone takes no less time than beside the others' (185-211 us for a quarter of the beams),
+    // its turn's wait adds about 200 us per iteration, and each group's own chain of dependent
+    // phases (two dot chains of about 190 us, its operator, finalize and update) sets the step.
+    hipEvent_t turn_ev[kMaxBranches] = {};
+    void step_turns(const std::vector<std::vector<int32_t>>& groups, int nsteps, hipStream_t s, int G) {
+        LaunchSet* ls[kMaxBranches] = {};
+        for (;;) {   // again if preparing a group's set emptied the cache under an earlier group's
+            const uint64_t d0 = sets_dropped;
+            for (int g = 0; g < G; g++) {
+                prepare_set(groups[g].data(), (int)groups[g].size(), s);
+                ls[g] = cur;
+            }
+            if (sets_dropped == d0) break;
+        }
+        int maxit = 0;
+        for (int g = 0; g < G; g++) {
+            maxit = std::max(maxit, ls[g]->maxit);
+            if (!turn_ev[g]) HIPCHK(hipEventCreateWithFlags(&turn_ev[g], hipEventDisableTiming));
+        }
+        auto stream = [&](int g) { return g == 0 ? s : aux[g - 1]; };
+        for (int st = 0; st < nsteps; st++) {
+            HIPCHK(hipEventRecord(fork_ev, s));
+            for (int g = 1; g < G; g++) HIPCHK(hipStreamWaitEvent(aux[g - 1], fork_ev, 0));
+            for (int g = 0; g < G; g++) {
+                cur = ls[g];
+                exact_open(cur->n, stream(g));
+            }
+            int last = -1;   // the group whose operator phase was enqueued last
+            for (int it = 1; it <= maxit; it++)
+                for (int g = 0; g < G; g++) {
+                    if (it > ls[g]->maxit) continue;
+                    cur = ls[g];
+                    const hipStream_t sg = stream(g);
+                    if (last >= 0 && last != g) HIPCHK(hipStreamWaitEvent(sg, turn_ev[last], 0));
+                    exact_operator(it, sg);
+                    HIPCHK(hipEventRecord(turn_ev[g], sg));
+                    last = g;
+                    exact_dots(it, cur->n, sg);
+                }
+            for (int g = 0; g < G; g++) {
+                cur = ls[g];
+                exact_close(cur->n, stream(g));
+            }
+            for (int g = 1; g < G; g++) {
+                HIPCHK(hipEventRecord(join_ev[g - 1], aux[g - 1]));
+                HIPCHK(hipStreamWaitEvent(s, join_ev[g - 1], 0));
+            }
+        }
+        HIPCHK(hipGetLastError());
     }
 
     void step(const int32_t* ids, int n, int nsteps, hipStream_t s) override {
@@ -3079,6 +3164,16 @@ int sofa_amd_engine_set_branches(sofa_amd_engine eng, int branches) {
     HIPCHK(hipSetDevice(eng->impl->device));
     HIPCHK(hipDeviceSynchronize());
     eng->impl->branches = branches;
+    return SOFA_AMD_OK;
+    GUARD_END
+}
+
+int sofa_amd_engine_set_branch_turns(sofa_amd_engine eng, int enable) {
+    GUARD_BEGIN
+    if (!eng || !eng->impl) return fail(SOFA_AMD_EINVAL, "null engine");
+    HIPCHK(hipSetDevice(eng->impl->device));
+    HIPCHK(hipDeviceSynchronize());
+    eng->impl->branch_turns = enable != 0;
     return SOFA_AMD_OK;
     GUARD_END
 }

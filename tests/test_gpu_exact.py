@@ -48,7 +48,7 @@ def _specs(which):
     return [mix[i] for i in MIX_BEAMS], 100
 
 
-def _run(prec, specs, nsteps, reference_order=False, graphs=True, branches=0):
+def _run(prec, specs, nsteps, reference_order=False, graphs=True, branches=0, turns=False):
     from sofa_amd import workloads
     capi = _capi()
     eng = capi.Engine(0, getattr(capi, prec))
@@ -58,6 +58,7 @@ def _run(prec, specs, nsteps, reference_order=False, graphs=True, branches=0):
     eng.commit()
     eng.set_graphs(graphs)
     eng.set_branches(branches)
+    eng.set_branch_turns(turns)
     eng.step(ids, nsteps)
     out = [eng.read(i)[:2] + (eng.stats(i),) for i in ids]
     eng.close()
@@ -104,15 +105,19 @@ def test_exact_tiled_eager_equals_graph():
         assert np.array_equal(xa, xb) and np.array_equal(va, vb)
 
 
-@pytest.mark.parametrize("graphs", [False, True])
-def test_exact_tiled_branches(graphs):
+@pytest.mark.parametrize("mode", ["turns", "free_eager", "free_graphs"])
+def test_exact_tiled_branches(mode):
     """A merged step in concurrent branches (scenes split over streams) equals the one-branch step
-    bit for bit, eager and replayed from graphs: scenes are independent."""
+    bit for bit, with the branches' operator phases taking turns and running freely (eager and
+    replayed from graphs): scenes are independent.  The scenes' CG budgets differ (10 to 25
+    iterations), so that groups end their iterations at different turns."""
     specs, _ = _specs("mix8")
-    print(f"case: exact tiled, 8 beams, branches 1 / 3 / 4, graphs {graphs}", flush=True)
-    one = _run("FP64_TILED", specs, 15, reference_order=True, graphs=graphs, branches=1)
-    for br in (3, 4):
-        got = _run("FP64_TILED", specs, 15, reference_order=True, graphs=graphs, branches=br)
+    specs = [dict(s, iters=10 + 5 * (k % 4)) for k, s in enumerate(specs)]
+    graphs, turns = mode == "free_graphs", mode == "turns"
+    print(f"case: exact tiled, 8 beams, branches 1 / 2 / 3 / 4, {mode}", flush=True)
+    one = _run("FP64_TILED", specs, 15, reference_order=True, graphs=graphs, branches=1, turns=turns)
+    for br in (2, 3, 4):
+        got = _run("FP64_TILED", specs, 15, reference_order=True, graphs=graphs, branches=br, turns=turns)
         for (xa, va, sa), (xb, vb, sb) in zip(one, got):
             assert np.array_equal(xa, xb) and np.array_equal(va, vb)
             assert sa["iterations"] == sb["iterations"]
