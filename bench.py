@@ -285,6 +285,12 @@ def mix_paths(specs, args):
         total_ms, lat = time_merged(eng, ids, args.steps, args.warmup)
         row.update({"value": E * args.steps / (total_ms / 1e3), "ms_per_step": total_ms / args.steps,
                     "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99))})
+        if prec == "FP64_TILED" and seq:
+            # the node-after-node dot products run in 8 segments at once from candidate starts
+            # (the library's default, the same bits); boundaries summed from the known start instead
+            row["dot_segments"] = {"segments": 8, "window": 128,
+                                   "fallbacks": int(sum(eng.stats(i)["dot_fallbacks"] for i in ids)),
+                                   "steps": args.error_steps + args.warmup + args.steps}
         if name in _PATH_BYTES:
             pe, pv, extra = _PATH_BYTES[name]
             r = apply_roofline(eng, args.apply_reps, pe, pv, 0, E, N, node_extra=extra)

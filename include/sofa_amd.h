@@ -67,6 +67,11 @@ typedef struct sofa_amd_cg_stats {
     double  err;            /* last sqrt(r.r)/|b|                                            */
     double  den;            /* last p.A.p                                                    */
     double  time;           /* context time of the scene                                     */
+    int32_t dot_fallbacks;  /* FP64_TILED in the reference order with segmented dot products:
+                               segment boundaries where no candidate start matched (the segment
+                               was summed from the known start instead) since the scene was
+                               placed; else 0                                                    */
+    int32_t reserved;       /* 0                                                                 */
 } sofa_amd_cg_stats;
 
 /* ---- library ---------------------------------------------------------------------------- */
@@ -268,6 +273,18 @@ int sofa_amd_engine_set_branches(sofa_amd_engine eng, int branches);
  * graph when graphs are on (measured faster on the 256-beam mix: 21.6 against 22.3 ms/step with
  * 4 branches).  Results do not depend on it.  No reference counterpart. */
 int sofa_amd_engine_set_branch_turns(sofa_amd_engine eng, int enable);
+
+/* FP64_TILED engines in the reference's summation order (set before the scenes are placed): how
+ * each scene's node-after-node dot product (|b|^2, p.q, r.r) is computed.  segments = 1: one
+ * single-wave chain of dependent adds per scene.  segments = 2, 4 or 8 (default 8): the chain's
+ * segments run at once, each segment after the first from 2 `window` candidate start values
+ * (consecutive doubles around a plain sum of the terms before it; window 0, 32, 64, 96 or 128,
+ * default 128), and the sum is then resolved boundary by boundary: the candidate equal to the
+ * sum so far, bit for bit, continues it; where none is, the segment is summed from the sum so
+ * far (counted in sofa_amd_cg_stats::dot_fallbacks).  Every add of the chain happens in the
+ * reference's order on its exact operands: the result has the same bits either way.  Scenes
+ * under 8192 nodes always use the single chain.  No reference counterpart. */
+int sofa_amd_engine_set_dot_segments(sofa_amd_engine eng, int segments, int window);
 
 /* SOFA_AMD_FP64 and SOFA_AMD_FP64_TILED engines: order of the sums.  0 (default): CG dot
  * products (|b|^2, p.q, r.r) as per-block sums combined in a fixed order (parallel,

@@ -27,7 +27,7 @@ EXPORTS = [
     "sofa_amd_scene_read", "sofa_amd_scene_write", "sofa_amd_scene_stats", "sofa_amd_scene_element",
     "sofa_amd_scene_von_mises", "sofa_amd_scene_stiffness",
     "sofa_amd_fem_add_force", "sofa_amd_fem_add_dforce", "sofa_amd_engine_bench_apply", "sofa_amd_scene_tiling",
-    "sofa_amd_engine_set_graphs", "sofa_amd_engine_set_branches", "sofa_amd_engine_set_branch_turns", "sofa_amd_scene_set_mass_density", "sofa_amd_scene_add_oscillators", "sofa_amd_scene_add_linear_movement", "sofa_amd_scene_set_preconditioner", "sofa_amd_scene_set_warm_start",
+    "sofa_amd_engine_set_graphs", "sofa_amd_engine_set_branches", "sofa_amd_engine_set_branch_turns", "sofa_amd_engine_set_dot_segments", "sofa_amd_scene_set_mass_density", "sofa_amd_scene_add_oscillators", "sofa_amd_scene_add_linear_movement", "sofa_amd_scene_set_preconditioner", "sofa_amd_scene_set_warm_start",
     "sofa_amd_scene_set_refinement", "sofa_amd_engine_reserve", "sofa_amd_scene_commit", "sofa_amd_scene_release",
     "sofa_amd_engine_usage", "sofa_amd_scene_set_fix_all", "sofa_amd_scene_set_integrator", "sofa_amd_scene_set_fem_hexa", "sofa_amd_scene_add_plane", "sofa_amd_scene_set_project_velocity", "sofa_amd_engine_set_dot_order",
     "sofa_amd_scene_prepare", "sofa_amd_engine_autotune", "sofa_amd_engine_set_apply_variant",
@@ -49,7 +49,7 @@ class SofaAmdError(RuntimeError):
 class CGStats(ctypes.Structure):
     _fields_ = [("iterations", ctypes.c_int32), ("end_reason", ctypes.c_int32), ("step_count", ctypes.c_int32),
                 ("ir_sweeps", ctypes.c_int32), ("err", ctypes.c_double), ("den", ctypes.c_double),
-                ("time", ctypes.c_double)]
+                ("time", ctypes.c_double), ("dot_fallbacks", ctypes.c_int32), ("reserved", ctypes.c_int32)]
 
 
 _lib = None
@@ -104,6 +104,7 @@ def lib():
         "sofa_amd_engine_set_graphs": (c_int, [_vp, c_int]),
         "sofa_amd_engine_set_branches": (c_int, [_vp, c_int]),
         "sofa_amd_engine_set_branch_turns": (c_int, [_vp, c_int]),
+        "sofa_amd_engine_set_dot_segments": (c_int, [_vp, c_int, c_int]),
         "sofa_amd_scene_set_mass_density": (c_int, [_vp, c_int, c_int, _dp, c_int]),
         "sofa_amd_scene_add_oscillators": (c_int, [_vp, c_int, c_int, _ip, _dp, _dp, _dp, _dp]),
         "sofa_amd_scene_add_linear_movement": (c_int, [_vp, c_int, c_int, _ip, c_int, _dp, _dp, c_int]),
@@ -463,6 +464,11 @@ class Engine:
         """Concurrent branches of a merged step (0: the engine's default)."""
         check(lib().sofa_amd_engine_set_branches(self._h, int(branches)))
 
+    def set_dot_segments(self, segments, window=128):
+        """Reference-order dot products in `segments` segments run at once (1: the single chain),
+        from 2 `window` candidate starts per boundary."""
+        check(lib().sofa_amd_engine_set_dot_segments(self._h, int(segments), int(window)))
+
     def set_branch_turns(self, enable):
         """Reference-order steps in branches: operator phases take turns (True) or the branches
         run freely (False, default)."""
@@ -489,7 +495,8 @@ class Engine:
         st = CGStats()
         check(lib().sofa_amd_scene_stats(self._h, int(scene), ctypes.byref(st)))
         return {"iterations": st.iterations, "end_reason": st.end_reason, "step_count": st.step_count,
-                "ir_sweeps": st.ir_sweeps, "err": st.err, "den": st.den, "time": st.time}
+                "ir_sweeps": st.ir_sweeps, "err": st.err, "den": st.den, "time": st.time,
+                "dot_fallbacks": st.dot_fallbacks}
 
     def element(self, scene, e):
         rot = np.zeros(9); k = np.zeros(3); rest = np.zeros(12)

@@ -330,6 +330,8 @@ struct EngineBase {
     bool stepped = false;             // a step has been enqueued
     int branches = 0;                 // concurrent branches of a merged step (0: the engine's default)
     bool branch_turns = false;        // the branches' operator phases take turns (reference order)
+    int dot_segments = 8;             // reference-order dots on FP64_TILED: segments run at once
+    int dot_window = 128;             // ... candidate start values per side of each boundary
     double* vdot_part = nullptr;      // vDot's per-block partials (grown on demand)
     int64_t vdot_len = 0;
     virtual int autotune(int reps, hipStream_t s, float* us_out) = 0;
@@ -391,6 +393,8 @@ struct EngineT : EngineBase {
     std::vector<std::vector<int>> iperm;           // per scene: original element -> position
     VT* p2 = nullptr;                              // previous CG direction (exact path)
     double* nd = nullptr;                          // per-node dot terms (exact path, reference order)
+    double* spec_part = nullptr;                   // segmented reference-order dots: prediction chunks
+    double* spec_seg = nullptr;                    // ... and the segments' candidate sums
     double* pinv = nullptr;                        // PCG: inverted preconditioner block per node (9)
     VT* zv = nullptr;                              // PCG: preconditioned residual z = P^-1 r
     int *em_ptr = nullptr, *em_nbr = nullptr;      // MeshMatrixMass edges per node (exact path)
@@ -481,7 +485,7 @@ struct EngineT : EngineBase {
                         ln_inc, sp0, sp1, pbb, pden, prho, p2, em_ptr, em_nbr, em_val, emass, osc_node,
                         osc_mean, osc_amp, osc_w, osc_phase, kmat32, emass32, r32, p32, q32,
                         d32, sp32, pden32, prho32, prr, inst_in, sh_of, nd, x0, x32, x0_32, contact, inc_rank, pinv, zv,
-                        inc_arank, sh_acnt, ln_lead, dinv32};
+                        inc_arank, sh_acnt, ln_lead, dinv32, spec_part, spec_seg};
         for (void* ptr : ptrs)
             if (ptr) (void)hipFree(ptr);
     }
@@ -953,6 +957,7 @@ struct EngineT : EngineBase {
         pbb = pden = prho = nullptr; kmat32 = emass32 = nullptr; x32 = x0_32 = nullptr;
         r32 = p32 = q32 = d32 = sp32 = nullptr; pden32 = prho32 = prr = nullptr; inst_in = nullptr; sh_of = nullptr; inc_rank = nullptr;
         pinv = nullptr; zv = nullptr; inc_arank = nullptr; sh_acnt = nullptr; ln_lead = nullptr; dinv32 = nullptr;
+        spec_part = spec_seg = nullptr;
         cap = c;
         if (cap.node >= (int64_t)1 << 30 || cap.inc >= (int64_t)1 << 31 || cap.lnode >= (int64_t)1 << 31 ||
             cap.slot >= (int64_t)kSlotOwner)
@@ -987,6 +992,10 @@ struct EngineT : EngineBase {
         osc_node = dalloc<int>(cap.osc); osc_mean = dalloc<double>(3 * cap.osc); osc_amp = dalloc<double>(3 * cap.osc);
         osc_w = dalloc<double>(cap.osc); osc_phase = dalloc<double>(cap.osc);
         inst = dalloc<InstDev>(cap.sid); ranges_d = dalloc<Ranges>(cap.sid);
+        if (seq_dots && !mixed && std::is_same<R, double>::value) {   // segmented reference-order dots
+            spec_part = dalloc<double>((size_t)std::max<int64_t>(1, cap.sid) * kSpecChunks);
+            spec_seg = dalloc<double>((size_t)std::max<int64_t>(1, cap.sid) * kSpecStride);
+        }
         for (double* pb : {pbb, pden}) HIPCHK(hipMemset(pb, 0, std::max<size_t>(1, np) * sizeof(double)));
         HIPCHK(hipMemset(prho, 0, std::max<int64_t>(1, cap.nblk) * sizeof(double)));
         if (mixed) {
@@ -1386,7 +1395,7 @@ struct EngineT : EngineBase {
         if (cur->nb > 0)
             k_fin_rhs<VT, true><<<cur->nb, kBlock, 0, s>>>(fb, inst, ranges_d, sn, mass, fixed, v, sp0, sp1, f, b, r, xcg,
                                                              pbb, x, contact, nd);
-        k_seqdot<<<n, 64, 0, s>>>(cur->scenes, inst, nd, kDotBB);
+        launch_seqdot(n, s, kDotBB);
     }
     // CG iteration `it`'s operator (q = A p, the p.q terms): the throughput-bound phase
     void exact_operator(int it, hipStream_t s) {
@@ -1401,11 +1410,25 @@ struct EngineT : EngineBase {
     void exact_dots(int it, int n, hipStream_t s) {
         const NodeBlk* nb = (const NodeBlk*)cur->c;
         const SharedNodes sn{sh_gid, sh_ptr, sh_of, sh_acnt};
-        k_seqdot<<<n, 64, 0, s>>>(cur->scenes, inst, nd, kDotPQ);
+        launch_seqdot(n, s, kDotPQ);
         if (cur->nc > 0)
             k_update<VT, true><<<cur->nc, kBlock, 0, s>>>(nb, inst, ranges_d, it, sn, sp0, mass, fixed, p, q, xcg, r,
                                                           pden, prho, contact, nd);
-        k_seqdot<<<n, 64, 0, s>>>(cur->scenes, inst, nd, kDotRR);
+        launch_seqdot(n, s, kDotRR);
+    }
+    // a reference-order dot product of the launch set's n scenes into InstDev::dots[which]: the
+    // single chain per scene (k_seqdot), or the chain in dot_segments segments run at once from
+    // 2 dot_window candidate starts per boundary (k_seqdot_part / _spec / _resolve, the same bits)
+    static constexpr int kSpecMinNodes = 8192;   // smaller scenes: the single chain (under 40 us)
+    void launch_seqdot(int n, hipStream_t s, int which) {
+        if (spec_seg && dot_segments > 1) {
+            const SpecDots sd{spec_part, spec_seg, dot_segments, dot_window / 32, kSpecMinNodes};
+            k_seqdot_part<<<n * kSpecChunks, 64, 0, s>>>(cur->scenes, inst, nd, sd);
+            k_seqdot_spec<<<n * (1 + (sd.K - 1) * sd.W), 64, 0, s>>>(cur->scenes, inst, nd, sd, which);
+            k_seqdot_resolve<<<n, 64, 0, s>>>(cur->scenes, inst, nd, sd, which);
+        } else {
+            k_seqdot<<<n, 64, 0, s>>>(cur->scenes, inst, nd, which);
+        }
     }
     // the Euler update and the step's end
     void exact_close(int n, hipStream_t s) {
@@ -1818,6 +1841,8 @@ struct EngineT : EngineBase {
         out->err = d.err;
         out->den = d.den;
         out->time = d.time;
+        out->dot_fallbacks = d.dot_fallbacks;
+        out->reserved = 0;
     }
     void element(int scene, int e, double* rot9, double* k3, double* rest12) override {
         HIPCHK(hipDeviceSynchronize());
@@ -2028,6 +2053,8 @@ struct EngineT : EngineBase {
             st->err = after.err;
             st->den = after.den;
             st->time = after.time;
+            st->dot_fallbacks = 0;
+            st->reserved = 0;
         }
         after.m_cg = saved.m_cg;
         after.k_cg = saved.k_cg;
@@ -3164,6 +3191,22 @@ int sofa_amd_engine_set_branches(sofa_amd_engine eng, int branches) {
     HIPCHK(hipSetDevice(eng->impl->device));
     HIPCHK(hipDeviceSynchronize());
     eng->impl->branches = branches;
+    return SOFA_AMD_OK;
+    GUARD_END
+}
+
+int sofa_amd_engine_set_dot_segments(sofa_amd_engine eng, int segments, int window) {
+    GUARD_BEGIN
+    if (!eng || !eng->impl) return fail(SOFA_AMD_EINVAL, "null engine");
+    if (segments != 1 && segments != 2 && segments != 4 && segments != 8)
+        return fail(SOFA_AMD_EINVAL, "segments must be 1, 2, 4 or 8");
+    if (window < 0 || window > 32 * kSpecMaxWaves || window % 32 != 0)
+        return fail(SOFA_AMD_EINVAL, "window must be 0, 32, 64, 96 or 128");
+    HIPCHK(hipSetDevice(eng->impl->device));
+    HIPCHK(hipDeviceSynchronize());
+    eng->impl->dot_segments = segments;
+    eng->impl->dot_window = window;
+    eng->impl->reset_graph();
     return SOFA_AMD_OK;
     GUARD_END
 }

@@ -92,6 +92,9 @@ struct InstDev {
     // the solve starting from the previous step's solution
     int pcg, warm_start;
     double bnorm2, pcg_tol;
+    // segmented reference-order dot products: boundaries where no candidate matched (counted
+    // since the scene was placed)
+    int dot_fallbacks, pad_fb;
 };
 
 // which entry of InstDev::dots a k_seqdot launch writes
@@ -126,30 +129,28 @@ __device__ __forceinline__ double seq_group_add(double s, const double g[kSeqGro
     for (int k = 0; k < kSeqGroup; k++) s = s + g[k];
     return s;
 }
-static __global__ __launch_bounds__(64) void k_seqdot(const int* __restrict__ scenes, InstDev* __restrict__ inst,
-                                                       const double* __restrict__ nd, int which) {
-    __shared__ __align__(16) double buf[2][kSeqChunk];
-    InstDev* ins = &inst[scenes[blockIdx.x]];
-    if (ins->stopped) return;
-    const int b0 = ins->node_begin, n = ins->node_count;
+// the chain of k_seqdot: s + t[begin] + t[begin + 1] + ... + t[end - 1], one dependent add after
+// the other from the start value s (which may differ from lane to lane), the terms staged through
+// `buf` as described above.  Called by every lane of a 64-thread workgroup (it synchronizes).
+__device__ __forceinline__ double seq_chain(const double* __restrict__ t, int begin, int end, double s,
+                                            double (*buf)[kSeqChunk]) {
     const int lane = threadIdx.x;
     constexpr int PER = kSeqChunk / 64;
     double reg[PER];
 #pragma unroll
     for (int k = 0; k < PER; k++) {
-        const int i = k * 64 + lane;
-        reg[k] = i < n ? nd[b0 + i] : 0.0;
+        const int i = begin + k * 64 + lane;
+        reg[k] = i < end ? t[i] : 0.0;
     }
-    double s = 0.0;
     int round = 0;
-    for (int base = 0; base < n; base += kSeqChunk, round ^= 1) {
+    for (int base = begin; base < end; base += kSeqChunk, round ^= 1) {
 #pragma unroll
         for (int k = 0; k < PER; k++) buf[round][k * 64 + lane] = reg[k];
         const int nb = base + kSeqChunk;
 #pragma unroll
         for (int k = 0; k < PER; k++) {
             const int i = nb + k * 64 + lane;
-            reg[k] = i < n ? nd[b0 + i] : 0.0;
+            reg[k] = i < end ? t[i] : 0.0;
         }
         __syncthreads();
         const double* c = buf[round];
@@ -169,7 +170,142 @@ static __global__ __launch_bounds__(64) void k_seqdot(const int* __restrict__ sc
         }
         __syncthreads();   // the buffer of this round is rewritten two rounds later
     }
-    if (lane == 0) ins->dots[which] = s;
+    return s;
+}
+static __global__ __launch_bounds__(64) void k_seqdot(const int* __restrict__ scenes, InstDev* __restrict__ inst,
+                                                       const double* __restrict__ nd, int which) {
+    __shared__ __align__(16) double buf[2][kSeqChunk];
+    InstDev* ins = &inst[scenes[blockIdx.x]];
+    if (ins->stopped) return;
+    const double s = seq_chain(nd + ins->node_begin, 0, ins->node_count, 0.0, buf);
+    if (threadIdx.x == 0) ins->dots[which] = s;
+}
+
+// ---------------------------------------------------------------------------------------
+// The same chain in K segments run at once, bit for bit.  A segment's chain can only start once
+// the previous segment's sum s_k is known; instead, W waves run segment k from 64 W candidate
+// start values at once, one per lane: the doubles whose ordered integer images (dbl_ord) lie
+// within 32 W of that of a prediction of s_k (a plain sum of the terms before the segment).  The
+// sequential sum s_k differs from such a prediction by its accumulated rounding, a random walk
+// of a few tens of ulps over tens of thousands of terms.  Afterwards one wave per scene walks
+// the boundaries: s_0 is segment 0's sum from +0.0; at boundary k the candidate equal to s_k
+// (compared bit for bit) gives s_{k+1} as the sequential chain from s_k would; when no candidate
+// equals s_k the wave runs segment k itself from s_k.  Either way every add of the reference's
+// chain happens in its order on its exact operands: the result is the sequential sum's bits.
+// Segments are aligned on the 64 prediction chunks of a scene (each n / 64 terms, a multiple of
+// 64, summed by k_seqdot_part); scenes below a node count run the single chain.
+// ---------------------------------------------------------------------------------------
+constexpr int kSpecChunks = 64;   // prediction chunks per scene
+// per scene record: [0] segment 0's sum; then per boundary k = 1 .. K - 1: the prediction, then
+// the 64 W candidates' sums (K <= 8, W <= 4)
+constexpr int kSpecMaxSeg = 8, kSpecMaxWaves = 4;
+constexpr int kSpecStride = 1 + (kSpecMaxSeg - 1) * (64 * kSpecMaxWaves + 1);
+struct SpecDots {
+    double* part;       // kSpecChunks per scene record
+    double* seg;        // kSpecStride per scene record
+    int K, W, min_n;    // segments (a power of two dividing kSpecChunks), candidate waves, least nodes
+};
+// a monotone integer image of a double (+0.0 and -0.0 both 0) and its inverse (to +0.0)
+__device__ __forceinline__ long long dbl_ord(double x) {
+    const long long b = __double_as_longlong(x);
+    return b >= 0 ? b : -(b & 0x7fffffffffffffffLL);
+}
+__device__ __forceinline__ double ord_dbl(long long o) {
+    return o >= 0 ? __longlong_as_double(o) : __longlong_as_double((-o) | (long long)0x8000000000000000ULL);
+}
+// candidate j of a boundary (wrapping integer arithmetic: the same bits wherever it is computed)
+__device__ __forceinline__ double spec_candidate(double pred, int j, int W) {
+    return ord_dbl((long long)((unsigned long long)dbl_ord(pred) + (unsigned long long)(long long)(j - 32 * W)));
+}
+__device__ __forceinline__ int spec_chunk(int n) { return ((n + kSpecChunks - 1) / kSpecChunks + 63) & ~63; }
+__device__ __forceinline__ int spec_bound(int n, int k, int K) {   // first term of segment k
+    return min(k * (kSpecChunks / K) * spec_chunk(n), n);
+}
+__device__ __forceinline__ double wave_sum_f64(double v) {
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+    return v;
+}
+// prediction chunks: one 64-thread workgroup per (scene, chunk), a plain sum (any order)
+static __global__ __launch_bounds__(64) void k_seqdot_part(const int* __restrict__ scenes, const InstDev* __restrict__ inst,
+                                                            const double* __restrict__ nd, SpecDots sd) {
+    const int sc = scenes[blockIdx.x / kSpecChunks], c = blockIdx.x % kSpecChunks;
+    const InstDev* ins = &inst[sc];
+    const int n = ins->node_count;
+    if (ins->stopped || n < sd.min_n) return;
+    const int cs = spec_chunk(n);
+    const int i0 = min(c * cs, n), i1 = min(i0 + cs, n);
+    const double* t = nd + ins->node_begin;
+    double a[4] = {0.0, 0.0, 0.0, 0.0};
+    int i = i0 + (int)threadIdx.x;
+    for (; i + 192 < i1; i += 256) {
+        a[0] += t[i]; a[1] += t[i + 64]; a[2] += t[i + 128]; a[3] += t[i + 192];
+    }
+    for (; i < i1; i += 64) a[0] += t[i];
+    const double v = wave_sum_f64((a[0] + a[1]) + (a[2] + a[3]));
+    if (threadIdx.x == 0) sd.part[(size_t)sc * kSpecChunks + c] = v;
+}
+// the segments: 1 + (K - 1) W workgroups of 64 per scene (slot 0: segment 0 from +0.0; slot
+// 1 + (k - 1) W + w: segment k from candidates w 64 .. w 64 + 63); a scene below min_n nodes
+// runs its whole chain in slot 0 and writes the result
+static __global__ __launch_bounds__(64) void k_seqdot_spec(const int* __restrict__ scenes, InstDev* __restrict__ inst,
+                                                            const double* __restrict__ nd, SpecDots sd, int which) {
+    __shared__ __align__(16) double buf[2][kSeqChunk];
+    const int slots = 1 + (sd.K - 1) * sd.W;
+    const int sc = scenes[blockIdx.x / slots], slot = blockIdx.x % slots;
+    InstDev* ins = &inst[sc];
+    if (ins->stopped) return;
+    const int n = ins->node_count;
+    const double* t = nd + ins->node_begin;
+    const int lane = threadIdx.x;
+    if (n < sd.min_n) {
+        if (slot != 0) return;
+        const double s = seq_chain(t, 0, n, 0.0, buf);
+        if (lane == 0) ins->dots[which] = s;
+        return;
+    }
+    double* out = sd.seg + (size_t)sc * kSpecStride;
+    if (slot == 0) {
+        const double s = seq_chain(t, 0, spec_bound(n, 1, sd.K), 0.0, buf);
+        if (lane == 0) out[0] = s;
+        return;
+    }
+    const int k = 1 + (slot - 1) / sd.W, w = (slot - 1) % sd.W;
+    const int nc = k * (kSpecChunks / sd.K);   // chunks before the segment
+    const double pred = wave_sum_f64(lane < nc ? sd.part[(size_t)sc * kSpecChunks + lane] : 0.0);
+    const double s = seq_chain(t, spec_bound(n, k, sd.K), spec_bound(n, k + 1, sd.K),
+                               spec_candidate(pred, w * 64 + lane, sd.W), buf);
+    double* o = out + 1 + (size_t)(k - 1) * (64 * sd.W + 1);
+    if (w == 0 && lane == 0) o[0] = pred;
+    o[1 + w * 64 + lane] = s;
+}
+// the boundaries, one wave per scene (every lane the same walk)
+static __global__ __launch_bounds__(64) void k_seqdot_resolve(const int* __restrict__ scenes, InstDev* __restrict__ inst,
+                                                               const double* __restrict__ nd, SpecDots sd, int which) {
+    __shared__ __align__(16) double buf[2][kSeqChunk];
+    const int sc = scenes[blockIdx.x];
+    InstDev* ins = &inst[sc];
+    const int n = ins->node_count;
+    if (ins->stopped || n < sd.min_n) return;
+    const double* out = sd.seg + (size_t)sc * kSpecStride;
+    const double* t = nd + ins->node_begin;
+    double s = out[0];
+    for (int k = 1; k < sd.K; k++) {
+        const double* o = out + 1 + (size_t)(k - 1) * (64 * sd.W + 1);
+        const double pred = o[0];
+        const long long d = (long long)((unsigned long long)dbl_ord(s) - (unsigned long long)dbl_ord(pred)) + 32LL * sd.W;
+        bool hit = false;
+        if (sd.W > 0 && d >= 0 && d < 64LL * sd.W) {
+            const int j = (int)d;
+            hit = __double_as_longlong(spec_candidate(pred, j, sd.W)) == __double_as_longlong(s);
+            if (hit) s = o[1 + j];
+        }
+        if (!hit) {
+            s = seq_chain(t, spec_bound(n, k, sd.K), spec_bound(n, k + 1, sd.K), s, buf);
+            if (threadIdx.x == 0) ins->dot_fallbacks++;
+        }
+    }
+    if (threadIdx.x == 0) ins->dots[which] = s;
 }
 
 struct Blk { int inst, begin, end, pad; };   // a block's scene and [begin, end) range

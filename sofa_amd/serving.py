@@ -44,10 +44,13 @@ def footprint_bytes(precision, nodes, tets, reference_order=False):
         # float r, p, q, d, x, x0 and shared partials; float K entries and edge masses
         per_node += 6 * 3 * 4 + 3 * 4
         per_tet += (3 + 1) * 4
+    per_scene = 0
     if reference_order:
         per_tet += 4 * 2 * 3 * pv + 4 * 2
         per_node += 8
-    return per_node * nodes + per_tet * tets
+        # the segmented dot products' prediction chunks and candidate sums (64 + 1800 doubles)
+        per_scene += 8 * (64 + 1800)
+    return per_node * nodes + per_tet * tets + per_scene
 
 
 class Server:

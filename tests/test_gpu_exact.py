@@ -48,12 +48,14 @@ def _specs(which):
     return [mix[i] for i in MIX_BEAMS], 100
 
 
-def _run(prec, specs, nsteps, reference_order=False, graphs=True, branches=0, turns=False):
+def _run(prec, specs, nsteps, reference_order=False, graphs=True, branches=0, turns=False, segments=None):
     from sofa_amd import workloads
     capi = _capi()
     eng = capi.Engine(0, getattr(capi, prec))
     if reference_order:
         eng.set_dot_order(True)
+    if segments is not None:
+        eng.set_dot_segments(*segments)
     ids = [workloads.add_to_engine(eng, s) for s in specs]
     eng.commit()
     eng.set_graphs(graphs)
@@ -121,6 +123,45 @@ def test_exact_tiled_branches(mode):
         for (xa, va, sa), (xb, vb, sb) in zip(one, got):
             assert np.array_equal(xa, xb) and np.array_equal(va, vb)
             assert sa["iterations"] == sb["iterations"]
+
+
+# beams of the 256-beam mix above the 8192 nodes from which the segmented dot products apply
+# (8528, 9088 and 9104 nodes), and one below
+SEG_BEAMS = [67, 151, 10, 53]
+
+
+def test_exact_tiled_dot_segments():
+    """Reference-order dot products with the chain's segments run at once from candidate starts
+    give the single chain's bits: 2, 4 and 8 segments, windows of 32 to 128 candidates per side,
+    and window 0, where every boundary falls back to summing its segment from the known start.
+    The largest beam's trajectory also equals the oracle's reference-order run bit for bit."""
+    from sofa_amd import workloads
+    mix = workloads.cfg3()
+    specs = [mix[i] for i in SEG_BEAMS]
+    nsteps = 20
+    print(f"case: exact tiled dot segments, beams {SEG_BEAMS}, {nsteps} steps", flush=True)
+    one = _run("FP64_TILED", specs, nsteps, reference_order=True, segments=(1, 128))
+    for seg in [(4, 128), (8, 0), (2, 32), (8, 96)]:
+        got = _run("FP64_TILED", specs, nsteps, reference_order=True, segments=seg)
+        for k, ((xa, va, sa), (xb, vb, sb)) in enumerate(zip(one, got)):
+            assert np.array_equal(xa, xb) and np.array_equal(va, vb), (seg, k, rel_err(xa, xb))
+            assert sa["iterations"] == sb["iterations"] and sa["end_reason"] == sb["end_reason"]
+            small = len(specs[k]["pos"]) < 8192
+            # per step |b|^2 and per CG iteration p.q and r.r, K - 1 boundaries each
+            bounds = (seg[0] - 1) * nsteps * (1 + 2 * sb["iterations"])
+            print(f"  segments {seg}: scene {k} fallbacks {sb['dot_fallbacks']} of about {bounds} boundaries",
+                  flush=True)
+            if small:
+                assert sb["dot_fallbacks"] == 0
+            elif seg[1] == 0:
+                assert sb["dot_fallbacks"] >= bounds // 2
+            else:   # the candidates cover most boundaries (the rest is summed from the known start)
+                assert sb["dot_fallbacks"] <= bounds // 2
+    o = oracle_from_spec(specs[2], dot_mode=0)
+    o.step(specs[2]["dt"], nsteps)
+    x, v, st = one[2]
+    assert np.array_equal(got[2][0], o.get("x")), rel_err(got[2][0], o.get("x"))
+    assert np.array_equal(got[2][1], o.get("v"))
 
 
 def test_exact_tiled_refuses_consistent_mass():
