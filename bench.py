@@ -288,7 +288,7 @@ def mix_paths(specs, args):
         if prec == "FP64_TILED" and seq:
             # the node-after-node dot products run in 8 segments at once from candidate starts
             # (the library's default, the same bits); boundaries summed from the known start instead
-            row["dot_segments"] = {"segments": 8, "window": 128,
+            row["dot_segments"] = {"segments": 8, "window": 32,
                                    "fallbacks": int(sum(eng.stats(i)["dot_fallbacks"] for i in ids)),
                                    "steps": args.error_steps + args.warmup + args.steps}
         if name in _PATH_BYTES:

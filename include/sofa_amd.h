@@ -278,8 +278,9 @@ int sofa_amd_engine_set_branch_turns(sofa_amd_engine eng, int enable);
  * each scene's node-after-node dot product (|b|^2, p.q, r.r) is computed.  segments = 1: one
  * single-wave chain of dependent adds per scene.  segments = 2, 4 or 8 (default 8): the chain's
  * segments run at once, each segment after the first from 2 `window` candidate start values
- * (consecutive doubles around a plain sum of the terms before it; window 0, 32, 64, 96 or 128,
- * default 128), and the sum is then resolved boundary by boundary: the candidate equal to the
+ * (consecutive doubles around a prediction of the sum before it: the terms each rounded as the
+ * chain would round them; window 0, 32, 64, 96 or 128, default 32), and the sum is then resolved
+ * boundary by boundary: the candidate equal to the
  * sum so far, bit for bit, continues it; where none is, the segment is summed from the sum so
  * far (counted in sofa_amd_cg_stats::dot_fallbacks).  Every add of the chain happens in the
  * reference's order on its exact operands: the result has the same bits either way.  Scenes

@@ -464,7 +464,7 @@ class Engine:
         """Concurrent branches of a merged step (0: the engine's default)."""
         check(lib().sofa_amd_engine_set_branches(self._h, int(branches)))
 
-    def set_dot_segments(self, segments, window=128):
+    def set_dot_segments(self, segments, window=32):
         """Reference-order dot products in `segments` segments run at once (1: the single chain),
         from 2 `window` candidate starts per boundary."""
         check(lib().sofa_amd_engine_set_dot_segments(self._h, int(segments), int(window)))

@@ -331,7 +331,7 @@ struct EngineBase {
     int branches = 0;                 // concurrent branches of a merged step (0: the engine's default)
     bool branch_turns = false;        // the branches' operator phases take turns (reference order)
     int dot_segments = 8;             // reference-order dots on FP64_TILED: segments run at once
-    int dot_window = 128;             // ... candidate start values per side of each boundary
+    int dot_window = 32;              // ... candidate start values per side of each boundary
     double* vdot_part = nullptr;      // vDot's per-block partials (grown on demand)
     int64_t vdot_len = 0;
     virtual int autotune(int reps, hipStream_t s, float* us_out) = 0;
@@ -394,6 +394,7 @@ struct EngineT : EngineBase {
     VT* p2 = nullptr;                              // previous CG direction (exact path)
     double* nd = nullptr;                          // per-node dot terms (exact path, reference order)
     double* spec_part = nullptr;                   // segmented reference-order dots: prediction chunks
+    double* spec_model = nullptr;                  // ... with the chain's roundings modelled
     double* spec_seg = nullptr;                    // ... and the segments' candidate sums
     double* pinv = nullptr;                        // PCG: inverted preconditioner block per node (9)
     VT* zv = nullptr;                              // PCG: preconditioned residual z = P^-1 r
@@ -485,7 +486,7 @@ struct EngineT : EngineBase {
                         ln_inc, sp0, sp1, pbb, pden, prho, p2, em_ptr, em_nbr, em_val, emass, osc_node,
                         osc_mean, osc_amp, osc_w, osc_phase, kmat32, emass32, r32, p32, q32,
                         d32, sp32, pden32, prho32, prr, inst_in, sh_of, nd, x0, x32, x0_32, contact, inc_rank, pinv, zv,
-                        inc_arank, sh_acnt, ln_lead, dinv32, spec_part, spec_seg};
+                        inc_arank, sh_acnt, ln_lead, dinv32, spec_part, spec_model, spec_seg};
         for (void* ptr : ptrs)
             if (ptr) (void)hipFree(ptr);
     }
@@ -957,7 +958,7 @@ struct EngineT : EngineBase {
         pbb = pden = prho = nullptr; kmat32 = emass32 = nullptr; x32 = x0_32 = nullptr;
         r32 = p32 = q32 = d32 = sp32 = nullptr; pden32 = prho32 = prr = nullptr; inst_in = nullptr; sh_of = nullptr; inc_rank = nullptr;
         pinv = nullptr; zv = nullptr; inc_arank = nullptr; sh_acnt = nullptr; ln_lead = nullptr; dinv32 = nullptr;
-        spec_part = spec_seg = nullptr;
+        spec_part = spec_model = spec_seg = nullptr;
         cap = c;
         if (cap.node >= (int64_t)1 << 30 || cap.inc >= (int64_t)1 << 31 || cap.lnode >= (int64_t)1 << 31 ||
             cap.slot >= (int64_t)kSlotOwner)
@@ -994,6 +995,7 @@ struct EngineT : EngineBase {
         inst = dalloc<InstDev>(cap.sid); ranges_d = dalloc<Ranges>(cap.sid);
         if (seq_dots && !mixed && std::is_same<R, double>::value) {   // segmented reference-order dots
             spec_part = dalloc<double>((size_t)std::max<int64_t>(1, cap.sid) * kSpecChunks);
+            spec_model = dalloc<double>((size_t)std::max<int64_t>(1, cap.sid) * kSpecChunks);
             spec_seg = dalloc<double>((size_t)std::max<int64_t>(1, cap.sid) * kSpecStride);
         }
         for (double* pb : {pbb, pden}) HIPCHK(hipMemset(pb, 0, std::max<size_t>(1, np) * sizeof(double)));
@@ -1418,12 +1420,14 @@ struct EngineT : EngineBase {
     }
     // a reference-order dot product of the launch set's n scenes into InstDev::dots[which]: the
     // single chain per scene (k_seqdot), or the chain in dot_segments segments run at once from
-    // 2 dot_window candidate starts per boundary (k_seqdot_part / _spec / _resolve, the same bits)
+    // 2 dot_window candidate starts per boundary (k_seqdot_part / _model / _spec / _resolve, the
+    // same bits)
     static constexpr int kSpecMinNodes = 8192;   // smaller scenes: the single chain (under 40 us)
     void launch_seqdot(int n, hipStream_t s, int which) {
         if (spec_seg && dot_segments > 1) {
-            const SpecDots sd{spec_part, spec_seg, dot_segments, dot_window / 32, kSpecMinNodes};
+            const SpecDots sd{spec_part, spec_model, spec_seg, dot_segments, dot_window / 32, kSpecMinNodes};
             k_seqdot_part<<<n * kSpecChunks, 64, 0, s>>>(cur->scenes, inst, nd, sd);
+            k_seqdot_model<<<n * kSpecChunks, 64, 0, s>>>(cur->scenes, inst, nd, sd);
             k_seqdot_spec<<<n * (1 + (sd.K - 1) * sd.W), 64, 0, s>>>(cur->scenes, inst, nd, sd, which);
             k_seqdot_resolve<<<n, 64, 0, s>>>(cur->scenes, inst, nd, sd, which);
         } else {

@@ -185,15 +185,14 @@ static __global__ __launch_bounds__(64) void k_seqdot(const int* __restrict__ sc
 // The same chain in K segments run at once, bit for bit.  A segment's chain can only start once
 // the previous segment's sum s_k is known; instead, W waves run segment k from 64 W candidate
 // start values at once, one per lane: the doubles whose ordered integer images (dbl_ord) lie
-// within 32 W of that of a prediction of s_k (a plain sum of the terms before the segment).  The
-// sequential sum s_k differs from such a prediction by its accumulated rounding, a random walk
-// of a few tens of ulps over tens of thousands of terms.  Afterwards one wave per scene walks
+// within 32 W of that of a prediction of s_k (the terms before the segment each rounded as the
+// chain would round them, k_seqdot_model).  Afterwards one wave per scene walks
 // the boundaries: s_0 is segment 0's sum from +0.0; at boundary k the candidate equal to s_k
 // (compared bit for bit) gives s_{k+1} as the sequential chain from s_k would; when no candidate
 // equals s_k the wave runs segment k itself from s_k.  Either way every add of the reference's
 // chain happens in its order on its exact operands: the result is the sequential sum's bits.
-// Segments are aligned on the 64 prediction chunks of a scene (each n / 64 terms, a multiple of
-// 64, summed by k_seqdot_part); scenes below a node count run the single chain.
+// Segments are aligned on the 64 prediction chunks of a scene (each n / 64 terms rounded up to a
+// multiple of 64; k_seqdot_part, k_seqdot_model); scenes below a node count run the single chain.
 // ---------------------------------------------------------------------------------------
 constexpr int kSpecChunks = 64;   // prediction chunks per scene
 // per scene record: [0] segment 0's sum; then per boundary k = 1 .. K - 1: the prediction, then
@@ -201,7 +200,8 @@ constexpr int kSpecChunks = 64;   // prediction chunks per scene
 constexpr int kSpecMaxSeg = 8, kSpecMaxWaves = 4;
 constexpr int kSpecStride = 1 + (kSpecMaxSeg - 1) * (64 * kSpecMaxWaves + 1);
 struct SpecDots {
-    double* part;       // kSpecChunks per scene record
+    double* part;       // kSpecChunks per scene record: plain chunk sums
+    double* model;      // kSpecChunks per scene record: chunk sums with the chain's roundings modelled
     double* seg;        // kSpecStride per scene record
     int K, W, min_n;    // segments (a power of two dividing kSpecChunks), candidate waves, least nodes
 };
@@ -245,6 +245,44 @@ static __global__ __launch_bounds__(64) void k_seqdot_part(const int* __restrict
     const double v = wave_sum_f64((a[0] + a[1]) + (a[2] + a[3]));
     if (threadIdx.x == 0) sd.part[(size_t)sc * kSpecChunks + c] = v;
 }
+// the prediction: a chunk's terms each rounded as the chain rounds them.  Adding t to a running
+// sum s that is a multiple of u = ulp(s + t) gives s + rint(t / u) u (round to nearest, ties to
+// even in t / u in place of the sum's last bit) as long as s + t stays in s's binade: terms under
+// half an ulp are absorbed, the others rounded to the sum's grid.  The running sum is taken as the
+// plain prefix (the chunk sums before the chunk, then a scan of the chunk), which decides only
+// the grid u; the chunk's modelled sum goes to sd.model.  One 64-thread workgroup per (scene,
+// chunk), after k_seqdot_part.
+__device__ __forceinline__ double spec_ulp(double a) {
+    const long long eb = __double_as_longlong(fabs(a)) & 0x7ff0000000000000LL;
+    return eb == 0 ? 4.9406564584124654e-324 : __longlong_as_double(eb) * 2.220446049250313e-16;
+}
+static __global__ __launch_bounds__(64) void k_seqdot_model(const int* __restrict__ scenes, const InstDev* __restrict__ inst,
+                                                             const double* __restrict__ nd, SpecDots sd) {
+    const int sc = scenes[blockIdx.x / kSpecChunks], c = blockIdx.x % kSpecChunks;
+    const InstDev* ins = &inst[sc];
+    const int n = ins->node_count;
+    if (ins->stopped || n < sd.min_n) return;
+    const int lane = threadIdx.x;
+    const int cs = spec_chunk(n);
+    const int i0 = min(c * cs, n), i1 = min(i0 + cs, n);
+    const double* t = nd + ins->node_begin;
+    double run = wave_sum_f64(lane < c ? sd.part[(size_t)sc * kSpecChunks + lane] : 0.0);
+    double acc = 0.0;
+    for (int g = i0; g < i1; g += 64) {
+        const int i = g + lane;
+        const double ti = i < i1 ? t[i] : 0.0;
+        double x = ti;   // inclusive scan of the group's terms
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const double y = __shfl_up(x, d, 64);
+            if (lane >= d) x += y;
+        }
+        const double u = spec_ulp(run + x);
+        acc += wave_sum_f64(rint(ti / u) * u);
+        run += __shfl(x, 63, 64);
+    }
+    if (lane == 0) sd.model[(size_t)sc * kSpecChunks + c] = acc;
+}
 // the segments: 1 + (K - 1) W workgroups of 64 per scene (slot 0: segment 0 from +0.0; slot
 // 1 + (k - 1) W + w: segment k from candidates w 64 .. w 64 + 63); a scene below min_n nodes
 // runs its whole chain in slot 0 and writes the result
@@ -272,7 +310,7 @@ static __global__ __launch_bounds__(64) void k_seqdot_spec(const int* __restrict
     }
     const int k = 1 + (slot - 1) / sd.W, w = (slot - 1) % sd.W;
     const int nc = k * (kSpecChunks / sd.K);   // chunks before the segment
-    const double pred = wave_sum_f64(lane < nc ? sd.part[(size_t)sc * kSpecChunks + lane] : 0.0);
+    const double pred = wave_sum_f64(lane < nc ? sd.model[(size_t)sc * kSpecChunks + lane] : 0.0);
     const double s = seq_chain(t, spec_bound(n, k, sd.K), spec_bound(n, k + 1, sd.K),
                                spec_candidate(pred, w * 64 + lane, sd.W), buf);
     double* o = out + 1 + (size_t)(k - 1) * (64 * sd.W + 1);

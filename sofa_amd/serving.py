@@ -48,8 +48,8 @@ def footprint_bytes(precision, nodes, tets, reference_order=False):
     if reference_order:
         per_tet += 4 * 2 * 3 * pv + 4 * 2
         per_node += 8
-        # the segmented dot products' prediction chunks and candidate sums (64 + 1800 doubles)
-        per_scene += 8 * (64 + 1800)
+        # the segmented dot products' prediction chunks (plain and modelled) and candidate sums
+        per_scene += 8 * (2 * 64 + 1800)
     return per_node * nodes + per_tet * tets + per_scene
 
 

@@ -141,7 +141,7 @@ def test_exact_tiled_dot_segments():
     nsteps = 20
     print(f"case: exact tiled dot segments, beams {SEG_BEAMS}, {nsteps} steps", flush=True)
     one = _run("FP64_TILED", specs, nsteps, reference_order=True, segments=(1, 128))
-    for seg in [(4, 128), (8, 0), (2, 32), (8, 96)]:
+    for seg in [(4, 128), (8, 0), (2, 32), (8, 96), (8, 32)]:
         got = _run("FP64_TILED", specs, nsteps, reference_order=True, segments=seg)
         for k, ((xa, va, sa), (xb, vb, sb)) in enumerate(zip(one, got)):
             assert np.array_equal(xa, xb) and np.array_equal(va, vb), (seg, k, rel_err(xa, xb))
@@ -155,8 +155,8 @@ def test_exact_tiled_dot_segments():
                 assert sb["dot_fallbacks"] == 0
             elif seg[1] == 0:
                 assert sb["dot_fallbacks"] >= bounds // 2
-            else:   # the candidates cover most boundaries (the rest is summed from the known start)
-                assert sb["dot_fallbacks"] <= bounds // 2
+            else:   # the modelled prediction lands within the candidates (0 misses measured)
+                assert sb["dot_fallbacks"] <= bounds // 10
     o = oracle_from_spec(specs[2], dot_mode=0)
     o.step(specs[2]["dt"], nsteps)
     x, v, st = one[2]

@@ -62,10 +62,10 @@ def test_footprint_monotonic():
     assert serving.footprint_bytes(capi.FP64_TILED, 1000, 5000) > serving.footprint_bytes(capi.FP32, 1000, 5000)
     assert serving.footprint_bytes(capi.MIXED, 1000, 5000) > serving.footprint_bytes(capi.FP64_TILED, 1000, 5000)
     # the reference summation order adds 4 x 2 slots of 3 doubles and 4 x 2 rank bytes per
-    # tetrahedron, a dot term per node, and per scene the segmented dot products' 64 prediction
+    # tetrahedron, a dot term per node, and per scene the segmented dot products' 2 x 64 prediction
     # chunks and 1 + 7 (64 x 4 + 1) candidate sums (doubles)
     d = serving.footprint_bytes(capi.FP64_TILED, 1000, 5000, True) - serving.footprint_bytes(capi.FP64_TILED, 1000, 5000)
-    assert d == 5000 * (4 * 2 * 3 * 8 + 8) + 1000 * 8 + 8 * (64 + 1 + 7 * (64 * 4 + 1))
+    assert d == 5000 * (4 * 2 * 3 * 8 + 8) + 1000 * 8 + 8 * (2 * 64 + 1 + 7 * (64 * 4 + 1))
 
 
 class _FakeEngine:
