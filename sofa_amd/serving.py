@@ -34,7 +34,7 @@ from . import capi, workloads
 # (16 + 8), rotation 9, rest 6 or 9, stiffness 3, edge mass 1 reals, element force 12 reals,
 # incidences 4 x (4 + 2); in the reference summation order also two per-incidence slot arrays
 # (4 x 2 x 3 reals) and two per-incidence rank tables (4 x 2 bytes)
-def footprint_bytes(precision, nodes, tets, reference_order=False):
+def footprint_bytes(precision, nodes, tets, reference_order=False, scenes=1):
     pv = 4 if precision == capi.FP32 else 8
     pe = 4 if precision == capi.FP32 else 8
     nr = 6 if precision == capi.FP32 else 9
@@ -50,7 +50,7 @@ def footprint_bytes(precision, nodes, tets, reference_order=False):
         per_node += 8
         # the segmented dot products' prediction chunks (plain and modelled) and candidate sums
         per_scene += 8 * (2 * 64 + 1800)
-    return per_node * nodes + per_tet * tets + per_scene
+    return per_node * nodes + per_tet * tets + per_scene * scenes
 
 
 class Server:
@@ -70,7 +70,7 @@ class Server:
         if engine is None:
             # arena capacity: the budget expressed in nodes and tetrahedra (a tetrahedral mesh
             # has about 5-6 tetrahedra per node; the node arena is sized for the sparser side)
-            per_tet = footprint_bytes(precision, 0, 1, reference_order)
+            per_tet = footprint_bytes(precision, 0, 1, reference_order, scenes=0)
             # the engine's incidence space (4 per tetrahedron) is addressed with 32-bit indices
             max_tets = min(self.budget // per_tet, 500_000_000)
             max_nodes = max_tets // 3

@@ -66,6 +66,10 @@ def test_footprint_monotonic():
     # chunks and 1 + 7 (64 x 4 + 1) candidate sums (doubles)
     d = serving.footprint_bytes(capi.FP64_TILED, 1000, 5000, True) - serving.footprint_bytes(capi.FP64_TILED, 1000, 5000)
     assert d == 5000 * (4 * 2 * 3 * 8 + 8) + 1000 * 8 + 8 * (2 * 64 + 1 + 7 * (64 * 4 + 1))
+    # the arenas are sized per tetrahedron, without the per-scene part
+    per_tet = serving.footprint_bytes(capi.FP64_TILED, 0, 1, True, scenes=0)
+    assert per_tet == serving.footprint_bytes(capi.FP64_TILED, 0, 2, True) - serving.footprint_bytes(capi.FP64_TILED, 0, 1, True)
+    assert per_tet < 1024
 
 
 class _FakeEngine:
