@@ -161,7 +161,9 @@ int sofa_amd_scene_set_preconditioner(sofa_amd_engine eng, int scene, int kind);
 /* CGLinearSolver Data warmStart (CGLinearSolver.inl:39,94-105): each solve starts from the
  * solver's solution vector, which keeps the previous step's solution (0 before the first step):
  * r = A x, then r.eq(b, r, -1.0); |b| and the loop as without it.  The first step is the cold
- * solve bit for bit.  FP64 engine; a PCGLinearSolver scene warm-starts anyway. */
+ * solve bit for bit.  FP64 engine, or FP64_TILED in the reference's summation order (set with
+ * sofa_amd_engine_set_dot_order before this call): there the operator of an iteration 0 gives
+ * A x.  A PCGLinearSolver scene warm-starts anyway. */
 int sofa_amd_scene_set_warm_start(sofa_amd_engine eng, int scene, int on);
 
 /* Context gravity (Node::setGravity), used by the masses' addForce. */

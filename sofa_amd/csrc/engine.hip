@@ -1299,6 +1299,7 @@ struct EngineT : EngineBase {
                 ls->maxpart = std::max({ls->maxpart, bparts, (int)pl.nblks.size()});
                 ls->plane = ls->plane || scenes[id].plane_on;
                 ls->pcg = ls->pcg || scenes[id].precond != 0;
+                ls->warm = ls->warm || (scenes[id].warm_start && scenes[id].precond == 0);
             }
             put(ls->a, ta); put(ls->b, fa); put(ls->c, na);
             ls->na = (int)ta.size(); ls->nb = (int)fa.size(); ls->nc = (int)na.size();
@@ -1398,6 +1399,18 @@ struct EngineT : EngineBase {
             k_fin_rhs<VT, true><<<cur->nb, kBlock, 0, s>>>(fb, inst, ranges_d, sn, mass, fixed, v, sp0, sp1, f, b, r, xcg,
                                                              pbb, x, contact, nd);
         launch_seqdot(n, s, kDotBB);
+        if (cur->warm) {
+            // CGLinearSolver warmStart: q = A x on the previous solution (the operator of
+            // iteration 0), r = b - q and r.r for the tolerance test of iteration 1 (scenes of
+            // the set without it pass through untouched: their x is 0 and their r, p, q and
+            // terms are rewritten before they are read)
+            const NodeBlk* nb = (const NodeBlk*)cur->c;
+            if (cur->na > 0) launch_apply(cur->na, s, tile_args(), 0, xcg, nullptr, q, sp0, nullptr, pden, 0);
+            if (cur->nb > 0)
+                k_fin_exact<VT><<<cur->nb, kBlock, 0, s>>>(fb, inst, sn, mass, fixed, 0, xcg, p, q, sp0, nd, contact);
+            if (cur->nc > 0) k_warm_begin_exact<VT><<<cur->nc, kBlock, 0, s>>>(nb, inst, b, q, r, nd);
+            launch_seqdot(n, s, kDotRR);
+        }
     }
     // CG iteration `it`'s operator (q = A p, the p.q terms): the throughput-bound phase
     void exact_operator(int it, hipStream_t s) {
@@ -2543,8 +2556,10 @@ int sofa_amd_scene_set_warm_start(sofa_amd_engine eng, int scene, int on) {
     SceneHost* s = scene_of(eng, scene);
     if (!s) return fail(SOFA_AMD_EINVAL, "invalid scene");
     if (!editable(eng, scene)) return fail(SOFA_AMD_ESTATE, "scene already committed");
-    if (on && eng->impl->precision != SOFA_AMD_FP64)
-        return fail(SOFA_AMD_EINVAL, "CGLinearSolver warmStart runs on the FP64 engine");
+    if (on && eng->impl->precision != SOFA_AMD_FP64 &&
+        !(eng->impl->precision == SOFA_AMD_FP64_TILED && eng->impl->seq_dots))
+        return fail(SOFA_AMD_EINVAL, "CGLinearSolver warmStart runs on the FP64 engine, or on FP64_TILED in the "
+                                     "reference order (sofa_amd_engine_set_dot_order first)");
     s->warm_start = on != 0;
     return SOFA_AMD_OK;
 }

@@ -472,7 +472,7 @@ __global__ __launch_bounds__(TB) void k_tile_rhs(TileArgs<R, NR> a, const VT* __
                 st3(b, g, bn);
                 st3(r, g, bn);
                 const VT z[3] = {0, 0, 0};
-                st3(xcg, g, z);
+                if (!in.warm_start) st3(xcg, g, z);   // warmStart: x stays the previous solution
                 if (EXACT) a.nd[g] = dot3d(bn, bn);
                 else bb += dot3d(bn, bn);
             } else if (EXACT) {
@@ -532,7 +532,7 @@ __global__ __launch_bounds__(kBlock) void k_fin_rhs(const FinBlk* __restrict__ f
         st3(b, g, bn);
         st3(r, g, bn);
         const VT z[3] = {0, 0, 0};
-        st3(xcg, g, z);
+        if (!in.warm_start) st3(xcg, g, z);   // warmStart: x stays the previous solution
         bb = dot3d(bn, bn);
         if (EXACT) nd[g] = bb;
     }
@@ -691,10 +691,12 @@ __global__ __launch_bounds__(TB) void k_tile_apply(TileArgs<R, NR> a, int iter, 
                           : rg.nf > 0 ? scene_sum<TB, WIDE>(pin, rg.f0, rg.nf, sh) : scene_sum<TB, WIDE>(pin, rg.t0, rg.nt, sh);
         const double normb = sqrt(bb);
         if (normb == 0.0) { record_stop(ins, first, 4, 0, 1.0); return; }
-        const double err = sqrt(bb) / normb;
+        // rho = r.r: b.b from a cold start; warmStart (reference order): the r.r of r = b - A x
+        const double rho = EXACT && ins->warm_start ? ins->dots[kDotRR] : bb;
+        const double err = sqrt(rho) / normb;
         if (err <= ins->tol && !ins->first_solve) { record_stop(ins, first, 1, 1, err); return; }
         if (first && threadIdx.x == 0) {
-            ins->normb = normb; ins->rho_slot[1] = bb; ins->iter = 1; ins->err = err;
+            ins->normb = normb; ins->rho_slot[1] = rho; ins->iter = 1; ins->err = err;
         }
     } else if (iter > 1) {
         if (iter > ins->max_iter) { record_stop(ins, first, 0, ins->max_iter, ins->err); return; }
@@ -880,6 +882,33 @@ __global__ __launch_bounds__(kBlock) void k_fin_exact(const FinBlk* __restrict__
     if (fixed[g]) { qn[0] = 0; qn[1] = 0; qn[2] = 0; }
     st3(q, g, qn);
     nd[g] = dot3d(pp, qn);
+}
+
+// CGLinearSolver warmStart in the reference order (CGLinearSolver.inl:95-99): after the operator
+// of iteration 0 on the previous solution (q = A x, shared nodes finished by k_fin_exact),
+// r.eq(b, q, -1.0): r = b + q (-1) per component, and each node's r.r term for k_seqdot
+template <typename VT>
+__global__ __launch_bounds__(kBlock) void k_warm_begin_exact(const NodeBlk* __restrict__ nb_,
+                                                              const InstDev* __restrict__ inst,
+                                                              const VT* __restrict__ b, const VT* __restrict__ q,
+                                                              VT* __restrict__ r, double* __restrict__ nd) {
+    const NodeBlk nb = nb_[blockIdx.x];
+    const InstDev& in = inst[nb.scene];
+    if (!in.warm_start || in.stopped) return;
+    const VT m1 = (VT)(-1.0);
+#pragma unroll
+    for (int k = 0; k < kNodesPerThread; k++) {
+        const int n = nb.begin + k * kBlock + threadIdx.x;
+        if (n < nb.end) {
+            VT bn[3], qn[3], rn[3];
+            ld3(b, n, bn);
+            ld3(q, n, qn);
+#pragma unroll
+            for (int i = 0; i < 3; i++) rn[i] = bn[i] + qn[i] * m1;
+            st3(r, n, rn);
+            nd[n] = dot3d(rn, rn);
+        }
+    }
 }
 
 template <typename VT>

@@ -363,6 +363,43 @@ def test_mixed_jacobi_inner_cg(which):
         assert totals[1] < totals[0]
 
 
+def test_cg_warm_start_tiled_reference_order():
+    """CGLinearSolver warmStart on FP64_TILED in the reference's summation order, batched with
+    cold-start scenes in one launch set: the cfg0 beam with spread stiffness (CG to 1e-4, the
+    solves end on the tolerance) and a 9,104-node beam of the 256-beam mix (segmented dot
+    products) warm-start, a second copy of the mix beam does not; every scene's positions,
+    velocities and last iteration count equal the oracle's reference-order run bit for bit."""
+    from sofa_amd import workloads
+    capi = _capi()
+    print("case: cg warm start, FP64_TILED reference order, 3 scenes", flush=True)
+    beam, _ = _pcg_spec("beam", 0)
+    beam["preconditioner"] = 0
+    beam["tol"] = 1e-4
+    big = dict(workloads.cfg3()[10])
+    big["tol"] = 1e-6
+    specs = [dict(beam, warm_start=True), dict(big, warm_start=True), dict(big)]
+    nsteps = 12
+    eng = capi.Engine(0, capi.FP64_TILED)
+    eng.set_dot_order(True)
+    ids = [workloads.add_to_engine(eng, sp) for sp in specs]
+    eng.commit()
+    eng.set_graphs(True)
+    eng.step(ids, nsteps)
+    got = [eng.read(i)[:2] + (eng.stats(i),) for i in ids]
+    eng.close()
+    for sp, (x, v, st) in zip(specs, got):
+        o = _oracle(sp, dot_mode=0)
+        o.step(sp["dt"], nsteps)
+        so = o.stats()
+        print(f"  warm {sp.get('warm_start', False)}: last solve {st['iterations']} iterations "
+              f"(oracle {so['iterations']}), |r|/|b| {st['err']:.3e}", flush=True)
+        assert st["iterations"] == so["iterations"]
+        assert np.array_equal(x, o.get("x")), float(np.max(np.abs(x - o.get("x"))))
+        assert np.array_equal(v, o.get("v"))
+    # the warm start changes the big beam's trajectory (its solves start elsewhere)
+    assert not np.array_equal(got[1][0], got[2][0])
+
+
 @pytest.mark.parametrize("order", [0, 1])
 @pytest.mark.parametrize("which", ["beam", "ellipsoid"])
 def test_cg_warm_start_bitexact(which, order):
