@@ -629,6 +629,17 @@ def main():
         iss = measured_issue(f"profiles/k_tile_apply_beam_mix_{head}_issue.md", hrow.get("variant"))
         if iss is not None:
             roof["issue"] = iss
+        fl = _profile_json(f"profiles/k_tile_apply_beam_mix_{head}_flops.md", None)
+        if fl is not None and fl.get("variant") == hrow.get("variant"):
+            # the same launch against its compute ceilings: FLOP rates (counted FP64 / FP32
+            # add, mul, fma) and the VALU issue floor of its instruction count, at this run's time
+            us = roof["us_per_launch"]
+            roof["compute"] = {"fp64_tflops": fl["fp64_flop_per_launch"] / us / 1e6,
+                               "fp32_tflops": fl["fp32_flop_per_launch"] / us / 1e6,
+                               "fp64_peak_tflops": 78.6, "fp32_peak_tflops": 157.3,
+                               "valu_issue_floor_us": fl["valu_issue_floor_us"],
+                               "valu_issue_frac": fl["valu_issue_floor_us"] / us,
+                               "source": f"profiles/k_tile_apply_beam_mix_{head}_flops.md"}
     log(f"headline path {head}: {value:.4g} element-steps/s")
     tiling = None
     plain = None
