@@ -587,20 +587,40 @@ def main():
                     help="wall seconds per mode of the stream served in the reference summation order (0: off)")
     ap.add_argument("--serve-exact-budget-gb", type=float, default=64.0)
     ap.add_argument("--serve-arrivals", type=float, default=60.0)
+    ap.add_argument("--trace-steps", type=int, default=0,
+                    help="only step the 256-beam mix in the reference order N times after one step, one branch, "
+                         "eager launches (for a kernel trace of the step's time budget), then exit")
     ap.add_argument("--serve-budget-gb", type=float, default=None,
                     help="admission budget of the served stream (default: 80 %% of the free HBM)")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus > 1 or world > 1:
-        print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported "
-              "(independent jobs shard across GPUs with no communication; see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported (the jobs are independent: "
+              "they shard across GPUs by job, with no communication)", file=sys.stderr)
         sys.exit(2)
 
     from sofa_amd import capi, workloads
 
     args.paths = set(args.paths.split(","))
     from sofa_amd import capi, workloads
+
+    if args.trace_steps > 0:
+        # the reference-order step's kernels one after another: their durations add up to the step
+        specs = workloads.cfg3()
+        eng, ids = make_engine(specs, capi.FP64_TILED, graphs=False, reference_order=True)
+        v, _ = eng.autotune(5)
+        eng.set_branches(1)
+        st = bench_stream()
+        eng.step(ids, 1, stream=st)
+        st.synchronize()
+        time.sleep(0.05)
+        t0 = time.perf_counter()
+        eng.step(ids, args.trace_steps, stream=st)
+        st.synchronize()
+        log(f"trace steps: operator variant {v}, {args.trace_steps} steps in {time.perf_counter() - t0:.3f} s")
+        eng.close()
+        return
 
     # ---- headline: the 256-beam job mix, every precision path batched, each with its error
     specs = workloads.cfg3()
