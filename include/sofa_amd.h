@@ -278,7 +278,8 @@ int sofa_amd_engine_set_branch_turns(sofa_amd_engine eng, int enable);
 
 /* FP64_TILED engines in the reference's summation order (set before the scenes are placed): how
  * each scene's node-after-node dot product (|b|^2, p.q, r.r) is computed.  segments = 1: one
- * single-wave chain of dependent adds per scene.  segments = 2, 4 or 8 (default 8): the chain's
+ * single-wave chain of dependent adds per scene.  segments = 2, 4, 8 (default) or 16 (with a
+ * window of 32 at most): the chain's
  * segments run at once, each segment after the first from 2 `window` candidate start values
  * (consecutive doubles around a prediction of the sum before it: the terms each rounded as the
  * chain would round them; window 0, 32, 64, 96 or 128, default 32), and the sum is then resolved

@@ -3217,10 +3217,12 @@ int sofa_amd_engine_set_branches(sofa_amd_engine eng, int branches) {
 int sofa_amd_engine_set_dot_segments(sofa_amd_engine eng, int segments, int window) {
     GUARD_BEGIN
     if (!eng || !eng->impl) return fail(SOFA_AMD_EINVAL, "null engine");
-    if (segments != 1 && segments != 2 && segments != 4 && segments != 8)
-        return fail(SOFA_AMD_EINVAL, "segments must be 1, 2, 4 or 8");
+    if (segments != 1 && segments != 2 && segments != 4 && segments != 8 && segments != 16)
+        return fail(SOFA_AMD_EINVAL, "segments must be 1, 2, 4, 8 or 16");
     if (window < 0 || window > 32 * kSpecMaxWaves || window % 32 != 0)
         return fail(SOFA_AMD_EINVAL, "window must be 0, 32, 64, 96 or 128");
+    if (1 + (segments - 1) * (2 * window + 1) > kSpecStride)
+        return fail(SOFA_AMD_EINVAL, "16 segments take a window of 32 at most");
     HIPCHK(hipSetDevice(eng->impl->device));
     HIPCHK(hipDeviceSynchronize());
     eng->impl->dot_segments = segments;

@@ -196,7 +196,8 @@ static __global__ __launch_bounds__(64) void k_seqdot(const int* __restrict__ sc
 // ---------------------------------------------------------------------------------------
 constexpr int kSpecChunks = 64;   // prediction chunks per scene
 // per scene record: [0] segment 0's sum; then per boundary k = 1 .. K - 1: the prediction, then
-// the 64 W candidates' sums (K <= 8, W <= 4)
+// the 64 W candidates' sums (1 + (K - 1)(64 W + 1) doubles: K <= 8 with W <= 4, or K = 16 with
+// W = 1)
 constexpr int kSpecMaxSeg = 8, kSpecMaxWaves = 4;
 constexpr int kSpecStride = 1 + (kSpecMaxSeg - 1) * (64 * kSpecMaxWaves + 1);
 struct SpecDots {
@@ -252,9 +253,13 @@ static __global__ __launch_bounds__(64) void k_seqdot_part(const int* __restrict
 // plain prefix (the chunk sums before the chunk, then a scan of the chunk), which decides only
 // the grid u; the chunk's modelled sum goes to sd.model.  One 64-thread workgroup per (scene,
 // chunk), after k_seqdot_part.
-__device__ __forceinline__ double spec_ulp(double a) {
-    const long long eb = __double_as_longlong(fabs(a)) & 0x7ff0000000000000LL;
-    return eb == 0 ? 4.9406564584124654e-324 : __longlong_as_double(eb) * 2.220446049250313e-16;
+// t rounded to the grid of a running sum s: rint(t / u) u with u = ulp(s), scaled by powers of
+// two (exact); every double is on the grid of a subnormal or zero s
+__device__ __forceinline__ double spec_round(double t, double s) {
+    const int eb = (int)((__double_as_longlong(s) >> 52) & 0x7ff);
+    if (eb == 0) return t;
+    const int ue = eb - 1075;   // u = 2^ue
+    return ldexp(rint(ldexp(t, -ue)), ue);
 }
 static __global__ __launch_bounds__(64) void k_seqdot_model(const int* __restrict__ scenes, const InstDev* __restrict__ inst,
                                                              const double* __restrict__ nd, SpecDots sd) {
@@ -268,18 +273,26 @@ static __global__ __launch_bounds__(64) void k_seqdot_model(const int* __restric
     const double* t = nd + ins->node_begin;
     double run = wave_sum_f64(lane < c ? sd.part[(size_t)sc * kSpecChunks + lane] : 0.0);
     double acc = 0.0;
-    for (int g = i0; g < i1; g += 64) {
-        const int i = g + lane;
-        const double ti = i < i1 ? t[i] : 0.0;
-        double x = ti;   // inclusive scan of the group's terms
+    constexpr int G = 8;   // groups of 64 terms loaded ahead (all in flight), then scanned
+    for (int g0 = i0; g0 < i1; g0 += G * 64) {
+        double tg[G];
 #pragma unroll
-        for (int d = 1; d < 64; d <<= 1) {
-            const double y = __shfl_up(x, d, 64);
-            if (lane >= d) x += y;
+        for (int k = 0; k < G; k++) {
+            const int i = g0 + k * 64 + lane;
+            tg[k] = i < i1 ? t[i] : 0.0;
         }
-        const double u = spec_ulp(run + x);
-        acc += wave_sum_f64(rint(ti / u) * u);
-        run += __shfl(x, 63, 64);
+#pragma unroll
+        for (int k = 0; k < G; k++) {
+            const double ti = tg[k];
+            double x = ti;   // inclusive scan of the group's terms
+#pragma unroll
+            for (int d = 1; d < 64; d <<= 1) {
+                const double y = __shfl_up(x, d, 64);
+                if (lane >= d) x += y;
+            }
+            acc += wave_sum_f64(spec_round(ti, run + x));
+            run += __shfl(x, 63, 64);
+        }
     }
     if (lane == 0) sd.model[(size_t)sc * kSpecChunks + c] = acc;
 }

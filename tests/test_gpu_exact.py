@@ -132,7 +132,7 @@ SEG_BEAMS = [67, 151, 10, 53]
 
 def test_exact_tiled_dot_segments():
     """Reference-order dot products with the chain's segments run at once from candidate starts
-    give the single chain's bits: 2, 4 and 8 segments, windows of 32 to 128 candidates per side,
+    give the single chain's bits: 2, 4, 8 and 16 segments, windows of 32 to 128 candidates per side,
     and window 0, where every boundary falls back to summing its segment from the known start.
     The largest beam's trajectory also equals the oracle's reference-order run bit for bit."""
     from sofa_amd import workloads
@@ -141,7 +141,7 @@ def test_exact_tiled_dot_segments():
     nsteps = 20
     print(f"case: exact tiled dot segments, beams {SEG_BEAMS}, {nsteps} steps", flush=True)
     one = _run("FP64_TILED", specs, nsteps, reference_order=True, segments=(1, 128))
-    for seg in [(4, 128), (8, 0), (2, 32), (8, 96), (8, 32)]:
+    for seg in [(4, 128), (8, 0), (2, 32), (8, 96), (8, 32), (16, 32), (16, 0)]:
         got = _run("FP64_TILED", specs, nsteps, reference_order=True, segments=seg)
         for k, ((xa, va, sa), (xb, vb, sb)) in enumerate(zip(one, got)):
             assert np.array_equal(xa, xb) and np.array_equal(va, vb), (seg, k, rel_err(xa, xb))
