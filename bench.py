@@ -381,7 +381,8 @@ def cfg2_pcg(c, steps, max_iter=20000):
            "steps": steps}
     sweeps = max_iter // 30
     for name, kind, tol, prec in (("cg", 0, 1e-8, capi.FP64), ("pcg_jacobi", 1, 1e-16, capi.FP64),
-                                  ("pcg_block_jacobi", 2, 1e-16, capi.FP64), ("mixed_jacobi", 1, 1e-8, capi.MIXED)):
+                                  ("pcg_block_jacobi", 2, 1e-16, capi.FP64), ("mixed_jacobi", 1, 1e-8, capi.MIXED),
+                                  ("mixed_block_jacobi", 2, 1e-8, capi.MIXED)):
         spec = dict(c, iters=max_iter, tol=tol)
         if kind:
             spec["preconditioner"] = kind

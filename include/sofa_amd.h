@@ -152,10 +152,10 @@ int sofa_amd_scene_add_linear_movement(sofa_amd_engine eng, int scene, int n, co
  * identity on fixed and LinearMovement nodes (Dirichlet rows), inverted.  The solve starts from
  * the previous step's solution, as the reference's does (the solver's solution vector persists);
  * the statistics report PCG's iteration count (maxIter + 1 when the budget runs out) and
- * err = r.z / b.b.  FP64 engine.  On a MIXED engine, kind 1 only: the inner FP32 CG runs
- * preconditioned by the same Jacobi diagonal (in float), z = dinv r, its residual measured as
- * r.z; the FP64 refinement and its |r| / |b| test are unchanged (no reference counterpart: MIXED
- * is compared with converged FP64). */
+ * err = r.z / b.b.  FP64 engine.  On a MIXED engine (kind 1 or 2): the inner FP32 CG runs
+ * preconditioned by the same inverted diagonal or 3x3 blocks (in float), z = P^-1 r, its
+ * residual measured as r.z; the FP64 refinement and its |r| / |b| test are unchanged (no
+ * reference counterpart: MIXED is compared with converged FP64). */
 int sofa_amd_scene_set_preconditioner(sofa_amd_engine eng, int scene, int kind);
 
 /* CGLinearSolver Data warmStart (CGLinearSolver.inl:39,94-105): each solve starts from the

@@ -369,6 +369,8 @@ struct InnerCg {
     bool plane;                     // some scene has a PlaneForceField
     SharedNodes sn;
     float *r, *p, *q, *d, *sp;
+    float* z;                       // with a preconditioner: z = P^-1 r, staged by the operator
+    const float *dinv, *binv;       // ... the inverted diagonal (3 per node) / 3x3 blocks (9)
     const float *x, *x0;            // float positions and rest positions (geometry recomputed)
     double *pden, *prho;            // prho: |r|^2 partials per node block (iteration 1 included)
 };
@@ -419,6 +421,8 @@ struct EngineT : EngineBase {
     uint8_t* sh_acnt = nullptr;                    // reference order, CG operator: slots after the first
     uint8_t* ln_lead = nullptr;                    // reference order, CG operator: folded incidences per local node
     float* dinv32 = nullptr;                       // MIXED: the inner CG's Jacobi preconditioner (3 per node)
+    float* binv32 = nullptr;                       // ... its block-Jacobi preconditioner (9 per node)
+    float* z32 = nullptr;                          // ... z = P^-1 r of its preconditioned CG
     uint16_t* ln_inc = nullptr;
     VT *sp0 = nullptr, *sp1 = nullptr;
     double *pbb = nullptr, *pden = nullptr, *prho = nullptr;
@@ -486,7 +490,7 @@ struct EngineT : EngineBase {
                         ln_inc, sp0, sp1, pbb, pden, prho, p2, em_ptr, em_nbr, em_val, emass, osc_node,
                         osc_mean, osc_amp, osc_w, osc_phase, kmat32, emass32, r32, p32, q32,
                         d32, sp32, pden32, prho32, prr, inst_in, sh_of, nd, x0, x32, x0_32, contact, inc_rank, pinv, zv,
-                        inc_arank, sh_acnt, ln_lead, dinv32, spec_part, spec_model, spec_seg};
+                        inc_arank, sh_acnt, ln_lead, dinv32, binv32, z32, spec_part, spec_model, spec_seg};
         for (void* ptr : ptrs)
             if (ptr) (void)hipFree(ptr);
     }
@@ -958,6 +962,7 @@ struct EngineT : EngineBase {
         pbb = pden = prho = nullptr; kmat32 = emass32 = nullptr; x32 = x0_32 = nullptr;
         r32 = p32 = q32 = d32 = sp32 = nullptr; pden32 = prho32 = prr = nullptr; inst_in = nullptr; sh_of = nullptr; inc_rank = nullptr;
         pinv = nullptr; zv = nullptr; inc_arank = nullptr; sh_acnt = nullptr; ln_lead = nullptr; dinv32 = nullptr;
+        binv32 = z32 = nullptr;
         spec_part = spec_model = spec_seg = nullptr;
         cap = c;
         if (cap.node >= (int64_t)1 << 30 || cap.inc >= (int64_t)1 << 31 || cap.lnode >= (int64_t)1 << 31 ||
@@ -977,7 +982,11 @@ struct EngineT : EngineBase {
         inc_arank = dalloc<uint8_t>(std::max<int64_t>(1, cap.inc));
         sh_acnt = dalloc<uint8_t>(std::max<int64_t>(1, cap.shn));
         ln_lead = dalloc<uint8_t>(std::max<int64_t>(1, cap.lnode));
-        if (mixed) dinv32 = dalloc<float>(3 * N);
+        if (mixed) {
+            dinv32 = dalloc<float>(3 * N);
+            binv32 = dalloc<float>(9 * N);
+            z32 = dalloc<float>(3 * N);
+        }
         contact = dalloc<int8_t>(N);
         HIPCHK(hipMemset(contact, 0, std::max<int64_t>(1, N)));
         tet = dalloc<int4>(E); rot = dalloc<R>(9 * E); rest = dalloc<R>(NR * E); kmat = dalloc<R>(3 * E);
@@ -1599,7 +1608,10 @@ struct EngineT : EngineBase {
         ic.ta.x_lo = nullptr;
         ic.ta.nd = nullptr;
         // PCGLinearSolver with a Jacobi preconditioner: the inner CG runs preconditioned
-        ic.ta.dinv = cur->pcg ? dinv32 : nullptr;
+        ic.ta.dinv = nullptr;   // the preconditioner acts in the node kernels (z), not in the operator
+        ic.z = cur->pcg ? z32 : nullptr;
+        ic.dinv = dinv32;
+        ic.binv = binv32;
         ic.x = x32; ic.x0 = x0_32;
         ic.fb = fb; ic.nb = nb;
         ic.na = cur->na; ic.nfb = cur->nb; ic.nnb = cur->nc; ic.maxit = cur->maxit;
@@ -1613,8 +1625,8 @@ struct EngineT : EngineBase {
         if (cur->nb > 0) k_fin_rhs<VT><<<cur->nb, kBlock, 0, s>>>(fb, inst, ranges_d, sn, mass, fixed, v, sp0, sp1, f, b, r, xcg, pbb, x, contact);
         if (cur->pcg) {   // the inner CG's Jacobi preconditioner from this step's rotations
             if (cur->nc > 0)
-                k_jacobi32<R, NR><<<cur->nc, kBlock, 0, s>>>(view(), nb, inst, inc_ptr, inc, mass, fixed, dinv32);
-            if (cur->nl > 0) k_jacobi32_lm<<<cur->nl, kBlock, 0, s>>>((const Blk*)cur->l, inst, dinv32);
+                k_jacobi32<R, NR><<<cur->nc, kBlock, 0, s>>>(view(), nb, inst, inc_ptr, inc, mass, fixed, dinv32, binv32);
+            if (cur->nl > 0) k_jacobi32_lm<<<cur->nl, kBlock, 0, s>>>((const Blk*)cur->l, inst, dinv32, binv32);
         }
         for (int sw = 1; sw <= cur->maxsw + 1; sw++) {
             if (sw > 1) {   // q = A x, FP64
@@ -1623,7 +1635,8 @@ struct EngineT : EngineBase {
                 if (cur->nb > 0)
                     k_fin_apply<VT><<<cur->nb, kBlock, 0, s>>>(fb, inst, ranges_d, sn, mass, fixed, 0, xcg, nullptr, q, sp0, pden, contact);
             }
-            if (cur->nc > 0) k_ir_residual<<<cur->nc, kBlock, 0, s>>>(nb, inst, sw, b, q, r32, d32, prr, prho32, ic.ta.dinv);
+            if (cur->nc > 0)
+                k_ir_residual<<<cur->nc, kBlock, 0, s>>>(nb, inst, sw, b, q, r32, d32, prr, prho32, dinv32, binv32, ic.z);
             k_ir_control<<<n, kBlock, 0, s>>>(cur->scenes, inst, inst_in, ranges_d, sw, prr);
             if (sw <= cur->maxsw) {
                 launch_inner_cg(ic, s);
@@ -2264,26 +2277,27 @@ EngineBase* make_engine_split() {
 }
 
 void launch_inner_cg(const InnerCg& c, hipStream_t s) {
+    const float* rs = c.z ? c.z : c.r;   // the vector p is formed from (p = rs + beta p)
     for (int it = 1; it <= c.maxit; it++) {
         if (c.na > 0)
         {
             if (c.wide && c.plane)
                 k_tile_apply<float, 6, float, kTileThreads, 2, true, true><<<c.na, kTileThreads, 0, s>>>(
-                    c.ta, it, c.r, c.p, c.q, c.sp, c.prho, c.pden, 1, c.x, c.x0);
+                    c.ta, it, rs, c.p, c.q, c.sp, c.prho, c.pden, 1, c.x, c.x0);
             else if (c.wide)
                 k_tile_apply<float, 6, float, kTileThreads, 2, true, false><<<c.na, kTileThreads, 0, s>>>(
-                    c.ta, it, c.r, c.p, c.q, c.sp, c.prho, c.pden, 1, c.x, c.x0);
+                    c.ta, it, rs, c.p, c.q, c.sp, c.prho, c.pden, 1, c.x, c.x0);
             else if (c.plane)
                 k_tile_apply<float, 6, float, kTileThreads, 2, false, true><<<c.na, kTileThreads, 0, s>>>(
-                    c.ta, it, c.r, c.p, c.q, c.sp, c.prho, c.pden, 1, c.x, c.x0);
+                    c.ta, it, rs, c.p, c.q, c.sp, c.prho, c.pden, 1, c.x, c.x0);
             else
                 k_tile_apply<float, 6, float, kTileThreads, 2, false, false><<<c.na, kTileThreads, 0, s>>>(
-                    c.ta, it, c.r, c.p, c.q, c.sp, c.prho, c.pden, 1, c.x, c.x0);
+                    c.ta, it, rs, c.p, c.q, c.sp, c.prho, c.pden, 1, c.x, c.x0);
         }
         if (c.nnb > 0)
             k_update<float><<<c.nnb, kBlock, 0, s>>>(c.nb, c.ta.inst, c.ta.ranges, it, c.sn, c.sp, c.ta.mass, c.ta.fixed,
                                                       c.p, c.q, c.d, c.r, c.pden, c.prho, c.ta.contact, nullptr,
-                                                      c.ta.dinv);
+                                                      c.dinv, c.binv, c.z);
     }
 }
 #else
@@ -2546,8 +2560,8 @@ int sofa_amd_scene_set_preconditioner(sofa_amd_engine eng, int scene, int kind) 
     SceneHost* s = scene_of(eng, scene);
     if (!s || kind < 0 || kind > 2) return fail(SOFA_AMD_EINVAL, "invalid argument");
     if (!editable(eng, scene)) return fail(SOFA_AMD_ESTATE, "scene already committed");
-    if (kind != 0 && eng->impl->precision != SOFA_AMD_FP64 && !(kind == 1 && eng->impl->precision == SOFA_AMD_MIXED))
-        return fail(SOFA_AMD_EINVAL, "PCGLinearSolver runs on the FP64 engine (Jacobi also in the MIXED inner CG)");
+    if (kind != 0 && eng->impl->precision != SOFA_AMD_FP64 && eng->impl->precision != SOFA_AMD_MIXED)
+        return fail(SOFA_AMD_EINVAL, "PCGLinearSolver runs on the FP64 engine (and in the MIXED inner CG)");
     s->precond = kind;
     return SOFA_AMD_OK;
 }

@@ -18,15 +18,17 @@
 namespace sofa_amd {
 
 // r = b (sweep 1) or b - A x (q = A x from the FP64 operator); r32 = (float) r; d = 0.
-// Partials of |r|^2 (FP64 values) and of |r32|^2 -- with a Jacobi preconditioner (dinv) r32.z,
-// z = dinv r32 -- per node block.
+// Partials of |r|^2 (FP64 values) and of |r32|^2 per node block -- with a preconditioner (zbuf
+// given) r32.z instead, z = P^-1 r32 stored in zbuf (precond_z).
 static __global__ __launch_bounds__(kBlock) void k_ir_residual(const NodeBlk* __restrict__ nb_,
                                                                 const InstDev* __restrict__ inst, int sweep,
                                                                 const double* __restrict__ b,
                                                                 const double* __restrict__ q,
                                                                 float* __restrict__ r32, float* __restrict__ d32,
                                                                 double* __restrict__ prr, double* __restrict__ prr32,
-                                                                const float* __restrict__ dinv) {
+                                                                const float* __restrict__ dinv,
+                                                                const float* __restrict__ binv,
+                                                                float* __restrict__ zbuf) {
     __shared__ double sh[kBlock / 64];
     const NodeBlk nb = nb_[blockIdx.x];
     if (inst[nb.scene].stopped) return;
@@ -48,10 +50,10 @@ static __global__ __launch_bounds__(kBlock) void k_ir_residual(const NodeBlk* __
             st3(r32, n, rf);
             st3(d32, n, z);
             rr += dot3d(rn, rn);
-            if (dinv) {
-                float dv[3];
-                ld3(dinv, n, dv);
-                const float zf[3] = {rf[0] * dv[0], rf[1] * dv[1], rf[2] * dv[2]};
+            if (zbuf) {
+                float zf[3];
+                precond_z(inst[nb.scene].pcg, dinv, binv, n, rf, zf);
+                st3(zbuf, n, zf);
                 rr32 += dot3d(rf, zf);
             } else {
                 rr32 += dot3d(rf, rf);
@@ -141,30 +143,40 @@ template <typename R, int NR>
 __global__ __launch_bounds__(kBlock) void k_jacobi32(ElemView<R, NR> ev, const NodeBlk* __restrict__ nblk,
                                                       const InstDev* __restrict__ inst, const int* __restrict__ inc_ptr,
                                                       const int* __restrict__ inc, const double* __restrict__ mass,
-                                                      const int8_t* __restrict__ fixed, float* __restrict__ dinv) {
+                                                      const int8_t* __restrict__ fixed, float* __restrict__ dinv,
+                                                      float* __restrict__ binv) {
     const NodeBlk bk = nblk[blockIdx.x];
     const InstDev& in = inst[bk.scene];
+    if (!in.pcg) return;   // z = r: neither array is read
     for (int n = bk.begin + threadIdx.x; n < bk.end; n += kBlock) {
-        float d[3] = {1.f, 1.f, 1.f};
-        if (in.pcg) {
-            double B[9];
-            pcg_block<R, NR>(in, ev, inc_ptr, inc, mass, fixed, n, B);
-            d[0] = (float)B[0]; d[1] = (float)B[4]; d[2] = (float)B[8];
+        double B[9];
+        pcg_block<R, NR>(in, ev, inc_ptr, inc, mass, fixed, n, B);
+        if (in.pcg == 1) {
+            const float d[3] = {(float)B[0], (float)B[4], (float)B[8]};
+            st3(dinv, n, d);
+        } else {
+#pragma unroll
+            for (int i = 0; i < 9; i++) binv[9 * (int64_t)n + i] = (float)B[i];
         }
-        st3(dinv, n, d);
     }
 }
 
 // LinearMovementProjectiveConstraint's nodes are Dirichlet rows: identity
 static __global__ __launch_bounds__(kBlock) void k_jacobi32_lm(const Blk* __restrict__ lblk,
                                                                 const InstDev* __restrict__ inst,
-                                                                float* __restrict__ dinv) {
+                                                                float* __restrict__ dinv, float* __restrict__ binv) {
     const Blk bk = lblk[blockIdx.x];
     const int i = bk.begin + threadIdx.x;
     const InstDev& in = inst[bk.inst];
     if (i >= bk.end || !in.pcg) return;
-    const float one[3] = {1.f, 1.f, 1.f};
-    st3(dinv, in.lm_node[i], one);
+    const int64_t n = in.lm_node[i];
+    if (in.pcg == 1) {
+        const float one[3] = {1.f, 1.f, 1.f};
+        st3(dinv, (int)n, one);
+    } else {
+#pragma unroll
+        for (int k = 0; k < 9; k++) binv[9 * n + k] = (k % 4 == 0) ? 1.f : 0.f;
+    }
 }
 
 // scene bookkeeping at the end of a MIXED step (one thread per scene of the launch set)

@@ -951,15 +951,40 @@ __global__ __launch_bounds__(kBlock) void k_fin_apply(const FinBlk* __restrict__
     if (threadIdx.x == 0) pden[fb.pslot] = fold_tiles(pden, ranges[fb.scene], fb.bi) + own;
 }
 
+// z = P^-1 r of a MIXED inner CG's preconditioner (the scene's kind: 0 none, 1 the inverted
+// diagonal dinv, 3 per node, 2 the inverted 3x3 block binv, 9 per node, row-major), in float
+template <typename VT>
+__device__ __forceinline__ void precond_z(int kind, const float* __restrict__ dinv, const float* __restrict__ binv,
+                                          int n, const VT r[3], VT z[3]) {
+    if (kind == 1) {
+        float dv[3];
+        ld3(dinv, n, dv);
+#pragma unroll
+        for (int i = 0; i < 3; i++) z[i] = r[i] * (VT)dv[i];
+    } else if (kind == 2) {
+        const float* B = binv + 9 * (int64_t)n;
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            VT t = r[0] * (VT)B[3 * i];
+            t += r[1] * (VT)B[3 * i + 1];
+            t += r[2] * (VT)B[3 * i + 2];
+            z[i] = t;
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < 3; i++) z[i] = r[i];
+    }
+}
+
 // CG iteration `iter` after the tile apply.  Nodes shared by several tiles are finished here:
 // p = r + beta p, q = (1 + h rm) M p + their tile partials in tile order, projection (the work
 // CGLinearSolver's p update and GraphScatteredMatrix::apply do for them).  Then den = p.q from
 // the tile partials (which already hold every node's term), the threshold test and alpha
 // (CGLinearSolver.inl:207-265), x += alpha p, r -= alpha q and the r.r partials.
 // EXACT: every node's p and q are final (k_fin_exact finished the shared ones), den is the
-// reference-order p.q of k_seqdot, and each node's r.r term goes to nd.  dinv (MIXED inner CG
-// with a Jacobi preconditioner): shared nodes' p = z + beta p with z = dinv r, and r.z partials
-// in place of r.r
+// reference-order p.q of k_seqdot, and each node's r.r term goes to nd.  zbuf (MIXED inner CG
+// with a preconditioner): it holds z = P^-1 r of the previous residual (shared nodes' p = z +
+// beta p); each node's new z (precond_z with dinv / binv) replaces it, r.z partials in place of r.r
 template <typename VT, bool EXACT = false>
 __global__ __launch_bounds__(kBlock) void k_update(const NodeBlk* __restrict__ nb_, InstDev* __restrict__ inst,
                                                     const Ranges* __restrict__ ranges, int iter, SharedNodes sn,
@@ -968,13 +993,14 @@ __global__ __launch_bounds__(kBlock) void k_update(const NodeBlk* __restrict__ n
                                                     const VT* __restrict__ q, VT* __restrict__ xcg, VT* __restrict__ r,
                                                     const double* __restrict__ pden, double* __restrict__ prho,
                                                     const int8_t* __restrict__ contact, double* __restrict__ nd = nullptr,
-                                                    const float* __restrict__ dinv = nullptr) {
+                                                    const float* __restrict__ dinv = nullptr,
+                                                    const float* __restrict__ binv = nullptr, VT* __restrict__ zbuf = nullptr) {
     __shared__ double sh[kBlock / 64];
     const NodeBlk nb = nb_[blockIdx.x];
     InstDev* ins = &inst[nb.scene];
     // node data first (registers), then the reduction
     VT pv[kNodesPerThread][3], qv[kNodesPerThread][3], xv[kNodesPerThread][3], rv[kNodesPerThread][3];
-    float dv[kNodesPerThread][3];
+    VT zv[kNodesPerThread][3];
     int shk[kNodesPerThread];
     // every load at a clamped node index (no branch: all in flight together; a node past the
     // block's end repeats its last node, unused), p and q also where the shared-node path below
@@ -985,7 +1011,7 @@ __global__ __launch_bounds__(kBlock) void k_update(const NodeBlk* __restrict__ n
         shk[k] = EXACT ? -1 : sn.of[n];
         ld3(xcg, n, xv[k]);
         ld3(r, n, rv[k]);
-        if (dinv) ld3(dinv, n, dv[k]);
+        if (zbuf) ld3(zbuf, n, zv[k]);
         ld3(p, n, pv[k]);
         ld3(q, n, qv[k]);
     }
@@ -1013,9 +1039,9 @@ __global__ __launch_bounds__(kBlock) void k_update(const NodeBlk* __restrict__ n
         if (n < nb.end) {
             if (shk[k] >= 0) {
                 VT pp[3] = {rv[k][0], rv[k][1], rv[k][2]};
-                if (dinv) {
+                if (zbuf) {
 #pragma unroll
-                    for (int i = 0; i < 3; i++) pp[i] = pp[i] * (VT)dv[k][i];
+                    for (int i = 0; i < 3; i++) pp[i] = zv[k][i];
                 }
                 if (iter > 1 || cont) {
 #pragma unroll
@@ -1041,8 +1067,10 @@ __global__ __launch_bounds__(kBlock) void k_update(const NodeBlk* __restrict__ n
             st3(r, n, rn);
             if (EXACT) {
                 nd[n] = dot3d(rn, rn);
-            } else if (dinv) {   // r.z
-                const VT zn[3] = {rn[0] * (VT)dv[k][0], rn[1] * (VT)dv[k][1], rn[2] * (VT)dv[k][2]};
+            } else if (zbuf) {   // z = P^-1 r for the next iteration, and r.z
+                VT zn[3];
+                precond_z(ins->pcg, dinv, binv, n, rn, zn);
+                st3(zbuf, n, zn);
                 rr += dot3d(rn, zn);
             } else {
                 rr += dot3d(rn, rn);

@@ -323,11 +323,12 @@ def test_pcg_bitexact(which, kind, order):
 
 @pytest.mark.parametrize("which", ["beam", "ellipsoid"])
 def test_mixed_jacobi_inner_cg(which):
-    """MIXED with PCGLinearSolver's Jacobi preconditioner in its inner FP32 CG (sweeps of 10
-    iterations continuing one search direction, until the FP64 |r|/|b| <= 1e-8) stays within 1e-6
-    relative of the oracle's FP64 Jacobi PCG run to convergence, and needs no more inner iterations
-    than the plain inner CG on the same scene (element stiffness spread over two decades; on the
-    ellipsoid, whose systems take the plain CG thousands of iterations, strictly fewer).  The
+    """MIXED with PCGLinearSolver's Jacobi (1) or block-Jacobi (2) preconditioner in its inner
+    FP32 CG (sweeps of 10 iterations continuing one search direction, until the FP64 |r|/|b| <=
+    1e-8) stays within 1e-6 relative of the oracle's FP64 Jacobi PCG run to convergence, and needs
+    no more inner iterations than the plain inner CG on the same scene (element stiffness spread
+    over two decades; on the ellipsoid, whose systems take the plain CG thousands of iterations,
+    strictly fewer).  The
     inner tolerance is 0: the inner CG measures its residual in the preconditioned norm, which
     can be below the outer test's bound while |r| is not, and would then end every sweep at once.
     MIXED has no reference counterpart: the comparison point is the converged FP64 solution."""
@@ -339,7 +340,7 @@ def test_mixed_jacobi_inner_cg(which):
     o.step(spec["dt"], nsteps)
     xo = o.get("x")
     totals = {}
-    for kind in (1, 0):
+    for kind in (2, 1, 0):
         s = dict(spec, tol=1e-8, preconditioner=kind,
                  refinement=dict(inner_iterations=10, inner_tolerance=0.0, max_sweeps=1000, keep_direction=True))
         eng = capi.Engine(0, capi.MIXED)
@@ -358,9 +359,9 @@ def test_mixed_jacobi_inner_cg(which):
               "against the converged FP64 PCG", flush=True)
         assert e <= 1e-6
         totals[kind] = its
-    assert totals[1] <= totals[0]
+    assert totals[1] <= totals[0] and totals[2] <= totals[0]
     if which == "ellipsoid":
-        assert totals[1] < totals[0]
+        assert totals[1] < totals[0] and totals[2] < totals[0]
 
 
 def test_cg_warm_start_tiled_reference_order():
